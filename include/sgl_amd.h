@@ -1,0 +1,162 @@
+/*
+ * sgl_amd.h -- C ABI of the MI355X magnification-PDF hot path.
+ *
+ * Plain pointers and sizes only.  Pointers are DEVICE pointers unless the
+ * parameter says "host".  Every call is stream-ordered on `stream` (a
+ * hipStream_t passed as void*, NULL = the default stream), never synchronises,
+ * never allocates: the caller owns every buffer (the *_bytes queries give
+ * the sizes).  Every entry point returns an sgl_status; nothing throws across
+ * the ABI.
+ *
+ * The reference (sgl_analytic, pure Python + Numba) has no C ABI; each entry
+ * point below names the reference function whose job it takes over, so a
+ * ctypes binding can sit where that function is called.
+ */
+#ifndef SGL_AMD_H
+#define SGL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sgl_status {
+  SGL_OK = 0,
+  SGL_ERR_INVALID = 1,     /* bad argument (shape, count, null pointer) */
+  SGL_ERR_HIP = 2,         /* a HIP launch or runtime call failed */
+  SGL_ERR_UNSUPPORTED = 3, /* precision / mode not built */
+} sgl_status;
+
+/* shape flags */
+#define SGL_FLAG_CLUSTERING 1 /* add the two-point term Psi_2 */
+#define SGL_FLAG_WINDOW 2     /* cubic B-spline window on phi(k) */
+
+/* precision of the MFMA contractions */
+#define SGL_PREC_FP64 0   /* plain path: FP64 vector ALU */
+#define SGL_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation */
+#define SGL_PREC_BF16X3 2 /* split-BF16 (hi+lo) x 3 products on MFMA, FP32 accumulation */
+
+/* Grids of one solve (host struct, passed by pointer). */
+typedef struct sgl_shape {
+  int32_t n_z_max; /* slices per instance (max over the batch), <= 96 */
+  int32_t n_m;     /* masses, log-uniform bins over [m_lo_h, m_hi_h] Msun/h */
+  int32_t n_b;     /* impact-parameter bins, log-uniform over [u_lo, u_hi] r_vir */
+  int32_t n_k;     /* log-spaced k nodes over [k_lo, k_hi]; a k = 0 node is added */
+  int32_t n_out;   /* output ln(mu) nodes, uniform over [out_lo, out_hi] */
+  int32_t flags;   /* SGL_FLAG_* */
+  double m_lo_h, m_hi_h, u_lo, u_hi, k_lo, k_hi, out_lo, out_hi;
+} sgl_shape;
+
+/* Per-instance input record: 16 doubles
+ *   h, Om, sigma8, Ob, ns, zeq, Ok, w0, wa, z_s, n_z, 0, 0, 0, 0, 0
+ * (the cosmology arguments of generate_pdf_lnmu, api.py:26-31, plus the
+ * source redshift and the slice count of this instance). */
+#define SGL_COSMO_STRIDE 16
+
+/* Buffer sizes in bytes for a batch of B instances. */
+size_t sgl_tables_bytes(const sgl_shape* shape, int32_t B);     /* derived+slices+mass+rows+cov+work */
+size_t sgl_hist_bytes(const sgl_shape* shape, int32_t B);       /* FP64 jump histograms */
+size_t sgl_filon_bytes(const sgl_shape* shape, int32_t precision); /* inversion operand E */
+size_t sgl_phi_bytes(const sgl_shape* shape, int32_t B, int32_t precision); /* operand phi */
+
+/* Library version string (host). */
+const char* sgl_version(void);
+
+/*
+ * Lens-population tables: cosmology (E, chi, D_A, Sigma_cr, growth ODE,
+ * EH98 transfer, sigma(M), Sheth-Tormen, Ludlow+16 c(M), r_vir, bias) and
+ * per-(slice, mass) rows, plus the slice covariance C_ij when
+ * SGL_FLAG_CLUSTERING is set.
+ * Replaces: pipeline.make_cosmology (sgl_analytic/pipeline.py:36-72) and the
+ * per-(z, M) loop of sgl_full._build_shell_into (sgl_analytic/sgl_full.py:1848-1885),
+ * shell_covariance (sgl_full.py:2514-2538).
+ * tables: sgl_tables_bytes(shape, B) bytes; status: one int32, OR-ed flags.
+ */
+sgl_status sgl_tables_build(const double* cosmo, int32_t B, const sgl_shape* shape,
+                            void* tables, int32_t* status, void* stream);
+
+/*
+ * Exact FP64 compensated exponent over every cell (plain path):
+ *   lam[b][n]    = sum_c w_c (e^{i k_n xi_c} - 1 - i k_n xi_c)
+ *   amp[b][i][n] = sum_{c in slice i} beta_c w_c (e^{i k_n xi_c} - 1)
+ * as interleaved (re, im) doubles, cells generated in-kernel from the rows.
+ * work: B * n_z_max * n_k * 2 doubles (per-slice partial sums of lam, added
+ * in slice order).
+ * Replaces: Lam.__call__ / _sep_raw / _sep_lam_kernel
+ * (sgl_full.py:3156-3160, 3040-3086, 2586-2619) on the 1-D jump chain of
+ * sgl.Lambda_of_k (sgl.py:961-980).
+ */
+sgl_status sgl_lnphi_exact(const void* tables, int32_t B, const sgl_shape* shape,
+                           double* lam, double* amp, double* work, void* stream);
+
+/*
+ * Batched path, step 1: histogram every cell into a fine signed-log jump
+ * lattice (w, w xi, w xi^2 per bin) and a per-slice coarse lattice (beta w
+ * moments), FP64 sums (LDS per slice, then one global add per bin).
+ * Replaces: Cells.add / _add_direct_kernel (sgl_full.py:1314-1383, 971-1046).
+ */
+sgl_status sgl_bin_cells(const void* tables, int32_t B, const sgl_shape* shape, void* hist,
+                         void* stream);
+
+/*
+ * Batched path, step 2: exponent from the histograms (bin centroid plus the
+ * in-bin variance correction), FP64 range reduction, FP32 sin/cos, FP64
+ * compensated small-phase terms and FP64 accumulation.
+ * Replaces: Lam._sep_raw (sgl_full.py:3040-3086) on binned cells.
+ */
+sgl_status sgl_lnphi_binned(const void* tables, const void* hist, int32_t B,
+                            const sgl_shape* shape, double* lam, double* amp, void* stream);
+
+/*
+ * Two-point term Psi_2[b][n] = 1/2 sum_ij C_ij amp_i amp_j (complex square,
+ * no conjugate).  precision: SGL_PREC_FP64 (vector ALU) or SGL_PREC_FP32 (MFMA).
+ * Replaces: Lam.Q (sgl_full.py:3353-3397), nord = 1.
+ */
+sgl_status sgl_quadform(const void* tables, const double* amp, int32_t B, const sgl_shape* shape,
+                        int32_t precision, double* psi2, void* stream);
+
+/*
+ * phi = window(k) exp(lam + psi2) packed as the inversion operand with the
+ * k = 0 node phi = 1: FP64 [B][n_k+1][2], FP32 [B][Kp], or split BF16
+ * (hi plane then lo plane, [B][Kp] each); Kp = round_up(2 (n_k+1), 64).
+ * psi2 may be NULL (no two-point term).
+ */
+sgl_status sgl_phi_pack(const double* lam, const double* psi2, const void* tables, int32_t B,
+                        const sgl_shape* shape, int32_t precision, void* phi_op, void* stream);
+
+/*
+ * Filon inversion operand E[n_out][Kp]: p(t_j) = sum_n E[j][2n] Re phi_n
+ * + E[j][2n+1] Im phi_n, phi piecewise linear between the k nodes (FP32, or
+ * split BF16 hi plane then lo plane).  tables/B: any tables buffer built
+ * with this shape (only its k grid is read).
+ * Replaces: the k-sum of sgl.P_of_xi (sgl.py:994-1012) / P_vector_joint
+ * (sgl.py:1127-1156).
+ */
+sgl_status sgl_filon_build(const void* tables, int32_t B, const sgl_shape* shape,
+                           int32_t precision, void* E_op, void* stream);
+
+/* p[b][j] (float) = E @ phi on MFMA (FP32 or split-BF16x3). */
+sgl_status sgl_invert_gemm(const void* E_op, const void* phi_op, int32_t B, const sgl_shape* shape,
+                           int32_t precision, float* p, void* stream);
+
+/* Plain path: p[b][j] (double) with the Filon weights evaluated on the fly
+ * in FP64; phi from sgl_phi_pack(..., SGL_PREC_FP64, ...) as [B][n_k+1] complex. */
+sgl_status sgl_invert_fp64(const double* phi, const void* tables, int32_t B,
+                           const sgl_shape* shape, double* p, void* stream);
+
+/*
+ * FP64 diagnostics of p on the output grid, 8 doubles per instance:
+ * grid mass, mean, variance, skewness, sigma_DL/D_L, <mu>, P(mu > 2),
+ * negative mass.  p_is_double selects double or float input.
+ * Replaces: the moment block of invert (sgl_full.py:362-371) and the
+ * diagnostics of generate_pdf_lnmu (api.py:71-79).
+ */
+sgl_status sgl_diagnostics(const void* p, int32_t p_is_double, int32_t B, const sgl_shape* shape,
+                           double* diag, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGL_AMD_H */

@@ -1,0 +1,328 @@
+/*
+ * FP64 C restatement of the magnification-PDF hot path (checker only).
+ *
+ * TEST INFRASTRUCTURE: loaded (ctypes) only by tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg, as the checker.  The product path never
+ * links it.
+ *
+ * What it restates, function by function:
+ *   nfw_Fh            sgl.py:694-720 (Wright & Brainerd F, h; series and
+ *                     cancellation-free branches)
+ *   tc_*              sgl_full.py:740-798 (truncated-NFW correction I1, I2)
+ *   trunc_kg          sgl_full.py:660-669 (kappa, gamma of NFW cut at x_t)
+ *   oracle_cells      one cell per (slice, mass, impact-parameter bin):
+ *                     jump xi = -ln detA (sgl.py:808-815), source-plane
+ *                     Esscher tilt w e^{-xi} (sgl.py:949-951)
+ *   oracle_lnphi      compensated compound-Poisson exponent
+ *                     Lambda(k) = sum_c w_c (e^{ik xi_c} - 1 - ik xi_c)
+ *                     with the reference's cancellation-free forms
+ *                     (sgl_full.py:2599-2613: cos-1 = -2 sin^2(ph/2),
+ *                     sin-ph by series below |ph| = 1e-2), plus the
+ *                     per-slice amplitudes a_i(k) = sum beta w (e^{ik xi}-1)
+ *                     of the two-point term (sgl_full.py:3353-3397)
+ *   oracle_invert     P(xi) = (1/pi) Re int_0^inf e^{-ik xi} phi(k) dk
+ *                     (sgl.py:994-1012) with phi linear between k nodes
+ *                     (Filon weights), phi(0) = 1
+ *
+ * Build: oracle/build.py (gcc -O2 -fopenmp -shared).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NFW_W_SERIES 0.25
+#define NFW_N_SERIES 28
+#define NFW_X_H 0.9
+#define TC_A_SMALL 1e-3
+#define TC_BAND 1e-5
+
+static void nfw_Fh(double x, double *F, double *h) {
+    double w = (1.0 - x) * (1.0 + x);
+    if (fabs(w) < NFW_W_SERIES) {
+        double T = 0.0;
+        for (int n = NFW_N_SERIES - 1; n >= 0; --n) T = T * w + 1.0 / (2 * n + 3);
+        *F = T;
+        *h = log(0.5 * x) + (1.0 + w * T);
+        return;
+    }
+    if (x < 1.0) {
+        double e = (1.0 - x) / x, s = sqrt(w);
+        double A = log1p(e + sqrt(e * (e + 2.0)));
+        *F = (A / s - 1.0) / w;
+        if (x < NFW_X_H)
+            *h = (log(2.0 / x) * x * x / (1.0 + s) + log1p(-x * x / (2.0 * (1.0 + s)))) / s;
+        else
+            *h = log(0.5 * x) + A / s;
+        return;
+    }
+    double v = -w, sv = sqrt(v), B = atan(sv);
+    *F = (1.0 - B / sv) / v;
+    *h = log(0.5 * x) + B / sv;
+}
+
+static void tc_closed_raw(double x, double c, double *I1, double *I2) {
+    double q = sqrt((c - x) * (c + x));
+    double d = (x - 1.0) * (x - c) / (x * (1.0 + c));
+    double L = log((c + q) / (2.0 * (1.0 + c))) - (q - c) / (1.0 + c);
+    if (x < 1.0) {
+        double u = (1.0 - x) * (1.0 + x), e = (1.0 - x) / x;
+        double A0 = log1p(e + sqrt(e * (e + 2.0)));
+        double At = log1p(d + sqrt(d * (d + 2.0)));
+        double su = sqrt(u);
+        *I1 = (q / (1.0 + c) - 1.0) / u + (A0 - At) / (u * su);
+        *I2 = (L + (A0 - At) / su) / (x * x);
+        return;
+    }
+    double v = (x - 1.0) * (x + 1.0);
+    double B0 = 2.0 * asin(sqrt((x - 1.0) / (2.0 * x)));
+    double md = -d > 0.0 ? -d : 0.0;
+    double Bt = 2.0 * asin(sqrt(md / 2.0));
+    double sv = sqrt(v);
+    *I1 = (1.0 - q / (1.0 + c)) / v - (B0 - Bt) / (v * sv);
+    *I2 = (L + (B0 - Bt) / sv) / (x * x);
+}
+
+/* 24-node Gauss-Legendre on [0,1]; nodes/weights filled once from the
+ * Golub-Welsch-free Newton iteration on P_24 (closed-form recurrence). */
+static double GLX[24], GLW[24];
+static int gl_ready = 0;
+static void gl_init(void) {
+    const int n = 24;
+    for (int i = 0; i < n; ++i) {
+        double z = cos(M_PI * (i + 0.75) / (n + 0.5)), pp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p1 = 1.0, p2 = 0.0;
+            for (int j = 1; j <= n; ++j) {
+                double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1.0);
+            double z1 = z;
+            z = z1 - p1 / pp;
+            if (fabs(z - z1) < 1e-16) break;
+        }
+        /* map [-1,1] -> [0,1] as sgl_full.py:635 */
+        GLX[n - 1 - i] = 0.5 * (z + 1.0);
+        GLW[n - 1 - i] = 0.5 * 2.0 / ((1.0 - z * z) * pp * pp);
+    }
+    gl_ready = 1;
+}
+
+static void tc_gl(double a, double xt, double *s1o, double *s2o) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int j = 0; j < 24; ++j) {
+        double v = GLX[j], s = 1.0 - v * v;
+        double d = 1.0 - a * a * s * s;
+        if (d < 1e-300) d = 1e-300;
+        s1 += GLW[j] * 2.0 * v * s / ((s + xt) * (s + xt) * sqrt(d));
+        double e = 1.0 - a * a * v * v;
+        if (e < 0.0) e = 0.0;
+        s2 += GLW[j] * v / ((v + xt) * (v + xt) * (1.0 + sqrt(e)));
+    }
+    *s1o = s1;
+    *s2o = s2;
+}
+
+static void tc_point(double x, double xt, double *I1, double *I2) {
+    double a = x / xt;
+    if (a >= 1.0) { *I1 = 0.0; *I2 = 0.0; return; }
+    if (a < TC_A_SMALL) {
+        double i0 = log((1.0 + xt) / xt) - 1.0 / (1.0 + xt), u1 = 1.0 + xt;
+        double jj = 0.5 * (u1 * u1 - xt * xt) - 3.0 * xt * (u1 - xt) + 3.0 * xt * xt * log(u1 / xt)
+                    + xt * xt * xt * (1.0 / u1 - 1.0 / xt);
+        *I1 = i0 + 0.5 * a * a * jj;
+        *I2 = 0.5 * i0 + 0.125 * a * a * jj;
+        return;
+    }
+    if (fabs(x - 1.0) < TC_BAND) {
+        if (xt <= 1.0 + 2.0 * TC_BAND) { tc_gl(a, xt, I1, I2); return; }
+        double a1, b1, a2, b2;
+        tc_closed_raw(1.0 - TC_BAND, xt, &a1, &b1);
+        tc_closed_raw(1.0 + TC_BAND, xt, &a2, &b2);
+        double f = (x - (1.0 - TC_BAND)) / (2.0 * TC_BAND);
+        *I1 = a1 + f * (a2 - a1);
+        *I2 = b1 + f * (b2 - b1);
+        return;
+    }
+    tc_closed_raw(x, xt, I1, I2);
+}
+
+void oracle_trunc_kg(int64_t n, const double *x, double ks, double xt, double *kap, double *gam) {
+    if (!gl_ready) gl_init();
+    for (int64_t i = 0; i < n; ++i) {
+        double F, h, xi = x[i];
+        nfw_Fh(xi, &F, &h);
+        double ku = 2.0 * F, gu = 4.0 * h / (xi * xi) - ku, k, kb;
+        if (xi < xt) {
+            double I1, I2;
+            tc_point(xi, xt, &I1, &I2);
+            k = ku - 2.0 * I1;
+            kb = ku + gu - 4.0 * I2;
+        } else {
+            double mut = log(1.0 + xt) - xt / (1.0 + xt);
+            k = 0.0;
+            kb = 4.0 * mut / (xi * xi);
+        }
+        kap[i] = ks * k;
+        gam[i] = ks * (kb - k);
+    }
+}
+
+void oracle_nfw_Fh(int64_t n, const double *x, double *F, double *h) {
+    for (int64_t i = 0; i < n; ++i) nfw_Fh(x[i], &F[i], &h[i]);
+}
+
+/*
+ * Cells of the kernel-shape population.  rows: [n_rows][6] =
+ * (k_s, r_s, x_t, r_vir, w_row, beta) with w_row = dchi (1+z)^2 dn [Mpc^-2],
+ * r_s and r_vir physical Mpc; u_edges[n_b+1] in units of r_vir.
+ * Outputs per cell (row-major [n_rows][n_b]): xi, ws = w e^{-xi} (0 for a
+ * dropped cell), kap, gam.  A cell is dropped when detA <= 0 (sgl.py:812-815)
+ * or xi > XI_MAX (the reference's jump grid ends at 30, sgl.py:744).
+ */
+#define XI_MAX 30.0
+void oracle_cells(int64_t n_rows, const double *rows, int64_t n_b, const double *u_edges,
+                  double *xi, double *ws, double *kap, double *gam) {
+    if (!gl_ready) gl_init();
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t r = 0; r < n_rows; ++r) {
+        const double *R = rows + 6 * r;
+        double ks = R[0], rs = R[1], xt = R[2], rvir = R[3], wrow = R[4];
+        for (int64_t l = 0; l < n_b; ++l) {
+            double b0 = rvir * u_edges[l], b1 = rvir * u_edges[l + 1];
+            double bc = sqrt(b0 * b1), x = bc / rs;
+            double area = M_PI * (b1 * b1 - b0 * b0);
+            double k, g;
+            oracle_trunc_kg(1, &x, ks, xt, &k, &g);
+            double detA = (1.0 - k) * (1.0 - k) - g * g;
+            int64_t o = r * n_b + l;
+            kap[o] = k;
+            gam[o] = g;
+            if (!(detA > 0.0)) { xi[o] = 0.0; ws[o] = 0.0; continue; }
+            double x_ = -log(detA);
+            if (x_ > XI_MAX) { xi[o] = 0.0; ws[o] = 0.0; continue; }
+            xi[o] = x_;
+            ws[o] = wrow * area * detA;     /* Esscher tilt e^{-xi} = detA */
+        }
+    }
+}
+
+/*
+ * lam[2*n_k] (re, im) = sum_c ws_c (e^{ik xi_c} - 1 - ik xi_c);
+ * amp[n_slices][2*n_k] = sum_{c in slice} beta_c ws_c (e^{ik xi_c} - 1).
+ * Cells are given per slice: slice_off[n_slices+1] offsets into xi/ws/bws.
+ */
+void oracle_lnphi(int64_t n_slices, const int64_t *slice_off, const double *xi, const double *ws,
+                  const double *bws, int64_t n_k, const double *k, double *lam, double *amp) {
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int64_t n = 0; n < n_k; ++n) {
+        double kn = k[n], lr = 0.0, li = 0.0;
+        for (int64_t s = 0; s < n_slices; ++s) {
+            double ar = 0.0, ai = 0.0;
+            for (int64_t c = slice_off[s]; c < slice_off[s + 1]; ++c) {
+                double w = ws[c];
+                if (w == 0.0) continue;
+                double ph = kn * xi[c];
+                double sn = sin(ph), hs = sin(0.5 * ph);
+                double cm1 = -2.0 * hs * hs;
+                double smp;
+                if (fabs(ph) < 1e-2) {
+                    double p2 = ph * ph;
+                    smp = -ph * p2 / 6.0 * (1.0 - p2 / 20.0 * (1.0 - p2 / 42.0));
+                } else {
+                    smp = sn - ph;
+                }
+                lr += w * cm1;
+                li += w * smp;
+                ar += bws[c] * cm1;
+                ai += bws[c] * sn;
+            }
+            amp[s * 2 * n_k + 2 * n] = ar;
+            amp[s * 2 * n_k + 2 * n + 1] = ai;
+        }
+        lam[2 * n] = lr;
+        lam[2 * n + 1] = li;
+    }
+}
+
+/* E1(th) = int_0^1 s e^{-i th s} ds, E0(th) = int_0^1 e^{-i th s} ds. */
+static void filon_e(double th, double *e0r, double *e0i, double *e1r, double *e1i) {
+    if (fabs(th) < 0.5) {
+        /* sum_m (-i th)^m / (m! (m+1)), (m! (m+2)) */
+        double tr = 1.0, ti = 0.0; /* (-i th)^m / m! */
+        double a0r = 0.0, a0i = 0.0, a1r = 0.0, a1i = 0.0;
+        for (int m = 0; m < 30; ++m) {
+            a0r += tr / (m + 1);
+            a0i += ti / (m + 1);
+            a1r += tr / (m + 2);
+            a1i += ti / (m + 2);
+            /* multiply by (-i th)/(m+1) */
+            double nr = ti * th / (m + 1), ni = -tr * th / (m + 1);
+            tr = nr;
+            ti = ni;
+        }
+        *e0r = a0r; *e0i = a0i; *e1r = a1r; *e1i = a1i;
+        return;
+    }
+    double c = cos(th), s = sin(th);
+    /* E0 = (1 - e^{-i th}) / (i th) = (sin th + i (cos th - 1)) / th */
+    *e0r = s / th;
+    *e0i = (c - 1.0) / th;
+    /* E1 = e^{-i th}(i/th + 1/th^2) - 1/th^2 */
+    double ar = 1.0 / (th * th), ai = 1.0 / th;
+    *e1r = c * ar + s * ai - 1.0 / (th * th);
+    *e1i = c * ai - s * ar;
+}
+
+/*
+ * Filon weights: A_n(t) = int hat_n(k) e^{-ikt} dk on nodes kq[0..nq-1]
+ * (kq[0] = 0), hat_n the piecewise-linear basis, the last node's hat
+ * one-sided.  wts[j][2*nq]: (Re A_n / pi, -Im A_n / pi), so that
+ * p(t_j) = sum_n wts[j][2n] Re phi_n + wts[j][2n+1] Im phi_n.
+ */
+void oracle_filon_weights(int64_t n_t, const double *t, int64_t nq, const double *kq, double *wts) {
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < n_t; ++j) {
+        double tj = t[j];
+        double *W = wts + j * 2 * nq;
+        for (int64_t n = 0; n < 2 * nq; ++n) W[n] = 0.0;
+        for (int64_t n = 0; n + 1 < nq; ++n) {
+            double a = kq[n], h = kq[n + 1] - a, th = h * tj;
+            double e0r, e0i, e1r, e1i;
+            filon_e(th, &e0r, &e0i, &e1r, &e1i);
+            double cr = cos(a * tj), ci = -sin(a * tj);   /* e^{-i a t} */
+            /* up half (node n+1): h e^{-iat} E1 ; down half (node n): h e^{-iat} (E0 - E1) */
+            double ur = h * (cr * e1r - ci * e1i), ui = h * (cr * e1i + ci * e1r);
+            double dr_ = e0r - e1r, di_ = e0i - e1i;
+            double dr = h * (cr * dr_ - ci * di_), di = h * (cr * di_ + ci * dr_);
+            W[2 * n] += dr / M_PI;
+            W[2 * n + 1] += -di / M_PI;
+            W[2 * (n + 1)] += ur / M_PI;
+            W[2 * (n + 1) + 1] += -ui / M_PI;
+        }
+    }
+}
+
+/* p[j] = sum_n wts(t_j)[2n] Re phi_n + wts[2n+1] Im phi_n, weights on the fly. */
+void oracle_invert(int64_t n_t, const double *t, int64_t nq, const double *kq, const double *phi,
+                   double *p) {
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < n_t; ++j) {
+        double tj = t[j], acc = 0.0;
+        for (int64_t n = 0; n + 1 < nq; ++n) {
+            double a = kq[n], h = kq[n + 1] - a, th = h * tj;
+            double e0r, e0i, e1r, e1i;
+            filon_e(th, &e0r, &e0i, &e1r, &e1i);
+            double cr = cos(a * tj), ci = -sin(a * tj);
+            double ur = h * (cr * e1r - ci * e1i), ui = h * (cr * e1i + ci * e1r);
+            double dr_ = e0r - e1r, di_ = e0i - e1i;
+            double dr = h * (cr * dr_ - ci * di_), di = h * (cr * di_ + ci * dr_);
+            /* Re(phi * A) = Re phi Re A - Im phi Im A */
+            acc += phi[2 * n] * dr - phi[2 * n + 1] * di;
+            acc += phi[2 * (n + 1)] * ur - phi[2 * (n + 1) + 1] * ui;
+        }
+        p[j] = acc / M_PI;
+    }
+}

@@ -1,0 +1,278 @@
+"""FP64 CPU oracle of the kernel-shape magnification-PDF chain (checker only).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg.  Never on the product path.
+
+The chain (one cosmology, one source redshift):
+
+1. slices: ``n_z`` slices uniform in comoving distance on [0, chi(z_s)],
+   node at the slice midpoint (the kernel shape of the lens-z quadrature
+   that sgl_full.z_shells, sgl_full.py:480-518, performs on the reference grid);
+2. rows per (slice, mass): Sheth-Tormen dn/dlnM, Ludlow+16 NFW parameters,
+   truncation at r200 (x_t = c, sgl_full.py:1876-1880), r_vir = eta r200
+   (subhalos.py:472-481), linear bias beta = D b_h (sgl_full.py:541-543,1871),
+   lens weight dchi (1+z)^2 dn (sgl_full.py:1860);
+3. cells: ``n_b`` log impact-parameter bins over [u_lo, u_hi] r_vir, each
+   cell at the geometric centre of its bin (sgl_full.py:1905), weight = row
+   weight x annulus area, kappa/gamma of the truncated NFW, jump
+   xi = -ln detA and the source-plane tilt w e^{-xi} (sgl.py:808-815,949-951);
+4. Lambda(k) compensated, a_i(k) per slice (oracle/chain.c);
+5. Psi_2(k) = 1/2 sum_ij C_ij a_i(k) a_j(k), C_ij the slice covariance of the
+   smoothed 1-D linear density field (sgl_full.py:2514-2538 restated on these
+   slices; the quadratic form of Lam.Q, sgl_full.py:3353-3397, nord = 1);
+6. P(xi) = (1/pi) Re int_0^inf e^{-ik xi + Lambda + Psi_2} dk (sgl.py:994-1012),
+   phi linear between the k nodes and phi(0) = 1 (Filon weights);
+   with phi multiplied by the reference's positive-definite cubic B-spline
+   window of support [0, k_max] (sgl_full.py:83-94, k_kappa factor): the
+   transform of p keeps unit mass exactly (phi(0) = 1), so p is not
+   renormalised;
+7. diagnostics on the output grid in FP64 (trapezoid, as api.py:75-79 and
+   sgl_full.py:362-371): grid mass, mean, variance, skewness, sigma_DL/D_L,
+   <mu>, P(mu > 2), negative mass.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+from scipy.optimize import brentq
+
+from .cosmo import Cosmology
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        from .build import LIB, build
+        if not os.path.exists(LIB):
+            build()
+        _LIB = ctypes.CDLL(LIB)
+        d = ctypes.POINTER(ctypes.c_double)
+        i64 = ctypes.c_int64
+        p64 = ctypes.POINTER(ctypes.c_int64)
+        _LIB.oracle_cells.argtypes = [i64, d, i64, d, d, d, d, d]
+        _LIB.oracle_lnphi.argtypes = [i64, p64, d, d, d, i64, d, d, d]
+        _LIB.oracle_invert.argtypes = [i64, d, i64, d, d, d]
+        _LIB.oracle_filon_weights.argtypes = [i64, d, i64, d, d]
+        _LIB.oracle_trunc_kg.argtypes = [i64, d, ctypes.c_double, ctypes.c_double, d, d]
+        _LIB.oracle_nfw_Fh.argtypes = [i64, d, d, d]
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+@dataclass
+class Shape:
+    """Kernel-shape grids.  Defaults are the 64 x 256 x 512 / 8192-k /
+    2048-node shape; tests shrink them."""
+    n_z: int = 64
+    n_m: int = 256
+    n_b: int = 512
+    m_lo_h: float = 1e8          # Msun/h
+    m_hi_h: float = 1e16
+    u_lo: float = 1e-4           # impact parameter / r_vir
+    u_hi: float = 10.0
+    n_k: int = 8192
+    k_lo: float = 1e-2
+    k_hi: float = 1e6
+    n_out: int = 2048
+    out_lo: float = -0.5
+    out_hi: float = 3.0
+    clustering: bool = True
+    window: bool = True
+
+    def k_nodes(self):
+        """k_q: 0 followed by n_k log-spaced nodes."""
+        return np.concatenate([[0.0], np.logspace(math.log10(self.k_lo), math.log10(self.k_hi), self.n_k)])
+
+    def out_grid(self):
+        return np.linspace(self.out_lo, self.out_hi, self.n_out)
+
+    def u_edges(self):
+        return np.logspace(math.log10(self.u_lo), math.log10(self.u_hi), self.n_b + 1)
+
+    def m_nodes(self, h):
+        """Mass bins log-uniform over [m_lo_h, m_hi_h] Msun/h; node = bin
+        centre in ln M, converted to Msun (M = M_h / h)."""
+        e = np.linspace(math.log(self.m_lo_h), math.log(self.m_hi_h), self.n_m + 1)
+        return np.exp(0.5 * (e[1:] + e[:-1])) / h, float(e[1] - e[0])
+
+
+def slice_nodes(cos: Cosmology, zs: float, n_z: int):
+    """Uniform-in-chi slices: returns (z_node, chi_node, dchi, chi_edges)."""
+    chis = cos.chi(zs)
+    edges = np.linspace(0.0, chis, n_z + 1)
+    mid = 0.5 * (edges[1:] + edges[:-1])
+    z = np.array([brentq(lambda zz: cos.chi(zz) - c, 0.0, zs, xtol=1e-14, rtol=1e-14) for c in mid])
+    return z, mid, chis / n_z, edges
+
+
+def slice_covariance(cos: Cosmology, chi_edges: np.ndarray, chis: float, Rs: float = 20.0):
+    """sgl_full.py:2514-2538 (BiasField1D, bias_window=1) on the given
+    comoving segments: exact mode sum over k_q = 2 pi q / L, L = 1.05 chi_s."""
+    a, b = chi_edges[:-1], chi_edges[1:]
+    L = 1.05 * chis
+    Nmax = max(4, int(np.floor(L / Rs)))
+    kq = 2 * np.pi * np.arange(1, Nmax + 1) / L
+    Kg = np.geomspace(kq[0] * 0.5, 200.0 / Rs, 20000)
+    x = Kg * Rs
+    W = np.where(x < 1e-2, 1 - x ** 2 / 10 * (1 - x ** 2 / 28), 3 * (np.sin(x) - x * np.cos(x)) / x ** 3)
+    f = Kg * cos.P0(Kg) * W ** 2
+    cum = np.concatenate([[0.0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(Kg))])
+    P1D = (cum[-1] - np.interp(kq, Kg, cum)) / (2 * np.pi)
+    Lh = b - a
+    rw = np.sqrt(2 * P1D / L)
+    cq = rw[:, None] * (np.sin(kq[:, None] * b) - np.sin(kq[:, None] * a)) / (kq[:, None] * Lh)
+    sq = rw[:, None] * (np.cos(kq[:, None] * a) - np.cos(kq[:, None] * b)) / (kq[:, None] * Lh)
+    return cq.T @ cq + sq.T @ sq
+
+
+def rows(cos: Cosmology, zs: float, shape: Shape):
+    """Per-(slice, mass) rows [n_z, n_m, 6] = (k_s, r_s, x_t, r_vir, w_row, beta)."""
+    z, chi_mid, dchi, chi_edges = slice_nodes(cos, zs, shape.n_z)
+    M, dlnM = shape.m_nodes(cos.h)
+    R = np.zeros((shape.n_z, shape.n_m, 6))
+    for i, zi in enumerate(z):
+        D = cos.D(zi)
+        wz = dchi * (1 + zi) ** 2
+        for j, Mj in enumerate(M):
+            C, rs, ks, fC = cos.nfw_params(Mj, zi, zs)
+            dn = cos.dndlnM(Mj, zi) * dlnM
+            rvir = cos.eta(C, zi) * C * rs
+            beta = D * cos.halo_bias(Mj, zi)
+            R[i, j] = (ks, rs, C, rvir, wz * dn, beta)
+    return dict(rows=R, z=z, chi=chi_mid, dchi=dchi, chi_edges=chi_edges, M=M, dlnM=dlnM,
+                chis=cos.chi(zs))
+
+
+def cells(R: np.ndarray, shape: Shape):
+    """Cell arrays [n_z, n_m, n_b]: xi, ws (tilted weight, 0 = dropped), kap, gam."""
+    L = lib()
+    n_rows = R.shape[0] * R.shape[1]
+    rr = np.ascontiguousarray(R.reshape(n_rows, 6))
+    ue = shape.u_edges()
+    out = [np.zeros(n_rows * shape.n_b) for _ in range(4)]
+    L.oracle_cells(n_rows, _p(rr), shape.n_b, _p(ue), *[_p(o) for o in out])
+    return [o.reshape(R.shape[0], R.shape[1], shape.n_b) for o in out]
+
+
+def lnphi(xi, ws, beta, k):
+    """Lambda[n_k] (complex) and a[n_z, n_k] (complex) over the given k."""
+    L = lib()
+    n_z = xi.shape[0]
+    per = xi[0].size
+    xs = np.ascontiguousarray(xi.reshape(-1))
+    w = np.ascontiguousarray(ws.reshape(-1))
+    bw = np.ascontiguousarray((ws * beta[:, :, None]).reshape(-1))
+    off = (np.arange(n_z + 1) * per).astype(np.int64)
+    kk = np.ascontiguousarray(k, dtype=float)
+    lam = np.zeros(2 * kk.size)
+    amp = np.zeros(n_z * 2 * kk.size)
+    L.oracle_lnphi(n_z, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _p(xs), _p(w), _p(bw),
+                   kk.size, _p(kk), _p(lam), _p(amp))
+    lam = lam[0::2] + 1j * lam[1::2]
+    amp = amp.reshape(n_z, kk.size, 2)
+    return lam, amp[..., 0] + 1j * amp[..., 1]
+
+
+def quadform(C, a):
+    """Psi_2[n_k] = 1/2 sum_ij C_ij a_i a_j (complex square, no conjugate)."""
+    return 0.5 * np.einsum("ik,ij,jk->k", a, C, a)
+
+
+def invert(t, kq, phi):
+    """p(t) for phi on nodes kq (phi[0] at k = 0)."""
+    L = lib()
+    tt = np.ascontiguousarray(t, dtype=float)
+    kk = np.ascontiguousarray(kq, dtype=float)
+    ph = np.ascontiguousarray(np.stack([phi.real, phi.imag], -1).reshape(-1))
+    p = np.zeros(tt.size)
+    L.oracle_invert(tt.size, _p(tt), kk.size, _p(kk), _p(ph), _p(p))
+    return p
+
+
+def filon_weights(t, kq):
+    L = lib()
+    tt = np.ascontiguousarray(t, dtype=float)
+    kk = np.ascontiguousarray(kq, dtype=float)
+    W = np.zeros((tt.size, 2 * kk.size))
+    L.oracle_filon_weights(tt.size, _p(tt), kk.size, _p(kk), _p(W))
+    return W
+
+
+def window(kq):
+    """Cubic B-spline in k of support [0, K = kq[-1]], 1 at k = 0
+    (sgl_full.py:83-94, the k_kappa factor of _window_pd)."""
+    K = float(kq[-1])
+    x = np.abs(kq) * 2.0 / K
+    return np.where(x < 1, 2 / 3 - x ** 2 + x ** 3 / 2, np.where(x < 2, (2 - x) ** 3 / 6, 0.0)) / (2 / 3)
+
+
+def moments(xw, pw):
+    """Diagnostics of p on its grid: sgl_full.py:362-371 (mean, var, skew,
+    sigma_DL with the grid mass as norm) plus <mu>, P(mu > 2) and the
+    negative mass of api.py:75."""
+    norm = float(np.trapezoid(pw, xw))
+    ps = pw / norm
+    mean = float(np.trapezoid(xw * ps, xw))
+    var = float(np.trapezoid((xw - mean) ** 2 * ps, xw))
+    skew = float(np.trapezoid((xw - mean) ** 3 * ps, xw) / var ** 1.5)
+    x = np.exp(-0.5 * xw)
+    md = float(np.trapezoid(x * ps, xw))
+    sdl = float(np.sqrt(np.trapezoid((x - md) ** 2 * ps, xw)) / md)
+    mu_mean = float(np.trapezoid(np.exp(xw) * ps, xw))
+    tail = float(tail_mass(xw, ps, math.log(2.0)))
+    neg = float(np.trapezoid(np.maximum(-pw, 0.0), xw))
+    return dict(norm=norm, mean=mean, var=var, skew=skew, sigmaDL=sdl, mu_mean=mu_mean,
+                p_mu_gt_2=tail, negative_mass=neg)
+
+
+def tail_mass(x, p, x0):
+    """Trapezoid of p over [x0, x_end] with linear interpolation at x0."""
+    i = int(np.searchsorted(x, x0, side="right"))
+    if i >= x.size:
+        return 0.0
+    if i == 0:
+        return float(np.trapezoid(p, x))
+    f = (x0 - x[i - 1]) / (x[i] - x[i - 1])
+    p0 = p[i - 1] + f * (p[i] - p[i - 1])
+    return float(0.5 * (p0 + p[i]) * (x[i] - x0) + np.trapezoid(p[i:], x[i:]))
+
+
+@dataclass
+class Result:
+    x: np.ndarray
+    p: np.ndarray
+    info: dict = field(default_factory=dict)
+
+
+def pdf(zs: float, shape: Shape = None, cosmo: dict = None, keep: bool = False) -> Result:
+    """End-to-end FP64 oracle PDF on the output grid."""
+    shape = shape or Shape()
+    cos = Cosmology(**(cosmo or {}))
+    T = rows(cos, zs, shape)
+    R = T["rows"]
+    xi, ws, kap, gam = cells(R, shape)
+    kq = shape.k_nodes()
+    lam, amp = lnphi(xi, ws, R[:, :, 5], kq[1:])
+    C = slice_covariance(cos, T["chi_edges"], T["chis"])
+    psi2 = quadform(C, amp) if shape.clustering else np.zeros_like(lam)
+    phi = np.concatenate([[1.0 + 0j], np.exp(lam + psi2)])
+    if shape.window:
+        phi = phi * window(kq)
+    xo = shape.out_grid()
+    po = invert(xo, kq, phi)
+    mom = moments(xo, po)
+    info = dict(moments=mom, n_cells=int(np.count_nonzero(ws)), sum_w=float(ws.sum()),
+                var_xi=float(np.sum(ws * xi ** 2)))
+    if keep:
+        info.update(cos=cos, tables=T, xi=xi, ws=ws, kap=kap, gam=gam, lam=lam, amp=amp, C=C,
+                    psi2=psi2, phi=phi, kq=kq)
+    return Result(xo, po, info)

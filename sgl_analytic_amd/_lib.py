@@ -1,0 +1,86 @@
+"""ctypes binding of libsgl_amd.so (the C ABI declared in include/sgl_amd.h).
+
+The library is built in-tree by ``python -m sgl_analytic_amd.build``.  There
+is no fallback: if the library is missing or a GPU is absent, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsgl_amd.so")
+
+SGL_OK, SGL_ERR_INVALID, SGL_ERR_HIP, SGL_ERR_UNSUPPORTED = 0, 1, 2, 3
+FLAG_CLUSTERING, FLAG_WINDOW = 1, 2
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+COSMO_STRIDE = 16
+
+EXPORTS = (
+    "sgl_version", "sgl_tables_bytes", "sgl_hist_bytes", "sgl_filon_bytes", "sgl_phi_bytes",
+    "sgl_tables_build", "sgl_lnphi_exact", "sgl_bin_cells", "sgl_lnphi_binned",
+    "sgl_quadform", "sgl_phi_pack", "sgl_filon_build", "sgl_invert_gemm", "sgl_invert_fp64",
+    "sgl_diagnostics",
+)
+
+
+class CShape(ctypes.Structure):
+    _fields_ = [("n_z_max", ctypes.c_int32), ("n_m", ctypes.c_int32), ("n_b", ctypes.c_int32),
+                ("n_k", ctypes.c_int32), ("n_out", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("m_lo_h", ctypes.c_double), ("m_hi_h", ctypes.c_double),
+                ("u_lo", ctypes.c_double), ("u_hi", ctypes.c_double),
+                ("k_lo", ctypes.c_double), ("k_hi", ctypes.c_double),
+                ("out_lo", ctypes.c_double), ("out_hi", ctypes.c_double)]
+
+
+class SglError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the library and declare every exported signature."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise SglError(f"{path} is missing: build it with `python -m sgl_analytic_amd.build` "
+                       "(there is no CPU fallback)")
+    L = ctypes.CDLL(path)
+    vp, i32, sz = ctypes.c_void_p, ctypes.c_int32, ctypes.c_size_t
+    sp = ctypes.POINTER(CShape)
+    L.sgl_version.restype = ctypes.c_char_p
+    L.sgl_version.argtypes = []
+    for name in ("sgl_tables_bytes", "sgl_hist_bytes", "sgl_phi_bytes"):
+        f = getattr(L, name)
+        f.restype = sz
+        f.argtypes = [sp, i32] + ([i32] if name == "sgl_phi_bytes" else [])
+    L.sgl_filon_bytes.restype = sz
+    L.sgl_filon_bytes.argtypes = [sp, i32]
+    sigs = {
+        "sgl_tables_build": [vp, i32, sp, vp, vp, vp],
+        "sgl_lnphi_exact": [vp, i32, sp, vp, vp, vp, vp],
+        "sgl_bin_cells": [vp, i32, sp, vp, vp],
+        "sgl_lnphi_binned": [vp, vp, i32, sp, vp, vp, vp],
+        "sgl_quadform": [vp, vp, i32, sp, i32, vp, vp],
+        "sgl_phi_pack": [vp, vp, vp, i32, sp, i32, vp, vp],
+        "sgl_filon_build": [vp, i32, sp, i32, vp, vp],
+        "sgl_invert_gemm": [vp, vp, i32, sp, i32, vp, vp],
+        "sgl_invert_fp64": [vp, vp, i32, sp, vp, vp],
+        "sgl_diagnostics": [vp, i32, i32, sp, vp, vp],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def check(rc: int, what: str):
+    if rc != SGL_OK:
+        names = {SGL_ERR_INVALID: "invalid argument", SGL_ERR_HIP: "HIP error",
+                 SGL_ERR_UNSUPPORTED: "unsupported"}
+        raise SglError(f"{what} failed: {names.get(rc, rc)}")

@@ -1,0 +1,172 @@
+// extern "C" entry points of libsgl_amd.so (declared in include/sgl_amd.h).
+// Argument checks happen here, on the host, before any launch: a shape or
+// count the kernels do not support returns SGL_ERR_INVALID and launches
+// nothing.
+#include <string.h>
+
+#include "sgl_amd_internal.h"
+
+using namespace sgl;
+
+namespace {
+
+bool shape_ok(const sgl_shape* s) {
+  if (s == nullptr) return false;
+  if (s->n_z_max < 1 || s->n_z_max > kMaxSlices) return false;
+  if (s->n_m < 1 || s->n_m > 4096) return false;
+  if (s->n_b < 1 || s->n_b > 1 << 16) return false;
+  if (s->n_k < 2 || s->n_k > 1 << 20) return false;
+  if (s->n_out < 2 || s->n_out > 1 << 16) return false;
+  if (!(s->m_lo_h > 0 && s->m_hi_h > s->m_lo_h)) return false;
+  if (!(s->u_lo > 0 && s->u_hi > s->u_lo)) return false;
+  if (!(s->k_lo > 0 && s->k_hi > s->k_lo)) return false;
+  if (!(s->out_hi > s->out_lo)) return false;
+  return true;
+}
+
+bool batch_ok(int32_t B) { return B >= 1 && B <= (1 << 20); }
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+__global__ void k_copy(const double* __restrict__ src, double* __restrict__ dst, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_zero_int(int* p) { *p = 0; }
+
+}  // namespace
+
+extern "C" {
+
+const char* sgl_version(void) { return "sgl_amd 0.1 (gfx950)"; }
+
+size_t sgl_tables_bytes(const sgl_shape* shape, int32_t B) {
+  if (!shape_ok(shape) || !batch_ok(B)) return 0;
+  return tables_layout(*shape, B).total * sizeof(double);
+}
+
+size_t sgl_hist_bytes(const sgl_shape* shape, int32_t B) {
+  if (!shape_ok(shape) || !batch_ok(B)) return 0;
+  return (hist_layout_fine(B) + hist_layout_slice(*shape, B)) * sizeof(double);
+}
+
+size_t sgl_filon_bytes(const sgl_shape* shape, int32_t precision) {
+  if (!shape_ok(shape)) return 0;
+  size_t n = (size_t)shape->n_out * k_padded_host(*shape);
+  if (precision == SGL_PREC_FP32) return n * sizeof(float);
+  if (precision == SGL_PREC_BF16X3) return 2 * n * sizeof(uint16_t);
+  return 0;
+}
+
+size_t sgl_phi_bytes(const sgl_shape* shape, int32_t B, int32_t precision) {
+  if (!shape_ok(shape) || !batch_ok(B)) return 0;
+  if (precision == SGL_PREC_FP64) return (size_t)B * (shape->n_k + 1) * 2 * sizeof(double);
+  size_t n = (size_t)B * k_padded_host(*shape);
+  if (precision == SGL_PREC_FP32) return n * sizeof(float);
+  if (precision == SGL_PREC_BF16X3) return 2 * n * sizeof(uint16_t);
+  return 0;
+}
+
+sgl_status sgl_tables_build(const double* cosmo, int32_t B, const sgl_shape* shape, void* tables,
+                            int32_t* status, void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || cosmo == nullptr || tables == nullptr ||
+      status == nullptr)
+    return SGL_ERR_INVALID;
+  const sgl_shape& sh = *shape;
+  hipStream_t st = as_stream(stream);
+  TablesLayout L = tables_layout(sh, B);
+  double* T = static_cast<double*>(tables);
+  size_t nc = (size_t)B * SGL_COSMO_STRIDE;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, cosmo,
+                     T + L.cosmo, nc);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, status);
+  int rc = launch_fill_grids(sh, T + L.uedge, T + L.knodes, T + L.m_nodes, st);
+  if (rc != SGL_OK) return (sgl_status)rc;
+  rc = tables_build(T + L.cosmo, B, sh, T + L.m_nodes, T + L.derived, T + L.slices, T + L.mass,
+                    T + L.rows, T + L.cov, T + L.work, status, st);
+  return (sgl_status)rc;
+}
+
+sgl_status sgl_lnphi_exact(const void* tables, int32_t B, const sgl_shape* shape, double* lam,
+                           double* amp, double* work, void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !tables || !lam || !amp || !work)
+    return SGL_ERR_INVALID;
+  TablesLayout L = tables_layout(*shape, B);
+  const double* T = static_cast<const double*>(tables);
+  return (sgl_status)launch_lnphi_exact(T + L.rows, T + L.cosmo, T + L.uedge, T + L.knodes, *shape,
+                                        B, lam, amp, work, as_stream(stream));
+}
+
+sgl_status sgl_bin_cells(const void* tables, int32_t B, const sgl_shape* shape, void* hist,
+                         void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !tables || !hist) return SGL_ERR_INVALID;
+  TablesLayout L = tables_layout(*shape, B);
+  const double* T = static_cast<const double*>(tables);
+  double* H = static_cast<double*>(hist);
+  return (sgl_status)launch_bin(T + L.rows, T + L.cosmo, T + L.uedge, *shape, B, H,
+                                H + hist_layout_fine(B), as_stream(stream));
+}
+
+sgl_status sgl_lnphi_binned(const void* tables, const void* hist, int32_t B,
+                              const sgl_shape* shape, double* lam, double* amp, void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !tables || !hist || !lam) return SGL_ERR_INVALID;
+  if ((shape->flags & SGL_FLAG_CLUSTERING) && !amp) return SGL_ERR_INVALID;
+  TablesLayout L = tables_layout(*shape, B);
+  const double* T = static_cast<const double*>(tables);
+  const double* H = static_cast<const double*>(hist);
+  return (sgl_status)launch_lnphi_binned(H, H + hist_layout_fine(B), T + L.cosmo, T + L.knodes,
+                                         *shape, B, lam, amp, as_stream(stream));
+}
+
+sgl_status sgl_quadform(const void* tables, const double* amp, int32_t B, const sgl_shape* shape,
+                        int32_t precision, double* psi2, void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !tables || !amp || !psi2) return SGL_ERR_INVALID;
+  if (precision != SGL_PREC_FP64 && precision != SGL_PREC_FP32) return SGL_ERR_UNSUPPORTED;
+  TablesLayout L = tables_layout(*shape, B);
+  const double* T = static_cast<const double*>(tables);
+  return (sgl_status)launch_quadform(T + L.cov, T + L.cosmo, amp, B, *shape, psi2,
+                                     as_stream(stream));
+}
+
+sgl_status sgl_phi_pack(const double* lam, const double* psi2, const void* tables, int32_t B,
+                        const sgl_shape* shape, int32_t precision, void* phi_op, void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !lam || !tables || !phi_op) return SGL_ERR_INVALID;
+  if (precision < SGL_PREC_FP64 || precision > SGL_PREC_BF16X3) return SGL_ERR_UNSUPPORTED;
+  TablesLayout L = tables_layout(*shape, B);
+  const double* T = static_cast<const double*>(tables);
+  return (sgl_status)launch_phi_pack(lam, psi2, T + L.knodes, *shape, B, precision, phi_op,
+                                     as_stream(stream));
+}
+
+sgl_status sgl_filon_build(const void* tables, int32_t B, const sgl_shape* shape,
+                             int32_t precision, void* E_op, void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !tables || !E_op) return SGL_ERR_INVALID;
+  if (precision != SGL_PREC_FP32 && precision != SGL_PREC_BF16X3) return SGL_ERR_UNSUPPORTED;
+  TablesLayout L = tables_layout(*shape, B);
+  const double* T = static_cast<const double*>(tables);
+  return (sgl_status)launch_filon(T + L.knodes, *shape, precision, E_op, as_stream(stream));
+}
+
+sgl_status sgl_invert_gemm(const void* E_op, const void* phi_op, int32_t B, const sgl_shape* shape,
+                           int32_t precision, float* p, void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !E_op || !phi_op || !p) return SGL_ERR_INVALID;
+  if (precision != SGL_PREC_FP32 && precision != SGL_PREC_BF16X3) return SGL_ERR_UNSUPPORTED;
+  return (sgl_status)launch_gemm(E_op, phi_op, *shape, B, precision, p, as_stream(stream));
+}
+
+sgl_status sgl_invert_fp64(const double* phi, const void* tables, int32_t B,
+                           const sgl_shape* shape, double* p, void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !phi || !tables || !p) return SGL_ERR_INVALID;
+  TablesLayout L = tables_layout(*shape, B);
+  const double* T = static_cast<const double*>(tables);
+  return (sgl_status)launch_invert_fp64(phi, T + L.knodes, *shape, B, p, as_stream(stream));
+}
+
+sgl_status sgl_diagnostics(const void* p, int32_t p_is_double, int32_t B, const sgl_shape* shape,
+                           double* diag, void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !p || !diag) return SGL_ERR_INVALID;
+  return (sgl_status)launch_diag(p, p_is_double, *shape, B, diag, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,462 @@
+// Two-point term, phi packing, Filon inversion (FP64 on the fly, or a dense
+// [n_out x 2 n_q] . [2 n_q x B] contraction on MFMA) and FP64 diagnostics.
+//
+// Inversion: p(t) = (1/pi) Re int_0^inf phi(k) e^{-ikt} dk (sgl.py:994-1012)
+// with phi linear between consecutive k nodes (k_0 = 0, phi_0 = 1): every
+// k-interval [a, a+h] contributes h e^{-iat} [E1(ht) phi_{n+1} + (E0 - E1)(ht) phi_n],
+// E0(th) = int_0^1 e^{-i th s} ds, E1(th) = int_0^1 s e^{-i th s} ds, so the
+// oscillation e^{-ikt} is integrated exactly and only phi is interpolated.
+#include "sgl_amd_internal.h"
+
+namespace sgl {
+
+// ------------------------------------------------------------- quadform --
+// Psi_2[b][n] = 1/2 sum_ij C_ij a_i a_j (complex square, Lam.Q nord = 1,
+// sgl_full.py:3353-3397): u_j = sum_i C_ij a_i, Psi_2 = 1/2 sum_j u_j a_j.
+__global__ __launch_bounds__(256) void k_quadform_fp64(const double* __restrict__ cov,
+                                                       const double* __restrict__ cosmo,
+                                                       const double* __restrict__ amp, int B,
+                                                       int n_z_max, int n_k,
+                                                       double* __restrict__ psi2) {
+  __shared__ double sC[kMaxSlices * kMaxSlices];
+  const int b = blockIdx.y, t = threadIdx.x;
+  const int n = blockIdx.x * 256 + t;
+  const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
+  const double* C = cov + (size_t)b * n_z_max * n_z_max;
+  for (int q = t; q < n_z * n_z; q += 256) sC[q] = C[(q / n_z) * n_z_max + (q % n_z)];
+  __syncthreads();
+  if (n >= n_k) return;
+  double ax[kMaxSlices], ay[kMaxSlices];
+  for (int i = 0; i < n_z; ++i) {
+    size_t o = 2 * (((size_t)b * n_z_max + i) * n_k + n);
+    ax[i] = amp[o];
+    ay[i] = amp[o + 1];
+  }
+  double sr = 0.0, si = 0.0;
+  for (int j = 0; j < n_z; ++j) {
+    double ux = 0.0, uy = 0.0;
+    for (int i = 0; i < n_z; ++i) {
+      double c = sC[i * n_z + j];
+      ux = fma(c, ax[i], ux);
+      uy = fma(c, ay[i], uy);
+    }
+    sr += ux * ax[j] - uy * ay[j];
+    si += ux * ay[j] + uy * ax[j];
+  }
+  psi2[2 * ((size_t)b * n_k + n)] = 0.5 * sr;
+  psi2[2 * ((size_t)b * n_k + n) + 1] = 0.5 * si;
+}
+
+// --------------------------------------------------------------- window --
+__device__ __forceinline__ double bspline_window(double k, double K) {
+  // cubic B-spline of support [0, K], 1 at k = 0 (sgl_full.py:83-94)
+  double x = fabs(k) * 2.0 / K;
+  double w;
+  if (x < 1.0) w = 2.0 / 3.0 - x * x + 0.5 * x * x * x;
+  else if (x < 2.0) w = (2.0 - x) * (2.0 - x) * (2.0 - x) / 6.0;
+  else w = 0.0;
+  return w * 1.5;
+}
+
+// Operand layout: K index kk = 2n (Re phi_n), 2n + 1 (Im phi_n), n = 0..n_q-1,
+// padded with zeros to Kp = round_up(2 n_q, 64).
+inline int k_padded(const sgl_shape& sh) { return ((2 * (sh.n_k + 1) + 63) / 64) * 64; }
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rn(float x) {
+  uint32_t u = __float_as_uint(x);
+  uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(r >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+__global__ void k_phi_pack(const double* __restrict__ lam, const double* __restrict__ psi2,
+                           const double* __restrict__ knodes, sgl_shape sh, int B, int prec,
+                           int Kp, void* __restrict__ out) {
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nq2 = Kp / 2;  // padded node count
+  if (gid >= (size_t)B * nq2) return;
+  const int n = (int)(gid % nq2);
+  const int b = (int)(gid / nq2);
+  double re = 0.0, im = 0.0;
+  if (n == 0) {
+    re = 1.0;
+  } else if (n <= sh.n_k) {
+    size_t o = 2 * ((size_t)b * sh.n_k + (n - 1));
+    double lr = lam[o], li = lam[o + 1];
+    if (psi2 != nullptr) { lr += psi2[o]; li += psi2[o + 1]; }
+    double k = knodes[n - 1];
+    double w = (sh.flags & SGL_FLAG_WINDOW) ? bspline_window(k, knodes[sh.n_k - 1]) : 1.0;
+    double a = w * exp(lr);
+    double s, c;
+    sincos(li, &s, &c);
+    re = a * c;
+    im = a * s;
+  }
+  if (prec == SGL_PREC_FP64) {
+    if (n <= sh.n_k) {
+      double* o = (double*)out + 2 * ((size_t)b * (sh.n_k + 1) + n);
+      o[0] = re;
+      o[1] = im;
+    }
+  } else if (prec == SGL_PREC_FP32) {
+    float* o = (float*)out + (size_t)b * Kp + 2 * n;
+    o[0] = (float)re;
+    o[1] = (float)im;
+  } else {
+    uint16_t* hi = (uint16_t*)out + (size_t)b * Kp + 2 * n;
+    uint16_t* lo = (uint16_t*)out + (size_t)B * Kp + (size_t)b * Kp + 2 * n;
+    float fr = (float)re, fi = (float)im;
+    uint16_t hr = f32_to_bf16_rn(fr), hi_ = f32_to_bf16_rn(fi);
+    hi[0] = hr;
+    hi[1] = hi_;
+    lo[0] = f32_to_bf16_rn(fr - bf16_to_f32(hr));
+    lo[1] = f32_to_bf16_rn(fi - bf16_to_f32(hi_));
+  }
+}
+
+// ---------------------------------------------------------------- Filon --
+__device__ inline void filon_e(double th, double& e0r, double& e0i, double& e1r, double& e1i) {
+  if (fabs(th) < 0.5) {
+    double tr = 1.0, ti = 0.0, a0r = 0.0, a0i = 0.0, a1r = 0.0, a1i = 0.0;
+    for (int m = 0; m < 24; ++m) {
+      a0r += tr / (m + 1);
+      a0i += ti / (m + 1);
+      a1r += tr / (m + 2);
+      a1i += ti / (m + 2);
+      double nr = ti * th / (m + 1), ni = -tr * th / (m + 1);
+      tr = nr;
+      ti = ni;
+    }
+    e0r = a0r; e0i = a0i; e1r = a1r; e1i = a1i;
+    return;
+  }
+  double s, c;
+  sincos(th, &s, &c);
+  double it = 1.0 / th, it2 = it * it;
+  e0r = s * it;
+  e0i = (c - 1.0) * it;
+  e1r = c * it2 + s * it - it2;
+  e1i = c * it - s * it2;
+}
+
+// Contribution of interval [a, a+h] at t: down weight on node n (dr, di), up
+// weight on node n+1 (ur, ui), complex, without the 1/pi.
+__device__ inline void filon_interval(double a, double h, double t, double& dr, double& di,
+                                      double& ur, double& ui) {
+  double e0r, e0i, e1r, e1i;
+  filon_e(h * t, e0r, e0i, e1r, e1i);
+  double s, c;
+  sincos(a * t, &s, &c);
+  double cr = c, ci = -s;  // e^{-iat}
+  ur = h * (cr * e1r - ci * e1i);
+  ui = h * (cr * e1i + ci * e1r);
+  double xr = e0r - e1r, xi = e0i - e1i;
+  dr = h * (cr * xr - ci * xi);
+  di = h * (cr * xi + ci * xr);
+}
+
+__device__ __forceinline__ double knode(const double* kn, int n) { return n == 0 ? 0.0 : kn[n - 1]; }
+
+// E[j][kk] (FP64 first, then converted): thread per (j, interval n).
+__global__ void k_filon_build(const double* __restrict__ knodes, sgl_shape sh, int Kp, int prec,
+                              void* __restrict__ out) {
+  const int j = blockIdx.y;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;  // node index n
+  const int nq = sh.n_k + 1;
+  if (t >= Kp / 2) return;
+  const double tj = sh.out_lo + (sh.out_hi - sh.out_lo) * j / (sh.n_out - 1);
+  double wr = 0.0, wi = 0.0;  // complex weight A_n(t)
+  if (t < nq) {
+    const int n = t;
+    if (n + 1 < nq) {  // down half of [k_n, k_{n+1}]
+      double a = knode(knodes, n), h = knode(knodes, n + 1) - a, dr, di, ur, ui;
+      filon_interval(a, h, tj, dr, di, ur, ui);
+      wr += dr; wi += di;
+    }
+    if (n > 0) {  // up half of [k_{n-1}, k_n]
+      double a = knode(knodes, n - 1), h = knode(knodes, n) - a, dr, di, ur, ui;
+      filon_interval(a, h, tj, dr, di, ur, ui);
+      wr += ur; wi += ui;
+    }
+  }
+  // p = (1/pi) Re(phi A) = (1/pi)(Re phi Re A - Im phi Im A)
+  double er = wr / kPi, ei = -wi / kPi;
+  if (prec == SGL_PREC_FP32) {
+    float* o = (float*)out + (size_t)j * Kp + 2 * t;
+    o[0] = (float)er;
+    o[1] = (float)ei;
+  } else {
+    const size_t plane = (size_t)sh.n_out * Kp;
+    uint16_t* hi = (uint16_t*)out + (size_t)j * Kp + 2 * t;
+    uint16_t* lo = hi + plane;
+    float fr = (float)er, fi = (float)ei;
+    uint16_t a = f32_to_bf16_rn(fr), c = f32_to_bf16_rn(fi);
+    hi[0] = a;
+    hi[1] = c;
+    lo[0] = f32_to_bf16_rn(fr - bf16_to_f32(a));
+    lo[1] = f32_to_bf16_rn(fi - bf16_to_f32(c));
+  }
+}
+
+// Plain path: thread per (b, j), FP64 Filon weights on the fly.
+__global__ __launch_bounds__(256) void k_invert_fp64(const double* __restrict__ phi,
+                                                     const double* __restrict__ knodes,
+                                                     sgl_shape sh, int B, double* __restrict__ p) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (j >= sh.n_out) return;
+  const double tj = sh.out_lo + (sh.out_hi - sh.out_lo) * j / (sh.n_out - 1);
+  const int nq = sh.n_k + 1;
+  const double* ph = phi + 2 * (size_t)b * nq;
+  double acc = 0.0;
+  for (int n = 0; n + 1 < nq; ++n) {
+    double a = knode(knodes, n), h = knode(knodes, n + 1) - a, dr, di, ur, ui;
+    filon_interval(a, h, tj, dr, di, ur, ui);
+    acc += ph[2 * n] * dr - ph[2 * n + 1] * di;
+    acc += ph[2 * (n + 1)] * ur - ph[2 * (n + 1) + 1] * ui;
+  }
+  p[(size_t)b * sh.n_out + j] = acc / kPi;
+}
+
+// ----------------------------------------------------------- MFMA GEMM --
+// p[b][j] = sum_kk E[j][kk] Phi[b][kk]; E: [M = n_out][Kp], Phi: [N = B][Kp],
+// both K-contiguous.  Workgroup tile 64 (j) x 64 (b), 4 waves of 32 x 32,
+// K step 32 through LDS.  FP32: v_mfma_f32_16x16x4_f32.  Split BF16:
+// v_mfma_f32_16x16x32_bf16 on (hi hi + hi lo + lo hi).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kGBM = 64, kGBN = 64, kGBK = 32, kGLd = kGBK + 4;
+
+__global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ E,
+                                                  const float* __restrict__ Phi, int M, int N,
+                                                  int Kp, float* __restrict__ P) {
+  __shared__ float sA[kGBM][kGLd];
+  __shared__ float sB[kGBN][kGLd];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int m0 = blockIdx.x * kGBM, n0 = blockIdx.y * kGBN;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  f32x4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c) acc[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // loader: 64 rows x 32 floats = 512 float4; 256 threads x 2
+  for (int k0 = 0; k0 < Kp; k0 += kGBK) {
+    __syncthreads();
+    for (int q = t; q < kGBM * kGBK / 4; q += 256) {
+      int r = q / (kGBK / 4), c4 = (q % (kGBK / 4)) * 4;
+      int gm = m0 + r, gn = n0 + r;
+      float4 va = (gm < M) ? *(const float4*)(E + (size_t)gm * Kp + k0 + c4) : float4{0, 0, 0, 0};
+      float4 vb = (gn < N) ? *(const float4*)(Phi + (size_t)gn * Kp + k0 + c4) : float4{0, 0, 0, 0};
+      sA[r][c4] = va.x; sA[r][c4 + 1] = va.y; sA[r][c4 + 2] = va.z; sA[r][c4 + 3] = va.w;
+      sB[r][c4] = vb.x; sB[r][c4 + 1] = vb.y; sB[r][c4 + 2] = vb.z; sB[r][c4 + 3] = vb.w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k4 = 0; k4 < kGBK; k4 += 4) {
+      float a0 = sA[wm + (lane & 15)][k4 + (lane >> 4)];
+      float a1 = sA[wm + 16 + (lane & 15)][k4 + (lane >> 4)];
+      float b0 = sB[wn + (lane & 15)][k4 + (lane >> 4)];
+      float b1 = sB[wn + 16 + (lane & 15)][k4 + (lane >> 4)];
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+    }
+  }
+  // D[i][j] of a 16x16 tile: lane holds i = 4 (lane / 16) + r, j = lane % 16
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c)
+      for (int r = 0; r < 4; ++r) {
+        int gm = m0 + wm + a * 16 + 4 * (lane >> 4) + r;
+        int gn = n0 + wn + c * 16 + (lane & 15);
+        if (gm < M && gn < N) P[(size_t)gn * M + gm] = acc[a][c][r];
+      }
+}
+
+constexpr int kHLd = kGBK + 8;  // bf16 row stride in LDS (shorts)
+
+__global__ __launch_bounds__(256) void k_gemm_bf16x3(const uint16_t* __restrict__ E,
+                                                     const uint16_t* __restrict__ Phi, int M,
+                                                     int N, int Kp, float* __restrict__ P) {
+  __shared__ uint16_t sAh[kGBM][kHLd], sAl[kGBM][kHLd];
+  __shared__ uint16_t sBh[kGBN][kHLd], sBl[kGBN][kHLd];
+  const uint16_t* Eh = E;
+  const uint16_t* El = E + (size_t)M * Kp;
+  const uint16_t* Bh = Phi;
+  const uint16_t* Bl = Phi + (size_t)N * Kp;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int m0 = blockIdx.x * kGBM, n0 = blockIdx.y * kGBN;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  f32x4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c) acc[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < Kp; k0 += kGBK) {
+    __syncthreads();
+    // 64 rows x 32 bf16 = 64 rows x 4 uint4 (8 bf16 each) = 256 uint4 per plane
+    {
+      int r = t >> 2, c8 = (t & 3) * 8;
+      int gm = m0 + r, gn = n0 + r;
+      uint4 z = {0, 0, 0, 0};
+      uint4 ah = (gm < M) ? *(const uint4*)(Eh + (size_t)gm * Kp + k0 + c8) : z;
+      uint4 al = (gm < M) ? *(const uint4*)(El + (size_t)gm * Kp + k0 + c8) : z;
+      uint4 bh = (gn < N) ? *(const uint4*)(Bh + (size_t)gn * Kp + k0 + c8) : z;
+      uint4 bl = (gn < N) ? *(const uint4*)(Bl + (size_t)gn * Kp + k0 + c8) : z;
+      *(uint4*)&sAh[r][c8] = ah;
+      *(uint4*)&sAl[r][c8] = al;
+      *(uint4*)&sBh[r][c8] = bh;
+      *(uint4*)&sBl[r][c8] = bl;
+    }
+    __syncthreads();
+    // one 16x16x32 step covers the whole K tile: lane supplies k = 8 (lane/16) .. +7
+    const int kq = 8 * (lane >> 4);
+    bf16x8 ah0 = *(const bf16x8*)&sAh[wm + (lane & 15)][kq];
+    bf16x8 ah1 = *(const bf16x8*)&sAh[wm + 16 + (lane & 15)][kq];
+    bf16x8 al0 = *(const bf16x8*)&sAl[wm + (lane & 15)][kq];
+    bf16x8 al1 = *(const bf16x8*)&sAl[wm + 16 + (lane & 15)][kq];
+    bf16x8 bh0 = *(const bf16x8*)&sBh[wn + (lane & 15)][kq];
+    bf16x8 bh1 = *(const bf16x8*)&sBh[wn + 16 + (lane & 15)][kq];
+    bf16x8 bl0 = *(const bf16x8*)&sBl[wn + (lane & 15)][kq];
+    bf16x8 bl1 = *(const bf16x8*)&sBl[wn + 16 + (lane & 15)][kq];
+#define SGL_MF(A, B, C) C = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, C, 0, 0, 0)
+    SGL_MF(al0, bh0, acc[0][0]); SGL_MF(ah0, bl0, acc[0][0]); SGL_MF(ah0, bh0, acc[0][0]);
+    SGL_MF(al0, bh1, acc[0][1]); SGL_MF(ah0, bl1, acc[0][1]); SGL_MF(ah0, bh1, acc[0][1]);
+    SGL_MF(al1, bh0, acc[1][0]); SGL_MF(ah1, bl0, acc[1][0]); SGL_MF(ah1, bh0, acc[1][0]);
+    SGL_MF(al1, bh1, acc[1][1]); SGL_MF(ah1, bl1, acc[1][1]); SGL_MF(ah1, bh1, acc[1][1]);
+#undef SGL_MF
+  }
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c)
+      for (int r = 0; r < 4; ++r) {
+        int gm = m0 + wm + a * 16 + 4 * (lane >> 4) + r;
+        int gn = n0 + wn + c * 16 + (lane & 15);
+        if (gm < M && gn < N) P[(size_t)gn * M + gm] = acc[a][c][r];
+      }
+}
+
+// ---------------------------------------------------------- diagnostics --
+template <typename T>
+__device__ inline double block_sum(T v, double* red) {
+  const int t = threadIdx.x;
+  red[t] = (double)v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+template <typename PT>
+__global__ __launch_bounds__(256) void k_diag(const PT* __restrict__ p, sgl_shape sh,
+                                              double* __restrict__ diag) {
+  __shared__ double red[256];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int n = sh.n_out;
+  const double h = (sh.out_hi - sh.out_lo) / (n - 1);
+  const PT* pb = p + (size_t)b * n;
+  auto xat = [&](int j) { return sh.out_lo + h * j; };
+  auto wt = [&](int j) { return (j == 0 || j == n - 1) ? 0.5 * h : h; };
+  double m0 = 0, m1 = 0, neg = 0, mu = 0, xd = 0;
+  for (int j = t; j < n; j += 256) {
+    double pj = (double)pb[j], x = xat(j), w = wt(j);
+    m0 += w * pj;
+    m1 += w * x * pj;
+    neg += w * fmax(-pj, 0.0);
+    mu += w * exp(x) * pj;
+    xd += w * exp(-0.5 * x) * pj;
+  }
+  double norm = block_sum(m0, red);
+  double mean = block_sum(m1, red) / norm;
+  double negm = block_sum(neg, red);
+  double mum = block_sum(mu, red) / norm;
+  double md = block_sum(xd, red) / norm;
+  double c2 = 0, c3 = 0, sd = 0;
+  for (int j = t; j < n; j += 256) {
+    double pj = (double)pb[j] / norm, x = xat(j) - mean, w = wt(j);
+    c2 += w * x * x * pj;
+    c3 += w * x * x * x * pj;
+    double y = exp(-0.5 * xat(j)) - md;
+    sd += w * y * y * pj;
+  }
+  double var = block_sum(c2, red);
+  double skw = block_sum(c3, red) / pow(var, 1.5);
+  double sdl = sqrt(block_sum(sd, red)) / md;
+  // P(mu > 2): trapezoid over [ln 2, out_hi] with linear interpolation at ln 2
+  double tail = 0.0;
+  const double x0 = log(2.0);
+  int i = (int)floor((x0 - sh.out_lo) / h) + 1;  // first node > x0
+  if (i < 1) i = 1;
+  if (i <= n - 1) {
+    for (int j = i + t; j < n - 1; j += 256)
+      tail += 0.5 * h * ((double)pb[j] + (double)pb[j + 1]);
+  }
+  tail = block_sum(tail, red);
+  if (t == 0) {
+    if (i <= n - 1) {
+      double xa = xat(i - 1), f = (x0 - xa) / h;
+      double p0 = (double)pb[i - 1] + f * ((double)pb[i] - (double)pb[i - 1]);
+      tail += 0.5 * (p0 + (double)pb[i]) * (xat(i) - x0);
+    }
+    double* d = diag + (size_t)b * 8;
+    d[0] = norm; d[1] = mean; d[2] = var; d[3] = skw;
+    d[4] = sdl; d[5] = mum; d[6] = tail / norm; d[7] = negm;
+  }
+}
+
+// ------------------------------------------------------------- launchers --
+int launch_quadform(const double* cov, const double* cosmo, const double* amp, int B,
+                    const sgl_shape& sh, double* psi2, hipStream_t st) {
+  dim3 grid((sh.n_k + 255) / 256, B);
+  hipLaunchKernelGGL(k_quadform_fp64, grid, dim3(256), 0, st, cov, cosmo, amp, B, sh.n_z_max,
+                     sh.n_k, psi2);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+int launch_phi_pack(const double* lam, const double* psi2, const double* knodes,
+                    const sgl_shape& sh, int B, int prec, void* out, hipStream_t st) {
+  int Kp = k_padded(sh);
+  size_t n = (size_t)B * (Kp / 2);
+  hipLaunchKernelGGL(k_phi_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, lam, psi2,
+                     knodes, sh, B, prec, Kp, out);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+int launch_filon(const double* knodes, const sgl_shape& sh, int prec, void* out, hipStream_t st) {
+  int Kp = k_padded(sh);
+  dim3 grid((Kp / 2 + 255) / 256, sh.n_out);
+  hipLaunchKernelGGL(k_filon_build, grid, dim3(256), 0, st, knodes, sh, Kp, prec, out);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+int launch_invert_fp64(const double* phi, const double* knodes, const sgl_shape& sh, int B,
+                       double* p, hipStream_t st) {
+  dim3 grid((sh.n_out + 255) / 256, B);
+  hipLaunchKernelGGL(k_invert_fp64, grid, dim3(256), 0, st, phi, knodes, sh, B, p);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+int launch_gemm(const void* E, const void* Phi, const sgl_shape& sh, int B, int prec, float* P,
+                hipStream_t st) {
+  int Kp = k_padded(sh);
+  dim3 grid((sh.n_out + kGBM - 1) / kGBM, (B + kGBN - 1) / kGBN);
+  if (prec == SGL_PREC_FP32)
+    hipLaunchKernelGGL(k_gemm_f32, grid, dim3(256), 0, st, (const float*)E, (const float*)Phi,
+                       sh.n_out, B, Kp, P);
+  else
+    hipLaunchKernelGGL(k_gemm_bf16x3, grid, dim3(256), 0, st, (const uint16_t*)E,
+                       (const uint16_t*)Phi, sh.n_out, B, Kp, P);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+int launch_diag(const void* p, int is_double, const sgl_shape& sh, int B, double* diag,
+                hipStream_t st) {
+  if (is_double)
+    hipLaunchKernelGGL(k_diag<double>, dim3(B), dim3(256), 0, st, (const double*)p, sh, diag);
+  else
+    hipLaunchKernelGGL(k_diag<float>, dim3(B), dim3(256), 0, st, (const float*)p, sh, diag);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+}  // namespace sgl

@@ -1,0 +1,98 @@
+// Internal layout and helpers shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/sgl_amd.h"
+#include "sgl_device.h"
+
+namespace sgl {
+
+constexpr int kMaxSlices = 96;
+constexpr int kCovMaxModes = 2048;
+constexpr int kStatusCovModes = 1;      // covariance mode count clamped
+constexpr int kStatusBadShape = 2;
+
+// slice record (8 doubles)
+constexpr int kSliceStride = 8;
+constexpr int kSlZ = 0, kSlChi = 1, kSlDchi = 2, kSlWz = 3, kSlD = 4, kSlScr = 5, kSlE2 = 6;
+// mass record (4 doubles): M [Msun], sigma(M, 0), dln sigma / dln M
+constexpr int kMassStride = 4;
+// row record (8 doubles)
+constexpr int kRowStride = 8;
+constexpr int kRowKs = 0, kRowRs = 1, kRowXt = 2, kRowRvir = 3, kRowW = 4, kRowBeta = 5;
+
+// Offsets (in doubles) of the sub-arrays of the tables buffer.
+struct TablesLayout {
+  size_t cosmo, derived, slices, mass, rows, cov, work, m_nodes, uedge, knodes, status, total;
+};
+
+inline TablesLayout tables_layout(const sgl_shape& s, int B) {
+  TablesLayout L;
+  size_t o = 0;
+  auto take = [&](size_t n) { size_t r = o; o += (n + 31) & ~size_t(31); return r; };
+  L.cosmo = take((size_t)B * 16);
+  L.derived = take((size_t)B * 16);
+  L.slices = take((size_t)B * s.n_z_max * kSliceStride);
+  L.mass = take((size_t)B * s.n_m * kMassStride);
+  L.rows = take((size_t)B * s.n_z_max * s.n_m * kRowStride);
+  L.cov = take((size_t)B * s.n_z_max * s.n_z_max);
+  L.work = take((size_t)B * 20000);
+  L.m_nodes = take((size_t)s.n_m);
+  L.uedge = take((size_t)s.n_b + 1);
+  L.knodes = take((size_t)s.n_k);
+  L.status = take(1);
+  L.total = o;
+  return L;
+}
+
+// Fine jump lattice of the batched path: signed log bins.
+//   positive side: kFinePos bins log-uniform on [kXiLo, kXiMax], bin 0 also
+//   takes 0 < xi < kXiLo; negative side: kFineNeg bins log-uniform on |xi|.
+constexpr int kFinePos = 2048;
+constexpr int kFineNeg = 256;
+constexpr int kFineBins = kFinePos + kFineNeg;
+constexpr double kXiLo = 1e-12;
+constexpr double kXiHi = 30.0;
+constexpr double kXiNegLo = 1e-8;
+constexpr double kXiNegHi = 30.0;
+// Per-slice coarse lattice (two-point amplitudes).
+constexpr int kSlicePos = 112;
+constexpr int kSliceNeg = 16;
+constexpr int kSliceBins = kSlicePos + kSliceNeg;
+// moments per bin (FP64): w, w xi, w xi^2
+constexpr int kMoments = 3;
+
+inline size_t hist_layout_fine(int B) { return (size_t)B * kFineBins * kMoments; }
+inline size_t hist_layout_slice(const sgl_shape& s, int B) {
+  return (size_t)B * s.n_z_max * kSliceBins * kMoments;
+}
+
+int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes_h,
+                 double* derived, double* slices, double* mass, double* rows, double* cov,
+                 double* work, int* status, hipStream_t st);
+int launch_fill_grids(const sgl_shape& sh, double* ue, double* kn, double* m_nodes_h,
+                      hipStream_t st);
+int launch_lnphi_exact(const double* rows, const double* cosmo, const double* uedge,
+                       const double* knodes, const sgl_shape& sh, int B, double* lam, double* amp,
+                       double* work, hipStream_t st);
+int launch_bin(const double* rows, const double* cosmo, const double* uedge, const sgl_shape& sh,
+               int B, double* hfine, double* hslice, hipStream_t st);
+int launch_lnphi_binned(const double* hfine, const double* hslice, const double* cosmo,
+                        const double* knodes, const sgl_shape& sh, int B, double* lam,
+                        double* amp, hipStream_t st);
+int launch_quadform(const double* cov, const double* cosmo, const double* amp, int B,
+                    const sgl_shape& sh, double* psi2, hipStream_t st);
+int launch_phi_pack(const double* lam, const double* psi2, const double* knodes,
+                    const sgl_shape& sh, int B, int prec, void* out, hipStream_t st);
+int launch_filon(const double* knodes, const sgl_shape& sh, int prec, void* out, hipStream_t st);
+int launch_invert_fp64(const double* phi, const double* knodes, const sgl_shape& sh, int B,
+                       double* p, hipStream_t st);
+int launch_gemm(const void* E, const void* Phi, const sgl_shape& sh, int B, int prec, float* P,
+                hipStream_t st);
+int launch_diag(const void* p, int is_double, const sgl_shape& sh, int B, double* diag,
+                hipStream_t st);
+inline int k_padded_host(const sgl_shape& sh) { return ((2 * (sh.n_k + 1) + 63) / 64) * 64; }
+
+}  // namespace sgl

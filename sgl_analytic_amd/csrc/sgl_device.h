@@ -1,0 +1,254 @@
+// Device-side FP64 physics shared by the table, cell and exponent kernels.
+//
+// Cosmology: the configuration sgl_analytic/pipeline.py:36-72 builds
+// (smooth-k sigma(M) window, Eisenstein & Hu 1998 transfer, top-hat sigma8
+// anchor, Ludlow+16 concentrations, growth ODE, radiation Omega_r = Om/(1+zeq)).
+// Lens: projected NFW of Wright & Brainerd with the series / cancellation-free
+// branches of sgl.py:694-720, truncated at x_t with the closed form of
+// sgl_full.py:740-798.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace sgl {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kCkms = 299792.458;          // km/s
+constexpr double kRhoc0Unit = 2.77536627e11;  // Msun / Mpc^3 / h^2
+constexpr double kGnewt = 4.30091e-9;         // Mpc (km/s)^2 / Msun
+constexpr double kTcmb = 2.7255;
+constexpr double kDeltaC = 1.686;             // Sheth-Tormen delta_c
+constexpr double kPst = 0.3, kQst = 0.8;
+
+// Cosmological parameters of one instance, as laid out in HBM (16 doubles).
+struct Cosmo {
+  double h, Om, s8, Ob, ns, zeq, Ok, w0, wa, zs;
+  double n_z;  // slice count of this instance (stored as double)
+  double pad[5];
+};
+
+// Derived per-instance scalars (16 doubles), written by the background kernel.
+struct Derived {
+  double Or, OL, H0, rhoc0, rhom, chis, DAs, Anorm;
+  double D0, dchi, n_z, pad[5];
+};
+
+__device__ __forceinline__ double e2_of_z(const Cosmo& c, double Or, double OL, double z) {
+  // E(z)^2 for CPL dark energy, curvature and radiation (sgl.py:185-197)
+  double zp1 = 1.0 + z;
+  double de;
+  if (c.wa == 0.0)
+    de = (c.w0 == -1.0) ? OL : OL * pow(zp1, 3.0 * (1.0 + c.w0));
+  else
+    de = OL * pow(zp1, 3.0 * (1.0 + c.w0 + c.wa)) * exp(-3.0 * c.wa * (1.0 - 1.0 / zp1));
+  double z2 = zp1 * zp1;
+  return c.Om * z2 * zp1 + Or * z2 * z2 + c.Ok * z2 + de;
+}
+
+// Transverse comoving distance f_K(chi) (sgl.py:219-238).
+__device__ __forceinline__ double f_k(double Ok, double H0, double chi) {
+  if (Ok == 0.0) return chi;
+  double u = chi * H0 / kCkms;
+  double t = Ok * u * u;
+  if (fabs(t) < 1e-5) return chi * (1.0 + t / 6.0 + t * t / 120.0 + t * t * t / 5040.0);
+  double x = sqrt(fabs(Ok)) * u;
+  return chi * ((Ok > 0.0) ? sinh(x) / x : sin(x) / x);
+}
+
+// Eisenstein & Hu 1998 CDM+baryon transfer function T(k), k in 1/Mpc
+// (the formula set of sgl.py:391-445).  The k-independent constants are
+// computed once per instance into EH98Const.
+struct EH98Const {
+  double keq, ksilk, ac, bc, s, ab, bb, bnode, fb, fc;
+};
+
+__device__ inline EH98Const eh98_const(const Cosmo& c) {
+  EH98Const k;
+  double h = c.h, Om = c.Om, Ob = c.Ob, Oc = Om - Ob;
+  double wm = Om * h * h, wb = Ob * h * h;
+  k.keq = (100.0 * h / kCkms) * sqrt(2.0 * Om) * sqrt(1.0 + c.zeq);
+  k.ksilk = 0.0016 * pow(wb, 0.52) * pow(wm, 0.73) * (1.0 + pow(10.4 * wm, -0.95));
+  double a1 = pow(46.9 * wm, 0.67) * (1.0 + pow(32.1 * wm, -0.532));
+  double a2 = pow(12.0 * wm, 0.424) * (1.0 + pow(45.0 * wm, -0.582));
+  double b1 = 0.944 / (1.0 + pow(458.0 * wm, -0.708));
+  double b2 = pow(0.395 * wm, -0.026);
+  double fbm = Ob / Om;
+  k.ac = pow(a1, -fbm) * pow(a2, -fbm * fbm * fbm);
+  k.bc = 1.0 / (1.0 + b1 * (pow(Oc / Om, b2) - 1.0));
+  k.s = 44.5 * log(9.83 / wm) / sqrt(1.0 + 10.0 * pow(wb, 0.75));
+  double b3 = 0.313 * pow(wm, -0.419) * (1.0 + 0.607 * pow(wm, 0.674));
+  double b4 = 0.238 * pow(wm, 0.223);
+  double zd = 1291.0 * pow(wm, 0.251) / (1.0 + 0.659 * pow(wm, 0.828)) * (1.0 + b3 * pow(wb, b4));
+  double T27 = kTcmb / 2.7;
+  double Rd = 31.5 * wb / (T27 * T27 * T27 * T27) / (zd / 1000.0);
+  double y = (1.0 + c.zeq) / (1.0 + zd);
+  double sy = sqrt(1.0 + y);
+  double g2 = y * (-6.0 * sy + (2.0 + 3.0 * y) * log((sy + 1.0) / (sy - 1.0)));
+  k.ab = 2.07 * k.keq * k.s * pow(1.0 + Rd, -0.75) * g2;
+  k.bb = 0.5 + fbm + (3.0 - 2.0 * fbm) * sqrt(1.0 + (17.2 * wm) * (17.2 * wm));
+  k.bnode = 8.41 * pow(wm, 0.435);
+  k.fb = fbm;
+  k.fc = Oc / Om;
+  return k;
+}
+
+__device__ __forceinline__ double eh98_t0(double q, double a_c, double b_c) {
+  double C1 = 14.2 / a_c + 386.0 / (1.0 + 69.9 * pow(q, 1.08));
+  double num = log(2.718281828459045 + 1.8 * b_c * q);
+  return num / (num + C1 * q * q);
+}
+
+__device__ inline double eh98_T(const EH98Const& e, double k) {
+  double q = k / (13.41 * e.keq);
+  double ks = k * e.s;
+  double t54 = ks / 5.4;
+  double fk = 1.0 / (1.0 + t54 * t54 * t54 * t54);
+  double TC = fk * eh98_t0(q, 1.0, e.bc) + (1.0 - fk) * eh98_t0(q, e.ac, e.bc);
+  double bn = e.bnode / ks;
+  double s3 = e.s / cbrt(1.0 + bn * bn * bn);
+  double x = k * s3;
+  double jo = (x < 1e-8) ? 1.0 - x * x / 6.0 : sin(x) / x;
+  double t52 = ks / 5.2;
+  double bbk = e.bb / ks;
+  double TB = (eh98_t0(q, 1.0, 1.0) / (1.0 + t52 * t52) +
+               e.ab / (1.0 + bbk * bbk * bbk) * exp(-pow(k / e.ksilk, 1.4))) * jo;
+  return e.fb * TB + e.fc * TC;
+}
+
+// ---------------------------------------------------------------- NFW ----
+constexpr double kNfwWSeries = 0.25;
+constexpr int kNfwNSeries = 28;
+constexpr double kNfwXH = 0.9;
+constexpr double kTcASmall = 1e-3;
+constexpr double kTcBand = 1e-5;
+
+__device__ inline void nfw_fh(double x, double& F, double& h) {
+  double w = (1.0 - x) * (1.0 + x);
+  if (fabs(w) < kNfwWSeries) {
+    double T = 0.0;
+#pragma unroll
+    for (int n = kNfwNSeries - 1; n >= 0; --n) T = T * w + 1.0 / (2 * n + 3);
+    F = T;
+    h = log(0.5 * x) + (1.0 + w * T);
+    return;
+  }
+  if (x < 1.0) {
+    double e = (1.0 - x) / x, s = sqrt(w);
+    double A = log1p(e + sqrt(e * (e + 2.0)));
+    F = (A / s - 1.0) / w;
+    h = (x < kNfwXH) ? (log(2.0 / x) * x * x / (1.0 + s) + log1p(-x * x / (2.0 * (1.0 + s)))) / s
+                     : log(0.5 * x) + A / s;
+    return;
+  }
+  double v = -w, sv = sqrt(v), B = atan(sv);
+  F = (1.0 - B / sv) / v;
+  h = log(0.5 * x) + B / sv;
+}
+
+__device__ inline void tc_closed_raw(double x, double c, double& I1, double& I2) {
+  double q = sqrt((c - x) * (c + x));
+  double d = (x - 1.0) * (x - c) / (x * (1.0 + c));
+  double L = log((c + q) / (2.0 * (1.0 + c))) - (q - c) / (1.0 + c);
+  if (x < 1.0) {
+    double u = (1.0 - x) * (1.0 + x), e = (1.0 - x) / x;
+    double A0 = log1p(e + sqrt(e * (e + 2.0)));
+    double At = log1p(d + sqrt(d * (d + 2.0)));
+    double su = sqrt(u);
+    I1 = (q / (1.0 + c) - 1.0) / u + (A0 - At) / (u * su);
+    I2 = (L + (A0 - At) / su) / (x * x);
+    return;
+  }
+  double v = (x - 1.0) * (x + 1.0);
+  double B0 = 2.0 * asin(sqrt((x - 1.0) / (2.0 * x)));
+  double md = (-d > 0.0) ? -d : 0.0;
+  double Bt = 2.0 * asin(sqrt(md / 2.0));
+  double sv = sqrt(v);
+  I1 = (1.0 - q / (1.0 + c)) / v - (B0 - Bt) / (v * sv);
+  I2 = (L + (B0 - Bt) / sv) / (x * x);
+}
+
+// 24-node Gauss-Legendre rule on [0, 1] (nodes of P_24 mapped by (z+1)/2).
+static __constant__ double kGl24X[24] = {
+    0.0024063900014893447, 0.012635722014345263, 0.0308627239986336, 0.05679223649779952,
+    0.08999900701304853, 0.12993790421072282, 0.17595317403151223, 0.22728926430558022,
+    0.28310324618697746, 0.3424786601519183, 0.40444056626319186, 0.4679715535686972,
+    0.5320284464313028, 0.5955594337368082, 0.6575213398480817, 0.7168967538130225,
+    0.7727107356944198, 0.8240468259684878, 0.8700620957892772, 0.9100009929869515,
+    0.9432077635022005, 0.9691372760013663, 0.9873642779856547, 0.9975936099985107};
+static __constant__ double kGl24W[24] = {
+    0.0061706148999935454, 0.014265694314466872, 0.022138719408709776, 0.02964929245771837,
+    0.036673240705540205, 0.043095080765976644, 0.04880932605205703, 0.0537221350579828,
+    0.05775283402686281, 0.06083523646390171, 0.06291872817341415, 0.06396909767337611,
+    0.06396909767337611, 0.06291872817341415, 0.06083523646390171, 0.05775283402686281,
+    0.0537221350579828, 0.04880932605205703, 0.043095080765976644, 0.036673240705540205,
+    0.02964929245771837, 0.022138719408709776, 0.014265694314466872, 0.0061706148999935454};
+
+__device__ inline void tc_gl(double a, double xt, double& s1o, double& s2o) {
+  double s1 = 0.0, s2 = 0.0;
+  for (int j = 0; j < 24; ++j) {
+    double v = kGl24X[j], s = 1.0 - v * v;
+    double d = fmax(1.0 - a * a * s * s, 1e-300);
+    s1 += kGl24W[j] * 2.0 * v * s / ((s + xt) * (s + xt) * sqrt(d));
+    double e = fmax(1.0 - a * a * v * v, 0.0);
+    s2 += kGl24W[j] * v / ((v + xt) * (v + xt) * (1.0 + sqrt(e)));
+  }
+  s1o = s1;
+  s2o = s2;
+}
+
+__device__ inline void tc_point(double x, double xt, double& I1, double& I2) {
+  double a = x / xt;
+  if (a >= 1.0) { I1 = 0.0; I2 = 0.0; return; }
+  if (a < kTcASmall) {
+    double i0 = log((1.0 + xt) / xt) - 1.0 / (1.0 + xt), u1 = 1.0 + xt;
+    double jj = 0.5 * (u1 * u1 - xt * xt) - 3.0 * xt * (u1 - xt) + 3.0 * xt * xt * log(u1 / xt) +
+                xt * xt * xt * (1.0 / u1 - 1.0 / xt);
+    I1 = i0 + 0.5 * a * a * jj;
+    I2 = 0.5 * i0 + 0.125 * a * a * jj;
+    return;
+  }
+  if (fabs(x - 1.0) < kTcBand) {
+    if (xt <= 1.0 + 2.0 * kTcBand) { tc_gl(a, xt, I1, I2); return; }
+    double a1, b1, a2, b2;
+    tc_closed_raw(1.0 - kTcBand, xt, a1, b1);
+    tc_closed_raw(1.0 + kTcBand, xt, a2, b2);
+    double f = (x - (1.0 - kTcBand)) / (2.0 * kTcBand);
+    I1 = a1 + f * (a2 - a1);
+    I2 = b1 + f * (b2 - b1);
+    return;
+  }
+  tc_closed_raw(x, xt, I1, I2);
+}
+
+// (kappa, gamma) of an NFW halo truncated at x_t (amplitude ks).
+__device__ inline void trunc_kg(double x, double ks, double xt, double& kap, double& gam) {
+  if (x < xt) {
+    double F, h;
+    nfw_fh(x, F, h);
+    double ku = 2.0 * F, gu = 4.0 * h / (x * x) - ku;
+    double I1, I2;
+    tc_point(x, xt, I1, I2);
+    double k = ku - 2.0 * I1, kb = ku + gu - 4.0 * I2;
+    kap = ks * k;
+    gam = ks * (kb - k);
+  } else {
+    double mut = log1p(xt) - xt / (1.0 + xt);
+    kap = 0.0;
+    gam = ks * 4.0 * mut / (x * x);
+  }
+}
+
+// Jump of one cell: xi = -ln detA and the tilted weight w e^{-xi} = w detA;
+// dropped (weight 0) when detA <= 0 or xi > 30.
+constexpr double kXiMax = 30.0;
+__device__ __forceinline__ void cell_jump(double kap, double gam, double w, double& xi, double& ws) {
+  double detA = (1.0 - kap) * (1.0 - kap) - gam * gam;
+  if (!(detA > 0.0)) { xi = 0.0; ws = 0.0; return; }
+  double x = -log(detA);
+  if (x > kXiMax) { xi = 0.0; ws = 0.0; return; }
+  xi = x;
+  ws = w * detA;
+}
+
+}  // namespace sgl

@@ -1,0 +1,465 @@
+// Lens-population tables on the device, batched over cosmologies.
+//
+// For every instance (cosmology + source redshift) the kernels below build
+//   derived scalars     Omega_r, Omega_DE, H0, rho_c0, rho_m, chi(z_s), D_A(z_s), A_norm
+//   slice table         z, chi, dchi, dchi (1+z)^2, D(z), Sigma_cr(z, z_s), E(z)
+//   mass table          M, sigma(M, 0), dln sigma / dln M
+//   row table           per (slice, mass): k_s, r_s, x_t = c, r_vir, w_row, beta
+//   slice covariance    C_ij of the smoothed 1-D linear density field
+// following sgl.Cosmology (sgl.py:65-615) on the make_cosmology configuration
+// (pipeline.py:36-72), the builder's per-(z, M) quantities
+// (sgl_full.py:1848-1885, 541-543; subhalos.py:381-385, 472-481) and
+// shell_covariance (sgl_full.py:2514-2538).
+#include "sgl_amd_internal.h"
+
+namespace sgl {
+
+// ------------------------------------------------------------ background --
+// One workgroup per instance.  chi(z) = (c/H0) int dz / E on NPANEL uniform
+// z-panels with 8-point Gauss-Legendre, fixed-order prefix sum, then Newton
+// on each slice midpoint chi_i = (i + 1/2) chi_s / n_z.
+constexpr int kPanels = 2048;
+__constant__ double kGl8X[8] = {-0.9602898564975363, -0.7966664774136267, -0.5255324099163290,
+                                -0.1834346424956498, 0.1834346424956498, 0.5255324099163290,
+                                0.7966664774136267, 0.9602898564975363};
+__constant__ double kGl8W[8] = {0.1012285362903763, 0.2223810344533745, 0.3137066458778873,
+                                0.3626837833783620, 0.3626837833783620, 0.3137066458778873,
+                                0.2223810344533745, 0.1012285362903763};
+
+__device__ inline double inv_e_integral(const Cosmo& c, double Or, double OL, double a, double b) {
+  double m = 0.5 * (a + b), r = 0.5 * (b - a), s = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s += kGl8W[j] / sqrt(e2_of_z(c, Or, OL, m + r * kGl8X[j]));
+  return s * r;
+}
+
+__global__ __launch_bounds__(256) void k_background(const Cosmo* __restrict__ cosmo, int n_z_max,
+                                                    Derived* __restrict__ der,
+                                                    double* __restrict__ slices) {
+  __shared__ double chi_e[kPanels + 1];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const Cosmo c = cosmo[b];
+  const double Or = c.Om / (1.0 + c.zeq);
+  const double OL = 1.0 - c.Om - c.Ok - Or;
+  const double H0 = 100.0 * c.h;
+  const double dh = kCkms / H0;
+  const double zs = c.zs;
+  const int n_z = (int)c.n_z;
+  const double dz = zs / kPanels;
+  for (int p = t; p < kPanels; p += blockDim.x)
+    chi_e[p + 1] = dh * inv_e_integral(c, Or, OL, p * dz, (p + 1) * dz);
+  __syncthreads();
+  if (t == 0) {
+    chi_e[0] = 0.0;
+    for (int p = 1; p <= kPanels; ++p) chi_e[p] += chi_e[p - 1];
+  }
+  __syncthreads();
+  const double chis = chi_e[kPanels];
+  const double dchi = chis / n_z;
+  const double DAs = f_k(c.Ok, H0, chis) / (1.0 + zs);
+  if (t == 0) {
+    Derived d = {};
+    d.Or = Or; d.OL = OL; d.H0 = H0; d.rhoc0 = kRhoc0Unit * c.h * c.h;
+    d.rhom = c.Om * d.rhoc0; d.chis = chis; d.DAs = DAs; d.dchi = dchi; d.n_z = n_z;
+    der[b] = d;
+  }
+  for (int i = t; i < n_z_max; i += blockDim.x) {
+    double* S = slices + ((size_t)b * n_z_max + i) * kSliceStride;
+    if (i >= n_z) {
+      for (int q = 0; q < kSliceStride; ++q) S[q] = 0.0;
+      continue;
+    }
+    double target = (i + 0.5) * dchi;
+    int lo = 0, hi = kPanels;  // chi_e[lo] <= target < chi_e[hi]
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (chi_e[mid] <= target) lo = mid; else hi = mid;
+    }
+    double z0 = lo * dz;
+    double z = z0 + (target - chi_e[lo]) / (chi_e[lo + 1] - chi_e[lo]) * dz;
+    for (int it = 0; it < 6; ++it) {
+      double F = chi_e[lo] + dh * inv_e_integral(c, Or, OL, z0, z) - target;
+      z -= F * sqrt(e2_of_z(c, Or, OL, z)) / dh;
+    }
+    double E2 = e2_of_z(c, Or, OL, z);
+    double DAl = f_k(c.Ok, H0, target) / (1.0 + z);
+    double DAls = f_k(c.Ok, H0, chis - target) / (1.0 + zs);
+    S[kSlZ] = z;
+    S[kSlChi] = target;
+    S[kSlDchi] = dchi;
+    S[kSlWz] = dchi * (1.0 + z) * (1.0 + z);
+    S[kSlD] = 0.0;
+    S[kSlScr] = kCkms * kCkms * DAs / (4.0 * kPi * kGnewt * DAl * DAls);
+    S[kSlE2] = E2;
+    S[7] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------- growth --
+// D'' + (2 + dlnH/dlna) D' - 3/2 Omega_m(a) D = 0 in x = ln a (sgl.py:310-347),
+// classical RK4 on kGrowthSteps uniform steps from a = 1e-6, cubic Hermite
+// dense output at every slice node, normalised by D(a = 1).
+constexpr int kGrowthSteps = 4096;
+
+struct GrowthRhs {
+  double Om, Ok, Or, OL, w, wa;
+  bool lambda_de;
+  __device__ inline void operator()(double x, double y0, double y1, double& d0, double& d1) const {
+    double e1 = exp(-x);
+    double e2 = e1 * e1;
+    double matter = Om * e2 * e1;
+    double curv = Ok * e2;
+    double rad = Or * e2 * e2;
+    double de, wz;
+    if (lambda_de) {
+      de = OL;
+      wz = -1.0;
+    } else {
+      double a = 1.0 / e1;
+      de = OL * exp(-3.0 * (1.0 + w + wa) * x + 3.0 * wa * (a - 1.0));
+      wz = w + wa * (1.0 - a);
+    }
+    double total = matter + rad + curv + de;
+    double dh = -0.5 * (3.0 * matter + 4.0 * rad + 2.0 * curv + 3.0 * (1.0 + wz) * de) / total;
+    d0 = y1;
+    d1 = -(2.0 + dh) * y1 + 1.5 * (matter / total) * y0;
+  }
+};
+
+__global__ __launch_bounds__(64) void k_growth(const Cosmo* __restrict__ cosmo, int B, int n_z_max,
+                                               Derived* __restrict__ der,
+                                               double* __restrict__ slices) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const Cosmo c = cosmo[b];
+  const Derived d = der[b];
+  GrowthRhs f{c.Om, c.Ok, d.Or, d.OL, c.w0, c.wa, c.w0 == -1.0 && c.wa == 0.0};
+  const int n_z = (int)c.n_z;
+  const double x0 = log(1e-6);
+  const double h = -x0 / kGrowthSteps;
+  double y0 = exp(x0) + (2.0 / 3.0) * d.Or / c.Om, y1 = exp(x0);
+  double* Sb = slices + (size_t)b * n_z_max * kSliceStride;
+  int next = n_z - 1;  // slices are ordered by increasing z, i.e. decreasing x
+  double xnext = (next >= 0) ? -log1p(Sb[next * kSliceStride + kSlZ]) : 1.0;
+  for (int n = 0; n < kGrowthSteps; ++n) {
+    double x = x0 + n * h;
+    double k1a, k1b, k2a, k2b, k3a, k3b, k4a, k4b;
+    f(x, y0, y1, k1a, k1b);
+    f(x + 0.5 * h, y0 + 0.5 * h * k1a, y1 + 0.5 * h * k1b, k2a, k2b);
+    f(x + 0.5 * h, y0 + 0.5 * h * k2a, y1 + 0.5 * h * k2b, k3a, k3b);
+    f(x + h, y0 + h * k3a, y1 + h * k3b, k4a, k4b);
+    double n0 = y0 + h / 6.0 * (k1a + 2.0 * k2a + 2.0 * k3a + k4a);
+    double n1 = y1 + h / 6.0 * (k1b + 2.0 * k2b + 2.0 * k3b + k4b);
+    double xe = (n == kGrowthSteps - 1) ? 0.0 : x + h;
+    while (next >= 0 && xnext <= xe) {
+      // cubic Hermite on [x, xe] with values y0, n0 and slopes y1, n1
+      double hh = xe - x, s = (xnext - x) / hh;
+      double s2 = s * s, s3 = s2 * s;
+      double h00 = 2 * s3 - 3 * s2 + 1, h10 = s3 - 2 * s2 + s, h01 = -2 * s3 + 3 * s2,
+             h11 = s3 - s2;
+      Sb[next * kSliceStride + kSlD] = h00 * y0 + h10 * hh * y1 + h01 * n0 + h11 * hh * n1;
+      --next;
+      xnext = (next >= 0) ? -log1p(Sb[next * kSliceStride + kSlZ]) : 1.0;
+    }
+    y0 = n0;
+    y1 = n1;
+  }
+  for (int i = 0; i < n_z; ++i) Sb[i * kSliceStride + kSlD] /= y0;
+  der[b].D0 = y0;
+}
+
+// ----------------------------------------------------------------- sigma --
+// sigma(R) by the trapezoid over k = logspace(-5, 3, 4000) (sgl.py:458-460,
+// 510-515), top-hat anchor at 8/h Mpc (sgl.py:517-538), smooth-k window
+// Ws = 1/(1 + (0.43 x)^6) for sigma(M); dln sigma/dln M from the same sum
+// with the analytic window derivative.
+constexpr int kSigmaK = 4000;
+
+__global__ __launch_bounds__(256) void k_sigma(const Cosmo* __restrict__ cosmo,
+                                               const double* __restrict__ m_nodes_h, int n_m,
+                                               Derived* __restrict__ der,
+                                               double* __restrict__ mass) {
+  __shared__ double g[kSigmaK];   // k^2 Pshape w_trap / (2 pi^2)
+  __shared__ double kk[kSigmaK];
+  __shared__ double red[256];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const Cosmo c = cosmo[b];
+  const EH98Const e = eh98_const(c);
+  const double lstep = 8.0 / (kSigmaK - 1);
+  for (int i = t; i < kSigmaK; i += blockDim.x) kk[i] = pow(10.0, -5.0 + lstep * i);
+  __syncthreads();
+  for (int i = t; i < kSigmaK; i += blockDim.x) {
+    double k = kk[i];
+    double T = eh98_T(e, k);
+    double wl = (i == 0) ? 0.5 * (kk[1] - kk[0])
+                : (i == kSigmaK - 1) ? 0.5 * (kk[i] - kk[i - 1]) : 0.5 * (kk[i + 1] - kk[i - 1]);
+    g[i] = k * k * pow(k, c.ns) * T * T * wl / (2.0 * kPi * kPi);
+  }
+  __syncthreads();
+  // top-hat anchor
+  const double R8 = 8.0 / c.h;
+  double s = 0.0;
+  for (int i = t; i < kSigmaK; i += blockDim.x) {
+    double x = kk[i] * R8;
+    double W = 3.0 * (sin(x) - x * cos(x)) / (x * x * x);
+    s += g[i] * W * W;
+  }
+  red[t] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  const double Anorm = c.s8 / sqrt(red[0]);
+  const double rhom = c.Om * kRhoc0Unit * c.h * c.h;
+  if (t == 0) der[b].Anorm = Anorm;
+  const double a6 = pow(0.43, 6.0);
+  for (int j = t; j < n_m; j += blockDim.x) {
+    double M = m_nodes_h[j] / c.h;
+    double R = cbrt(3.0 * M / (4.0 * kPi * rhom));
+    double S = 0.0, Sd = 0.0;
+    for (int i = 0; i < kSigmaK; ++i) {
+      double x = kk[i] * R;
+      double x2 = x * x, x3 = x2 * x;
+      double y = a6 * x3 * x3;
+      double Ws = 1.0 / (1.0 + y);
+      double dW = -6.0 * a6 * x2 * x3 * Ws * Ws;  // dWs/dx
+      S += g[i] * Ws * Ws;
+      Sd += g[i] * 2.0 * Ws * dW * kk[i];
+    }
+    double* Mr = mass + ((size_t)b * n_m + j) * kMassStride;
+    Mr[0] = M;
+    Mr[1] = Anorm * sqrt(S);
+    Mr[2] = 0.5 * (Sd / S) * (R / 3.0);
+    Mr[3] = 0.0;
+  }
+}
+
+// ------------------------------------------------------------------ rows --
+__device__ inline double mu_nfw(double y) { return log1p(y) - y / (1.0 + y); }
+
+// r_vir / r200 = eta solving m^3 / mu(m) = 200 c^3 / (mu(c) Delta_vir) on
+// [c, 3c] (subhalos.py:409-441): safeguarded Newton on ln(m^3 / mu(m)).
+__device__ inline double eta_rvir(double c, double Dvir) {
+  double lt = log(200.0 * c * c * c / mu_nfw(c) / Dvir);
+  double lo = c, hi = 3.0 * c, m = 0.5 * (lo + hi);
+  for (int it = 0; it < 80; ++it) {
+    double mu = mu_nfw(m);
+    double f = log(m * m * m / mu) - lt;
+    if (f < 0.0) lo = m; else hi = m;
+    double dmu = m / ((1.0 + m) * (1.0 + m));
+    double fp = 3.0 / m - dmu / mu;
+    double mn = m - f / fp;
+    if (!(lo < mn && mn < hi)) mn = 0.5 * (lo + hi);
+    if (fabs(mn - m) <= 4e-16 * m || hi - lo <= 4e-16 * m) { m = mn; break; }
+    m = mn;
+  }
+  return m / c;
+}
+
+__global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
+                                              const Derived* __restrict__ der,
+                                              const double* __restrict__ slices,
+                                              const double* __restrict__ mass, int B, int n_z_max,
+                                              int n_m, double dlnM, double* __restrict__ rows) {
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)B * n_z_max * n_m;
+  if (gid >= total) return;
+  const int j = (int)(gid % n_m);
+  const int i = (int)((gid / n_m) % n_z_max);
+  const int b = (int)(gid / ((size_t)n_m * n_z_max));
+  double* R = rows + gid * kRowStride;
+  const Cosmo c = cosmo[b];
+  if (i >= (int)c.n_z) {
+    for (int q = 0; q < kRowStride; ++q) R[q] = 0.0;
+    return;
+  }
+  const Derived d = der[b];
+  const double* S = slices + ((size_t)b * n_z_max + i) * kSliceStride;
+  const double* Mr = mass + ((size_t)b * n_m + j) * kMassStride;
+  const double z = S[kSlZ], D = S[kSlD], E2 = S[kSlE2];
+  const double M = Mr[0], sig0 = Mr[1], dls = Mr[2];
+  // Sheth-Tormen dn/dlnM (sgl.py:558-567)
+  const double A_ST = 1.0 / (1.0 + pow(2.0, -0.3) * tgamma(0.2) / sqrt(kPi));
+  double sig = sig0 * D;
+  double nu = (kDeltaC / sig) * (kDeltaC / sig);
+  double qn = kQst * nu;
+  double mult = A_ST * (1.0 + pow(qn, -kPst)) * sqrt(qn / (2.0 * kPi)) * exp(-0.5 * qn);
+  double dndlnM = (d.rhom / M) * mult * (-2.0 * dls);
+  // Ludlow+16 concentration with the z <= 7 clamp (sgl.py:590-606)
+  double zc = fmin(z, 7.0), a = 1.0 + zc;
+  double poly = 4.135 - 0.564 * a - 0.210 * a * a + 0.0557 * a * a * a - 0.00348 * a * a * a * a;
+  double q = kDeltaC / (sig0 * poly);
+  double c0 = 3.395 * pow(a, -0.215), beta_ = 0.307 * pow(a, 0.540);
+  double g1 = 0.628 * pow(a, -0.047), g2 = 0.317 * pow(a, -0.893);
+  double C = c0 * pow(q, -g1) * pow(1.0 + pow(q, 1.0 / beta_), -beta_ * (g2 - g1));
+  // nfw_params (sgl.py:608-615)
+  double rhoc = d.rhoc0 * E2;
+  double r200 = cbrt(3.0 * M / (4.0 * kPi * 200.0 * rhoc));
+  double rs = r200 / C;
+  double fC = log1p(C) - C / (1.0 + C);
+  double rhos = (200.0 / 3.0) * rhoc * C * C * C / fC;
+  double ks = rhos * rs / S[kSlScr];
+  // r_vir with the Bryan-Norman Delta_vir at Omega_m(z) (subhalos.py:381-385)
+  double dd = c.Om * (1.0 + z) * (1.0 + z) * (1.0 + z) / E2 - 1.0;
+  double Dvir = 18.0 * kPi * kPi + 82.0 * dd - 39.0 * dd * dd;
+  double rvir = eta_rvir(C, Dvir) * C * rs;
+  // linear halo bias with the growth D (sgl_full.py:541-543)
+  const double dc0 = 0.6 * pow(1.5 * kPi, 2.0 / 3.0);
+  double qn2 = 0.8 * (dc0 / D / sig0) * (dc0 / D / sig0);
+  double bh = 1.0 + (qn2 - 1.0) / dc0 + 0.6 / (dc0 * (1.0 + pow(qn2, 0.3)));
+  R[kRowKs] = ks;
+  R[kRowRs] = rs;
+  R[kRowXt] = C;
+  R[kRowRvir] = rvir;
+  R[kRowW] = S[kSlWz] * dndlnM * dlnM;
+  R[kRowBeta] = D * bh;
+  R[6] = 0.0;
+  R[7] = 0.0;
+}
+
+// ------------------------------------------------------------ covariance --
+// C_ij = Cov of slice averages of the 1-D field smoothed with a top hat of
+// Rs = 20 Mpc: modes k_q = 2 pi q / L, L = 1.05 chi_s, q = 1..max(4, L/Rs),
+// P_1D(k) = (1/2pi) int_k^Kmax K P0(K) W_TH^2(K Rs) dK (trapezoid on 20000
+// log-spaced K from k_1 / 2 to 200 / Rs) -- sgl_full.py:2514-2538.
+constexpr int kCovK = 20000;
+constexpr int kCovQChunk = 32;
+
+__global__ __launch_bounds__(256) void k_cov(const Cosmo* __restrict__ cosmo,
+                                             const Derived* __restrict__ der, int n_z_max,
+                                             double* __restrict__ work,  // [B][kCovK]
+                                             int* __restrict__ status, double* __restrict__ cov) {
+  __shared__ double part[256];
+  __shared__ double tabc[kCovQChunk][kMaxSlices + 1];
+  __shared__ double tabs[kCovQChunk][kMaxSlices + 1];
+  __shared__ double rwq[kCovQChunk];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const Cosmo c = cosmo[b];
+  const Derived d = der[b];
+  const int n_z = (int)c.n_z;
+  const double Rs = 20.0;
+  const double L = 1.05 * d.chis;
+  int nq = (int)floor(L / Rs);
+  if (nq < 4) nq = 4;
+  if (nq > kCovMaxModes) {
+    if (t == 0) atomicOr(status, kStatusCovModes);
+    nq = kCovMaxModes;
+  }
+  const double k1 = 2.0 * kPi / L;
+  const double K0 = 0.5 * k1, K1 = 200.0 / Rs;
+  const double lr = log(K1 / K0) / (kCovK - 1);
+  const EH98Const e = eh98_const(c);
+  double* cum = work + (size_t)b * kCovK;
+  // f(K) into cum, then chunk-wise trapezoid prefix sums
+  for (int i = t; i < kCovK; i += blockDim.x) {
+    double K = (i == kCovK - 1) ? K1 : K0 * exp(lr * i);
+    double x = K * Rs;
+    double W = (x < 1e-2) ? 1.0 - x * x / 10.0 * (1.0 - x * x / 28.0)
+                          : 3.0 * (sin(x) - x * cos(x)) / (x * x * x);
+    double T = eh98_T(e, K);
+    cum[i] = K * d.Anorm * d.Anorm * pow(K, c.ns) * T * T * W * W;
+  }
+  __syncthreads();
+  const int per = (kCovK + 255) / 256;
+  const int i0 = t * per, i1 = min(kCovK, i0 + per);
+  // local trapezoid partials over [i0, i1] (segment i -> i+1 owned by the thread of i)
+  double loc = 0.0;
+  double fprev = (i0 < kCovK) ? cum[i0] : 0.0;
+  double Kprev = K0 * exp(lr * i0);
+  double seg[80];
+  int ns = 0;
+  for (int i = i0; i < i1 && i + 1 < kCovK; ++i) {
+    double Kn = (i + 1 == kCovK - 1) ? K1 : K0 * exp(lr * (i + 1));
+    double fn = cum[i + 1];
+    loc += 0.5 * (fprev + fn) * (Kn - Kprev);
+    seg[ns++] = loc;
+    fprev = fn;
+    Kprev = Kn;
+  }
+  part[t] = loc;
+  __syncthreads();
+  if (t == 0) {
+    double run = 0.0;
+    for (int q = 0; q < 256; ++q) { double v = part[q]; part[q] = run; run += v; }
+  }
+  __syncthreads();
+  // cum[i] = integral from K0 to K_i; thread of segment (i-1 -> i) writes cum[i]
+  // (wait: every thread has read its f values above before anyone writes)
+  __syncthreads();
+  for (int q = 0; q < ns; ++q) cum[i0 + q + 1] = part[t] + seg[q];
+  if (t == 0) cum[0] = 0.0;
+  __syncthreads();
+  const double total = cum[kCovK - 1];
+  // accumulate C_ij over chunks of modes
+  double acc[(kMaxSlices * kMaxSlices + 255) / 256];
+  const int npair = n_z * n_z;
+  const int nacc = (npair + 255) / 256;
+  for (int a = 0; a < nacc; ++a) acc[a] = 0.0;
+  const double dchi = d.dchi;
+  for (int q0 = 0; q0 < nq; q0 += kCovQChunk) {
+    const int qn = min(kCovQChunk, nq - q0);
+    if (t < qn) {
+      double kq = k1 * (q0 + t + 1);
+      double u = log(kq / K0) / lr;
+      int iu = (int)floor(u);
+      if (iu < 0) iu = 0;
+      if (iu > kCovK - 2) iu = kCovK - 2;
+      double Ka = K0 * exp(lr * iu), Kb = (iu + 1 == kCovK - 1) ? K1 : K0 * exp(lr * (iu + 1));
+      double cv = cum[iu] + (kq - Ka) / (Kb - Ka) * (cum[iu + 1] - cum[iu]);
+      double P1D = (total - cv) / (2.0 * kPi);
+      rwq[t] = sqrt(fmax(2.0 * P1D / L, 0.0)) / (kq * dchi);
+    }
+    for (int p = t; p < qn * (n_z + 1); p += blockDim.x) {
+      int ql = p / (n_z + 1), i = p % (n_z + 1);
+      double kq = k1 * (q0 + ql + 1);
+      double sn, cs;
+      sincos(kq * (i * dchi), &sn, &cs);
+      tabs[ql][i] = sn;
+      tabc[ql][i] = cs;
+    }
+    __syncthreads();
+    for (int a = 0; a < nacc; ++a) {
+      int p = t + a * 256;
+      if (p >= npair) break;
+      int i = p / n_z, j = p % n_z;
+      double s = 0.0;
+      for (int ql = 0; ql < qn; ++ql) {
+        double r2 = rwq[ql] * rwq[ql];
+        double ci = tabs[ql][i + 1] - tabs[ql][i], cj = tabs[ql][j + 1] - tabs[ql][j];
+        double si = tabc[ql][i] - tabc[ql][i + 1], sj = tabc[ql][j] - tabc[ql][j + 1];
+        s += r2 * (ci * cj + si * sj);
+      }
+      acc[a] += s;
+    }
+    __syncthreads();
+  }
+  double* C = cov + (size_t)b * n_z_max * n_z_max;
+  for (int p = t; p < n_z_max * n_z_max; p += blockDim.x) C[p] = 0.0;
+  __syncthreads();
+  for (int a = 0; a < nacc; ++a) {
+    int p = t + a * 256;
+    if (p >= npair) break;
+    int i = p / n_z, j = p % n_z;
+    C[i * n_z_max + j] = acc[a];
+  }
+}
+
+int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes_h,
+                 double* derived, double* slices, double* mass, double* rows, double* cov,
+                 double* work, int* status, hipStream_t st) {
+  const Cosmo* C = reinterpret_cast<const Cosmo*>(cosmo);
+  Derived* D = reinterpret_cast<Derived*>(derived);
+  const double dlnM = (log(sh.m_hi_h) - log(sh.m_lo_h)) / sh.n_m;
+  hipLaunchKernelGGL(k_background, dim3(B), dim3(256), 0, st, C, sh.n_z_max, D, slices);
+  hipLaunchKernelGGL(k_growth, dim3((B + 63) / 64), dim3(64), 0, st, C, B, sh.n_z_max, D, slices);
+  hipLaunchKernelGGL(k_sigma, dim3(B), dim3(256), 0, st, C, m_nodes_h, sh.n_m, D, mass);
+  const size_t nrow = (size_t)B * sh.n_z_max * sh.n_m;
+  hipLaunchKernelGGL(k_rows, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, st, C, D, slices,
+                     mass, B, sh.n_z_max, sh.n_m, dlnM, rows);
+  if (sh.flags & SGL_FLAG_CLUSTERING)
+    hipLaunchKernelGGL(k_cov, dim3(B), dim3(256), 0, st, C, D, sh.n_z_max, work, status, cov);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+}  // namespace sgl

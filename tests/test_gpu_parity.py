@@ -1,0 +1,125 @@
+"""HIP path vs the FP64 CPU oracle on identical inputs (needs a GPU)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import chain  # noqa: E402
+from oracle.cosmo import Cosmology  # noqa: E402
+
+PLANCK = dict(h=0.674, Om=0.315, s8=0.811, Ob=0.0493, ns=0.965, zeq=3402.0)
+SMALL = dict(n_z=8, n_m=32, n_b=64, n_k=1024, n_out=256)
+
+
+def _shapes(**kw):
+    from sgl_analytic_amd.solver import Shape
+    d = dict(SMALL)
+    d.update(kw)
+    osh = chain.Shape(n_z=d["n_z"], n_m=d["n_m"], n_b=d["n_b"], n_k=d["n_k"], n_out=d["n_out"])
+    dsh = Shape(n_z_max=d["n_z"], n_m=d["n_m"], n_b=d["n_b"], n_k=d["n_k"], n_out=d["n_out"])
+    return osh, dsh
+
+
+def _record(zs, n_z, cos=PLANCK):
+    from sgl_analytic_amd.solver import cosmo_records
+    p = [cos["h"], cos["Om"], cos["s8"], cos["Ob"], cos["ns"], cos["zeq"], 0.0, -1.0, 0.0, zs]
+    return cosmo_records(np.array([p]), n_z)
+
+
+def _l1(x, a, b):
+    return float(np.trapezoid(np.abs(a - b), x))
+
+
+@pytest.fixture(scope="module")
+def plain():
+    from sgl_analytic_amd.solver import Solver
+    osh, dsh = _shapes()
+    s = Solver(dsh, 1, path="exact")
+    return osh, dsh, s
+
+
+@pytest.mark.parametrize("zs", [0.5, 1.0, 3.0])
+def test_tables_match_oracle(plain, zs):
+    osh, dsh, s = plain
+    print(f"tables zs={zs} shape={dsh}", flush=True)
+    s.run(_record(zs, dsh.n_z_max))
+    torch.cuda.synchronize()
+    tv = s.table_views()
+    cos = Cosmology(**PLANCK)
+    T = chain.rows(cos, zs, osh)
+    R = T["rows"]
+    Rd = tv["rows"][0, :, :, :6]
+    # k_s, r_s, x_t, r_vir, w_row, beta
+    rel = np.abs(Rd - R) / np.abs(R)
+    worst = rel.reshape(-1, 6).max(axis=0)
+    print("max rel err per column", worst)
+    assert np.all(worst < 2e-6), worst
+    z_dev = tv["slices"][0, :, 0]
+    assert np.max(np.abs(z_dev - T["z"])) < 1e-10
+    if osh.clustering:
+        C = chain.slice_covariance(cos, T["chi_edges"], T["chis"])
+        Cd = tv["cov"][0]
+        assert np.max(np.abs(Cd - C)) < 1e-6 * np.max(np.abs(C))
+
+
+@pytest.mark.parametrize("zs", [0.5, 1.0, 3.0])
+def test_lnphi_exact_same_rows(plain, zs):
+    """Exponent and slice amplitudes on the device's own rows: FP64 to 1e-10."""
+    osh, dsh, s = plain
+    print(f"lnphi exact zs={zs}", flush=True)
+    s.run(_record(zs, dsh.n_z_max))
+    torch.cuda.synchronize()
+    R = s.table_views()["rows"][0, :, :, :6].copy()
+    xi, ws, kap, gam = chain.cells(R, osh)
+    kq = osh.k_nodes()
+    lam, amp = chain.lnphi(xi, ws, R[:, :, 5], kq[1:])
+    lam_d = s.lam[0].cpu().numpy()
+    lam_d = lam_d[:, 0] + 1j * lam_d[:, 1]
+    amp_d = s.amp[0].cpu().numpy()
+    amp_d = amp_d[..., 0] + 1j * amp_d[..., 1]
+    scale = np.maximum(np.abs(lam), 1e-3)
+    err = np.max(np.abs(lam_d - lam) / scale)
+    aerr = np.max(np.abs(amp_d - amp)) / np.max(np.abs(amp))
+    print("lam rel err", err, "amp rel err", aerr)
+    assert err < 1e-10
+    assert aerr < 1e-10
+
+
+@pytest.mark.parametrize("zs", [0.5, 1.0, 3.0])
+def test_plain_pdf_vs_oracle(plain, zs):
+    osh, dsh, s = plain
+    print(f"plain pdf zs={zs}", flush=True)
+    p, diag = s.run(_record(zs, dsh.n_z_max))
+    torch.cuda.synchronize()
+    r = chain.pdf(zs, osh)
+    pd = p[0].cpu().numpy()
+    l1 = _l1(r.x, pd, r.p)
+    m = r.info["moments"]
+    d = diag[0].cpu().numpy()
+    print("L1", l1, "mass", d[0], m["norm"], "<mu>", d[5], m["mu_mean"])
+    assert l1 < 1e-6
+    assert abs(d[0] - m["norm"]) < 1e-6
+    assert abs(d[5] - m["mu_mean"]) < 1e-6
+    assert abs(d[6] - m["p_mu_gt_2"]) < 1e-7
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+@pytest.mark.parametrize("zs", [0.5, 1.0, 3.0])
+def test_batched_pdf_vs_oracle(zs, prec):
+    from sgl_analytic_amd.solver import Solver
+    osh, dsh = _shapes()
+    print(f"batched pdf zs={zs} prec={prec}", flush=True)
+    s = Solver(dsh, 1, path="binned", precision=prec)
+    p, diag = s.run(_record(zs, dsh.n_z_max))
+    torch.cuda.synchronize()
+    r = chain.pdf(zs, osh)
+    pd = p[0].cpu().numpy().astype(np.float64)
+    l1 = _l1(r.x, pd, r.p)
+    d = diag[0].cpu().numpy()
+    m = r.info["moments"]
+    print("L1", l1, "mass", d[0], m["norm"], "<mu>", d[5], m["mu_mean"], "tail", d[6],
+          m["p_mu_gt_2"])
+    assert l1 < 1e-3
+    assert abs(d[5] - m["mu_mean"]) < 1e-3
