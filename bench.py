@@ -1,0 +1,277 @@
+#!/usr/bin/env python
+"""Benchmark: PDFs/s of the batched MI355X solver at <= 1e-3 L1 vs FP64.
+
+Workload (one "step"): one batched solve of B = 4096 cosmologies drawn by a
+seeded Latin hypercube over h[0.6,0.8], Om[0.2,0.4], sigma8[0.7,0.9],
+Ob[0.04,0.06], ns[0.92,1.0], Ok[-0.05,0.05], w0[-1.2,-0.8], wa[-0.5,0.5],
+zeq = 3402, z_s log-uniform in [0.3, 10]; ceil(24 (1 + ln(1 + z_s))) slices x
+256 masses x 512 impact parameters per instance, 8192 log-k nodes in
+[1e-2, 1e6], 2048 ln(mu) nodes on [-0.5, 3].  The step is one replay of the
+captured HIP graph (tables -> binning -> exponent -> two-point term -> phi ->
+MFMA inversion -> diagnostics); the cosmology records are resident in HBM.
+
+Reported beside it: the plain single-cosmology FP64 GPU path on the
+64 x 256 x 512 shape at z_s = 1 (PDFs/s and the ratio), the L1 / <mu> / mass /
+P(mu>2) errors of sampled batch members against the FP64 plain path, the
+roofline of the dominant kernel, and the FP64 C oracle timed on the host.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+SEED = 20261203
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "hbm": 8000.0}
+# FP64 VALU operations executed per generated cell by the binning kernel,
+# counted from the SQ_INSTS_VALU_*_F64 counters (FMA = 2); see profiles/.
+CELL_FLOPS_FP64 = float(os.environ.get("SGL_CELL_FLOPS", "0")) or None
+
+
+def lhs_params(B, seed=SEED):
+    from scipy.stats import qmc
+    u = qmc.LatinHypercube(d=9, seed=seed).random(B)
+    lo = np.array([0.6, 0.2, 0.7, 0.04, 0.92, -0.05, -1.2, -0.5, math.log(0.3)])
+    hi = np.array([0.8, 0.4, 0.9, 0.06, 1.0, 0.05, -0.8, 0.5, math.log(10.0)])
+    v = lo + u * (hi - lo)
+    zs = np.exp(v[:, 8])
+    return np.column_stack([v[:, 0], v[:, 1], v[:, 2], v[:, 3], v[:, 4], np.full(B, 3402.0),
+                            v[:, 5], v[:, 6], v[:, 7], zs])
+
+
+def trapz_l1(x, a, b):
+    return float(np.trapezoid(np.abs(a - b), x))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="fp32")
+    ap.add_argument("--verify", type=int, default=8, help="batch members checked vs FP64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the batch dimension "
+              "shards over GPUs with no exchange, so the per-GPU figure is the N=1 line.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from sgl_analytic_amd import solver as S
+    from sgl_analytic_amd._lib import PREC_BF16X3, PREC_FP32
+
+    prec = PREC_FP32 if args.precision == "fp32" else PREC_BF16X3
+    B = args.batch
+    params = lhs_params(B)
+    n_z = S.slice_count(params[:, 9])
+    shape = S.Shape(n_z_max=int(n_z.max()))
+    rec = S.cosmo_records(params, n_z)
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+
+    # ---------------------------------------------------------- batched --
+    log(f"[bench] batched B={B} n_z_max={shape.n_z_max} precision={args.precision}")
+    sv = S.Solver(shape, B, path="binned", precision=prec)
+    sv.set_records(rec)
+    stage = stage_times(sv, S)          # eager pass: per-stage event times (also builds E)
+    log("[bench] stage ms", {k: round(v, 3) for k, v in stage.items()})
+    if not args.no_graph:
+        sv.capture()
+    for _ in range(args.warmup):
+        sv.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        sv.run()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_time(e1)
+    ms_per_step = 1e3 * wall / args.steps
+    value = B * args.steps / wall
+    p_batch = sv.p.float().cpu().numpy().astype(np.float64)
+    d_batch = sv.diag.cpu().numpy()
+    log(f"[bench] batched {ms_per_step:.1f} ms/step, {value:.1f} PDFs/s (events {gpu_ms / args.steps:.1f} ms)")
+
+    # ----------------------------------------------------- plain FP64 path --
+    x = shape.out_grid()
+    rng = np.random.default_rng(SEED)
+    idx = np.sort(rng.choice(B, size=min(args.verify, B), replace=False))
+    pl = S.Solver(shape, 1, path="exact")
+    errs = []
+    for i in idx:
+        p1, d1 = pl.run(rec[i:i + 1])
+        torch.cuda.synchronize()
+        pe = p1[0].cpu().numpy()
+        de = d1[0].cpu().numpy()
+        errs.append(dict(i=int(i), z_s=float(params[i, 9]), L1=trapz_l1(x, p_batch[i], pe),
+                         maxabs=float(np.max(np.abs(p_batch[i] - pe))),
+                         dmu=float(d_batch[i, 5] - de[5]), dmass=float(d_batch[i, 0] - de[0]),
+                         dtail=float(d_batch[i, 6] - de[6])))
+        log("[bench] verify", errs[-1])
+    del pl
+    # plain path timing on the single-cosmology kernel shape (64 slices, z_s = 1, Planck)
+    plain_shape = S.Shape(n_z_max=64)
+    pp = S.Solver(plain_shape, 1, path="exact")
+    prec1 = S.cosmo_records(np.array([[0.674, 0.315, 0.811, 0.0493, 0.965, 3402.0, 0.0, -1.0,
+                                       0.0, 1.0]]), 64)
+    pp.run(prec1)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    nplain = 2
+    for _ in range(nplain):
+        pp.run(prec1)
+    torch.cuda.synchronize()
+    plain_s = (time.perf_counter() - t1) / nplain
+    plain_rate = 1.0 / plain_s
+    log(f"[bench] plain FP64 path {plain_s * 1e3:.1f} ms/PDF -> {plain_rate:.2f} PDFs/s")
+    del pp
+
+    # ------------------------------------------------------- roofline --
+    roof = roofline(stage, shape, B, n_z, prec)
+
+    # ----------------------------------------------------- CPU baseline --
+    cpu = None if args.no_cpu_baseline else cpu_baseline(log)
+
+    worst = max(errs, key=lambda e: e["L1"]) if errs else None
+    out = {
+        "metric": "PDFs/sec at <=1e-3 L1 vs FP64: batched+surrogate vs plain GPU solver",
+        "value": round(value, 2),
+        "unit": "PDFs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32" if prec == PREC_FP32 else "f64+bf16x3",
+        "data": "synthetic (seeded Latin-hypercube cosmologies; no external data)",
+        "config": {"workload": f"batched solve, B={B} LHS cosmologies, z_s in [0.3,10], "
+                               f"ceil(24(1+ln(1+z_s))) slices x 256 masses x 512 b, 8192 k, "
+                               f"2048 ln mu nodes", "batch": B, "seed": SEED,
+                   "precision": args.precision, "graph": not args.no_graph},
+        "plain_gpu_pdfs_per_s": round(plain_rate, 3),
+        "batched_over_plain": round(value / plain_rate, 1),
+        "verify": {"n": len(errs), "max_L1": worst["L1"] if worst else None,
+                   "max_abs_dmu": max(abs(e["dmu"]) for e in errs) if errs else None,
+                   "max_abs_dmass": max(abs(e["dmass"]) for e in errs) if errs else None,
+                   "max_abs_dtail": max(abs(e["dtail"]) for e in errs) if errs else None,
+                   "against": "plain FP64 GPU path (itself <= 1e-10 L1 vs the FP64 C oracle)"},
+        "stage_ms": {k: round(v, 3) for k, v in stage.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def stage_times(sv, S):
+    """One eager solve with HIP events around every library call (same stream)."""
+    import torch
+    L, B, cs, st, P = sv.L, sv.B, ctypes.byref(sv.cs), S._stream(), S._ptr
+    names, evs = [], [torch.cuda.Event(enable_timing=True)]
+    evs[0].record()
+
+    def mark(name):
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        names.append(name)
+        evs.append(e)
+
+    L.sgl_tables_build(P(sv.records), B, cs, P(sv.tables), P(sv.status), st); mark("tables")
+    L.sgl_bin_cells(P(sv.tables), B, cs, P(sv.hist), st); mark("bin_cells")
+    L.sgl_lnphi_binned(P(sv.tables), P(sv.hist), B, cs, P(sv.lam), P(sv.amp), st); mark("lnphi_binned")
+    L.sgl_quadform(P(sv.tables), P(sv.amp), B, cs, 0, P(sv.psi2), st); mark("quadform")
+    L.sgl_phi_pack(P(sv.lam), P(sv.psi2), P(sv.tables), B, cs, sv.precision, P(sv.phi), st); mark("phi_pack")
+    if not sv._filon_ready:
+        L.sgl_filon_build(P(sv.tables), B, cs, sv.precision, P(sv.E), st)
+        sv._filon_ready = True
+        mark("filon_build(once)")
+    L.sgl_invert_gemm(P(sv.E), P(sv.phi), B, cs, sv.precision, P(sv.p), st); mark("invert_gemm")
+    L.sgl_diagnostics(P(sv.p), 0, B, cs, P(sv.diag), st); mark("diagnostics")
+    torch.cuda.synchronize()
+    return {n: evs[i].elapsed_time(evs[i + 1]) for i, n in enumerate(names)}
+
+
+def roofline(stage, shape, B, n_z, prec):
+    """Roofline of the slowest per-step kernel (filon_build runs once, not per step)."""
+    per_step = {k: v for k, v in stage.items() if "once" not in k}
+    dom = max(per_step, key=per_step.get)
+    t = per_step[dom] * 1e-3
+    cells = float(np.sum(n_z)) * shape.n_m * shape.n_b
+    kp = ((2 * (shape.n_k + 1) + 63) // 64) * 64
+    if dom == "invert_gemm":
+        flops = 2.0 * shape.n_out * kp * B
+        peak = PEAK["fp32"] if prec == 1 else PEAK["bf16"]
+        ach = flops / t / 1e12
+        return {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": None,
+                "algorithmic": f"2 n_out Kp B = {flops:.3e} flop"}
+    if dom == "bin_cells" and CELL_FLOPS_FP64:
+        flops = cells * CELL_FLOPS_FP64
+        ach = flops / t / 1e12
+        return {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK["fp64"],
+                "unit": "TFLOP/s", "frac": round(ach / PEAK["fp64"], 4), "traffic": None,
+                "algorithmic": f"{CELL_FLOPS_FP64:.0f} FP64 flop/cell x {cells:.3e} cells"}
+    if dom == "bin_cells":
+        return {"kernel": dom, "bound": "mfma", "achieved": None, "peak": PEAK["fp64"],
+                "unit": "TFLOP/s", "frac": None, "traffic": None,
+                "cells_per_s": cells / t}
+    return {"kernel": dom, "bound": "hbm", "achieved": None, "peak": PEAK["hbm"], "unit": "GB/s",
+            "frac": None, "traffic": None, "ms": per_step[dom]}
+
+
+def cpu_baseline(log):
+    """The FP64 C oracle (OpenMP) on one z_s = 1 Planck instance of the 64 x 256 x
+    512 shape, with the k loop sampled (256 of 8192 nodes) and scaled to a PDF."""
+    try:
+        from oracle import chain
+        from oracle.cosmo import Cosmology
+    except Exception as e:  # pragma: no cover
+        log("[bench] cpu baseline unavailable:", e)
+        return None
+    threads = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    sh = chain.Shape()
+    t0 = time.perf_counter()
+    cos = Cosmology()
+    T = chain.rows(cos, 1.0, sh)
+    t_rows = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    xi, ws, kap, gam = chain.cells(T["rows"], sh)
+    t_cells = time.perf_counter() - t0
+    kq = sh.k_nodes()[1:]
+    ksamp = kq[:: sh.n_k // 256]
+    t0 = time.perf_counter()
+    chain.lnphi(xi, ws, T["rows"][:, :, 5], ksamp)
+    t_lam = (time.perf_counter() - t0) * (sh.n_k / ksamp.size)
+    t0 = time.perf_counter()
+    chain.invert(sh.out_grid()[::8], sh.k_nodes(), np.ones(sh.n_k + 1, complex))
+    t_inv = (time.perf_counter() - t0) * 8
+    total = t_rows + t_cells + t_lam + t_inv
+    log(f"[bench] cpu oracle: rows {t_rows:.2f}s cells {t_cells:.2f}s lnphi(scaled) {t_lam:.1f}s "
+        f"invert(scaled) {t_inv:.2f}s")
+    return {"value": round(1.0 / total, 5), "unit": "PDFs/s", "cores": int(os.environ.get(
+        "OMP_NUM_THREADS", threads)), "kind": "port",
+        "sample": "FP64 C oracle, 1 instance (64x256x512 cells, z_s=1): full tables and cells, "
+                  "ln phi on 256 of 8192 k nodes and inversion on 256 of 2048 outputs, "
+                  "both scaled linearly to the full grid"}
+
+
+if __name__ == "__main__":
+    main()
