@@ -146,6 +146,13 @@ sgl_status sgl_invert_gemm(const void* E_op, const void* phi_op, int32_t B, cons
 sgl_status sgl_invert_fp64(const double* phi, const void* tables, int32_t B,
                            const sgl_shape* shape, double* p, void* stream);
 
+/* As sgl_invert_fp64 at n_t caller-given ln(mu) points t (device array):
+ * p[b][j] for j < n_t.  The drop-in generate_pdf_lnmu evaluates the user's
+ * grid with it (api.py:26-84 accepts any strictly increasing grid). */
+sgl_status sgl_invert_fp64_at(const double* phi, const void* tables, int32_t B,
+                              const sgl_shape* shape, const double* t, int32_t n_t, double* p,
+                              void* stream);
+
 /*
  * FP64 diagnostics of p on the output grid, 8 doubles per instance:
  * grid mass, mean, variance, skewness, sigma_DL/D_L, <mu>, P(mu > 2),

@@ -39,11 +39,12 @@ class Cosmology:
     """sgl.Cosmology restricted to the make_cosmology configuration."""
 
     def __init__(self, h=0.674, Om=0.315, s8=0.811, Ob=0.0493, ns=0.965,
-                 zeq=3402.0, Ok=0.0, w0=-1.0, wa=0.0):
-        # sgl.py:68-167 (defaults, radiation closure, DE domain checks)
+                 zeq=3402.0, Ok=0.0, w0=-1.0, wa=0.0, radiation="engine"):
+        # sgl.py:68-167 (defaults, radiation closure, DE domain checks);
+        # radiation=None only for closed-form checks (the pipeline uses "engine")
         self.h, self.Om, self.s8, self.Ob, self.ns = h, Om, s8, Ob, ns
         self.zeq, self.Ok, self.w0, self.wa = float(zeq), float(Ok), float(w0), float(wa)
-        self.Or = self.Om / (1.0 + self.zeq)          # sgl.py:100 radiation="engine"
+        self.Or = self.Om / (1.0 + self.zeq) if radiation == "engine" else 0.0   # sgl.py:100
         self.OL = 1.0 - self.Om - self.Ok - self.Or    # sgl.py:101
         if not (-2.0 <= self.w0 <= 1.0):
             raise ValueError("dark energy requires -2 <= w0 <= 1")

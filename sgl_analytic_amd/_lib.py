@@ -20,7 +20,7 @@ EXPORTS = (
     "sgl_version", "sgl_tables_bytes", "sgl_hist_bytes", "sgl_filon_bytes", "sgl_phi_bytes",
     "sgl_tables_build", "sgl_lnphi_exact", "sgl_bin_cells", "sgl_lnphi_binned",
     "sgl_quadform", "sgl_phi_pack", "sgl_filon_build", "sgl_invert_gemm", "sgl_invert_fp64",
-    "sgl_diagnostics",
+    "sgl_diagnostics", "sgl_invert_fp64_at",
 )
 
 
@@ -70,6 +70,7 @@ def load(path: str = LIB_PATH):
         "sgl_invert_gemm": [vp, vp, i32, sp, i32, vp, vp],
         "sgl_invert_fp64": [vp, vp, i32, sp, vp, vp],
         "sgl_diagnostics": [vp, i32, i32, sp, vp, vp],
+        "sgl_invert_fp64_at": [vp, vp, i32, sp, vp, i32, vp, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -1,0 +1,267 @@
+"""Drop-in API: generate_pdf_lnmu / generate_pdf with the reference's
+signatures, defaults, argument checks and error behaviour
+(sgl_analytic/api.py:11-113, pipeline.py:36-94, 139-164), computed by the
+MI355X kernels.
+
+Physics of this path (the kernel-shape chain): spherical NFW halos truncated
+at r200c with Sheth-Tormen abundances and Ludlow+16 concentrations on
+``n_z`` slices uniform in comoving distance x 256 masses x 512 impact
+parameters; each cell is a jump xi = ln mu = -ln detA(kappa, gamma) of the
+1-D compound-Poisson chain of sgl.Lambda_of_k / P_of_xi (sgl.py:961-1012),
+tilted to the source plane; linear-bias clustering enters as the two-point
+term 1/2 sum_ij C_ij a_i a_j (sgl_full.py:2514-2538, 3353-3397).  The
+reference's ellipticity, filament and subhalo ingredients and its "vp"
+closure are not part of this path; asking for them raises
+NotImplementedError.
+
+Batched entry point (no reference counterpart): generate_pdf_lnmu_batch.
+"""
+from __future__ import annotations
+
+import math
+import warnings
+
+import numpy as np
+
+P = dict(h=0.674, Om=0.315, s8=0.811, Ob=0.0493, ns=0.965, zeq=3402.0)  # pipeline.py:20
+EXTENSIONS = dict(Ok=0.0, w0=-1.0, wa=0.0, growth_mode="ode", radiation="engine")
+INGREDIENTS = ("ell", "fil", "sub", "bias")
+MODELLED = {"bias"}            # ingredients this path computes on top of NFW halos
+DEFAULT_SLICES = 64
+
+
+def _grid(values, lo, hi, count, name):
+    """api.py:11-23: a finite, 1-D, strictly increasing grid with >= 2 points."""
+    if values is None:
+        if isinstance(count, bool) or not isinstance(count, (int, np.integer)) or count < 2:
+            raise ValueError(f"n_{name} must be an integer >= 2")
+        if not np.isfinite(lo) or not np.isfinite(hi) or lo >= hi:
+            raise ValueError(f"{name}_min and {name}_max must be finite and increasing")
+        values = np.linspace(lo, hi, count)
+    values = np.asarray(values, dtype=float)
+    if values.ndim != 1 or values.size < 2 or not np.all(np.isfinite(values)):
+        raise ValueError(f"{name} must be a finite one-dimensional grid with >= 2 points")
+    if np.any(np.diff(values) <= 0):
+        raise ValueError(f"{name} must be strictly increasing")
+    return values
+
+
+def on(cfg):
+    """pipeline.py:78-94: ingredient set of a config string or iterable."""
+    ing = set(INGREDIENTS)
+    if not isinstance(cfg, str):
+        s = set(cfg)
+        if s - ing:
+            raise ValueError(f"unknown ingredient(s) {sorted(s - ing)}; choose from {INGREDIENTS}")
+        return s
+    if cfg == "halo":
+        return set()
+    if cfg == "full":
+        return ing
+    if len(cfg) < 2 or cfg[0] not in "+-" or cfg[1:] not in ing:
+        raise ValueError(f"unknown configuration {cfg!r}; use full, halo, or +/-{INGREDIENTS}")
+    return {cfg[1:]} if cfg[0] == "+" else ing - {cfg[1:]}
+
+
+def check_cosmology(cosmo=None):
+    """pipeline.make_cosmology's checks (pipeline.py:41-72) and the domain of
+    sgl.Cosmology (sgl.py:121-151); returns the completed parameter dict."""
+    supplied = dict(cosmo or {})
+    if "sigma8" in supplied:
+        if "s8" in supplied:
+            raise ValueError("supply sigma8 or s8, not both")
+        supplied["s8"] = supplied.pop("sigma8")
+    cp = {**P, **EXTENSIONS, **supplied}
+    bad = set(cp) - set(P) - set(EXTENSIONS)
+    if bad:
+        raise ValueError(f"unknown cosmology parameter(s) {sorted(bad)}")
+    for key in P:
+        value = cp[key]
+        if not np.isscalar(value) or not np.isfinite(value) or value <= 0:
+            raise ValueError(f"{key} must be a finite positive number")
+    for key in ("Ok", "w0", "wa"):
+        value = cp[key]
+        if isinstance(value, bool) or not np.isscalar(value) or not np.isfinite(value):
+            raise ValueError(f"{key} must be a finite number")
+    if cp["growth_mode"] not in ("auto", "ode", "legacy"):
+        raise ValueError("growth_mode must be 'auto', 'ode' or 'legacy'")
+    if cp["radiation"] in ("none", False):
+        cp["radiation"] = None
+    if cp["radiation"] not in ("engine", None):
+        raise ValueError("radiation must be 'engine' or None")
+    if not 0 < cp["Ob"] < cp["Om"] <= 1:
+        raise ValueError("require 0 < Ob < Om <= 1")
+    w0, wa = float(cp["w0"]), float(cp["wa"])
+    if not -2.0 <= w0 <= 1.0:
+        raise ValueError("dark energy requires -2 <= w0, growth_w0 <= 1")
+    if w0 + wa >= 0.0:
+        raise ValueError("dark energy requires w0 + wa < 0 (matter-dominated early universe)")
+    Om, Ok = float(cp["Om"]), float(cp["Ok"])
+    Or = Om / (1.0 + float(cp["zeq"])) if cp["radiation"] == "engine" else 0.0
+    OL = 1.0 - Om - Ok - Or
+    z = 1100.0
+    if OL / Om * (1 + z) ** (3 * (w0 + wa)) * math.exp(-3 * wa * z / (1 + z)) > 1e-2:
+        raise ValueError("dark energy exceeds de_rec_cap of the matter density at z = 1100")
+    zp1 = 1.0 / np.geomspace(1e-6, 1.0, 4096)
+    de = OL * zp1 ** (3 * (1 + w0 + wa)) * np.exp(-3 * wa * (1 - 1 / zp1))
+    if not np.all(Om * zp1 ** 3 + Or * zp1 ** 4 + Ok * zp1 ** 2 + de > 0):
+        raise ValueError("E(z)^2 <= 0 somewhere in 0 <= z <= 1e6: outside the supported domain")
+    # what this path computes: the ODE growth with the engine radiation term
+    lambda_de = w0 == -1.0 and wa == 0.0
+    if cp["growth_mode"] == "legacy" and not lambda_de:
+        raise ValueError("legacy growth is only valid for LambdaCDM")
+    if cp["growth_mode"] != "ode" and lambda_de:
+        raise NotImplementedError("only growth_mode='ode' (the default) is computed on this path")
+    if cp["radiation"] != "engine":
+        raise NotImplementedError("only radiation='engine' (the default) is computed on this path")
+    return cp
+
+
+_SOLVERS = {}
+
+
+def _solver(n_z):
+    from .solver import Shape, Solver
+    key = ("exact", n_z)
+    if key not in _SOLVERS:
+        _SOLVERS[key] = Solver(Shape(n_z_max=n_z), 1, path="exact")
+    return _SOLVERS[key]
+
+
+def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAULT_SLICES):
+    """Host driver (the role of pipeline.run, pipeline.py:114-274) for one
+    cosmology on the plain FP64 device path; returns (p on xi_out, info)."""
+    import ctypes
+
+    import torch
+
+    from . import _lib
+    from .solver import _ptr, _stream, cosmo_records
+
+    if not np.isscalar(z_s) or not np.isfinite(z_s) or z_s <= 0:
+        raise ValueError("z_s must be a finite positive number")
+    if closure not in ("gauss", "vp"):
+        raise ValueError(f"xi_closure must be 'gauss' or 'vp', got {closure!r}")
+    s = on(config)
+    cp = check_cosmology(cosmo)
+    if closure != "gauss":
+        if "bias" not in s:
+            raise ValueError(f"xi_closure={closure!r} needs xilin clustering and the bias ingredient")
+        raise NotImplementedError("the 'vp' positive closure is not part of this path")
+    explicit = s - MODELLED
+    if explicit and config != "full":
+        raise NotImplementedError(f"ingredient(s) {sorted(explicit)} are not part of this path")
+    sv = _solver(int(n_z))
+    sv_shape = sv.shape
+    if ("bias" in s) != sv_shape.clustering:
+        from dataclasses import replace
+        from .solver import Solver
+        key = ("exact", int(n_z), "bias" in s)
+        if key not in _SOLVERS:
+            _SOLVERS[key] = Solver(replace(sv_shape, clustering="bias" in s), 1, path="exact")
+        sv = _SOLVERS[key]
+    rec = cosmo_records(np.array([[cp["h"], cp["Om"], cp["s8"], cp["Ob"], cp["ns"], cp["zeq"],
+                                   cp["Ok"], cp["w0"], cp["wa"], float(z_s)]]), int(n_z))
+    sv.run(rec)
+    x = np.asarray(xi_out, dtype=np.float64)
+    t = torch.as_tensor(x, device=sv.dev)
+    p = torch.zeros(x.size, dtype=torch.float64, device=sv.dev)
+    _lib.check(sv.L.sgl_invert_fp64_at(_ptr(sv.phi), _ptr(sv.tables), 1, ctypes.byref(sv.cs),
+                                       _ptr(t), int(x.size), _ptr(p), _stream()),
+               "sgl_invert_fp64_at")
+    torch.cuda.synchronize()
+    d = sv.diag[0].cpu().numpy()
+    info = dict(zs=float(z_s), cfg=config, ingredients=["halo"] + sorted(s & MODELLED),
+                clustering="slice-covariance" if "bias" in s else None, closure=closure,
+                n_slices=int(n_z), grid=dict(n_k=sv_shape.n_k, k_min=sv_shape.k_lo,
+                                             k_max=sv_shape.k_hi, n_m=sv_shape.n_m,
+                                             n_b=sv_shape.n_b),
+                kernel_grid=dict(xi_min=sv_shape.out_lo, xi_max=sv_shape.out_hi,
+                                 n=sv_shape.n_out, mass=float(d[0]), mean=float(d[1]),
+                                 var=float(d[2]), skew=float(d[3]), sigmaDL=float(d[4]),
+                                 mu_mean=float(d[5]), p_mu_gt_2=float(d[6])),
+                h=cp["h"], Om=cp["Om"], s8=cp["s8"], sigma8=cp["s8"], Ob=cp["Ob"], ns=cp["ns"],
+                zeq=cp["zeq"], Ok=cp["Ok"], w0=cp["w0"], wa=cp["wa"],
+                growth_mode=cp["growth_mode"], radiation=cp["radiation"],
+                backend="MI355X HIP (libsgl_amd.so), FP64 plain path")
+    return p.cpu().numpy(), info
+
+
+def generate_pdf_lnmu(z_s, h=0.674, Om=0.315, sigma8=0.811, Ob=0.0493,
+                      ns=0.965, zeq=3402.0, lnmu=None, *, Ok=0.0, w0=-1.0,
+                      wa=0.0, growth_mode="ode", radiation="engine",
+                      closure="gauss", lnmu_min=-1.0, lnmu_max=9.0,
+                      n_lnmu=3201, config="full", nproc=1,
+                      return_info=False):
+    """Return (lnmu, dP_S/dlnmu) as NumPy arrays (api.py:26-84).
+
+    Same arguments and checks as the reference; the PDF comes from the
+    MI355X FP64 plain path.  The density is not renormalised to the user
+    grid; negative residuals stay visible with a RuntimeWarning."""
+    x = _grid(lnmu, lnmu_min, lnmu_max, n_lnmu, "lnmu")
+    if isinstance(nproc, bool) or not isinstance(nproc, (int, np.integer)) or nproc < 1:
+        raise ValueError("nproc must be a positive integer")
+    p, out = run(z_s, config, cosmo=dict(h=h, Om=Om, sigma8=sigma8, Ob=Ob, ns=ns, zeq=zeq,
+                                         Ok=Ok, w0=w0, wa=wa, growth_mode=growth_mode,
+                                         radiation=radiation),
+                 xi_out=x, closure=closure)
+    p = np.asarray(p, dtype=np.float64)
+    if not np.all(np.isfinite(p)):
+        raise RuntimeError("PDF inversion returned non-finite values")
+    negative_mass = float(np.trapezoid(np.maximum(-p, 0), x))
+    out["diagnostics"] = dict(grid_mass=float(np.trapezoid(p, x)), negative_mass=negative_mass,
+                              minimum_density=float(p.min()))
+    out["moment_interval"] = [out["kernel_grid"]["xi_min"], out["kernel_grid"]["xi_max"]]
+    if negative_mass > 0:
+        warnings.warn(f"PDF inversion has negative residuals (integrated mass "
+                      f"{negative_mass:.3g}); inspect diagnostics before using tail probabilities.",
+                      RuntimeWarning, stacklevel=2)
+    return (x, p, out) if return_info else (x, p)
+
+
+def generate_pdf(z_s, h=0.674, Om=0.315, sigma8=0.811, Ob=0.0493,
+                 ns=0.965, zeq=3402.0, mu=None, *, Ok=0.0, w0=-1.0, wa=0.0,
+                 growth_mode="ode", radiation="engine", closure="gauss",
+                 mu_min=0.3, mu_max=30.0, n_mu=2000, config="full", nproc=1,
+                 return_info=False):
+    """Return (mu, p_S(mu)) with p(mu) = (dP/dlnmu)/mu (api.py:87-113)."""
+    if mu is None:
+        if mu_min <= 0 or mu_max <= 0:
+            raise ValueError("mu bounds must be positive")
+        x = _grid(None, np.log(mu_min), np.log(mu_max), n_mu, "mu")
+        mu = np.exp(x)
+    else:
+        mu = _grid(mu, 0, 1, n_mu, "mu")
+        if np.any(mu <= 0):
+            raise ValueError("mu must be positive")
+        x = np.log(mu)
+    result = generate_pdf_lnmu(z_s, h, Om, sigma8, Ob, ns, zeq, x, Ok=Ok, w0=w0, wa=wa,
+                               growth_mode=growth_mode, radiation=radiation, closure=closure,
+                               config=config, nproc=nproc, return_info=return_info)
+    return (mu, result[1] / mu, result[2]) if return_info else (mu, result[1] / mu)
+
+
+def generate_pdf_lnmu_batch(params, precision="fp32", graph=True, shape=None):
+    """Batched solve (no reference counterpart): params [B, 10] in the order
+    (h, Om, sigma8, Ob, ns, zeq, Ok, w0, wa, z_s); slice count per instance
+    ceil(24 (1 + ln(1 + z_s))).  Returns (lnmu grid, p [B, n_out] float32
+    numpy, diagnostics [B, 8])."""
+    from ._lib import PREC_BF16X3, PREC_FP32
+    from .solver import Shape, Solver, cosmo_records, slice_count
+    params = np.atleast_2d(np.asarray(params, dtype=np.float64))
+    for row in params:
+        check_cosmology(dict(h=row[0], Om=row[1], s8=row[2], Ob=row[3], ns=row[4], zeq=row[5],
+                             Ok=row[6], w0=row[7], wa=row[8]))
+        if not np.isfinite(row[9]) or row[9] <= 0:
+            raise ValueError("z_s must be a finite positive number")
+    n_z = slice_count(params[:, 9])
+    sh = shape or Shape(n_z_max=int(n_z.max()))
+    prec = PREC_FP32 if precision == "fp32" else PREC_BF16X3
+    sv = Solver(sh, params.shape[0], path="binned", precision=prec)
+    sv.set_records(cosmo_records(params, n_z))
+    if graph:
+        sv.capture()
+    p, d = sv.run()
+    import torch
+    torch.cuda.synchronize()
+    return sh.out_grid(), p.cpu().numpy(), d.cpu().numpy()

@@ -160,7 +160,19 @@ sgl_status sgl_invert_fp64(const double* phi, const void* tables, int32_t B,
   if (!shape_ok(shape) || !batch_ok(B) || !phi || !tables || !p) return SGL_ERR_INVALID;
   TablesLayout L = tables_layout(*shape, B);
   const double* T = static_cast<const double*>(tables);
-  return (sgl_status)launch_invert_fp64(phi, T + L.knodes, *shape, B, p, as_stream(stream));
+  return (sgl_status)launch_invert_fp64(phi, T + L.knodes, *shape, B, nullptr, 0, p,
+                                        as_stream(stream));
+}
+
+sgl_status sgl_invert_fp64_at(const double* phi, const void* tables, int32_t B,
+                              const sgl_shape* shape, const double* t, int32_t n_t, double* p,
+                              void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !phi || !tables || !t || !p) return SGL_ERR_INVALID;
+  if (n_t < 1 || n_t > (1 << 24)) return SGL_ERR_INVALID;
+  TablesLayout L = tables_layout(*shape, B);
+  const double* T = static_cast<const double*>(tables);
+  return (sgl_status)launch_invert_fp64(phi, T + L.knodes, *shape, B, t, n_t, p,
+                                        as_stream(stream));
 }
 
 sgl_status sgl_diagnostics(const void* p, int32_t p_is_double, int32_t B, const sgl_shape* shape,

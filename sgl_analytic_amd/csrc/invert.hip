@@ -198,14 +198,17 @@ __global__ void k_filon_build(const double* __restrict__ knodes, sgl_shape sh, i
   }
 }
 
-// Plain path: thread per (b, j), FP64 Filon weights on the fly.
+// Plain path: thread per (b, j), FP64 Filon weights on the fly; output
+// points from `t` when non-null (n_t of them), else the shape's uniform grid.
 __global__ __launch_bounds__(256) void k_invert_fp64(const double* __restrict__ phi,
                                                      const double* __restrict__ knodes,
-                                                     sgl_shape sh, int B, double* __restrict__ p) {
+                                                     sgl_shape sh, int B,
+                                                     const double* __restrict__ t, int n_t,
+                                                     double* __restrict__ p) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
-  if (j >= sh.n_out) return;
-  const double tj = sh.out_lo + (sh.out_hi - sh.out_lo) * j / (sh.n_out - 1);
+  if (j >= n_t) return;
+  const double tj = t ? t[j] : sh.out_lo + (sh.out_hi - sh.out_lo) * j / (sh.n_out - 1);
   const int nq = sh.n_k + 1;
   const double* ph = phi + 2 * (size_t)b * nq;
   double acc = 0.0;
@@ -215,7 +218,7 @@ __global__ __launch_bounds__(256) void k_invert_fp64(const double* __restrict__ 
     acc += ph[2 * n] * dr - ph[2 * n + 1] * di;
     acc += ph[2 * (n + 1)] * ur - ph[2 * (n + 1) + 1] * ui;
   }
-  p[(size_t)b * sh.n_out + j] = acc / kPi;
+  p[(size_t)b * n_t + j] = acc / kPi;
 }
 
 // ----------------------------------------------------------- MFMA GEMM --
@@ -431,9 +434,10 @@ int launch_filon(const double* knodes, const sgl_shape& sh, int prec, void* out,
 }
 
 int launch_invert_fp64(const double* phi, const double* knodes, const sgl_shape& sh, int B,
-                       double* p, hipStream_t st) {
-  dim3 grid((sh.n_out + 255) / 256, B);
-  hipLaunchKernelGGL(k_invert_fp64, grid, dim3(256), 0, st, phi, knodes, sh, B, p);
+                       const double* t, int n_t, double* p, hipStream_t st) {
+  if (t == nullptr) n_t = sh.n_out;
+  dim3 grid((n_t + 255) / 256, B);
+  hipLaunchKernelGGL(k_invert_fp64, grid, dim3(256), 0, st, phi, knodes, sh, B, t, n_t, p);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 

@@ -88,7 +88,7 @@ int launch_phi_pack(const double* lam, const double* psi2, const double* knodes,
                     const sgl_shape& sh, int B, int prec, void* out, hipStream_t st);
 int launch_filon(const double* knodes, const sgl_shape& sh, int prec, void* out, hipStream_t st);
 int launch_invert_fp64(const double* phi, const double* knodes, const sgl_shape& sh, int B,
-                       double* p, hipStream_t st);
+                       const double* t, int n_t, double* p, hipStream_t st);
 int launch_gemm(const void* E, const void* Phi, const sgl_shape& sh, int B, int prec, float* P,
                 hipStream_t st);
 int launch_diag(const void* p, int is_double, const sgl_shape& sh, int B, double* diag,

@@ -1,0 +1,49 @@
+"""Write tests/golden/reference_scalars.json from the reference's own data.
+
+Reads (as JSON/text, nothing executed) the stored PDFs
+tests/reference/pdf_zs*.json and the walkthrough notebook's printed outputs
+of /root/reference, and keeps the scalar values this repo's oracle is pinned
+on.  Run in the build container: python tests/golden/make_golden.py
+"""
+import json
+import os
+import re
+
+REF = "/root/reference"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_scalars.json")
+
+
+def main():
+    fx = {}
+    for name in ("pdf_zs0.5", "pdf_zs1", "pdf_zs3", "pdf_zs10"):
+        with open(os.path.join(REF, "tests", "reference", name + ".json")) as f:
+            d = json.load(f)
+        fx[name] = dict(Omega_r=d["dark_energy"]["Omega_r"],
+                        primordial_growth=d["dark_energy"]["primordial_growth"],
+                        sigma_kappa=d["sigma_kappa"], kbar=d["kbar"],
+                        cosmo=dict(h=d["h"], Om=d["Om"], s8=d["s8"], Ob=d["Ob"], ns=d["ns"],
+                                   zeq=d["zeq"]))
+    with open(os.path.join(REF, "notebooks", "sgl_analytic_walkthrough.ipynb")) as f:
+        nb = json.load(f)
+    text = ""
+    for c in nb["cells"]:
+        for o in c.get("outputs", []):
+            if "text" in o:
+                text += "".join(o["text"])
+    m = re.search(r"sigma_8 \(top hat, the anchor\) = ([0-9.]+)", text)
+    m2 = re.search(r"D\(z=1\) = ([0-9.]+),\s+D\(z=3\) = ([0-9.]+)", text)
+    m3 = re.search(r"c = ([0-9.]+), r_s = ([0-9.]+) kpc, kappa_s = ([0-9.]+) \(z_s = 1.0\), "
+                   r"x_t = r200/r_s = [0-9.]+ \(r_vir/r_s = ([0-9.]+)\)", text)
+    nbv = dict(sigma8_tophat=float(m.group(1)), D1=float(m2.group(1)), D3=float(m2.group(2)),
+               nfw_1e14_z04=dict(c=float(m3.group(1)), rs_kpc=float(m3.group(2)),
+                                 kappa_s=float(m3.group(3)), rvir_over_rs=float(m3.group(4))))
+    out = dict(provenance="sgl_analytic v0.2.0 (MIT): tests/reference/pdf_zs*.json and the "
+                          "printed outputs of notebooks/sgl_analytic_walkthrough.ipynb cells 3, 7",
+               fixtures=fx, walkthrough=nbv)
+    with open(OUT, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    print("wrote", OUT)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,51 @@
+"""The C-ABI library loads on a host without a GPU and exports every entry
+point include/sgl_amd.h declares; host-only size queries work."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "sgl_amd.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\w[\w\s\*]*?\b(sgl_\w+)\(", text, flags=re.M)))
+
+
+def test_header_lists_entry_points():
+    names = declared()
+    assert "sgl_tables_build" in names and "sgl_invert_gemm" in names
+    assert len(names) >= 15
+
+
+def test_library_exports_every_declared_symbol():
+    from sgl_analytic_amd import _lib
+    L = _lib.load()
+    for name in declared():
+        assert hasattr(L, name), name
+    assert set(_lib.EXPORTS) <= set(declared())
+
+
+def test_size_queries_and_rejections():
+    from sgl_analytic_amd import _lib
+    from sgl_analytic_amd.solver import Shape
+    L = _lib.load()
+    sh = Shape().c()
+    n = L.sgl_tables_bytes(ctypes.byref(sh), 4096)
+    assert n > 4096 * 64 * 256 * 8 * 8
+    assert L.sgl_hist_bytes(ctypes.byref(sh), 2) > 0
+    assert L.sgl_filon_bytes(ctypes.byref(sh), _lib.PREC_FP32) == 2048 * 16448 * 4
+    assert L.sgl_filon_bytes(ctypes.byref(sh), _lib.PREC_BF16X3) == 2 * 2048 * 16448 * 2
+    assert L.sgl_phi_bytes(ctypes.byref(sh), 3, _lib.PREC_FP64) == 3 * 8193 * 16
+    bad = Shape(n_z_max=200).c()                 # above the 96-slice limit
+    assert L.sgl_tables_bytes(ctypes.byref(bad), 1) == 0
+    assert L.sgl_tables_bytes(ctypes.byref(sh), 0) == 0
+    # invalid arguments are rejected on the host before any launch
+    rc = L.sgl_tables_build(None, 1, ctypes.byref(sh), None, None, None)
+    assert rc == _lib.SGL_ERR_INVALID
+    rc = L.sgl_invert_gemm(None, None, 1, ctypes.byref(sh), 7, None, None)
+    assert rc == _lib.SGL_ERR_INVALID
+    assert L.sgl_version().startswith(b"sgl_amd")

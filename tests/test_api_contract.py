@@ -1,0 +1,109 @@
+"""Public input contracts of the drop-in API without device solves (mirrors
+the reference's tests/test_api.py with the solver mocked out)."""
+import warnings
+from unittest.mock import patch
+
+import numpy as np
+import pytest
+
+from sgl_analytic_amd import api, generate_pdf, generate_pdf_lnmu
+
+
+def fake_info(x):
+    return dict(kernel_grid=dict(xi_min=-0.5, xi_max=3.0))
+
+
+def test_invalid_configuration():
+    for cfg in ("", "garbage", "+typo", "-typo"):
+        with pytest.raises(ValueError):
+            api.on(cfg)
+    assert api.on("halo") == set()
+    assert api.on("+bias") == {"bias"}
+    assert api.on("-sub") == {"ell", "fil", "bias"}
+
+
+def test_invalid_grid_fails_before_solver():
+    with patch("sgl_analytic_amd.api.run") as solver:
+        for grid in ([0, 0], [1, 0], [0, np.nan], [[0, 1]], [0]):
+            with pytest.raises(ValueError):
+                generate_pdf_lnmu(1, lnmu=grid)
+        with pytest.raises(ValueError):
+            generate_pdf(1, mu=[0, 1])
+        with pytest.raises(ValueError):
+            generate_pdf_lnmu(1, n_lnmu=1)
+        solver.assert_not_called()
+
+
+def test_cosmology_validation():
+    for cp in (dict(h=0), dict(Om=np.nan), dict(Ob=0.4), dict(zeq=-1), dict(sigma8=.8, s8=.8),
+               dict(Ok=np.nan), dict(Ok=True), dict(w0=0.0), dict(wa=np.inf), dict(wa=1.2),
+               dict(w0=-.5, wa=.45), dict(growth_mode="exact"), dict(Ok=-1.5),
+               dict(w0=-.9, growth_mode="legacy"), dict(radiation="yes"), dict(foo=1)):
+        with pytest.raises(ValueError):
+            api.check_cosmology(cp)
+    cp = api.check_cosmology(dict(Ok=-.05, w0=-1.1, wa=.2))
+    assert (cp["Ok"], cp["w0"], cp["wa"]) == (-.05, -1.1, .2)
+
+
+def test_paths_not_computed_here_raise():
+    with patch("sgl_analytic_amd.api._solver") as solver:
+        with pytest.raises(NotImplementedError):
+            api.run(1.0, "full", cosmo={}, xi_out=np.array([0.0, 1.0]), closure="vp")
+        with pytest.raises(NotImplementedError):
+            api.run(1.0, "+ell", cosmo={}, xi_out=np.array([0.0, 1.0]))
+        with pytest.raises(NotImplementedError):
+            api.run(1.0, "full", cosmo=dict(radiation=None), xi_out=np.array([0.0, 1.0]))
+        with pytest.raises(ValueError):     # vp needs the bias ingredient, as in the reference
+            api.run(1.0, "-bias", cosmo={}, xi_out=np.array([0.0, 1.0]), closure="vp")
+        solver.assert_not_called()
+
+
+def test_invalid_redshift_and_workers_fail_before_build():
+    with patch("sgl_analytic_amd.api._solver") as solver:
+        for zs in (0, -1, np.nan, np.inf):
+            with pytest.raises(ValueError):
+                api.run(zs, "full", xi_out=np.array([0.0, 1.0]))
+        for nproc in (0, -1, 1.5, True):
+            with pytest.raises(ValueError):
+                generate_pdf_lnmu(1, nproc=nproc)
+        solver.assert_not_called()
+
+
+def test_parameter_forwarding_and_jacobian():
+    def fake_run(zs, cfg, **kw):
+        assert zs == 2
+        assert kw["cosmo"] == dict(h=.7, Om=.3, sigma8=.9, Ob=.05, ns=.97, zeq=3450, Ok=0.0,
+                                   w0=-1.0, wa=0.0, growth_mode="ode", radiation="engine")
+        assert kw["closure"] == "gauss"
+        x = kw["xi_out"]
+        return np.ones_like(x), fake_info(x)
+    mu = np.array([.5, 1, 2])
+    with patch("sgl_analytic_amd.api.run", side_effect=fake_run):
+        x, p, info = generate_pdf(2, .7, .3, .9, .05, .97, 3450, mu=mu, return_info=True)
+    np.testing.assert_array_equal(x, mu)
+    np.testing.assert_array_equal(p, 1 / mu)
+    assert info["moment_interval"] == [-.5, 3.0]
+
+
+def test_negative_residual_is_visible():
+    with patch("sgl_analytic_amd.api.run", return_value=(np.array([-.1, 1.0]), fake_info(None))):
+        with pytest.warns(RuntimeWarning):
+            _, p, info = generate_pdf_lnmu(1, lnmu=[0, 1], return_info=True)
+    assert p[0] == -.1
+    assert abs(info["diagnostics"]["negative_mass"] - .05) < 1e-15
+
+
+def test_nonfinite_output_raises():
+    with patch("sgl_analytic_amd.api.run", return_value=(np.array([np.nan, 1.0]), fake_info(None))):
+        with pytest.raises(RuntimeError):
+            generate_pdf_lnmu(1, lnmu=[0, 1])
+
+
+def test_solver_refuses_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from sgl_analytic_amd import _lib
+    from sgl_analytic_amd.solver import Shape, Solver
+    with pytest.raises(_lib.SglError):
+        Solver(Shape(n_z_max=4, n_m=4, n_b=4, n_k=16, n_out=8), 1, path="exact")
