@@ -111,7 +111,8 @@ sgl_status sgl_lnphi_binned(const void* tables, const void* hist, int32_t B,
 
 /*
  * Two-point term Psi_2[b][n] = 1/2 sum_ij C_ij amp_i amp_j (complex square,
- * no conjugate).  precision: SGL_PREC_FP64 (vector ALU) or SGL_PREC_FP32 (MFMA).
+ * no conjugate).  precision: SGL_PREC_FP64 or SGL_PREC_FP32 (the arithmetic of
+ * the sum; amplitudes and C staged in LDS at that precision, result FP64).
  * Replaces: Lam.Q (sgl_full.py:3353-3397), nord = 1.
  */
 sgl_status sgl_quadform(const void* tables, const double* amp, int32_t B, const sgl_shape* shape,

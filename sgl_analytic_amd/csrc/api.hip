@@ -13,8 +13,8 @@ namespace {
 bool shape_ok(const sgl_shape* s) {
   if (s == nullptr) return false;
   if (s->n_z_max < 1 || s->n_z_max > kMaxSlices) return false;
-  if (s->n_m < 1 || s->n_m > 4096) return false;
-  if (s->n_b < 1 || s->n_b > 1 << 16) return false;
+  if (s->n_m < 1 || s->n_m > kMaxMasses) return false;
+  if (s->n_b < 1 || s->n_b > kMaxB) return false;
   if (s->n_k < 2 || s->n_k > 1 << 20) return false;
   if (s->n_out < 2 || s->n_out > 1 << 16) return false;
   if (!(s->m_lo_h > 0 && s->m_hi_h > s->m_lo_h)) return false;
@@ -48,7 +48,7 @@ size_t sgl_tables_bytes(const sgl_shape* shape, int32_t B) {
 
 size_t sgl_hist_bytes(const sgl_shape* shape, int32_t B) {
   if (!shape_ok(shape) || !batch_ok(B)) return 0;
-  return (hist_layout_fine(B) + hist_layout_slice(*shape, B)) * sizeof(double);
+  return hist_layout(*shape, B).total * sizeof(double);
 }
 
 size_t sgl_filon_bytes(const sgl_shape* shape, int32_t precision) {
@@ -103,9 +103,8 @@ sgl_status sgl_bin_cells(const void* tables, int32_t B, const sgl_shape* shape, 
   if (!shape_ok(shape) || !batch_ok(B) || !tables || !hist) return SGL_ERR_INVALID;
   TablesLayout L = tables_layout(*shape, B);
   const double* T = static_cast<const double*>(tables);
-  double* H = static_cast<double*>(hist);
-  return (sgl_status)launch_bin(T + L.rows, T + L.cosmo, T + L.uedge, *shape, B, H,
-                                H + hist_layout_fine(B), as_stream(stream));
+  return (sgl_status)launch_bin(T + L.rows, T + L.cosmo, T + L.uedge, *shape, B,
+                                static_cast<double*>(hist), as_stream(stream));
 }
 
 sgl_status sgl_lnphi_binned(const void* tables, const void* hist, int32_t B,
@@ -114,9 +113,8 @@ sgl_status sgl_lnphi_binned(const void* tables, const void* hist, int32_t B,
   if ((shape->flags & SGL_FLAG_CLUSTERING) && !amp) return SGL_ERR_INVALID;
   TablesLayout L = tables_layout(*shape, B);
   const double* T = static_cast<const double*>(tables);
-  const double* H = static_cast<const double*>(hist);
-  return (sgl_status)launch_lnphi_binned(H, H + hist_layout_fine(B), T + L.cosmo, T + L.knodes,
-                                         *shape, B, lam, amp, as_stream(stream));
+  return (sgl_status)launch_lnphi_binned(static_cast<const double*>(hist), T + L.cosmo,
+                                         T + L.knodes, *shape, B, lam, amp, as_stream(stream));
 }
 
 sgl_status sgl_quadform(const void* tables, const double* amp, int32_t B, const sgl_shape* shape,
@@ -125,7 +123,7 @@ sgl_status sgl_quadform(const void* tables, const double* amp, int32_t B, const 
   if (precision != SGL_PREC_FP64 && precision != SGL_PREC_FP32) return SGL_ERR_UNSUPPORTED;
   TablesLayout L = tables_layout(*shape, B);
   const double* T = static_cast<const double*>(tables);
-  return (sgl_status)launch_quadform(T + L.cov, T + L.cosmo, amp, B, *shape, psi2,
+  return (sgl_status)launch_quadform(T + L.cov, T + L.cosmo, amp, B, *shape, precision, psi2,
                                      as_stream(stream));
 }
 

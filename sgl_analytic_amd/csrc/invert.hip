@@ -12,39 +12,54 @@ namespace sgl {
 
 // ------------------------------------------------------------- quadform --
 // Psi_2[b][n] = 1/2 sum_ij C_ij a_i a_j (complex square, Lam.Q nord = 1,
-// sgl_full.py:3353-3397): u_j = sum_i C_ij a_i, Psi_2 = 1/2 sum_j u_j a_j.
-__global__ __launch_bounds__(256) void k_quadform_fp64(const double* __restrict__ cov,
-                                                       const double* __restrict__ cosmo,
-                                                       const double* __restrict__ amp, int B,
-                                                       int n_z_max, int n_k,
-                                                       double* __restrict__ psi2) {
-  __shared__ double sC[kMaxSlices * kMaxSlices];
+// sgl_full.py:3353-3397).  Workgroup = (instance, 32 k nodes): the tile's
+// amplitudes and C in LDS (T = double for the plain path, float for the
+// batched path), 8 threads per k node each taking every 8th column j of
+// u_j = sum_i C_ij a_i, partial sums combined with lane shuffles.
+constexpr int kQfTile = 32;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_quadform(const double* __restrict__ cov,
+                                                  const double* __restrict__ cosmo,
+                                                  const double* __restrict__ amp, int n_z_max,
+                                                  int n_k, double* __restrict__ psi2) {
+  __shared__ T sC[kMaxSlices * kMaxSlices];
+  __shared__ T sa[kMaxSlices][kQfTile][2];
   const int b = blockIdx.y, t = threadIdx.x;
-  const int n = blockIdx.x * 256 + t;
+  const int n0 = blockIdx.x * kQfTile;
   const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
   const double* C = cov + (size_t)b * n_z_max * n_z_max;
-  for (int q = t; q < n_z * n_z; q += 256) sC[q] = C[(q / n_z) * n_z_max + (q % n_z)];
-  __syncthreads();
-  if (n >= n_k) return;
-  double ax[kMaxSlices], ay[kMaxSlices];
-  for (int i = 0; i < n_z; ++i) {
+  for (int q = t; q < n_z * n_z; q += 256) sC[q] = (T)C[(q / n_z) * n_z_max + (q % n_z)];
+  for (int q = t; q < n_z * kQfTile; q += 256) {
+    int i = q / kQfTile, kk = q % kQfTile, n = n0 + kk;
     size_t o = 2 * (((size_t)b * n_z_max + i) * n_k + n);
-    ax[i] = amp[o];
-    ay[i] = amp[o + 1];
+    sa[i][kk][0] = (n < n_k) ? (T)amp[o] : (T)0;
+    sa[i][kk][1] = (n < n_k) ? (T)amp[o + 1] : (T)0;
   }
-  double sr = 0.0, si = 0.0;
-  for (int j = 0; j < n_z; ++j) {
-    double ux = 0.0, uy = 0.0;
+  __syncthreads();
+  const int kk = t >> 3, part = t & 7;
+  T sr = 0, si = 0;
+  for (int j = part; j < n_z; j += 8) {
+    T ux = 0, uy = 0;
     for (int i = 0; i < n_z; ++i) {
-      double c = sC[i * n_z + j];
-      ux = fma(c, ax[i], ux);
-      uy = fma(c, ay[i], uy);
+      T c = sC[i * n_z + j];
+      ux += c * sa[i][kk][0];
+      uy += c * sa[i][kk][1];
     }
-    sr += ux * ax[j] - uy * ay[j];
-    si += ux * ay[j] + uy * ax[j];
+    T ax = sa[j][kk][0], ay = sa[j][kk][1];
+    sr += ux * ax - uy * ay;
+    si += ux * ay + uy * ax;
   }
-  psi2[2 * ((size_t)b * n_k + n)] = 0.5 * sr;
-  psi2[2 * ((size_t)b * n_k + n) + 1] = 0.5 * si;
+  double dr = (double)sr, di = (double)si;
+  for (int m = 4; m > 0; m >>= 1) {
+    dr += __shfl_xor(dr, m, 64);
+    di += __shfl_xor(di, m, 64);
+  }
+  const int n = n0 + kk;
+  if (part == 0 && n < n_k) {
+    psi2[2 * ((size_t)b * n_k + n)] = 0.5 * dr;
+    psi2[2 * ((size_t)b * n_k + n) + 1] = 0.5 * di;
+  }
 }
 
 // --------------------------------------------------------------- window --
@@ -410,10 +425,14 @@ __global__ __launch_bounds__(256) void k_diag(const PT* __restrict__ p, sgl_shap
 
 // ------------------------------------------------------------- launchers --
 int launch_quadform(const double* cov, const double* cosmo, const double* amp, int B,
-                    const sgl_shape& sh, double* psi2, hipStream_t st) {
-  dim3 grid((sh.n_k + 255) / 256, B);
-  hipLaunchKernelGGL(k_quadform_fp64, grid, dim3(256), 0, st, cov, cosmo, amp, B, sh.n_z_max,
-                     sh.n_k, psi2);
+                    const sgl_shape& sh, int prec, double* psi2, hipStream_t st) {
+  dim3 grid((sh.n_k + kQfTile - 1) / kQfTile, B);
+  if (prec == SGL_PREC_FP64)
+    hipLaunchKernelGGL(k_quadform<double>, grid, dim3(256), 0, st, cov, cosmo, amp, sh.n_z_max,
+                       sh.n_k, psi2);
+  else
+    hipLaunchKernelGGL(k_quadform<float>, grid, dim3(256), 0, st, cov, cosmo, amp, sh.n_z_max,
+                       sh.n_k, psi2);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 

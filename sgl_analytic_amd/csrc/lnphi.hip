@@ -4,42 +4,24 @@
 //   a_i(k)    = sum_{c in slice i} beta_c w_c (e^{i k xi_c} - 1)
 //
 // Cells are never stored: every workgroup regenerates the cells of its slice
-// from the (slice, mass) rows (truncated NFW, jump xi = -ln detA, tilted
-// weight w detA) into LDS and streams them against its tile of k nodes.
+// from the (slice, mass) rows -- per-row constants and the impact-parameter
+// grid in LDS, truncated NFW (kappa, gamma), jump xi = -ln detA, tilted
+// weight w detA -- and streams them against its k nodes or its histogram.
 //
 // Exact path (k_lnphi_exact): every cell, FP64 sincos, the reference's
 // cancellation-free forms cos - 1 = -2 sin^2(ph/2), sin - ph by series below
 // |ph| = 1e-2 (sgl_full.py:2599-2613).
-// Binned path (k_bin_cells + k_lnphi_binned): cells histogrammed on a
-// signed-log jump lattice with (w, w xi, w xi^2) per bin; each bin acts as
-// w e^{i k xibar - k^2 v / 2}, v the in-bin variance.  Phase in FP64,
-// reduced to revolutions, sin/cos on the FP32 hardware path, compensated
-// terms and accumulation in FP64.
+// Binned path: k_bin_cells histograms each slice's cells on a signed-log jump
+// lattice in LDS (w, w xi and beta w per fine bin), writes the slice's
+// fine moments as a partial and its coarse two-point bins as compacted
+// records; k_reduce_fine adds the partials in slice order (no atomics across
+// workgroups: reproducible) and compacts the non-empty bins; k_lnphi_binned
+// evaluates each bin as w e^{i k xibar - k^2 v / 2}, v the in-bin variance,
+// phase in FP64 reduced to revolutions, sin/cos on the FP32 hardware path,
+// compensated small-phase terms and accumulation in FP64.
 #include "sgl_amd_internal.h"
 
 namespace sgl {
-
-constexpr int kCellTile = 256;
-
-struct CellGen {
-  const double* rows;   // [n_m][kRowStride] of this (instance, slice)
-  const double* uedge;  // [n_b + 1]
-  int n_b;
-  __device__ inline void gen(int c, double& xi, double& ws, double& bws) const {
-    const int j = c / n_b, l = c - j * n_b;
-    const double* R = rows + (size_t)j * kRowStride;
-    const double ks = R[kRowKs], rs = R[kRowRs], xt = R[kRowXt], rvir = R[kRowRvir];
-    const double wrow = R[kRowW], beta = R[kRowBeta];
-    if (!(wrow > 0.0)) { xi = 0.0; ws = 0.0; bws = 0.0; return; }
-    const double b0 = rvir * uedge[l], b1 = rvir * uedge[l + 1];
-    const double x = sqrt(b0 * b1) / rs;
-    const double area = kPi * (b1 * b1 - b0 * b0);
-    double kap, gam;
-    trunc_kg(x, ks, xt, kap, gam);
-    cell_jump(kap, gam, wrow * area, xi, ws);
-    bws = beta * ws;
-  }
-};
 
 __global__ void k_fill_uedges(const sgl_shape sh, double* __restrict__ ue, double* __restrict__ kn,
                               double* __restrict__ m_nodes_h) {
@@ -59,8 +41,44 @@ __global__ void k_fill_uedges(const sgl_shape sh, double* __restrict__ ue, doubl
   }
 }
 
+// Per-slice LDS staging: row constants and the impact-parameter geometry
+// (geometric bin centre and u1^2 - u0^2 of every column).
+struct SliceStage {
+  RowConst rc[kMaxMasses];
+  double ug[kMaxB];
+  double du2[kMaxB];
+};
+
+__device__ inline void stage_slice(SliceStage& S, const double* __restrict__ rows,
+                                   const double* __restrict__ uedge, const sgl_shape& sh) {
+  for (int j = threadIdx.x; j < sh.n_m; j += blockDim.x) S.rc[j] = row_const(rows + (size_t)j * kRowStride);
+  for (int l = threadIdx.x; l < sh.n_b; l += blockDim.x) {
+    double u0 = uedge[l], u1 = uedge[l + 1];
+    S.ug[l] = sqrt(u0 * u1);
+    S.du2[l] = u1 * u1 - u0 * u0;
+  }
+}
+
+// Cell c = (row j, column l) of the staged slice.  FastLog: the batched
+// path's FP32 log1p for the jump (the exact path keeps the FP64 log).
+template <bool FastLog = false>
+__device__ __forceinline__ void gen_cell(const SliceStage& S, int n_b, int c, double& xi,
+                                         double& ws, double& bw) {
+  const int j = c / n_b, l = c - j * n_b;
+  const RowConst& rc = S.rc[j];
+  if (rc.valid == 0.0) { xi = 0.0; ws = 0.0; bw = 0.0; return; }
+  double kap, gam;
+  trunc_kg_row(S.ug[l] * rc.s, rc, kap, gam);
+  if (FastLog)
+    cell_jump_f32log(kap, gam, rc.warea * S.du2[l], xi, ws);
+  else
+    cell_jump(kap, gam, rc.warea * S.du2[l], xi, ws);
+  bw = rc.beta * ws;
+}
+
 // ------------------------------------------------------------ exact path --
-constexpr int kKpt = 4;  // k nodes per thread
+constexpr int kCellTile = 256;
+constexpr int kKpt = 2;  // k nodes per thread
 
 __global__ __launch_bounds__(256) void k_lnphi_exact(const double* __restrict__ rows,
                                                      const double* __restrict__ cosmo,
@@ -68,6 +86,7 @@ __global__ __launch_bounds__(256) void k_lnphi_exact(const double* __restrict__ 
                                                      const double* __restrict__ knodes,
                                                      sgl_shape sh, double* __restrict__ part,
                                                      double* __restrict__ amp) {
+  __shared__ SliceStage S;
   __shared__ double sxi[kCellTile], sws[kCellTile], sbw[kCellTile];
   const int t = threadIdx.x;
   const int i = blockIdx.y, b = blockIdx.z;
@@ -82,14 +101,14 @@ __global__ __launch_bounds__(256) void k_lnphi_exact(const double* __restrict__ 
     lr[q] = li[q] = ar[q] = ai[q] = 0.0;
   }
   if (i < n_z) {
-    CellGen g{rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh.n_b};
+    stage_slice(S, rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh);
     const int n_cells = sh.n_m * sh.n_b;
     for (int c0 = 0; c0 < n_cells; c0 += kCellTile) {
       __syncthreads();
       {
         int c = c0 + t;
         double xi = 0.0, ws = 0.0, bw = 0.0;
-        if (c < n_cells) g.gen(c, xi, ws, bw);
+        if (c < n_cells) gen_cell(S, sh.n_b, c, xi, ws, bw);
         sxi[t] = xi; sws[t] = ws; sbw[t] = bw;
       }
       __syncthreads();
@@ -151,91 +170,135 @@ __global__ void k_sum_slices(const double* __restrict__ part, int B, int n_z_max
 }
 
 // ----------------------------------------------------------- binned path --
+// Bin index from xi with the FP32 hardware log: which of two neighbouring
+// bins an edge case lands in does not matter, every bin keeps its exact
+// centroid and variance.
 __device__ __forceinline__ int fine_bin(double xi) {
+  constexpr float kPosScale = (float)(kFinePos / 31.032218497590705);   // ln(30 / 1e-12)
+  constexpr float kNegScale = (float)(kFineNeg / 21.82187812561452);    // ln(30 / 1e-8)
+  constexpr float kLnLo = -27.631021115928547f;                          // ln 1e-12
+  constexpr float kLnNegLo = -18.420680743952367f;                       // ln 1e-8
   if (xi > 0.0) {
-    if (xi < kXiLo) return 0;
-    int k = (int)(log(xi / kXiLo) * (kFinePos / log(kXiHi / kXiLo)));
+    float lx = __logf((float)xi);
+    int k = (int)((lx - kLnLo) * kPosScale);
     return min(max(k, 0), kFinePos - 1);
   }
-  double a = -xi;
-  int k = (a < kXiNegLo) ? 0 : (int)(log(a / kXiNegLo) * (kFineNeg / log(kXiNegHi / kXiNegLo)));
+  float lx = __logf((float)(-xi));
+  int k = (int)((lx - kLnNegLo) * kNegScale);
   return kFinePos + min(max(k, 0), kFineNeg - 1);
 }
 
-__device__ __forceinline__ int slice_bin(double xi) {
-  if (xi > 0.0) {
-    if (xi < kXiLo) return 0;
-    int k = (int)(log(xi / kXiLo) * (kSlicePos / log(kXiHi / kXiLo)));
-    return min(max(k, 0), kSlicePos - 1);
-  }
-  double a = -xi;
-  int k = (a < kXiNegLo) ? 0 : (int)(log(a / kXiNegLo) * (kSliceNeg / log(kXiNegHi / kXiNegLo)));
-  return kSlicePos + min(max(k, 0), kSliceNeg - 1);
-}
+constexpr int kBinThreads = 1024;
 
-__global__ void k_zero(double* __restrict__ p, size_t n) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t s = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += s) p[i] = 0.0;
-}
-
-__global__ __launch_bounds__(256) void k_bin_cells(const double* __restrict__ rows,
-                                                   const double* __restrict__ cosmo,
-                                                   const double* __restrict__ uedge, sgl_shape sh,
-                                                   double* __restrict__ hfine,
-                                                   double* __restrict__ hslice) {
+__global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restrict__ rows,
+                                                           const double* __restrict__ cosmo,
+                                                           const double* __restrict__ uedge,
+                                                           sgl_shape sh, double* __restrict__ hist) {
+  __shared__ SliceStage S;
   __shared__ double lf[kFineBins * kMoments];
-  __shared__ double ls[kSliceBins * kMoments];
+  __shared__ double lb[kFineBins];
+  __shared__ double lc[kSliceBins * 3];
+  __shared__ int cnt;
   const int t = threadIdx.x, i = blockIdx.x, b = blockIdx.y;
   const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
-  for (int q = t; q < kFineBins * kMoments; q += 256) lf[q] = 0.0;
-  for (int q = t; q < kSliceBins * kMoments; q += 256) ls[q] = 0.0;
+  const HistLayout HL = hist_layout(sh, gridDim.y);
+  for (int q = t; q < kFineBins * kMoments; q += kBinThreads) lf[q] = 0.0;
+  for (int q = t; q < kFineBins; q += kBinThreads) lb[q] = 0.0;
+  for (int q = t; q < kSliceBins * 3; q += kBinThreads) lc[q] = 0.0;
+  if (t == 0) cnt = 0;
+  if (i < n_z) stage_slice(S, rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh);
   __syncthreads();
   if (i < n_z) {
-    CellGen g{rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh.n_b};
     const int n_cells = sh.n_m * sh.n_b;
-    for (int c = t; c < n_cells; c += 256) {
+    for (int c = t; c < n_cells; c += kBinThreads) {
       double xi, ws, bw;
-      g.gen(c, xi, ws, bw);
+      gen_cell<true>(S, sh.n_b, c, xi, ws, bw);
       if (ws == 0.0) continue;
-      int f = fine_bin(xi) * kMoments;
-      atomicAdd(&lf[f], ws);
-      atomicAdd(&lf[f + 1], ws * xi);
-      atomicAdd(&lf[f + 2], ws * xi * xi);
-      int s = slice_bin(xi) * kMoments;
-      atomicAdd(&ls[s], bw);
-      atomicAdd(&ls[s + 1], bw * xi);
-      atomicAdd(&ls[s + 2], bw * xi * xi);
+      const int f = fine_bin(xi);
+      atomicAdd(&lf[f * kMoments], ws);
+      atomicAdd(&lf[f * kMoments + 1], ws * xi);
+      atomicAdd(&lb[f], bw);
     }
   }
   __syncthreads();
-  double* gf = hfine + (size_t)b * kFineBins * kMoments;
-  for (int q = t; q < kFineBins * kMoments; q += 256)
-    if (lf[q] != 0.0) unsafeAtomicAdd(&gf[q], lf[q]);
-  double* gs = hslice + ((size_t)b * sh.n_z_max + i) * kSliceBins * kMoments;
-  for (int q = t; q < kSliceBins * kMoments; q += 256) gs[q] = ls[q];
+  // coarse two-point bins from the fine ones: beta w placed at the fine
+  // centroid; the spread of fine centroids gives the coarse variance
+  for (int f = t; f < kFineBins; f += kBinThreads) {
+    const double w = lf[f * kMoments];
+    if (w == 0.0) continue;
+    const double m = lf[f * kMoments + 1] / w;
+    const double bwv = lb[f];
+    const int cb = f / kCoarse;
+    atomicAdd(&lc[cb * 3], bwv);
+    atomicAdd(&lc[cb * 3 + 1], bwv * m);
+    atomicAdd(&lc[cb * 3 + 2], bwv * m * m);
+  }
+  // this slice's fine partial (fixed position: added in slice order later)
+  double* P = hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments;
+  for (int q = t; q < kFineBins * kMoments; q += kBinThreads) P[q] = lf[q];
+  __syncthreads();
+  // compacted coarse records of this slice (order of non-empty bins kept by
+  // a serial pass: 144 bins)
+  double* R = hist + HL.srec + ((size_t)b * sh.n_z_max + i) * kSliceBins * kRec;
+  if (t == 0) {
+    int n = 0;
+    for (int cb = 0; cb < kSliceBins; ++cb) {
+      const double w = lc[cb * 3];
+      if (w == 0.0) continue;
+      const double m = lc[cb * 3 + 1] / w;
+      const double v = lc[cb * 3 + 2] / w - m * m;
+      R[n * kRec + 0] = m * (0.25 / kPi);
+      R[n * kRec + 1] = m;
+      R[n * kRec + 2] = w;
+      R[n * kRec + 3] = (v > 0.0) ? v : 0.0;
+      ++n;
+    }
+    hist[HL.scnt + (size_t)b * sh.n_z_max + i] = (double)n;
+  }
 }
 
-// One bin as a pair (xibar / (4 pi), xibar, w, v).
+// Sum the slice partials of one instance in slice order, compact non-empty bins.
+__global__ __launch_bounds__(256) void k_reduce_fine(const double* __restrict__ cosmo,
+                                                     sgl_shape sh, double* __restrict__ hist) {
+  __shared__ int flag[kFineBins];
+  __shared__ int pos[kFineBins];
+  __shared__ double acc[kFineBins * kMoments];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const HistLayout HL = hist_layout(sh, gridDim.x);
+  const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
+  for (int q = t; q < kFineBins * kMoments; q += 256) {
+    double s = 0.0;
+    for (int i = 0; i < n_z; ++i)
+      s += hist[HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments + q];
+    acc[q] = s;
+  }
+  __syncthreads();
+  for (int f = t; f < kFineBins; f += 256) flag[f] = (acc[f * kMoments] != 0.0) ? 1 : 0;
+  __syncthreads();
+  if (t == 0) {
+    int n = 0;
+    for (int f = 0; f < kFineBins; ++f) { pos[f] = n; n += flag[f]; }
+    hist[HL.fcnt + b] = (double)n;
+  }
+  __syncthreads();
+  double* R = hist + HL.frec + (size_t)b * kFineBins * kRec;
+  for (int f = t; f < kFineBins; f += 256) {
+    if (!flag[f]) continue;
+    const double w = acc[f * kMoments];
+    const double m = acc[f * kMoments + 1] / w;
+    double* r = R + (size_t)pos[f] * kRec;
+    r[0] = m * (0.25 / kPi);
+    r[1] = m;
+    r[2] = w;
+    r[3] = 0.0;
+  }
+}
+
 struct BinRec {
   double x4pi, xi, w, v;
 };
 
-__device__ __forceinline__ BinRec load_bin(const double* h) {
-  BinRec r;
-  double w = h[0];
-  if (w == 0.0) { r.x4pi = r.xi = r.w = r.v = 0.0; return r; }
-  double m = h[1] / w;
-  double v = h[2] / w - m * m;
-  r.xi = m;
-  r.x4pi = m * (0.25 / kPi);
-  r.w = w;
-  r.v = (v > 0.0) ? v : 0.0;
-  return r;
-}
-
-// e^{i ph - k^2 v/2} - 1 split into (cm1g, sng) with ph = k xibar:
-//   re = cos(ph) g - 1, im = sin(ph) g, and smpg = sin(ph) g - ph.
+// e^{i ph - k^2 v/2} - 1 = re + i im with ph = k xibar, and smpg = im - ph.
 __device__ __forceinline__ void bin_phase(double k, const BinRec& r, double& re, double& im,
                                           double& smpg) {
   double ph = k * r.xi;
@@ -263,8 +326,7 @@ __device__ __forceinline__ void bin_phase(double k, const BinRec& r, double& re,
 
 constexpr int kBinTile = 256;
 
-__global__ __launch_bounds__(256) void k_lnphi_binned(const double* __restrict__ hfine,
-                                                      const double* __restrict__ hslice,
+__global__ __launch_bounds__(256) void k_lnphi_binned(const double* __restrict__ hist,
                                                       const double* __restrict__ cosmo,
                                                       const double* __restrict__ knodes,
                                                       sgl_shape sh, double* __restrict__ lam,
@@ -273,17 +335,22 @@ __global__ __launch_bounds__(256) void k_lnphi_binned(const double* __restrict__
   const int t = threadIdx.x, b = blockIdx.y;
   const int n = blockIdx.x * 256 + t;
   const int n_k = sh.n_k;
+  const HistLayout HL = hist_layout(sh, gridDim.y);
   const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
   const double k = (n < n_k) ? knodes[n] : 0.0;
   double lr = 0.0, li = 0.0;
-  const double* hf = hfine + (size_t)b * kFineBins * kMoments;
-  for (int c0 = 0; c0 < kFineBins; c0 += kBinTile) {
+  const int nf = (int)hist[HL.fcnt + b];
+  const double* fr = hist + HL.frec + (size_t)b * kFineBins * kRec;
+  for (int c0 = 0; c0 < nf; c0 += kBinTile) {
     __syncthreads();
-    sb[t] = (c0 + t < kFineBins) ? load_bin(hf + (size_t)(c0 + t) * kMoments) : BinRec{0, 0, 0, 0};
+    if (c0 + t < nf) {
+      const double* r = fr + (size_t)(c0 + t) * kRec;
+      sb[t] = BinRec{r[0], r[1], r[2], r[3]};
+    }
     __syncthreads();
-    for (int c = 0; c < kBinTile; ++c) {
+    const int nb = min(kBinTile, nf - c0);
+    for (int c = 0; c < nb; ++c) {
       const BinRec r = sb[c];
-      if (r.w == 0.0) continue;
       double re, im, smpg;
       bin_phase(k, r, re, im, smpg);
       lr = fma(r.w, re, lr);
@@ -298,21 +365,20 @@ __global__ __launch_bounds__(256) void k_lnphi_binned(const double* __restrict__
   for (int i = 0; i < sh.n_z_max; ++i) {
     double ar = 0.0, ai = 0.0;
     if (i < n_z) {
-      const double* hs = hslice + ((size_t)b * sh.n_z_max + i) * kSliceBins * kMoments;
-      for (int c0 = 0; c0 < kSliceBins; c0 += kBinTile) {
-        __syncthreads();
-        sb[t] = (c0 + t < kSliceBins) ? load_bin(hs + (size_t)(c0 + t) * kMoments)
-                                      : BinRec{0, 0, 0, 0};
-        __syncthreads();
-        const int nb = min(kBinTile, kSliceBins - c0);
-        for (int c = 0; c < nb; ++c) {
-          const BinRec r = sb[c];
-          if (r.w == 0.0) continue;
-          double re, im, smpg;
-          bin_phase(k, r, re, im, smpg);
-          ar = fma(r.w, re, ar);
-          ai = fma(r.w, im, ai);
-        }
+      const int ns = (int)hist[HL.scnt + (size_t)b * sh.n_z_max + i];
+      const double* sr = hist + HL.srec + ((size_t)b * sh.n_z_max + i) * kSliceBins * kRec;
+      __syncthreads();
+      if (t < ns) {
+        const double* r = sr + (size_t)t * kRec;
+        sb[t] = BinRec{r[0], r[1], r[2], r[3]};
+      }
+      __syncthreads();
+      for (int c = 0; c < ns; ++c) {
+        const BinRec r = sb[c];
+        double re, im, smpg;
+        bin_phase(k, r, re, im, smpg);
+        ar = fma(r.w, re, ar);
+        ai = fma(r.w, im, ai);
       }
     }
     if (n < n_k) {
@@ -336,20 +402,17 @@ int launch_lnphi_exact(const double* rows, const double* cosmo, const double* ue
 }
 
 int launch_bin(const double* rows, const double* cosmo, const double* uedge, const sgl_shape& sh,
-               int B, double* hfine, double* hslice, hipStream_t st) {
-  size_t nf = hist_layout_fine(B);
-  hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, st, hfine, nf);
-  hipLaunchKernelGGL(k_bin_cells, dim3(sh.n_z_max, B), dim3(256), 0, st, rows, cosmo, uedge, sh,
-                     hfine, hslice);
+               int B, double* hist, hipStream_t st) {
+  hipLaunchKernelGGL(k_bin_cells, dim3(sh.n_z_max, B), dim3(kBinThreads), 0, st, rows, cosmo, uedge,
+                     sh, hist);
+  hipLaunchKernelGGL(k_reduce_fine, dim3(B), dim3(256), 0, st, cosmo, sh, hist);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
-int launch_lnphi_binned(const double* hfine, const double* hslice, const double* cosmo,
-                        const double* knodes, const sgl_shape& sh, int B, double* lam,
-                        double* amp, hipStream_t st) {
+int launch_lnphi_binned(const double* hist, const double* cosmo, const double* knodes,
+                        const sgl_shape& sh, int B, double* lam, double* amp, hipStream_t st) {
   dim3 grid((sh.n_k + 255) / 256, B);
-  hipLaunchKernelGGL(k_lnphi_binned, grid, dim3(256), 0, st, hfine, hslice, cosmo, knodes, sh, lam,
-                     amp);
+  hipLaunchKernelGGL(k_lnphi_binned, grid, dim3(256), 0, st, hist, cosmo, knodes, sh, lam, amp);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 

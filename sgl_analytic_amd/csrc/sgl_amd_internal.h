@@ -48,8 +48,9 @@ inline TablesLayout tables_layout(const sgl_shape& s, int B) {
 }
 
 // Fine jump lattice of the batched path: signed log bins.
-//   positive side: kFinePos bins log-uniform on [kXiLo, kXiMax], bin 0 also
-//   takes 0 < xi < kXiLo; negative side: kFineNeg bins log-uniform on |xi|.
+//   positive side: kFinePos bins log-uniform on [kXiLo, kXiHi], bin 0 also
+//   takes 0 < xi < kXiLo; negative side: kFineNeg bins log-uniform on |xi|
+//   in [kXiNegLo, kXiNegHi].
 constexpr int kFinePos = 2048;
 constexpr int kFineNeg = 256;
 constexpr int kFineBins = kFinePos + kFineNeg;
@@ -57,16 +58,29 @@ constexpr double kXiLo = 1e-12;
 constexpr double kXiHi = 30.0;
 constexpr double kXiNegLo = 1e-8;
 constexpr double kXiNegHi = 30.0;
-// Per-slice coarse lattice (two-point amplitudes).
-constexpr int kSlicePos = 112;
-constexpr int kSliceNeg = 16;
-constexpr int kSliceBins = kSlicePos + kSliceNeg;
-// moments per bin (FP64): w, w xi, w xi^2
-constexpr int kMoments = 3;
+// Per-slice coarse lattice of the two-point amplitudes: 32 fine bins each.
+constexpr int kCoarse = 32;
+constexpr int kSliceBins = kFineBins / kCoarse;
+constexpr int kMoments = 2;        // fine bins carry w, w xi (centroid)
+constexpr int kRec = 4;            // compacted bin record: xi/(4 pi), xi, w, v
+constexpr int kMaxMasses = 256;
+constexpr int kMaxB = 1024;
 
-inline size_t hist_layout_fine(int B) { return (size_t)B * kFineBins * kMoments; }
-inline size_t hist_layout_slice(const sgl_shape& s, int B) {
-  return (size_t)B * s.n_z_max * kSliceBins * kMoments;
+// Histogram buffer (doubles): per-slice partial fine moments, compacted fine
+// records + count per instance, compacted coarse records + count per slice.
+struct HistLayout {
+  size_t partial, frec, fcnt, srec, scnt, total;
+};
+__host__ __device__ inline size_t round32(size_t n) { return (n + 31) & ~size_t(31); }
+__host__ __device__ inline HistLayout hist_layout(const sgl_shape& s, int B) {
+  HistLayout L;
+  L.partial = 0;
+  L.frec = L.partial + round32((size_t)B * s.n_z_max * kFineBins * kMoments);
+  L.fcnt = L.frec + round32((size_t)B * kFineBins * kRec);
+  L.srec = L.fcnt + round32((size_t)B);
+  L.scnt = L.srec + round32((size_t)B * s.n_z_max * kSliceBins * kRec);
+  L.total = L.scnt + round32((size_t)B * s.n_z_max);
+  return L;
 }
 
 int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes_h,
@@ -78,12 +92,11 @@ int launch_lnphi_exact(const double* rows, const double* cosmo, const double* ue
                        const double* knodes, const sgl_shape& sh, int B, double* lam, double* amp,
                        double* work, hipStream_t st);
 int launch_bin(const double* rows, const double* cosmo, const double* uedge, const sgl_shape& sh,
-               int B, double* hfine, double* hslice, hipStream_t st);
-int launch_lnphi_binned(const double* hfine, const double* hslice, const double* cosmo,
-                        const double* knodes, const sgl_shape& sh, int B, double* lam,
-                        double* amp, hipStream_t st);
+               int B, double* hist, hipStream_t st);
+int launch_lnphi_binned(const double* hist, const double* cosmo, const double* knodes,
+                        const sgl_shape& sh, int B, double* lam, double* amp, hipStream_t st);
 int launch_quadform(const double* cov, const double* cosmo, const double* amp, int B,
-                    const sgl_shape& sh, double* psi2, hipStream_t st);
+                    const sgl_shape& sh, int prec, double* psi2, hipStream_t st);
 int launch_phi_pack(const double* lam, const double* psi2, const double* knodes,
                     const sgl_shape& sh, int B, int prec, void* out, hipStream_t st);
 int launch_filon(const double* knodes, const sgl_shape& sh, int prec, void* out, hipStream_t st);

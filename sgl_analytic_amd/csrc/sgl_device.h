@@ -239,9 +239,120 @@ __device__ inline void trunc_kg(double x, double ks, double xt, double& kap, dou
   }
 }
 
+// Per-row constants of a truncated NFW lens row (one (slice, mass) pair).
+struct RowConst {
+  double ks, s, xt, ic1, mut4ks, warea, beta, i0, jj, valid;
+};
+
+__device__ inline RowConst row_const(const double* R) {
+  // R: k_s, r_s, x_t, r_vir, w_row, beta
+  RowConst c;
+  c.ks = R[0];
+  c.xt = R[2];
+  c.s = R[3] / R[1];
+  c.ic1 = 1.0 / (1.0 + c.xt);
+  c.mut4ks = 4.0 * c.ks * (log1p(c.xt) - c.xt * c.ic1);
+  c.warea = R[4] * kPi * R[3] * R[3];
+  c.beta = R[5];
+  const double xt = c.xt, u1 = 1.0 + xt;
+  c.i0 = log(u1 / xt) - 1.0 / u1;
+  c.jj = 0.5 * (u1 * u1 - xt * xt) - 3.0 * xt * (u1 - xt) + 3.0 * xt * xt * log(u1 / xt) +
+         xt * xt * xt * (1.0 / u1 - 1.0 / xt);
+  c.valid = (R[4] > 0.0) ? 1.0 : 0.0;
+  return c;
+}
+
+// (kappa, gamma) of the truncated NFW row at x, the same closed forms as
+// trunc_kg with the shared subexpressions (arccosh(1/x), log x, 1/x, per-row
+// constants) evaluated once; the rare |x - 1| < 1e-5 band takes trunc_kg.
+__device__ inline void trunc_kg_row(double x, const RowConst& rc, double& kap, double& gam) {
+  const double xt = rc.xt;
+  if (x >= xt) {
+    kap = 0.0;
+    gam = rc.mut4ks / (x * x);
+    return;
+  }
+  if (fabs(x - 1.0) < kTcBand) {
+    trunc_kg(x, rc.ks, xt, kap, gam);
+    return;
+  }
+  constexpr double kLn2 = 0.6931471805599453;
+  const double a = x / xt;
+  const double rx = 1.0 / x, rx2 = rx * rx;
+  const double lx = log(x);
+  double F, h, I1, I2;
+  const double w = (1.0 - x) * (1.0 + x);
+  const bool series = fabs(w) < kNfwWSeries;
+  double T = 0.0;
+  if (series) {
+#pragma unroll
+    for (int n = kNfwNSeries - 1; n >= 0; --n) T = T * w + 1.0 / (2 * n + 3);
+    F = T;
+    h = lx - kLn2 + (1.0 + w * T);
+  }
+  const bool small = a < kTcASmall;
+  if (small) {
+    I1 = rc.i0 + 0.5 * a * a * rc.jj;
+    I2 = 0.5 * rc.i0 + 0.125 * a * a * rc.jj;
+  }
+  if (x < 1.0) {
+    const double e = (1.0 - x) * rx, su = sqrt(w), isu = 1.0 / su, iu = 1.0 / w;
+    const double A0 = log1p(e + sqrt(e * (e + 2.0)));
+    if (!series) {
+      F = (A0 * isu - 1.0) * iu;
+      h = (x < kNfwXH) ? ((kLn2 - lx) * x * x / (1.0 + su) + log1p(-x * x / (2.0 * (1.0 + su)))) * isu
+                       : lx - kLn2 + A0 * isu;
+    }
+    if (!small) {
+      const double c = xt, q = sqrt((c - x) * (c + x));
+      const double d = (x - 1.0) * (x - c) * rx * rc.ic1;
+      const double L = log((c + q) * rc.ic1 * 0.5) - (q - c) * rc.ic1;
+      const double At = log1p(d + sqrt(d * (d + 2.0)));
+      I1 = (q * rc.ic1 - 1.0) * iu + (A0 - At) * iu * isu;
+      I2 = (L + (A0 - At) * isu) * rx2;
+    }
+  } else {
+    const double v = -w, sv = sqrt(v), isv = 1.0 / sv, iv = 1.0 / v;
+    const double B0 = 2.0 * asin(sqrt((x - 1.0) * 0.5 * rx));
+    if (!series) {
+      F = (1.0 - B0 * isv) * iv;
+      h = lx - kLn2 + B0 * isv;
+    }
+    if (!small) {
+      const double c = xt, q = sqrt((c - x) * (c + x));
+      const double d = (x - 1.0) * (x - c) * rx * rc.ic1;
+      const double L = log((c + q) * rc.ic1 * 0.5) - (q - c) * rc.ic1;
+      const double Bt = 2.0 * asin(sqrt(fmax(-d, 0.0) * 0.5));
+      I1 = (1.0 - q * rc.ic1) * iv - (B0 - Bt) * iv * isv;
+      I2 = (L + (B0 - Bt) * isv) * rx2;
+    }
+  }
+  const double ku = 2.0 * F - 2.0 * I1;
+  const double kb = 4.0 * h * rx2 - 4.0 * I2;
+  kap = rc.ks * ku;
+  gam = rc.ks * (kb - ku);
+}
+
 // Jump of one cell: xi = -ln detA and the tilted weight w e^{-xi} = w detA;
 // dropped (weight 0) when detA <= 0 or xi > 30.
 constexpr double kXiMax = 30.0;
+// Batched-path variant: xi = -log1p(detA - 1) with detA - 1 = kappa (kappa - 2)
+// - gamma^2 formed in FP64 (no cancellation) and the log1p on the FP32
+// hardware path (relative error ~1e-7 in xi; the bin index and moments need
+// no more).
+__device__ __forceinline__ void cell_jump_f32log(double kap, double gam, double w, double& xi,
+                                                 double& ws) {
+  double dm1 = kap * (kap - 2.0) - gam * gam;
+  double detA = 1.0 + dm1;
+  if (!(detA > 0.0)) { xi = 0.0; ws = 0.0; return; }
+  float fd = (float)dm1;
+  float l1p = (fabsf(fd) < 1e-3f) ? fd * (1.0f - fd * (0.5f - fd * (1.0f / 3.0f)))
+                                  : __logf(1.0f + fd);
+  double x = -(double)l1p;
+  if (x > kXiMax) { xi = 0.0; ws = 0.0; return; }
+  xi = x;
+  ws = w * detA;
+}
 __device__ __forceinline__ void cell_jump(double kap, double gam, double w, double& xi, double& ws) {
   double detA = (1.0 - kap) * (1.0 - kap) - gam * gam;
   if (!(detA > 0.0)) { xi = 0.0; ws = 0.0; return; }

@@ -148,7 +148,8 @@ class Solver:
             ck(L.sgl_lnphi_binned(_ptr(self.tables), _ptr(self.hist), B, cs, _ptr(self.lam),
                                   _ptr(self.amp), st), "sgl_lnphi_binned")
         if self.shape.clustering:
-            ck(L.sgl_quadform(_ptr(self.tables), _ptr(self.amp), B, cs, PREC_FP64,
+            qp = PREC_FP64 if self.path == "exact" else PREC_FP32
+            ck(L.sgl_quadform(_ptr(self.tables), _ptr(self.amp), B, cs, qp,
                               _ptr(self.psi2), st), "sgl_quadform")
             psi = _ptr(self.psi2)
         ck(L.sgl_phi_pack(_ptr(self.lam), psi, _ptr(self.tables), B, cs, self.precision,
