@@ -164,6 +164,37 @@ sgl_status sgl_invert_fp64_at(const double* phi, const void* tables, int32_t B,
 sgl_status sgl_diagnostics(const void* p, int32_t p_is_double, int32_t B, const sgl_shape* shape,
                            double* diag, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Surrogate MLP (no reference counterpart: the reference is a solver, not an
+ * emulator, README.md:8).  A linear layer's forward and both backward
+ * products are one BF16 MFMA GEMM with strided operands:
+ *   C[m][n] = sum_k A(m,k) B(n,k),  A(m,k) = A[m lda_m + k lda_k] (bf16),
+ *   B(n,k) = B[n ldb_n + k ldb_k] (bf16), FP32 accumulation, and epilogue
+ *   PLAIN      C = beta C + acc
+ *   BIAS       C = acc + bias[n]
+ *   BIAS_SILU  Zout = acc + bias[n]; C = silu(Zout)
+ *   DSILU      C = acc * silu'(Z[m][n])
+ * C (FP32) and/or Cb (BF16) receive the result, row stride ldc.
+ */
+#define SGL_MLP_EPI_PLAIN 0
+#define SGL_MLP_EPI_BIAS 1
+#define SGL_MLP_EPI_BIAS_SILU 2
+#define SGL_MLP_EPI_DSILU 3
+sgl_status sgl_mlp_gemm(const void* A, long lda_m, long lda_k, const void* B, long ldb_n,
+                        long ldb_k, int32_t M, int32_t N, int32_t K, int32_t epi,
+                        const float* bias, const float* Z, long ldz, float* Zout, float* C,
+                        void* Cb, long ldc, float beta, void* stream);
+/* db[j] = sum_n dZ[n][j] over an [N][M] FP32 matrix (bias gradient). */
+sgl_status sgl_mlp_colsum(const float* dZ, int32_t N, int32_t M, float* db, void* stream);
+/* y = bf16(x) (round to nearest even), n elements. */
+sgl_status sgl_to_bf16(const float* x, size_t n, void* y, void* stream);
+/* Surrogate output -> exponent on the solver's k grid: y[b][2 j + (0|1)] is
+ * Re/Im ln phi at n_c coarse nodes (every (n_k-1)/(n_c-1)-th k node, first and
+ * last coinciding); cubic Lagrange in ln k onto all n_k nodes, written as
+ * lam[b][n] (re, im) for sgl_phi_pack (psi2 = NULL). */
+sgl_status sgl_lnphi_interp(const float* y, long ldy, int32_t n_c, int32_t B,
+                            const sgl_shape* shape, double* lam, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,8 +20,10 @@ EXPORTS = (
     "sgl_version", "sgl_tables_bytes", "sgl_hist_bytes", "sgl_filon_bytes", "sgl_phi_bytes",
     "sgl_tables_build", "sgl_lnphi_exact", "sgl_bin_cells", "sgl_lnphi_binned",
     "sgl_quadform", "sgl_phi_pack", "sgl_filon_build", "sgl_invert_gemm", "sgl_invert_fp64",
-    "sgl_diagnostics", "sgl_invert_fp64_at",
+    "sgl_diagnostics", "sgl_invert_fp64_at", "sgl_mlp_gemm", "sgl_mlp_colsum", "sgl_to_bf16",
+    "sgl_lnphi_interp",
 )
+EPI_PLAIN, EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU = 0, 1, 2, 3
 
 
 class CShape(ctypes.Structure):
@@ -71,6 +73,12 @@ def load(path: str = LIB_PATH):
         "sgl_invert_fp64": [vp, vp, i32, sp, vp, vp],
         "sgl_diagnostics": [vp, i32, i32, sp, vp, vp],
         "sgl_invert_fp64_at": [vp, vp, i32, sp, vp, i32, vp, vp],
+        "sgl_mlp_gemm": [vp, ctypes.c_long, ctypes.c_long, vp, ctypes.c_long, ctypes.c_long,
+                         i32, i32, i32, i32, vp, vp, ctypes.c_long, vp, vp, vp, ctypes.c_long,
+                         ctypes.c_float, vp],
+        "sgl_mlp_colsum": [vp, i32, i32, vp, vp],
+        "sgl_to_bf16": [vp, sz, vp, vp],
+        "sgl_lnphi_interp": [vp, ctypes.c_long, i32, i32, sp, vp, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -60,7 +60,8 @@ __device__ inline void stage_slice(SliceStage& S, const double* __restrict__ row
 }
 
 // Cell c = (row j, column l) of the staged slice.  FastLog: the batched
-// path's FP32 log1p for the jump (the exact path keeps the FP64 log).
+// path's FP32 lens profile away from x = 1 and FP32 log1p for the jump (the
+// exact path keeps FP64 throughout).
 template <bool FastLog = false>
 __device__ __forceinline__ void gen_cell(const SliceStage& S, int n_b, int c, double& xi,
                                          double& ws, double& bw) {
@@ -68,7 +69,15 @@ __device__ __forceinline__ void gen_cell(const SliceStage& S, int n_b, int c, do
   const RowConst& rc = S.rc[j];
   if (rc.valid == 0.0) { xi = 0.0; ws = 0.0; bw = 0.0; return; }
   double kap, gam;
-  trunc_kg_row(S.ug[l] * rc.s, rc, kap, gam);
+  const double x = S.ug[l] * rc.s;
+  if (FastLog && fabs((1.0 - x) * (1.0 + x)) >= kNfwWSeries) {
+    float kf, gf;
+    trunc_kg_row_f32((float)x, rc, kf, gf);
+    kap = kf;
+    gam = gf;
+  } else {
+    trunc_kg_row(x, rc, kap, gam);
+  }
   if (FastLog)
     cell_jump_f32log(kap, gam, rc.warea * S.du2[l], xi, ws);
   else

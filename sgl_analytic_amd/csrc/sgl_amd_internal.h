@@ -28,10 +28,17 @@ struct TablesLayout {
   size_t cosmo, derived, slices, mass, rows, cov, work, m_nodes, uedge, knodes, status, total;
 };
 
+// The shape-only grids come first, so their offsets do not depend on B: a
+// tables buffer built for any batch size serves every call that reads only
+// the grids (phi packing, Filon weights).
 inline TablesLayout tables_layout(const sgl_shape& s, int B) {
   TablesLayout L;
   size_t o = 0;
   auto take = [&](size_t n) { size_t r = o; o += (n + 31) & ~size_t(31); return r; };
+  L.m_nodes = take((size_t)s.n_m);
+  L.uedge = take((size_t)s.n_b + 1);
+  L.knodes = take((size_t)s.n_k);
+  L.status = take(1);
   L.cosmo = take((size_t)B * 16);
   L.derived = take((size_t)B * 16);
   L.slices = take((size_t)B * s.n_z_max * kSliceStride);
@@ -39,10 +46,6 @@ inline TablesLayout tables_layout(const sgl_shape& s, int B) {
   L.rows = take((size_t)B * s.n_z_max * s.n_m * kRowStride);
   L.cov = take((size_t)B * s.n_z_max * s.n_z_max);
   L.work = take((size_t)B * 20000);
-  L.m_nodes = take((size_t)s.n_m);
-  L.uedge = take((size_t)s.n_b + 1);
-  L.knodes = take((size_t)s.n_k);
-  L.status = take(1);
   L.total = o;
   return L;
 }

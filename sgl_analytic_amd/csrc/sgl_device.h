@@ -333,6 +333,66 @@ __device__ inline void trunc_kg_row(double x, const RowConst& rc, double& kap, d
   gam = rc.ks * (kb - ku);
 }
 
+// FP32 evaluation of the same closed forms for the batched path, used away
+// from x = 1 (|1 - x^2| >= 1/4) where none of the terms cancels badly; the
+// caller takes the FP64 trunc_kg_row inside that band.  Relative error of
+// kappa and gamma ~1e-6, far below the lattice binning error.
+__device__ inline void trunc_kg_row_f32(float x, const RowConst& rc, float& kap, float& gam) {
+  const float xt = (float)rc.xt;
+  if (x >= xt) {
+    kap = 0.0f;
+    gam = (float)rc.mut4ks / (x * x);
+    return;
+  }
+  constexpr float kLn2f = 0.69314718f;
+  const float a = x / xt;
+  const float rx = 1.0f / x, rx2 = rx * rx;
+  const float lx = __logf(x);
+  const float w = (1.0f - x) * (1.0f + x);
+  const float ic1 = (float)rc.ic1;
+  float F, h, I1, I2;
+  const bool small = a < (float)kTcASmall;
+  if (small) {
+    const float i0 = (float)rc.i0, jj = (float)rc.jj;
+    I1 = i0 + 0.5f * a * a * jj;
+    I2 = 0.5f * i0 + 0.125f * a * a * jj;
+  }
+  if (x < 1.0f) {
+    const float e = (1.0f - x) * rx, su = sqrtf(w), isu = 1.0f / su, iu = 1.0f / w;
+    const float A0 = log1pf(e + sqrtf(e * (e + 2.0f)));
+    F = (A0 * isu - 1.0f) * iu;
+    h = (x < (float)kNfwXH)
+            ? ((kLn2f - lx) * x * x / (1.0f + su) + log1pf(-x * x / (2.0f * (1.0f + su)))) * isu
+            : lx - kLn2f + A0 * isu;
+    if (!small) {
+      const float c = xt, q = sqrtf((c - x) * (c + x));
+      const float d = (x - 1.0f) * (x - c) * rx * ic1;
+      const float L = __logf((c + q) * ic1 * 0.5f) - (q - c) * ic1;
+      const float At = log1pf(d + sqrtf(d * (d + 2.0f)));
+      I1 = (q * ic1 - 1.0f) * iu + (A0 - At) * iu * isu;
+      I2 = (L + (A0 - At) * isu) * rx2;
+    }
+  } else {
+    const float v = -w, sv = sqrtf(v), isv = 1.0f / sv, iv = 1.0f / v;
+    const float B0 = 2.0f * asinf(sqrtf((x - 1.0f) * 0.5f * rx));
+    F = (1.0f - B0 * isv) * iv;
+    h = lx - kLn2f + B0 * isv;
+    if (!small) {
+      const float c = xt, q = sqrtf((c - x) * (c + x));
+      const float d = (x - 1.0f) * (x - c) * rx * ic1;
+      const float L = __logf((c + q) * ic1 * 0.5f) - (q - c) * ic1;
+      const float Bt = 2.0f * asinf(sqrtf(fmaxf(-d, 0.0f) * 0.5f));
+      I1 = (1.0f - q * ic1) * iv - (B0 - Bt) * iv * isv;
+      I2 = (L + (B0 - Bt) * isv) * rx2;
+    }
+  }
+  const float ku = 2.0f * F - 2.0f * I1;
+  const float kb = 4.0f * h * rx2 - 4.0f * I2;
+  const float ks = (float)rc.ks;
+  kap = ks * ku;
+  gam = ks * (kb - ku);
+}
+
 // Jump of one cell: xi = -ln detA and the tilted weight w e^{-xi} = w detA;
 // dropped (weight 0) when detA <= 0 or xi > 30.
 constexpr double kXiMax = 30.0;

@@ -189,10 +189,10 @@ class Solver:
         """Host copies of the table sub-arrays (same offsets as the library's
         TablesLayout: each sub-array starts on a 32-double boundary)."""
         sh, B = self.shape, self.B
-        sizes = [("cosmo", B * 16), ("derived", B * 16), ("slices", B * sh.n_z_max * 8),
+        sizes = [("m_nodes", sh.n_m), ("uedge", sh.n_b + 1), ("knodes", sh.n_k), ("status", 1),
+                 ("cosmo", B * 16), ("derived", B * 16), ("slices", B * sh.n_z_max * 8),
                  ("mass", B * sh.n_m * 4), ("rows", B * sh.n_z_max * sh.n_m * 8),
-                 ("cov", B * sh.n_z_max * sh.n_z_max), ("work", B * 20000), ("m_nodes", sh.n_m),
-                 ("uedge", sh.n_b + 1), ("knodes", sh.n_k), ("status", 1)]
+                 ("cov", B * sh.n_z_max * sh.n_z_max), ("work", B * 20000)]
         T = self.tables.view(torch.float64).cpu().numpy()
         out, o = {}, 0
         for name, n in sizes:

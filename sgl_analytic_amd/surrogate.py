@@ -1,0 +1,425 @@
+"""Learned surrogate of the exponent: (cosmology, z_s) -> ln phi at 512 log-k nodes.
+
+Network: 9 inputs (h, Om, sigma8, Ob, ns, Ok, w0, wa, ln z_s, standardised
+over the sampling box) -> 4 x 512 SiLU -> 1026 outputs: Re/Im ln phi at the
+512 nodes logspace(log10 k_lo, log10 k_hi, 512) of the solver's k range,
+interleaved, plus two gate scalars (the solver's grid mass and <mu> on the
+output grid).  BF16 weights and activations, FP32 accumulation and FP32
+master weights; every product of the forward and backward pass runs in
+libsgl_amd.so (sgl_mlp_gemm with fused bias/SiLU/SiLU' epilogues,
+sgl_mlp_colsum, sgl_to_bf16) behind a torch.autograd.Function.
+
+Inference: network -> ln phi at 512 nodes -> cubic interpolation onto the
+8192 solver nodes (sgl_lnphi_interp) -> window x exp -> the same MFMA Filon
+inversion as the solver -> FP64 diagnostics.  A PDF is accepted when its
+grid mass and <mu> agree with the network's own gate outputs within the
+tolerance; the rest fall back to the batched solver.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU, EPI_PLAIN, PREC_FP32
+from .solver import Shape, Solver, _ptr, _stream, cosmo_records, slice_count
+
+N_C = 512
+HIDDEN = (512, 512, 512, 512)
+N_IN = 9
+N_OUT = 2 * N_C + 2
+# sampling box of the batched configuration (h, Om, sigma8, Ob, ns, Ok, w0, wa, ln z_s)
+BOX_LO = np.array([0.6, 0.2, 0.7, 0.04, 0.92, -0.05, -1.2, -0.5, math.log(0.3)])
+BOX_HI = np.array([0.8, 0.4, 0.9, 0.06, 1.0, 0.05, -0.8, 0.5, math.log(10.0)])
+
+
+def lhs_params(n, seed):
+    """Latin-hypercube cosmologies over the box; columns PARAM order of solver."""
+    from scipy.stats import qmc
+    u = qmc.LatinHypercube(d=9, seed=seed).random(n)
+    v = BOX_LO + u * (BOX_HI - BOX_LO)
+    zs = np.exp(v[:, 8])
+    return np.column_stack([v[:, 0], v[:, 1], v[:, 2], v[:, 3], v[:, 4], np.full(n, 3402.0),
+                            v[:, 5], v[:, 6], v[:, 7], zs])
+
+
+def features(params: np.ndarray) -> np.ndarray:
+    """[B, 10] solver parameters -> [B, 9] standardised network inputs."""
+    p = np.asarray(params, dtype=np.float64)
+    x = np.column_stack([p[:, 0], p[:, 1], p[:, 2], p[:, 3], p[:, 4], p[:, 6], p[:, 7], p[:, 8],
+                         np.log(p[:, 9])])
+    mid, half = 0.5 * (BOX_HI + BOX_LO), 0.5 * (BOX_HI - BOX_LO)
+    return (x - mid) / half
+
+
+# ------------------------------------------------------------------ kernels --
+def _gemm(A, lda_m, lda_k, B, ldb_n, ldb_k, M, N, K, epi, bias=None, Z=None, ldz=0, Zout=None,
+          C=None, Cb=None, ldc=None, beta=0.0):
+    L = _lib.load()
+    p = lambda t: None if t is None else _ptr(t)
+    rc = L.sgl_mlp_gemm(_ptr(A), lda_m, lda_k, _ptr(B), ldb_n, ldb_k, M, N, K, epi, p(bias), p(Z),
+                        ldz, p(Zout), p(C), p(Cb), ldc if ldc is not None else N, beta, _stream())
+    _lib.check(rc, "sgl_mlp_gemm")
+
+
+def to_bf16(x: torch.Tensor) -> torch.Tensor:
+    x = x.contiguous()
+    y = torch.empty(x.shape, dtype=torch.int16, device=x.device)
+    _lib.check(_lib.load().sgl_to_bf16(_ptr(x), x.numel(), _ptr(y), _stream()), "sgl_to_bf16")
+    return y
+
+
+def colsum(dZ: torch.Tensor) -> torch.Tensor:
+    N, M = dZ.shape
+    db = torch.empty(M, dtype=torch.float32, device=dZ.device)
+    _lib.check(_lib.load().sgl_mlp_colsum(_ptr(dZ), N, M, _ptr(db), _stream()), "sgl_mlp_colsum")
+    return db
+
+
+class _MLPFunction(torch.autograd.Function):
+    """y = MLP(x) with the device GEMM kernels; params = (W0, b0, W1, b1, ...)."""
+
+    @staticmethod
+    def forward(ctx, x, *params):
+        Ws, bs = params[0::2], params[1::2]
+        n_layers = len(Ws)
+        N = x.shape[0]
+        h_b = to_bf16(x.float())
+        W_b = [to_bf16(W) for W in Ws]
+        acts_b, pre = [h_b], []
+        y = None
+        for i in range(n_layers):
+            M, K = Ws[i].shape
+            out = torch.empty((N, M), dtype=torch.float32, device=x.device)
+            if i < n_layers - 1:
+                z = torch.empty((N, M), dtype=torch.float32, device=x.device)
+                hb = torch.empty((N, M), dtype=torch.int16, device=x.device)
+                _gemm(acts_b[-1], K, 1, W_b[i], K, 1, N, M, K, EPI_BIAS_SILU, bias=bs[i], Zout=z,
+                      C=None, Cb=hb, ldc=M)
+                pre.append(z)
+                acts_b.append(hb)
+            else:
+                _gemm(acts_b[-1], K, 1, W_b[i], K, 1, N, M, K, EPI_BIAS, bias=bs[i], C=out, ldc=M)
+                y = out
+        ctx.save_for_backward(*acts_b, *pre, *W_b)
+        ctx.n_layers = n_layers
+        ctx.shapes = [tuple(W.shape) for W in Ws]
+        return y
+
+    @staticmethod
+    def backward(ctx, dy):
+        n = ctx.n_layers
+        saved = ctx.saved_tensors
+        acts_b = saved[:n]
+        pre = saved[n:2 * n - 1]
+        W_b = saved[2 * n - 1:]
+        dy = dy.contiguous().float()
+        N = dy.shape[0]
+        grads = [None] * (2 * n)
+        g, g_b = dy, to_bf16(dy)
+        for i in range(n - 1, -1, -1):
+            M, K = ctx.shapes[i]
+            dW = torch.empty((M, K), dtype=torch.float32, device=dy.device)
+            # dW[m][k] = sum_n g[n][m] X[n][k]
+            _gemm(g_b, 1, M, acts_b[i], 1, K, M, K, N, EPI_PLAIN, C=dW, ldc=K)
+            grads[2 * i] = dW
+            grads[2 * i + 1] = colsum(g)
+            if i > 0:
+                # dX[n][k] = sum_m g[n][m] W[m][k], then times silu'(z_{i-1})
+                gn = torch.empty((N, K), dtype=torch.float32, device=dy.device)
+                gnb = torch.empty((N, K), dtype=torch.int16, device=dy.device)
+                _gemm(g_b, M, 1, W_b[i], 1, K, N, K, M, EPI_DSILU, Z=pre[i - 1], ldz=K, C=gn,
+                      Cb=gnb, ldc=K)
+                g, g_b = gn, gnb
+        return (None, *grads)
+
+
+class SurrogateNet(torch.nn.Module):
+    def __init__(self, dims=(N_IN, *HIDDEN, N_OUT), seed=0):
+        super().__init__()
+        gen = torch.Generator().manual_seed(seed)
+        self.Ws = torch.nn.ParameterList()
+        self.bs = torch.nn.ParameterList()
+        for a, b in zip(dims[:-1], dims[1:]):
+            W = torch.randn(b, a, generator=gen) * math.sqrt(2.0 / a)
+            self.Ws.append(torch.nn.Parameter(W))
+            self.bs.append(torch.nn.Parameter(torch.zeros(b)))
+        self.register_buffer("y_mean", torch.zeros(N_OUT))
+        self.register_buffer("y_std", torch.ones(N_OUT))
+
+    def forward(self, x):
+        flat = []
+        for W, b in zip(self.Ws, self.bs):
+            flat += [W, b]
+        return _MLPFunction.apply(x, *flat)
+
+    def predict(self, x):
+        """Exponent values [B, N_OUT] (FP32): Re/Im ln phi interleaved, gate scalars."""
+        with torch.no_grad():
+            return from_targets(self(x) * self.y_std + self.y_mean)
+
+
+# ------------------------------------------------------- output transform --
+# Network targets per node: y_re = ln(-Re ln phi) (the decay, multiplicative
+# across cosmologies), y_im = Im ln phi / (1 + |Re ln phi|) (phase relative to
+# the decay); gate scalars as they are.
+def to_targets(Y: torch.Tensor) -> torch.Tensor:
+    T = Y.clone()
+    re = Y[:, 0:2 * N_C:2]
+    T[:, 0:2 * N_C:2] = torch.log(torch.clamp(-re, min=1e-30))
+    T[:, 1:2 * N_C:2] = Y[:, 1:2 * N_C:2] / (1.0 + re.abs())
+    return T
+
+
+def from_targets(T: torch.Tensor) -> torch.Tensor:
+    re = -torch.exp(T[:, 0:2 * N_C:2])
+    Y = T.clone()
+    Y[:, 0:2 * N_C:2] = re
+    Y[:, 1:2 * N_C:2] = T[:, 1:2 * N_C:2] * (1.0 + re.abs())
+    return Y
+
+
+# --------------------------------------------------------------------- data --
+def coarse_shape(full: Shape) -> Shape:
+    from dataclasses import replace
+    return replace(full, n_k=N_C)
+
+
+def generate(params: np.ndarray, full: Shape, chunk=1024, with_pdf=False, log=print):
+    """Solver outputs for the given cosmologies: targets [B, N_OUT] (ln phi on the
+    512-node grid from the batched solver on that grid, then the full solver's
+    grid mass and <mu>), optionally the full solver's p [B, n_out]."""
+    B = params.shape[0]
+    n_z = slice_count(params[:, 9])
+    sh = Shape(**{**full.__dict__, "n_z_max": int(n_z.max())})
+    csh = coarse_shape(sh)
+    s_full = Solver(sh, chunk, path="binned", precision=PREC_FP32)
+    s_c = Solver(csh, chunk, path="binned", precision=PREC_FP32)
+    Y = np.zeros((B, N_OUT), dtype=np.float32)
+    P = np.zeros((B, sh.n_out), dtype=np.float32) if with_pdf else None
+    t0 = time.time()
+    for a in range(0, B, chunk):
+        b = min(B, a + chunk)
+        rec = cosmo_records(params[a:b], n_z[a:b])
+        if b - a < chunk:
+            rec = np.concatenate([rec, np.repeat(rec[-1:], chunk - (b - a), 0)])
+        s_c.run(rec)
+        lp = (s_c.lam + s_c.psi2) if csh.clustering else s_c.lam
+        p, d = s_full.run(rec)
+        torch.cuda.synchronize()
+        Y[a:b, :2 * N_C] = lp[:b - a].reshape(b - a, 2 * N_C).float().cpu().numpy()
+        dd = d[:b - a].cpu().numpy()
+        Y[a:b, 2 * N_C] = dd[:, 0]
+        Y[a:b, 2 * N_C + 1] = dd[:, 5]
+        if with_pdf:
+            P[a:b] = p[:b - a].cpu().numpy()
+        if log and (a // chunk) % 20 == 0:
+            log(f"[surrogate] generated {b}/{B} in {time.time() - t0:.1f} s")
+    return Y, P
+
+
+def loss_weights(Y: torch.Tensor, k_c: torch.Tensor) -> torch.Tensor:
+    """Per-sample weights of the ln phi outputs: |phi(k)| k (the integrand scale
+    of the inversion on a log grid), normalised per sample."""
+    re = Y[:, 0:2 * N_C:2]
+    w = torch.exp(torch.clamp(re, min=-60.0)) * k_c[None, :]
+    return w / w.sum(dim=1, keepdim=True)
+
+
+def train(net: SurrogateNet, X: np.ndarray, Y: np.ndarray, full: Shape, epochs=60, batch=1024,
+          lr=2e-3, log=print, seed=0):
+    dev = torch.device("cuda:0")
+    net.to(dev)
+    Xt = torch.as_tensor(X, dtype=torch.float32, device=dev)
+    Yt = torch.as_tensor(Y, dtype=torch.float32, device=dev)
+    Tt = to_targets(Yt)
+    net.y_mean.copy_(Tt.mean(0))
+    net.y_std.copy_(Tt.std(0).clamp_min(1e-6))
+    k_c = torch.as_tensor(np.logspace(math.log10(full.k_lo), math.log10(full.k_hi), N_C),
+                          dtype=torch.float32, device=dev)
+    Wt = loss_weights(Yt, k_c)
+    opt = torch.optim.Adam(net.parameters(), lr=lr)
+    n = Xt.shape[0]
+    steps = epochs * (n // batch)
+    sched = torch.optim.lr_scheduler.OneCycleLR(opt, max_lr=lr, total_steps=steps, pct_start=0.05)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    t0 = time.time()
+    step = 0
+    for ep in range(epochs):
+        perm = torch.randperm(n, device=dev, generator=g)
+        tot = 0.0
+        for i in range(n // batch):
+            idx = perm[i * batch:(i + 1) * batch]
+            out = net(Xt[idx])
+            ys = (Tt[idx] - net.y_mean) / net.y_std
+            err = out - ys
+            raw = from_targets(out * net.y_std + net.y_mean) - Yt[idx]
+            w = Wt[idx]
+            l_phi = ((raw[:, 0:2 * N_C:2] ** 2 + raw[:, 1:2 * N_C:2] ** 2) * w).sum(1).mean()
+            l_gate = (raw[:, 2 * N_C:] ** 2).sum(1).mean()
+            l_std = (err ** 2).mean()
+            loss = l_phi + 10.0 * l_gate + 1e-3 * l_std
+            opt.zero_grad(set_to_none=True)
+            loss.backward()
+            opt.step()
+            sched.step()
+            tot += float(loss.detach())
+            step += 1
+        if log and (ep % 5 == 0 or ep == epochs - 1):
+            log(f"[surrogate] epoch {ep} loss {tot / (n // batch):.4e} ({time.time() - t0:.1f} s)")
+    return net
+
+
+@dataclass
+class Inference:
+    """Preallocated surrogate inference on B instances: network -> phi ->
+    MFMA inversion -> diagnostics -> gates."""
+    net: SurrogateNet
+    shape: Shape
+    B: int
+    tol: float = 1e-3
+
+    def __post_init__(self):
+        from dataclasses import replace
+        # psi2 is inside the prediction; one slice: only the k grid of the
+        # tables is read (its offset does not depend on the batch size)
+        self.sh = replace(self.shape, clustering=False, n_z_max=1)
+        self.L = _lib.load()
+        L, dev = self.L, torch.device("cuda:0")
+        self.cs = self.sh.c()
+        cs = ctypes.byref(self.cs)
+        rec = torch.as_tensor(cosmo_records(lhs_params(1, 1), 1), device=dev)
+        self.tables = torch.zeros(L.sgl_tables_bytes(cs, 1), dtype=torch.uint8, device=dev)
+        self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+        _lib.check(L.sgl_tables_build(_ptr(rec), 1, cs, _ptr(self.tables), _ptr(self.status),
+                                      _stream()), "sgl_tables_build")
+        self.E = torch.zeros(L.sgl_filon_bytes(cs, PREC_FP32), dtype=torch.uint8, device=dev)
+        _lib.check(L.sgl_filon_build(_ptr(self.tables), 1, cs, PREC_FP32, _ptr(self.E),
+                                     _stream()), "sgl_filon_build")
+        self.lam = torch.zeros((self.B, self.sh.n_k, 2), dtype=torch.float64, device=dev)
+        self.phi = torch.zeros(L.sgl_phi_bytes(cs, self.B, PREC_FP32), dtype=torch.uint8,
+                               device=dev)
+        self.p = torch.zeros((self.B, self.sh.n_out), dtype=torch.float32, device=dev)
+        self.diag = torch.zeros((self.B, 8), dtype=torch.float64, device=dev)
+
+    def run(self, x: torch.Tensor):
+        """x [B, 9] standardised features on the device -> (p, diag, accept, pred)."""
+        L, cs, st = self.L, ctypes.byref(self.cs), _stream()
+        y = self.net.predict(x).contiguous()
+        _lib.check(L.sgl_lnphi_interp(_ptr(y), N_OUT, N_C, self.B, cs, _ptr(self.lam), st),
+                   "sgl_lnphi_interp")
+        _lib.check(L.sgl_phi_pack(_ptr(self.lam), None, _ptr(self.tables), self.B, cs, PREC_FP32,
+                                  _ptr(self.phi), st), "sgl_phi_pack")
+        _lib.check(L.sgl_invert_gemm(_ptr(self.E), _ptr(self.phi), self.B, cs, PREC_FP32,
+                                     _ptr(self.p), st), "sgl_invert_gemm")
+        _lib.check(L.sgl_diagnostics(_ptr(self.p), 0, self.B, cs, _ptr(self.diag), st),
+                   "sgl_diagnostics")
+        gm, gmu = y[:, 2 * N_C].double(), y[:, 2 * N_C + 1].double()
+        accept = (((self.diag[:, 0] - gm).abs() <= self.tol)
+                  & ((self.diag[:, 5] - gmu).abs() <= self.tol))
+        return self.p, self.diag, accept, y
+
+
+# -------------------------------------------------------------- evaluation --
+def evaluate(net: SurrogateNet, params: np.ndarray, P_ref: np.ndarray, Y_ref: np.ndarray,
+             full: Shape, batch=4096, tol=1e-3):
+    """Surrogate PDFs for held-out cosmologies vs the solver's: per-instance L1
+    and the gate decision; also the floor from interpolating the solver's own
+    512-node ln phi (what a perfect network would reach)."""
+    dev = torch.device("cuda:0")
+    n = params.shape[0]
+    x_all = torch.as_tensor(features(params), dtype=torch.float32, device=dev)
+    xo = full.out_grid()
+    out = dict(L1=np.zeros(n), accept=np.zeros(n, bool), L1_floor=np.zeros(n))
+    inf = Inference(net, full, batch, tol)
+    for a in range(0, n, batch):
+        b = min(n, a + batch)
+        x = x_all[a:b]
+        if b - a < batch:
+            x = torch.cat([x, x[-1:].expand(batch - (b - a), -1)])
+        p, d, acc, y = inf.run(x)
+        torch.cuda.synchronize()
+        pp = p[:b - a].cpu().numpy().astype(np.float64)
+        out["L1"][a:b] = np.trapezoid(np.abs(pp - P_ref[a:b]), xo, axis=1)
+        out["accept"][a:b] = acc[:b - a].cpu().numpy()
+        # floor: the solver's own 512-node exponent through the same interpolation
+        yt = torch.as_tensor(Y_ref[a:b], dtype=torch.float32, device=dev)
+        if b - a < batch:
+            yt = torch.cat([yt, yt[-1:].expand(batch - (b - a), -1)])
+        L, cs, st = inf.L, ctypes.byref(inf.cs), _stream()
+        _lib.check(L.sgl_lnphi_interp(_ptr(yt.contiguous()), N_OUT, N_C, batch, cs, _ptr(inf.lam),
+                                      st), "sgl_lnphi_interp")
+        _lib.check(L.sgl_phi_pack(_ptr(inf.lam), None, _ptr(inf.tables), batch, cs, PREC_FP32,
+                                  _ptr(inf.phi), st), "sgl_phi_pack")
+        _lib.check(L.sgl_invert_gemm(_ptr(inf.E), _ptr(inf.phi), batch, cs, PREC_FP32,
+                                     _ptr(inf.p), st), "sgl_invert_gemm")
+        torch.cuda.synchronize()
+        pf = inf.p[:b - a].cpu().numpy().astype(np.float64)
+        out["L1_floor"][a:b] = np.trapezoid(np.abs(pf - P_ref[a:b]), xo, axis=1)
+    return out
+
+
+def save(net: SurrogateNet, path: str, meta: dict):
+    torch.save({"state": {k: v.detach().cpu() for k, v in net.state_dict().items()},
+                "meta": meta}, path)
+
+
+def load(path: str) -> SurrogateNet:
+    blob = torch.load(path, map_location="cpu", weights_only=True)
+    net = SurrogateNet()
+    net.load_state_dict(blob["state"])
+    return net.to(torch.device("cuda:0"))
+
+
+def main(argv=None):
+    """Generate solver data, train on the device kernels, evaluate, save."""
+    import argparse
+    import json
+    import os
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n-train", type=int, default=200_000)
+    ap.add_argument("--n-hold", type=int, default=20_000)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch", type=int, default=1024)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                  "data", "surrogate.pt"))
+    ap.add_argument("--report", default=None)
+    a = ap.parse_args(argv)
+    log = lambda *m: print(*m, flush=True)
+    full = Shape()
+    t0 = time.time()
+    ptr = lhs_params(a.n_train, seed=11)
+    phd = lhs_params(a.n_hold, seed=12)
+    Ytr, _ = generate(ptr, full, log=log)
+    Yho, Pho = generate(phd, full, with_pdf=True, log=log)
+    t_gen = time.time() - t0
+    log(f"[surrogate] data: {a.n_train} + {a.n_hold} solves in {t_gen:.1f} s")
+    net = SurrogateNet()
+    t1 = time.time()
+    train(net, features(ptr), Ytr, full, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+    t_train = time.time() - t1
+    ev = evaluate(net, phd, Pho.astype(np.float64), Yho, full)
+    acc = ev["accept"]
+    rep = dict(n_train=a.n_train, n_hold=a.n_hold, epochs=a.epochs, gen_s=t_gen, train_s=t_train,
+               L1_median=float(np.median(ev["L1"])), L1_p90=float(np.quantile(ev["L1"], 0.9)),
+               L1_max=float(ev["L1"].max()), accept_rate=float(acc.mean()),
+               accepted_L1_max=float(ev["L1"][acc].max()) if acc.any() else None,
+               accepted_within_1e3=float((ev["L1"][acc] <= 1e-3).mean()) if acc.any() else None,
+               floor_L1_median=float(np.median(ev["L1_floor"])),
+               floor_L1_max=float(ev["L1_floor"].max()))
+    log("[surrogate] report", json.dumps(rep))
+    os.makedirs(os.path.dirname(a.out), exist_ok=True)
+    save(net, a.out, rep)
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(rep, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
