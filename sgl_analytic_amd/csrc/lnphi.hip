@@ -12,10 +12,12 @@
 // cancellation-free forms cos - 1 = -2 sin^2(ph/2), sin - ph by series below
 // |ph| = 1e-2 (sgl_full.py:2599-2613).
 // Binned path: k_bin_cells histograms each slice's cells on a signed-log jump
-// lattice in LDS (w, w xi and beta w per fine bin), writes the slice's
-// fine moments as a partial and its coarse two-point bins as compacted
-// records; k_reduce_fine adds the partials in slice order (no atomics across
-// workgroups: reproducible) and compacts the non-empty bins; k_lnphi_binned
+// lattice in LDS (w, w xi per fine bin; beta w, beta w xi, beta w xi^2 per
+// coarse bin of 32 fine ones) with the FP32 lens profile, k_bin_band adds the
+// cells of the band |1 - x^2| < 1/4 with the FP64 profile, both into the
+// slice's own partials; k_reduce_fine adds the partials in slice order (no
+// atomics across workgroups: reproducible) and compacts the non-empty bins
+// of the instance and of every slice; k_lnphi_binned
 // evaluates each bin as w e^{i k xibar - k^2 v / 2}, v the in-bin variance,
 // phase in FP64 reduced to revolutions, sin/cos on the FP32 hardware path,
 // compensated small-phase terms and accumulation in FP64.
@@ -197,76 +199,109 @@ __device__ __forceinline__ int fine_bin(double xi) {
   return kFinePos + min(max(k, 0), kFineNeg - 1);
 }
 
-constexpr int kBinThreads = 1024;
+constexpr int kBinThreads = 512;
 
+// Cells whose profile is evaluated in FP64 on the batched path: the band
+// |1 - x^2| < 1/4 around the NFW branch point.  Row j covers it for columns
+// l with ug_l s_j inside the band; everything else takes the FP32 profile.
+__device__ __forceinline__ bool in_band(double x) { return fabs((1.0 - x) * (1.0 + x)) < kNfwWSeries; }
+
+__device__ __forceinline__ void deposit(double* lf, double* lc, double xi, double ws, double bw) {
+  const int f = fine_bin(xi);
+  atomicAdd(&lf[f * kMoments], ws);
+  atomicAdd(&lf[f * kMoments + 1], ws * xi);
+  const int cb = f / kCoarse;
+  atomicAdd(&lc[cb * 3], bw);
+  atomicAdd(&lc[cb * 3 + 1], bw * xi);
+  atomicAdd(&lc[cb * 3 + 2], bw * xi * xi);
+}
+
+// Main pass: every cell outside the band, FP32 lens profile, FP32 log1p for
+// the jump; the slice's fine and coarse moments are written as partials.
 __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restrict__ rows,
                                                            const double* __restrict__ cosmo,
                                                            const double* __restrict__ uedge,
                                                            sgl_shape sh, double* __restrict__ hist) {
   __shared__ SliceStage S;
   __shared__ double lf[kFineBins * kMoments];
-  __shared__ double lb[kFineBins];
   __shared__ double lc[kSliceBins * 3];
-  __shared__ int cnt;
   const int t = threadIdx.x, i = blockIdx.x, b = blockIdx.y;
   const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
   const HistLayout HL = hist_layout(sh, gridDim.y);
   for (int q = t; q < kFineBins * kMoments; q += kBinThreads) lf[q] = 0.0;
-  for (int q = t; q < kFineBins; q += kBinThreads) lb[q] = 0.0;
   for (int q = t; q < kSliceBins * 3; q += kBinThreads) lc[q] = 0.0;
-  if (t == 0) cnt = 0;
   if (i < n_z) stage_slice(S, rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh);
   __syncthreads();
   if (i < n_z) {
     const int n_cells = sh.n_m * sh.n_b;
     for (int c = t; c < n_cells; c += kBinThreads) {
-      double xi, ws, bw;
-      gen_cell<true>(S, sh.n_b, c, xi, ws, bw);
+      const int j = c / sh.n_b, l = c - j * sh.n_b;
+      const RowConst& rc = S.rc[j];
+      if (rc.valid == 0.0) continue;
+      const double x = S.ug[l] * rc.s;
+      if (in_band(x)) continue;                 // the FP64 band pass takes it
+      float kf, gf;
+      trunc_kg_row_f32((float)x, rc, kf, gf);
+      double xi, ws;
+      cell_jump_f32log(kf, gf, rc.warea * S.du2[l], xi, ws);
       if (ws == 0.0) continue;
-      const int f = fine_bin(xi);
-      atomicAdd(&lf[f * kMoments], ws);
-      atomicAdd(&lf[f * kMoments + 1], ws * xi);
-      atomicAdd(&lb[f], bw);
+      deposit(lf, lc, xi, ws, rc.beta * ws);
     }
   }
   __syncthreads();
-  // coarse two-point bins from the fine ones: beta w placed at the fine
-  // centroid; the spread of fine centroids gives the coarse variance
-  for (int f = t; f < kFineBins; f += kBinThreads) {
-    const double w = lf[f * kMoments];
-    if (w == 0.0) continue;
-    const double m = lf[f * kMoments + 1] / w;
-    const double bwv = lb[f];
-    const int cb = f / kCoarse;
-    atomicAdd(&lc[cb * 3], bwv);
-    atomicAdd(&lc[cb * 3 + 1], bwv * m);
-    atomicAdd(&lc[cb * 3 + 2], bwv * m * m);
-  }
-  // this slice's fine partial (fixed position: added in slice order later)
   double* P = hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments;
   for (int q = t; q < kFineBins * kMoments; q += kBinThreads) P[q] = lf[q];
-  __syncthreads();
-  // compacted coarse records of this slice (order of non-empty bins kept by
-  // a serial pass: 144 bins)
-  double* R = hist + HL.srec + ((size_t)b * sh.n_z_max + i) * kSliceBins * kRec;
-  if (t == 0) {
-    int n = 0;
-    for (int cb = 0; cb < kSliceBins; ++cb) {
-      const double w = lc[cb * 3];
-      if (w == 0.0) continue;
-      const double m = lc[cb * 3 + 1] / w;
-      const double v = lc[cb * 3 + 2] / w - m * m;
-      R[n * kRec + 0] = m * (0.25 / kPi);
-      R[n * kRec + 1] = m;
-      R[n * kRec + 2] = w;
-      R[n * kRec + 3] = (v > 0.0) ? v : 0.0;
-      ++n;
-    }
-    hist[HL.scnt + (size_t)b * sh.n_z_max + i] = (double)n;
-  }
+  double* Q = hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3;
+  for (int q = t; q < kSliceBins * 3; q += kBinThreads) Q[q] = lc[q];
 }
 
-// Sum the slice partials of one instance in slice order, compact non-empty bins.
+// Band pass: the cells with |1 - x^2| < 1/4 in FP64 (series branch of F, h
+// and the |x - 1| < 1e-5 bridge of the truncation term), added into the
+// slice's partials by the only workgroup that touches them.
+constexpr int kBandThreads = 256;
+__global__ __launch_bounds__(kBandThreads) void k_bin_band(const double* __restrict__ rows,
+                                                           const double* __restrict__ cosmo,
+                                                           const double* __restrict__ uedge,
+                                                           sgl_shape sh, double* __restrict__ hist) {
+  __shared__ SliceStage S;
+  __shared__ double lf[kFineBins * kMoments];
+  __shared__ double lc[kSliceBins * 3];
+  const int t = threadIdx.x, i = blockIdx.x, b = blockIdx.y;
+  const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
+  if (i >= n_z) return;
+  const HistLayout HL = hist_layout(sh, gridDim.y);
+  for (int q = t; q < kFineBins * kMoments; q += kBandThreads) lf[q] = 0.0;
+  for (int q = t; q < kSliceBins * 3; q += kBandThreads) lc[q] = 0.0;
+  stage_slice(S, rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh);
+  __syncthreads();
+  // band columns of row j: ug_l in [sqrt(0.75)/s, sqrt(1.25)/s]; ug is increasing
+  for (int j = t; j < sh.n_m; j += kBandThreads) {
+    const RowConst& rc = S.rc[j];
+    if (rc.valid == 0.0) continue;
+    const double lo = sqrt(0.75) / rc.s, hi = sqrt(1.25) / rc.s;
+    int a = 0, z = sh.n_b;                    // first l with ug_l >= lo
+    while (a < z) { int m = (a + z) >> 1; if (S.ug[m] < lo) a = m + 1; else z = m; }
+    for (int l = (a > 0 ? a - 1 : 0); l < sh.n_b && S.ug[l] <= hi * 1.0000001; ++l) {
+      const double x = S.ug[l] * rc.s;
+      if (!in_band(x)) continue;
+      double kap, gam, xi, ws;
+      trunc_kg_row(x, rc, kap, gam);
+      cell_jump_f32log(kap, gam, rc.warea * S.du2[l], xi, ws);
+      if (ws == 0.0) continue;
+      deposit(lf, lc, xi, ws, rc.beta * ws);
+    }
+  }
+  __syncthreads();
+  double* P = hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments;
+  for (int q = t; q < kFineBins * kMoments; q += kBandThreads)
+    if (lf[q] != 0.0) P[q] += lf[q];
+  double* Q = hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3;
+  for (int q = t; q < kSliceBins * 3; q += kBandThreads)
+    if (lc[q] != 0.0) Q[q] += lc[q];
+}
+
+// Per instance: add the slice partials in slice order and compact the
+// non-empty fine bins; compact every slice's coarse bins.
 __global__ __launch_bounds__(256) void k_reduce_fine(const double* __restrict__ cosmo,
                                                      sgl_shape sh, double* __restrict__ hist) {
   __shared__ int flag[kFineBins];
@@ -300,6 +335,26 @@ __global__ __launch_bounds__(256) void k_reduce_fine(const double* __restrict__ 
     r[1] = m;
     r[2] = w;
     r[3] = 0.0;
+  }
+  // coarse records, one slice per thread (72 bins each, order kept)
+  for (int i = t; i < sh.n_z_max; i += 256) {
+    const double* Q = hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3;
+    double* Rs = hist + HL.srec + ((size_t)b * sh.n_z_max + i) * kSliceBins * kRec;
+    int n = 0;
+    if (i < n_z) {
+      for (int cb = 0; cb < kSliceBins; ++cb) {
+        const double w = Q[cb * 3];
+        if (w == 0.0) continue;
+        const double m = Q[cb * 3 + 1] / w;
+        const double v = Q[cb * 3 + 2] / w - m * m;
+        Rs[n * kRec + 0] = m * (0.25 / kPi);
+        Rs[n * kRec + 1] = m;
+        Rs[n * kRec + 2] = w;
+        Rs[n * kRec + 3] = (v > 0.0) ? v : 0.0;
+        ++n;
+      }
+    }
+    hist[HL.scnt + (size_t)b * sh.n_z_max + i] = (double)n;
   }
 }
 
@@ -413,6 +468,8 @@ int launch_lnphi_exact(const double* rows, const double* cosmo, const double* ue
 int launch_bin(const double* rows, const double* cosmo, const double* uedge, const sgl_shape& sh,
                int B, double* hist, hipStream_t st) {
   hipLaunchKernelGGL(k_bin_cells, dim3(sh.n_z_max, B), dim3(kBinThreads), 0, st, rows, cosmo, uedge,
+                     sh, hist);
+  hipLaunchKernelGGL(k_bin_band, dim3(sh.n_z_max, B), dim3(kBandThreads), 0, st, rows, cosmo, uedge,
                      sh, hist);
   hipLaunchKernelGGL(k_reduce_fine, dim3(B), dim3(256), 0, st, cosmo, sh, hist);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;

@@ -72,13 +72,14 @@ constexpr int kMaxB = 1024;
 // Histogram buffer (doubles): per-slice partial fine moments, compacted fine
 // records + count per instance, compacted coarse records + count per slice.
 struct HistLayout {
-  size_t partial, frec, fcnt, srec, scnt, total;
+  size_t partial, cpart, frec, fcnt, srec, scnt, total;
 };
 __host__ __device__ inline size_t round32(size_t n) { return (n + 31) & ~size_t(31); }
 __host__ __device__ inline HistLayout hist_layout(const sgl_shape& s, int B) {
   HistLayout L;
   L.partial = 0;
-  L.frec = L.partial + round32((size_t)B * s.n_z_max * kFineBins * kMoments);
+  L.cpart = L.partial + round32((size_t)B * s.n_z_max * kFineBins * kMoments);
+  L.frec = L.cpart + round32((size_t)B * s.n_z_max * kSliceBins * 3);
   L.fcnt = L.frec + round32((size_t)B * kFineBins * kRec);
   L.srec = L.fcnt + round32((size_t)B);
   L.scnt = L.srec + round32((size_t)B * s.n_z_max * kSliceBins * kRec);
