@@ -62,6 +62,10 @@ def main():
     ap.add_argument("--verify", type=int, default=8, help="batch members checked vs FP64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--precision-study", type=int, default=0, metavar="N",
+                    help="instead of the bench line: FP32 vs split-BF16x3 inversion on the "
+                         "same batch, N members each checked against the FP64 plain path")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the batch dimension "
@@ -75,6 +79,9 @@ def main():
 
     prec = PREC_FP32 if args.precision == "fp32" else PREC_BF16X3
     B = args.batch
+    if args.precision_study:
+        precision_study(args, B)
+        return
     params = lhs_params(B)
     n_z = S.slice_count(params[:, 9])
     shape = S.Shape(n_z_max=int(n_z.max()))
@@ -85,6 +92,12 @@ def main():
     log(f"[bench] batched B={B} n_z_max={shape.n_z_max} precision={args.precision}")
     sv = S.Solver(shape, B, path="binned", precision=prec)
     sv.set_records(rec)
+    tuning = None
+    if not args.no_tune:
+        from sgl_analytic_amd import autotune
+        tuning = autotune.tune(sv)
+        log("[bench] tuned launch shapes", tuning["choice"],
+            {k: {s_: round(v, 2) for s_, v in m.items()} for k, m in tuning["ms"].items()})
     stage = stage_times(sv, S)          # eager pass: per-stage event times (also builds E)
     log("[bench] stage ms", {k: round(v, 3) for k, v in stage.items()})
     if not args.no_graph:
@@ -174,10 +187,54 @@ def main():
                    "max_abs_dtail": max(abs(e["dtail"]) for e in errs) if errs else None,
                    "against": "plain FP64 GPU path (itself <= 1e-10 L1 vs the FP64 C oracle)"},
         "stage_ms": {k: round(v, 3) for k, v in stage.items()},
+        "tuning": tuning["choice"] if tuning else None,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def precision_study(args, B):
+    """Both inversion precisions on one batch; every checked member is compared
+    with the FP64 plain path (L1, max |dp|, d<mu>, d mass, d P(mu>2))."""
+    import torch
+    from sgl_analytic_amd import solver as S
+    from sgl_analytic_amd._lib import PREC_BF16X3, PREC_FP32
+    params = lhs_params(B)
+    n_z = S.slice_count(params[:, 9])
+    shape = S.Shape(n_z_max=int(n_z.max()))
+    rec = S.cosmo_records(params, n_z)
+    x = shape.out_grid()
+    rng = np.random.default_rng(SEED + 1)
+    idx = np.sort(rng.choice(B, size=min(args.precision_study, B), replace=False))
+    out = {"batch": B, "members": len(idx)}
+    runs = {}
+    for name, pr in (("fp32", PREC_FP32), ("bf16x3", PREC_BF16X3)):
+        sv = S.Solver(shape, B, path="binned", precision=pr)
+        p, d = sv.run(rec)
+        torch.cuda.synchronize()
+        runs[name] = (p.float().cpu().numpy().astype(np.float64), d.cpu().numpy())
+        del sv
+        torch.cuda.empty_cache()
+    pl = S.Solver(shape, 1, path="exact")
+    rows = {k: [] for k in runs}
+    for n, i in enumerate(idx):
+        p1, d1 = pl.run(rec[i:i + 1])
+        torch.cuda.synchronize()
+        pe, de = p1[0].cpu().numpy(), d1[0].cpu().numpy()
+        for k, (pb, db) in runs.items():
+            rows[k].append((trapz_l1(x, pb[i], pe), float(np.max(np.abs(pb[i] - pe))),
+                            abs(db[i, 5] - de[5]), abs(db[i, 0] - de[0]), abs(db[i, 6] - de[6])))
+        if n % 16 == 0:
+            print(f"[study] {n + 1}/{len(idx)} checked", file=sys.stderr, flush=True)
+    for k, r in rows.items():
+        a = np.array(r)
+        out[k] = {"L1_median": float(np.median(a[:, 0])), "L1_max": float(a[:, 0].max()),
+                  "maxabs_max": float(a[:, 1].max()), "dmu_max": float(a[:, 2].max()),
+                  "dmass_max": float(a[:, 3].max()), "dtail_max": float(a[:, 4].max())}
+    pf, pb = runs["fp32"][0], runs["bf16x3"][0]
+    out["fp32_vs_bf16x3_L1_max"] = float(max(trapz_l1(x, pf[i], pb[i]) for i in range(B)))
+    print(json.dumps({"precision_study": out}), flush=True)
 
 
 def stage_times(sv, S):

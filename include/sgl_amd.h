@@ -38,6 +38,14 @@ typedef enum sgl_status {
 #define SGL_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation */
 #define SGL_PREC_BF16X3 2 /* split-BF16 (hi+lo) x 3 products on MFMA, FP32 accumulation */
 
+/* Launch-shape knobs (sgl_set_tuning): chosen per device/shape by the
+ * project's autotuner; every value gives identical results up to the
+ * floating-point order of the histogram adds. */
+#define SGL_TUNE_BIN_THREADS 0   /* binning block: 0 = 256, 1 = 512, 2 = 1024 threads */
+#define SGL_TUNE_LNPHI_BLOCK 1   /* binned exponent block: 0 = 128, 1 = 256 threads */
+#define SGL_TUNE_GEMM_BM 2       /* inversion GEMM tile rows: 0 = 64, 1 = 128 */
+#define SGL_TUNE_COUNT 3
+
 /* Grids of one solve (host struct, passed by pointer). */
 typedef struct sgl_shape {
   int32_t n_z_max; /* slices per instance (max over the batch), <= 96 */
@@ -63,6 +71,12 @@ size_t sgl_phi_bytes(const sgl_shape* shape, int32_t B, int32_t precision); /* o
 
 /* Library version string (host). */
 const char* sgl_version(void);
+
+/* Set / read a launch-shape knob (host-side, process-wide; not for
+ * concurrent use from several threads).  No reference counterpart: the
+ * reference has no kernels to tune. */
+sgl_status sgl_set_tuning(int32_t knob, int32_t value);
+int32_t sgl_get_tuning(int32_t knob);
 
 /*
  * Lens-population tables: cosmology (E, chi, D_A, Sigma_cr, growth ODE,

@@ -21,9 +21,10 @@ EXPORTS = (
     "sgl_tables_build", "sgl_lnphi_exact", "sgl_bin_cells", "sgl_lnphi_binned",
     "sgl_quadform", "sgl_phi_pack", "sgl_filon_build", "sgl_invert_gemm", "sgl_invert_fp64",
     "sgl_diagnostics", "sgl_invert_fp64_at", "sgl_mlp_gemm", "sgl_mlp_colsum", "sgl_to_bf16",
-    "sgl_lnphi_interp",
+    "sgl_lnphi_interp", "sgl_set_tuning", "sgl_get_tuning",
 )
 EPI_PLAIN, EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU = 0, 1, 2, 3
+TUNE_BIN_THREADS, TUNE_LNPHI_BLOCK, TUNE_GEMM_BM = 0, 1, 2
 
 
 class CShape(ctypes.Structure):
@@ -61,6 +62,8 @@ def load(path: str = LIB_PATH):
         f.argtypes = [sp, i32] + ([i32] if name == "sgl_phi_bytes" else [])
     L.sgl_filon_bytes.restype = sz
     L.sgl_filon_bytes.argtypes = [sp, i32]
+    L.sgl_get_tuning.restype = i32
+    L.sgl_get_tuning.argtypes = [i32]
     sigs = {
         "sgl_tables_build": [vp, i32, sp, vp, vp, vp],
         "sgl_lnphi_exact": [vp, i32, sp, vp, vp, vp, vp],
@@ -79,6 +82,7 @@ def load(path: str = LIB_PATH):
         "sgl_mlp_colsum": [vp, i32, i32, vp, vp],
         "sgl_to_bf16": [vp, sz, vp, vp],
         "sgl_lnphi_interp": [vp, ctypes.c_long, i32, i32, sp, vp, vp],
+        "sgl_set_tuning": [i32, i32],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -8,6 +8,10 @@
 
 using namespace sgl;
 
+namespace sgl {
+int g_sgl_tune[SGL_TUNE_COUNT] = {2, 1, 0};
+}
+
 namespace {
 
 bool shape_ok(const sgl_shape* s) {
@@ -40,6 +44,18 @@ __global__ void k_zero_int(int* p) { *p = 0; }
 extern "C" {
 
 const char* sgl_version(void) { return "sgl_amd 0.1 (gfx950)"; }
+
+sgl_status sgl_set_tuning(int32_t knob, int32_t value) {
+  static const int kMax[SGL_TUNE_COUNT] = {2, 1, 1};
+  if (knob < 0 || knob >= SGL_TUNE_COUNT || value < 0 || value > kMax[knob]) return SGL_ERR_INVALID;
+  g_sgl_tune[knob] = value;
+  return SGL_OK;
+}
+
+int32_t sgl_get_tuning(int32_t knob) {
+  if (knob < 0 || knob >= SGL_TUNE_COUNT) return -1;
+  return g_sgl_tune[knob];
+}
 
 size_t sgl_tables_bytes(const sgl_shape* shape, int32_t B) {
   if (!shape_ok(shape) || !batch_ok(B)) return 0;

@@ -238,51 +238,56 @@ __global__ __launch_bounds__(256) void k_invert_fp64(const double* __restrict__ 
 
 // ----------------------------------------------------------- MFMA GEMM --
 // p[b][j] = sum_kk E[j][kk] Phi[b][kk]; E: [M = n_out][Kp], Phi: [N = B][Kp],
-// both K-contiguous.  Workgroup tile 64 (j) x 64 (b), 4 waves of 32 x 32,
-// K step 32 through LDS.  FP32: v_mfma_f32_16x16x4_f32.  Split BF16:
+// both K-contiguous.  Workgroup tile 32 MT (j) x 64 (b), 4 waves of
+// 16 MT x 32, K step 32 through LDS; MT = 2 or 4 is a tuning knob.  FP32: v_mfma_f32_16x16x4_f32.  Split BF16:
 // v_mfma_f32_16x16x32_bf16 on (hi hi + hi lo + lo hi).
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef short bf16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kGBM = 64, kGBN = 64, kGBK = 32, kGLd = kGBK + 4;
+constexpr int kGBN = 64, kGBK = 32, kGLd = kGBK + 4;
 
+// MT = 16-row MFMA tiles per wave along j: the workgroup covers 32 MT rows
+// of E (2 waves along j) by 64 columns of Phi (2 waves of 32 along b).
+template <int MT>
 __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ E,
                                                   const float* __restrict__ Phi, int M, int N,
                                                   int Kp, float* __restrict__ P) {
-  __shared__ float sA[kGBM][kGLd];
+  constexpr int BM = 32 * MT;
+  __shared__ float sA[BM][kGLd];
   __shared__ float sB[kGBN][kGLd];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int m0 = blockIdx.x * kGBM, n0 = blockIdx.y * kGBN;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  f32x4 acc[2][2];
-  for (int a = 0; a < 2; ++a)
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * kGBN;
+  const int wm = (wave & 1) * 16 * MT, wn = (wave >> 1) * 32;
+  f32x4 acc[MT][2];
+  for (int a = 0; a < MT; ++a)
     for (int c = 0; c < 2; ++c) acc[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // loader: 64 rows x 32 floats = 512 float4; 256 threads x 2
   for (int k0 = 0; k0 < Kp; k0 += kGBK) {
     __syncthreads();
-    for (int q = t; q < kGBM * kGBK / 4; q += 256) {
+    for (int q = t; q < BM * kGBK / 4; q += 256) {
       int r = q / (kGBK / 4), c4 = (q % (kGBK / 4)) * 4;
-      int gm = m0 + r, gn = n0 + r;
+      int gm = m0 + r;
       float4 va = (gm < M) ? *(const float4*)(E + (size_t)gm * Kp + k0 + c4) : float4{0, 0, 0, 0};
-      float4 vb = (gn < N) ? *(const float4*)(Phi + (size_t)gn * Kp + k0 + c4) : float4{0, 0, 0, 0};
       sA[r][c4] = va.x; sA[r][c4 + 1] = va.y; sA[r][c4 + 2] = va.z; sA[r][c4 + 3] = va.w;
+    }
+    for (int q = t; q < kGBN * kGBK / 4; q += 256) {
+      int r = q / (kGBK / 4), c4 = (q % (kGBK / 4)) * 4;
+      int gn = n0 + r;
+      float4 vb = (gn < N) ? *(const float4*)(Phi + (size_t)gn * Kp + k0 + c4) : float4{0, 0, 0, 0};
       sB[r][c4] = vb.x; sB[r][c4 + 1] = vb.y; sB[r][c4 + 2] = vb.z; sB[r][c4 + 3] = vb.w;
     }
     __syncthreads();
 #pragma unroll
     for (int k4 = 0; k4 < kGBK; k4 += 4) {
-      float a0 = sA[wm + (lane & 15)][k4 + (lane >> 4)];
-      float a1 = sA[wm + 16 + (lane & 15)][k4 + (lane >> 4)];
-      float b0 = sB[wn + (lane & 15)][k4 + (lane >> 4)];
-      float b1 = sB[wn + 16 + (lane & 15)][k4 + (lane >> 4)];
-      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+      float bv[2], av[MT];
+      for (int c = 0; c < 2; ++c) bv[c] = sB[wn + 16 * c + (lane & 15)][k4 + (lane >> 4)];
+      for (int a = 0; a < MT; ++a) av[a] = sA[wm + 16 * a + (lane & 15)][k4 + (lane >> 4)];
+      for (int a = 0; a < MT; ++a)
+        for (int c = 0; c < 2; ++c)
+          acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[c], acc[a][c], 0, 0, 0);
     }
   }
   // D[i][j] of a 16x16 tile: lane holds i = 4 (lane / 16) + r, j = lane % 16
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < MT; ++a)
     for (int c = 0; c < 2; ++c)
       for (int r = 0; r < 4; ++r) {
         int gm = m0 + wm + a * 16 + 4 * (lane >> 4) + r;
@@ -293,56 +298,60 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ E,
 
 constexpr int kHLd = kGBK + 8;  // bf16 row stride in LDS (shorts)
 
+template <int MT>
 __global__ __launch_bounds__(256) void k_gemm_bf16x3(const uint16_t* __restrict__ E,
                                                      const uint16_t* __restrict__ Phi, int M,
                                                      int N, int Kp, float* __restrict__ P) {
-  __shared__ uint16_t sAh[kGBM][kHLd], sAl[kGBM][kHLd];
+  constexpr int BM = 32 * MT;
+  __shared__ uint16_t sAh[BM][kHLd], sAl[BM][kHLd];
   __shared__ uint16_t sBh[kGBN][kHLd], sBl[kGBN][kHLd];
   const uint16_t* Eh = E;
   const uint16_t* El = E + (size_t)M * Kp;
   const uint16_t* Bh = Phi;
   const uint16_t* Bl = Phi + (size_t)N * Kp;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int m0 = blockIdx.x * kGBM, n0 = blockIdx.y * kGBN;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  f32x4 acc[2][2];
-  for (int a = 0; a < 2; ++a)
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * kGBN;
+  const int wm = (wave & 1) * 16 * MT, wn = (wave >> 1) * 32;
+  f32x4 acc[MT][2];
+  for (int a = 0; a < MT; ++a)
     for (int c = 0; c < 2; ++c) acc[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const uint4 z = {0, 0, 0, 0};
   for (int k0 = 0; k0 < Kp; k0 += kGBK) {
     __syncthreads();
-    // 64 rows x 32 bf16 = 64 rows x 4 uint4 (8 bf16 each) = 256 uint4 per plane
-    {
-      int r = t >> 2, c8 = (t & 3) * 8;
-      int gm = m0 + r, gn = n0 + r;
-      uint4 z = {0, 0, 0, 0};
+    // a row of 32 bf16 is 4 uint4; 256 threads load 64 rows per pass
+    for (int r = t >> 2; r < BM; r += 64) {
+      int c8 = (t & 3) * 8, gm = m0 + r;
       uint4 ah = (gm < M) ? *(const uint4*)(Eh + (size_t)gm * Kp + k0 + c8) : z;
       uint4 al = (gm < M) ? *(const uint4*)(El + (size_t)gm * Kp + k0 + c8) : z;
-      uint4 bh = (gn < N) ? *(const uint4*)(Bh + (size_t)gn * Kp + k0 + c8) : z;
-      uint4 bl = (gn < N) ? *(const uint4*)(Bl + (size_t)gn * Kp + k0 + c8) : z;
       *(uint4*)&sAh[r][c8] = ah;
       *(uint4*)&sAl[r][c8] = al;
+    }
+    {
+      int r = t >> 2, c8 = (t & 3) * 8, gn = n0 + r;
+      uint4 bh = (gn < N) ? *(const uint4*)(Bh + (size_t)gn * Kp + k0 + c8) : z;
+      uint4 bl = (gn < N) ? *(const uint4*)(Bl + (size_t)gn * Kp + k0 + c8) : z;
       *(uint4*)&sBh[r][c8] = bh;
       *(uint4*)&sBl[r][c8] = bl;
     }
     __syncthreads();
     // one 16x16x32 step covers the whole K tile: lane supplies k = 8 (lane/16) .. +7
     const int kq = 8 * (lane >> 4);
-    bf16x8 ah0 = *(const bf16x8*)&sAh[wm + (lane & 15)][kq];
-    bf16x8 ah1 = *(const bf16x8*)&sAh[wm + 16 + (lane & 15)][kq];
-    bf16x8 al0 = *(const bf16x8*)&sAl[wm + (lane & 15)][kq];
-    bf16x8 al1 = *(const bf16x8*)&sAl[wm + 16 + (lane & 15)][kq];
-    bf16x8 bh0 = *(const bf16x8*)&sBh[wn + (lane & 15)][kq];
-    bf16x8 bh1 = *(const bf16x8*)&sBh[wn + 16 + (lane & 15)][kq];
-    bf16x8 bl0 = *(const bf16x8*)&sBl[wn + (lane & 15)][kq];
-    bf16x8 bl1 = *(const bf16x8*)&sBl[wn + 16 + (lane & 15)][kq];
-#define SGL_MF(A, B, C) C = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, C, 0, 0, 0)
-    SGL_MF(al0, bh0, acc[0][0]); SGL_MF(ah0, bl0, acc[0][0]); SGL_MF(ah0, bh0, acc[0][0]);
-    SGL_MF(al0, bh1, acc[0][1]); SGL_MF(ah0, bl1, acc[0][1]); SGL_MF(ah0, bh1, acc[0][1]);
-    SGL_MF(al1, bh0, acc[1][0]); SGL_MF(ah1, bl0, acc[1][0]); SGL_MF(ah1, bh0, acc[1][0]);
-    SGL_MF(al1, bh1, acc[1][1]); SGL_MF(ah1, bl1, acc[1][1]); SGL_MF(ah1, bh1, acc[1][1]);
-#undef SGL_MF
+    bf16x8 bh[2], bl[2];
+    for (int c = 0; c < 2; ++c) {
+      bh[c] = *(const bf16x8*)&sBh[wn + 16 * c + (lane & 15)][kq];
+      bl[c] = *(const bf16x8*)&sBl[wn + 16 * c + (lane & 15)][kq];
+    }
+    for (int a = 0; a < MT; ++a) {
+      bf16x8 ah = *(const bf16x8*)&sAh[wm + 16 * a + (lane & 15)][kq];
+      bf16x8 al = *(const bf16x8*)&sAl[wm + 16 * a + (lane & 15)][kq];
+      for (int c = 0; c < 2; ++c) {
+        acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[c], acc[a][c], 0, 0, 0);
+        acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[c], acc[a][c], 0, 0, 0);
+        acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[c], acc[a][c], 0, 0, 0);
+      }
+    }
   }
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < MT; ++a)
     for (int c = 0; c < 2; ++c)
       for (int r = 0; r < 4; ++r) {
         int gm = m0 + wm + a * 16 + 4 * (lane >> 4) + r;
@@ -463,13 +472,20 @@ int launch_invert_fp64(const double* phi, const double* knodes, const sgl_shape&
 int launch_gemm(const void* E, const void* Phi, const sgl_shape& sh, int B, int prec, float* P,
                 hipStream_t st) {
   int Kp = k_padded(sh);
-  dim3 grid((sh.n_out + kGBM - 1) / kGBM, (B + kGBN - 1) / kGBN);
-  if (prec == SGL_PREC_FP32)
-    hipLaunchKernelGGL(k_gemm_f32, grid, dim3(256), 0, st, (const float*)E, (const float*)Phi,
-                       sh.n_out, B, Kp, P);
-  else
-    hipLaunchKernelGGL(k_gemm_bf16x3, grid, dim3(256), 0, st, (const uint16_t*)E,
-                       (const uint16_t*)Phi, sh.n_out, B, Kp, P);
+  const bool wide = g_sgl_tune[SGL_TUNE_GEMM_BM] == 1;
+  const int BM = wide ? 128 : 64;
+  dim3 grid((sh.n_out + BM - 1) / BM, (B + kGBN - 1) / kGBN);
+  const float* Ef = (const float*)E;
+  const float* Pf = (const float*)Phi;
+  const uint16_t* Eb = (const uint16_t*)E;
+  const uint16_t* Pb = (const uint16_t*)Phi;
+  if (prec == SGL_PREC_FP32) {
+    if (wide) hipLaunchKernelGGL(k_gemm_f32<4>, grid, dim3(256), 0, st, Ef, Pf, sh.n_out, B, Kp, P);
+    else hipLaunchKernelGGL(k_gemm_f32<2>, grid, dim3(256), 0, st, Ef, Pf, sh.n_out, B, Kp, P);
+  } else {
+    if (wide) hipLaunchKernelGGL(k_gemm_bf16x3<4>, grid, dim3(256), 0, st, Eb, Pb, sh.n_out, B, Kp, P);
+    else hipLaunchKernelGGL(k_gemm_bf16x3<2>, grid, dim3(256), 0, st, Eb, Pb, sh.n_out, B, Kp, P);
+  }
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 

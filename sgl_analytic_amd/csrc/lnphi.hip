@@ -61,30 +61,34 @@ __device__ inline void stage_slice(SliceStage& S, const double* __restrict__ row
   }
 }
 
-// Cell c = (row j, column l) of the staged slice.  FastLog: the batched
-// path's FP32 lens profile away from x = 1 and FP32 log1p for the jump (the
-// exact path keeps FP64 throughout).
-template <bool FastLog = false>
+// Cell c = (row j, column l) of the staged slice, FP64 throughout.
 __device__ __forceinline__ void gen_cell(const SliceStage& S, int n_b, int c, double& xi,
                                          double& ws, double& bw) {
   const int j = c / n_b, l = c - j * n_b;
   const RowConst& rc = S.rc[j];
   if (rc.valid == 0.0) { xi = 0.0; ws = 0.0; bw = 0.0; return; }
   double kap, gam;
-  const double x = S.ug[l] * rc.s;
-  if (FastLog && fabs((1.0 - x) * (1.0 + x)) >= kNfwWSeries) {
-    float kf, gf;
-    trunc_kg_row_f32((float)x, rc, kf, gf);
-    kap = kf;
-    gam = gf;
-  } else {
-    trunc_kg_row(x, rc, kap, gam);
-  }
-  if (FastLog)
-    cell_jump_f32log(kap, gam, rc.warea * S.du2[l], xi, ws);
-  else
-    cell_jump(kap, gam, rc.warea * S.du2[l], xi, ws);
+  trunc_kg_row(S.ug[l] * rc.s, rc, kap, gam);
+  cell_jump(kap, gam, rc.warea * S.du2[l], xi, ws);
   bw = rc.beta * ws;
+}
+
+// The batched main pass stages the compact FP32 row constants.
+struct SliceStageF {
+  RowF rc[kMaxMasses];
+  double ug[kMaxB];
+  double du2[kMaxB];
+};
+
+__device__ inline void stage_slice_f(SliceStageF& S, const double* __restrict__ rows,
+                                     const double* __restrict__ uedge, const sgl_shape& sh) {
+  for (int j = threadIdx.x; j < sh.n_m; j += blockDim.x)
+    S.rc[j] = row_f(row_const(rows + (size_t)j * kRowStride));
+  for (int l = threadIdx.x; l < sh.n_b; l += blockDim.x) {
+    double u0 = uedge[l], u1 = uedge[l + 1];
+    S.ug[l] = sqrt(u0 * u1);
+    S.du2[l] = u1 * u1 - u0 * u0;
+  }
 }
 
 // ------------------------------------------------------------ exact path --
@@ -199,8 +203,6 @@ __device__ __forceinline__ int fine_bin(double xi) {
   return kFinePos + min(max(k, 0), kFineNeg - 1);
 }
 
-constexpr int kBinThreads = 512;
-
 // Cells whose profile is evaluated in FP64 on the batched path: the band
 // |1 - x^2| < 1/4 around the NFW branch point.  Row j covers it for columns
 // l with ug_l s_j inside the band; everything else takes the FP32 profile.
@@ -216,43 +218,81 @@ __device__ __forceinline__ void deposit(double* lf, double* lc, double xi, doubl
   atomicAdd(&lc[cb * 3 + 2], bw * xi * xi);
 }
 
-// Main pass: every cell outside the band, FP32 lens profile, FP32 log1p for
-// the jump; the slice's fine and coarse moments are written as partials.
+// Main-pass fine bins carry four moments: w, w xi, beta w, beta w xi.  The
+// slice's coarse two-point moments are formed from them at the end (coarse
+// bin = 32 fine bins; second moment sum_f (beta w xi)_f^2 / (beta w)_f, i.e.
+// each fine bin at its beta-weighted centroid), so a cell costs four LDS
+// atomics on fine bins, which neighbouring lanes rarely share, instead of
+// three more on coarse bins, which most lanes of a wave share.
+constexpr int kFineMom = 4;
+
+__device__ __forceinline__ void deposit4(double* lf, double xi, double ws, double bw) {
+  double* q = lf + fine_bin(xi) * kFineMom;
+  atomicAdd(q, ws);
+  atomicAdd(q + 1, ws * xi);
+  atomicAdd(q + 2, bw);
+  atomicAdd(q + 3, bw * xi);
+}
+
+// Main pass: every cell outside the band, FP32 lens profile, FP32 log for
+// the jump.  A thread owns one column l (impact parameter) at a time and
+// walks the masses, so the column geometry stays in registers and the row
+// constants are wave-uniform LDS reads.
+template <int kBinThreads>
 __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restrict__ rows,
                                                            const double* __restrict__ cosmo,
                                                            const double* __restrict__ uedge,
                                                            sgl_shape sh, double* __restrict__ hist) {
-  __shared__ SliceStage S;
-  __shared__ double lf[kFineBins * kMoments];
-  __shared__ double lc[kSliceBins * 3];
+  __shared__ SliceStageF S;
+  __shared__ double lf[kFineBins * kFineMom];
   const int t = threadIdx.x, i = blockIdx.x, b = blockIdx.y;
   const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
+  if (i >= n_z) return;                         // slices past n_z are never read
   const HistLayout HL = hist_layout(sh, gridDim.y);
-  for (int q = t; q < kFineBins * kMoments; q += kBinThreads) lf[q] = 0.0;
-  for (int q = t; q < kSliceBins * 3; q += kBinThreads) lc[q] = 0.0;
-  if (i < n_z) stage_slice(S, rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh);
+  for (int q = t; q < kFineBins * kFineMom; q += kBinThreads) lf[q] = 0.0;
+  stage_slice_f(S, rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh);
   __syncthreads();
-  if (i < n_z) {
-    const int n_cells = sh.n_m * sh.n_b;
-    for (int c = t; c < n_cells; c += kBinThreads) {
-      const int j = c / sh.n_b, l = c - j * sh.n_b;
-      const RowConst& rc = S.rc[j];
-      if (rc.valid == 0.0) continue;
-      const double x = S.ug[l] * rc.s;
-      if (in_band(x)) continue;                 // the FP64 band pass takes it
-      float kf, gf;
-      trunc_kg_row_f32((float)x, rc, kf, gf);
-      double xi, ws;
-      cell_jump_f32log(kf, gf, rc.warea * S.du2[l], xi, ws);
-      if (ws == 0.0) continue;
-      deposit(lf, lc, xi, ws, rc.beta * ws);
+  const int n_b = sh.n_b;
+  const int cols = min(n_b, kBinThreads);
+  const int rows_par = (n_b <= kBinThreads) ? kBinThreads / n_b : 1;
+  const int tr = t / cols, tc = t - tr * cols;
+  if (tr < rows_par) {
+    for (int l = tc; l < n_b; l += cols) {
+      const double ug = S.ug[l], du2 = S.du2[l];
+      for (int j = tr; j < sh.n_m; j += rows_par) {
+        const RowF& rc = S.rc[j];
+        if (rc.valid == 0.0f) continue;
+        const double x = ug * rc.s;
+        if (in_band(x)) continue;               // the FP64 band pass takes it
+        float kf, gf;
+        trunc_kg_row_f32((float)x, rc, kf, gf);
+        double xi, ws;
+        cell_jump_f32log(kf, gf, rc.warea * du2, xi, ws);
+        if (ws == 0.0) continue;
+        deposit4(lf, xi, ws, rc.beta * ws);
+      }
     }
   }
   __syncthreads();
   double* P = hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments;
-  for (int q = t; q < kFineBins * kMoments; q += kBinThreads) P[q] = lf[q];
+  for (int f = t; f < kFineBins; f += kBinThreads) {
+    P[f * kMoments] = lf[f * kFineMom];
+    P[f * kMoments + 1] = lf[f * kFineMom + 1];
+  }
   double* Q = hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3;
-  for (int q = t; q < kSliceBins * 3; q += kBinThreads) Q[q] = lc[q];
+  for (int cb = t; cb < kSliceBins; cb += kBinThreads) {
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    for (int f = cb * kCoarse; f < (cb + 1) * kCoarse; ++f) {
+      const double bw = lf[f * kFineMom + 2], bwx = lf[f * kFineMom + 3];
+      if (bw == 0.0) continue;
+      m0 += bw;
+      m1 += bwx;
+      m2 += bwx * bwx / bw;
+    }
+    Q[cb * 3] = m0;
+    Q[cb * 3 + 1] = m1;
+    Q[cb * 3 + 2] = m2;
+  }
 }
 
 // Band pass: the cells with |1 - x^2| < 1/4 in FP64 (series branch of F, h
@@ -336,25 +376,96 @@ __global__ __launch_bounds__(256) void k_reduce_fine(const double* __restrict__ 
     r[2] = w;
     r[3] = 0.0;
   }
-  // coarse records, one slice per thread (72 bins each, order kept)
+  double* FP = hist + HL.fpos + (size_t)b * (kFineBins + 1);
+  for (int f = t; f < kFineBins; f += 256) FP[f] = (double)pos[f];
+  if (t == 0) FP[kFineBins] = hist[HL.fcnt + b];
+  // prefix sums of w m^p (p = 2 .. 9) over the positive records in bin
+  // order: each thread sums a chunk of kFinePos / 256 bins, the chunk totals
+  // are scanned in order by kTay threads, then every record R gets the sum
+  // over records 0 .. R-1 (entry np is the total).
+  {
+    constexpr int kChunk = kFinePos / 256;
+    static_assert(kFinePos % 256 == 0 && 257 * kTay <= kFineBins * kMoments, "scan fits in acc");
+    double* csum = acc + 0;                      // acc is free again: reuse as [256][kTay]
+    double mom[kTay];
+    for (int p = 0; p < kTay; ++p) mom[p] = 0.0;
+    double wm[kChunk], mm[kChunk];
+    for (int q = 0; q < kChunk; ++q) {
+      const int f = t * kChunk + q;
+      wm[q] = flag[f] ? acc[f * kMoments] : 0.0;
+      mm[q] = flag[f] ? acc[f * kMoments + 1] / wm[q] : 0.0;
+    }
+    __syncthreads();                             // every thread has read acc
+    for (int q = 0; q < kChunk; ++q) {
+      double v = wm[q] * mm[q];
+      for (int p = 0; p < kTay; ++p) { v *= mm[q]; mom[p] += v; }
+    }
+    for (int p = 0; p < kTay; ++p) csum[t * kTay + p] = mom[p];
+    __syncthreads();
+    if (t < kTay) {
+      double run = 0.0;
+      for (int c = 0; c < 256; ++c) {
+        const double v = csum[c * kTay + t];
+        csum[c * kTay + t] = run;                // exclusive prefix of the chunk totals
+        run += v;
+      }
+      csum[256 * kTay + t] = run;
+    }
+    __syncthreads();
+    double* PF = hist + HL.fpfx + (size_t)b * (kFinePos + 1) * kTay;
+    for (int p = 0; p < kTay; ++p) mom[p] = csum[t * kTay + p];
+    for (int q = 0; q < kChunk; ++q) {
+      const int f = t * kChunk + q;
+      if (!flag[f]) continue;
+      double* e = PF + (size_t)pos[f] * kTay;
+      double v = wm[q] * mm[q];
+      for (int p = 0; p < kTay; ++p) { e[p] = mom[p]; v *= mm[q]; mom[p] += v; }
+    }
+    if (t < kTay) PF[(size_t)pos[kFinePos] * kTay + t] = csum[256 * kTay + t];
+  }
+  // coarse records, one slice per thread (72 bins each, order kept), with
+  // the bin -> first-record map and the prefix Gaussian moments of the
+  // positive records
   for (int i = t; i < sh.n_z_max; i += 256) {
-    const double* Q = hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3;
-    double* Rs = hist + HL.srec + ((size_t)b * sh.n_z_max + i) * kSliceBins * kRec;
+    const size_t si = (size_t)b * sh.n_z_max + i;
+    const double* Q = hist + HL.cpart + si * kSliceBins * 3;
+    double* Rs = hist + HL.srec + si * kSliceBins * kRec;
+    double* CPo = hist + HL.cpos + si * (kCoarsePos + 1);
+    double* CPf = hist + HL.cpfx + si * (kCoarsePos + 1) * kTayC;
+    double run[kTayC];
+    for (int p = 0; p < kTayC; ++p) run[p] = 0.0;
     int n = 0;
     if (i < n_z) {
       for (int cb = 0; cb < kSliceBins; ++cb) {
+        if (cb <= kCoarsePos) {
+          CPo[cb] = (double)n;
+          if (cb == kCoarsePos)
+            for (int p = 0; p < kTayC; ++p) CPf[(size_t)n * kTayC + p] = run[p];
+        }
         const double w = Q[cb * 3];
         if (w == 0.0) continue;
         const double m = Q[cb * 3 + 1] / w;
-        const double v = Q[cb * 3 + 2] / w - m * m;
+        double v = Q[cb * 3 + 2] / w - m * m;
+        v = (v > 0.0) ? v : 0.0;
         Rs[n * kRec + 0] = m * (0.25 / kPi);
         Rs[n * kRec + 1] = m;
         Rs[n * kRec + 2] = w;
-        Rs[n * kRec + 3] = (v > 0.0) ? v : 0.0;
+        Rs[n * kRec + 3] = v;
+        if (cb < kCoarsePos) {
+          // raw moments of N(m, v): mu_1 = m, mu_2 = m^2 + v, mu_{n+1} = m mu_n + n v mu_{n-1}
+          double mu_prev = 1.0, mu = m;
+          for (int p = 0; p < kTayC; ++p) {
+            CPf[(size_t)n * kTayC + p] = run[p];
+            run[p] += w * mu;
+            const double nx = m * mu + (p + 1) * v * mu_prev;
+            mu_prev = mu;
+            mu = nx;
+          }
+        }
         ++n;
       }
     }
-    hist[HL.scnt + (size_t)b * sh.n_z_max + i] = (double)n;
+    hist[HL.scnt + si] = (double)n;
   }
 }
 
@@ -388,24 +499,44 @@ __device__ __forceinline__ void bin_phase(double k, const BinRec& r, double& re,
   smpg = smp + (double)(sn * gm1);
 }
 
-constexpr int kBinTile = 256;
-
-__global__ __launch_bounds__(256) void k_lnphi_binned(const double* __restrict__ hist,
+template <int kBinTile>
+__global__ __launch_bounds__(kBinTile) void k_lnphi_binned(const double* __restrict__ hist,
                                                       const double* __restrict__ cosmo,
                                                       const double* __restrict__ knodes,
                                                       sgl_shape sh, double* __restrict__ lam,
                                                       double* __restrict__ amp) {
+  static_assert(kBinTile >= kSliceBins, "a slice's coarse records are staged in one pass");
   __shared__ BinRec sb[kBinTile];
   const int t = threadIdx.x, b = blockIdx.y;
-  const int n = blockIdx.x * 256 + t;
+  const int n = blockIdx.x * kBinTile + t;
   const int n_k = sh.n_k;
   const HistLayout HL = hist_layout(sh, gridDim.y);
   const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
-  const double k = (n < n_k) ? knodes[n] : 0.0;
-  double lr = 0.0, li = 0.0;
+  const double k = knodes[min(n, n_k - 1)];
   const int nf = (int)hist[HL.fcnt + b];
   const double* fr = hist + HL.frec + (size_t)b * kFineBins * kRec;
-  for (int c0 = 0; c0 < nf; c0 += kBinTile) {
+  // Records of positive bins whose upper edge is <= kTayEps / k have
+  // k xi <= 1/2: their sum of w (e^{i k xi} - 1 - i k xi) is the Taylor
+  // series through (k xi)^9 on the prefix moments (truncation < 1e-9
+  // relative); the rest are summed directly.  The workgroup's k nodes are
+  // increasing, so its last node has the lowest first direct record.
+  const double* fpos = hist + HL.fpos + (size_t)b * (kFineBins + 1);
+  auto first_direct = [&](double kk) {
+    const double z = (log(kTayEps / kk) - log(kXiLo)) * (kFinePos / log(kXiHi / kXiLo));
+    const int F = (z <= 0.0) ? 0 : (z >= (double)kFinePos ? kFinePos : (int)z);
+    return (int)fpos[F];
+  };
+  const int r0 = first_direct(k);
+  const int r_lo = first_direct(knodes[min((int)(blockIdx.x + 1) * kBinTile, n_k) - 1]);
+  double lr, li;
+  {
+    const double* M = hist + HL.fpfx + ((size_t)b * (kFinePos + 1) + r0) * kTay;
+    const double q = k * k;
+    // cos - 1 = sum_m (-1)^m (k xi)^{2m} / (2m)!,  sin - k xi = sum_m (-1)^m (k xi)^{2m+1} / (2m+1)!
+    lr = q * (-M[0] * 0.5 + q * (M[2] * (1.0 / 24) + q * (-M[4] * (1.0 / 720) + q * (M[6] * (1.0 / 40320)))));
+    li = k * q * (-M[1] * (1.0 / 6) + q * (M[3] * (1.0 / 120) + q * (-M[5] * (1.0 / 5040) + q * (M[7] * (1.0 / 362880)))));
+  }
+  for (int c0 = r_lo; c0 < nf; c0 += kBinTile) {
     __syncthreads();
     if (c0 + t < nf) {
       const double* r = fr + (size_t)(c0 + t) * kRec;
@@ -413,7 +544,8 @@ __global__ __launch_bounds__(256) void k_lnphi_binned(const double* __restrict__
     }
     __syncthreads();
     const int nb = min(kBinTile, nf - c0);
-    for (int c = 0; c < nb; ++c) {
+    const int c_from = max(r0 - c0, 0);
+    for (int c = c_from; c < nb; ++c) {
       const BinRec r = sb[c];
       double re, im, smpg;
       bin_phase(k, r, re, im, smpg);
@@ -426,18 +558,38 @@ __global__ __launch_bounds__(256) void k_lnphi_binned(const double* __restrict__
     lam[2 * ((size_t)b * n_k + n) + 1] = li;
   }
   if (!(sh.flags & SGL_FLAG_CLUSTERING)) return;
+  // Coarse records of positive bins below coarse bin cb have mean + 2 sd
+  // <= 2 x (upper edge) (the spread of a bin's content is at most half its
+  // width), so with 2 k edge <= kTayEps their e^{i k xi - k^2 v/2} - 1 is the
+  // moment series through n = 9 (truncation < 1e-10 relative).
+  auto coarse_cut = [&](double kk) {
+    const double z = (log(0.5 * kTayEps / kk) - log(kXiLo)) * (kFinePos / log(kXiHi / kXiLo));
+    const int F = (z <= 0.0) ? 0 : (z >= (double)kFinePos ? kFinePos : (int)z);
+    return F / kCoarse;
+  };
+  const int cb0 = coarse_cut(k);
+  const int cb_lo = coarse_cut(knodes[min((int)(blockIdx.x + 1) * kBinTile, n_k) - 1]);
+  const double q = k * k;
   for (int i = 0; i < sh.n_z_max; ++i) {
     double ar = 0.0, ai = 0.0;
     if (i < n_z) {
-      const int ns = (int)hist[HL.scnt + (size_t)b * sh.n_z_max + i];
-      const double* sr = hist + HL.srec + ((size_t)b * sh.n_z_max + i) * kSliceBins * kRec;
+      const size_t si = (size_t)b * sh.n_z_max + i;
+      const int ns = (int)hist[HL.scnt + si];
+      const double* sr = hist + HL.srec + si * kSliceBins * kRec;
+      const double* cpo = hist + HL.cpos + si * (kCoarsePos + 1);
+      const int c0 = (int)cpo[cb0], c_lo = (int)cpo[cb_lo];
+      {
+        const double* M = hist + HL.cpfx + (si * (kCoarsePos + 1) + c0) * kTayC;
+        ar = q * (-M[1] * 0.5 + q * (M[3] * (1.0 / 24) + q * (-M[5] * (1.0 / 720) + q * (M[7] * (1.0 / 40320)))));
+        ai = k * (M[0] + q * (-M[2] * (1.0 / 6) + q * (M[4] * (1.0 / 120) + q * (-M[6] * (1.0 / 5040) + q * (M[8] * (1.0 / 362880))))));
+      }
       __syncthreads();
-      if (t < ns) {
-        const double* r = sr + (size_t)t * kRec;
+      if (c_lo + t < ns) {
+        const double* r = sr + (size_t)(c_lo + t) * kRec;
         sb[t] = BinRec{r[0], r[1], r[2], r[3]};
       }
       __syncthreads();
-      for (int c = 0; c < ns; ++c) {
+      for (int c = c0 - c_lo; c < ns - c_lo; ++c) {
         const BinRec r = sb[c];
         double re, im, smpg;
         bin_phase(k, r, re, im, smpg);
@@ -467,8 +619,12 @@ int launch_lnphi_exact(const double* rows, const double* cosmo, const double* ue
 
 int launch_bin(const double* rows, const double* cosmo, const double* uedge, const sgl_shape& sh,
                int B, double* hist, hipStream_t st) {
-  hipLaunchKernelGGL(k_bin_cells, dim3(sh.n_z_max, B), dim3(kBinThreads), 0, st, rows, cosmo, uedge,
-                     sh, hist);
+  const dim3 g(sh.n_z_max, B);
+  switch (g_sgl_tune[SGL_TUNE_BIN_THREADS]) {
+    case 0: hipLaunchKernelGGL(k_bin_cells<256>, g, dim3(256), 0, st, rows, cosmo, uedge, sh, hist); break;
+    case 2: hipLaunchKernelGGL(k_bin_cells<1024>, g, dim3(1024), 0, st, rows, cosmo, uedge, sh, hist); break;
+    default: hipLaunchKernelGGL(k_bin_cells<512>, g, dim3(512), 0, st, rows, cosmo, uedge, sh, hist); break;
+  }
   hipLaunchKernelGGL(k_bin_band, dim3(sh.n_z_max, B), dim3(kBandThreads), 0, st, rows, cosmo, uedge,
                      sh, hist);
   hipLaunchKernelGGL(k_reduce_fine, dim3(B), dim3(256), 0, st, cosmo, sh, hist);
@@ -477,8 +633,13 @@ int launch_bin(const double* rows, const double* cosmo, const double* uedge, con
 
 int launch_lnphi_binned(const double* hist, const double* cosmo, const double* knodes,
                         const sgl_shape& sh, int B, double* lam, double* amp, hipStream_t st) {
-  dim3 grid((sh.n_k + 255) / 256, B);
-  hipLaunchKernelGGL(k_lnphi_binned, grid, dim3(256), 0, st, hist, cosmo, knodes, sh, lam, amp);
+  if (g_sgl_tune[SGL_TUNE_LNPHI_BLOCK] == 0) {
+    dim3 grid((sh.n_k + 127) / 128, B);
+    hipLaunchKernelGGL(k_lnphi_binned<128>, grid, dim3(128), 0, st, hist, cosmo, knodes, sh, lam, amp);
+  } else {
+    dim3 grid((sh.n_k + 255) / 256, B);
+    hipLaunchKernelGGL(k_lnphi_binned<256>, grid, dim3(256), 0, st, hist, cosmo, knodes, sh, lam, amp);
+  }
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 

@@ -68,11 +68,22 @@ constexpr int kMoments = 2;        // fine bins carry w, w xi (centroid)
 constexpr int kRec = 4;            // compacted bin record: xi/(4 pi), xi, w, v
 constexpr int kMaxMasses = 256;
 constexpr int kMaxB = 1024;
+// Small-phase split of the fine-record sum: records with k xi <= kTayEps
+// enter through prefix sums of w xi^p, p = 2 .. 9 (kTay moments).
+constexpr int kTay = 8;
+constexpr double kTayEps = 0.5;
+// Coarse (per-slice) records with a Gaussian in-bin spread: prefix sums of
+// the raw Gaussian moments w mu_n, n = 1 .. 9, over the positive bins.
+constexpr int kTayC = 9;
+constexpr int kCoarsePos = kFinePos / kCoarse;
 
 // Histogram buffer (doubles): per-slice partial fine moments, compacted fine
-// records + count per instance, compacted coarse records + count per slice.
+// records + count per instance, compacted coarse records + count per slice,
+// per instance the bin -> first-record map (kFineBins + 1 entries) and the
+// prefix moments of the positive records ((kFinePos + 1) x kTay), and the
+// same two per slice for the positive coarse records.
 struct HistLayout {
-  size_t partial, cpart, frec, fcnt, srec, scnt, total;
+  size_t partial, cpart, frec, fcnt, srec, scnt, fpos, fpfx, cpos, cpfx, total;
 };
 __host__ __device__ inline size_t round32(size_t n) { return (n + 31) & ~size_t(31); }
 __host__ __device__ inline HistLayout hist_layout(const sgl_shape& s, int B) {
@@ -83,9 +94,17 @@ __host__ __device__ inline HistLayout hist_layout(const sgl_shape& s, int B) {
   L.fcnt = L.frec + round32((size_t)B * kFineBins * kRec);
   L.srec = L.fcnt + round32((size_t)B);
   L.scnt = L.srec + round32((size_t)B * s.n_z_max * kSliceBins * kRec);
-  L.total = L.scnt + round32((size_t)B * s.n_z_max);
+  L.fpos = L.scnt + round32((size_t)B * s.n_z_max);
+  L.fpfx = L.fpos + round32((size_t)B * (kFineBins + 1));
+  L.cpos = L.fpfx + round32((size_t)B * (kFinePos + 1) * kTay);
+  L.cpfx = L.cpos + round32((size_t)B * s.n_z_max * (kCoarsePos + 1));
+  L.total = L.cpfx + round32((size_t)B * s.n_z_max * (kCoarsePos + 1) * kTayC);
   return L;
 }
+
+// Launch-shape knobs set by the autotuner (sgl_set_tuning); host-side, read
+// by the launchers at launch time.
+extern int g_sgl_tune[SGL_TUNE_COUNT];
 
 int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes_h,
                  double* derived, double* slices, double* mass, double* rows, double* cov,

@@ -333,73 +333,103 @@ __device__ inline void trunc_kg_row(double x, const RowConst& rc, double& kap, d
   gam = rc.ks * (kb - ku);
 }
 
+// Per-row constants of the batched main pass: FP32 profile constants, FP64
+// where the band test and the cell weights need it.
+struct RowF {
+  double s, warea, beta;
+  float ks, xt, ixt, ic1, mut4ks, i0, jj, valid;
+};
+
+__device__ inline RowF row_f(const RowConst& c) {
+  RowF f;
+  f.s = c.s; f.warea = c.warea; f.beta = c.beta;
+  f.ks = (float)c.ks; f.xt = (float)c.xt; f.ixt = (float)(1.0 / c.xt); f.ic1 = (float)c.ic1;
+  f.mut4ks = (float)c.mut4ks; f.i0 = (float)c.i0; f.jj = (float)c.jj; f.valid = (float)c.valid;
+  return f;
+}
+
+// Hardware FP32 reciprocal, reciprocal square root and square root (1 ulp).
+__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsq_hw(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
+// log1p(y), y > -1, from the hardware log: with u = fl(1 + y),
+// log(u) y / (u - 1) cancels the rounding error of 1 + y (Goldberg 1991,
+// "What every computer scientist should know about floating-point
+// arithmetic", Theorem 4), leaving a few ulp.
+__device__ __forceinline__ float log1p_hw(float y) {
+  const float u = 1.0f + y, d = u - 1.0f;
+  return (d == 0.0f) ? y : __logf(u) * (y * rcp_hw(d));
+}
+
 // FP32 evaluation of the same closed forms for the batched path, used away
-// from x = 1 (|1 - x^2| >= 1/4) where none of the terms cancels badly; the
-// caller takes the FP64 trunc_kg_row inside that band.  Relative error of
-// kappa and gamma ~1e-6, far below the lattice binning error.
-__device__ inline void trunc_kg_row_f32(float x, const RowConst& rc, float& kap, float& gam) {
-  const float xt = (float)rc.xt;
+// from x = 1 only (|1 - x^2| >= 1/4, so x <= 0.866 or x >= 1.118): there
+// none of the terms cancels badly, 1 - x^2 and x^2 - 1 are >= 1/4, and the
+// small-x form of h applies on the whole x < 1 side.  Hardware rcp / rsq /
+// sqrt / log; relative error of kappa and gamma ~1e-6, far below the lattice
+// binning error.  The caller takes the FP64 trunc_kg_row inside the band.
+__device__ inline void trunc_kg_row_f32(float x, const RowF& rc, float& kap, float& gam) {
+  const float xt = rc.xt;
+  const float rx = rcp_hw(x), rx2 = rx * rx;
   if (x >= xt) {
     kap = 0.0f;
-    gam = (float)rc.mut4ks / (x * x);
+    gam = rc.mut4ks * rx2;
     return;
   }
   constexpr float kLn2f = 0.69314718f;
-  const float a = x / xt;
-  const float rx = 1.0f / x, rx2 = rx * rx;
+  const float a = x * rc.ixt;
   const float lx = __logf(x);
   const float w = (1.0f - x) * (1.0f + x);
-  const float ic1 = (float)rc.ic1;
+  const float ic1 = rc.ic1;
   float F, h, I1, I2;
   const bool small = a < (float)kTcASmall;
   if (small) {
-    const float i0 = (float)rc.i0, jj = (float)rc.jj;
-    I1 = i0 + 0.5f * a * a * jj;
-    I2 = 0.5f * i0 + 0.125f * a * a * jj;
+    I1 = rc.i0 + 0.5f * a * a * rc.jj;
+    I2 = 0.5f * rc.i0 + 0.125f * a * a * rc.jj;
   }
   if (x < 1.0f) {
-    const float e = (1.0f - x) * rx, su = sqrtf(w), isu = 1.0f / su, iu = 1.0f / w;
-    const float A0 = log1pf(e + sqrtf(e * (e + 2.0f)));
+    const float isu = rsq_hw(w), su = w * isu, iu = isu * isu;
+    const float e = (1.0f - x) * rx;
+    const float A0 = __logf(1.0f + e + sqrt_hw(e * (e + 2.0f)));   // arccosh(1/x), 1/x - 1 >= 0.15
+    const float i1s = rcp_hw(1.0f + su);
     F = (A0 * isu - 1.0f) * iu;
-    h = (x < (float)kNfwXH)
-            ? ((kLn2f - lx) * x * x / (1.0f + su) + log1pf(-x * x / (2.0f * (1.0f + su)))) * isu
-            : lx - kLn2f + A0 * isu;
+    h = ((kLn2f - lx) * x * x * i1s + log1p_hw(-0.5f * x * x * i1s)) * isu;
     if (!small) {
-      const float c = xt, q = sqrtf((c - x) * (c + x));
+      const float c = xt, q = sqrt_hw((c - x) * (c + x));
       const float d = (x - 1.0f) * (x - c) * rx * ic1;
       const float L = __logf((c + q) * ic1 * 0.5f) - (q - c) * ic1;
-      const float At = log1pf(d + sqrtf(d * (d + 2.0f)));
+      const float At = log1p_hw(d + sqrt_hw(d * (d + 2.0f)));
       I1 = (q * ic1 - 1.0f) * iu + (A0 - At) * iu * isu;
       I2 = (L + (A0 - At) * isu) * rx2;
     }
   } else {
-    const float v = -w, sv = sqrtf(v), isv = 1.0f / sv, iv = 1.0f / v;
-    const float B0 = 2.0f * asinf(sqrtf((x - 1.0f) * 0.5f * rx));
+    const float v = -w, isv = rsq_hw(v), iv = isv * isv;
+    const float B0 = 2.0f * asinf(sqrt_hw((x - 1.0f) * 0.5f * rx));
     F = (1.0f - B0 * isv) * iv;
     h = lx - kLn2f + B0 * isv;
     if (!small) {
-      const float c = xt, q = sqrtf((c - x) * (c + x));
+      const float c = xt, q = sqrt_hw((c - x) * (c + x));
       const float d = (x - 1.0f) * (x - c) * rx * ic1;
       const float L = __logf((c + q) * ic1 * 0.5f) - (q - c) * ic1;
-      const float Bt = 2.0f * asinf(sqrtf(fmaxf(-d, 0.0f) * 0.5f));
+      const float Bt = 2.0f * asinf(sqrt_hw(fmaxf(-d, 0.0f) * 0.5f));
       I1 = (1.0f - q * ic1) * iv - (B0 - Bt) * iv * isv;
       I2 = (L + (B0 - Bt) * isv) * rx2;
     }
   }
   const float ku = 2.0f * F - 2.0f * I1;
   const float kb = 4.0f * h * rx2 - 4.0f * I2;
-  const float ks = (float)rc.ks;
-  kap = ks * ku;
-  gam = ks * (kb - ku);
+  kap = rc.ks * ku;
+  gam = rc.ks * (kb - ku);
 }
 
 // Jump of one cell: xi = -ln detA and the tilted weight w e^{-xi} = w detA;
 // dropped (weight 0) when detA <= 0 or xi > 30.
 constexpr double kXiMax = 30.0;
-// Batched-path variant: xi = -log1p(detA - 1) with detA - 1 = kappa (kappa - 2)
-// - gamma^2 formed in FP64 (no cancellation) and the log1p on the FP32
-// hardware path (relative error ~1e-7 in xi; the bin index and moments need
-// no more).
+// Batched-path variant: detA - 1 = kappa (kappa - 2) - gamma^2 and detA
+// formed in FP64 (no cancellation), the log on the FP32 hardware path: a
+// short log1p series for |detA - 1| < 1e-3, else log of detA rounded to FP32
+// (relative precision kept down to detA ~ 1e-13, i.e. over the whole
+// xi <= 30 range).  Error ~1e-7 in xi; the bin index and moments need no
+// more.
 __device__ __forceinline__ void cell_jump_f32log(double kap, double gam, double w, double& xi,
                                                  double& ws) {
   double dm1 = kap * (kap - 2.0) - gam * gam;
@@ -407,7 +437,7 @@ __device__ __forceinline__ void cell_jump_f32log(double kap, double gam, double 
   if (!(detA > 0.0)) { xi = 0.0; ws = 0.0; return; }
   float fd = (float)dm1;
   float l1p = (fabsf(fd) < 1e-3f) ? fd * (1.0f - fd * (0.5f - fd * (1.0f / 3.0f)))
-                                  : __logf(1.0f + fd);
+                                  : __logf((float)detA);
   double x = -(double)l1p;
   if (x > kXiMax) { xi = 0.0; ws = 0.0; return; }
   xi = x;

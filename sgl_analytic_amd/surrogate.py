@@ -1,10 +1,14 @@
 """Learned surrogate of the exponent: (cosmology, z_s) -> ln phi at 512 log-k nodes.
 
 Network: 9 inputs (h, Om, sigma8, Ob, ns, Ok, w0, wa, ln z_s, standardised
-over the sampling box) -> 4 x 512 SiLU -> 1026 outputs: Re/Im ln phi at the
-512 nodes logspace(log10 k_lo, log10 k_hi, 512) of the solver's k range,
+over the sampling box) -> 4 x 512 SiLU -> 1026 outputs: Re/Im of ln phi at
+the 512 nodes logspace(log10 k_lo, log10 k_hi, 512) of the solver's k range,
 interleaved, plus two gate scalars (the solver's grid mass and <mu> on the
-output grid).  BF16 weights and activations, FP32 accumulation and FP32
+output grid).  The ln phi outputs are the correction from a coarse solve
+(about n_z/4 slices x 64 masses x 128 impact parameters, ~60x fewer cells,
+evaluated on the 512 nodes by the same batched kernels) to the full solve:
+the network learns a discretisation correction, smooth in the cosmology,
+rather than the whole exponent.  BF16 weights and activations, FP32 accumulation and FP32
 master weights; every product of the forward and backward pass runs in
 libsgl_amd.so (sgl_mlp_gemm with fused bias/SiLU/SiLU' epilogues,
 sgl_mlp_colsum, sgl_to_bf16) behind a torch.autograd.Function.
@@ -165,10 +169,19 @@ class SurrogateNet(torch.nn.Module):
 
 
 # ------------------------------------------------------- output transform --
-# Network targets per node: y_re = ln(-Re ln phi) (the decay, multiplicative
-# across cosmologies), y_im = Im ln phi / (1 + |Re ln phi|) (phase relative to
-# the decay); gate scalars as they are.
+# The network's ln phi outputs are the correction full - cheap, standardised
+# per output (to_targets / from_targets are the identity on it).  The *_log
+# pair is the parametrisation of a direct (no coarse solve) exponent model,
+# y_re = ln(-Re ln phi), y_im = Im ln phi / (1 + |Re ln phi|).
 def to_targets(Y: torch.Tensor) -> torch.Tensor:
+    return Y.clone()
+
+
+def from_targets(T: torch.Tensor) -> torch.Tensor:
+    return T
+
+
+def to_targets_log(Y: torch.Tensor) -> torch.Tensor:
     T = Y.clone()
     re = Y[:, 0:2 * N_C:2]
     T[:, 0:2 * N_C:2] = torch.log(torch.clamp(-re, min=1e-30))
@@ -176,7 +189,7 @@ def to_targets(Y: torch.Tensor) -> torch.Tensor:
     return T
 
 
-def from_targets(T: torch.Tensor) -> torch.Tensor:
+def from_targets_log(T: torch.Tensor) -> torch.Tensor:
     re = -torch.exp(T[:, 0:2 * N_C:2])
     Y = T.clone()
     Y[:, 0:2 * N_C:2] = re
@@ -186,8 +199,20 @@ def from_targets(T: torch.Tensor) -> torch.Tensor:
 
 # --------------------------------------------------------------------- data --
 def coarse_shape(full: Shape) -> Shape:
+    """The full lens population on the 512 surrogate k nodes."""
     from dataclasses import replace
     return replace(full, n_k=N_C)
+
+
+def cheap_shape(full: Shape) -> Shape:
+    """The coarse solve the network corrects: n_z/4 slices, 64 masses, 128
+    impact parameters, the 512 surrogate k nodes."""
+    from dataclasses import replace
+    return replace(full, n_z_max=max(4, -(-full.n_z_max // 4)), n_m=64, n_b=128, n_k=N_C)
+
+
+def cheap_slices(n_z):
+    return np.maximum(4, -(-np.asarray(n_z) // 4))
 
 
 def generate(params: np.ndarray, full: Shape, chunk=1024, with_pdf=False, log=print):
@@ -198,9 +223,12 @@ def generate(params: np.ndarray, full: Shape, chunk=1024, with_pdf=False, log=pr
     n_z = slice_count(params[:, 9])
     sh = Shape(**{**full.__dict__, "n_z_max": int(n_z.max())})
     csh = coarse_shape(sh)
+    hsh = cheap_shape(sh)
     s_full = Solver(sh, chunk, path="binned", precision=PREC_FP32)
     s_c = Solver(csh, chunk, path="binned", precision=PREC_FP32)
+    s_h = Solver(hsh, chunk, path="binned", precision=PREC_FP32)
     Y = np.zeros((B, N_OUT), dtype=np.float32)
+    H = np.zeros((B, 2 * N_C), dtype=np.float32)
     P = np.zeros((B, sh.n_out), dtype=np.float32) if with_pdf else None
     t0 = time.time()
     for a in range(0, B, chunk):
@@ -210,9 +238,14 @@ def generate(params: np.ndarray, full: Shape, chunk=1024, with_pdf=False, log=pr
             rec = np.concatenate([rec, np.repeat(rec[-1:], chunk - (b - a), 0)])
         s_c.run(rec)
         lp = (s_c.lam + s_c.psi2) if csh.clustering else s_c.lam
+        rec_h = rec.copy()
+        rec_h[:, 10] = cheap_slices(rec[:, 10])
+        s_h.run(rec_h)
+        lh = (s_h.lam + s_h.psi2) if hsh.clustering else s_h.lam
         p, d = s_full.run(rec)
         torch.cuda.synchronize()
         Y[a:b, :2 * N_C] = lp[:b - a].reshape(b - a, 2 * N_C).float().cpu().numpy()
+        H[a:b] = lh[:b - a].reshape(b - a, 2 * N_C).float().cpu().numpy()
         dd = d[:b - a].cpu().numpy()
         Y[a:b, 2 * N_C] = dd[:, 0]
         Y[a:b, 2 * N_C + 1] = dd[:, 5]
@@ -220,7 +253,7 @@ def generate(params: np.ndarray, full: Shape, chunk=1024, with_pdf=False, log=pr
             P[a:b] = p[:b - a].cpu().numpy()
         if log and (a // chunk) % 20 == 0:
             log(f"[surrogate] generated {b}/{B} in {time.time() - t0:.1f} s")
-    return Y, P
+    return Y, P, H
 
 
 def loss_weights(Y: torch.Tensor, k_c: torch.Tensor) -> torch.Tensor:
@@ -231,18 +264,26 @@ def loss_weights(Y: torch.Tensor, k_c: torch.Tensor) -> torch.Tensor:
     return w / w.sum(dim=1, keepdim=True)
 
 
-def train(net: SurrogateNet, X: np.ndarray, Y: np.ndarray, full: Shape, epochs=60, batch=1024,
-          lr=2e-3, log=print, seed=0):
+def correction(Y: np.ndarray, H: np.ndarray) -> np.ndarray:
+    """Network targets: [full - cheap ln phi at 512 nodes, gate scalars]."""
+    D = Y.copy()
+    D[:, :2 * N_C] = Y[:, :2 * N_C] - H
+    return D
+
+
+def train(net: SurrogateNet, X: np.ndarray, Y: np.ndarray, H: np.ndarray, full: Shape, epochs=60,
+          batch=1024, lr=2e-3, log=print, seed=0):
     dev = torch.device("cuda:0")
     net.to(dev)
     Xt = torch.as_tensor(X, dtype=torch.float32, device=dev)
-    Yt = torch.as_tensor(Y, dtype=torch.float32, device=dev)
+    Yt = torch.as_tensor(correction(Y, H), dtype=torch.float32, device=dev)
+    Yfull = torch.as_tensor(Y, dtype=torch.float32, device=dev)
     Tt = to_targets(Yt)
     net.y_mean.copy_(Tt.mean(0))
     net.y_std.copy_(Tt.std(0).clamp_min(1e-6))
     k_c = torch.as_tensor(np.logspace(math.log10(full.k_lo), math.log10(full.k_hi), N_C),
                           dtype=torch.float32, device=dev)
-    Wt = loss_weights(Yt, k_c)
+    Wt = loss_weights(Yfull, k_c)
     opt = torch.optim.Adam(net.parameters(), lr=lr)
     n = Xt.shape[0]
     steps = epochs * (n // batch)
@@ -277,20 +318,26 @@ def train(net: SurrogateNet, X: np.ndarray, Y: np.ndarray, full: Shape, epochs=6
 
 @dataclass
 class Inference:
-    """Preallocated surrogate inference on B instances: network -> phi ->
-    MFMA inversion -> diagnostics -> gates."""
+    """Preallocated surrogate inference on B instances: cheap solve on the 512
+    nodes + network correction -> interpolation onto the solver's k grid ->
+    window x exp -> MFMA Filon inversion -> FP64 diagnostics -> gates."""
     net: SurrogateNet
     shape: Shape
     B: int
     tol: float = 1e-3
+    chunk: int = 16384
+    n_z_max: int = 82
 
     def __post_init__(self):
         from dataclasses import replace
-        # psi2 is inside the prediction; one slice: only the k grid of the
-        # tables is read (its offset does not depend on the batch size)
-        self.sh = replace(self.shape, clustering=False, n_z_max=1)
+        dev = torch.device("cuda:0")
         self.L = _lib.load()
-        L, dev = self.L, torch.device("cuda:0")
+        L = self.L
+        self.hsh = cheap_shape(replace(self.shape, n_z_max=self.n_z_max))
+        self.cheap = Solver(self.hsh, min(self.B, self.chunk), path="binned", precision=PREC_FP32)
+        # inversion side: psi2 is inside the exponent; one slice, only the k
+        # grid of the tables is read (its offset does not depend on B)
+        self.sh = replace(self.shape, clustering=False, n_z_max=1)
         self.cs = self.sh.c()
         cs = ctypes.byref(self.cs)
         rec = torch.as_tensor(cosmo_records(lhs_params(1, 1), 1), device=dev)
@@ -306,12 +353,31 @@ class Inference:
                                device=dev)
         self.p = torch.zeros((self.B, self.sh.n_out), dtype=torch.float32, device=dev)
         self.diag = torch.zeros((self.B, 8), dtype=torch.float64, device=dev)
+        self.base = torch.zeros((self.B, N_OUT), dtype=torch.float32, device=dev)
+        self.records = None
 
-    def run(self, x: torch.Tensor):
-        """x [B, 9] standardised features on the device -> (p, diag, accept, pred)."""
+    def set_records(self, records: np.ndarray):
+        """Solver records [B, 16] (full slice counts in column 10)."""
+        r = np.asarray(records, dtype=np.float64).copy()
+        r[:, 10] = cheap_slices(r[:, 10])
+        self.records = torch.as_tensor(r, device=torch.device("cuda:0"))
+
+    def cheap_solve(self):
+        c = self.cheap.B
+        for a in range(0, self.B, c):
+            b = min(self.B, a + c)
+            rec = self.records[a:b]
+            if b - a < c:
+                rec = torch.cat([rec, rec[-1:].expand(c - (b - a), -1)])
+            self.cheap.records.copy_(rec)
+            self.cheap.run()
+            lp = self.cheap.lam + self.cheap.psi2
+            self.base[a:b, :2 * N_C] = lp[:b - a].reshape(b - a, 2 * N_C).float()
+
+    def invert(self, y: torch.Tensor):
         L, cs, st = self.L, ctypes.byref(self.cs), _stream()
-        y = self.net.predict(x).contiguous()
-        _lib.check(L.sgl_lnphi_interp(_ptr(y), N_OUT, N_C, self.B, cs, _ptr(self.lam), st),
+        y = y.contiguous()
+        _lib.check(L.sgl_lnphi_interp(_ptr(y), y.shape[1], N_C, self.B, cs, _ptr(self.lam), st),
                    "sgl_lnphi_interp")
         _lib.check(L.sgl_phi_pack(_ptr(self.lam), None, _ptr(self.tables), self.B, cs, PREC_FP32,
                                   _ptr(self.phi), st), "sgl_phi_pack")
@@ -319,47 +385,56 @@ class Inference:
                                      _ptr(self.p), st), "sgl_invert_gemm")
         _lib.check(L.sgl_diagnostics(_ptr(self.p), 0, self.B, cs, _ptr(self.diag), st),
                    "sgl_diagnostics")
+        return self.p, self.diag
+
+    def run(self, x: torch.Tensor, use_network: bool = True):
+        """x [B, 9] standardised features -> (p, diag, accept, y)."""
+        self.cheap_solve()
+        y = self.base.clone()
+        if use_network:
+            y += self.net.predict(x)
+        p, d = self.invert(y)
         gm, gmu = y[:, 2 * N_C].double(), y[:, 2 * N_C + 1].double()
-        accept = (((self.diag[:, 0] - gm).abs() <= self.tol)
-                  & ((self.diag[:, 5] - gmu).abs() <= self.tol))
-        return self.p, self.diag, accept, y
+        accept = (((d[:, 0] - gm).abs() <= self.tol) & ((d[:, 5] - gmu).abs() <= self.tol)
+                  & (d[:, 7] <= self.tol))
+        return p, d, accept, y
 
 
 # -------------------------------------------------------------- evaluation --
 def evaluate(net: SurrogateNet, params: np.ndarray, P_ref: np.ndarray, Y_ref: np.ndarray,
              full: Shape, batch=4096, tol=1e-3):
     """Surrogate PDFs for held-out cosmologies vs the solver's: per-instance L1
-    and the gate decision; also the floor from interpolating the solver's own
-    512-node ln phi (what a perfect network would reach)."""
+    and the gate decision, the cheap solve alone (no network), and the floor
+    from interpolating the full solver's own 512-node exponent."""
     dev = torch.device("cuda:0")
     n = params.shape[0]
     x_all = torch.as_tensor(features(params), dtype=torch.float32, device=dev)
+    rec_all = cosmo_records(params, slice_count(params[:, 9]))
     xo = full.out_grid()
-    out = dict(L1=np.zeros(n), accept=np.zeros(n, bool), L1_floor=np.zeros(n))
+    out = dict(L1=np.zeros(n), accept=np.zeros(n, bool), L1_cheap=np.zeros(n),
+               L1_floor=np.zeros(n))
     inf = Inference(net, full, batch, tol)
     for a in range(0, n, batch):
         b = min(n, a + batch)
         x = x_all[a:b]
+        rec = rec_all[a:b]
         if b - a < batch:
             x = torch.cat([x, x[-1:].expand(batch - (b - a), -1)])
-        p, d, acc, y = inf.run(x)
-        torch.cuda.synchronize()
-        pp = p[:b - a].cpu().numpy().astype(np.float64)
-        out["L1"][a:b] = np.trapezoid(np.abs(pp - P_ref[a:b]), xo, axis=1)
-        out["accept"][a:b] = acc[:b - a].cpu().numpy()
-        # floor: the solver's own 512-node exponent through the same interpolation
+            rec = np.concatenate([rec, np.repeat(rec[-1:], batch - (b - a), 0)])
+        inf.set_records(rec)
+        for key, use in (("L1", True), ("L1_cheap", False)):
+            p, d, acc, y = inf.run(x, use_network=use)
+            torch.cuda.synchronize()
+            pp = p[:b - a].cpu().numpy().astype(np.float64)
+            out[key][a:b] = np.trapezoid(np.abs(pp - P_ref[a:b]), xo, axis=1)
+            if use:
+                out["accept"][a:b] = acc[:b - a].cpu().numpy()
         yt = torch.as_tensor(Y_ref[a:b], dtype=torch.float32, device=dev)
         if b - a < batch:
             yt = torch.cat([yt, yt[-1:].expand(batch - (b - a), -1)])
-        L, cs, st = inf.L, ctypes.byref(inf.cs), _stream()
-        _lib.check(L.sgl_lnphi_interp(_ptr(yt.contiguous()), N_OUT, N_C, batch, cs, _ptr(inf.lam),
-                                      st), "sgl_lnphi_interp")
-        _lib.check(L.sgl_phi_pack(_ptr(inf.lam), None, _ptr(inf.tables), batch, cs, PREC_FP32,
-                                  _ptr(inf.phi), st), "sgl_phi_pack")
-        _lib.check(L.sgl_invert_gemm(_ptr(inf.E), _ptr(inf.phi), batch, cs, PREC_FP32,
-                                     _ptr(inf.p), st), "sgl_invert_gemm")
+        p, d = inf.invert(yt)
         torch.cuda.synchronize()
-        pf = inf.p[:b - a].cpu().numpy().astype(np.float64)
+        pf = p[:b - a].cpu().numpy().astype(np.float64)
         out["L1_floor"][a:b] = np.trapezoid(np.abs(pf - P_ref[a:b]), xo, axis=1)
     return out
 
@@ -396,13 +471,13 @@ def main(argv=None):
     t0 = time.time()
     ptr = lhs_params(a.n_train, seed=11)
     phd = lhs_params(a.n_hold, seed=12)
-    Ytr, _ = generate(ptr, full, log=log)
-    Yho, Pho = generate(phd, full, with_pdf=True, log=log)
+    Ytr, _, Htr = generate(ptr, full, log=log)
+    Yho, Pho, Hho = generate(phd, full, with_pdf=True, log=log)
     t_gen = time.time() - t0
     log(f"[surrogate] data: {a.n_train} + {a.n_hold} solves in {t_gen:.1f} s")
     net = SurrogateNet()
     t1 = time.time()
-    train(net, features(ptr), Ytr, full, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+    train(net, features(ptr), Ytr, Htr, full, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
     t_train = time.time() - t1
     ev = evaluate(net, phd, Pho.astype(np.float64), Yho, full)
     acc = ev["accept"]
@@ -411,6 +486,11 @@ def main(argv=None):
                L1_max=float(ev["L1"].max()), accept_rate=float(acc.mean()),
                accepted_L1_max=float(ev["L1"][acc].max()) if acc.any() else None,
                accepted_within_1e3=float((ev["L1"][acc] <= 1e-3).mean()) if acc.any() else None,
+               cheap_L1_median=float(np.median(ev["L1_cheap"])),
+               cheap_L1_p90=float(np.quantile(ev["L1_cheap"], 0.9)),
+               cheap_L1_max=float(ev["L1_cheap"].max()),
+               within_1e3=float((ev["L1"] <= 1e-3).mean()),
+               cheap_within_1e3=float((ev["L1_cheap"] <= 1e-3).mean()),
                floor_L1_median=float(np.median(ev["L1_floor"])),
                floor_L1_max=float(ev["L1_floor"].max()))
     log("[surrogate] report", json.dumps(rep))
