@@ -66,6 +66,13 @@ def main():
                     help="instead of the bench line: FP32 vs split-BF16x3 inversion on the "
                          "same batch, N members each checked against the FP64 plain path")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--sur-batch", type=int, default=65536, help="surrogate inference batch")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-sur-verify", action="store_true",
+                    help="skip the solver check of the surrogate batch (for kernel profiles)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only a 1 GiB device-to-device copy: the reference kernel for scaling "
+                         "the FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the batch dimension "
@@ -74,6 +81,13 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": a.numel() * 4}))
+        return
     from sgl_analytic_amd import solver as S
     from sgl_analytic_amd._lib import PREC_BF16X3, PREC_FP32
 
@@ -155,6 +169,10 @@ def main():
     log(f"[bench] plain FP64 path {plain_s * 1e3:.1f} ms/PDF -> {plain_rate:.2f} PDFs/s")
     del pp
 
+    # ------------------------------------------------------ surrogate --
+    sur = None if args.no_surrogate else surrogate_leg(args, shape, sv, value, log)
+    del sv
+
     # ------------------------------------------------------- roofline --
     roof = roofline(stage, shape, B, n_z, prec)
 
@@ -188,10 +206,92 @@ def main():
                    "against": "plain FP64 GPU path (itself <= 1e-10 L1 vs the FP64 C oracle)"},
         "stage_ms": {k: round(v, 3) for k, v in stage.items()},
         "tuning": tuning["choice"] if tuning else None,
+        "surrogate": sur,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def surrogate_leg(args, shape, sv, solver_rate, log):
+    """Surrogate path on an inference batch of fresh LHS cosmologies: timed
+    throughput (cheap solve + both networks + interpolation + MFMA inversion +
+    diagnostics + gates), then outside the timing every instance is solved by
+    the batched solver `sv`, every accepted surrogate PDF is checked against it,
+    and the same batch without the networks (the cheap solve alone) shows what
+    the networks gain."""
+    import torch
+    from sgl_analytic_amd import solver as S
+    from sgl_analytic_amd import surrogate as SU
+    path = SU.default_weights()
+    if not os.path.exists(path):
+        log(f"[bench] surrogate weights missing at {path}; train with "
+            "`python -m sgl_analytic_amd.surrogate`")
+        return {"status": "no trained weights"}
+    nets = SU.load(path)
+    Bs = args.sur_batch
+    params = SU.lhs_params(Bs, seed=SEED + 2)
+    n_z = S.slice_count(params[:, 9])
+    rec = S.cosmo_records(params, n_z)
+    x = torch.as_tensor(SU.features(params), dtype=torch.float32, device="cuda")
+    inf = SU.Inference(nets, shape, Bs, n_z_max=shape.n_z_max)
+    inf.set_records(rec)
+    inf.run(x)
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        p, d, acc, y, dis = inf.run(x)
+    torch.cuda.synchronize()
+    t_sur = (time.perf_counter() - t0) / reps
+    rate = Bs / t_sur
+    p_sur = p.cpu().numpy().astype(np.float64)
+    acc = acc.cpu().numpy()
+    log(f"[bench] surrogate B={Bs}: {t_sur * 1e3:.1f} ms -> {rate:.0f} PDFs/s, accept {acc.mean():.4f}")
+    base = {"pdfs_per_s": round(rate, 1), "batch": Bs, "ms_per_batch": round(t_sur * 1e3, 2),
+            "n_nets": len(nets), "accept_rate": round(float(acc.mean()), 5),
+            "accepted_pdfs_per_s": round(rate * float(acc.mean()), 1), "seed": SEED + 2}
+    if args.no_sur_verify:
+        return base
+    p0, _, _, _, _ = inf.run(x, use_network=False)
+    torch.cuda.synchronize()
+    p_cheap = p0.cpu().numpy().astype(np.float64)
+    del inf
+    torch.cuda.empty_cache()
+    xo = shape.out_grid()
+    L1 = np.zeros(Bs)
+    L1c = np.zeros(Bs)
+    Bv = sv.B
+    for a in range(0, Bs, Bv):
+        b = min(Bs, a + Bv)
+        r = rec[a:b]
+        if b - a < Bv:
+            r = np.concatenate([r, np.repeat(r[-1:], Bv - (b - a), 0)])
+        pr, _ = sv.run(r)
+        torch.cuda.synchronize()
+        pr = pr[:b - a].float().cpu().numpy().astype(np.float64)
+        L1[a:b] = np.trapezoid(np.abs(p_sur[a:b] - pr), xo, axis=1)
+        L1c[a:b] = np.trapezoid(np.abs(p_cheap[a:b] - pr), xo, axis=1)
+    n_acc = int(acc.sum())
+    ok = L1[acc] <= 1e-3
+    blended = Bs / (t_sur + (Bs - n_acc) / solver_rate)
+    log(f"[bench] surrogate verified: {int(ok.sum())}/{n_acc} accepted within 1e-3 (max "
+        f"{L1[acc].max() if n_acc else float('nan'):.3g}); L1 median with nets "
+        f"{np.median(L1):.3g}, without {np.median(L1c):.3g}")
+    return {
+        **base,
+        "blended_pdfs_per_s_with_solver_fallback": round(blended, 1),
+        "verified": {"against": "batched solver, every instance (itself <= 1e-3 L1 vs FP64)",
+                     "accepted": n_acc, "accepted_within_1e3": int(ok.sum()),
+                     "accepted_L1_max": float(L1[acc].max()) if n_acc else None,
+                     "accepted_L1_median": float(np.median(L1[acc])) if n_acc else None},
+        "network_gain": {"L1_median_with": float(np.median(L1)),
+                         "L1_median_without": float(np.median(L1c)),
+                         "L1_p90_with": float(np.quantile(L1, 0.9)),
+                         "L1_p90_without": float(np.quantile(L1c, 0.9)),
+                         "within_1e3_with": float((L1 <= 1e-3).mean()),
+                         "within_1e3_without": float((L1c <= 1e-3).mean())},
+    }
 
 
 def precision_study(args, B):

@@ -62,6 +62,88 @@ __global__ __launch_bounds__(256) void k_quadform(const double* __restrict__ cov
   }
 }
 
+// FP32 two-point term of the batched path on MFMA (v_mfma_f32_16x16x4_f32):
+// per instance U = A C with A[n][i] the slice amplitudes at k node n (Re and
+// Im as two real matrices, K = slices) and C the slice covariance, then
+// Psi_2[n] = 1/2 sum_j (Ur + i Ui)[n][j] (ar + i ai)[n][j].  A wave takes 16
+// k nodes at a time with the A fragments of every slice in registers and C
+// from LDS; 4 waves x kQmTiles x 16 nodes per workgroup.
+constexpr int kQmPad = 96;                 // slices padded to a multiple of 16
+constexpr int kQmTiles = 4;                // 16-node groups per wave
+static_assert(kMaxSlices <= kQmPad, "slice padding");
+
+typedef float f32x4q __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_quadform_mfma(const double* __restrict__ cov,
+                                                       const double* __restrict__ cosmo,
+                                                       const double* __restrict__ amp,
+                                                       int n_z_max, int n_k,
+                                                       double* __restrict__ psi2) {
+  __shared__ float sC[kQmPad][kQmPad + 1];
+  const int b = blockIdx.y, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
+  const double* C = cov + (size_t)b * n_z_max * n_z_max;
+  for (int q = t; q < kQmPad * kQmPad; q += 256) {
+    const int i = q / kQmPad, j = q % kQmPad;
+    sC[i][j] = (i < n_z && j < n_z) ? (float)C[(size_t)i * n_z_max + j] : 0.0f;
+  }
+  __syncthreads();
+  const int ns = (n_z + 3) / 4, nt = (n_z + 15) / 16;
+  const int col = lane & 15, grp = lane >> 4;
+  const double2* A2 = (const double2*)amp + (size_t)b * n_z_max * n_k;
+  for (int g = 0; g < kQmTiles; ++g) {
+    const int n0 = (blockIdx.x * 4 * kQmTiles + wave * kQmTiles + g) * 16;
+    if (n0 >= n_k) break;
+    // A fragments: lane supplies A[n0 + col][4 s + grp]
+    float ar[kQmPad / 4], ai[kQmPad / 4];
+#pragma unroll
+    for (int s = 0; s < kQmPad / 4; ++s) {
+      const int i = 4 * s + grp, n = n0 + col;
+      double2 v = make_double2(0.0, 0.0);
+      if (s < ns && i < n_z && n < n_k) v = A2[(size_t)i * n_k + n];
+      ar[s] = (float)v.x;
+      ai[s] = (float)v.y;
+    }
+    float sr[4] = {0.f, 0.f, 0.f, 0.f}, si[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int jt = 0; jt < nt; ++jt) {
+      f32x4q ur = {0.f, 0.f, 0.f, 0.f}, ui = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < kQmPad / 4; ++s) {
+        if (s < ns) {
+          const float c = sC[4 * s + grp][16 * jt + col];
+          ur = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[s], c, ur, 0, 0, 0);
+          ui = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[s], c, ui, 0, 0, 0);
+        }
+      }
+      // lane holds U[n0 + 4 grp + r][16 jt + col]
+      const int j = 16 * jt + col;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + 4 * grp + r;
+        if (j < n_z && n < n_k) {
+          const double2 v = A2[(size_t)j * n_k + n];
+          const float xr = (float)v.x, xi = (float)v.y;
+          sr[r] += ur[r] * xr - ui[r] * xi;
+          si[r] += ur[r] * xi + ui[r] * xr;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double dr = sr[r], di = si[r];
+      for (int m = 8; m > 0; m >>= 1) {
+        dr += __shfl_xor(dr, m, 64);
+        di += __shfl_xor(di, m, 64);
+      }
+      const int n = n0 + 4 * grp + r;
+      if (col == 0 && n < n_k) {
+        psi2[2 * ((size_t)b * n_k + n)] = 0.5 * dr;
+        psi2[2 * ((size_t)b * n_k + n) + 1] = 0.5 * di;
+      }
+    }
+  }
+}
+
 // --------------------------------------------------------------- window --
 __device__ __forceinline__ double bspline_window(double k, double K) {
   // cubic B-spline of support [0, K], 1 at k = 0 (sgl_full.py:83-94)
@@ -435,13 +517,16 @@ __global__ __launch_bounds__(256) void k_diag(const PT* __restrict__ p, sgl_shap
 // ------------------------------------------------------------- launchers --
 int launch_quadform(const double* cov, const double* cosmo, const double* amp, int B,
                     const sgl_shape& sh, int prec, double* psi2, hipStream_t st) {
-  dim3 grid((sh.n_k + kQfTile - 1) / kQfTile, B);
-  if (prec == SGL_PREC_FP64)
+  if (prec == SGL_PREC_FP64) {
+    dim3 grid((sh.n_k + kQfTile - 1) / kQfTile, B);
     hipLaunchKernelGGL(k_quadform<double>, grid, dim3(256), 0, st, cov, cosmo, amp, sh.n_z_max,
                        sh.n_k, psi2);
-  else
-    hipLaunchKernelGGL(k_quadform<float>, grid, dim3(256), 0, st, cov, cosmo, amp, sh.n_z_max,
+  } else {
+    const int per_wg = 4 * kQmTiles * 16;
+    dim3 grid((sh.n_k + per_wg - 1) / per_wg, B);
+    hipLaunchKernelGGL(k_quadform_mfma, grid, dim3(256), 0, st, cov, cosmo, amp, sh.n_z_max,
                        sh.n_k, psi2);
+  }
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 

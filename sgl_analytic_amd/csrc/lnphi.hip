@@ -499,6 +499,25 @@ __device__ __forceinline__ void bin_phase(double k, const BinRec& r, double& re,
   smpg = smp + (double)(sn * gm1);
 }
 
+// Fine records carry no in-bin variance: e^{i ph} - 1 = re + i sn and
+// smp = sin ph - ph, the same reduced-phase evaluation without the Gaussian.
+__device__ __forceinline__ void bin_phase0(double k, const BinRec& r, double& re, double& smp) {
+  const double ph = k * r.xi;
+  const double rv = k * r.x4pi;
+  const double f = rv - rint(rv);
+  const float ff = (float)f;
+  const float hs = __builtin_amdgcn_sinf(ff);
+  const float hc = __builtin_amdgcn_cosf(ff);
+  re = (double)(-2.0f * hs * hs);
+  if (fabs(ph) < 0.5) {
+    const double p2 = ph * ph;
+    smp = -ph * p2 * (1.0 / 6.0) *
+          (1.0 - p2 / 20.0 * (1.0 - p2 / 42.0 * (1.0 - p2 / 72.0 * (1.0 - p2 / 110.0))));
+  } else {
+    smp = (double)(2.0f * hs * hc) - ph;
+  }
+}
+
 template <int kBinTile>
 __global__ __launch_bounds__(kBinTile) void k_lnphi_binned(const double* __restrict__ hist,
                                                       const double* __restrict__ cosmo,
@@ -547,10 +566,10 @@ __global__ __launch_bounds__(kBinTile) void k_lnphi_binned(const double* __restr
     const int c_from = max(r0 - c0, 0);
     for (int c = c_from; c < nb; ++c) {
       const BinRec r = sb[c];
-      double re, im, smpg;
-      bin_phase(k, r, re, im, smpg);
+      double re, smp;
+      bin_phase0(k, r, re, smp);
       lr = fma(r.w, re, lr);
-      li = fma(r.w, smpg, li);
+      li = fma(r.w, smp, li);
     }
   }
   if (n < n_k) {
@@ -591,6 +610,10 @@ __global__ __launch_bounds__(kBinTile) void k_lnphi_binned(const double* __restr
       __syncthreads();
       for (int c = c0 - c_lo; c < ns - c_lo; ++c) {
         const BinRec r = sb[c];
+        if (q * r.v > 80.0) {              // e^{-k^2 v / 2} < e^{-40}: the record adds -w
+          ar -= r.w;
+          continue;
+        }
         double re, im, smpg;
         bin_phase(k, r, re, im, smpg);
         ar = fma(r.w, re, ar);

@@ -13,11 +13,15 @@ master weights; every product of the forward and backward pass runs in
 libsgl_amd.so (sgl_mlp_gemm with fused bias/SiLU/SiLU' epilogues,
 sgl_mlp_colsum, sgl_to_bf16) behind a torch.autograd.Function.
 
-Inference: network -> ln phi at 512 nodes -> cubic interpolation onto the
-8192 solver nodes (sgl_lnphi_interp) -> window x exp -> the same MFMA Filon
-inversion as the solver -> FP64 diagnostics.  A PDF is accepted when its
-grid mass and <mu> agree with the network's own gate outputs within the
-tolerance; the rest fall back to the batched solver.
+Two networks with different initialisations and sample orders are trained;
+inference uses their mean.
+
+Inference: coarse solve + mean network correction -> ln phi at 512 nodes ->
+cubic interpolation onto the 8192 solver nodes (sgl_lnphi_interp) -> window x
+exp -> the same MFMA Filon inversion as the solver -> FP64 diagnostics.  A PDF
+is accepted when its grid mass and <mu> agree with the networks' gate outputs
+within the tolerance and the two networks agree (GATE_DISAGREE); the rest
+fall back to the batched solver.
 """
 from __future__ import annotations
 
@@ -316,20 +320,36 @@ def train(net: SurrogateNet, X: np.ndarray, Y: np.ndarray, H: np.ndarray, full: 
     return net
 
 
+# Gate threshold on the two networks' disagreement, (1/pi) int |phi| |d ln phi| dk
+# over the 512 nodes.  On 20000 held-out cosmologies (seed 12) its logarithm
+# correlates with that of the L1 error at 0.95; at 0.01 it accepts 76 % with
+# the largest accepted L1 7.5e-4 against the solver (0.02 would accept 82 %
+# with 8.6e-4; 0.05 lets a 1.4e-3 one through).
+GATE_DISAGREE = 0.01
+
+
 @dataclass
 class Inference:
     """Preallocated surrogate inference on B instances: cheap solve on the 512
-    nodes + network correction -> interpolation onto the solver's k grid ->
-    window x exp -> MFMA Filon inversion -> FP64 diagnostics -> gates."""
-    net: SurrogateNet
+    nodes + mean of the networks' corrections -> interpolation onto the solver's
+    k grid -> window x exp -> MFMA Filon inversion -> FP64 diagnostics -> gates.
+
+    A PDF is accepted when (1) its grid mass and <mu> agree with the networks'
+    gate outputs within tol and its negative mass is below tol, and (2) the two
+    networks' exponents disagree by less than GATE_DISAGREE in the L1 bound
+    (1/pi) int |phi| |d ln phi| dk.  The rest fall back to the solver."""
+    nets: list
     shape: Shape
     B: int
     tol: float = 1e-3
     chunk: int = 16384
     n_z_max: int = 82
+    gate_disagree: float = GATE_DISAGREE
 
     def __post_init__(self):
         from dataclasses import replace
+        if isinstance(self.nets, SurrogateNet):
+            self.nets = [self.nets]
         dev = torch.device("cuda:0")
         self.L = _lib.load()
         L = self.L
@@ -354,6 +374,9 @@ class Inference:
         self.p = torch.zeros((self.B, self.sh.n_out), dtype=torch.float32, device=dev)
         self.diag = torch.zeros((self.B, 8), dtype=torch.float64, device=dev)
         self.base = torch.zeros((self.B, N_OUT), dtype=torch.float32, device=dev)
+        k_c = np.logspace(math.log10(self.shape.k_lo), math.log10(self.shape.k_hi), N_C)
+        dk = np.gradient(k_c)
+        self.dk_pi = torch.as_tensor(dk / math.pi, dtype=torch.float32, device=dev)
         self.records = None
 
     def set_records(self, records: np.ndarray):
@@ -388,20 +411,29 @@ class Inference:
         return self.p, self.diag
 
     def run(self, x: torch.Tensor, use_network: bool = True):
-        """x [B, 9] standardised features -> (p, diag, accept, y)."""
+        """x [B, 9] standardised features -> (p, diag, accept, y, disagreement)."""
         self.cheap_solve()
         y = self.base.clone()
+        dis = torch.zeros(self.B, dtype=torch.float32, device=y.device)
         if use_network:
-            y += self.net.predict(x)
-        p, d = self.invert(y)
+            outs = [net.predict(x) for net in self.nets]
+            y += sum(outs) / len(outs)
+            if len(outs) > 1:
+                d = outs[0][:, :2 * N_C] - outs[1][:, :2 * N_C]
+                amp = torch.exp(torch.clamp(y[:, 0:2 * N_C:2], max=0.0))
+                dl = torch.sqrt(d[:, 0::2] ** 2 + d[:, 1::2] ** 2)
+                dis = (amp * dl * self.dk_pi[None, :]).sum(1)
+        p, dg = self.invert(y)
         gm, gmu = y[:, 2 * N_C].double(), y[:, 2 * N_C + 1].double()
-        accept = (((d[:, 0] - gm).abs() <= self.tol) & ((d[:, 5] - gmu).abs() <= self.tol)
-                  & (d[:, 7] <= self.tol))
-        return p, d, accept, y
+        accept = (((dg[:, 0] - gm).abs() <= self.tol) & ((dg[:, 5] - gmu).abs() <= self.tol)
+                  & (dg[:, 7] <= self.tol))
+        if use_network and len(self.nets) > 1:
+            accept &= dis <= self.gate_disagree
+        return p, dg, accept, y, dis
 
 
 # -------------------------------------------------------------- evaluation --
-def evaluate(net: SurrogateNet, params: np.ndarray, P_ref: np.ndarray, Y_ref: np.ndarray,
+def evaluate(nets, params: np.ndarray, P_ref: np.ndarray, Y_ref: np.ndarray,
              full: Shape, batch=4096, tol=1e-3):
     """Surrogate PDFs for held-out cosmologies vs the solver's: per-instance L1
     and the gate decision, the cheap solve alone (no network), and the floor
@@ -412,8 +444,9 @@ def evaluate(net: SurrogateNet, params: np.ndarray, P_ref: np.ndarray, Y_ref: np
     rec_all = cosmo_records(params, slice_count(params[:, 9]))
     xo = full.out_grid()
     out = dict(L1=np.zeros(n), accept=np.zeros(n, bool), L1_cheap=np.zeros(n),
-               L1_floor=np.zeros(n))
-    inf = Inference(net, full, batch, tol)
+               L1_floor=np.zeros(n), disagree=np.zeros(n),
+               dmass=np.zeros(n), dmu=np.zeros(n), negm=np.zeros(n))
+    inf = Inference(nets, full, batch, tol)
     for a in range(0, n, batch):
         b = min(n, a + batch)
         x = x_all[a:b]
@@ -423,12 +456,19 @@ def evaluate(net: SurrogateNet, params: np.ndarray, P_ref: np.ndarray, Y_ref: np
             rec = np.concatenate([rec, np.repeat(rec[-1:], batch - (b - a), 0)])
         inf.set_records(rec)
         for key, use in (("L1", True), ("L1_cheap", False)):
-            p, d, acc, y = inf.run(x, use_network=use)
+            p, d, acc, y, dis = inf.run(x, use_network=use)
             torch.cuda.synchronize()
             pp = p[:b - a].cpu().numpy().astype(np.float64)
             out[key][a:b] = np.trapezoid(np.abs(pp - P_ref[a:b]), xo, axis=1)
             if use:
-                out["accept"][a:b] = acc[:b - a].cpu().numpy()
+                m = b - a
+                out["accept"][a:b] = acc[:m].cpu().numpy()
+                out["disagree"][a:b] = dis[:m].cpu().numpy()
+                dd = d[:m].cpu().numpy()
+                yy = y[:m, 2 * N_C:].double().cpu().numpy()
+                out["dmass"][a:b] = np.abs(dd[:, 0] - yy[:, 0])
+                out["dmu"][a:b] = np.abs(dd[:, 5] - yy[:, 1])
+                out["negm"][a:b] = dd[:, 7]
         yt = torch.as_tensor(Y_ref[a:b], dtype=torch.float32, device=dev)
         if b - a < batch:
             yt = torch.cat([yt, yt[-1:].expand(batch - (b - a), -1)])
@@ -439,16 +479,25 @@ def evaluate(net: SurrogateNet, params: np.ndarray, P_ref: np.ndarray, Y_ref: np
     return out
 
 
-def save(net: SurrogateNet, path: str, meta: dict):
-    torch.save({"state": {k: v.detach().cpu() for k, v in net.state_dict().items()},
+def save(nets, path: str, meta: dict):
+    torch.save({"nets": [{k: v.detach().cpu() for k, v in n.state_dict().items()} for n in nets],
                 "meta": meta}, path)
 
 
-def load(path: str) -> SurrogateNet:
+def load(path: str) -> list:
+    """The trained networks (weights only; nothing executable is read)."""
     blob = torch.load(path, map_location="cpu", weights_only=True)
-    net = SurrogateNet()
-    net.load_state_dict(blob["state"])
-    return net.to(torch.device("cuda:0"))
+    nets = []
+    for st in blob["nets"]:
+        net = SurrogateNet()
+        net.load_state_dict(st)
+        nets.append(net.to(torch.device("cuda:0")))
+    return nets
+
+
+def default_weights() -> str:
+    import os
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate.pt")
 
 
 def main(argv=None):
@@ -465,6 +514,8 @@ def main(argv=None):
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                   "data", "surrogate.pt"))
     ap.add_argument("--report", default=None)
+    ap.add_argument("--n-nets", type=int, default=2)
+    ap.add_argument("--dump", default=None, help="npz of per-instance hold-out diagnostics")
     a = ap.parse_args(argv)
     log = lambda *m: print(*m, flush=True)
     full = Shape()
@@ -475,13 +526,20 @@ def main(argv=None):
     Yho, Pho, Hho = generate(phd, full, with_pdf=True, log=log)
     t_gen = time.time() - t0
     log(f"[surrogate] data: {a.n_train} + {a.n_hold} solves in {t_gen:.1f} s")
-    net = SurrogateNet()
     t1 = time.time()
-    train(net, features(ptr), Ytr, Htr, full, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+    nets = []
+    for i in range(a.n_nets):
+        net = SurrogateNet(seed=i)
+        train(net, features(ptr), Ytr, Htr, full, epochs=a.epochs, batch=a.batch, lr=a.lr,
+              log=log, seed=i)
+        nets.append(net)
     t_train = time.time() - t1
-    ev = evaluate(net, phd, Pho.astype(np.float64), Yho, full)
+    ev = evaluate(nets, phd, Pho.astype(np.float64), Yho, full)
+    if a.dump:
+        np.savez(a.dump, params=phd, **ev)
     acc = ev["accept"]
-    rep = dict(n_train=a.n_train, n_hold=a.n_hold, epochs=a.epochs, gen_s=t_gen, train_s=t_train,
+    rep = dict(n_train=a.n_train, n_hold=a.n_hold, epochs=a.epochs, n_nets=a.n_nets, gen_s=t_gen,
+               train_s=t_train,
                L1_median=float(np.median(ev["L1"])), L1_p90=float(np.quantile(ev["L1"], 0.9)),
                L1_max=float(ev["L1"].max()), accept_rate=float(acc.mean()),
                accepted_L1_max=float(ev["L1"][acc].max()) if acc.any() else None,
@@ -495,7 +553,7 @@ def main(argv=None):
                floor_L1_max=float(ev["L1_floor"].max()))
     log("[surrogate] report", json.dumps(rep))
     os.makedirs(os.path.dirname(a.out), exist_ok=True)
-    save(net, a.out, rep)
+    save(nets, a.out, rep)
     if a.report:
         with open(a.report, "w") as f:
             json.dump(rep, f, indent=1)

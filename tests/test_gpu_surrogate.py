@@ -67,3 +67,50 @@ def test_lnphi_interp_is_cubic_exact():
     assert np.max(np.abs(L[0, :, 0] - ref)) < 1e-5
     assert np.max(np.abs(L[0, :, 1] + ref)) < 1e-5
     assert np.max(np.abs(L[1, :, 1] - 0.5)) < 1e-6
+
+
+def _trained():
+    import os
+    from sgl_analytic_amd import surrogate as SU
+    path = SU.default_weights()
+    if not os.path.exists(path):
+        pytest.skip("no trained surrogate weights in sgl_analytic_amd/data")
+    return SU, SU.load(path)
+
+
+def test_trained_network_gain_and_gates():
+    """On cosmologies the networks never saw, the trained networks must cut
+    the cheap solve's L1 error against the batched solver, and every PDF the
+    gates accept must be within 1e-3 L1 of the solver."""
+    from sgl_analytic_amd import solver as S
+    SU, nets = _trained()
+    B = 1024
+    params = SU.lhs_params(B, seed=4242)
+    n_z = S.slice_count(params[:, 9])
+    shape = S.Shape(n_z_max=82)
+    rec = S.cosmo_records(params, n_z)
+    print(f"surrogate gain B={B} seed=4242", flush=True)
+    x = torch.as_tensor(SU.features(params), dtype=torch.float32, device="cuda:0")
+    inf = SU.Inference(nets, shape, B, n_z_max=82)
+    inf.set_records(rec)
+    p, d, acc, y, _ = inf.run(x)
+    torch.cuda.synchronize()
+    p_net = p.cpu().numpy().astype(np.float64)
+    acc = acc.cpu().numpy()
+    p0, _, _, _, _ = inf.run(x, use_network=False)
+    torch.cuda.synchronize()
+    p_cheap = p0.cpu().numpy().astype(np.float64)
+    del inf
+    sv = S.Solver(shape, B, path="binned")
+    pr, _ = sv.run(rec)
+    torch.cuda.synchronize()
+    pr = pr.float().cpu().numpy().astype(np.float64)
+    xo = shape.out_grid()
+    l1 = np.trapezoid(np.abs(p_net - pr), xo, axis=1)
+    l1c = np.trapezoid(np.abs(p_cheap - pr), xo, axis=1)
+    print("median L1 with networks", np.median(l1), "without", np.median(l1c),
+          "accept", acc.mean(), "accepted max", l1[acc].max() if acc.any() else None)
+    assert np.median(l1) < 0.25 * np.median(l1c)
+    assert np.quantile(l1, 0.9) < 0.5 * np.quantile(l1c, 0.9)
+    assert acc.mean() > 0.5
+    assert np.all(l1[acc] <= 1e-3)
