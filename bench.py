@@ -1,5 +1,6 @@
 #!/usr/bin/env python
-"""Benchmark: PDFs/s of the batched MI355X solver at <= 1e-3 L1 vs FP64.
+"""Benchmark: PDFs/s of the batched MI355X solver at <= 1e-3 L1 vs FP64, and of
+the surrogate path.
 
 Workload (one "step"): one batched solve of B = 4096 cosmologies drawn by a
 seeded Latin hypercube over h[0.6,0.8], Om[0.2,0.4], sigma8[0.7,0.9],
@@ -9,11 +10,18 @@ zeq = 3402, z_s log-uniform in [0.3, 10]; ceil(24 (1 + ln(1 + z_s))) slices x
 [1e-2, 1e6], 2048 ln(mu) nodes on [-0.5, 3].  The step is one replay of the
 captured HIP graph (tables -> binning -> exponent -> two-point term -> phi ->
 MFMA inversion -> diagnostics); the cosmology records are resident in HBM.
+Launch shapes are picked first by the autotuner on this batch.
 
-Reported beside it: the plain single-cosmology FP64 GPU path on the
-64 x 256 x 512 shape at z_s = 1 (PDFs/s and the ratio), the L1 / <mu> / mass /
-P(mu>2) errors of sampled batch members against the FP64 plain path, the
-roofline of the dominant kernel, and the FP64 C oracle timed on the host.
+Reported beside it:
+- the plain single-cosmology FP64 GPU path on the 64 x 256 x 512 shape at
+  z_s = 1 (PDFs/s and the ratio);
+- the L1 / <mu> / mass / P(mu>2) errors of sampled batch members against the
+  FP64 plain path;
+- the surrogate path on 65536 fresh LHS cosmologies: throughput, acceptance,
+  every accepted PDF checked against the batched solver, and the same batch
+  without the networks (what the networks gain);
+- the roofline of the dominant kernel (k_bin_cells) and the FP64 C oracle
+  timed on the host.
 """
 from __future__ import annotations
 
@@ -32,9 +40,16 @@ sys.path.insert(0, ROOT)
 
 SEED = 20261203
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "hbm": 8000.0}
-# FP64 VALU operations executed per generated cell by the binning kernel,
-# counted from the SQ_INSTS_VALU_*_F64 counters (FMA = 2); see profiles/.
-CELL_FLOPS_FP64 = float(os.environ.get("SGL_CELL_FLOPS", "0")) or None
+# Work of the dominant kernel, k_bin_cells, per generated cell: the FP32 and
+# FP64 VALU operations it issues (FMA = 2, transcendental = 1, 64 lanes per
+# wave instruction), i.e. SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_{F32,F64} summed
+# over one dispatch of the B = 4096 workload (27.55e9 cells) divided by its
+# cell count (profiles/batched_B4096_counters.txt).
+CELL_FLOPS = 156.3
+# HBM bytes of one k_bin_cells dispatch of the B = 4096 workload: FETCH_SIZE x 2.0
+# + WRITE_SIZE x 1.0, the factors being what those counters read for a 1 GiB
+# device copy (bench.py --copy-calibration) divided by 1 GiB.
+BIN_TRAFFIC_B4096 = 1.684e6 * 1024 * 2.0 + 7.922e6 * 1024 * 1.0
 
 
 def lhs_params(B, seed=SEED):
@@ -114,6 +129,8 @@ def main():
             {k: {s_: round(v, 2) for s_, v in m.items()} for k, m in tuning["ms"].items()})
     stage = stage_times(sv, S)          # eager pass: per-stage event times (also builds E)
     log("[bench] stage ms", {k: round(v, 3) for k, v in stage.items()})
+    t_bin = bin_pass_ms(sv, S)
+    log(f"[bench] k_bin_cells {t_bin:.2f} ms per launch")
     if not args.no_graph:
         sv.capture()
     for _ in range(args.warmup):
@@ -174,7 +191,7 @@ def main():
     del sv
 
     # ------------------------------------------------------- roofline --
-    roof = roofline(stage, shape, B, n_z, prec)
+    roof = roofline(t_bin, shape, B, n_z)
 
     # ----------------------------------------------------- CPU baseline --
     cpu = None if args.no_cpu_baseline else cpu_baseline(log)
@@ -338,7 +355,8 @@ def precision_study(args, B):
 
 
 def stage_times(sv, S):
-    """One eager solve with HIP events around every library call (same stream)."""
+    """One eager solve with HIP events around every library call (same stream);
+    the binning stage is timed as its two halves (sgl_bin_pass, sgl_bin_reduce)."""
     import torch
     L, B, cs, st, P = sv.L, sv.B, ctypes.byref(sv.cs), S._stream(), S._ptr
     names, evs = [], [torch.cuda.Event(enable_timing=True)]
@@ -351,7 +369,8 @@ def stage_times(sv, S):
         evs.append(e)
 
     L.sgl_tables_build(P(sv.records), B, cs, P(sv.tables), P(sv.status), st); mark("tables")
-    L.sgl_bin_cells(P(sv.tables), B, cs, P(sv.hist), st); mark("bin_cells")
+    L.sgl_bin_pass(P(sv.tables), B, cs, P(sv.hist), st); mark("bin_pass")
+    L.sgl_bin_reduce(P(sv.tables), B, cs, P(sv.hist), st); mark("bin_reduce")
     L.sgl_lnphi_binned(P(sv.tables), P(sv.hist), B, cs, P(sv.lam), P(sv.amp), st); mark("lnphi_binned")
     L.sgl_quadform(P(sv.tables), P(sv.amp), B, cs, 1, P(sv.psi2), st); mark("quadform")
     L.sgl_phi_pack(P(sv.lam), P(sv.psi2), P(sv.tables), B, cs, sv.precision, P(sv.phi), st); mark("phi_pack")
@@ -365,32 +384,38 @@ def stage_times(sv, S):
     return {n: evs[i].elapsed_time(evs[i + 1]) for i, n in enumerate(names)}
 
 
-def roofline(stage, shape, B, n_z, prec):
-    """Roofline of the slowest per-step kernel (filon_build runs once, not per step)."""
-    per_step = {k: v for k, v in stage.items() if "once" not in k}
-    dom = max(per_step, key=per_step.get)
-    t = per_step[dom] * 1e-3
+def bin_pass_ms(sv, S, reps=3):
+    """Average duration of the dominant kernel (k_bin_cells, the only kernel
+    sgl_bin_pass launches), HIP events on the library stream around each call."""
+    import torch
+    L, B, cs, st, P = sv.L, sv.B, ctypes.byref(sv.cs), S._stream(), S._ptr
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        L.sgl_bin_pass(P(sv.tables), B, cs, P(sv.hist), st)
+    b.record()
+    b.synchronize()
+    L.sgl_bin_reduce(P(sv.tables), B, cs, P(sv.hist), st)   # leave the histogram complete
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def roofline(t_bin_ms, shape, B, n_z):
+    """Roofline of the dominant kernel, k_bin_cells: VALU-bound (per cell ~10
+    hardware transcendentals and ~150 other FP32/FP64 operations, no reuse to
+    stage through LDS beyond the per-slice row constants), priced against the
+    FP32 vector peak; HBM traffic from the committed FETCH/WRITE counters."""
     cells = float(np.sum(n_z)) * shape.n_m * shape.n_b
-    kp = ((2 * (shape.n_k + 1) + 63) // 64) * 64
-    if dom == "invert_gemm":
-        flops = 2.0 * shape.n_out * kp * B
-        peak = PEAK["fp32"] if prec == 1 else PEAK["bf16"]
-        ach = flops / t / 1e12
-        return {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": peak,
-                "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": None,
-                "algorithmic": f"2 n_out Kp B = {flops:.3e} flop"}
-    if dom == "bin_cells" and CELL_FLOPS_FP64:
-        flops = cells * CELL_FLOPS_FP64
-        ach = flops / t / 1e12
-        return {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK["fp64"],
-                "unit": "TFLOP/s", "frac": round(ach / PEAK["fp64"], 4), "traffic": None,
-                "algorithmic": f"{CELL_FLOPS_FP64:.0f} FP64 flop/cell x {cells:.3e} cells"}
-    if dom == "bin_cells":
-        return {"kernel": dom, "bound": "mfma", "achieved": None, "peak": PEAK["fp64"],
-                "unit": "TFLOP/s", "frac": None, "traffic": None,
-                "cells_per_s": cells / t}
-    return {"kernel": dom, "bound": "hbm", "achieved": None, "peak": PEAK["hbm"], "unit": "GB/s",
-            "frac": None, "traffic": None, "ms": per_step[dom]}
+    flops = CELL_FLOPS * cells
+    ach = flops / (t_bin_ms * 1e-3) / 1e12
+    default = (B == 4096 and shape.n_m == 256 and shape.n_b == 512)
+    return {"kernel": "k_bin_cells (sgl_bin_pass)", "bound": "valu", "achieved": round(ach, 3),
+            "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": round(ach / PEAK["fp32"], 4),
+            "traffic": BIN_TRAFFIC_B4096 if default else None,
+            "ms": round(t_bin_ms, 3),
+            "algorithmic": f"{CELL_FLOPS} VALU flop per cell x {cells:.4e} cells",
+            "hbm_GBps": round(BIN_TRAFFIC_B4096 / (t_bin_ms * 1e-3) / 1e9, 1) if default else None}
 
 
 def cpu_baseline(log):

@@ -107,29 +107,42 @@ sgl_status sgl_lnphi_exact(const void* tables, int32_t B, const sgl_shape* shape
 
 /*
  * Batched path, step 1: histogram every cell into a fine signed-log jump
- * lattice (w, w xi, w xi^2 per bin) and a per-slice coarse lattice (beta w
- * moments), FP64 sums (LDS per slice, then one global add per bin).
+ * lattice (w, w xi per bin; FP64 sums in LDS per slice) and a per-slice
+ * coarse lattice (beta w, beta w xi, beta w xi^2), then reduce the slices in
+ * order into compacted bin records with their prefix moments.
+ * sgl_bin_cells = sgl_bin_pass (the FP32-profile pass over every cell off
+ * the |1 - x^2| < 1/4 band) followed by sgl_bin_reduce (the FP64 band pass
+ * and the slice-order reduction); the two halves are exported so that each
+ * can be timed on its own.
  * Replaces: Cells.add / _add_direct_kernel (sgl_full.py:1314-1383, 971-1046).
  */
 sgl_status sgl_bin_cells(const void* tables, int32_t B, const sgl_shape* shape, void* hist,
                          void* stream);
+sgl_status sgl_bin_pass(const void* tables, int32_t B, const sgl_shape* shape, void* hist,
+                        void* stream);
+sgl_status sgl_bin_reduce(const void* tables, int32_t B, const sgl_shape* shape, void* hist,
+                          void* stream);
 
 /*
  * Batched path, step 2: exponent from the histograms (bin centroid plus the
  * in-bin variance correction), FP64 range reduction, FP32 sin/cos, FP64
- * compensated small-phase terms and FP64 accumulation.
+ * compensated small-phase terms and FP64 accumulation.  lam: FP64
+ * [B][n_k][2]; amp (with SGL_FLAG_CLUSTERING): the slice amplitudes as FP32
+ * [B][n_z_max][n_k][2], written for the instance's own slices only (the
+ * two-point term consumes them in FP32).
  * Replaces: Lam._sep_raw (sgl_full.py:3040-3086) on binned cells.
  */
 sgl_status sgl_lnphi_binned(const void* tables, const void* hist, int32_t B,
-                            const sgl_shape* shape, double* lam, double* amp, void* stream);
+                            const sgl_shape* shape, double* lam, float* amp, void* stream);
 
 /*
  * Two-point term Psi_2[b][n] = 1/2 sum_ij C_ij amp_i amp_j (complex square,
- * no conjugate).  precision: SGL_PREC_FP64 or SGL_PREC_FP32 (the arithmetic of
- * the sum; amplitudes and C staged in LDS at that precision, result FP64).
+ * no conjugate), result FP64.  precision SGL_PREC_FP64: amp is FP64 (as
+ * written by sgl_lnphi_exact), FP64 sums; SGL_PREC_FP32: amp is FP32 (as
+ * written by sgl_lnphi_binned), U = A C on FP32 MFMA with FP32 accumulation.
  * Replaces: Lam.Q (sgl_full.py:3353-3397), nord = 1.
  */
-sgl_status sgl_quadform(const void* tables, const double* amp, int32_t B, const sgl_shape* shape,
+sgl_status sgl_quadform(const void* tables, const void* amp, int32_t B, const sgl_shape* shape,
                         int32_t precision, double* psi2, void* stream);
 
 /*

@@ -22,6 +22,7 @@ EXPORTS = (
     "sgl_quadform", "sgl_phi_pack", "sgl_filon_build", "sgl_invert_gemm", "sgl_invert_fp64",
     "sgl_diagnostics", "sgl_invert_fp64_at", "sgl_mlp_gemm", "sgl_mlp_colsum", "sgl_to_bf16",
     "sgl_lnphi_interp", "sgl_set_tuning", "sgl_get_tuning",
+    "sgl_bin_pass", "sgl_bin_reduce",
 )
 EPI_PLAIN, EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU = 0, 1, 2, 3
 TUNE_BIN_THREADS, TUNE_LNPHI_BLOCK, TUNE_GEMM_BM = 0, 1, 2
@@ -68,6 +69,8 @@ def load(path: str = LIB_PATH):
         "sgl_tables_build": [vp, i32, sp, vp, vp, vp],
         "sgl_lnphi_exact": [vp, i32, sp, vp, vp, vp, vp],
         "sgl_bin_cells": [vp, i32, sp, vp, vp],
+        "sgl_bin_pass": [vp, i32, sp, vp, vp],
+        "sgl_bin_reduce": [vp, i32, sp, vp, vp],
         "sgl_lnphi_binned": [vp, vp, i32, sp, vp, vp, vp],
         "sgl_quadform": [vp, vp, i32, sp, i32, vp, vp],
         "sgl_phi_pack": [vp, vp, vp, i32, sp, i32, vp, vp],

@@ -9,7 +9,7 @@
 using namespace sgl;
 
 namespace sgl {
-int g_sgl_tune[SGL_TUNE_COUNT] = {2, 1, 0};
+int g_sgl_tune[SGL_TUNE_COUNT] = {2, 0, 0};
 }
 
 namespace {
@@ -123,8 +123,26 @@ sgl_status sgl_bin_cells(const void* tables, int32_t B, const sgl_shape* shape, 
                                 static_cast<double*>(hist), as_stream(stream));
 }
 
+sgl_status sgl_bin_pass(const void* tables, int32_t B, const sgl_shape* shape, void* hist,
+                        void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !tables || !hist) return SGL_ERR_INVALID;
+  TablesLayout L = tables_layout(*shape, B);
+  const double* T = static_cast<const double*>(tables);
+  return (sgl_status)launch_bin_pass(T + L.rows, T + L.cosmo, T + L.uedge, *shape, B,
+                                     static_cast<double*>(hist), as_stream(stream));
+}
+
+sgl_status sgl_bin_reduce(const void* tables, int32_t B, const sgl_shape* shape, void* hist,
+                          void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !tables || !hist) return SGL_ERR_INVALID;
+  TablesLayout L = tables_layout(*shape, B);
+  const double* T = static_cast<const double*>(tables);
+  return (sgl_status)launch_bin_reduce(T + L.rows, T + L.cosmo, T + L.uedge, *shape, B,
+                                       static_cast<double*>(hist), as_stream(stream));
+}
+
 sgl_status sgl_lnphi_binned(const void* tables, const void* hist, int32_t B,
-                              const sgl_shape* shape, double* lam, double* amp, void* stream) {
+                              const sgl_shape* shape, double* lam, float* amp, void* stream) {
   if (!shape_ok(shape) || !batch_ok(B) || !tables || !hist || !lam) return SGL_ERR_INVALID;
   if ((shape->flags & SGL_FLAG_CLUSTERING) && !amp) return SGL_ERR_INVALID;
   TablesLayout L = tables_layout(*shape, B);
@@ -133,7 +151,7 @@ sgl_status sgl_lnphi_binned(const void* tables, const void* hist, int32_t B,
                                          T + L.knodes, *shape, B, lam, amp, as_stream(stream));
 }
 
-sgl_status sgl_quadform(const void* tables, const double* amp, int32_t B, const sgl_shape* shape,
+sgl_status sgl_quadform(const void* tables, const void* amp, int32_t B, const sgl_shape* shape,
                         int32_t precision, double* psi2, void* stream) {
   if (!shape_ok(shape) || !batch_ok(B) || !tables || !amp || !psi2) return SGL_ERR_INVALID;
   if (precision != SGL_PREC_FP64 && precision != SGL_PREC_FP32) return SGL_ERR_UNSUPPORTED;

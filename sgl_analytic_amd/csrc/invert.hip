@@ -12,17 +12,17 @@ namespace sgl {
 
 // ------------------------------------------------------------- quadform --
 // Psi_2[b][n] = 1/2 sum_ij C_ij a_i a_j (complex square, Lam.Q nord = 1,
-// sgl_full.py:3353-3397).  Workgroup = (instance, 32 k nodes): the tile's
-// amplitudes and C in LDS (T = double for the plain path, float for the
-// batched path), 8 threads per k node each taking every 8th column j of
-// u_j = sum_i C_ij a_i, partial sums combined with lane shuffles.
+// sgl_full.py:3353-3397), FP64 for the plain path.  Workgroup = (instance,
+// 32 k nodes): the tile's amplitudes and C in LDS, 8 threads per k node each
+// taking every 8th column j of u_j = sum_i C_ij a_i, partial sums combined
+// with lane shuffles.
 constexpr int kQfTile = 32;
 
-template <typename T>
 __global__ __launch_bounds__(256) void k_quadform(const double* __restrict__ cov,
                                                   const double* __restrict__ cosmo,
                                                   const double* __restrict__ amp, int n_z_max,
                                                   int n_k, double* __restrict__ psi2) {
+  using T = double;
   __shared__ T sC[kMaxSlices * kMaxSlices];
   __shared__ T sa[kMaxSlices][kQfTile][2];
   const int b = blockIdx.y, t = threadIdx.x;
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_quadform(const double* __restrict__ cov
 }
 
 // FP32 two-point term of the batched path on MFMA (v_mfma_f32_16x16x4_f32):
-// per instance U = A C with A[n][i] the slice amplitudes at k node n (Re and
+// per instance U = A C with A[n][i] the FP32 slice amplitudes at k node n (Re and
 // Im as two real matrices, K = slices) and C the slice covariance, then
 // Psi_2[n] = 1/2 sum_j (Ur + i Ui)[n][j] (ar + i ai)[n][j].  A wave takes 16
 // k nodes at a time with the A fragments of every slice in registers and C
@@ -76,7 +76,7 @@ typedef float f32x4q __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void k_quadform_mfma(const double* __restrict__ cov,
                                                        const double* __restrict__ cosmo,
-                                                       const double* __restrict__ amp,
+                                                       const float* __restrict__ amp,
                                                        int n_z_max, int n_k,
                                                        double* __restrict__ psi2) {
   __shared__ float sC[kQmPad][kQmPad + 1];
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_quadform_mfma(const double* __restrict_
   __syncthreads();
   const int ns = (n_z + 3) / 4, nt = (n_z + 15) / 16;
   const int col = lane & 15, grp = lane >> 4;
-  const double2* A2 = (const double2*)amp + (size_t)b * n_z_max * n_k;
+  const float2* A2 = (const float2*)amp + (size_t)b * n_z_max * n_k;
   for (int g = 0; g < kQmTiles; ++g) {
     const int n0 = (blockIdx.x * 4 * kQmTiles + wave * kQmTiles + g) * 16;
     if (n0 >= n_k) break;
@@ -99,10 +99,10 @@ __global__ __launch_bounds__(256) void k_quadform_mfma(const double* __restrict_
 #pragma unroll
     for (int s = 0; s < kQmPad / 4; ++s) {
       const int i = 4 * s + grp, n = n0 + col;
-      double2 v = make_double2(0.0, 0.0);
+      float2 v = make_float2(0.0f, 0.0f);
       if (s < ns && i < n_z && n < n_k) v = A2[(size_t)i * n_k + n];
-      ar[s] = (float)v.x;
-      ai[s] = (float)v.y;
+      ar[s] = v.x;
+      ai[s] = v.y;
     }
     float sr[4] = {0.f, 0.f, 0.f, 0.f}, si[4] = {0.f, 0.f, 0.f, 0.f};
     for (int jt = 0; jt < nt; ++jt) {
@@ -121,8 +121,8 @@ __global__ __launch_bounds__(256) void k_quadform_mfma(const double* __restrict_
       for (int r = 0; r < 4; ++r) {
         const int n = n0 + 4 * grp + r;
         if (j < n_z && n < n_k) {
-          const double2 v = A2[(size_t)j * n_k + n];
-          const float xr = (float)v.x, xi = (float)v.y;
+          const float2 v = A2[(size_t)j * n_k + n];
+          const float xr = v.x, xi = v.y;
           sr[r] += ur[r] * xr - ui[r] * xi;
           si[r] += ur[r] * xi + ui[r] * xr;
         }
@@ -515,17 +515,17 @@ __global__ __launch_bounds__(256) void k_diag(const PT* __restrict__ p, sgl_shap
 }
 
 // ------------------------------------------------------------- launchers --
-int launch_quadform(const double* cov, const double* cosmo, const double* amp, int B,
+int launch_quadform(const double* cov, const double* cosmo, const void* amp, int B,
                     const sgl_shape& sh, int prec, double* psi2, hipStream_t st) {
   if (prec == SGL_PREC_FP64) {
     dim3 grid((sh.n_k + kQfTile - 1) / kQfTile, B);
-    hipLaunchKernelGGL(k_quadform<double>, grid, dim3(256), 0, st, cov, cosmo, amp, sh.n_z_max,
-                       sh.n_k, psi2);
+    hipLaunchKernelGGL(k_quadform, grid, dim3(256), 0, st, cov, cosmo,
+                       (const double*)amp, sh.n_z_max, sh.n_k, psi2);
   } else {
     const int per_wg = 4 * kQmTiles * 16;
     dim3 grid((sh.n_k + per_wg - 1) / per_wg, B);
-    hipLaunchKernelGGL(k_quadform_mfma, grid, dim3(256), 0, st, cov, cosmo, amp, sh.n_z_max,
-                       sh.n_k, psi2);
+    hipLaunchKernelGGL(k_quadform_mfma, grid, dim3(256), 0, st, cov, cosmo, (const float*)amp,
+                       sh.n_z_max, sh.n_k, psi2);
   }
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }

@@ -523,7 +523,7 @@ __global__ __launch_bounds__(kBinTile) void k_lnphi_binned(const double* __restr
                                                       const double* __restrict__ cosmo,
                                                       const double* __restrict__ knodes,
                                                       sgl_shape sh, double* __restrict__ lam,
-                                                      double* __restrict__ amp) {
+                                                      float* __restrict__ amp) {
   static_assert(kBinTile >= kSliceBins, "a slice's coarse records are staged in one pass");
   __shared__ BinRec sb[kBinTile];
   const int t = threadIdx.x, b = blockIdx.y;
@@ -620,10 +620,9 @@ __global__ __launch_bounds__(kBinTile) void k_lnphi_binned(const double* __restr
         ai = fma(r.w, im, ai);
       }
     }
-    if (n < n_k) {
-      size_t o = 2 * (((size_t)b * sh.n_z_max + i) * n_k + n);
-      amp[o] = ar;
-      amp[o + 1] = ai;
+    if (n < n_k && i < n_z) {
+      const size_t o = ((size_t)b * sh.n_z_max + i) * n_k + n;
+      reinterpret_cast<float2*>(amp)[o] = make_float2((float)ar, (float)ai);
     }
   }
 }
@@ -640,22 +639,33 @@ int launch_lnphi_exact(const double* rows, const double* cosmo, const double* ue
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
-int launch_bin(const double* rows, const double* cosmo, const double* uedge, const sgl_shape& sh,
-               int B, double* hist, hipStream_t st) {
+int launch_bin_pass(const double* rows, const double* cosmo, const double* uedge,
+                    const sgl_shape& sh, int B, double* hist, hipStream_t st) {
   const dim3 g(sh.n_z_max, B);
   switch (g_sgl_tune[SGL_TUNE_BIN_THREADS]) {
     case 0: hipLaunchKernelGGL(k_bin_cells<256>, g, dim3(256), 0, st, rows, cosmo, uedge, sh, hist); break;
     case 2: hipLaunchKernelGGL(k_bin_cells<1024>, g, dim3(1024), 0, st, rows, cosmo, uedge, sh, hist); break;
     default: hipLaunchKernelGGL(k_bin_cells<512>, g, dim3(512), 0, st, rows, cosmo, uedge, sh, hist); break;
   }
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+int launch_bin_reduce(const double* rows, const double* cosmo, const double* uedge,
+                      const sgl_shape& sh, int B, double* hist, hipStream_t st) {
   hipLaunchKernelGGL(k_bin_band, dim3(sh.n_z_max, B), dim3(kBandThreads), 0, st, rows, cosmo, uedge,
                      sh, hist);
   hipLaunchKernelGGL(k_reduce_fine, dim3(B), dim3(256), 0, st, cosmo, sh, hist);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
+int launch_bin(const double* rows, const double* cosmo, const double* uedge, const sgl_shape& sh,
+               int B, double* hist, hipStream_t st) {
+  int rc = launch_bin_pass(rows, cosmo, uedge, sh, B, hist, st);
+  return rc == SGL_OK ? launch_bin_reduce(rows, cosmo, uedge, sh, B, hist, st) : rc;
+}
+
 int launch_lnphi_binned(const double* hist, const double* cosmo, const double* knodes,
-                        const sgl_shape& sh, int B, double* lam, double* amp, hipStream_t st) {
+                        const sgl_shape& sh, int B, double* lam, float* amp, hipStream_t st) {
   if (g_sgl_tune[SGL_TUNE_LNPHI_BLOCK] == 0) {
     dim3 grid((sh.n_k + 127) / 128, B);
     hipLaunchKernelGGL(k_lnphi_binned<128>, grid, dim3(128), 0, st, hist, cosmo, knodes, sh, lam, amp);

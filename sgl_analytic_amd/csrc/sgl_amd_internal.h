@@ -116,9 +116,13 @@ int launch_lnphi_exact(const double* rows, const double* cosmo, const double* ue
                        double* work, hipStream_t st);
 int launch_bin(const double* rows, const double* cosmo, const double* uedge, const sgl_shape& sh,
                int B, double* hist, hipStream_t st);
+int launch_bin_pass(const double* rows, const double* cosmo, const double* uedge, const sgl_shape& sh,
+               int B, double* hist, hipStream_t st);
+int launch_bin_reduce(const double* rows, const double* cosmo, const double* uedge, const sgl_shape& sh,
+               int B, double* hist, hipStream_t st);
 int launch_lnphi_binned(const double* hist, const double* cosmo, const double* knodes,
-                        const sgl_shape& sh, int B, double* lam, double* amp, hipStream_t st);
-int launch_quadform(const double* cov, const double* cosmo, const double* amp, int B,
+                        const sgl_shape& sh, int B, double* lam, float* amp, hipStream_t st);
+int launch_quadform(const double* cov, const double* cosmo, const void* amp, int B,
                     const sgl_shape& sh, int prec, double* psi2, hipStream_t st);
 int launch_phi_pack(const double* lam, const double* psi2, const double* knodes,
                     const sgl_shape& sh, int B, int prec, void* out, hipStream_t st);

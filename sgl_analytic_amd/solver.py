@@ -118,7 +118,9 @@ class Solver:
         self.tables = buf(L.sgl_tables_bytes(cs, B))
         self.status = torch.zeros(1, dtype=torch.int32, device=self.dev)
         self.lam = torch.zeros((B, nk, 2), dtype=torch.float64, device=self.dev)
-        self.amp = torch.zeros((B, nz, nk, 2), dtype=torch.float64, device=self.dev)
+        # slice amplitudes: FP64 on the exact path, FP32 on the batched path
+        adt = torch.float64 if path == "exact" else torch.float32
+        self.amp = torch.zeros((B, nz, nk, 2), dtype=adt, device=self.dev)
         self.psi2 = torch.zeros((B, nk, 2), dtype=torch.float64, device=self.dev)
         self.diag = torch.zeros((B, 8), dtype=torch.float64, device=self.dev)
         if path == "exact":
