@@ -73,24 +73,6 @@ __device__ __forceinline__ void gen_cell(const SliceStage& S, int n_b, int c, do
   bw = rc.beta * ws;
 }
 
-// The batched main pass stages the compact FP32 row constants.
-struct SliceStageF {
-  RowF rc[kMaxMasses];
-  double ug[kMaxB];
-  double du2[kMaxB];
-};
-
-__device__ inline void stage_slice_f(SliceStageF& S, const double* __restrict__ rows,
-                                     const double* __restrict__ uedge, const sgl_shape& sh) {
-  for (int j = threadIdx.x; j < sh.n_m; j += blockDim.x)
-    S.rc[j] = row_f(row_const(rows + (size_t)j * kRowStride));
-  for (int l = threadIdx.x; l < sh.n_b; l += blockDim.x) {
-    double u0 = uedge[l], u1 = uedge[l + 1];
-    S.ug[l] = sqrt(u0 * u1);
-    S.du2[l] = u1 * u1 - u0 * u0;
-  }
-}
-
 // ------------------------------------------------------------ exact path --
 constexpr int kCellTile = 256;
 constexpr int kKpt = 2;  // k nodes per thread
@@ -194,11 +176,11 @@ __device__ __forceinline__ int fine_bin(double xi) {
   constexpr float kLnLo = -27.631021115928547f;                          // ln 1e-12
   constexpr float kLnNegLo = -18.420680743952367f;                       // ln 1e-8
   if (xi > 0.0) {
-    float lx = __logf((float)xi);
+    float lx = log_hw(fmaxf((float)xi, 1e-37f));
     int k = (int)((lx - kLnLo) * kPosScale);
     return min(max(k, 0), kFinePos - 1);
   }
-  float lx = __logf((float)(-xi));
+  float lx = log_hw(fmaxf((float)(-xi), 1e-37f));
   int k = (int)((lx - kLnNegLo) * kNegScale);
   return kFinePos + min(max(k, 0), kFineNeg - 1);
 }
@@ -226,48 +208,114 @@ __device__ __forceinline__ void deposit(double* lf, double* lc, double xi, doubl
 // three more on coarse bins, which most lanes of a wave share.
 constexpr int kFineMom = 4;
 
+// The four moments are four arrays (moment-major), so a bin's moment sits at
+// an 8-byte stride and the lanes of a wave spread over all LDS banks.
 __device__ __forceinline__ void deposit4(double* lf, double xi, double ws, double bw) {
-  double* q = lf + fine_bin(xi) * kFineMom;
+  double* q = lf + fine_bin(xi);
   atomicAdd(q, ws);
-  atomicAdd(q + 1, ws * xi);
-  atomicAdd(q + 2, bw);
-  atomicAdd(q + 3, bw * xi);
+  atomicAdd(q + kFineBins, ws * xi);
+  atomicAdd(q + 2 * kFineBins, bw);
+  atomicAdd(q + 3 * kFineBins, bw * xi);
+}
+
+// Per impact-parameter column (shared by every instance): geometric bin
+// centre u_g in FP32 with its log and reciprocal, and u1^2 - u0^2 in FP64.
+struct ColF {
+  float ug, lug, rug, pad;
+  double du2, pad2;
+};
+static_assert(sizeof(ColF) == 32, "ColF is one 32-byte record");
+
+__global__ void k_cols(const double* __restrict__ uedge, sgl_shape sh, int B,
+                       double* __restrict__ hist) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= sh.n_b) return;
+  const double u0 = uedge[l], u1 = uedge[l + 1];
+  const double ug = sqrt(u0 * u1);
+  ColF c;
+  c.ug = (float)ug;
+  c.lug = (float)log(ug);
+  c.rug = (float)(1.0 / ug);
+  c.pad = 0.0f;
+  c.du2 = u1 * u1 - u0 * u0;
+  c.pad2 = 0.0;
+  reinterpret_cast<ColF*>(hist + hist_layout(sh, B).cols)[l] = c;
+}
+
+// One thread per (instance, slice, mass) row: the 64-byte RowF record of the
+// main pass, with the band columns found by bisection on the same FP64
+// predicate the band pass would evaluate, so the two passes split every row
+// exactly.  (1 - x^2) decreases along the row (x = ug_l s increases with l).
+__global__ __launch_bounds__(256) void k_rowf(const double* __restrict__ rows,
+                                              const double* __restrict__ cosmo,
+                                              const double* __restrict__ uedge, sgl_shape sh,
+                                              int B, double* __restrict__ hist) {
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t nrow = (size_t)B * sh.n_z_max * sh.n_m;
+  if (gid >= nrow) return;
+  const int b = (int)(gid / ((size_t)sh.n_z_max * sh.n_m));
+  const int i = (int)((gid / sh.n_m) % sh.n_z_max);
+  if (i >= (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10]) return;
+  const RowConst c = row_const(rows + gid * kRowStride);
+  RowF f;
+  f.warea = (c.valid != 0.0) ? c.warea : 0.0;
+  f.beta = c.beta;
+  f.ks = (float)c.ks; f.xt = (float)c.xt; f.ixt = (float)(1.0 / c.xt); f.ic1 = (float)c.ic1;
+  f.mut4ks = (float)c.mut4ks; f.i0 = (float)c.i0; f.jj = (float)c.jj;
+  f.s = (float)c.s; f.log_s = (float)log(c.s); f.inv_s = (float)(1.0 / c.s);
+  auto wcol = [&](int l) {
+    const double x = sqrt(uedge[l] * uedge[l + 1]) * c.s;
+    return (1.0 - x) * (1.0 + x);
+  };
+  auto first = [&](double thr, bool strict) {   // first l with w(l) < thr (or <= thr)
+    int a = 0, z = sh.n_b;
+    while (a < z) {
+      const int m = (a + z) >> 1;
+      const double w = wcol(m);
+      if (strict ? (w < thr) : (w <= thr)) z = m; else a = m + 1;
+    }
+    return a;
+  };
+  f.la = first(kNfwWSeries, true);
+  f.lb = first(-kNfwWSeries, false);
+  reinterpret_cast<RowF*>(hist + hist_layout(sh, B).rowf)[gid] = f;
 }
 
 // Main pass: every cell outside the band, FP32 lens profile, FP32 log for
-// the jump.  A thread owns one column l (impact parameter) at a time and
-// walks the masses, so the column geometry stays in registers and the row
-// constants are wave-uniform LDS reads.
+// the jump.  A thread owns one mass row (its RowF record in registers) and
+// walks the impact-parameter columns; the lanes of a wave are consecutive
+// masses at the same column, which keeps the x >= x_t and x < 1 branches
+// nearly uniform (x / x_t = u r_vir / r_t hardly depends on the mass) and
+// spreads the lanes over more histogram bins than consecutive columns do.
 template <int kBinThreads>
-__global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restrict__ rows,
-                                                           const double* __restrict__ cosmo,
+__global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restrict__ cosmo,
                                                            const double* __restrict__ uedge,
                                                            sgl_shape sh, double* __restrict__ hist) {
-  __shared__ SliceStageF S;
   __shared__ double lf[kFineBins * kFineMom];
   const int t = threadIdx.x, i = blockIdx.x, b = blockIdx.y;
   const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
   if (i >= n_z) return;                         // slices past n_z are never read
   const HistLayout HL = hist_layout(sh, gridDim.y);
   for (int q = t; q < kFineBins * kFineMom; q += kBinThreads) lf[q] = 0.0;
-  stage_slice_f(S, rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh);
   __syncthreads();
+  const RowF* RF = reinterpret_cast<const RowF*>(hist + HL.rowf) +
+                   ((size_t)b * sh.n_z_max + i) * sh.n_m;
+  const ColF* CF = reinterpret_cast<const ColF*>(hist + HL.cols);
   const int n_b = sh.n_b;
-  const int cols = min(n_b, kBinThreads);
-  const int rows_par = (n_b <= kBinThreads) ? kBinThreads / n_b : 1;
-  const int tr = t / cols, tc = t - tr * cols;
-  if (tr < rows_par) {
-    for (int l = tc; l < n_b; l += cols) {
-      const double ug = S.ug[l], du2 = S.du2[l];
-      for (int j = tr; j < sh.n_m; j += rows_par) {
-        const RowF& rc = S.rc[j];
-        if (rc.valid == 0.0f) continue;
-        const double x = ug * rc.s;
-        if (in_band(x)) continue;               // the FP64 band pass takes it
+  const int rpp = min(sh.n_m, kBinThreads);     // rows per pass
+  const int parts = kBinThreads / rpp;          // column phases
+  const int tj = t % rpp, part = t / rpp;
+  if (part < parts) {
+    for (int j = tj; j < sh.n_m; j += rpp) {
+      const RowF rc = RF[j];
+      if (rc.warea == 0.0) continue;
+      for (int l = part; l < n_b; l += parts) {
+        if (l >= rc.la && l < rc.lb) continue;  // the FP64 band pass takes it
+        const ColF cf = CF[l];
         float kf, gf;
-        trunc_kg_row_f32((float)x, rc, kf, gf);
+        trunc_kg_row_f32(cf.ug * rc.s, cf.lug + rc.log_s, cf.rug * rc.inv_s, rc, kf, gf);
         double xi, ws;
-        cell_jump_f32log(kf, gf, rc.warea * du2, xi, ws);
+        cell_jump_f32log(kf, gf, rc.warea * cf.du2, xi, ws);
         if (ws == 0.0) continue;
         deposit4(lf, xi, ws, rc.beta * ws);
       }
@@ -276,14 +324,14 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
   __syncthreads();
   double* P = hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments;
   for (int f = t; f < kFineBins; f += kBinThreads) {
-    P[f * kMoments] = lf[f * kFineMom];
-    P[f * kMoments + 1] = lf[f * kFineMom + 1];
+    P[f * kMoments] = lf[f];
+    P[f * kMoments + 1] = lf[kFineBins + f];
   }
   double* Q = hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3;
   for (int cb = t; cb < kSliceBins; cb += kBinThreads) {
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;
     for (int f = cb * kCoarse; f < (cb + 1) * kCoarse; ++f) {
-      const double bw = lf[f * kFineMom + 2], bwx = lf[f * kFineMom + 3];
+      const double bw = lf[2 * kFineBins + f], bwx = lf[3 * kFineBins + f];
       if (bw == 0.0) continue;
       m0 += bw;
       m1 += bwx;
@@ -297,12 +345,16 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
 
 // Band pass: the cells with |1 - x^2| < 1/4 in FP64 (series branch of F, h
 // and the |x - 1| < 1e-5 bridge of the truncation term), added into the
-// slice's partials by the only workgroup that touches them.
-constexpr int kBandThreads = 256;
+// slice's partials by the only workgroup that touches them.  The band
+// columns [la, lb) of each row come from its RowF record.  Lanes are rows
+// and walk their bands in step, so a wave's lanes sit at the same offset
+// into their bands and the x < 1 / x > 1 branches stay nearly uniform.
+constexpr int kBandThreads = 512;
 __global__ __launch_bounds__(kBandThreads) void k_bin_band(const double* __restrict__ rows,
                                                            const double* __restrict__ cosmo,
                                                            const double* __restrict__ uedge,
-                                                           sgl_shape sh, double* __restrict__ hist) {
+                                                           sgl_shape sh,
+                                                           double* __restrict__ hist) {
   __shared__ SliceStage S;
   __shared__ double lf[kFineBins * kMoments];
   __shared__ double lc[kSliceBins * 3];
@@ -314,21 +366,23 @@ __global__ __launch_bounds__(kBandThreads) void k_bin_band(const double* __restr
   for (int q = t; q < kSliceBins * 3; q += kBandThreads) lc[q] = 0.0;
   stage_slice(S, rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh);
   __syncthreads();
-  // band columns of row j: ug_l in [sqrt(0.75)/s, sqrt(1.25)/s]; ug is increasing
-  for (int j = t; j < sh.n_m; j += kBandThreads) {
-    const RowConst& rc = S.rc[j];
-    if (rc.valid == 0.0) continue;
-    const double lo = sqrt(0.75) / rc.s, hi = sqrt(1.25) / rc.s;
-    int a = 0, z = sh.n_b;                    // first l with ug_l >= lo
-    while (a < z) { int m = (a + z) >> 1; if (S.ug[m] < lo) a = m + 1; else z = m; }
-    for (int l = (a > 0 ? a - 1 : 0); l < sh.n_b && S.ug[l] <= hi * 1.0000001; ++l) {
-      const double x = S.ug[l] * rc.s;
-      if (!in_band(x)) continue;
-      double kap, gam, xi, ws;
-      trunc_kg_row(x, rc, kap, gam);
-      cell_jump_f32log(kap, gam, rc.warea * S.du2[l], xi, ws);
-      if (ws == 0.0) continue;
-      deposit(lf, lc, xi, ws, rc.beta * ws);
+  const RowF* RF = reinterpret_cast<const RowF*>(hist + HL.rowf) +
+                   ((size_t)b * sh.n_z_max + i) * sh.n_m;
+  const int rpp = min(sh.n_m, kBandThreads);    // rows per pass
+  const int parts = kBandThreads / rpp;         // column phases
+  const int tj = t % rpp, part = t / rpp;
+  if (part < parts) {
+    for (int j = tj; j < sh.n_m; j += rpp) {
+      const RowF& rf = RF[j];
+      if (rf.warea == 0.0) continue;
+      const RowConst& rc = S.rc[j];
+      for (int l = rf.la + part; l < rf.lb; l += parts) {
+        double kap, gam, xi, ws;
+        trunc_kg_row(S.ug[l] * rc.s, rc, kap, gam);
+        cell_jump_f32log(kap, gam, rc.warea * S.du2[l], xi, ws);
+        if (ws == 0.0) continue;
+        deposit(lf, lc, xi, ws, rc.beta * ws);
+      }
     }
   }
   __syncthreads();
@@ -641,11 +695,15 @@ int launch_lnphi_exact(const double* rows, const double* cosmo, const double* ue
 
 int launch_bin_pass(const double* rows, const double* cosmo, const double* uedge,
                     const sgl_shape& sh, int B, double* hist, hipStream_t st) {
+  const size_t nrow = (size_t)B * sh.n_z_max * sh.n_m;
+  hipLaunchKernelGGL(k_rowf, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, st, rows, cosmo,
+                     uedge, sh, B, hist);
+  hipLaunchKernelGGL(k_cols, dim3((sh.n_b + 255) / 256), dim3(256), 0, st, uedge, sh, B, hist);
   const dim3 g(sh.n_z_max, B);
   switch (g_sgl_tune[SGL_TUNE_BIN_THREADS]) {
-    case 0: hipLaunchKernelGGL(k_bin_cells<256>, g, dim3(256), 0, st, rows, cosmo, uedge, sh, hist); break;
-    case 2: hipLaunchKernelGGL(k_bin_cells<1024>, g, dim3(1024), 0, st, rows, cosmo, uedge, sh, hist); break;
-    default: hipLaunchKernelGGL(k_bin_cells<512>, g, dim3(512), 0, st, rows, cosmo, uedge, sh, hist); break;
+    case 0: hipLaunchKernelGGL(k_bin_cells<256>, g, dim3(256), 0, st, cosmo, uedge, sh, hist); break;
+    case 2: hipLaunchKernelGGL(k_bin_cells<1024>, g, dim3(1024), 0, st, cosmo, uedge, sh, hist); break;
+    default: hipLaunchKernelGGL(k_bin_cells<512>, g, dim3(512), 0, st, cosmo, uedge, sh, hist); break;
   }
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }

@@ -80,10 +80,11 @@ constexpr int kCoarsePos = kFinePos / kCoarse;
 // Histogram buffer (doubles): per-slice partial fine moments, compacted fine
 // records + count per instance, compacted coarse records + count per slice,
 // per instance the bin -> first-record map (kFineBins + 1 entries) and the
-// prefix moments of the positive records ((kFinePos + 1) x kTay), and the
-// same two per slice for the positive coarse records.
+// prefix moments of the positive records ((kFinePos + 1) x kTay), the same
+// two per slice for the positive coarse records, the 64-byte RowF record of
+// every (slice, mass) row, and the 32-byte record of every column.
 struct HistLayout {
-  size_t partial, cpart, frec, fcnt, srec, scnt, fpos, fpfx, cpos, cpfx, total;
+  size_t partial, cpart, frec, fcnt, srec, scnt, fpos, fpfx, cpos, cpfx, rowf, cols, total;
 };
 __host__ __device__ inline size_t round32(size_t n) { return (n + 31) & ~size_t(31); }
 __host__ __device__ inline HistLayout hist_layout(const sgl_shape& s, int B) {
@@ -98,7 +99,9 @@ __host__ __device__ inline HistLayout hist_layout(const sgl_shape& s, int B) {
   L.fpfx = L.fpos + round32((size_t)B * (kFineBins + 1));
   L.cpos = L.fpfx + round32((size_t)B * (kFinePos + 1) * kTay);
   L.cpfx = L.cpos + round32((size_t)B * s.n_z_max * (kCoarsePos + 1));
-  L.total = L.cpfx + round32((size_t)B * s.n_z_max * (kCoarsePos + 1) * kTayC);
+  L.rowf = L.cpfx + round32((size_t)B * s.n_z_max * (kCoarsePos + 1) * kTayC);
+  L.cols = L.rowf + round32((size_t)B * s.n_z_max * s.n_m * 8);
+  L.total = L.cols + round32((size_t)s.n_b * 4);
   return L;
 }
 

@@ -333,23 +333,22 @@ __device__ inline void trunc_kg_row(double x, const RowConst& rc, double& kap, d
   gam = rc.ks * (kb - ku);
 }
 
-// Per-row constants of the batched main pass: FP32 profile constants, FP64
-// where the band test and the cell weights need it.
+// Per-row record of the batched main pass (64 B, built once per row by
+// k_rowf): FP64 weight and bias, FP32 profile constants with log s and 1/s
+// (so that log x and 1/x of a cell are one add and one multiply), and the
+// columns [la, lb) of the row inside the band |1 - x^2| < 1/4, which the FP64
+// band pass takes.  warea = 0 marks an empty row.
 struct RowF {
-  double s, warea, beta;
-  float ks, xt, ixt, ic1, mut4ks, i0, jj, valid;
+  double warea, beta;
+  float ks, xt, ixt, ic1, mut4ks, i0, jj, s, log_s, inv_s;
+  int la, lb;
 };
+static_assert(sizeof(RowF) == 64, "RowF is one 64-byte record");
 
-__device__ inline RowF row_f(const RowConst& c) {
-  RowF f;
-  f.s = c.s; f.warea = c.warea; f.beta = c.beta;
-  f.ks = (float)c.ks; f.xt = (float)c.xt; f.ixt = (float)(1.0 / c.xt); f.ic1 = (float)c.ic1;
-  f.mut4ks = (float)c.mut4ks; f.i0 = (float)c.i0; f.jj = (float)c.jj; f.valid = (float)c.valid;
-  return f;
-}
-
-// Hardware FP32 reciprocal, reciprocal square root and square root (1 ulp).
+// Hardware FP32 reciprocal, reciprocal square root, square root (1 ulp) and
+// natural log (v_log_f32 is log2; positive normal arguments only).
 __device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float log_hw(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
 __device__ __forceinline__ float rsq_hw(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 // log1p(y), y > -1, from the hardware log: with u = fl(1 + y),
@@ -358,7 +357,7 @@ __device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrt
 // arithmetic", Theorem 4), leaving a few ulp.
 __device__ __forceinline__ float log1p_hw(float y) {
   const float u = 1.0f + y, d = u - 1.0f;
-  return (d == 0.0f) ? y : __logf(u) * (y * rcp_hw(d));
+  return (d == 0.0f) ? y : log_hw(u) * (y * rcp_hw(d));
 }
 
 // FP32 evaluation of the same closed forms for the batched path, used away
@@ -367,9 +366,10 @@ __device__ __forceinline__ float log1p_hw(float y) {
 // small-x form of h applies on the whole x < 1 side.  Hardware rcp / rsq /
 // sqrt / log; relative error of kappa and gamma ~1e-6, far below the lattice
 // binning error.  The caller takes the FP64 trunc_kg_row inside the band.
-__device__ inline void trunc_kg_row_f32(float x, const RowF& rc, float& kap, float& gam) {
+__device__ inline void trunc_kg_row_f32(float x, float lx, float rx, const RowF& rc, float& kap,
+                                        float& gam) {
   const float xt = rc.xt;
-  const float rx = rcp_hw(x), rx2 = rx * rx;
+  const float rx2 = rx * rx;
   if (x >= xt) {
     kap = 0.0f;
     gam = rc.mut4ks * rx2;
@@ -377,7 +377,6 @@ __device__ inline void trunc_kg_row_f32(float x, const RowF& rc, float& kap, flo
   }
   constexpr float kLn2f = 0.69314718f;
   const float a = x * rc.ixt;
-  const float lx = __logf(x);
   const float w = (1.0f - x) * (1.0f + x);
   const float ic1 = rc.ic1;
   float F, h, I1, I2;
@@ -389,14 +388,14 @@ __device__ inline void trunc_kg_row_f32(float x, const RowF& rc, float& kap, flo
   if (x < 1.0f) {
     const float isu = rsq_hw(w), su = w * isu, iu = isu * isu;
     const float e = (1.0f - x) * rx;
-    const float A0 = __logf(1.0f + e + sqrt_hw(e * (e + 2.0f)));   // arccosh(1/x), 1/x - 1 >= 0.15
+    const float A0 = log_hw(1.0f + e + sqrt_hw(e * (e + 2.0f)));   // arccosh(1/x), 1/x - 1 >= 0.15
     const float i1s = rcp_hw(1.0f + su);
     F = (A0 * isu - 1.0f) * iu;
     h = ((kLn2f - lx) * x * x * i1s + log1p_hw(-0.5f * x * x * i1s)) * isu;
     if (!small) {
       const float c = xt, q = sqrt_hw((c - x) * (c + x));
       const float d = (x - 1.0f) * (x - c) * rx * ic1;
-      const float L = __logf((c + q) * ic1 * 0.5f) - (q - c) * ic1;
+      const float L = log_hw((c + q) * ic1 * 0.5f) - (q - c) * ic1;
       const float At = log1p_hw(d + sqrt_hw(d * (d + 2.0f)));
       I1 = (q * ic1 - 1.0f) * iu + (A0 - At) * iu * isu;
       I2 = (L + (A0 - At) * isu) * rx2;
@@ -409,7 +408,7 @@ __device__ inline void trunc_kg_row_f32(float x, const RowF& rc, float& kap, flo
     if (!small) {
       const float c = xt, q = sqrt_hw((c - x) * (c + x));
       const float d = (x - 1.0f) * (x - c) * rx * ic1;
-      const float L = __logf((c + q) * ic1 * 0.5f) - (q - c) * ic1;
+      const float L = log_hw((c + q) * ic1 * 0.5f) - (q - c) * ic1;
       const float Bt = 2.0f * asinf(sqrt_hw(fmaxf(-d, 0.0f) * 0.5f));
       I1 = (1.0f - q * ic1) * iv - (B0 - Bt) * iv * isv;
       I2 = (L + (B0 - Bt) * isv) * rx2;
@@ -437,7 +436,7 @@ __device__ __forceinline__ void cell_jump_f32log(double kap, double gam, double 
   if (!(detA > 0.0)) { xi = 0.0; ws = 0.0; return; }
   float fd = (float)dm1;
   float l1p = (fabsf(fd) < 1e-3f) ? fd * (1.0f - fd * (0.5f - fd * (1.0f / 3.0f)))
-                                  : __logf((float)detA);
+                                  : log_hw(fmaxf((float)detA, 1e-30f));
   double x = -(double)l1p;
   if (x > kXiMax) { xi = 0.0; ws = 0.0; return; }
   xi = x;
