@@ -4,23 +4,29 @@
 //   a_i(k)    = sum_{c in slice i} beta_c w_c (e^{i k xi_c} - 1)
 //
 // Cells are never stored: every workgroup regenerates the cells of its slice
-// from the (slice, mass) rows -- per-row constants and the impact-parameter
-// grid in LDS, truncated NFW (kappa, gamma), jump xi = -ln detA, tilted
-// weight w detA -- and streams them against its k nodes or its histogram.
+// from the (slice, mass) rows -- truncated NFW (kappa, gamma), jump
+// xi = -ln detA, tilted weight w detA -- and streams them against its k nodes
+// or its histogram.
 //
 // Exact path (k_lnphi_exact): every cell, FP64 sincos, the reference's
 // cancellation-free forms cos - 1 = -2 sin^2(ph/2), sin - ph by series below
 // |ph| = 1e-2 (sgl_full.py:2599-2613).
-// Binned path: k_bin_cells histograms each slice's cells on a signed-log jump
-// lattice in LDS (w, w xi per fine bin; beta w, beta w xi, beta w xi^2 per
-// coarse bin of 32 fine ones) with the FP32 lens profile, k_bin_band adds the
-// cells of the band |1 - x^2| < 1/4 with the FP64 profile, both into the
-// slice's own partials; k_reduce_fine adds the partials in slice order (no
-// atomics across workgroups: reproducible) and compacts the non-empty bins
-// of the instance and of every slice; k_lnphi_binned
-// evaluates each bin as w e^{i k xibar - k^2 v / 2}, v the in-bin variance,
-// phase in FP64 reduced to revolutions, sin/cos on the FP32 hardware path,
-// compensated small-phase terms and accumulation in FP64.
+// Binned path:
+//   k_rowf / k_cols   per-row and per-column records of the slice lattice;
+//   k_bin_cells       every cell off the band |1 - x^2| < kBandW with the FP32
+//                     lens profile into a per-slice LDS histogram of fine
+//                     signed-log jump bins (w, w xi, beta w, beta w xi), then
+//                     the slice's fine partials and its coarse two-point
+//                     moments (beta w, beta w xi, beta w xi^2 per 32 fine bins);
+//   k_bin_band        the band cells with the FP64 profile, added into the
+//                     slice's partials;
+//   k_reduce_fine     slice-order sum of the partials (no atomics across
+//                     workgroups: reproducible), compaction, prefix moments;
+//   k_lnphi_binned    per k: records with k xi <= 1/2 through Taylor series
+//                     on prefix moments, the rest as w e^{i k xibar - k^2 v/2}
+//                     with the phase in FP64 reduced to revolutions, sin/cos
+//                     on the FP32 hardware path, compensated small-phase terms
+//                     and accumulation in FP64.
 #include "sgl_amd_internal.h"
 
 namespace sgl {
@@ -185,10 +191,6 @@ __device__ __forceinline__ int fine_bin(double xi) {
   return kFinePos + min(max(k, 0), kFineNeg - 1);
 }
 
-// Cells whose profile is evaluated in FP64 on the batched path: the band
-// |1 - x^2| < 1/4 around the NFW branch point.  Row j covers it for columns
-// l with ug_l s_j inside the band; everything else takes the FP32 profile.
-__device__ __forceinline__ bool in_band(double x) { return fabs((1.0 - x) * (1.0 + x)) < kNfwWSeries; }
 
 __device__ __forceinline__ void deposit(double* lf, double* lc, double xi, double ws, double bw) {
   const int f = fine_bin(xi);
@@ -276,8 +278,8 @@ __global__ __launch_bounds__(256) void k_rowf(const double* __restrict__ rows,
     }
     return a;
   };
-  f.la = first(kNfwWSeries, true);
-  f.lb = first(-kNfwWSeries, false);
+  f.la = first(kBandW, true);
+  f.lb = first(-kBandW, false);
   reinterpret_cast<RowF*>(hist + hist_layout(sh, B).rowf)[gid] = f;
 }
 
@@ -309,9 +311,11 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
     for (int j = tj; j < sh.n_m; j += rpp) {
       const RowF rc = RF[j];
       if (rc.warea == 0.0) continue;
+      ColF nxt = CF[min(part, n_b - 1)];
       for (int l = part; l < n_b; l += parts) {
+        const ColF cf = nxt;                     // the next column is fetched a cell ahead
+        nxt = CF[min(l + parts, n_b - 1)];
         if (l >= rc.la && l < rc.lb) continue;  // the FP64 band pass takes it
-        const ColF cf = CF[l];
         float kf, gf;
         trunc_kg_row_f32(cf.ug * rc.s, cf.lug + rc.log_s, cf.rug * rc.inv_s, rc, kf, gf);
         double xi, ws;
@@ -343,7 +347,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
   }
 }
 
-// Band pass: the cells with |1 - x^2| < 1/4 in FP64 (series branch of F, h
+// Band pass: the cells with |1 - x^2| < kBandW in FP64 (series branch of F, h
 // and the |x - 1| < 1e-5 bridge of the truncation term), added into the
 // slice's partials by the only workgroup that touches them.  The band
 // columns [la, lb) of each row come from its RowF record.  Lanes are rows

@@ -336,8 +336,8 @@ __device__ inline void trunc_kg_row(double x, const RowConst& rc, double& kap, d
 // Per-row record of the batched main pass (64 B, built once per row by
 // k_rowf): FP64 weight and bias, FP32 profile constants with log s and 1/s
 // (so that log x and 1/x of a cell are one add and one multiply), and the
-// columns [la, lb) of the row inside the band |1 - x^2| < 1/4, which the FP64
-// band pass takes.  warea = 0 marks an empty row.
+// columns [la, lb) of the row inside the band |1 - x^2| < kBandW, which the
+// FP64 band pass takes.  warea = 0 marks an empty row.
 struct RowF {
   double warea, beta;
   float ks, xt, ixt, ic1, mut4ks, i0, jj, s, log_s, inv_s;
@@ -361,11 +361,12 @@ __device__ __forceinline__ float log1p_hw(float y) {
 }
 
 // FP32 evaluation of the same closed forms for the batched path, used away
-// from x = 1 only (|1 - x^2| >= 1/4, so x <= 0.866 or x >= 1.118): there
-// none of the terms cancels badly, 1 - x^2 and x^2 - 1 are >= 1/4, and the
-// small-x form of h applies on the whole x < 1 side.  Hardware rcp / rsq /
-// sqrt / log; relative error of kappa and gamma ~1e-6, far below the lattice
-// binning error.  The caller takes the FP64 trunc_kg_row inside the band.
+// from x = 1 only (|1 - x^2| >= kBandW = 0.1, so x <= 0.949 or x >= 1.049):
+// the terms that cancel as x -> 1 lose at most ~1/|1 - x^2|^{3/2} ~ 30 ulp
+// there.  Hardware rcp / rsq / sqrt / log; relative error of kappa and gamma
+// ~2e-6, far below the lattice binning error.  The caller takes the FP64
+// trunc_kg_row inside the band.
+constexpr double kBandW = 0.1;
 __device__ inline void trunc_kg_row_f32(float x, float lx, float rx, const RowF& rc, float& kap,
                                         float& gam) {
   const float xt = rc.xt;
@@ -388,10 +389,14 @@ __device__ inline void trunc_kg_row_f32(float x, float lx, float rx, const RowF&
   if (x < 1.0f) {
     const float isu = rsq_hw(w), su = w * isu, iu = isu * isu;
     const float e = (1.0f - x) * rx;
-    const float A0 = log_hw(1.0f + e + sqrt_hw(e * (e + 2.0f)));   // arccosh(1/x), 1/x - 1 >= 0.15
-    const float i1s = rcp_hw(1.0f + su);
+    const float A0 = log_hw(1.0f + e + sqrt_hw(e * (e + 2.0f)));   // arccosh(1/x), 1/x - 1 >= 0.05
     F = (A0 * isu - 1.0f) * iu;
-    h = ((kLn2f - lx) * x * x * i1s + log1p_hw(-0.5f * x * x * i1s)) * isu;
+    if (x < (float)kNfwXH) {
+      const float i1s = rcp_hw(1.0f + su);
+      h = ((kLn2f - lx) * x * x * i1s + log1p_hw(-0.5f * x * x * i1s)) * isu;
+    } else {
+      h = lx - kLn2f + A0 * isu;
+    }
     if (!small) {
       const float c = xt, q = sqrt_hw((c - x) * (c + x));
       const float d = (x - 1.0f) * (x - c) * rx * ic1;

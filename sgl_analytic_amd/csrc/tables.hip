@@ -10,6 +10,8 @@
 // (pipeline.py:36-72), the builder's per-(z, M) quantities
 // (sgl_full.py:1848-1885, 541-543; subhalos.py:381-385, 472-481) and
 // shell_covariance (sgl_full.py:2514-2538).
+#include <cmath>
+
 #include "sgl_amd_internal.h"
 
 namespace sgl {
@@ -251,71 +253,87 @@ __device__ inline double eta_rvir(double c, double Dvir) {
     double fp = 3.0 / m - dmu / mu;
     double mn = m - f / fp;
     if (!(lo < mn && mn < hi)) mn = 0.5 * (lo + hi);
-    if (fabs(mn - m) <= 4e-16 * m || hi - lo <= 4e-16 * m) { m = mn; break; }
+    // Newton is quadratic here: a step below 1e-13 m leaves an error far below
+    // 1 ulp, and stopping there avoids last-ulp oscillation that would keep a
+    // few lanes (and so their whole wave) iterating
+    if (fabs(mn - m) <= 1e-13 * m || hi - lo <= 4e-16 * m) { m = mn; break; }
     m = mn;
   }
   return m / c;
 }
 
+// One workgroup per (instance, slice), threads over masses: the slice's
+// redshift-only factors (Ludlow+16 parameters, rho_c, Delta_vir) are formed
+// once in LDS.  a_st and dc0 are the Sheth-Tormen normalisation and the
+// bias delta_c constant, computed on the host.
 __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
                                               const Derived* __restrict__ der,
                                               const double* __restrict__ slices,
-                                              const double* __restrict__ mass, int B, int n_z_max,
-                                              int n_m, double dlnM, double* __restrict__ rows) {
-  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t total = (size_t)B * n_z_max * n_m;
-  if (gid >= total) return;
-  const int j = (int)(gid % n_m);
-  const int i = (int)((gid / n_m) % n_z_max);
-  const int b = (int)(gid / ((size_t)n_m * n_z_max));
-  double* R = rows + gid * kRowStride;
+                                              const double* __restrict__ mass, int n_z_max,
+                                              int n_m, double dlnM, double a_st, double dc0,
+                                              double* __restrict__ rows) {
+  __shared__ double sk[10];
+  const int i = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
   const Cosmo c = cosmo[b];
+  double* Rs = rows + ((size_t)b * n_z_max + i) * n_m * kRowStride;
   if (i >= (int)c.n_z) {
-    for (int q = 0; q < kRowStride; ++q) R[q] = 0.0;
+    for (int q = t; q < n_m * kRowStride; q += blockDim.x) Rs[q] = 0.0;
     return;
   }
-  const Derived d = der[b];
   const double* S = slices + ((size_t)b * n_z_max + i) * kSliceStride;
-  const double* Mr = mass + ((size_t)b * n_m + j) * kMassStride;
-  const double z = S[kSlZ], D = S[kSlD], E2 = S[kSlE2];
-  const double M = Mr[0], sig0 = Mr[1], dls = Mr[2];
-  // Sheth-Tormen dn/dlnM (sgl.py:558-567)
-  const double A_ST = 1.0 / (1.0 + pow(2.0, -0.3) * tgamma(0.2) / sqrt(kPi));
-  double sig = sig0 * D;
-  double nu = (kDeltaC / sig) * (kDeltaC / sig);
-  double qn = kQst * nu;
-  double mult = A_ST * (1.0 + pow(qn, -kPst)) * sqrt(qn / (2.0 * kPi)) * exp(-0.5 * qn);
-  double dndlnM = (d.rhom / M) * mult * (-2.0 * dls);
-  // Ludlow+16 concentration with the z <= 7 clamp (sgl.py:590-606)
-  double zc = fmin(z, 7.0), a = 1.0 + zc;
-  double poly = 4.135 - 0.564 * a - 0.210 * a * a + 0.0557 * a * a * a - 0.00348 * a * a * a * a;
-  double q = kDeltaC / (sig0 * poly);
-  double c0 = 3.395 * pow(a, -0.215), beta_ = 0.307 * pow(a, 0.540);
-  double g1 = 0.628 * pow(a, -0.047), g2 = 0.317 * pow(a, -0.893);
-  double C = c0 * pow(q, -g1) * pow(1.0 + pow(q, 1.0 / beta_), -beta_ * (g2 - g1));
-  // nfw_params (sgl.py:608-615)
-  double rhoc = d.rhoc0 * E2;
-  double r200 = cbrt(3.0 * M / (4.0 * kPi * 200.0 * rhoc));
-  double rs = r200 / C;
-  double fC = log1p(C) - C / (1.0 + C);
-  double rhos = (200.0 / 3.0) * rhoc * C * C * C / fC;
-  double ks = rhos * rs / S[kSlScr];
-  // r_vir with the Bryan-Norman Delta_vir at Omega_m(z) (subhalos.py:381-385)
-  double dd = c.Om * (1.0 + z) * (1.0 + z) * (1.0 + z) / E2 - 1.0;
-  double Dvir = 18.0 * kPi * kPi + 82.0 * dd - 39.0 * dd * dd;
-  double rvir = eta_rvir(C, Dvir) * C * rs;
-  // linear halo bias with the growth D (sgl_full.py:541-543)
-  const double dc0 = 0.6 * pow(1.5 * kPi, 2.0 / 3.0);
-  double qn2 = 0.8 * (dc0 / D / sig0) * (dc0 / D / sig0);
-  double bh = 1.0 + (qn2 - 1.0) / dc0 + 0.6 / (dc0 * (1.0 + pow(qn2, 0.3)));
-  R[kRowKs] = ks;
-  R[kRowRs] = rs;
-  R[kRowXt] = C;
-  R[kRowRvir] = rvir;
-  R[kRowW] = S[kSlWz] * dndlnM * dlnM;
-  R[kRowBeta] = D * bh;
-  R[6] = 0.0;
-  R[7] = 0.0;
+  if (t == 0) {
+    const Derived d = der[b];
+    const double z = S[kSlZ], E2 = S[kSlE2];
+    // Ludlow+16 with the z <= 7 clamp (sgl.py:590-606): redshift-only factors
+    const double zc = fmin(z, 7.0), a = 1.0 + zc;
+    sk[0] = 4.135 - 0.564 * a - 0.210 * a * a + 0.0557 * a * a * a - 0.00348 * a * a * a * a;
+    sk[1] = 3.395 * pow(a, -0.215);
+    sk[2] = 0.307 * pow(a, 0.540);
+    sk[3] = 0.628 * pow(a, -0.047);
+    sk[4] = 0.317 * pow(a, -0.893);
+    sk[5] = d.rhoc0 * E2;                       // rho_c(z)
+    // Bryan-Norman Delta_vir at Omega_m(z) (subhalos.py:381-385)
+    const double dd = c.Om * (1.0 + z) * (1.0 + z) * (1.0 + z) / E2 - 1.0;
+    sk[6] = 18.0 * kPi * kPi + 82.0 * dd - 39.0 * dd * dd;
+    sk[7] = d.rhom;
+  }
+  __syncthreads();
+  const double D = S[kSlD];
+  const double poly = sk[0], c0 = sk[1], beta_ = sk[2], g1 = sk[3], g2 = sk[4];
+  const double rhoc = sk[5], Dvir = sk[6], rhom = sk[7];
+  for (int j = t; j < n_m; j += blockDim.x) {
+    double* R = Rs + (size_t)j * kRowStride;
+    const double* Mr = mass + ((size_t)b * n_m + j) * kMassStride;
+    const double M = Mr[0], sig0 = Mr[1], dls = Mr[2];
+    // Sheth-Tormen dn/dlnM (sgl.py:558-567)
+    const double sig = sig0 * D;
+    const double nu = (kDeltaC / sig) * (kDeltaC / sig);
+    const double qn = kQst * nu;
+    const double mult = a_st * (1.0 + pow(qn, -kPst)) * sqrt(qn / (2.0 * kPi)) * exp(-0.5 * qn);
+    const double dndlnM = (rhom / M) * mult * (-2.0 * dls);
+    // concentration
+    const double q = kDeltaC / (sig0 * poly);
+    const double C = c0 * pow(q, -g1) * pow(1.0 + pow(q, 1.0 / beta_), -beta_ * (g2 - g1));
+    // nfw_params (sgl.py:608-615)
+    const double r200 = cbrt(3.0 * M / (4.0 * kPi * 200.0 * rhoc));
+    const double rs = r200 / C;
+    const double fC = log1p(C) - C / (1.0 + C);
+    const double rhos = (200.0 / 3.0) * rhoc * C * C * C / fC;
+    const double ks = rhos * rs / S[kSlScr];
+    // r_vir (subhalos.py:381-385)
+    const double rvir = eta_rvir(C, Dvir) * C * rs;
+    // linear halo bias with the growth D (sgl_full.py:541-543)
+    const double qn2 = 0.8 * (dc0 / D / sig0) * (dc0 / D / sig0);
+    const double bh = 1.0 + (qn2 - 1.0) / dc0 + 0.6 / (dc0 * (1.0 + pow(qn2, 0.3)));
+    R[kRowKs] = ks;
+    R[kRowRs] = rs;
+    R[kRowXt] = C;
+    R[kRowRvir] = rvir;
+    R[kRowW] = S[kSlWz] * dndlnM * dlnM;
+    R[kRowBeta] = D * bh;
+    R[6] = 0.0;
+    R[7] = 0.0;
+  }
 }
 
 // ------------------------------------------------------------ covariance --
@@ -454,9 +472,12 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
   hipLaunchKernelGGL(k_background, dim3(B), dim3(256), 0, st, C, sh.n_z_max, D, slices);
   hipLaunchKernelGGL(k_growth, dim3((B + 63) / 64), dim3(64), 0, st, C, B, sh.n_z_max, D, slices);
   hipLaunchKernelGGL(k_sigma, dim3(B), dim3(256), 0, st, C, m_nodes_h, sh.n_m, D, mass);
-  const size_t nrow = (size_t)B * sh.n_z_max * sh.n_m;
-  hipLaunchKernelGGL(k_rows, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, st, C, D, slices,
-                     mass, B, sh.n_z_max, sh.n_m, dlnM, rows);
+  // Sheth-Tormen normalisation 1 / (1 + 2^-p Gamma(1/2 - p) / sqrt(pi)), p = 0.3, and
+  // the bias delta_c constant 0.6 (3 pi / 2)^(2/3)
+  const double a_st = 1.0 / (1.0 + std::pow(2.0, -kPst) * std::tgamma(0.5 - kPst) / std::sqrt(kPi));
+  const double dc0 = 0.6 * std::pow(1.5 * kPi, 2.0 / 3.0);
+  hipLaunchKernelGGL(k_rows, dim3(sh.n_z_max, B), dim3(256), 0, st, C, D, slices, mass,
+                     sh.n_z_max, sh.n_m, dlnM, a_st, dc0, rows);
   if (sh.flags & SGL_FLAG_CLUSTERING)
     hipLaunchKernelGGL(k_cov, dim3(B), dim3(256), 0, st, C, D, sh.n_z_max, work, status, cov);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
