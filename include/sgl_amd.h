@@ -32,6 +32,8 @@ typedef enum sgl_status {
 /* shape flags */
 #define SGL_FLAG_CLUSTERING 1 /* add the two-point term Psi_2 */
 #define SGL_FLAG_WINDOW 2     /* cubic B-spline window on phi(k) */
+#define SGL_FLAG_FILAMENTS 4  /* add the cylinder-filament population: per (slice, mass)
+                                 120 u bins x 13 orientation nodes (the "fil" ingredient) */
 
 /* precision of the MFMA contractions */
 #define SGL_PREC_FP64 0   /* plain path: FP64 vector ALU */

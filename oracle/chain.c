@@ -13,6 +13,8 @@
  *   oracle_cells      one cell per (slice, mass, impact-parameter bin):
  *                     jump xi = -ln detA (sgl.py:808-815), source-plane
  *                     Esscher tilt w e^{-xi} (sgl.py:949-951)
+ *   oracle_cells_fil  one cell per (slice, mass, u bin, orientation) of the
+ *                     filament population (sgl_full.py:1984-2007)
  *   oracle_lnphi      compensated compound-Poisson exponent
  *                     Lambda(k) = sum_c w_c (e^{ik xi_c} - 1 - ik xi_c)
  *                     with the reference's cancellation-free forms
@@ -205,6 +207,48 @@ void oracle_cells(int64_t n_rows, const double *rows, int64_t n_b, const double 
             if (x_ > XI_MAX) { xi[o] = 0.0; ws[o] = 0.0; continue; }
             xi[o] = x_;
             ws[o] = wrow * area * detA;     /* Esscher tilt e^{-xi} = detA */
+        }
+    }
+}
+
+/*
+ * Filament cells (the filament block of the reference's population builder,
+ * sgl_full.py:1984-2007, with the cylinder profile of filaments.py:54-66):
+ * per row (k0base, rs, w_row, beta) -- kappa amplitude k0 = k0base /
+ * max(2 rs, |cos phi| L), L = 20 rs -- one cell per (u bin, orientation node):
+ * u bins from u_edges[n_u + 1] (first edge 0, centre u/2 there, else the
+ * geometric centre), orientations phi[n_phi] with weights wphi[n_phi];
+ * kappa = 2 pi k0 / s (u <= 1), gamma = 4 pi k0 / (s + 1) - kappa,
+ * s = sqrt(1 + u^2); weight w_row pi rs^2 (u1^2 - u0^2) wphi; same jump rule
+ * and tilt as the halo cells.  Output cell order: row, u bin, orientation.
+ */
+void oracle_cells_fil(int64_t n_rows, const double *rows, int64_t n_u, const double *u_edges,
+                      int64_t n_phi, const double *phi, const double *wphi, double *xi,
+                      double *ws, double *kap, double *gam) {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t r = 0; r < n_rows; ++r) {
+        const double *R = rows + 4 * r;
+        double k0base = R[0], rs = R[1], wrow = R[2];
+        for (int64_t l = 0; l < n_u; ++l) {
+            double u0 = u_edges[l], u1 = u_edges[l + 1];
+            double uc = (u0 > 0.0) ? sqrt(u0 * u1) : 0.5 * u1;
+            double s = sqrt(1.0 + uc * uc);
+            double area = M_PI * rs * rs * (u1 * u1 - u0 * u0);
+            for (int64_t n = 0; n < n_phi; ++n) {
+                double Lf = 20.0 * rs, c = fabs(cos(phi[n])) * Lf;
+                double k0 = k0base / (2.0 * rs > c ? 2.0 * rs : c);
+                double k = (uc <= 1.0) ? 2.0 * M_PI * k0 / s : 0.0;
+                double g = 4.0 * M_PI * k0 / (s + 1.0) - k;
+                int64_t o = (r * n_u + l) * n_phi + n;
+                kap[o] = k;
+                gam[o] = g;
+                double detA = (1.0 - k) * (1.0 - k) - g * g;
+                if (!(detA > 0.0)) { xi[o] = 0.0; ws[o] = 0.0; continue; }
+                double x_ = -log(detA);
+                if (x_ > XI_MAX) { xi[o] = 0.0; ws[o] = 0.0; continue; }
+                xi[o] = x_;
+                ws[o] = wrow * area * wphi[n] * detA;
+            }
         }
     }
 }

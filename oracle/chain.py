@@ -16,6 +16,8 @@ The chain (one cosmology, one source redshift):
    cell at the geometric centre of its bin (sgl_full.py:1905), weight = row
    weight x annulus area, kappa/gamma of the truncated NFW, jump
    xi = -ln detA and the source-plane tilt w e^{-xi} (sgl.py:808-815,949-951);
+   with ``filaments``, a second population per (slice, mass): cylinder
+   filaments on 120 u bins x 13 orientation nodes (sgl_full.py:1984-2007);
 4. Lambda(k) compensated, a_i(k) per slice (oracle/chain.c);
 5. Psi_2(k) = 1/2 sum_ij C_ij a_i(k) a_j(k), C_ij the slice covariance of the
    smoothed 1-D linear density field (sgl_full.py:2514-2538 restated on these
@@ -56,6 +58,7 @@ def lib():
         i64 = ctypes.c_int64
         p64 = ctypes.POINTER(ctypes.c_int64)
         _LIB.oracle_cells.argtypes = [i64, d, i64, d, d, d, d, d]
+        _LIB.oracle_cells_fil.argtypes = [i64, d, i64, d, i64, d, d, d, d, d, d]
         _LIB.oracle_lnphi.argtypes = [i64, p64, d, d, d, i64, d, d, d]
         _LIB.oracle_invert.argtypes = [i64, d, i64, d, d, d]
         _LIB.oracle_filon_weights.argtypes = [i64, d, i64, d, d]
@@ -87,6 +90,7 @@ class Shape:
     out_hi: float = 3.0
     clustering: bool = True
     window: bool = True
+    filaments: bool = False
 
     def k_nodes(self):
         """k_q: 0 followed by n_k log-spaced nodes."""
@@ -150,6 +154,80 @@ def rows(cos: Cosmology, zs: float, shape: Shape):
             R[i, j] = (ks, rs, C, rvir, wz * dn, beta)
     return dict(rows=R, z=z, chi=chi_mid, dchi=dchi, chi_edges=chi_edges, M=M, dlnM=dlnM,
                 chis=cos.chi(zs))
+
+
+# ----------------------------------------------------------- filaments --
+FIL_U = 120      # u bins of a filament row, log-uniform from 1e-5 to 1 (sgl_full.py:1993)
+FIL_RC = 0.1     # 2 r_s / L: the kink of the orientation rule (sgl_full.py:1828-1845)
+P_FIL, Q_FIL, A_FIL = 0.0, 0.7, 0.5   # flat barrier of the filament mass function (filaments.py:39-50)
+
+
+def fil_phi_rule(n: int = 12, rc: float = FIL_RC):
+    """sgl_full.py:1828-1845: n Gauss-Legendre nodes on [0, acos rc] (where
+    k0 depends on phi) plus one node carrying the clamped measure; weights
+    are fractions of the quarter turn and sum to 1."""
+    pc = math.acos(rc)
+    x, w = np.polynomial.legendre.leggauss(n)
+    ph = 0.5 * pc * (x + 1)
+    wt = 0.5 * pc * w / (0.5 * math.pi)
+    return (np.concatenate([ph, [0.5 * (pc + 0.5 * math.pi)]]),
+            np.concatenate([wt, [(0.5 * math.pi - pc) / (0.5 * math.pi)]]))
+
+
+def fil_u_edges():
+    """[0, geomspace(1e-5, 1, FIL_U)] (sgl_full.py:1993): FIL_U bins, the
+    first a disc of radius 1e-5."""
+    return np.concatenate([[0.0], np.geomspace(1e-5, 1.0, FIL_U)])
+
+
+def dndlnM_fil(cos: Cosmology, M, z):
+    """filaments.py:43-50: the Sheth-Tormen form with (p, q, A) = (0, 0.7, 1/2)."""
+    from .cosmo import DC
+    nu = (DC / cos.sigmaM(M, z)) ** 2
+    qn = Q_FIL * nu
+    mult = A_FIL * 2.0 * math.sqrt(qn / (2 * math.pi)) * math.exp(-qn / 2)
+    return (cos.rhom / M) * mult * (-2 * cos.dlnsig_dlnM(M))
+
+
+def fil_bias(cos: Cosmology, M, z):
+    """sgl_full.py:546-548: flat-barrier peak-background bias (q = 0.7)."""
+    from .cosmo import DC0
+    qn2 = 0.7 * (DC0 / cos.D(z) / cos.sigmaM(M, 0.0)) ** 2
+    return 1 + (qn2 - 1) / DC0
+
+
+def fil_rows(cos: Cosmology, zs: float, shape: Shape):
+    """Per-(slice, mass) filament rows [n_z, n_m, 4] = (k0base, r_s, w_row, beta):
+    r_s = (M / 1e14 Msun)^(1/3) Mpc, L = 20 r_s, k0base = r_s 14.4 rho_c0 L /
+    Sigma_cr (k0 = k0base / max(2 r_s, |cos phi| L)), w_row = dchi (1+z)^2
+    dn_fil, beta = D b_fil (sgl_full.py:1984-2002, filaments.py:69-77)."""
+    z, chi_mid, dchi, chi_edges = slice_nodes(cos, zs, shape.n_z)
+    M, dlnM = shape.m_nodes(cos.h)
+    F = np.zeros((shape.n_z, shape.n_m, 4))
+    for i, zi in enumerate(z):
+        D = cos.D(zi)
+        wz = dchi * (1 + zi) ** 2
+        scr = cos.Sigma_cr(zi, zs)
+        for j, Mj in enumerate(M):
+            rs = (Mj / 1e14) ** (1.0 / 3.0)
+            k0base = rs * 14.4 * cos.rhoc0 * (20.0 * rs) / scr
+            F[i, j] = (k0base, rs, wz * dndlnM_fil(cos, Mj, zi) * dlnM, D * fil_bias(cos, Mj, zi))
+    return F
+
+
+def cells_fil(F: np.ndarray):
+    """Filament cell arrays [n_z, n_m, FIL_U * 13]: xi, ws, kap, gam (cell
+    order: u bin, then orientation node)."""
+    L = lib()
+    n_rows = F.shape[0] * F.shape[1]
+    ff = np.ascontiguousarray(F.reshape(n_rows, 4))
+    ue = fil_u_edges()
+    ph, wp = fil_phi_rule()
+    n = FIL_U * ph.size
+    out = [np.zeros(n_rows * n) for _ in range(4)]
+    L.oracle_cells_fil(n_rows, _p(ff), FIL_U, _p(ue), ph.size, _p(np.ascontiguousarray(ph)),
+                       _p(np.ascontiguousarray(wp)), *[_p(o) for o in out])
+    return [o.reshape(F.shape[0], F.shape[1], n) for o in out]
 
 
 def cells(R: np.ndarray, shape: Shape):
@@ -262,6 +340,11 @@ def pdf(zs: float, shape: Shape = None, cosmo: dict = None, keep: bool = False) 
     xi, ws, kap, gam = cells(R, shape)
     kq = shape.k_nodes()
     lam, amp = lnphi(xi, ws, R[:, :, 5], kq[1:])
+    if shape.filaments:
+        F = fil_rows(cos, zs, shape)
+        fxi, fws, _, _ = cells_fil(F)
+        lam_f, amp_f = lnphi(fxi, fws, F[:, :, 3], kq[1:])
+        lam, amp = lam + lam_f, amp + amp_f
     C = slice_covariance(cos, T["chi_edges"], T["chis"])
     psi2 = quadform(C, amp) if shape.clustering else np.zeros_like(lam)
     phi = np.concatenate([[1.0 + 0j], np.exp(lam + psi2)])
@@ -272,6 +355,9 @@ def pdf(zs: float, shape: Shape = None, cosmo: dict = None, keep: bool = False) 
     mom = moments(xo, po)
     info = dict(moments=mom, n_cells=int(np.count_nonzero(ws)), sum_w=float(ws.sum()),
                 var_xi=float(np.sum(ws * xi ** 2)))
+    if shape.filaments:
+        info.update(n_cells_fil=int(np.count_nonzero(fws)), sum_w_fil=float(fws.sum()),
+                    var_xi_fil=float(np.sum(fws * fxi ** 2)))
     if keep:
         info.update(cos=cos, tables=T, xi=xi, ws=ws, kap=kap, gam=gam, lam=lam, amp=amp, C=C,
                     psi2=psi2, phi=phi, kq=kq)

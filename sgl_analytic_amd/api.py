@@ -26,7 +26,7 @@ import numpy as np
 P = dict(h=0.674, Om=0.315, s8=0.811, Ob=0.0493, ns=0.965, zeq=3402.0)  # pipeline.py:20
 EXTENSIONS = dict(Ok=0.0, w0=-1.0, wa=0.0, growth_mode="ode", radiation="engine")
 INGREDIENTS = ("ell", "fil", "sub", "bias")
-MODELLED = {"bias"}            # ingredients this path computes on top of NFW halos
+MODELLED = {"bias", "fil"}     # ingredients this path computes on top of NFW halos
 DEFAULT_SLICES = 64
 
 
@@ -153,13 +153,15 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAUL
         raise NotImplementedError(f"ingredient(s) {sorted(explicit)} are not part of this path")
     sv = _solver(int(n_z))
     sv_shape = sv.shape
-    if ("bias" in s) != sv_shape.clustering:
+    if ("bias" in s) != sv_shape.clustering or ("fil" in s) != sv_shape.filaments:
         from dataclasses import replace
         from .solver import Solver
-        key = ("exact", int(n_z), "bias" in s)
+        key = ("exact", int(n_z), "bias" in s, "fil" in s)
         if key not in _SOLVERS:
-            _SOLVERS[key] = Solver(replace(sv_shape, clustering="bias" in s), 1, path="exact")
+            _SOLVERS[key] = Solver(replace(sv_shape, clustering="bias" in s, filaments="fil" in s),
+                                   1, path="exact")
         sv = _SOLVERS[key]
+        sv_shape = sv.shape
     rec = cosmo_records(np.array([[cp["h"], cp["Om"], cp["s8"], cp["Ob"], cp["ns"], cp["zeq"],
                                    cp["Ok"], cp["w0"], cp["wa"], float(z_s)]]), int(n_z))
     sv.run(rec)

@@ -100,7 +100,7 @@ sgl_status sgl_tables_build(const double* cosmo, int32_t B, const sgl_shape* sha
   int rc = launch_fill_grids(sh, T + L.uedge, T + L.knodes, T + L.m_nodes, st);
   if (rc != SGL_OK) return (sgl_status)rc;
   rc = tables_build(T + L.cosmo, B, sh, T + L.m_nodes, T + L.derived, T + L.slices, T + L.mass,
-                    T + L.rows, T + L.cov, T + L.work, status, st);
+                    T + L.rows, T + L.frows, T + L.cov, T + L.work, status, st);
   return (sgl_status)rc;
 }
 
@@ -110,8 +110,9 @@ sgl_status sgl_lnphi_exact(const void* tables, int32_t B, const sgl_shape* shape
     return SGL_ERR_INVALID;
   TablesLayout L = tables_layout(*shape, B);
   const double* T = static_cast<const double*>(tables);
-  return (sgl_status)launch_lnphi_exact(T + L.rows, T + L.cosmo, T + L.uedge, T + L.knodes, *shape,
-                                        B, lam, amp, work, as_stream(stream));
+  return (sgl_status)launch_lnphi_exact(T + L.rows, T + L.frows, T + L.cosmo, T + L.uedge,
+                                        T + L.knodes, *shape, B, lam, amp, work,
+                                        as_stream(stream));
 }
 
 sgl_status sgl_bin_cells(const void* tables, int32_t B, const sgl_shape* shape, void* hist,
@@ -119,7 +120,7 @@ sgl_status sgl_bin_cells(const void* tables, int32_t B, const sgl_shape* shape, 
   if (!shape_ok(shape) || !batch_ok(B) || !tables || !hist) return SGL_ERR_INVALID;
   TablesLayout L = tables_layout(*shape, B);
   const double* T = static_cast<const double*>(tables);
-  return (sgl_status)launch_bin(T + L.rows, T + L.cosmo, T + L.uedge, *shape, B,
+  return (sgl_status)launch_bin(T + L.rows, T + L.frows, T + L.cosmo, T + L.uedge, *shape, B,
                                 static_cast<double*>(hist), as_stream(stream));
 }
 
@@ -128,8 +129,8 @@ sgl_status sgl_bin_pass(const void* tables, int32_t B, const sgl_shape* shape, v
   if (!shape_ok(shape) || !batch_ok(B) || !tables || !hist) return SGL_ERR_INVALID;
   TablesLayout L = tables_layout(*shape, B);
   const double* T = static_cast<const double*>(tables);
-  return (sgl_status)launch_bin_pass(T + L.rows, T + L.cosmo, T + L.uedge, *shape, B,
-                                     static_cast<double*>(hist), as_stream(stream));
+  return (sgl_status)launch_bin_pass(T + L.rows, T + L.frows, T + L.cosmo, T + L.uedge, *shape,
+                                     B, static_cast<double*>(hist), as_stream(stream));
 }
 
 sgl_status sgl_bin_reduce(const void* tables, int32_t B, const sgl_shape* shape, void* hist,

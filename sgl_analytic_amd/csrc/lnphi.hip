@@ -79,17 +79,56 @@ __device__ __forceinline__ void gen_cell(const SliceStage& S, int n_b, int c, do
   bw = rc.beta * ws;
 }
 
+// Filament rows of a slice and the filament cell lattice (shape-only part),
+// staged in LDS: per u bin the profile factors fk = 2 pi / s and
+// fg = 4 pi / (s + 1) - fk (s = sqrt(1 + u^2); kappa = k0 fk, gamma = k0 fg,
+// u <= 1 on the whole lattice) and u1^2 - u0^2; per orientation node the
+// amplitude factor and weight.
+struct FilStage {
+  double fr[kMaxMasses][kFilRowStride];
+  double fk[kFilU], fg[kFilU], du2[kFilU];
+  double kf[kFilPhi], wp[kFilPhi];
+};
+
+__device__ inline void stage_fil(FilStage& F, const double* __restrict__ frows, const sgl_shape& sh) {
+  for (int q = threadIdx.x; q < sh.n_m * kFilRowStride; q += blockDim.x)
+    F.fr[q / kFilRowStride][q % kFilRowStride] = frows[q];
+  for (int l = threadIdx.x; l < kFilU; l += blockDim.x) {
+    double uc, du2;
+    fil_ubin(l, uc, du2);
+    const double s = sqrt(1.0 + uc * uc);
+    F.fk[l] = 2.0 * kPi / s;
+    F.fg[l] = 4.0 * kPi / (s + 1.0) - F.fk[l];
+    F.du2[l] = du2;
+  }
+  for (int n = threadIdx.x; n < kFilPhi; n += blockDim.x) fil_orientation(n, F.kf[n], F.wp[n]);
+}
+
+// Filament cell c = (row j, u bin l, orientation n) of the staged slice, FP64.
+__device__ __forceinline__ void gen_fil_cell(const FilStage& F, int c, double& xi, double& ws,
+                                             double& bw) {
+  const int per = kFilU * kFilPhi;
+  const int j = c / per, r = c - j * per, l = r / kFilPhi, n = r - l * kFilPhi;
+  const double* R = F.fr[j];
+  if (R[2] == 0.0) { xi = 0.0; ws = 0.0; bw = 0.0; return; }
+  const double k0 = R[0] * F.kf[n] / R[1];
+  cell_jump(k0 * F.fk[l], k0 * F.fg[l], R[2] * kPi * R[1] * R[1] * F.du2[l] * F.wp[n], xi, ws);
+  bw = R[3] * ws;
+}
+
 // ------------------------------------------------------------ exact path --
 constexpr int kCellTile = 256;
 constexpr int kKpt = 2;  // k nodes per thread
 
 __global__ __launch_bounds__(256) void k_lnphi_exact(const double* __restrict__ rows,
+                                                     const double* __restrict__ frows,
                                                      const double* __restrict__ cosmo,
                                                      const double* __restrict__ uedge,
                                                      const double* __restrict__ knodes,
                                                      sgl_shape sh, double* __restrict__ part,
                                                      double* __restrict__ amp) {
   __shared__ SliceStage S;
+  __shared__ FilStage FS;
   __shared__ double sxi[kCellTile], sws[kCellTile], sbw[kCellTile];
   const int t = threadIdx.x;
   const int i = blockIdx.y, b = blockIdx.z;
@@ -105,13 +144,17 @@ __global__ __launch_bounds__(256) void k_lnphi_exact(const double* __restrict__ 
   }
   if (i < n_z) {
     stage_slice(S, rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh);
-    const int n_cells = sh.n_m * sh.n_b;
+    const bool fil = (sh.flags & SGL_FLAG_FILAMENTS) != 0;
+    if (fil) stage_fil(FS, frows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kFilRowStride, sh);
+    const int n_halo = sh.n_m * sh.n_b;
+    const int n_cells = n_halo + (fil ? sh.n_m * kFilU * kFilPhi : 0);
     for (int c0 = 0; c0 < n_cells; c0 += kCellTile) {
       __syncthreads();
       {
         int c = c0 + t;
         double xi = 0.0, ws = 0.0, bw = 0.0;
-        if (c < n_cells) gen_cell(S, sh.n_b, c, xi, ws, bw);
+        if (c < n_halo) gen_cell(S, sh.n_b, c, xi, ws, bw);
+        else if (c < n_cells) gen_fil_cell(FS, c - n_halo, xi, ws, bw);
         sxi[t] = xi; sws[t] = ws; sbw[t] = bw;
       }
       __syncthreads();
@@ -344,6 +387,77 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
     Q[cb * 3] = m0;
     Q[cb * 3 + 1] = m1;
     Q[cb * 3 + 2] = m2;
+  }
+}
+
+// Filament pass (SGL_FLAG_FILAMENTS): every filament cell of the slice --
+// kappa = k0 fk(u), gamma = k0 fg(u) in FP32 (no cancellation anywhere on the
+// cylinder profile), FP64 weights, detA and accumulation -- into a
+// moment-major LDS histogram, then added into the slice's fine partials and
+// coarse moments (the main pass has written them; this workgroup is the only
+// one touching the slice now).  Lanes run over orientation nodes and u bins
+// of one row, whose jumps spread over many bins.
+constexpr int kFilThreads = 1024;
+__global__ __launch_bounds__(kFilThreads) void k_bin_fil(const double* __restrict__ frows,
+                                                         const double* __restrict__ cosmo,
+                                                         sgl_shape sh, double* __restrict__ hist) {
+  __shared__ double lf[kFineBins * kFineMom];
+  __shared__ float sfk[kFilU], sfg[kFilU], skf[kFilPhi];
+  __shared__ double sdu2[kFilU], swp[kFilPhi];
+  const int t = threadIdx.x, i = blockIdx.x, b = blockIdx.y;
+  const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
+  if (i >= n_z) return;
+  const HistLayout HL = hist_layout(sh, gridDim.y);
+  for (int q = t; q < kFineBins * kFineMom; q += kFilThreads) lf[q] = 0.0;
+  if (t < kFilU) {
+    double uc, du2;
+    fil_ubin(t, uc, du2);
+    const double sq = sqrt(1.0 + uc * uc);
+    sfk[t] = (float)(2.0 * kPi / sq);
+    sfg[t] = (float)(4.0 * kPi / (sq + 1.0) - 2.0 * kPi / sq);
+    sdu2[t] = du2;
+  }
+  if (t < kFilPhi) {
+    double kf, wp;
+    fil_orientation(t, kf, wp);
+    skf[t] = (float)kf;
+    swp[t] = wp;
+  }
+  __syncthreads();
+  const double* FR = frows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kFilRowStride;
+  const int per = kFilU * kFilPhi;
+  const int n_cells = sh.n_m * per;
+  for (int c = t; c < n_cells; c += kFilThreads) {
+    const int j = c / per, r = c - j * per, l = r / kFilPhi, n = r - l * kFilPhi;
+    const double* R = FR + (size_t)j * kFilRowStride;
+    const double wrow = R[2];
+    if (wrow == 0.0) continue;
+    const float k0 = (float)(R[0] / R[1]) * skf[n];
+    double xi, ws;
+    cell_jump_f32log(k0 * sfk[l], k0 * sfg[l], wrow * kPi * R[1] * R[1] * sdu2[l] * swp[n], xi, ws);
+    if (ws == 0.0) continue;
+    deposit4(lf, xi, ws, R[3] * ws);
+  }
+  __syncthreads();
+  double* P = hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments;
+  for (int f = t; f < kFineBins; f += kFilThreads) {
+    if (lf[f] == 0.0) continue;
+    P[f * kMoments] += lf[f];
+    P[f * kMoments + 1] += lf[kFineBins + f];
+  }
+  double* Q = hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3;
+  for (int cb = t; cb < kSliceBins; cb += kFilThreads) {
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    for (int f = cb * kCoarse; f < (cb + 1) * kCoarse; ++f) {
+      const double bw = lf[2 * kFineBins + f], bwx = lf[3 * kFineBins + f];
+      if (bw == 0.0) continue;
+      m0 += bw;
+      m1 += bwx;
+      m2 += bwx * bwx / bw;
+    }
+    Q[cb * 3] += m0;
+    Q[cb * 3 + 1] += m1;
+    Q[cb * 3 + 2] += m2;
   }
 }
 
@@ -685,20 +799,21 @@ __global__ __launch_bounds__(kBinTile) void k_lnphi_binned(const double* __restr
   }
 }
 
-int launch_lnphi_exact(const double* rows, const double* cosmo, const double* uedge,
-                       const double* knodes, const sgl_shape& sh, int B, double* lam, double* amp,
-                       double* work, hipStream_t st) {
+int launch_lnphi_exact(const double* rows, const double* frows, const double* cosmo,
+                       const double* uedge, const double* knodes, const sgl_shape& sh, int B,
+                       double* lam, double* amp, double* work, hipStream_t st) {
   dim3 grid((sh.n_k + 256 * kKpt - 1) / (256 * kKpt), sh.n_z_max, B);
-  hipLaunchKernelGGL(k_lnphi_exact, grid, dim3(256), 0, st, rows, cosmo, uedge, knodes, sh, work,
-                     amp);
+  hipLaunchKernelGGL(k_lnphi_exact, grid, dim3(256), 0, st, rows, frows, cosmo, uedge, knodes, sh,
+                     work, amp);
   size_t n = (size_t)B * sh.n_k;
   hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, work, B,
                      sh.n_z_max, sh.n_k, lam);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
-int launch_bin_pass(const double* rows, const double* cosmo, const double* uedge,
-                    const sgl_shape& sh, int B, double* hist, hipStream_t st) {
+int launch_bin_pass(const double* rows, const double* frows, const double* cosmo,
+                    const double* uedge, const sgl_shape& sh, int B, double* hist,
+                    hipStream_t st) {
   const size_t nrow = (size_t)B * sh.n_z_max * sh.n_m;
   hipLaunchKernelGGL(k_rowf, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, st, rows, cosmo,
                      uedge, sh, B, hist);
@@ -709,6 +824,8 @@ int launch_bin_pass(const double* rows, const double* cosmo, const double* uedge
     case 2: hipLaunchKernelGGL(k_bin_cells<1024>, g, dim3(1024), 0, st, cosmo, uedge, sh, hist); break;
     default: hipLaunchKernelGGL(k_bin_cells<512>, g, dim3(512), 0, st, cosmo, uedge, sh, hist); break;
   }
+  if (sh.flags & SGL_FLAG_FILAMENTS)
+    hipLaunchKernelGGL(k_bin_fil, g, dim3(kFilThreads), 0, st, frows, cosmo, sh, hist);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
@@ -720,9 +837,9 @@ int launch_bin_reduce(const double* rows, const double* cosmo, const double* ued
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
-int launch_bin(const double* rows, const double* cosmo, const double* uedge, const sgl_shape& sh,
-               int B, double* hist, hipStream_t st) {
-  int rc = launch_bin_pass(rows, cosmo, uedge, sh, B, hist, st);
+int launch_bin(const double* rows, const double* frows, const double* cosmo, const double* uedge,
+               const sgl_shape& sh, int B, double* hist, hipStream_t st) {
+  int rc = launch_bin_pass(rows, frows, cosmo, uedge, sh, B, hist, st);
   return rc == SGL_OK ? launch_bin_reduce(rows, cosmo, uedge, sh, B, hist, st) : rc;
 }
 

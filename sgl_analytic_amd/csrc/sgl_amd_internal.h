@@ -21,11 +21,17 @@ constexpr int kSlZ = 0, kSlChi = 1, kSlDchi = 2, kSlWz = 3, kSlD = 4, kSlScr = 5
 constexpr int kMassStride = 4;
 // row record (8 doubles)
 constexpr int kRowStride = 8;
+// Filament rows (k0base, r_s, w_row, beta) and the filament cell lattice:
+// kFilU u bins log-uniform from 1e-5 to 1 (first bin the disc u < 1e-5) x
+// kFilPhi orientation nodes.
+constexpr int kFilRowStride = 4;
+constexpr int kFilU = 120;
+constexpr int kFilPhi = 13;
 constexpr int kRowKs = 0, kRowRs = 1, kRowXt = 2, kRowRvir = 3, kRowW = 4, kRowBeta = 5;
 
 // Offsets (in doubles) of the sub-arrays of the tables buffer.
 struct TablesLayout {
-  size_t cosmo, derived, slices, mass, rows, cov, work, m_nodes, uedge, knodes, status, total;
+  size_t cosmo, derived, slices, mass, rows, frows, cov, work, m_nodes, uedge, knodes, status, total;
 };
 
 // The shape-only grids come first, so their offsets do not depend on B: a
@@ -44,6 +50,7 @@ inline TablesLayout tables_layout(const sgl_shape& s, int B) {
   L.slices = take((size_t)B * s.n_z_max * kSliceStride);
   L.mass = take((size_t)B * s.n_m * kMassStride);
   L.rows = take((size_t)B * s.n_z_max * s.n_m * kRowStride);
+  L.frows = take((s.flags & SGL_FLAG_FILAMENTS) ? (size_t)B * s.n_z_max * s.n_m * kFilRowStride : 0);
   L.cov = take((size_t)B * s.n_z_max * s.n_z_max);
   L.work = take((size_t)B * 20000);
   L.total = o;
@@ -110,16 +117,16 @@ __host__ __device__ inline HistLayout hist_layout(const sgl_shape& s, int B) {
 extern int g_sgl_tune[SGL_TUNE_COUNT];
 
 int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes_h,
-                 double* derived, double* slices, double* mass, double* rows, double* cov,
-                 double* work, int* status, hipStream_t st);
+                 double* derived, double* slices, double* mass, double* rows, double* frows,
+                 double* cov, double* work, int* status, hipStream_t st);
 int launch_fill_grids(const sgl_shape& sh, double* ue, double* kn, double* m_nodes_h,
                       hipStream_t st);
-int launch_lnphi_exact(const double* rows, const double* cosmo, const double* uedge,
+int launch_lnphi_exact(const double* rows, const double* frows, const double* cosmo, const double* uedge,
                        const double* knodes, const sgl_shape& sh, int B, double* lam, double* amp,
                        double* work, hipStream_t st);
-int launch_bin(const double* rows, const double* cosmo, const double* uedge, const sgl_shape& sh,
+int launch_bin(const double* rows, const double* frows, const double* cosmo, const double* uedge, const sgl_shape& sh,
                int B, double* hist, hipStream_t st);
-int launch_bin_pass(const double* rows, const double* cosmo, const double* uedge, const sgl_shape& sh,
+int launch_bin_pass(const double* rows, const double* frows, const double* cosmo, const double* uedge, const sgl_shape& sh,
                int B, double* hist, hipStream_t st);
 int launch_bin_reduce(const double* rows, const double* cosmo, const double* uedge, const sgl_shape& sh,
                int B, double* hist, hipStream_t st);

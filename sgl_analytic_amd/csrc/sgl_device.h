@@ -239,6 +239,36 @@ __device__ inline void trunc_kg(double x, double ks, double xt, double& kap, dou
   }
 }
 
+// ------------------------------------------------------------ filaments --
+// Orientation rule of the filament population (sgl_full.py:1828-1845): 12
+// Gauss-Legendre nodes on [0, acos 0.1] plus one node carrying the clamped
+// measure; weights are fractions of the quarter turn.  The amplitude factor
+// kf = r_s / max(2 r_s, |cos phi| 20 r_s) turns the row's k0base into k0.
+static __constant__ double kGl12X[12] = {-0.9815606342467192, -0.9041172563704748, -0.7699026741943047, -0.5873179542866175, -0.3678314989981802, -0.1252334085114689, 0.1252334085114689, 0.3678314989981802, 0.5873179542866175, 0.7699026741943047, 0.9041172563704748, 0.9815606342467192};
+static __constant__ double kGl12W[12] = {0.04717533638651202, 0.10693932599531888, 0.1600783285433461, 0.20316742672306565, 0.23349253653835464, 0.2491470458134027, 0.2491470458134027, 0.23349253653835464, 0.20316742672306565, 0.1600783285433461, 0.10693932599531888, 0.04717533638651202};
+
+__device__ inline void fil_orientation(int n, double& kf, double& wphi) {
+  const double pc = acos(0.1), half_pi = 0.5 * kPi;
+  double ph;
+  if (n < 12) {
+    ph = 0.5 * pc * (kGl12X[n] + 1.0);
+    wphi = 0.5 * pc * kGl12W[n] / half_pi;
+  } else {
+    ph = 0.5 * (pc + half_pi);
+    wphi = (half_pi - pc) / half_pi;
+  }
+  kf = 1.0 / fmax(2.0, fabs(cos(ph)) * 20.0);
+}
+
+// u bin l of a filament row (edges 0, geomspace(1e-5, 1, 120)): centre
+// (geometric, u/2 for the central disc) and u1^2 - u0^2.
+__device__ inline void fil_ubin(int l, double& uc, double& du2) {
+  auto edge = [](int m) { return pow(10.0, -5.0 + 5.0 * m / 119.0); };
+  const double u1 = edge(l), u0 = (l > 0) ? edge(l - 1) : 0.0;
+  uc = (l > 0) ? sqrt(u0 * u1) : 0.5 * u1;
+  du2 = u1 * u1 - u0 * u0;
+}
+
 // Per-row constants of a truncated NFW lens row (one (slice, mass) pair).
 struct RowConst {
   double ks, s, xt, ic1, mut4ks, warea, beta, i0, jj, valid;

@@ -271,13 +271,18 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
                                               const double* __restrict__ slices,
                                               const double* __restrict__ mass, int n_z_max,
                                               int n_m, double dlnM, double a_st, double dc0,
-                                              double* __restrict__ rows) {
+                                              double* __restrict__ rows,
+                                              double* __restrict__ frows) {
   __shared__ double sk[10];
+  // (sk[8] = rho_c0 for the filament rows)
   const int i = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
   const Cosmo c = cosmo[b];
   double* Rs = rows + ((size_t)b * n_z_max + i) * n_m * kRowStride;
+  double* Fs = frows ? frows + ((size_t)b * n_z_max + i) * n_m * kFilRowStride : nullptr;
   if (i >= (int)c.n_z) {
     for (int q = t; q < n_m * kRowStride; q += blockDim.x) Rs[q] = 0.0;
+    if (Fs)
+      for (int q = t; q < n_m * kFilRowStride; q += blockDim.x) Fs[q] = 0.0;
     return;
   }
   const double* S = slices + ((size_t)b * n_z_max + i) * kSliceStride;
@@ -296,11 +301,12 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
     const double dd = c.Om * (1.0 + z) * (1.0 + z) * (1.0 + z) / E2 - 1.0;
     sk[6] = 18.0 * kPi * kPi + 82.0 * dd - 39.0 * dd * dd;
     sk[7] = d.rhom;
+    sk[8] = d.rhoc0;
   }
   __syncthreads();
   const double D = S[kSlD];
   const double poly = sk[0], c0 = sk[1], beta_ = sk[2], g1 = sk[3], g2 = sk[4];
-  const double rhoc = sk[5], Dvir = sk[6], rhom = sk[7];
+  const double rhoc = sk[5], Dvir = sk[6], rhom = sk[7], rhoc0 = sk[8];
   for (int j = t; j < n_m; j += blockDim.x) {
     double* R = Rs + (size_t)j * kRowStride;
     const double* Mr = mass + ((size_t)b * n_m + j) * kMassStride;
@@ -333,6 +339,22 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
     R[kRowBeta] = D * bh;
     R[6] = 0.0;
     R[7] = 0.0;
+    if (Fs) {
+      // filament row (sgl_full.py:1984-2002): flat-barrier mass function
+      // (p, q, A) = (0, 0.7, 1/2) (filaments.py:43-50), r_s = (M/1e14)^(1/3)
+      // Mpc, L = 20 r_s, k0base = r_s 14.4 rho_c0 L / Sigma_cr, bias of the
+      // flat barrier (sgl_full.py:546-548)
+      const double qf = 0.7 * nu;
+      const double multf = 0.5 * 2.0 * sqrt(qf / (2.0 * kPi)) * exp(-0.5 * qf);
+      const double dnf = (rhom / M) * multf * (-2.0 * dls);
+      const double rsf = cbrt(M / 1e14);
+      const double qb = 0.7 * (dc0 / D / sig0) * (dc0 / D / sig0);
+      double* F = Fs + (size_t)j * kFilRowStride;
+      F[0] = rsf * 14.4 * rhoc0 * (20.0 * rsf) / S[kSlScr];
+      F[1] = rsf;
+      F[2] = S[kSlWz] * dnf * dlnM;
+      F[3] = D * (1.0 + (qb - 1.0) / dc0);
+    }
   }
 }
 
@@ -464,8 +486,8 @@ __global__ __launch_bounds__(256) void k_cov(const Cosmo* __restrict__ cosmo,
 }
 
 int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes_h,
-                 double* derived, double* slices, double* mass, double* rows, double* cov,
-                 double* work, int* status, hipStream_t st) {
+                 double* derived, double* slices, double* mass, double* rows, double* frows,
+                 double* cov, double* work, int* status, hipStream_t st) {
   const Cosmo* C = reinterpret_cast<const Cosmo*>(cosmo);
   Derived* D = reinterpret_cast<Derived*>(derived);
   const double dlnM = (log(sh.m_hi_h) - log(sh.m_lo_h)) / sh.n_m;
@@ -477,7 +499,8 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
   const double a_st = 1.0 / (1.0 + std::pow(2.0, -kPst) * std::tgamma(0.5 - kPst) / std::sqrt(kPi));
   const double dc0 = 0.6 * std::pow(1.5 * kPi, 2.0 / 3.0);
   hipLaunchKernelGGL(k_rows, dim3(sh.n_z_max, B), dim3(256), 0, st, C, D, slices, mass,
-                     sh.n_z_max, sh.n_m, dlnM, a_st, dc0, rows);
+                     sh.n_z_max, sh.n_m, dlnM, a_st, dc0, rows,
+                     (sh.flags & SGL_FLAG_FILAMENTS) ? frows : (double*)nullptr);
   if (sh.flags & SGL_FLAG_CLUSTERING)
     hipLaunchKernelGGL(k_cov, dim3(B), dim3(256), 0, st, C, D, sh.n_z_max, work, status, cov);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;

@@ -20,7 +20,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import COSMO_STRIDE, FLAG_CLUSTERING, FLAG_WINDOW, PREC_BF16X3, PREC_FP32, PREC_FP64
+from ._lib import (COSMO_STRIDE, FLAG_CLUSTERING, FLAG_FILAMENTS, FLAG_WINDOW, PREC_BF16X3, PREC_FP32,
+                   PREC_FP64)
 
 PARAM_NAMES = ("h", "Om", "sigma8", "Ob", "ns", "zeq", "Ok", "w0", "wa", "z_s")
 
@@ -28,7 +29,9 @@ PARAM_NAMES = ("h", "Om", "sigma8", "Ob", "ns", "zeq", "Ok", "w0", "wa", "z_s")
 @dataclass(frozen=True)
 class Shape:
     """Grids of one solve.  Defaults: 64 slices x 256 masses x 512 impact
-    parameters, 8192 log-k nodes in [1e-2, 1e6], 2048 ln(mu) nodes on [-0.5, 3]."""
+    parameters, 8192 log-k nodes in [1e-2, 1e6], 2048 ln(mu) nodes on [-0.5, 3].
+    ``filaments`` adds the cylinder-filament population (120 u bins x 13
+    orientations per (slice, mass))."""
     n_z_max: int = 64
     n_m: int = 256
     n_b: int = 512
@@ -36,6 +39,7 @@ class Shape:
     n_out: int = 2048
     clustering: bool = True
     window: bool = True
+    filaments: bool = False
     m_lo_h: float = 1e8
     m_hi_h: float = 1e16
     u_lo: float = 1e-4
@@ -46,7 +50,8 @@ class Shape:
     out_hi: float = 3.0
 
     def c(self) -> _lib.CShape:
-        flags = (FLAG_CLUSTERING if self.clustering else 0) | (FLAG_WINDOW if self.window else 0)
+        flags = ((FLAG_CLUSTERING if self.clustering else 0) | (FLAG_WINDOW if self.window else 0)
+                 | (FLAG_FILAMENTS if self.filaments else 0))
         return _lib.CShape(self.n_z_max, self.n_m, self.n_b, self.n_k, self.n_out, flags,
                            self.m_lo_h, self.m_hi_h, self.u_lo, self.u_hi, self.k_lo, self.k_hi,
                            self.out_lo, self.out_hi)
@@ -194,6 +199,7 @@ class Solver:
         sizes = [("m_nodes", sh.n_m), ("uedge", sh.n_b + 1), ("knodes", sh.n_k), ("status", 1),
                  ("cosmo", B * 16), ("derived", B * 16), ("slices", B * sh.n_z_max * 8),
                  ("mass", B * sh.n_m * 4), ("rows", B * sh.n_z_max * sh.n_m * 8),
+                 ("frows", B * sh.n_z_max * sh.n_m * 4 if sh.filaments else 0),
                  ("cov", B * sh.n_z_max * sh.n_z_max), ("work", B * 20000)]
         T = self.tables.view(torch.float64).cpu().numpy()
         out, o = {}, 0
@@ -204,6 +210,8 @@ class Solver:
         out["slices"] = out["slices"].reshape(B, sh.n_z_max, 8)
         out["mass"] = out["mass"].reshape(B, sh.n_m, 4)
         out["rows"] = out["rows"].reshape(B, sh.n_z_max, sh.n_m, 8)
+        if sh.filaments:
+            out["frows"] = out["frows"].reshape(B, sh.n_z_max, sh.n_m, 4)
         out["cov"] = out["cov"].reshape(B, sh.n_z_max, sh.n_z_max)
         return out
 
