@@ -239,28 +239,79 @@ __device__ __forceinline__ void deposit(double* lf, double* lc, double xi, doubl
   const int f = fine_bin(xi);
   atomicAdd(&lf[f * kMoments], ws);
   atomicAdd(&lf[f * kMoments + 1], ws * xi);
+  atomicAdd(&lf[f * kMoments + 2], ws * xi * xi);
   const int cb = f / kCoarse;
   atomicAdd(&lc[cb * 3], bw);
   atomicAdd(&lc[cb * 3 + 1], bw * xi);
   atomicAdd(&lc[cb * 3 + 2], bw * xi * xi);
 }
 
-// Main-pass fine bins carry four moments: w, w xi, beta w, beta w xi.  The
-// slice's coarse two-point moments are formed from them at the end (coarse
-// bin = 32 fine bins; second moment sum_f (beta w xi)_f^2 / (beta w)_f, i.e.
-// each fine bin at its beta-weighted centroid), so a cell costs four LDS
-// atomics on fine bins, which neighbouring lanes rarely share, instead of
-// three more on coarse bins, which most lanes of a wave share.
+// Fine histogram of a slice in LDS, moment-major (a bin's moment at an 8-byte
+// stride, so the lanes of a wave spread over all banks), four FP64 moments:
+// w, w xi, w xi^2 (centroid and in-bin variance of the exponent's records)
+// and beta w (the two-point weight).  The slice's coarse two-point moments
+// are formed from them at the end, kCoarse fine bins per coarse bin, each fine
+// bin's beta w placed at the bin's w centroid m_f with its w variance v_f:
+// sum beta w, sum beta w m_f, sum beta w (m_f^2 + v_f).  Within one fine bin
+// (3.1 % relative width) beta and xi are taken as uncorrelated, which moves
+// a coarse record's centroid by at most 1.6 % of xi.  A cell costs four LDS
+// atomics on fine bins, which neighbouring lanes rarely share.  (FP32 LDS
+// atomic adds measured four times slower here than FP64 ones.)
 constexpr int kFineMom = 4;
+struct FineHist {
+  double d[kFineMom * kFineBins];
+};
 
-// The four moments are four arrays (moment-major), so a bin's moment sits at
-// an 8-byte stride and the lanes of a wave spread over all LDS banks.
-__device__ __forceinline__ void deposit4(double* lf, double xi, double ws, double bw) {
-  double* q = lf + fine_bin(xi);
+__device__ __forceinline__ void fine_hist_clear(FineHist& H, int t, int nt) {
+  for (int q = t; q < kFineMom * kFineBins; q += nt) H.d[q] = 0.0;
+}
+
+__device__ __forceinline__ void deposit4(FineHist& H, double xi, double ws, double bw) {
+  double* q = H.d + fine_bin(xi);
   atomicAdd(q, ws);
   atomicAdd(q + kFineBins, ws * xi);
-  atomicAdd(q + 2 * kFineBins, bw);
-  atomicAdd(q + 3 * kFineBins, bw * xi);
+  atomicAdd(q + 2 * kFineBins, ws * xi * xi);
+  atomicAdd(q + 3 * kFineBins, bw);
+}
+
+// Add (or store) the slice's fine partials and coarse moments from H.
+__device__ inline void fine_hist_flush(const FineHist& H, double* __restrict__ P,
+                                       double* __restrict__ Q, bool add, int t, int nt) {
+  for (int f = t; f < kFineBins; f += nt) {
+    const double w = H.d[f], wx = H.d[kFineBins + f], wxx = H.d[2 * kFineBins + f];
+    if (add) {
+      if (w == 0.0) continue;
+      P[f * kMoments] += w;
+      P[f * kMoments + 1] += wx;
+      P[f * kMoments + 2] += wxx;
+    } else {
+      P[f * kMoments] = w;
+      P[f * kMoments + 1] = wx;
+      P[f * kMoments + 2] = wxx;
+    }
+  }
+  for (int cb = t; cb < kSliceBins; cb += nt) {
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    for (int f = cb * kCoarse; f < (cb + 1) * kCoarse; ++f) {
+      const double w = H.d[f];
+      if (w == 0.0) continue;
+      const double bw = H.d[3 * kFineBins + f];
+      const double m = H.d[kFineBins + f] / w;
+      const double v = fmax(H.d[2 * kFineBins + f] / w - m * m, 0.0);
+      m0 += bw;
+      m1 += bw * m;
+      m2 += bw * (m * m + v);
+    }
+    if (add) {
+      Q[cb * 3] += m0;
+      Q[cb * 3 + 1] += m1;
+      Q[cb * 3 + 2] += m2;
+    } else {
+      Q[cb * 3] = m0;
+      Q[cb * 3 + 1] = m1;
+      Q[cb * 3 + 2] = m2;
+    }
+  }
 }
 
 // Per impact-parameter column (shared by every instance): geometric bin
@@ -336,12 +387,12 @@ template <int kBinThreads>
 __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restrict__ cosmo,
                                                            const double* __restrict__ uedge,
                                                            sgl_shape sh, double* __restrict__ hist) {
-  __shared__ double lf[kFineBins * kFineMom];
+  __shared__ FineHist H;
   const int t = threadIdx.x, i = blockIdx.x, b = blockIdx.y;
   const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
   if (i >= n_z) return;                         // slices past n_z are never read
   const HistLayout HL = hist_layout(sh, gridDim.y);
-  for (int q = t; q < kFineBins * kFineMom; q += kBinThreads) lf[q] = 0.0;
+  fine_hist_clear(H, t, kBinThreads);
   __syncthreads();
   const RowF* RF = reinterpret_cast<const RowF*>(hist + HL.rowf) +
                    ((size_t)b * sh.n_z_max + i) * sh.n_m;
@@ -364,30 +415,14 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
         double xi, ws;
         cell_jump_f32log(kf, gf, rc.warea * cf.du2, xi, ws);
         if (ws == 0.0) continue;
-        deposit4(lf, xi, ws, rc.beta * ws);
+        deposit4(H, xi, ws, rc.beta * ws);
       }
     }
   }
   __syncthreads();
-  double* P = hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments;
-  for (int f = t; f < kFineBins; f += kBinThreads) {
-    P[f * kMoments] = lf[f];
-    P[f * kMoments + 1] = lf[kFineBins + f];
-  }
-  double* Q = hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3;
-  for (int cb = t; cb < kSliceBins; cb += kBinThreads) {
-    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
-    for (int f = cb * kCoarse; f < (cb + 1) * kCoarse; ++f) {
-      const double bw = lf[2 * kFineBins + f], bwx = lf[3 * kFineBins + f];
-      if (bw == 0.0) continue;
-      m0 += bw;
-      m1 += bwx;
-      m2 += bwx * bwx / bw;
-    }
-    Q[cb * 3] = m0;
-    Q[cb * 3 + 1] = m1;
-    Q[cb * 3 + 2] = m2;
-  }
+  fine_hist_flush(H, hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments,
+                  hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3, false, t,
+                  kBinThreads);
 }
 
 // Filament pass (SGL_FLAG_FILAMENTS): every filament cell of the slice --
@@ -401,14 +436,14 @@ constexpr int kFilThreads = 1024;
 __global__ __launch_bounds__(kFilThreads) void k_bin_fil(const double* __restrict__ frows,
                                                          const double* __restrict__ cosmo,
                                                          sgl_shape sh, double* __restrict__ hist) {
-  __shared__ double lf[kFineBins * kFineMom];
+  __shared__ FineHist H;
   __shared__ float sfk[kFilU], sfg[kFilU], skf[kFilPhi];
   __shared__ double sdu2[kFilU], swp[kFilPhi];
   const int t = threadIdx.x, i = blockIdx.x, b = blockIdx.y;
   const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
   if (i >= n_z) return;
   const HistLayout HL = hist_layout(sh, gridDim.y);
-  for (int q = t; q < kFineBins * kFineMom; q += kFilThreads) lf[q] = 0.0;
+  fine_hist_clear(H, t, kFilThreads);
   if (t < kFilU) {
     double uc, du2;
     fil_ubin(t, uc, du2);
@@ -436,29 +471,12 @@ __global__ __launch_bounds__(kFilThreads) void k_bin_fil(const double* __restric
     double xi, ws;
     cell_jump_f32log(k0 * sfk[l], k0 * sfg[l], wrow * kPi * R[1] * R[1] * sdu2[l] * swp[n], xi, ws);
     if (ws == 0.0) continue;
-    deposit4(lf, xi, ws, R[3] * ws);
+    deposit4(H, xi, ws, R[3] * ws);
   }
   __syncthreads();
-  double* P = hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments;
-  for (int f = t; f < kFineBins; f += kFilThreads) {
-    if (lf[f] == 0.0) continue;
-    P[f * kMoments] += lf[f];
-    P[f * kMoments + 1] += lf[kFineBins + f];
-  }
-  double* Q = hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3;
-  for (int cb = t; cb < kSliceBins; cb += kFilThreads) {
-    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
-    for (int f = cb * kCoarse; f < (cb + 1) * kCoarse; ++f) {
-      const double bw = lf[2 * kFineBins + f], bwx = lf[3 * kFineBins + f];
-      if (bw == 0.0) continue;
-      m0 += bw;
-      m1 += bwx;
-      m2 += bwx * bwx / bw;
-    }
-    Q[cb * 3] += m0;
-    Q[cb * 3 + 1] += m1;
-    Q[cb * 3 + 2] += m2;
-  }
+  fine_hist_flush(H, hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments,
+                  hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3, true, t,
+                  kFilThreads);
 }
 
 // Band pass: the cells with |1 - x^2| < kBandW in FP64 (series branch of F, h
@@ -542,35 +560,50 @@ __global__ __launch_bounds__(256) void k_reduce_fine(const double* __restrict__ 
     if (!flag[f]) continue;
     const double w = acc[f * kMoments];
     const double m = acc[f * kMoments + 1] / w;
+    const double v = acc[f * kMoments + 2] / w - m * m;
     double* r = R + (size_t)pos[f] * kRec;
     r[0] = m * (0.25 / kPi);
     r[1] = m;
     r[2] = w;
-    r[3] = 0.0;
+    r[3] = (v > 0.0) ? v : 0.0;
   }
   double* FP = hist + HL.fpos + (size_t)b * (kFineBins + 1);
   for (int f = t; f < kFineBins; f += 256) FP[f] = (double)pos[f];
   if (t == 0) FP[kFineBins] = hist[HL.fcnt + b];
-  // prefix sums of w m^p (p = 2 .. 9) over the positive records in bin
-  // order: each thread sums a chunk of kFinePos / 256 bins, the chunk totals
-  // are scanned in order by kTay threads, then every record R gets the sum
-  // over records 0 .. R-1 (entry np is the total).
+  // prefix sums of w mu_p (p = 2 .. 9; mu_p the raw moments of N(m, v)) over
+  // the positive records in bin order: each thread sums a chunk of
+  // kFinePos / 256 bins, the chunk totals are scanned in order by kTay
+  // threads, then every record R gets the sum over records 0 .. R-1 (entry
+  // np is the total).
   {
     constexpr int kChunk = kFinePos / 256;
     static_assert(kFinePos % 256 == 0 && 257 * kTay <= kFineBins * kMoments, "scan fits in acc");
     double* csum = acc + 0;                      // acc is free again: reuse as [256][kTay]
     double mom[kTay];
     for (int p = 0; p < kTay; ++p) mom[p] = 0.0;
-    double wm[kChunk], mm[kChunk];
+    double wm[kChunk], mm[kChunk], vm[kChunk];
     for (int q = 0; q < kChunk; ++q) {
       const int f = t * kChunk + q;
       wm[q] = flag[f] ? acc[f * kMoments] : 0.0;
       mm[q] = flag[f] ? acc[f * kMoments + 1] / wm[q] : 0.0;
+      const double v = flag[f] ? acc[f * kMoments + 2] / wm[q] - mm[q] * mm[q] : 0.0;
+      vm[q] = (v > 0.0) ? v : 0.0;
     }
     __syncthreads();                             // every thread has read acc
+    // w mu_p for p = 2 .. 9 by mu_{p+1} = m mu_p + p v mu_{p-1}
+    auto gmoments = [&](int q, double* out) {
+      double mu_prev = 1.0, mu = mm[q];
+      for (int p = 1; p <= kTay + 1; ++p) {
+        const double nx = mm[q] * mu + p * vm[q] * mu_prev;
+        mu_prev = mu;
+        mu = nx;                                  // mu_{p+1}
+        if (p >= 1) out[p - 1] = wm[q] * mu;      // out[0] = w mu_2
+      }
+    };
     for (int q = 0; q < kChunk; ++q) {
-      double v = wm[q] * mm[q];
-      for (int p = 0; p < kTay; ++p) { v *= mm[q]; mom[p] += v; }
+      double g[kTay + 1];
+      gmoments(q, g);
+      for (int p = 0; p < kTay; ++p) mom[p] += g[p];
     }
     for (int p = 0; p < kTay; ++p) csum[t * kTay + p] = mom[p];
     __syncthreads();
@@ -590,8 +623,9 @@ __global__ __launch_bounds__(256) void k_reduce_fine(const double* __restrict__ 
       const int f = t * kChunk + q;
       if (!flag[f]) continue;
       double* e = PF + (size_t)pos[f] * kTay;
-      double v = wm[q] * mm[q];
-      for (int p = 0; p < kTay; ++p) { e[p] = mom[p]; v *= mm[q]; mom[p] += v; }
+      double g[kTay + 1];
+      gmoments(q, g);
+      for (int p = 0; p < kTay; ++p) { e[p] = mom[p]; mom[p] += g[p]; }
     }
     if (t < kTay) PF[(size_t)pos[kFinePos] * kTay + t] = csum[256 * kTay + t];
   }
@@ -671,25 +705,6 @@ __device__ __forceinline__ void bin_phase(double k, const BinRec& r, double& re,
   smpg = smp + (double)(sn * gm1);
 }
 
-// Fine records carry no in-bin variance: e^{i ph} - 1 = re + i sn and
-// smp = sin ph - ph, the same reduced-phase evaluation without the Gaussian.
-__device__ __forceinline__ void bin_phase0(double k, const BinRec& r, double& re, double& smp) {
-  const double ph = k * r.xi;
-  const double rv = k * r.x4pi;
-  const double f = rv - rint(rv);
-  const float ff = (float)f;
-  const float hs = __builtin_amdgcn_sinf(ff);
-  const float hc = __builtin_amdgcn_cosf(ff);
-  re = (double)(-2.0f * hs * hs);
-  if (fabs(ph) < 0.5) {
-    const double p2 = ph * ph;
-    smp = -ph * p2 * (1.0 / 6.0) *
-          (1.0 - p2 / 20.0 * (1.0 - p2 / 42.0 * (1.0 - p2 / 72.0 * (1.0 - p2 / 110.0))));
-  } else {
-    smp = (double)(2.0f * hs * hc) - ph;
-  }
-}
-
 template <int kBinTile>
 __global__ __launch_bounds__(kBinTile) void k_lnphi_binned(const double* __restrict__ hist,
                                                       const double* __restrict__ cosmo,
@@ -706,14 +721,16 @@ __global__ __launch_bounds__(kBinTile) void k_lnphi_binned(const double* __restr
   const double k = knodes[min(n, n_k - 1)];
   const int nf = (int)hist[HL.fcnt + b];
   const double* fr = hist + HL.frec + (size_t)b * kFineBins * kRec;
-  // Records of positive bins whose upper edge is <= kTayEps / k have
-  // k xi <= 1/2: their sum of w (e^{i k xi} - 1 - i k xi) is the Taylor
-  // series through (k xi)^9 on the prefix moments (truncation < 1e-9
-  // relative); the rest are summed directly.  The workgroup's k nodes are
-  // increasing, so its last node has the lowest first direct record.
+  // Records of positive bins whose upper edge is <= kTayEps / (1.05 k) have
+  // k (|xi| + 3 sd) <= 1/2 (a bin's content spreads over at most half its
+  // relative width of 3.1 %): their sum of w (e^{i k xi - k^2 v/2} - 1 - i k xi)
+  // is the series through n = 9 on the prefix raw Gaussian moments
+  // (truncation < 1e-9 relative); the rest are summed directly.  The
+  // workgroup's k nodes are increasing, so its last node has the lowest
+  // first direct record.
   const double* fpos = hist + HL.fpos + (size_t)b * (kFineBins + 1);
   auto first_direct = [&](double kk) {
-    const double z = (log(kTayEps / kk) - log(kXiLo)) * (kFinePos / log(kXiHi / kXiLo));
+    const double z = (log(kTayEps / (1.05 * kk)) - log(kXiLo)) * (kFinePos / log(kXiHi / kXiLo));
     const int F = (z <= 0.0) ? 0 : (z >= (double)kFinePos ? kFinePos : (int)z);
     return (int)fpos[F];
   };
@@ -738,10 +755,15 @@ __global__ __launch_bounds__(kBinTile) void k_lnphi_binned(const double* __restr
     const int c_from = max(r0 - c0, 0);
     for (int c = c_from; c < nb; ++c) {
       const BinRec r = sb[c];
-      double re, smp;
-      bin_phase0(k, r, re, smp);
+      if (k * k * r.v > 80.0) {                // e^{-k^2 v / 2} < e^{-40}: w (-1 - i k xi)
+        lr -= r.w;
+        li -= r.w * k * r.xi;
+        continue;
+      }
+      double re, im, smpg;
+      bin_phase(k, r, re, im, smpg);
       lr = fma(r.w, re, lr);
-      li = fma(r.w, smp, li);
+      li = fma(r.w, smpg, li);
     }
   }
   if (n < n_k) {

@@ -61,17 +61,17 @@ inline TablesLayout tables_layout(const sgl_shape& s, int B) {
 //   positive side: kFinePos bins log-uniform on [kXiLo, kXiHi], bin 0 also
 //   takes 0 < xi < kXiLo; negative side: kFineNeg bins log-uniform on |xi|
 //   in [kXiNegLo, kXiNegHi].
-constexpr int kFinePos = 2048;
-constexpr int kFineNeg = 256;
+constexpr int kFinePos = 1024;
+constexpr int kFineNeg = 128;
 constexpr int kFineBins = kFinePos + kFineNeg;
 constexpr double kXiLo = 1e-12;
 constexpr double kXiHi = 30.0;
 constexpr double kXiNegLo = 1e-8;
 constexpr double kXiNegHi = 30.0;
-// Per-slice coarse lattice of the two-point amplitudes: 32 fine bins each.
-constexpr int kCoarse = 32;
+// Per-slice coarse lattice of the two-point amplitudes: 16 fine bins each.
+constexpr int kCoarse = 16;
 constexpr int kSliceBins = kFineBins / kCoarse;
-constexpr int kMoments = 2;        // fine bins carry w, w xi (centroid)
+constexpr int kMoments = 3;        // fine bins carry w, w xi, w xi^2 (centroid and in-bin variance)
 constexpr int kRec = 4;            // compacted bin record: xi/(4 pi), xi, w, v
 constexpr int kMaxMasses = 256;
 constexpr int kMaxB = 1024;
