@@ -372,7 +372,8 @@ def stage_times(sv, S):
     L.sgl_bin_pass(P(sv.tables), B, cs, P(sv.hist), st); mark("bin_pass")
     L.sgl_bin_reduce(P(sv.tables), B, cs, P(sv.hist), st); mark("bin_reduce")
     L.sgl_lnphi_binned(P(sv.tables), P(sv.hist), B, cs, P(sv.lam), P(sv.amp), st); mark("lnphi_binned")
-    L.sgl_quadform(P(sv.tables), P(sv.amp), B, cs, 1, P(sv.psi2), st); mark("quadform")
+    L.sgl_quadform(P(sv.tables), P(sv.amp), B, cs, 1, P(sv.psi2c), st); mark("quadform")
+    L.sgl_psi2_interp(P(sv.psi2c), B, cs, P(sv.psi2), st); mark("psi2_interp")
     L.sgl_phi_pack(P(sv.lam), P(sv.psi2), P(sv.tables), B, cs, sv.precision, P(sv.phi), st); mark("phi_pack")
     if not sv._filon_ready:
         L.sgl_filon_build(P(sv.tables), B, cs, sv.precision, P(sv.E), st)

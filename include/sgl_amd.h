@@ -35,6 +35,13 @@ typedef enum sgl_status {
 #define SGL_FLAG_FILAMENTS 4  /* add the cylinder-filament population: per (slice, mass)
                                  120 u bins x 13 orientation nodes (the "fil" ingredient) */
 
+/* Batched path: slice amplitudes and Psi_2 at every SGL_PSI2_STRIDE-th k node
+ * (sub-grid node jc = k node jc * stride of the log grid, the last one up to
+ * stride - 1 spacings past k_hi), then cubic Lagrange in ln k onto every
+ * node (sgl_psi2_interp).  Grids with n_k - 1 < 3 * SGL_PSI2_STRIDE use
+ * every node. */
+#define SGL_PSI2_STRIDE 8
+
 /* precision of the MFMA contractions */
 #define SGL_PREC_FP64 0   /* plain path: FP64 vector ALU */
 #define SGL_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation */
@@ -126,11 +133,12 @@ sgl_status sgl_bin_reduce(const void* tables, int32_t B, const sgl_shape* shape,
                           void* stream);
 
 /*
- * Batched path, step 2: exponent from the histograms (bin centroid plus the
- * in-bin variance correction), FP64 range reduction, FP32 sin/cos, FP64
- * compensated small-phase terms and FP64 accumulation.  lam: FP64
- * [B][n_k][2]; amp (with SGL_FLAG_CLUSTERING): the slice amplitudes as FP32
- * [B][n_z_max][n_k][2], written for the instance's own slices only (the
+ * Batched path, step 2: exponent from the histograms (each bin a Gaussian
+ * record: centroid and in-bin variance), FP64 range reduction, FP32
+ * sin/cos, FP64 compensated small-phase terms and FP64 accumulation.  lam:
+ * FP64 [B][n_k][2]; amp (with SGL_FLAG_CLUSTERING): the slice amplitudes as
+ * FP32 [B][n_z_max][n_kc][2] on the two-point sub-grid, n_kc =
+ * sgl_psi2_nodes(shape), written for the instance's own slices only (the
  * two-point term consumes them in FP32).
  * Replaces: Lam._sep_raw (sgl_full.py:3040-3086) on binned cells.
  */
@@ -140,12 +148,27 @@ sgl_status sgl_lnphi_binned(const void* tables, const void* hist, int32_t B,
 /*
  * Two-point term Psi_2[b][n] = 1/2 sum_ij C_ij amp_i amp_j (complex square,
  * no conjugate), result FP64.  precision SGL_PREC_FP64: amp is FP64 (as
- * written by sgl_lnphi_exact), FP64 sums; SGL_PREC_FP32: amp is FP32 (as
- * written by sgl_lnphi_binned), U = A C on FP32 MFMA with FP32 accumulation.
+ * written by sgl_lnphi_exact), FP64 sums, psi2 [B][n_k][2] on every node;
+ * SGL_PREC_FP32: amp is FP32 on the sub-grid (as written by
+ * sgl_lnphi_binned), U = A C on FP32 MFMA with FP32 accumulation, psi2
+ * [B][n_kc][2] on the sub-grid (expand it with sgl_psi2_interp).
  * Replaces: Lam.Q (sgl_full.py:3353-3397), nord = 1.
  */
 sgl_status sgl_quadform(const void* tables, const void* amp, int32_t B, const sgl_shape* shape,
                         int32_t precision, double* psi2, void* stream);
+
+/* Node count n_kc of the batched two-point sub-grid (host). */
+int32_t sgl_psi2_nodes(const sgl_shape* shape);
+
+/*
+ * psi2[b][n] (FP64, every k node) from psi2c[b][jc] on the sub-grid by
+ * cubic Lagrange in ln k (four nearest sub-grid nodes, one-sided at the
+ * ends); exact at the sub-grid nodes.
+ * Replaces: Lam._Q_interpolated (sgl_full.py:2953-3022), the reference's
+ * spline of Psi_2 from its coarse grid onto the inversion grid.
+ */
+sgl_status sgl_psi2_interp(const double* psi2c, int32_t B, const sgl_shape* shape, double* psi2,
+                           void* stream);
 
 /*
  * phi = window(k) exp(lam + psi2) packed as the inversion operand with the

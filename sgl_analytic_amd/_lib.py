@@ -22,10 +22,11 @@ EXPORTS = (
     "sgl_quadform", "sgl_phi_pack", "sgl_filon_build", "sgl_invert_gemm", "sgl_invert_fp64",
     "sgl_diagnostics", "sgl_invert_fp64_at", "sgl_mlp_gemm", "sgl_mlp_colsum", "sgl_to_bf16",
     "sgl_lnphi_interp", "sgl_set_tuning", "sgl_get_tuning",
-    "sgl_bin_pass", "sgl_bin_reduce",
+    "sgl_bin_pass", "sgl_bin_reduce", "sgl_psi2_nodes", "sgl_psi2_interp",
 )
 EPI_PLAIN, EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU = 0, 1, 2, 3
 TUNE_BIN_THREADS, TUNE_LNPHI_BLOCK, TUNE_GEMM_BM = 0, 1, 2
+PSI2_STRIDE = 8  # SGL_PSI2_STRIDE: batched two-point sub-grid stride
 
 
 class CShape(ctypes.Structure):
@@ -65,6 +66,8 @@ def load(path: str = LIB_PATH):
     L.sgl_filon_bytes.argtypes = [sp, i32]
     L.sgl_get_tuning.restype = i32
     L.sgl_get_tuning.argtypes = [i32]
+    L.sgl_psi2_nodes.restype = i32
+    L.sgl_psi2_nodes.argtypes = [sp]
     sigs = {
         "sgl_tables_build": [vp, i32, sp, vp, vp, vp],
         "sgl_lnphi_exact": [vp, i32, sp, vp, vp, vp, vp],
@@ -86,6 +89,7 @@ def load(path: str = LIB_PATH):
         "sgl_to_bf16": [vp, sz, vp, vp],
         "sgl_lnphi_interp": [vp, ctypes.c_long, i32, i32, sp, vp, vp],
         "sgl_set_tuning": [i32, i32],
+        "sgl_psi2_interp": [vp, i32, sp, vp, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

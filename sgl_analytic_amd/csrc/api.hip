@@ -162,6 +162,17 @@ sgl_status sgl_quadform(const void* tables, const void* amp, int32_t B, const sg
                                      as_stream(stream));
 }
 
+int32_t sgl_psi2_nodes(const sgl_shape* shape) {
+  if (!shape_ok(shape)) return 0;
+  return psi2_nodes(*shape);
+}
+
+sgl_status sgl_psi2_interp(const double* psi2c, int32_t B, const sgl_shape* shape, double* psi2,
+                           void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !psi2c || !psi2) return SGL_ERR_INVALID;
+  return (sgl_status)launch_psi2_interp(psi2c, *shape, B, psi2, as_stream(stream));
+}
+
 sgl_status sgl_phi_pack(const double* lam, const double* psi2, const void* tables, int32_t B,
                         const sgl_shape* shape, int32_t precision, void* phi_op, void* stream) {
   if (!shape_ok(shape) || !batch_ok(B) || !lam || !tables || !phi_op) return SGL_ERR_INVALID;

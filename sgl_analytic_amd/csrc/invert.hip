@@ -144,6 +144,33 @@ __global__ __launch_bounds__(256) void k_quadform_mfma(const double* __restrict_
   }
 }
 
+// Psi_2 from the sub-grid (node jc at k node jc S) onto every k node: cubic
+// Lagrange in the node index (ln k is uniform in it) on the four sub-grid
+// nodes around n, one-sided in the first and last interval.  At n = jc S the
+// weights are exactly (0, 1, 0, 0) or its shifts, so sub-grid values pass
+// through unchanged.
+__global__ void k_psi2_interp(const double* __restrict__ psi2c, sgl_shape sh, int B,
+                              double* __restrict__ psi2) {
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (size_t)B * sh.n_k) return;
+  const int n = (int)(gid % sh.n_k);
+  const int b = (int)(gid / sh.n_k);
+  const int S = psi2_stride(sh), nc = psi2_nodes(sh);
+  if (S == 1) {                                  // every node on the sub-grid
+    psi2[2 * gid] = psi2c[2 * gid];
+    psi2[2 * gid + 1] = psi2c[2 * gid + 1];
+    return;
+  }
+  int j = n / S - 1;
+  j = max(0, min(j, nc - 4));
+  const double s = (double)(n - j * S) / S;   // in [0, 3]
+  const double l0 = -(s - 1) * (s - 2) * (s - 3) / 6.0, l1 = s * (s - 2) * (s - 3) / 2.0;
+  const double l2 = -s * (s - 1) * (s - 3) / 2.0, l3 = s * (s - 1) * (s - 2) / 6.0;
+  const double* y = psi2c + 2 * ((size_t)b * nc + j);
+  psi2[2 * gid] = l0 * y[0] + l1 * y[2] + l2 * y[4] + l3 * y[6];
+  psi2[2 * gid + 1] = l0 * y[1] + l1 * y[3] + l2 * y[5] + l3 * y[7];
+}
+
 // --------------------------------------------------------------- window --
 __device__ __forceinline__ double bspline_window(double k, double K) {
   // cubic B-spline of support [0, K], 1 at k = 0 (sgl_full.py:83-94)
@@ -522,10 +549,11 @@ int launch_quadform(const double* cov, const double* cosmo, const void* amp, int
     hipLaunchKernelGGL(k_quadform, grid, dim3(256), 0, st, cov, cosmo,
                        (const double*)amp, sh.n_z_max, sh.n_k, psi2);
   } else {
-    const int per_wg = 4 * kQmTiles * 16;
-    dim3 grid((sh.n_k + per_wg - 1) / per_wg, B);
+    // batched path: amplitudes and Psi_2 on the sub-grid
+    const int per_wg = 4 * kQmTiles * 16, n_kc = psi2_nodes(sh);
+    dim3 grid((n_kc + per_wg - 1) / per_wg, B);
     hipLaunchKernelGGL(k_quadform_mfma, grid, dim3(256), 0, st, cov, cosmo, (const float*)amp,
-                       sh.n_z_max, sh.n_k, psi2);
+                       sh.n_z_max, n_kc, psi2);
   }
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
@@ -536,6 +564,14 @@ int launch_phi_pack(const double* lam, const double* psi2, const double* knodes,
   size_t n = (size_t)B * (Kp / 2);
   hipLaunchKernelGGL(k_phi_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, lam, psi2,
                      knodes, sh, B, prec, Kp, out);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+int launch_psi2_interp(const double* psi2c, const sgl_shape& sh, int B, double* psi2,
+                       hipStream_t st) {
+  const size_t n = (size_t)B * sh.n_k;
+  hipLaunchKernelGGL(k_psi2_interp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, psi2c, sh,
+                     B, psi2);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 

@@ -15,18 +15,22 @@
 //   k_rowf / k_cols   per-row and per-column records of the slice lattice;
 //   k_bin_cells       every cell off the band |1 - x^2| < kBandW with the FP32
 //                     lens profile into a per-slice LDS histogram of fine
-//                     signed-log jump bins (w, w xi, beta w, beta w xi), then
+//                     signed-log jump bins (w, w xi, w xi^2, beta w), then
 //                     the slice's fine partials and its coarse two-point
-//                     moments (beta w, beta w xi, beta w xi^2 per 32 fine bins);
+//                     moments (beta w, beta w xi, beta w xi^2 per kCoarse
+//                     fine bins);
 //   k_bin_band        the band cells with the FP64 profile, added into the
 //                     slice's partials;
 //   k_reduce_fine     slice-order sum of the partials (no atomics across
-//                     workgroups: reproducible), compaction, prefix moments;
+//                     workgroups: reproducible), compaction into Gaussian
+//                     records (centroid, in-bin variance), prefix moments;
 //   k_lnphi_binned    per k: records with k xi <= 1/2 through Taylor series
 //                     on prefix moments, the rest as w e^{i k xibar - k^2 v/2}
 //                     with the phase in FP64 reduced to revolutions, sin/cos
 //                     on the FP32 hardware path, compensated small-phase terms
-//                     and accumulation in FP64.
+//                     and accumulation in FP64;
+//   k_amp_binned      the slice amplitudes from the coarse records, the same
+//                     way, at every kPsi2Stride-th k node.
 #include "sgl_amd_internal.h"
 
 namespace sgl {
@@ -709,15 +713,12 @@ template <int kBinTile>
 __global__ __launch_bounds__(kBinTile) void k_lnphi_binned(const double* __restrict__ hist,
                                                       const double* __restrict__ cosmo,
                                                       const double* __restrict__ knodes,
-                                                      sgl_shape sh, double* __restrict__ lam,
-                                                      float* __restrict__ amp) {
-  static_assert(kBinTile >= kSliceBins, "a slice's coarse records are staged in one pass");
+                                                      sgl_shape sh, double* __restrict__ lam) {
   __shared__ BinRec sb[kBinTile];
   const int t = threadIdx.x, b = blockIdx.y;
   const int n = blockIdx.x * kBinTile + t;
   const int n_k = sh.n_k;
   const HistLayout HL = hist_layout(sh, gridDim.y);
-  const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
   const double k = knodes[min(n, n_k - 1)];
   const int nf = (int)hist[HL.fcnt + b];
   const double* fr = hist + HL.frec + (size_t)b * kFineBins * kRec;
@@ -770,54 +771,68 @@ __global__ __launch_bounds__(kBinTile) void k_lnphi_binned(const double* __restr
     lam[2 * ((size_t)b * n_k + n)] = lr;
     lam[2 * ((size_t)b * n_k + n) + 1] = li;
   }
-  if (!(sh.flags & SGL_FLAG_CLUSTERING)) return;
+}
+
+// Slice amplitudes a_i(k) = sum_c beta w_c (e^{i k xi_c - k^2 v_c/2} - 1)
+// over the slice's coarse Gaussian records, at the two-point sub-grid nodes
+// (node jc at k node jc S of the log grid): workgroup = (instance, kAmpTile
+// sub-grid nodes), one node per thread.  FP32 output: the two-point term
+// consumes the amplitudes in FP32.
+constexpr int kAmpTile = 128;
+__global__ __launch_bounds__(kAmpTile) void k_amp_binned(const double* __restrict__ hist,
+                                                         const double* __restrict__ cosmo,
+                                                         sgl_shape sh, float* __restrict__ amp) {
+  static_assert(kAmpTile >= kSliceBins, "a slice's coarse records are staged in one pass");
+  __shared__ BinRec sb[kAmpTile];
+  const int t = threadIdx.x, b = blockIdx.y;
+  const int nc = psi2_nodes(sh), S = psi2_stride(sh);
+  const int jc = blockIdx.x * kAmpTile + t;
+  const HistLayout HL = hist_layout(sh, gridDim.y);
+  const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
+  const double k = knode_at(sh, min(jc, nc - 1) * S);
   // Coarse records of positive bins below coarse bin cb have mean + 2 sd
   // <= 2 x (upper edge) (the spread of a bin's content is at most half its
   // width), so with 2 k edge <= kTayEps their e^{i k xi - k^2 v/2} - 1 is the
-  // moment series through n = 9 (truncation < 1e-10 relative).
+  // moment series through n = 9 (truncation < 1e-10 relative).  The
+  // workgroup's last node has the lowest first direct record.
   auto coarse_cut = [&](double kk) {
     const double z = (log(0.5 * kTayEps / kk) - log(kXiLo)) * (kFinePos / log(kXiHi / kXiLo));
     const int F = (z <= 0.0) ? 0 : (z >= (double)kFinePos ? kFinePos : (int)z);
     return F / kCoarse;
   };
   const int cb0 = coarse_cut(k);
-  const int cb_lo = coarse_cut(knodes[min((int)(blockIdx.x + 1) * kBinTile, n_k) - 1]);
+  const int cb_lo = coarse_cut(knode_at(sh, (min((int)(blockIdx.x + 1) * kAmpTile, nc) - 1) * S));
   const double q = k * k;
-  for (int i = 0; i < sh.n_z_max; ++i) {
-    double ar = 0.0, ai = 0.0;
-    if (i < n_z) {
-      const size_t si = (size_t)b * sh.n_z_max + i;
-      const int ns = (int)hist[HL.scnt + si];
-      const double* sr = hist + HL.srec + si * kSliceBins * kRec;
-      const double* cpo = hist + HL.cpos + si * (kCoarsePos + 1);
-      const int c0 = (int)cpo[cb0], c_lo = (int)cpo[cb_lo];
-      {
-        const double* M = hist + HL.cpfx + (si * (kCoarsePos + 1) + c0) * kTayC;
-        ar = q * (-M[1] * 0.5 + q * (M[3] * (1.0 / 24) + q * (-M[5] * (1.0 / 720) + q * (M[7] * (1.0 / 40320)))));
-        ai = k * (M[0] + q * (-M[2] * (1.0 / 6) + q * (M[4] * (1.0 / 120) + q * (-M[6] * (1.0 / 5040) + q * (M[8] * (1.0 / 362880))))));
-      }
-      __syncthreads();
-      if (c_lo + t < ns) {
-        const double* r = sr + (size_t)(c_lo + t) * kRec;
-        sb[t] = BinRec{r[0], r[1], r[2], r[3]};
-      }
-      __syncthreads();
-      for (int c = c0 - c_lo; c < ns - c_lo; ++c) {
-        const BinRec r = sb[c];
-        if (q * r.v > 80.0) {              // e^{-k^2 v / 2} < e^{-40}: the record adds -w
-          ar -= r.w;
-          continue;
-        }
-        double re, im, smpg;
-        bin_phase(k, r, re, im, smpg);
-        ar = fma(r.w, re, ar);
-        ai = fma(r.w, im, ai);
-      }
+  for (int i = 0; i < n_z; ++i) {                 // n_z is uniform over the workgroup
+    const size_t si = (size_t)b * sh.n_z_max + i;
+    const int ns = (int)hist[HL.scnt + si];
+    const double* sr = hist + HL.srec + si * kSliceBins * kRec;
+    const double* cpo = hist + HL.cpos + si * (kCoarsePos + 1);
+    const int c0 = (int)cpo[cb0], c_lo = (int)cpo[cb_lo];
+    double ar, ai;
+    {
+      const double* M = hist + HL.cpfx + (si * (kCoarsePos + 1) + c0) * kTayC;
+      ar = q * (-M[1] * 0.5 + q * (M[3] * (1.0 / 24) + q * (-M[5] * (1.0 / 720) + q * (M[7] * (1.0 / 40320)))));
+      ai = k * (M[0] + q * (-M[2] * (1.0 / 6) + q * (M[4] * (1.0 / 120) + q * (-M[6] * (1.0 / 5040) + q * (M[8] * (1.0 / 362880))))));
     }
-    if (n < n_k && i < n_z) {
-      const size_t o = ((size_t)b * sh.n_z_max + i) * n_k + n;
-      reinterpret_cast<float2*>(amp)[o] = make_float2((float)ar, (float)ai);
+    __syncthreads();
+    if (c_lo + t < ns) {
+      const double* r = sr + (size_t)(c_lo + t) * kRec;
+      sb[t] = BinRec{r[0], r[1], r[2], r[3]};
     }
+    __syncthreads();
+    for (int c = c0 - c_lo; c < ns - c_lo; ++c) {
+      const BinRec r = sb[c];
+      if (q * r.v > 80.0) {              // e^{-k^2 v / 2} < e^{-40}: the record adds -w
+        ar -= r.w;
+        continue;
+      }
+      double re, im, smpg;
+      bin_phase(k, r, re, im, smpg);
+      ar = fma(r.w, re, ar);
+      ai = fma(r.w, im, ai);
+    }
+    if (jc < nc) reinterpret_cast<float2*>(amp)[si * nc + jc] = make_float2((float)ar, (float)ai);
   }
 }
 
@@ -869,10 +884,14 @@ int launch_lnphi_binned(const double* hist, const double* cosmo, const double* k
                         const sgl_shape& sh, int B, double* lam, float* amp, hipStream_t st) {
   if (g_sgl_tune[SGL_TUNE_LNPHI_BLOCK] == 0) {
     dim3 grid((sh.n_k + 127) / 128, B);
-    hipLaunchKernelGGL(k_lnphi_binned<128>, grid, dim3(128), 0, st, hist, cosmo, knodes, sh, lam, amp);
+    hipLaunchKernelGGL(k_lnphi_binned<128>, grid, dim3(128), 0, st, hist, cosmo, knodes, sh, lam);
   } else {
     dim3 grid((sh.n_k + 255) / 256, B);
-    hipLaunchKernelGGL(k_lnphi_binned<256>, grid, dim3(256), 0, st, hist, cosmo, knodes, sh, lam, amp);
+    hipLaunchKernelGGL(k_lnphi_binned<256>, grid, dim3(256), 0, st, hist, cosmo, knodes, sh, lam);
+  }
+  if (sh.flags & SGL_FLAG_CLUSTERING) {
+    dim3 grid((psi2_nodes(sh) + kAmpTile - 1) / kAmpTile, B);
+    hipLaunchKernelGGL(k_amp_binned, grid, dim3(kAmpTile), 0, st, hist, cosmo, sh, amp);
   }
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }

@@ -84,6 +84,26 @@ constexpr double kTayEps = 0.5;
 constexpr int kTayC = 9;
 constexpr int kCoarsePos = kFinePos / kCoarse;
 
+// Two-point term of the batched path on a sub-grid of the k nodes: every
+// kPsi2Stride-th node (node jc is k node jc * S of the log grid; the last one
+// may lie up to S - 1 node spacings beyond k_hi, on the same log grid), then
+// cubic Lagrange in ln k back onto every node.  Grids too short for a
+// four-point rule keep every node (S = 1).
+constexpr int kPsi2Stride = SGL_PSI2_STRIDE;
+__host__ __device__ inline int psi2_stride(const sgl_shape& s) {
+  return (s.n_k - 1 >= 3 * kPsi2Stride) ? kPsi2Stride : 1;
+}
+__host__ __device__ inline int psi2_nodes(const sgl_shape& s) {
+  const int S = psi2_stride(s);
+  return (s.n_k - 1 + S - 1) / S + 1;
+}
+// k at node n of the log grid (the formula that fills the knodes array; n
+// may exceed n_k - 1).
+__device__ inline double knode_at(const sgl_shape& s, int n) {
+  const double l0 = log10(s.k_lo), l1 = log10(s.k_hi);
+  return pow(10.0, l0 + (l1 - l0) * n / (s.n_k - 1));
+}
+
 // Histogram buffer (doubles): per-slice partial fine moments, compacted fine
 // records + count per instance, compacted coarse records + count per slice,
 // per instance the bin -> first-record map (kFineBins + 1 entries) and the
@@ -134,6 +154,8 @@ int launch_lnphi_binned(const double* hist, const double* cosmo, const double* k
                         const sgl_shape& sh, int B, double* lam, float* amp, hipStream_t st);
 int launch_quadform(const double* cov, const double* cosmo, const void* amp, int B,
                     const sgl_shape& sh, int prec, double* psi2, hipStream_t st);
+int launch_psi2_interp(const double* psi2c, const sgl_shape& sh, int B, double* psi2,
+                       hipStream_t st);
 int launch_phi_pack(const double* lam, const double* psi2, const double* knodes,
                     const sgl_shape& sh, int B, int prec, void* out, hipStream_t st);
 int launch_filon(const double* knodes, const sgl_shape& sh, int prec, void* out, hipStream_t st);

@@ -123,9 +123,15 @@ class Solver:
         self.tables = buf(L.sgl_tables_bytes(cs, B))
         self.status = torch.zeros(1, dtype=torch.int32, device=self.dev)
         self.lam = torch.zeros((B, nk, 2), dtype=torch.float64, device=self.dev)
-        # slice amplitudes: FP64 on the exact path, FP32 on the batched path
-        adt = torch.float64 if path == "exact" else torch.float32
-        self.amp = torch.zeros((B, nz, nk, 2), dtype=adt, device=self.dev)
+        # slice amplitudes: FP64 on every k node on the exact path; FP32 on the
+        # two-point sub-grid (every PSI2_STRIDE-th node) on the batched path,
+        # with Psi_2 computed there (psi2c) and interpolated onto every node
+        if path == "exact":
+            self.amp = torch.zeros((B, nz, nk, 2), dtype=torch.float64, device=self.dev)
+        else:
+            self.n_kc = int(L.sgl_psi2_nodes(cs))
+            self.amp = torch.zeros((B, nz, self.n_kc, 2), dtype=torch.float32, device=self.dev)
+            self.psi2c = torch.zeros((B, self.n_kc, 2), dtype=torch.float64, device=self.dev)
         self.psi2 = torch.zeros((B, nk, 2), dtype=torch.float64, device=self.dev)
         self.diag = torch.zeros((B, 8), dtype=torch.float64, device=self.dev)
         if path == "exact":
@@ -155,9 +161,14 @@ class Solver:
             ck(L.sgl_lnphi_binned(_ptr(self.tables), _ptr(self.hist), B, cs, _ptr(self.lam),
                                   _ptr(self.amp), st), "sgl_lnphi_binned")
         if self.shape.clustering:
-            qp = PREC_FP64 if self.path == "exact" else PREC_FP32
-            ck(L.sgl_quadform(_ptr(self.tables), _ptr(self.amp), B, cs, qp,
-                              _ptr(self.psi2), st), "sgl_quadform")
+            if self.path == "exact":
+                ck(L.sgl_quadform(_ptr(self.tables), _ptr(self.amp), B, cs, PREC_FP64,
+                                  _ptr(self.psi2), st), "sgl_quadform")
+            else:
+                ck(L.sgl_quadform(_ptr(self.tables), _ptr(self.amp), B, cs, PREC_FP32,
+                                  _ptr(self.psi2c), st), "sgl_quadform")
+                ck(L.sgl_psi2_interp(_ptr(self.psi2c), B, cs, _ptr(self.psi2), st),
+                   "sgl_psi2_interp")
             psi = _ptr(self.psi2)
         ck(L.sgl_phi_pack(_ptr(self.lam), psi, _ptr(self.tables), B, cs, self.precision,
                           _ptr(self.phi), st), "sgl_phi_pack")

@@ -49,3 +49,21 @@ def test_size_queries_and_rejections():
     rc = L.sgl_invert_gemm(None, None, 1, ctypes.byref(sh), 7, None, None)
     assert rc == _lib.SGL_ERR_INVALID
     assert L.sgl_version().startswith(b"sgl_amd")
+
+
+def test_psi2_subgrid_node_count():
+    """Two-point sub-grid: node jc is k node jc * 8; the last sub-node covers
+    k node n_k - 1 (it may lie past k_hi); short grids keep every node."""
+    from sgl_analytic_amd import _lib
+    from sgl_analytic_amd.solver import Shape
+    L = _lib.load()
+    S = _lib.PSI2_STRIDE
+    for n_k in (8192, 8193, 1024, 512, 25, 24, 2):
+        sh = Shape(n_k=n_k).c()
+        n_kc = L.sgl_psi2_nodes(ctypes.byref(sh))
+        if n_k - 1 >= 3 * S:
+            assert (n_kc - 1) * S >= n_k - 1 > (n_kc - 2) * S, (n_k, n_kc)
+        else:
+            assert n_kc == n_k
+    assert L.sgl_psi2_nodes(ctypes.byref(Shape().c())) == 1025
+    assert L.sgl_psi2_nodes(None) == 0
