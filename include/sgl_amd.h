@@ -115,14 +115,26 @@ sgl_status sgl_lnphi_exact(const void* tables, int32_t B, const sgl_shape* shape
                            double* lam, double* amp, double* work, void* stream);
 
 /*
+ * Convergence and shear kap[i], gam[i] of an NFW profile of amplitude ks
+ * truncated at x_t, at n points x[i] (device arrays, x = R / r_s > 0):
+ * SGL_PREC_FP64, the exact path's
+ * closed forms; SGL_PREC_FP32, the batched path's FP32 form (the one
+ * sgl_bin_pass evaluates per cell).  Outside x_t: kap = 0, gam = the
+ * monopole shear of the enclosed mass.
+ * Replaces: nfw_trunc_kg (sgl_full.py:660-669).
+ */
+sgl_status sgl_trunc_nfw_kg(const double* x, int32_t n, double ks, double xt, int32_t precision,
+                            double* kap, double* gam, void* stream);
+
+/*
  * Batched path, step 1: histogram every cell into a fine signed-log jump
- * lattice (w, w xi per bin; FP64 sums in LDS per slice) and a per-slice
- * coarse lattice (beta w, beta w xi, beta w xi^2), then reduce the slices in
- * order into compacted bin records with their prefix moments.
- * sgl_bin_cells = sgl_bin_pass (the FP32-profile pass over every cell off
- * the |1 - x^2| < 1/4 band) followed by sgl_bin_reduce (the FP64 band pass
- * and the slice-order reduction); the two halves are exported so that each
- * can be timed on its own.
+ * lattice (w, w xi, w xi^2 per bin; FP64 sums in LDS per slice) and a
+ * per-slice coarse lattice (beta w, beta w xi, beta w xi^2), then reduce
+ * the slices in order into compacted Gaussian bin records with their prefix
+ * moments.  sgl_bin_cells = sgl_bin_pass (every cell, FP32 lens profile in
+ * a form without cancellation at x = 1, FP32 jump log) followed by
+ * sgl_bin_reduce (the slice-order reduction); the two halves are exported
+ * so that each can be timed on its own.
  * Replaces: Cells.add / _add_direct_kernel (sgl_full.py:1314-1383, 971-1046).
  */
 sgl_status sgl_bin_cells(const void* tables, int32_t B, const sgl_shape* shape, void* hist,

@@ -23,6 +23,7 @@ EXPORTS = (
     "sgl_diagnostics", "sgl_invert_fp64_at", "sgl_mlp_gemm", "sgl_mlp_colsum", "sgl_to_bf16",
     "sgl_lnphi_interp", "sgl_set_tuning", "sgl_get_tuning",
     "sgl_bin_pass", "sgl_bin_reduce", "sgl_psi2_nodes", "sgl_psi2_interp",
+    "sgl_trunc_nfw_kg",
 )
 EPI_PLAIN, EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU = 0, 1, 2, 3
 TUNE_BIN_THREADS, TUNE_LNPHI_BLOCK, TUNE_GEMM_BM = 0, 1, 2
@@ -90,6 +91,7 @@ def load(path: str = LIB_PATH):
         "sgl_lnphi_interp": [vp, ctypes.c_long, i32, i32, sp, vp, vp],
         "sgl_set_tuning": [i32, i32],
         "sgl_psi2_interp": [vp, i32, sp, vp, vp],
+        "sgl_trunc_nfw_kg": [vp, i32, ctypes.c_double, ctypes.c_double, i32, vp, vp, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -4,6 +4,8 @@
 // nothing.
 #include <string.h>
 
+#include <cmath>
+
 #include "sgl_amd_internal.h"
 
 using namespace sgl;
@@ -160,6 +162,13 @@ sgl_status sgl_quadform(const void* tables, const void* amp, int32_t B, const sg
   const double* T = static_cast<const double*>(tables);
   return (sgl_status)launch_quadform(T + L.cov, T + L.cosmo, amp, B, *shape, precision, psi2,
                                      as_stream(stream));
+}
+
+sgl_status sgl_trunc_nfw_kg(const double* x, int32_t n, double ks, double xt, int32_t precision,
+                            double* kap, double* gam, void* stream) {
+  if (!x || !kap || !gam || n < 1 || !(xt > 0.0) || !std::isfinite(ks)) return SGL_ERR_INVALID;
+  if (precision != SGL_PREC_FP64 && precision != SGL_PREC_FP32) return SGL_ERR_UNSUPPORTED;
+  return (sgl_status)launch_trunc_nfw(x, n, ks, xt, precision, kap, gam, as_stream(stream));
 }
 
 int32_t sgl_psi2_nodes(const sgl_shape* shape) {

@@ -13,14 +13,12 @@
 // |ph| = 1e-2 (sgl_full.py:2599-2613).
 // Binned path:
 //   k_rowf / k_cols   per-row and per-column records of the slice lattice;
-//   k_bin_cells       every cell off the band |1 - x^2| < kBandW with the FP32
-//                     lens profile into a per-slice LDS histogram of fine
+//   k_bin_cells       every cell with the FP32 lens profile (a form without
+//                     cancellation at x = 1) into a per-slice LDS histogram of fine
 //                     signed-log jump bins (w, w xi, w xi^2, beta w), then
 //                     the slice's fine partials and its coarse two-point
 //                     moments (beta w, beta w xi, beta w xi^2 per kCoarse
 //                     fine bins);
-//   k_bin_band        the band cells with the FP64 profile, added into the
-//                     slice's partials;
 //   k_reduce_fine     slice-order sum of the partials (no atomics across
 //                     workgroups: reproducible), compaction into Gaussian
 //                     records (centroid, in-bin variance), prefix moments;
@@ -239,17 +237,6 @@ __device__ __forceinline__ int fine_bin(double xi) {
 }
 
 
-__device__ __forceinline__ void deposit(double* lf, double* lc, double xi, double ws, double bw) {
-  const int f = fine_bin(xi);
-  atomicAdd(&lf[f * kMoments], ws);
-  atomicAdd(&lf[f * kMoments + 1], ws * xi);
-  atomicAdd(&lf[f * kMoments + 2], ws * xi * xi);
-  const int cb = f / kCoarse;
-  atomicAdd(&lc[cb * 3], bw);
-  atomicAdd(&lc[cb * 3 + 1], bw * xi);
-  atomicAdd(&lc[cb * 3 + 2], bw * xi * xi);
-}
-
 // Fine histogram of a slice in LDS, moment-major (a bin's moment at an 8-byte
 // stride, so the lanes of a wave spread over all banks), four FP64 moments:
 // w, w xi, w xi^2 (centroid and in-bin variance of the exponent's records)
@@ -342,10 +329,7 @@ __global__ void k_cols(const double* __restrict__ uedge, sgl_shape sh, int B,
   reinterpret_cast<ColF*>(hist + hist_layout(sh, B).cols)[l] = c;
 }
 
-// One thread per (instance, slice, mass) row: the 64-byte RowF record of the
-// main pass, with the band columns found by bisection on the same FP64
-// predicate the band pass would evaluate, so the two passes split every row
-// exactly.  (1 - x^2) decreases along the row (x = ug_l s increases with l).
+// One thread per (instance, slice, mass) row: its RowF record.
 __global__ __launch_bounds__(256) void k_rowf(const double* __restrict__ rows,
                                               const double* __restrict__ cosmo,
                                               const double* __restrict__ uedge, sgl_shape sh,
@@ -356,33 +340,12 @@ __global__ __launch_bounds__(256) void k_rowf(const double* __restrict__ rows,
   const int b = (int)(gid / ((size_t)sh.n_z_max * sh.n_m));
   const int i = (int)((gid / sh.n_m) % sh.n_z_max);
   if (i >= (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10]) return;
-  const RowConst c = row_const(rows + gid * kRowStride);
-  RowF f;
-  f.warea = (c.valid != 0.0) ? c.warea : 0.0;
-  f.beta = c.beta;
-  f.ks = (float)c.ks; f.xt = (float)c.xt; f.ixt = (float)(1.0 / c.xt); f.ic1 = (float)c.ic1;
-  f.mut4ks = (float)c.mut4ks; f.i0 = (float)c.i0; f.jj = (float)c.jj;
-  f.s = (float)c.s; f.log_s = (float)log(c.s); f.inv_s = (float)(1.0 / c.s);
-  auto wcol = [&](int l) {
-    const double x = sqrt(uedge[l] * uedge[l + 1]) * c.s;
-    return (1.0 - x) * (1.0 + x);
-  };
-  auto first = [&](double thr, bool strict) {   // first l with w(l) < thr (or <= thr)
-    int a = 0, z = sh.n_b;
-    while (a < z) {
-      const int m = (a + z) >> 1;
-      const double w = wcol(m);
-      if (strict ? (w < thr) : (w <= thr)) z = m; else a = m + 1;
-    }
-    return a;
-  };
-  f.la = first(kBandW, true);
-  f.lb = first(-kBandW, false);
-  reinterpret_cast<RowF*>(hist + hist_layout(sh, B).rowf)[gid] = f;
+  reinterpret_cast<RowF*>(hist + hist_layout(sh, B).rowf)[gid] =
+      make_rowf(row_const(rows + gid * kRowStride));
 }
 
-// Main pass: every cell outside the band, FP32 lens profile, FP32 log for
-// the jump.  A thread owns one mass row (its RowF record in registers) and
+// Binning pass: every cell, FP32 lens profile (trunc_kg_row_f32), FP32 log
+// for the jump.  A thread owns one mass row (its RowF record in registers) and
 // walks the impact-parameter columns; the lanes of a wave are consecutive
 // masses at the same column, which keeps the x >= x_t and x < 1 branches
 // nearly uniform (x / x_t = u r_vir / r_t hardly depends on the mass) and
@@ -413,7 +376,6 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
       for (int l = part; l < n_b; l += parts) {
         const ColF cf = nxt;                     // the next column is fetched a cell ahead
         nxt = CF[min(l + parts, n_b - 1)];
-        if (l >= rc.la && l < rc.lb) continue;  // the FP64 band pass takes it
         float kf, gf;
         trunc_kg_row_f32(cf.ug * rc.s, cf.lug + rc.log_s, cf.rug * rc.inv_s, rc, kf, gf);
         double xi, ws;
@@ -481,57 +443,6 @@ __global__ __launch_bounds__(kFilThreads) void k_bin_fil(const double* __restric
   fine_hist_flush(H, hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments,
                   hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3, true, t,
                   kFilThreads);
-}
-
-// Band pass: the cells with |1 - x^2| < kBandW in FP64 (series branch of F, h
-// and the |x - 1| < 1e-5 bridge of the truncation term), added into the
-// slice's partials by the only workgroup that touches them.  The band
-// columns [la, lb) of each row come from its RowF record.  Lanes are rows
-// and walk their bands in step, so a wave's lanes sit at the same offset
-// into their bands and the x < 1 / x > 1 branches stay nearly uniform.
-constexpr int kBandThreads = 512;
-__global__ __launch_bounds__(kBandThreads) void k_bin_band(const double* __restrict__ rows,
-                                                           const double* __restrict__ cosmo,
-                                                           const double* __restrict__ uedge,
-                                                           sgl_shape sh,
-                                                           double* __restrict__ hist) {
-  __shared__ SliceStage S;
-  __shared__ double lf[kFineBins * kMoments];
-  __shared__ double lc[kSliceBins * 3];
-  const int t = threadIdx.x, i = blockIdx.x, b = blockIdx.y;
-  const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
-  if (i >= n_z) return;
-  const HistLayout HL = hist_layout(sh, gridDim.y);
-  for (int q = t; q < kFineBins * kMoments; q += kBandThreads) lf[q] = 0.0;
-  for (int q = t; q < kSliceBins * 3; q += kBandThreads) lc[q] = 0.0;
-  stage_slice(S, rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh);
-  __syncthreads();
-  const RowF* RF = reinterpret_cast<const RowF*>(hist + HL.rowf) +
-                   ((size_t)b * sh.n_z_max + i) * sh.n_m;
-  const int rpp = min(sh.n_m, kBandThreads);    // rows per pass
-  const int parts = kBandThreads / rpp;         // column phases
-  const int tj = t % rpp, part = t / rpp;
-  if (part < parts) {
-    for (int j = tj; j < sh.n_m; j += rpp) {
-      const RowF& rf = RF[j];
-      if (rf.warea == 0.0) continue;
-      const RowConst& rc = S.rc[j];
-      for (int l = rf.la + part; l < rf.lb; l += parts) {
-        double kap, gam, xi, ws;
-        trunc_kg_row(S.ug[l] * rc.s, rc, kap, gam);
-        cell_jump_f32log(kap, gam, rc.warea * S.du2[l], xi, ws);
-        if (ws == 0.0) continue;
-        deposit(lf, lc, xi, ws, rc.beta * ws);
-      }
-    }
-  }
-  __syncthreads();
-  double* P = hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments;
-  for (int q = t; q < kFineBins * kMoments; q += kBandThreads)
-    if (lf[q] != 0.0) P[q] += lf[q];
-  double* Q = hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3;
-  for (int q = t; q < kSliceBins * 3; q += kBandThreads)
-    if (lc[q] != 0.0) Q[q] += lc[q];
 }
 
 // Per instance: add the slice partials in slice order and compact the
@@ -868,8 +779,6 @@ int launch_bin_pass(const double* rows, const double* frows, const double* cosmo
 
 int launch_bin_reduce(const double* rows, const double* cosmo, const double* uedge,
                       const sgl_shape& sh, int B, double* hist, hipStream_t st) {
-  hipLaunchKernelGGL(k_bin_band, dim3(sh.n_z_max, B), dim3(kBandThreads), 0, st, rows, cosmo, uedge,
-                     sh, hist);
   hipLaunchKernelGGL(k_reduce_fine, dim3(B), dim3(256), 0, st, cosmo, sh, hist);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
@@ -893,6 +802,36 @@ int launch_lnphi_binned(const double* hist, const double* cosmo, const double* k
     dim3 grid((psi2_nodes(sh) + kAmpTile - 1) / kAmpTile, B);
     hipLaunchKernelGGL(k_amp_binned, grid, dim3(kAmpTile), 0, st, hist, cosmo, sh, amp);
   }
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+// Truncated-NFW (kappa, gamma) at n given x for one row (k_s, x_t): FP64
+// closed forms (the exact path's trunc_kg_row) or the batched path's FP32
+// form (trunc_kg_row_f32 on the row's RowF record).
+__global__ void k_trunc_nfw(const double* __restrict__ x, int n, double ks, double xt, int prec,
+                            double* __restrict__ kap, double* __restrict__ gam) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double R[kRowStride] = {ks, 1.0, xt, 1.0, 1.0, 0.0, 0.0, 0.0};
+  const RowConst rc = row_const(R);
+  double k, g;
+  if (prec == SGL_PREC_FP64) {
+    trunc_kg_row(x[i], rc, k, g);
+  } else {
+    const RowF f = make_rowf(rc);
+    float kf, gf;
+    trunc_kg_row_f32((float)x[i], (float)log(x[i]), (float)(1.0 / x[i]), f, kf, gf);
+    k = kf;
+    g = gf;
+  }
+  kap[i] = k;
+  gam[i] = g;
+}
+
+int launch_trunc_nfw(const double* x, int n, double ks, double xt, int prec, double* kap,
+                     double* gam, hipStream_t st) {
+  hipLaunchKernelGGL(k_trunc_nfw, dim3((n + 255) / 256), dim3(256), 0, st, x, n, ks, xt, prec, kap,
+                     gam);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 

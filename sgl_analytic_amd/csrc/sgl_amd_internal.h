@@ -108,7 +108,7 @@ __device__ inline double knode_at(const sgl_shape& s, int n) {
 // records + count per instance, compacted coarse records + count per slice,
 // per instance the bin -> first-record map (kFineBins + 1 entries) and the
 // prefix moments of the positive records ((kFinePos + 1) x kTay), the same
-// two per slice for the positive coarse records, the 64-byte RowF record of
+// two per slice for the positive coarse records, the 80-byte RowF record of
 // every (slice, mass) row, and the 32-byte record of every column.
 struct HistLayout {
   size_t partial, cpart, frec, fcnt, srec, scnt, fpos, fpfx, cpos, cpfx, rowf, cols, total;
@@ -127,7 +127,7 @@ __host__ __device__ inline HistLayout hist_layout(const sgl_shape& s, int B) {
   L.cpos = L.fpfx + round32((size_t)B * (kFinePos + 1) * kTay);
   L.cpfx = L.cpos + round32((size_t)B * s.n_z_max * (kCoarsePos + 1));
   L.rowf = L.cpfx + round32((size_t)B * s.n_z_max * (kCoarsePos + 1) * kTayC);
-  L.cols = L.rowf + round32((size_t)B * s.n_z_max * s.n_m * 8);
+  L.cols = L.rowf + round32((size_t)B * s.n_z_max * s.n_m * (sizeof(RowF) / sizeof(double)));
   L.total = L.cols + round32((size_t)s.n_b * 4);
   return L;
 }
@@ -154,6 +154,8 @@ int launch_lnphi_binned(const double* hist, const double* cosmo, const double* k
                         const sgl_shape& sh, int B, double* lam, float* amp, hipStream_t st);
 int launch_quadform(const double* cov, const double* cosmo, const void* amp, int B,
                     const sgl_shape& sh, int prec, double* psi2, hipStream_t st);
+int launch_trunc_nfw(const double* x, int n, double ks, double xt, int prec, double* kap,
+                     double* gam, hipStream_t st);
 int launch_psi2_interp(const double* psi2c, const sgl_shape& sh, int B, double* psi2,
                        hipStream_t st);
 int launch_phi_pack(const double* lam, const double* psi2, const double* knodes,

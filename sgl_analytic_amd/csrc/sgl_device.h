@@ -363,17 +363,18 @@ __device__ inline void trunc_kg_row(double x, const RowConst& rc, double& kap, d
   gam = rc.ks * (kb - ku);
 }
 
-// Per-row record of the batched main pass (64 B, built once per row by
-// k_rowf): FP64 weight and bias, FP32 profile constants with log s and 1/s
-// (so that log x and 1/x of a cell are one add and one multiply), and the
-// columns [la, lb) of the row inside the band |1 - x^2| < kBandW, which the
-// FP64 band pass takes.  warea = 0 marks an empty row.
+// Per-row record of the batched pass (80 B, built once per row by k_rowf):
+// FP64 weight and bias, FP32 profile constants with log s and 1/s (so that
+// log x and 1/x of a cell are one add and one multiply), and the FP32
+// coefficients s2[n] of the small-x series of the truncation term I2 (see
+// trunc_kg_row_f32).  warea = 0 marks an empty row.
+constexpr int kI2Terms = 8;
 struct RowF {
   double warea, beta;
-  float ks, xt, ixt, ic1, mut4ks, i0, jj, s, log_s, inv_s;
-  int la, lb;
+  float ks, xt, ixt, ic1, mut4ks, s, log_s, inv_s;
+  float s2[kI2Terms];
 };
-static_assert(sizeof(RowF) == 64, "RowF is one 64-byte record");
+static_assert(sizeof(RowF) == 80, "RowF is one 80-byte record");
 
 // Hardware FP32 reciprocal, reciprocal square root, square root (1 ulp) and
 // natural log (v_log_f32 is log2; positive normal arguments only).
@@ -390,69 +391,133 @@ __device__ __forceinline__ float log1p_hw(float y) {
   return (d == 0.0f) ? y : log_hw(u) * (y * rcp_hw(d));
 }
 
-// FP32 evaluation of the same closed forms for the batched path, used away
-// from x = 1 only (|1 - x^2| >= kBandW = 0.1, so x <= 0.949 or x >= 1.049):
-// the terms that cancel as x -> 1 lose at most ~1/|1 - x^2|^{3/2} ~ 30 ulp
-// there.  Hardware rcp / rsq / sqrt / log; relative error of kappa and gamma
-// ~2e-6, far below the lattice binning error.  The caller takes the FP64
-// trunc_kg_row inside the band.
-constexpr double kBandW = 0.1;
+// FP32 truncated-NFW profile of the batched path, in a form without the
+// cancellation of the closed forms at x = 1, so no cell needs FP64.  With
+// c = x_t, u = 1 - x^2 and
+//   P = sqrt(2 / (x (1 + x))),  Q = sqrt((c + x) / (1 + c)),
+//   R = sqrt((c - x) / (1 + c)),  z = (x - 1)(c - x) / (2 x (1 + c)),
+//   G(z) = 2F1(1/2, 1/2; 3/2; z) = asinh(sqrt(-z)) / sqrt(-z) (z < 0),
+//                                  asin(sqrt(z)) / sqrt(z)    (z > 0),
+// the arccosh / arccos terms of the closed forms (trunc_kg_row) are
+// sqrt(|u|) P G(z_e) and sqrt(|u|) P R G(z) on both sides of x = 1 (the
+// truncation term's arccosh argument (x^2 + c) / (x (1 + c)) is 1 - 2z; the
+// projected NFW truncated at c in this form: Takada & Jain 2003, MNRAS 340,
+// 580), and the untruncated terms cancel exactly:
+//   kappa / ks = -2 R (Q - P G(z)) / u,
+//   kbar / ks  = 4 (ln(x/2) - L + P R G(z)) / x^2,   gamma = kbar - kappa,
+// L = ln((c + q) / (2 (1 + c))) - (q - c) / (1 + c), q = (1 + c) R Q.
+// Near x = 1 (|x - 1| < kProfT1) Q - P G vanishes like t = x - 1: it is
+// summed from (Q - 1)/t, (P - 1)/t and (G - 1)/t, each formed without a
+// difference, and u = -t (1 + x) cancels t exactly.  G - 1 for |z| <
+// kProfZs is the hypergeometric series.  For x < kProfXs and x / c <
+// kProfAs the bracket of kbar cancels (it is O(x^2)); there
+//   kbar / ks = 4 h(x) / x^2 - 4 I2,  I2 = sum_n s2[n] (x/c)^(2n)
+// with h the untruncated mass in the small-x form and s2 from k_rowf.
+// Relative error of kappa and gamma < 1e-5 over x < c (hardware rcp / rsq /
+// sqrt / log), against the FP64 closed forms.
+constexpr float kProfT1 = 0.1f, kProfZs = 0.1f, kProfXs = 0.5f, kProfAs = 0.35f;
+
+// (G(z) - 1) / z = sum_{n >= 1} g_n z^(n-1), g_{n+1} = g_n (n + 1/2)^2 / ((n + 1)(n + 3/2))
+__device__ __forceinline__ float hyp_gm1z(float z) {
+  float s = 12155.0f / 1245184.0f;
+  s = s * z + 6435.0f / 557056.0f;
+  s = s * z + 143.0f / 10240.0f;
+  s = s * z + 231.0f / 13312.0f;
+  s = s * z + 63.0f / 2816.0f;
+  s = s * z + 35.0f / 1152.0f;
+  s = s * z + 5.0f / 112.0f;
+  s = s * z + 3.0f / 40.0f;
+  s = s * z + 1.0f / 6.0f;
+  return s;
+}
+
 __device__ inline void trunc_kg_row_f32(float x, float lx, float rx, const RowF& rc, float& kap,
                                         float& gam) {
-  const float xt = rc.xt;
+  constexpr float kLn2f = 0.69314718f;
+  const float c = rc.xt, ic1 = rc.ic1;
   const float rx2 = rx * rx;
-  if (x >= xt) {
+  if (x >= c) {
     kap = 0.0f;
     gam = rc.mut4ks * rx2;
     return;
   }
-  constexpr float kLn2f = 0.69314718f;
-  const float a = x * rc.ixt;
-  const float w = (1.0f - x) * (1.0f + x);
-  const float ic1 = rc.ic1;
-  float F, h, I1, I2;
-  const bool small = a < (float)kTcASmall;
-  if (small) {
-    I1 = rc.i0 + 0.5f * a * a * rc.jj;
-    I2 = 0.5f * rc.i0 + 0.125f * a * a * rc.jj;
-  }
-  if (x < 1.0f) {
-    const float isu = rsq_hw(w), su = w * isu, iu = isu * isu;
-    const float e = (1.0f - x) * rx;
-    const float A0 = log_hw(1.0f + e + sqrt_hw(e * (e + 2.0f)));   // arccosh(1/x), 1/x - 1 >= 0.05
-    F = (A0 * isu - 1.0f) * iu;
-    if (x < (float)kNfwXH) {
-      const float i1s = rcp_hw(1.0f + su);
-      h = ((kLn2f - lx) * x * x * i1s + log1p_hw(-0.5f * x * x * i1s)) * isu;
-    } else {
-      h = lx - kLn2f + A0 * isu;
-    }
-    if (!small) {
-      const float c = xt, q = sqrt_hw((c - x) * (c + x));
-      const float d = (x - 1.0f) * (x - c) * rx * ic1;
-      const float L = log_hw((c + q) * ic1 * 0.5f) - (q - c) * ic1;
-      const float At = log1p_hw(d + sqrt_hw(d * (d + 2.0f)));
-      I1 = (q * ic1 - 1.0f) * iu + (A0 - At) * iu * isu;
-      I2 = (L + (A0 - At) * isu) * rx2;
-    }
+  const float t = x - 1.0f, opx = 1.0f + x;
+  const float P = rsq_hw(0.5f * x * opx);
+  const float R2 = (c - x) * ic1;
+  const float Q = sqrt_hw((c + x) * ic1), R = sqrt_hw(R2);
+  const float z = 0.5f * t * R2 * rx;
+  float G, gm1z = 0.0f;
+  if (fabsf(z) < kProfZs) {
+    gm1z = hyp_gm1z(z);
+    G = 1.0f + z * gm1z;
+  } else if (z < 0.0f) {
+    const float is = rsq_hw(-z), s = -z * is;
+    G = log_hw(s + sqrt_hw(1.0f - z)) * is;
   } else {
-    const float v = -w, isv = rsq_hw(v), iv = isv * isv;
-    const float B0 = 2.0f * asinf(sqrt_hw((x - 1.0f) * 0.5f * rx));
-    F = (1.0f - B0 * isv) * iv;
-    h = lx - kLn2f + B0 * isv;
-    if (!small) {
-      const float c = xt, q = sqrt_hw((c - x) * (c + x));
-      const float d = (x - 1.0f) * (x - c) * rx * ic1;
-      const float L = log_hw((c + q) * ic1 * 0.5f) - (q - c) * ic1;
-      const float Bt = 2.0f * asinf(sqrt_hw(fmaxf(-d, 0.0f) * 0.5f));
-      I1 = (1.0f - q * ic1) * iv - (B0 - Bt) * iv * isv;
-      I2 = (L + (B0 - Bt) * isv) * rx2;
-    }
+    const float is = rsq_hw(z), s = z * is;
+    G = asinf(s) * is;
   }
-  const float ku = 2.0f * F - 2.0f * I1;
-  const float kb = 4.0f * h * rx2 - 4.0f * I2;
+  float ku;
+  if (fabsf(t) < kProfT1) {
+    const float ro = rcp_hw(opx);
+    const float qm1t = ic1 * rcp_hw(Q + 1.0f);
+    const float pm1t = -(2.0f + x) * rx * ro * rcp_hw(P + 1.0f);
+    const float gm1t = gm1z * 0.5f * R2 * rx;
+    ku = 2.0f * R * (qm1t - pm1t - P * gm1t) * ro;
+  } else {
+    ku = -2.0f * R * (Q - P * G) * rcp_hw(-t * opx);
+  }
+  float kb;
+  const float a = x * rc.ixt;
+  if (x < kProfXs && a < kProfAs) {
+    const float w = -t * opx, isu = rsq_hw(w), su = w * isu;
+    const float i1s = rcp_hw(1.0f + su);
+    const float h = ((kLn2f - lx) * x * x * i1s + log1p_hw(-0.5f * x * x * i1s)) * isu;
+    const float y = a * a;
+    float I2 = rc.s2[kI2Terms - 1];
+#pragma unroll
+    for (int n = kI2Terms - 2; n >= 0; --n) I2 = I2 * y + rc.s2[n];
+    kb = 4.0f * h * rx2 - 4.0f * I2;
+  } else {
+    const float q = R * Q * (1.0f + c);
+    const float L = log_hw((c + q) * ic1 * 0.5f) - (q - c) * ic1;
+    kb = 4.0f * (lx - kLn2f - L + P * R * G) * rx2;
+  }
   kap = rc.ks * ku;
   gam = rc.ks * (kb - ku);
+}
+
+// Row record of a truncated NFW row (FP64 constants in, FP32 record out):
+// s2[n] = b_n M_{2n+1}(c), b_n the coefficients of (1 - sqrt(1 - y)) / y =
+// sum b_n y^n and M_m(c) = int_0^1 v^m / (v + c)^2 dv (24-point
+// Gauss-Legendre, exact to round-off for c > 0.2), so that the truncation
+// term I2 = int_0^1 v / (v + c)^2 (1 - sqrt(1 - a^2 v^2)) / (a^2 v^2) dv is
+// sum_n s2[n] a^(2n).
+__device__ inline RowF make_rowf(const RowConst& c) {
+  RowF f;
+  f.warea = (c.valid != 0.0) ? c.warea : 0.0;
+  f.beta = c.beta;
+  f.ks = (float)c.ks;
+  f.xt = (float)c.xt;
+  f.ixt = (float)(1.0 / c.xt);
+  f.ic1 = (float)c.ic1;
+  f.mut4ks = (float)c.mut4ks;
+  f.s = (float)c.s;
+  f.log_s = (float)log(c.s);
+  f.inv_s = (float)(1.0 / c.s);
+  double m[kI2Terms];
+  for (int n = 0; n < kI2Terms; ++n) m[n] = 0.0;
+  for (int j = 0; j < 24; ++j) {
+    const double v = kGl24X[j], g = kGl24W[j] * v / ((v + c.xt) * (v + c.xt)), v2 = v * v;
+    double p = g;
+    for (int n = 0; n < kI2Terms; ++n) { m[n] += p; p *= v2; }
+  }
+  double bn = 0.5;
+  for (int n = 0; n < kI2Terms; ++n) {
+    f.s2[n] = (float)(bn * m[n]);
+    bn *= (2.0 * n + 1.0) / (2.0 * n + 4.0);
+  }
+  return f;
 }
 
 // Jump of one cell: xi = -ln detA and the tilted weight w e^{-xi} = w detA;
