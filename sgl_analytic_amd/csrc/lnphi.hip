@@ -368,7 +368,23 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
   const int rpp = min(sh.n_m, kBinThreads);     // rows per pass
   const int parts = kBinThreads / rpp;          // column phases
   const int tj = t % rpp, part = t / rpp;
-  if (part < parts) {
+  auto cell = [&](const RowF& rc, const ColF& cf) {
+    float kf, gf;
+    trunc_kg_row_f32(cf.ug * rc.s, cf.lug + rc.log_s, cf.rug * rc.inv_s, rc, kf, gf);
+    double xi, ws;
+    cell_jump_f32log(kf, gf, rc.warea * cf.du2, xi, ws);
+    if (ws != 0.0) deposit4(H, xi, ws, rc.beta * ws);
+  };
+  if (rpp % 64 == 0) {
+    // a wave's lanes share the column phase: the column index is wave-uniform
+    // and the column record comes through the scalar cache
+    const int wpart = __builtin_amdgcn_readfirstlane(part);
+    for (int j = tj; j < sh.n_m; j += rpp) {
+      const RowF rc = RF[j];
+      if (rc.warea == 0.0) continue;
+      for (int l = wpart; l < n_b; l += parts) cell(rc, CF[l]);
+    }
+  } else if (part < parts) {
     for (int j = tj; j < sh.n_m; j += rpp) {
       const RowF rc = RF[j];
       if (rc.warea == 0.0) continue;
@@ -376,12 +392,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
       for (int l = part; l < n_b; l += parts) {
         const ColF cf = nxt;                     // the next column is fetched a cell ahead
         nxt = CF[min(l + parts, n_b - 1)];
-        float kf, gf;
-        trunc_kg_row_f32(cf.ug * rc.s, cf.lug + rc.log_s, cf.rug * rc.inv_s, rc, kf, gf);
-        double xi, ws;
-        cell_jump_f32log(kf, gf, rc.warea * cf.du2, xi, ws);
-        if (ws == 0.0) continue;
-        deposit4(H, xi, ws, rc.beta * ws);
+        cell(rc, cf);
       }
     }
   }

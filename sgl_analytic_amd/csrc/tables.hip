@@ -485,24 +485,83 @@ __global__ __launch_bounds__(256) void k_cov(const Cosmo* __restrict__ cosmo,
   }
 }
 
+// Side stream of the table build, one per device, created on the first call
+// that is not under graph capture (never during capture; without it the
+// build runs serially on the caller's stream).  The growth ODE (one thread
+// per instance, latency-bound) overlaps sigma(M), and the slice covariance
+// overlaps the rows; the caller's stream waits for both before the build
+// returns, so the result is ordered on the caller's stream as before.
+// Host-side state: not for concurrent calls from several host threads.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t e_bg = nullptr, e_sig = nullptr, j_growth = nullptr, j_cov = nullptr;
+  bool ok = false, tried = false;
+};
+constexpr int kMaxDevices = 64;
+static SideStream g_side[kMaxDevices];
+
+static SideStream* side_stream(hipStream_t st) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  SideStream& S = g_side[dev];
+  if (S.ok) return &S;
+  if (S.tried) return nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return nullptr;
+  S.tried = true;
+  if (hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  hipEvent_t* ev[4] = {&S.e_bg, &S.e_sig, &S.j_growth, &S.j_cov};
+  for (hipEvent_t* e : ev)
+    if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return nullptr;
+  S.ok = true;
+  return &S;
+}
+
 int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes_h,
                  double* derived, double* slices, double* mass, double* rows, double* frows,
                  double* cov, double* work, int* status, hipStream_t st) {
   const Cosmo* C = reinterpret_cast<const Cosmo*>(cosmo);
   Derived* D = reinterpret_cast<Derived*>(derived);
   const double dlnM = (log(sh.m_hi_h) - log(sh.m_lo_h)) / sh.n_m;
+  SideStream* S = side_stream(st);
+  const bool cov_on = (sh.flags & SGL_FLAG_CLUSTERING) != 0;
+  bool fail = false;
+  auto ck = [&](hipError_t e) { fail |= (e != hipSuccess); };
   hipLaunchKernelGGL(k_background, dim3(B), dim3(256), 0, st, C, sh.n_z_max, D, slices);
-  hipLaunchKernelGGL(k_growth, dim3((B + 63) / 64), dim3(64), 0, st, C, B, sh.n_z_max, D, slices);
+  // growth needs the slices and Omega_r, Omega_DE (background); it writes
+  // D(z) into the slices and D0 into the derived record
+  hipStream_t sg = st;
+  if (S) {
+    ck(hipEventRecord(S->e_bg, st));
+    ck(hipStreamWaitEvent(S->s, S->e_bg, 0));
+    sg = S->s;
+  }
+  hipLaunchKernelGGL(k_growth, dim3((B + 63) / 64), dim3(64), 0, sg, C, B, sh.n_z_max, D, slices);
+  if (S) ck(hipEventRecord(S->j_growth, S->s));
+  // sigma(M) needs the background only; it writes A_norm and the mass table
   hipLaunchKernelGGL(k_sigma, dim3(B), dim3(256), 0, st, C, m_nodes_h, sh.n_m, D, mass);
+  // the covariance needs chi and A_norm (background, sigma), not the growth
+  if (cov_on) {
+    hipStream_t sc = st;
+    if (S) {
+      ck(hipEventRecord(S->e_sig, st));
+      ck(hipStreamWaitEvent(S->s, S->e_sig, 0));
+      sc = S->s;
+    }
+    hipLaunchKernelGGL(k_cov, dim3(B), dim3(256), 0, sc, C, D, sh.n_z_max, work, status, cov);
+    if (S) ck(hipEventRecord(S->j_cov, S->s));
+  }
   // Sheth-Tormen normalisation 1 / (1 + 2^-p Gamma(1/2 - p) / sqrt(pi)), p = 0.3, and
   // the bias delta_c constant 0.6 (3 pi / 2)^(2/3)
   const double a_st = 1.0 / (1.0 + std::pow(2.0, -kPst) * std::tgamma(0.5 - kPst) / std::sqrt(kPi));
   const double dc0 = 0.6 * std::pow(1.5 * kPi, 2.0 / 3.0);
+  if (S) ck(hipStreamWaitEvent(st, S->j_growth, 0));
   hipLaunchKernelGGL(k_rows, dim3(sh.n_z_max, B), dim3(256), 0, st, C, D, slices, mass,
                      sh.n_z_max, sh.n_m, dlnM, a_st, dc0, rows,
                      (sh.flags & SGL_FLAG_FILAMENTS) ? frows : (double*)nullptr);
-  if (sh.flags & SGL_FLAG_CLUSTERING)
-    hipLaunchKernelGGL(k_cov, dim3(B), dim3(256), 0, st, C, D, sh.n_z_max, work, status, cov);
+  if (S && cov_on) ck(hipStreamWaitEvent(st, S->j_cov, 0));
+  if (fail) return SGL_ERR_HIP;
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
