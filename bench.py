@@ -45,11 +45,11 @@ PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "hbm": 8000.0}
 # wave instruction), i.e. SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_{F32,F64} summed
 # over one dispatch of the B = 4096 workload (27.55e9 cells) divided by its
 # cell count (profiles/batched_B4096_counters.txt).
-CELL_FLOPS = 98.6
+CELL_FLOPS = 85.2
 # HBM bytes of one k_bin_cells dispatch of the B = 4096 workload: FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 KiB, the factors being what those counters read for a 1 GiB
 # device copy (bench.py --copy-calibration) divided by 1 GiB.
-BIN_TRAFFIC_B4096 = 1.68375e6 * 1024 * 2.0 + 7.92176e6 * 1024 * 1.0
+BIN_TRAFFIC_B4096 = 2.10816e6 * 1024 * 2.0 + 6.03000e6 * 1024 * 1.0
 
 
 def lhs_params(B, seed=SEED):
