@@ -61,6 +61,7 @@ __device__ __forceinline__ double f_k(double Ok, double H0, double chi) {
 // computed once per instance into EH98Const.
 struct EH98Const {
   double keq, ksilk, ac, bc, s, ab, bb, bnode, fb, fc;
+  double lq0, lksilk;   // ln(13.41 k_eq), ln k_silk
 };
 
 __device__ inline EH98Const eh98_const(const Cosmo& c) {
@@ -90,29 +91,38 @@ __device__ inline EH98Const eh98_const(const Cosmo& c) {
   k.bnode = 8.41 * pow(wm, 0.435);
   k.fb = fbm;
   k.fc = Oc / Om;
+  k.lq0 = log(13.41 * k.keq);
+  k.lksilk = log(k.ksilk);
   return k;
 }
 
-__device__ __forceinline__ double eh98_t0(double q, double a_c, double b_c) {
-  double C1 = 14.2 / a_c + 386.0 / (1.0 + 69.9 * pow(q, 1.08));
-  double num = log(2.718281828459045 + 1.8 * b_c * q);
-  return num / (num + C1 * q * q);
-}
-
-__device__ inline double eh98_T(const EH98Const& e, double k) {
-  double q = k / (13.41 * e.keq);
-  double ks = k * e.s;
-  double t54 = ks / 5.4;
-  double fk = 1.0 / (1.0 + t54 * t54 * t54 * t54);
-  double TC = fk * eh98_t0(q, 1.0, e.bc) + (1.0 - fk) * eh98_t0(q, e.ac, e.bc);
-  double bn = e.bnode / ks;
-  double s3 = e.s / cbrt(1.0 + bn * bn * bn);
-  double x = k * s3;
-  double jo = (x < 1e-8) ? 1.0 - x * x / 6.0 : sin(x) / x;
-  double t52 = ks / 5.2;
-  double bbk = e.bb / ks;
-  double TB = (eh98_t0(q, 1.0, 1.0) / (1.0 + t52 * t52) +
-               e.ab / (1.0 + bbk * bbk * bbk) * exp(-pow(k / e.ksilk, 1.4))) * jo;
+// T(k) of Eisenstein & Hu 1998 (sgl.py:391-445, _T_eh98) with lk = ln k given by
+// the caller: the powers q^1.08 and (k / k_silk)^1.4 are exp of multiples
+// of ln q and ln(k / k_silk), q^1.08 and ln(e + 1.8 b_c q) are shared by the
+// three T_0 terms (relative difference from separate pow / log calls
+// ~1e-15).
+__device__ inline double eh98_T(const EH98Const& e, double k, double lk) {
+  const double q = k / (13.41 * e.keq);
+  const double pq = exp(1.08 * (lk - e.lq0));               // q^1.08
+  const double c386 = 386.0 / (1.0 + 69.9 * pq);
+  const double q2 = q * q;
+  const double nbc = log(2.718281828459045 + 1.8 * e.bc * q);
+  const double n1 = log(2.718281828459045 + 1.8 * q);
+  const double ks = k * e.s;
+  const double t54 = ks / 5.4;
+  const double fk = 1.0 / (1.0 + t54 * t54 * t54 * t54);
+  const double t0_1bc = nbc / (nbc + (14.2 + c386) * q2);          // T0(q, 1, b_c)
+  const double t0_acbc = nbc / (nbc + (14.2 / e.ac + c386) * q2);  // T0(q, a_c, b_c)
+  const double t0_11 = n1 / (n1 + (14.2 + c386) * q2);             // T0(q, 1, 1)
+  const double TC = fk * t0_1bc + (1.0 - fk) * t0_acbc;
+  const double bn = e.bnode / ks;
+  const double s3 = e.s / cbrt(1.0 + bn * bn * bn);
+  const double x = k * s3;
+  const double jo = (x < 1e-8) ? 1.0 - x * x / 6.0 : sin(x) / x;
+  const double t52 = ks / 5.2;
+  const double bbk = e.bb / ks;
+  const double TB = (t0_11 / (1.0 + t52 * t52) +
+                     e.ab / (1.0 + bbk * bbk * bbk) * exp(-exp(1.4 * (lk - e.lksilk)))) * jo;
   return e.fb * TB + e.fc * TC;
 }
 

@@ -192,10 +192,11 @@ __global__ __launch_bounds__(256) void k_sigma(const Cosmo* __restrict__ cosmo,
   __syncthreads();
   for (int i = t; i < kSigmaK; i += blockDim.x) {
     double k = kk[i];
-    double T = eh98_T(e, k);
+    const double lk = (-5.0 + lstep * i) * 2.302585092994046;   // ln k
+    double T = eh98_T(e, k, lk);
     double wl = (i == 0) ? 0.5 * (kk[1] - kk[0])
                 : (i == kSigmaK - 1) ? 0.5 * (kk[i] - kk[i - 1]) : 0.5 * (kk[i + 1] - kk[i - 1]);
-    g[i] = k * k * pow(k, c.ns) * T * T * wl / (2.0 * kPi * kPi);
+    g[i] = k * k * exp(c.ns * lk) * T * T * wl / (2.0 * kPi * kPi);
   }
   __syncthreads();
   // top-hat anchor
@@ -216,24 +217,49 @@ __global__ __launch_bounds__(256) void k_sigma(const Cosmo* __restrict__ cosmo,
   const double rhom = c.Om * kRhoc0Unit * c.h * c.h;
   if (t == 0) der[b].Anorm = Anorm;
   const double a6 = pow(0.43, 6.0);
-  for (int j = t; j < n_m; j += blockDim.x) {
-    double M = m_nodes_h[j] / c.h;
-    double R = cbrt(3.0 * M / (4.0 * kPi * rhom));
+  // G threads per mass when n_m < 256 (G = 256 / nm2, nm2 the power of two
+  // >= n_m): thread (g, j) sums the g-th contiguous chunk of the k nodes for
+  // mass j; the chunks are added in order afterwards.
+  int nm2 = 1;
+  while (nm2 < n_m) nm2 <<= 1;
+  const int G = 256 / nm2;
+  const int j = t % nm2, gq = t / nm2;
+  const int ia = gq * kSigmaK / G, ib = (gq + 1) * kSigmaK / G;
+  __shared__ double pS[256], pSd[256];
+  double R = 0.0;
+  if (j < n_m) {
+    const double M = m_nodes_h[j] / c.h;
+    R = cbrt(3.0 * M / (4.0 * kPi * rhom));
     double S = 0.0, Sd = 0.0;
-    for (int i = 0; i < kSigmaK; ++i) {
-      double x = kk[i] * R;
-      double x2 = x * x, x3 = x2 * x;
-      double y = a6 * x3 * x3;
-      double Ws = 1.0 / (1.0 + y);
-      double dW = -6.0 * a6 * x2 * x3 * Ws * Ws;  // dWs/dx
+    for (int i = ia; i < ib; ++i) {
+      const double x = kk[i] * R;
+      const double x2 = x * x, x3 = x2 * x;
+      const double y = a6 * x3 * x3;
+      // 1 / (1 + y): hardware reciprocal and two Newton steps (full FP64)
+      const double den = 1.0 + y;
+      double Ws = __builtin_amdgcn_rcp(den);
+      Ws = Ws * (2.0 - den * Ws);
+      Ws = Ws * (2.0 - den * Ws);
+      const double dW = -6.0 * a6 * x2 * x3 * Ws * Ws;  // dWs/dx
       S += g[i] * Ws * Ws;
       Sd += g[i] * 2.0 * Ws * dW * kk[i];
+    }
+    pS[t] = S;
+    pSd[t] = Sd;
+  }
+  __syncthreads();
+  if (gq == 0 && j < n_m) {
+    const double M = m_nodes_h[j] / c.h;
+    double S = 0.0, Sd = 0.0;
+    for (int q = 0; q < G; ++q) {
+      S += pS[q * nm2 + j];
+      Sd += pSd[q * nm2 + j];
     }
     double* Mr = mass + ((size_t)b * n_m + j) * kMassStride;
     Mr[0] = M;
     Mr[1] = Anorm * sqrt(S);
     Mr[2] = 0.5 * (Sd / S) * (R / 3.0);
-    Mr[3] = 0.0;
+    Mr[3] = log(Anorm) + 0.5 * log(S);           // ln sigma(M, 0), read by k_rows
   }
 }
 
@@ -242,9 +268,11 @@ __device__ inline double mu_nfw(double y) { return log1p(y) - y / (1.0 + y); }
 
 // r_vir / r200 = eta solving m^3 / mu(m) = 200 c^3 / (mu(c) Delta_vir) on
 // [c, 3c] (subhalos.py:409-441): safeguarded Newton on ln(m^3 / mu(m)).
-__device__ inline double eta_rvir(double c, double Dvir) {
-  double lt = log(200.0 * c * c * c / mu_nfw(c) / Dvir);
-  double lo = c, hi = 3.0 * c, m = 0.5 * (lo + hi);
+// mu_c = mu(c) and cr = cbrt(200 / Delta_vir) come from the caller; Newton
+// starts at m = c cr (the root if mu(m) = mu(c), a few per cent off).
+__device__ inline double eta_rvir(double c, double Dvir, double mu_c, double cr) {
+  double lt = log(200.0 * c * c * c / mu_c / Dvir);
+  double lo = c, hi = 3.0 * c, m = fmin(fmax(c * cr, lo), hi);
   for (int it = 0; it < 80; ++it) {
     double mu = mu_nfw(m);
     double f = log(m * m * m / mu) - lt;
@@ -273,7 +301,7 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
                                               int n_m, double dlnM, double a_st, double dc0,
                                               double* __restrict__ rows,
                                               double* __restrict__ frows) {
-  __shared__ double sk[10];
+  __shared__ double sk[13];
   // (sk[8] = rho_c0 for the filament rows)
   const int i = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
   const Cosmo c = cosmo[b];
@@ -302,35 +330,45 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
     sk[6] = 18.0 * kPi * kPi + 82.0 * dd - 39.0 * dd * dd;
     sk[7] = d.rhom;
     sk[8] = d.rhoc0;
+    sk[9] = cbrt(200.0 / sk[6]);
+    sk[10] = log(sk[0]);                         // ln of the Ludlow polynomial
+    sk[11] = log(S[kSlD]);                       // ln D(z)
+    sk[12] = cbrt(3.0 / (4.0 * kPi * 200.0 * sk[5]));
   }
   __syncthreads();
   const double D = S[kSlD];
-  const double poly = sk[0], c0 = sk[1], beta_ = sk[2], g1 = sk[3], g2 = sk[4];
-  const double rhoc = sk[5], Dvir = sk[6], rhom = sk[7], rhoc0 = sk[8];
+  const double c0 = sk[1], beta_ = sk[2], g1 = sk[3], g2 = sk[4];
+  const double Dvir = sk[6], rhom = sk[7], rhoc0 = sk[8], cr = sk[9];
+  const double lpoly = sk[10], lD = sk[11], r200f = sk[12], rhoc = sk[5];
+  const double ldc = log(kDeltaC), lqst = log(kQst), l08dc0 = log(0.8) + 2.0 * log(dc0);
   for (int j = t; j < n_m; j += blockDim.x) {
     double* R = Rs + (size_t)j * kRowStride;
     const double* Mr = mass + ((size_t)b * n_m + j) * kMassStride;
-    const double M = Mr[0], sig0 = Mr[1], dls = Mr[2];
+    const double M = Mr[0], sig0 = Mr[1], dls = Mr[2], lsig0 = Mr[3];
+    // powers as exp(b ln a) from the logs of sigma, D and the Ludlow
+    // polynomial (relative error ~1e-15 for these exponents and arguments)
+    const double lsig = lsig0 + lD;
     // Sheth-Tormen dn/dlnM (sgl.py:558-567)
     const double sig = sig0 * D;
     const double nu = (kDeltaC / sig) * (kDeltaC / sig);
     const double qn = kQst * nu;
-    const double mult = a_st * (1.0 + pow(qn, -kPst)) * sqrt(qn / (2.0 * kPi)) * exp(-0.5 * qn);
+    const double lqn = lqst + 2.0 * (ldc - lsig);
+    const double mult = a_st * (1.0 + exp(-kPst * lqn)) * sqrt(qn / (2.0 * kPi)) * exp(-0.5 * qn);
     const double dndlnM = (rhom / M) * mult * (-2.0 * dls);
-    // concentration
-    const double q = kDeltaC / (sig0 * poly);
-    const double C = c0 * pow(q, -g1) * pow(1.0 + pow(q, 1.0 / beta_), -beta_ * (g2 - g1));
+    // concentration: c0 q^-g1 (1 + q^(1/beta))^(-beta (g2 - g1)), q = delta_c / (sigma0 poly)
+    const double lq = ldc - lsig0 - lpoly;
+    const double C = c0 * exp(-g1 * lq - beta_ * (g2 - g1) * log1p(exp(lq / beta_)));
     // nfw_params (sgl.py:608-615)
-    const double r200 = cbrt(3.0 * M / (4.0 * kPi * 200.0 * rhoc));
+    const double r200 = cbrt(M) * r200f;
     const double rs = r200 / C;
     const double fC = log1p(C) - C / (1.0 + C);
     const double rhos = (200.0 / 3.0) * rhoc * C * C * C / fC;
     const double ks = rhos * rs / S[kSlScr];
     // r_vir (subhalos.py:381-385)
-    const double rvir = eta_rvir(C, Dvir) * C * rs;
+    const double rvir = eta_rvir(C, Dvir, fC, cr) * C * rs;
     // linear halo bias with the growth D (sgl_full.py:541-543)
     const double qn2 = 0.8 * (dc0 / D / sig0) * (dc0 / D / sig0);
-    const double bh = 1.0 + (qn2 - 1.0) / dc0 + 0.6 / (dc0 * (1.0 + pow(qn2, 0.3)));
+    const double bh = 1.0 + (qn2 - 1.0) / dc0 + 0.6 / (dc0 * (1.0 + exp(0.3 * (l08dc0 - 2.0 * lsig))));
     R[kRowKs] = ks;
     R[kRowRs] = rs;
     R[kRowXt] = C;
@@ -365,6 +403,7 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
 // log-spaced K from k_1 / 2 to 200 / Rs) -- sgl_full.py:2514-2538.
 constexpr int kCovK = 20000;
 constexpr int kCovQChunk = 32;
+constexpr int kCovPad = (kMaxSlices + 3) / 4 * 4;   // slices padded to whole 4-blocks
 
 __global__ __launch_bounds__(256) void k_cov(const Cosmo* __restrict__ cosmo,
                                              const Derived* __restrict__ der, int n_z_max,
@@ -373,6 +412,7 @@ __global__ __launch_bounds__(256) void k_cov(const Cosmo* __restrict__ cosmo,
   __shared__ double part[256];
   __shared__ double tabc[kCovQChunk][kMaxSlices + 1];
   __shared__ double tabs[kCovQChunk][kMaxSlices + 1];
+  __shared__ double Yc[kCovQChunk][kCovPad], Ys[kCovQChunk][kCovPad];
   __shared__ double rwq[kCovQChunk];
   const int b = blockIdx.x, t = threadIdx.x;
   const Cosmo c = cosmo[b];
@@ -392,13 +432,21 @@ __global__ __launch_bounds__(256) void k_cov(const Cosmo* __restrict__ cosmo,
   const EH98Const e = eh98_const(c);
   double* cum = work + (size_t)b * kCovK;
   // f(K) into cum, then chunk-wise trapezoid prefix sums
+  const double lK0 = log(K0), lK1 = log(K1);
   for (int i = t; i < kCovK; i += blockDim.x) {
+    const double lK = (i == kCovK - 1) ? lK1 : lK0 + lr * i;
     double K = (i == kCovK - 1) ? K1 : K0 * exp(lr * i);
     double x = K * Rs;
-    double W = (x < 1e-2) ? 1.0 - x * x / 10.0 * (1.0 - x * x / 28.0)
-                          : 3.0 * (sin(x) - x * cos(x)) / (x * x * x);
-    double T = eh98_T(e, K);
-    cum[i] = K * d.Anorm * d.Anorm * pow(K, c.ns) * T * T * W * W;
+    double W;
+    if (x < 1e-2) {
+      W = 1.0 - x * x / 10.0 * (1.0 - x * x / 28.0);
+    } else {
+      double sx, cx;
+      sincos(x, &sx, &cx);
+      W = 3.0 * (sx - x * cx) / (x * x * x);
+    }
+    double T = eh98_T(e, K, lK);
+    cum[i] = K * d.Anorm * d.Anorm * exp(c.ns * lK) * T * T * W * W;
   }
   __syncthreads();
   const int per = (kCovK + 255) / 256;
@@ -431,11 +479,33 @@ __global__ __launch_bounds__(256) void k_cov(const Cosmo* __restrict__ cosmo,
   if (t == 0) cum[0] = 0.0;
   __syncthreads();
   const double total = cum[kCovK - 1];
-  // accumulate C_ij over chunks of modes
-  double acc[(kMaxSlices * kMaxSlices + 255) / 256];
-  const int npair = n_z * n_z;
-  const int nacc = (npair + 255) / 256;
-  for (int a = 0; a < nacc; ++a) acc[a] = 0.0;
+  // C_ij = sum_q r_q^2 (c_qi c_qj + s_qi s_qj): per chunk of modes the
+  // weighted differences Yc = r (sin k chi_{i+1} - sin k chi_i), Ys = r (cos
+  // k chi_i - cos k chi_{i+1}) go to LDS once; each thread accumulates 4 x 4
+  // blocks (i0.., j0..) of the upper triangle in registers, 2 FMAs per pair
+  // and mode; C_ji is written from C_ij at the end.
+  // With ntile <= 256 blocks, G = 256 / ntile thread groups split the modes
+  // of every chunk (group gm takes modes gm, gm + G, ..) and are summed in
+  // group order at the end; beyond 256 blocks each thread takes two.
+  const int nt = (n_z + 3) / 4;                  // 4-slice blocks per side
+  const int ntile = nt * (nt + 1) / 2;           // blocks of the upper triangle
+  constexpr int kTilesPerThread = ((kMaxSlices + 3) / 4 * ((kMaxSlices + 3) / 4 + 1) / 2 + 255) / 256;
+  const int G = (ntile <= 256) ? 256 / ntile : 1;
+  const int gm = (ntile <= 256) ? t / ntile : 0;
+  double acc[kTilesPerThread][16];
+  int ti[kTilesPerThread], tj[kTilesPerThread];
+  for (int a = 0; a < kTilesPerThread; ++a) {
+    for (int e = 0; e < 16; ++e) acc[a][e] = 0.0;
+    int p = (ntile <= 256) ? ((a == 0 && gm < G) ? t % ntile : ntile) : t + a * 256;
+    int bi = 0;
+    if (p < ntile) {                             // p -> (bi, bj), bi <= bj, row-major
+      while (p >= nt - bi) { p -= nt - bi; ++bi; }
+      ti[a] = 4 * bi;
+      tj[a] = 4 * (bi + p);
+    } else {
+      ti[a] = tj[a] = -1;
+    }
+  }
   const double dchi = d.dchi;
   for (int q0 = 0; q0 < nq; q0 += kCovQChunk) {
     const int qn = min(kCovQChunk, nq - q0);
@@ -459,29 +529,64 @@ __global__ __launch_bounds__(256) void k_cov(const Cosmo* __restrict__ cosmo,
       tabc[ql][i] = cs;
     }
     __syncthreads();
-    for (int a = 0; a < nacc; ++a) {
-      int p = t + a * 256;
-      if (p >= npair) break;
-      int i = p / n_z, j = p % n_z;
-      double s = 0.0;
-      for (int ql = 0; ql < qn; ++ql) {
-        double r2 = rwq[ql] * rwq[ql];
-        double ci = tabs[ql][i + 1] - tabs[ql][i], cj = tabs[ql][j + 1] - tabs[ql][j];
-        double si = tabc[ql][i] - tabc[ql][i + 1], sj = tabc[ql][j] - tabc[ql][j + 1];
-        s += r2 * (ci * cj + si * sj);
+    // Yc, Ys over the slices (padding slices i >= n_z are zero)
+    for (int p = t; p < qn * kCovPad; p += blockDim.x) {
+      int ql = p / kCovPad, i = p % kCovPad;
+      double yc = 0.0, ys = 0.0;
+      if (i < n_z) {
+        yc = rwq[ql] * (tabs[ql][i + 1] - tabs[ql][i]);
+        ys = rwq[ql] * (tabc[ql][i] - tabc[ql][i + 1]);
       }
-      acc[a] += s;
+      Yc[ql][i] = yc;
+      Ys[ql][i] = ys;
+    }
+    __syncthreads();
+    for (int a = 0; a < kTilesPerThread; ++a) {
+      if (ti[a] < 0) continue;
+      const int i0 = ti[a], j0 = tj[a];
+      for (int ql = gm; ql < qn; ql += G) {
+        double ci[4], si[4], cj[4], sj[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          ci[e] = Yc[ql][i0 + e];
+          si[e] = Ys[ql][i0 + e];
+          cj[e] = Yc[ql][j0 + e];
+          sj[e] = Ys[ql][j0 + e];
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int f = 0; f < 4; ++f) acc[a][4 * e + f] = fma(ci[e], cj[f], fma(si[e], sj[f], acc[a][4 * e + f]));
+      }
     }
     __syncthreads();
   }
   double* C = cov + (size_t)b * n_z_max * n_z_max;
   for (int p = t; p < n_z_max * n_z_max; p += blockDim.x) C[p] = 0.0;
+  if (G > 1) {                                   // sum the mode groups in order
+    for (int e = 0; e < 16; ++e) {
+      __syncthreads();
+      part[t] = (ti[0] >= 0) ? acc[0][e] : 0.0;
+      __syncthreads();
+      if (gm == 0 && ti[0] >= 0) {
+        double s = 0.0;
+        for (int g = 0; g < G; ++g) s += part[t + g * ntile];
+        acc[0][e] = s;
+      }
+    }
+    if (gm != 0) ti[0] = -1;                      // group 0 writes the blocks
+  }
   __syncthreads();
-  for (int a = 0; a < nacc; ++a) {
-    int p = t + a * 256;
-    if (p >= npair) break;
-    int i = p / n_z, j = p % n_z;
-    C[i * n_z_max + j] = acc[a];
+  for (int a = 0; a < kTilesPerThread; ++a) {
+    if (ti[a] < 0) continue;
+    for (int e = 0; e < 4; ++e)
+      for (int f = 0; f < 4; ++f) {
+        const int i = ti[a] + e, j = tj[a] + f;
+        if (i < n_z && j < n_z && i <= j) {
+          C[i * n_z_max + j] = acc[a][4 * e + f];
+          C[j * n_z_max + i] = acc[a][4 * e + f];
+        }
+      }
   }
 }
 
@@ -557,7 +662,8 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
   const double a_st = 1.0 / (1.0 + std::pow(2.0, -kPst) * std::tgamma(0.5 - kPst) / std::sqrt(kPi));
   const double dc0 = 0.6 * std::pow(1.5 * kPi, 2.0 / 3.0);
   if (S) ck(hipStreamWaitEvent(st, S->j_growth, 0));
-  hipLaunchKernelGGL(k_rows, dim3(sh.n_z_max, B), dim3(256), 0, st, C, D, slices, mass,
+  const int rows_tpb = std::min(256, (sh.n_m + 63) / 64 * 64);
+  hipLaunchKernelGGL(k_rows, dim3(sh.n_z_max, B), dim3(rows_tpb), 0, st, C, D, slices, mass,
                      sh.n_z_max, sh.n_m, dlnM, a_st, dc0, rows,
                      (sh.flags & SGL_FLAG_FILAMENTS) ? frows : (double*)nullptr);
   if (S && cov_on) ck(hipStreamWaitEvent(st, S->j_cov, 0));
