@@ -276,15 +276,18 @@ __device__ inline double eta_rvir(double c, double Dvir, double mu_c, double cr)
   for (int it = 0; it < 80; ++it) {
     double mu = mu_nfw(m);
     double f = log(m * m * m / mu) - lt;
-    if (f < 0.0) lo = m; else hi = m;
     double dmu = m / ((1.0 + m) * (1.0 + m));
     double fp = 3.0 / m - dmu / mu;
-    double mn = m - f / fp;
-    if (!(lo < mn && mn < hi)) mn = 0.5 * (lo + hi);
+    double step = f / fp;
     // Newton is quadratic here: a step below 1e-13 m leaves an error far below
-    // 1 ulp, and stopping there avoids last-ulp oscillation that would keep a
-    // few lanes (and so their whole wave) iterating
-    if (fabs(mn - m) <= 1e-13 * m || hi - lo <= 4e-16 * m) { m = mn; break; }
+    // 1 ulp.  The test comes before the bracket update, so that a point where
+    // f rounds to 0 (the step lands on the new bracket end) ends the solve
+    // instead of falling back to bisection for the last bits.
+    if (fabs(step) <= 1e-13 * m) { m -= step; break; }
+    if (f < 0.0) lo = m; else hi = m;
+    double mn = m - step;
+    if (!(lo < mn && mn < hi)) mn = 0.5 * (lo + hi);
+    if (hi - lo <= 4e-16 * m) { m = mn; break; }
     m = mn;
   }
   return m / c;
