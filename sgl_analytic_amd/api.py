@@ -243,6 +243,28 @@ def generate_pdf(z_s, h=0.674, Om=0.315, sigma8=0.811, Ob=0.0493,
     return (mu, result[1] / mu, result[2]) if return_info else (mu, result[1] / mu)
 
 
+def to_image_plane(xi, P_s):
+    """Image-plane dP_I/dln mu from the source-plane PDF on xi = ln mu:
+    dP_I is proportional to mu dP_S, normalised on the given grid
+    (pipeline.to_image_plane, pipeline.py:106-111)."""
+    xi, P_s = np.asarray(xi, float), np.asarray(P_s, float)
+    P = np.exp(xi) * P_s
+    return P / np.trapezoid(P, xi)
+
+
+def clipped_moments(xi, P, clip=1.0):
+    """Mean, variance and skewness of P restricted to |xi| <= clip and
+    renormalised there (pipeline.clipped_moments, pipeline.py:277-284)."""
+    xi, P = np.asarray(xi, float), np.asarray(P, float)
+    m = np.abs(xi) <= clip
+    x, p = xi[m], P[m]
+    p = p / np.trapezoid(p, x)
+    mean = np.trapezoid(x * p, x)
+    var = np.trapezoid((x - mean) ** 2 * p, x)
+    return dict(mean=float(mean), var=float(var),
+                skew=float(np.trapezoid((x - mean) ** 3 * p, x) / var ** 1.5))
+
+
 def generate_pdf_lnmu_batch(params, precision="fp32", graph=True, shape=None):
     """Batched solve (no reference counterpart): params [B, 10] in the order
     (h, Om, sigma8, Ob, ns, zeq, Ok, w0, wa, z_s); slice count per instance

@@ -109,3 +109,29 @@ def test_solver_refuses_without_gpu():
     from sgl_analytic_amd.solver import Shape, Solver
     with pytest.raises(_lib.SglError):
         Solver(Shape(n_z_max=4, n_m=4, n_b=4, n_k=16, n_out=8), 1, path="exact")
+
+
+def test_to_image_plane_closed_form():
+    """A Gaussian source-plane PDF N(m, s^2) in ln mu maps to N(m + s^2, s^2)
+    in the image plane (mu N(m, s^2) = e^{m + s^2/2} N(m + s^2, s^2))."""
+    from sgl_analytic_amd import to_image_plane
+    x = np.linspace(-3.0, 3.0, 6001)
+    m, s = -0.02, 0.15
+    g = lambda mm: np.exp(-0.5 * ((x - mm) / s) ** 2) / (s * np.sqrt(2 * np.pi))
+    pi = to_image_plane(x, g(m))
+    assert abs(np.trapezoid(pi, x) - 1.0) < 1e-12
+    assert np.max(np.abs(pi - g(m + s * s))) < 1e-9 * np.max(g(m))
+
+
+def test_clipped_moments_closed_form():
+    """Uniform density on [-1, 1] clipped at 0.5: mean 0, variance 1/12,
+    skewness 0; a shifted Gaussian well inside the clip keeps its moments."""
+    from sgl_analytic_amd import clipped_moments
+    x = np.linspace(-1.0, 1.0, 200001)
+    c = clipped_moments(x, np.full_like(x, 0.5), clip=0.5)
+    # the clip takes whole grid points (step 1e-5): mean and variance to a step
+    assert abs(c["mean"]) < 1e-5 and abs(c["var"] - 1.0 / 12.0) < 1e-5 and abs(c["skew"]) < 1e-4
+    s = 0.05
+    g = np.exp(-0.5 * ((x - 0.1) / s) ** 2)
+    c = clipped_moments(x, g, clip=1.0)
+    assert abs(c["mean"] - 0.1) < 1e-9 and abs(c["var"] - s * s) < 1e-9 and abs(c["skew"]) < 1e-6
