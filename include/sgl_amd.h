@@ -34,6 +34,8 @@ typedef enum sgl_status {
 #define SGL_FLAG_WINDOW 2     /* cubic B-spline window on phi(k) */
 #define SGL_FLAG_FILAMENTS 4  /* add the cylinder-filament population: per (slice, mass)
                                  120 u bins x 13 orientation nodes (the "fil" ingredient) */
+#define SGL_FLAG_EDGE_RVIR 8  /* truncate the halos at r_vir (x_t = r_vir / r_s) instead of
+                                 r200 (x_t = c): the reference's edge="rvir" */
 
 /* Batched path: slice amplitudes and Psi_2 at every SGL_PSI2_STRIDE-th k node
  * (sub-grid node jc = k node jc * stride of the log grid, the last one up to

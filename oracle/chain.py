@@ -91,6 +91,7 @@ class Shape:
     clustering: bool = True
     window: bool = True
     filaments: bool = False
+    edge: str = "r200"           # halo truncation: "r200" (x_t = c) or "rvir" (x_t = eta c)
 
     def k_nodes(self):
         """k_q: 0 followed by n_k log-spaced nodes."""
@@ -140,6 +141,8 @@ def slice_covariance(cos: Cosmology, chi_edges: np.ndarray, chis: float, Rs: flo
 
 def rows(cos: Cosmology, zs: float, shape: Shape):
     """Per-(slice, mass) rows [n_z, n_m, 6] = (k_s, r_s, x_t, r_vir, w_row, beta)."""
+    if shape.edge not in ("r200", "rvir"):
+        raise ValueError(f"edge must be 'r200' or 'rvir', not {shape.edge!r}")
     z, chi_mid, dchi, chi_edges = slice_nodes(cos, zs, shape.n_z)
     M, dlnM = shape.m_nodes(cos.h)
     R = np.zeros((shape.n_z, shape.n_m, 6))
@@ -151,7 +154,10 @@ def rows(cos: Cosmology, zs: float, shape: Shape):
             dn = cos.dndlnM(Mj, zi) * dlnM
             rvir = cos.eta(C, zi) * C * rs
             beta = D * cos.halo_bias(Mj, zi)
-            R[i, j] = (ks, rs, C, rvir, wz * dn, beta)
+            # truncation radius (sgl_full.py:1874-1880): r200 (x_t = c) or
+            # r_vir (x_t = r_vir / r_s)
+            xt = C if shape.edge == "r200" else rvir / rs
+            R[i, j] = (ks, rs, xt, rvir, wz * dn, beta)
     return dict(rows=R, z=z, chi=chi_mid, dchi=dchi, chi_edges=chi_edges, M=M, dlnM=dlnM,
                 chis=cos.chi(zs))
 

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsgl_amd.so")
 
 SGL_OK, SGL_ERR_INVALID, SGL_ERR_HIP, SGL_ERR_UNSUPPORTED = 0, 1, 2, 3
-FLAG_CLUSTERING, FLAG_WINDOW, FLAG_FILAMENTS = 1, 2, 4
+FLAG_CLUSTERING, FLAG_WINDOW, FLAG_FILAMENTS, FLAG_EDGE_RVIR = 1, 2, 4, 8
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 COSMO_STRIDE = 16
 

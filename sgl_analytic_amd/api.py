@@ -128,9 +128,12 @@ def _solver(n_z):
     return _SOLVERS[key]
 
 
-def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAULT_SLICES):
+def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAULT_SLICES,
+        edge="r200"):
     """Host driver (the role of pipeline.run, pipeline.py:114-274) for one
-    cosmology on the plain FP64 device path; returns (p on xi_out, info)."""
+    cosmology on the plain FP64 device path; returns (p on xi_out, info).
+    edge: halo truncation, "r200" (the reference default) or "rvir"
+    (pipeline.run's edge / build_population's trunc, sgl_full.py:1874-1880)."""
     import ctypes
 
     import torch
@@ -140,6 +143,10 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAUL
 
     if not np.isscalar(z_s) or not np.isfinite(z_s) or z_s <= 0:
         raise ValueError("z_s must be a finite positive number")
+    if edge not in ("r200", "rvir"):
+        if edge == "kthr" or isinstance(edge, (int, float)):
+            raise NotImplementedError(f"edge={edge!r} is not part of this path (r200, rvir)")
+        raise ValueError(f"edge must be 'r200' or 'rvir', got {edge!r}")
     if closure not in ("gauss", "vp"):
         raise ValueError(f"xi_closure must be 'gauss' or 'vp', got {closure!r}")
     s = on(config)
@@ -153,13 +160,14 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAUL
         raise NotImplementedError(f"ingredient(s) {sorted(explicit)} are not part of this path")
     sv = _solver(int(n_z))
     sv_shape = sv.shape
-    if ("bias" in s) != sv_shape.clustering or ("fil" in s) != sv_shape.filaments:
+    if (("bias" in s) != sv_shape.clustering or ("fil" in s) != sv_shape.filaments
+            or edge != sv_shape.edge):
         from dataclasses import replace
         from .solver import Solver
-        key = ("exact", int(n_z), "bias" in s, "fil" in s)
+        key = ("exact", int(n_z), "bias" in s, "fil" in s, edge)
         if key not in _SOLVERS:
-            _SOLVERS[key] = Solver(replace(sv_shape, clustering="bias" in s, filaments="fil" in s),
-                                   1, path="exact")
+            _SOLVERS[key] = Solver(replace(sv_shape, clustering="bias" in s, filaments="fil" in s,
+                                           edge=edge), 1, path="exact")
         sv = _SOLVERS[key]
         sv_shape = sv.shape
     rec = cosmo_records(np.array([[cp["h"], cp["Om"], cp["s8"], cp["Ob"], cp["ns"], cp["zeq"],
@@ -175,6 +183,7 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAUL
     d = sv.diag[0].cpu().numpy()
     info = dict(zs=float(z_s), cfg=config, ingredients=["halo"] + sorted(s & MODELLED),
                 clustering="slice-covariance" if "bias" in s else None, closure=closure,
+                edge=edge,
                 n_slices=int(n_z), grid=dict(n_k=sv_shape.n_k, k_min=sv_shape.k_lo,
                                              k_max=sv_shape.k_hi, n_m=sv_shape.n_m,
                                              n_b=sv_shape.n_b),

@@ -302,7 +302,7 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
                                               const double* __restrict__ slices,
                                               const double* __restrict__ mass, int n_z_max,
                                               int n_m, double dlnM, double a_st, double dc0,
-                                              double* __restrict__ rows,
+                                              int edge_rvir, double* __restrict__ rows,
                                               double* __restrict__ frows) {
   __shared__ double sk[13];
   // (sk[8] = rho_c0 for the filament rows)
@@ -367,14 +367,16 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
     const double fC = log1p(C) - C / (1.0 + C);
     const double rhos = (200.0 / 3.0) * rhoc * C * C * C / fC;
     const double ks = rhos * rs / S[kSlScr];
-    // r_vir (subhalos.py:381-385)
-    const double rvir = eta_rvir(C, Dvir, fC, cr) * C * rs;
+    // r_vir (subhalos.py:381-385); truncation at r200 (x_t = c) or at r_vir
+    // (sgl_full.py:1874-1880)
+    const double eta = eta_rvir(C, Dvir, fC, cr);
+    const double rvir = eta * C * rs;
     // linear halo bias with the growth D (sgl_full.py:541-543)
     const double qn2 = 0.8 * (dc0 / D / sig0) * (dc0 / D / sig0);
     const double bh = 1.0 + (qn2 - 1.0) / dc0 + 0.6 / (dc0 * (1.0 + exp(0.3 * (l08dc0 - 2.0 * lsig))));
     R[kRowKs] = ks;
     R[kRowRs] = rs;
-    R[kRowXt] = C;
+    R[kRowXt] = edge_rvir ? eta * C : C;
     R[kRowRvir] = rvir;
     R[kRowW] = S[kSlWz] * dndlnM * dlnM;
     R[kRowBeta] = D * bh;
@@ -667,7 +669,7 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
   if (S) ck(hipStreamWaitEvent(st, S->j_growth, 0));
   const int rows_tpb = std::min(256, (sh.n_m + 63) / 64 * 64);
   hipLaunchKernelGGL(k_rows, dim3(sh.n_z_max, B), dim3(rows_tpb), 0, st, C, D, slices, mass,
-                     sh.n_z_max, sh.n_m, dlnM, a_st, dc0, rows,
+                     sh.n_z_max, sh.n_m, dlnM, a_st, dc0, (sh.flags & SGL_FLAG_EDGE_RVIR) ? 1 : 0, rows,
                      (sh.flags & SGL_FLAG_FILAMENTS) ? frows : (double*)nullptr);
   if (S && cov_on) ck(hipStreamWaitEvent(st, S->j_cov, 0));
   if (fail) return SGL_ERR_HIP;

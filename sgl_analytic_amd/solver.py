@@ -20,7 +20,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (COSMO_STRIDE, FLAG_CLUSTERING, FLAG_FILAMENTS, FLAG_WINDOW, PREC_BF16X3, PREC_FP32,
+from ._lib import (COSMO_STRIDE, FLAG_CLUSTERING, FLAG_EDGE_RVIR, FLAG_FILAMENTS, FLAG_WINDOW,
+                   PREC_BF16X3, PREC_FP32,
                    PREC_FP64)
 
 PARAM_NAMES = ("h", "Om", "sigma8", "Ob", "ns", "zeq", "Ok", "w0", "wa", "z_s")
@@ -31,7 +32,8 @@ class Shape:
     """Grids of one solve.  Defaults: 64 slices x 256 masses x 512 impact
     parameters, 8192 log-k nodes in [1e-2, 1e6], 2048 ln(mu) nodes on [-0.5, 3].
     ``filaments`` adds the cylinder-filament population (120 u bins x 13
-    orientations per (slice, mass))."""
+    orientations per (slice, mass)).  ``edge`` is the halo truncation: "r200"
+    (x_t = c, the reference default) or "rvir" (x_t = r_vir / r_s)."""
     n_z_max: int = 64
     n_m: int = 256
     n_b: int = 512
@@ -40,6 +42,7 @@ class Shape:
     clustering: bool = True
     window: bool = True
     filaments: bool = False
+    edge: str = "r200"
     m_lo_h: float = 1e8
     m_hi_h: float = 1e16
     u_lo: float = 1e-4
@@ -50,8 +53,11 @@ class Shape:
     out_hi: float = 3.0
 
     def c(self) -> _lib.CShape:
+        if self.edge not in ("r200", "rvir"):
+            raise ValueError(f"edge must be 'r200' or 'rvir', not {self.edge!r}")
         flags = ((FLAG_CLUSTERING if self.clustering else 0) | (FLAG_WINDOW if self.window else 0)
-                 | (FLAG_FILAMENTS if self.filaments else 0))
+                 | (FLAG_FILAMENTS if self.filaments else 0)
+                 | (FLAG_EDGE_RVIR if self.edge == "rvir" else 0))
         return _lib.CShape(self.n_z_max, self.n_m, self.n_b, self.n_k, self.n_out, flags,
                            self.m_lo_h, self.m_hi_h, self.u_lo, self.u_hi, self.k_lo, self.k_hi,
                            self.out_lo, self.out_hi)

@@ -57,6 +57,10 @@ def test_paths_not_computed_here_raise():
             api.run(1.0, "full", cosmo=dict(radiation=None), xi_out=np.array([0.0, 1.0]))
         with pytest.raises(ValueError):     # vp needs the bias ingredient, as in the reference
             api.run(1.0, "-bias", cosmo={}, xi_out=np.array([0.0, 1.0]), closure="vp")
+        with pytest.raises(NotImplementedError):    # the threshold edge is not built
+            api.run(1.0, "full", cosmo={}, xi_out=np.array([0.0, 1.0]), edge="kthr")
+        with pytest.raises(ValueError):
+            api.run(1.0, "full", cosmo={}, xi_out=np.array([0.0, 1.0]), edge="r500")
         solver.assert_not_called()
 
 

@@ -159,3 +159,22 @@ def test_pdf_small_shape_properties():
     assert abs(m["mean"]) < 1e-3                  # compensated: <xi> = 0
     var_pred = r.info["var_xi"] + 2 * abs(r.info["psi2"][0].real) / r.info["kq"][1] ** 2
     assert abs(m["var"] / var_pred - 1) < 0.05
+
+
+def test_edge_rvir_rows():
+    """edge="rvir": x_t = r_vir / r_s = eta c, above c where Delta_vir < 200
+    (eta > 1) and equal to c where the r_vir root sits at the bracket end
+    (eta = 1); every other row entry is the r200 row's."""
+    from oracle import chain
+    from oracle.cosmo import Cosmology
+    cos = Cosmology(h=0.674, Om=0.315, s8=0.811, Ob=0.0493, ns=0.965, zeq=3402.0)
+    sh200 = chain.Shape(n_z=6, n_m=8, n_b=16, n_k=64, n_out=32)
+    shv = chain.Shape(n_z=6, n_m=8, n_b=16, n_k=64, n_out=32, edge="rvir")
+    a = chain.rows(cos, 2.0, sh200)["rows"]
+    b = chain.rows(cos, 2.0, shv)["rows"]
+    assert np.array_equal(np.delete(a, 2, axis=2), np.delete(b, 2, axis=2))
+    eta = b[:, :, 3] / b[:, :, 1] / a[:, :, 2]          # r_vir / (r_s c)
+    assert np.allclose(b[:, :, 2], eta * a[:, :, 2], rtol=1e-14)
+    assert np.all(eta >= 1.0 - 1e-15) and np.any(eta > 1.05)
+    with pytest.raises(ValueError):
+        chain.rows(cos, 2.0, chain.Shape(n_z=6, n_m=8, n_b=16, n_k=64, n_out=32, edge="r500"))
