@@ -75,7 +75,7 @@ typedef struct sgl_shape {
 #define SGL_COSMO_STRIDE 16
 
 /* Buffer sizes in bytes for a batch of B instances. */
-size_t sgl_tables_bytes(const sgl_shape* shape, int32_t B);     /* derived+slices+mass+rows+cov+work */
+size_t sgl_tables_bytes(const sgl_shape* shape, int32_t B);     /* grids+derived+slices+mass+rows+cov */
 size_t sgl_hist_bytes(const sgl_shape* shape, int32_t B);       /* FP64 jump histograms */
 size_t sgl_filon_bytes(const sgl_shape* shape, int32_t precision); /* inversion operand E */
 size_t sgl_phi_bytes(const sgl_shape* shape, int32_t B, int32_t precision); /* operand phi */

@@ -31,7 +31,7 @@ constexpr int kRowKs = 0, kRowRs = 1, kRowXt = 2, kRowRvir = 3, kRowW = 4, kRowB
 
 // Offsets (in doubles) of the sub-arrays of the tables buffer.
 struct TablesLayout {
-  size_t cosmo, derived, slices, mass, rows, frows, cov, work, m_nodes, uedge, knodes, status, total;
+  size_t cosmo, derived, slices, mass, rows, frows, cov, m_nodes, uedge, knodes, status, total;
 };
 
 // The shape-only grids come first, so their offsets do not depend on B: a
@@ -52,7 +52,6 @@ inline TablesLayout tables_layout(const sgl_shape& s, int B) {
   L.rows = take((size_t)B * s.n_z_max * s.n_m * kRowStride);
   L.frows = take((s.flags & SGL_FLAG_FILAMENTS) ? (size_t)B * s.n_z_max * s.n_m * kFilRowStride : 0);
   L.cov = take((size_t)B * s.n_z_max * s.n_z_max);
-  L.work = take((size_t)B * 20000);
   L.total = o;
   return L;
 }
@@ -138,7 +137,7 @@ extern int g_sgl_tune[SGL_TUNE_COUNT];
 
 int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes_h,
                  double* derived, double* slices, double* mass, double* rows, double* frows,
-                 double* cov, double* work, int* status, hipStream_t st);
+                 double* cov, int* status, hipStream_t st);
 int launch_fill_grids(const sgl_shape& sh, double* ue, double* kn, double* m_nodes_h,
                       hipStream_t st);
 int launch_lnphi_exact(const double* rows, const double* frows, const double* cosmo, const double* uedge,

@@ -217,7 +217,7 @@ class Solver:
                  ("cosmo", B * 16), ("derived", B * 16), ("slices", B * sh.n_z_max * 8),
                  ("mass", B * sh.n_m * 4), ("rows", B * sh.n_z_max * sh.n_m * 8),
                  ("frows", B * sh.n_z_max * sh.n_m * 4 if sh.filaments else 0),
-                 ("cov", B * sh.n_z_max * sh.n_z_max), ("work", B * 20000)]
+                 ("cov", B * sh.n_z_max * sh.n_z_max)]
         T = self.tables.view(torch.float64).cpu().numpy()
         out, o = {}, 0
         for name, n in sizes:
