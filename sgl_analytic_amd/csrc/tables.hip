@@ -101,12 +101,24 @@ __global__ __launch_bounds__(256) void k_background(const Cosmo* __restrict__ co
 // D'' + (2 + dlnH/dlna) D' - 3/2 Omega_m(a) D = 0 in x = ln a (sgl.py:310-347),
 // classical RK4 on kGrowthSteps uniform steps from a = 1e-6, cubic Hermite
 // dense output at every slice node, normalised by D(a = 1).
+//
+// The equation is linear, y' = A(x) y with y = (D, D') and
+// A = [[0, 1], [beta(x), alpha(x)]], so one RK4 step is a 2x2 matrix T_n and
+// the solve is the ordered product of 4096 of them.  One wave per instance:
+// lane L forms the product of its kGrowthLane consecutive steps, a
+// Hillis-Steele scan over the lanes (shuffles) gives every lane the product of
+// the steps before its own, and each lane then steps its own chunk once more
+// from its starting state, writing the slice nodes that fall in it.  The
+// serial depth is 2 x 64 steps and 6 scan levels instead of 4096 steps.
 constexpr int kGrowthSteps = 4096;
+constexpr int kGrowthLane = kGrowthSteps / 64;
+static_assert(kGrowthSteps % 64 == 0, "whole chunks per lane");
 
 struct GrowthRhs {
   double Om, Ok, Or, OL, w, wa;
   bool lambda_de;
-  __device__ inline void operator()(double x, double y0, double y1, double& d0, double& d1) const {
+  // alpha = -(2 + dlnH/dx), beta = 3/2 Omega_m(x)
+  __device__ inline void coef(double x, double& alpha, double& beta) const {
     double e1 = exp(-x);
     double e2 = e1 * e1;
     double matter = Om * e2 * e1;
@@ -121,53 +133,109 @@ struct GrowthRhs {
       de = OL * exp(-3.0 * (1.0 + w + wa) * x + 3.0 * wa * (a - 1.0));
       wz = w + wa * (1.0 - a);
     }
-    double total = matter + rad + curv + de;
-    double dh = -0.5 * (3.0 * matter + 4.0 * rad + 2.0 * curv + 3.0 * (1.0 + wz) * de) / total;
-    d0 = y1;
-    d1 = -(2.0 + dh) * y1 + 1.5 * (matter / total) * y0;
+    double itot = 1.0 / (matter + rad + curv + de);
+    double dh = -0.5 * (3.0 * matter + 4.0 * rad + 2.0 * curv + 3.0 * (1.0 + wz) * de) * itot;
+    alpha = -(2.0 + dh);
+    beta = 1.5 * matter * itot;
   }
 };
 
-__global__ __launch_bounds__(64) void k_growth(const Cosmo* __restrict__ cosmo, int B, int n_z_max,
+// One RK4 step of y' = A y on (y0, y1), A at the step's start (a1, b1),
+// midpoint (a2, b2) and end (a4, b4).
+__device__ inline void rk4_lin(double h, double a1, double b1, double a2, double b2, double a4,
+                               double b4, double& y0, double& y1) {
+  const double k1a = y1, k1b = b1 * y0 + a1 * y1;
+  const double u0 = y0 + 0.5 * h * k1a, u1 = y1 + 0.5 * h * k1b;
+  const double k2a = u1, k2b = b2 * u0 + a2 * u1;
+  const double v0 = y0 + 0.5 * h * k2a, v1 = y1 + 0.5 * h * k2b;
+  const double k3a = v1, k3b = b2 * v0 + a2 * v1;
+  const double w0 = y0 + h * k3a, w1 = y1 + h * k3b;
+  const double k4a = w1, k4b = b4 * w0 + a4 * w1;
+  y0 += h / 6.0 * (k1a + 2.0 * k2a + 2.0 * k3a + k4a);
+  y1 += h / 6.0 * (k1b + 2.0 * k2b + 2.0 * k3b + k4b);
+}
+
+__global__ __launch_bounds__(64) void k_growth(const Cosmo* __restrict__ cosmo, int n_z_max,
                                                Derived* __restrict__ der,
                                                double* __restrict__ slices) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
+  __shared__ double xs[kMaxSlices];
+  const int b = blockIdx.x, L = threadIdx.x;
   const Cosmo c = cosmo[b];
   const Derived d = der[b];
-  GrowthRhs f{c.Om, c.Ok, d.Or, d.OL, c.w0, c.wa, c.w0 == -1.0 && c.wa == 0.0};
+  const GrowthRhs f{c.Om, c.Ok, d.Or, d.OL, c.w0, c.wa, c.w0 == -1.0 && c.wa == 0.0};
   const int n_z = (int)c.n_z;
+  double* Sb = slices + (size_t)b * n_z_max * kSliceStride;
+  for (int i = L; i < n_z; i += 64) xs[i] = -log1p(Sb[i * kSliceStride + kSlZ]);
   const double x0 = log(1e-6);
   const double h = -x0 / kGrowthSteps;
-  double y0 = exp(x0) + (2.0 / 3.0) * d.Or / c.Om, y1 = exp(x0);
-  double* Sb = slices + (size_t)b * n_z_max * kSliceStride;
-  int next = n_z - 1;  // slices are ordered by increasing z, i.e. decreasing x
-  double xnext = (next >= 0) ? -log1p(Sb[next * kSliceStride + kSlZ]) : 1.0;
-  for (int n = 0; n < kGrowthSteps; ++n) {
-    double x = x0 + n * h;
-    double k1a, k1b, k2a, k2b, k3a, k3b, k4a, k4b;
-    f(x, y0, y1, k1a, k1b);
-    f(x + 0.5 * h, y0 + 0.5 * h * k1a, y1 + 0.5 * h * k1b, k2a, k2b);
-    f(x + 0.5 * h, y0 + 0.5 * h * k2a, y1 + 0.5 * h * k2b, k3a, k3b);
-    f(x + h, y0 + h * k3a, y1 + h * k3b, k4a, k4b);
-    double n0 = y0 + h / 6.0 * (k1a + 2.0 * k2a + 2.0 * k3a + k4a);
-    double n1 = y1 + h / 6.0 * (k1b + 2.0 * k2b + 2.0 * k3b + k4b);
-    double xe = (n == kGrowthSteps - 1) ? 0.0 : x + h;
-    while (next >= 0 && xnext <= xe) {
-      // cubic Hermite on [x, xe] with values y0, n0 and slopes y1, n1
-      double hh = xe - x, s = (xnext - x) / hh;
-      double s2 = s * s, s3 = s2 * s;
-      double h00 = 2 * s3 - 3 * s2 + 1, h10 = s3 - 2 * s2 + s, h01 = -2 * s3 + 3 * s2,
-             h11 = s3 - s2;
-      Sb[next * kSliceStride + kSlD] = h00 * y0 + h10 * hh * y1 + h01 * n0 + h11 * hh * n1;
-      --next;
-      xnext = (next >= 0) ? -log1p(Sb[next * kSliceStride + kSlZ]) : 1.0;
-    }
-    y0 = n0;
-    y1 = n1;
+  const int n0 = L * kGrowthLane;
+  // step n spans [x0 + n h, x0 + (n + 1) h]; the last step ends at exactly 0
+  auto xat = [&](int n) { return (n == kGrowthSteps) ? 0.0 : x0 + n * h; };
+  // phase 1: P = T_{n0 + 63} ... T_{n0}, column by column
+  double p00 = 1.0, p10 = 0.0, p01 = 0.0, p11 = 1.0;
+  double a1, b1;
+  f.coef(xat(n0), a1, b1);
+  for (int n = n0; n < n0 + kGrowthLane; ++n) {
+    const double x = xat(n), xe = xat(n + 1), hh = xe - x;
+    double a2, b2, a4, b4;
+    f.coef(x + 0.5 * hh, a2, b2);
+    f.coef(xe, a4, b4);
+    rk4_lin(hh, a1, b1, a2, b2, a4, b4, p00, p10);
+    rk4_lin(hh, a1, b1, a2, b2, a4, b4, p01, p11);
+    a1 = a4;
+    b1 = b4;
   }
-  for (int i = 0; i < n_z; ++i) Sb[i * kSliceStride + kSlD] /= y0;
-  der[b].D0 = y0;
+  // phase 2: inclusive scan, lane L ends with M_L ... M_0
+  for (int off = 1; off < 64; off <<= 1) {
+    const double q00 = __shfl_up(p00, off), q10 = __shfl_up(p10, off);
+    const double q01 = __shfl_up(p01, off), q11 = __shfl_up(p11, off);
+    if (L >= off) {
+      const double r00 = p00 * q00 + p01 * q10, r01 = p00 * q01 + p01 * q11;
+      const double r10 = p10 * q00 + p11 * q10, r11 = p10 * q01 + p11 * q11;
+      p00 = r00; p01 = r01; p10 = r10; p11 = r11;
+    }
+  }
+  const double yi0 = exp(x0) + (2.0 / 3.0) * d.Or / c.Om, yi1 = exp(x0);
+  // D(a = 1) from the whole product (lane 63)
+  const double Dend = __shfl(p00 * yi0 + p01 * yi1, 63);
+  // the state at the start of this lane's chunk: the exclusive product
+  double e00 = __shfl_up(p00, 1), e10 = __shfl_up(p10, 1);
+  double e01 = __shfl_up(p01, 1), e11 = __shfl_up(p11, 1);
+  if (L == 0) { e00 = 1.0; e10 = 0.0; e01 = 0.0; e11 = 1.0; }
+  double y0 = e00 * yi0 + e01 * yi1, y1 = e10 * yi0 + e11 * yi1;
+  __syncthreads();                                 // xs[] complete
+  // phase 3: this lane's slice nodes lie in (x_{n0}, x_{n0 + 64}] (slices are
+  // ordered by increasing z, i.e. decreasing x); lane 0 also takes any node
+  // below x0, as the first step of a sequential solve would
+  const double xa = xat(n0), xb = xat(n0 + kGrowthLane);
+  int next = n_z - 1;
+  while (L > 0 && next >= 0 && xs[next] <= xa) --next;
+  if (next >= 0 && xs[next] <= xb) {
+    f.coef(xa, a1, b1);
+    for (int n = n0; n < n0 + kGrowthLane && next >= 0 && xs[next] <= xb; ++n) {
+      const double x = xat(n), xe = xat(n + 1), hh = xe - x;
+      double a2, b2, a4, b4;
+      f.coef(x + 0.5 * hh, a2, b2);
+      f.coef(xe, a4, b4);
+      double n0v = y0, n1v = y1;
+      rk4_lin(hh, a1, b1, a2, b2, a4, b4, n0v, n1v);
+      while (next >= 0 && xs[next] <= xe) {
+        // cubic Hermite on [x, xe] with values y0, n0v and slopes y1, n1v
+        const double s = (xs[next] - x) / hh;
+        const double s2 = s * s, s3 = s2 * s;
+        const double h00 = 2 * s3 - 3 * s2 + 1, h10 = s3 - 2 * s2 + s, h01 = -2 * s3 + 3 * s2,
+                     h11 = s3 - s2;
+        Sb[next * kSliceStride + kSlD] =
+            (h00 * y0 + h10 * hh * y1 + h01 * n0v + h11 * hh * n1v) / Dend;
+        --next;
+      }
+      y0 = n0v;
+      y1 = n1v;
+      a1 = a4;
+      b1 = b4;
+    }
+  }
+  if (L == 0) der[b].D0 = Dend;
 }
 
 // ----------------------------------------------------------------- sigma --
@@ -176,88 +244,116 @@ __global__ __launch_bounds__(64) void k_growth(const Cosmo* __restrict__ cosmo, 
 // Ws = 1/(1 + (0.43 x)^6) for sigma(M); dln sigma/dln M from the same sum
 // with the analytic window derivative.
 constexpr int kSigmaK = 4000;
+constexpr int kSigChunk = 1000;   // k nodes staged in LDS at a time (16 KB)
+// Window terms past y = (0.43 k R)^6 = kSigYCut are skipped: Ws^2 ~ y^-2
+// falls by 5.7 % a node there while g grows about like k.  A CPU check over
+// 1e8..1e16 Msun/h puts the skipped tail of either sum below 1e-17 of it.
+constexpr double kSigYCut = 1e10;
+static_assert(kSigmaK % kSigChunk == 0, "whole chunks");
 
+// One workgroup per instance.  The k nodes go through LDS in chunks of
+// kSigChunk (16 KB per workgroup, so that LDS does not cap occupancy): each
+// chunk's integrand g = k^2 Pshape w_trap / (2 pi^2) and k^6 are formed, the
+// top-hat anchor and every mass's window sums take their share of it, and
+// the next chunk overwrites it.  The mass sums run on four independent
+// accumulators (the FP64 add latency, not the issue rate, bounds one chain).
 __global__ __launch_bounds__(256) void k_sigma(const Cosmo* __restrict__ cosmo,
                                                const double* __restrict__ m_nodes_h, int n_m,
                                                Derived* __restrict__ der,
                                                double* __restrict__ mass) {
-  __shared__ double g[kSigmaK];   // k^2 Pshape w_trap / (2 pi^2)
-  __shared__ double kk[kSigmaK];
-  __shared__ double red[256];
+  __shared__ double g[kSigChunk];
+  __shared__ double k6[kSigChunk];
+  __shared__ double red[256], pS[256], pSd[256];
   const int b = blockIdx.x, t = threadIdx.x;
   const Cosmo c = cosmo[b];
   const EH98Const e = eh98_const(c);
   const double lstep = 8.0 / (kSigmaK - 1);
-  for (int i = t; i < kSigmaK; i += blockDim.x) kk[i] = pow(10.0, -5.0 + lstep * i);
-  __syncthreads();
-  for (int i = t; i < kSigmaK; i += blockDim.x) {
-    double k = kk[i];
-    const double lk = (-5.0 + lstep * i) * 2.302585092994046;   // ln k
-    double T = eh98_T(e, k, lk);
-    double wl = (i == 0) ? 0.5 * (kk[1] - kk[0])
-                : (i == kSigmaK - 1) ? 0.5 * (kk[i] - kk[i - 1]) : 0.5 * (kk[i + 1] - kk[i - 1]);
-    g[i] = k * k * exp(c.ns * lk) * T * T * wl / (2.0 * kPi * kPi);
-  }
-  __syncthreads();
-  // top-hat anchor
+  const double kLn10 = 2.302585092994046;
+  // trapezoid weights on k = 10^(-5 + lstep i): (k_{i+1} - k_{i-1}) / 2 =
+  // k_i (r - 1/r) / 2 with r = 10^lstep, half intervals at the two ends
+  const double r = exp(kLn10 * lstep), ir = 1.0 / r;
   const double R8 = 8.0 / c.h;
-  double s = 0.0;
-  for (int i = t; i < kSigmaK; i += blockDim.x) {
-    const double x = kk[i] * R8;
-    double sx, cx;
-    sincos(x, &sx, &cx);
-    const double W = 3.0 * (sx - x * cx) / (x * x * x);
-    s += g[i] * W * W;
+  const double rhom = c.Om * kRhoc0Unit * c.h * c.h;
+  const double a6 = pow(0.43, 6.0);
+  // G threads per mass when n_m < 256 (G = 256 / nm2, nm2 the power of two
+  // >= n_m): thread (g, j) sums the g-th contiguous part of every chunk for
+  // mass j; the parts are added in order afterwards.
+  int nm2 = 1;
+  while (nm2 < n_m) nm2 <<= 1;
+  const int G = 256 / nm2;
+  const int j = t % nm2, gq = t / nm2;
+  const int ia = gq * kSigChunk / G, ib = (gq + 1) * kSigChunk / G;
+  double aR = 0.0;
+  int qcut = kSigmaK;   // first node with y = aR k^6 > kSigYCut
+  if (j < n_m) {
+    const double M = m_nodes_h[j] / c.h;
+    const double R = cbrt(3.0 * M / (4.0 * kPi * rhom));
+    const double R2 = R * R;
+    aR = a6 * R2 * R2 * R2;
+    // log10 k > (ln kSigYCut - ln aR) / (6 ln 10), i.e. node index above
+    const double qc = ((log(kSigYCut) - log(aR)) / (6.0 * kLn10) + 5.0) / lstep;
+    qcut = (qc < kSigmaK) ? (int)fmax(ceil(qc), 0.0) : kSigmaK;
   }
-  red[t] = s;
+  // S = sum g Ws^2 and, from dWs/dx = -6 y Ws^2 / x, dS/dln R = -12 sum g y Ws^3,
+  // so dln sigma / dln M = (dS/dln R) / (2 S) / 3 = -2 sum g y Ws^3 / S
+  double S0 = 0.0, S1 = 0.0, S2 = 0.0, S3 = 0.0;
+  double D0 = 0.0, D1 = 0.0, D2 = 0.0, D3 = 0.0;
+  double sa = 0.0;                                   // top-hat anchor partial
+  for (int i0 = 0; i0 < kSigmaK; i0 += kSigChunk) {
+    __syncthreads();                                 // the previous chunk is consumed
+    for (int q = t; q < kSigChunk; q += blockDim.x) {
+      const int i = i0 + q;
+      const double lk = (-5.0 + lstep * i) * kLn10;  // ln k
+      const double k = exp(lk);
+      const double T = eh98_T(e, k, lk);
+      const double wl = (i == 0) ? 0.5 * k * (r - 1.0)
+                        : (i == kSigmaK - 1) ? 0.5 * k * (1.0 - ir) : 0.5 * k * (r - ir);
+      const double gi = k * k * exp(c.ns * lk) * T * T * wl / (2.0 * kPi * kPi);
+      g[q] = gi;
+      const double k2 = k * k;
+      k6[q] = k2 * k2 * k2;
+      const double x = k * R8;
+      double sx, cx;
+      sincos(x, &sx, &cx);
+      const double W = 3.0 * (sx - x * cx) / (x * x * x);
+      sa += gi * W * W;
+    }
+    __syncthreads();
+    if (j < n_m) {
+      auto term = [&](int q, double& A, double& Dd) {
+        const double y = aR * k6[q];
+        const double den = 1.0 + y;
+        // 1 / (1 + y): hardware reciprocal and two Newton steps (full FP64)
+        double Ws = __builtin_amdgcn_rcp(den);
+        Ws = Ws * (2.0 - den * Ws);
+        Ws = Ws * (2.0 - den * Ws);
+        const double gw = g[q] * Ws * Ws;
+        A += gw;
+        Dd += gw * y * Ws;
+      };
+      int q = ia;
+      const int qe = min(ib, max(qcut - i0, ia));
+      for (; q + 4 <= qe; q += 4) {
+        term(q, S0, D0);
+        term(q + 1, S1, D1);
+        term(q + 2, S2, D2);
+        term(q + 3, S3, D3);
+      }
+      for (; q < qe; ++q) term(q, S0, D0);
+    }
+  }
+  red[t] = sa;
+  if (j < n_m) {
+    pS[t] = (S0 + S1) + (S2 + S3);
+    pSd[t] = (D0 + D1) + (D2 + D3);
+  }
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (t < w) red[t] += red[t + w];
     __syncthreads();
   }
   const double Anorm = c.s8 / sqrt(red[0]);
-  const double rhom = c.Om * kRhoc0Unit * c.h * c.h;
   if (t == 0) der[b].Anorm = Anorm;
-  const double a6 = pow(0.43, 6.0);
-  // the mass sums need k only as k^6 (y = a6 (k R)^6 = a6 R^6 k^6): kk becomes k^6
-  __syncthreads();
-  for (int i = t; i < kSigmaK; i += blockDim.x) {
-    const double k2 = kk[i] * kk[i];
-    kk[i] = k2 * k2 * k2;
-  }
-  __syncthreads();
-  // G threads per mass when n_m < 256 (G = 256 / nm2, nm2 the power of two
-  // >= n_m): thread (g, j) sums the g-th contiguous chunk of the k nodes for
-  // mass j; the chunks are added in order afterwards.
-  int nm2 = 1;
-  while (nm2 < n_m) nm2 <<= 1;
-  const int G = 256 / nm2;
-  const int j = t % nm2, gq = t / nm2;
-  const int ia = gq * kSigmaK / G, ib = (gq + 1) * kSigmaK / G;
-  __shared__ double pS[256], pSd[256];
-  // S = sum g Ws^2 and, from dWs/dx = -6 y Ws^2 / x, dS/dln R = -12 sum g y Ws^3,
-  // so dln sigma / dln M = (dS/dln R) / (2 S) / 3 = -2 sum g y Ws^3 / S
-  if (j < n_m) {
-    const double M = m_nodes_h[j] / c.h;
-    const double R = cbrt(3.0 * M / (4.0 * kPi * rhom));
-    const double R2 = R * R;
-    const double aR = a6 * R2 * R2 * R2;
-    double S = 0.0, Sd = 0.0;
-    for (int i = ia; i < ib; ++i) {
-      const double y = aR * kk[i];
-      // 1 / (1 + y): hardware reciprocal and two Newton steps (full FP64)
-      const double den = 1.0 + y;
-      double Ws = __builtin_amdgcn_rcp(den);
-      Ws = Ws * (2.0 - den * Ws);
-      Ws = Ws * (2.0 - den * Ws);
-      const double gw = g[i] * Ws * Ws;
-      S += gw;
-      Sd += gw * y * Ws;
-    }
-    pS[t] = S;
-    pSd[t] = Sd;
-  }
-  __syncthreads();
   if (gq == 0 && j < n_m) {
     const double M = m_nodes_h[j] / c.h;
     double S = 0.0, Sd = 0.0;
@@ -274,27 +370,26 @@ __global__ __launch_bounds__(256) void k_sigma(const Cosmo* __restrict__ cosmo,
 }
 
 // ------------------------------------------------------------------ rows --
-__device__ inline double mu_nfw(double y) { return log1p(y) - y / (1.0 + y); }
-
 // r_vir / r200 = eta solving m^3 / mu(m) = 200 c^3 / (mu(c) Delta_vir) on
-// [c, 3c] (subhalos.py:409-441): safeguarded Newton on ln(m^3 / mu(m)).
+// [c, 3c] (subhalos.py:409-441): safeguarded Newton on g(m) = m^3 - T mu(m),
+// T = 200 c^3 / (mu(c) Delta_vir) (one log1p and two divisions a step).
 // mu_c = mu(c) and cr = cbrt(200 / Delta_vir) come from the caller; Newton
 // starts at m = c cr (the root if mu(m) = mu(c), a few per cent off).
 __device__ inline double eta_rvir(double c, double Dvir, double mu_c, double cr) {
-  double lt = log(200.0 * c * c * c / mu_c / Dvir);
+  const double T = 200.0 * c * c * c / (mu_c * Dvir);
   double lo = c, hi = 3.0 * c, m = fmin(fmax(c * cr, lo), hi);
   for (int it = 0; it < 80; ++it) {
-    double mu = mu_nfw(m);
-    double f = log(m * m * m / mu) - lt;
-    double dmu = m / ((1.0 + m) * (1.0 + m));
-    double fp = 3.0 / m - dmu / mu;
-    double step = f / fp;
+    const double iop = 1.0 / (1.0 + m);
+    const double mu = log1p(m) - m * iop;
+    const double g = m * m * m - T * mu;
+    const double gp = 3.0 * m * m - T * m * iop * iop;
+    const double step = g / gp;
     // Newton is quadratic here: a step below 1e-13 m leaves an error far below
     // 1 ulp.  The test comes before the bracket update, so that a point where
-    // f rounds to 0 (the step lands on the new bracket end) ends the solve
+    // g rounds to 0 (the step lands on the new bracket end) ends the solve
     // instead of falling back to bisection for the last bits.
     if (fabs(step) <= 1e-13 * m) { m -= step; break; }
-    if (f < 0.0) lo = m; else hi = m;
+    if (g < 0.0) lo = m; else hi = m;
     double mn = m - step;
     if (!(lo < mn && mn < hi)) mn = 0.5 * (lo + hi);
     if (hi - lo <= 4e-16 * m) { m = mn; break; }
@@ -306,7 +401,8 @@ __device__ inline double eta_rvir(double c, double Dvir, double mu_c, double cr)
 // One workgroup per (instance, slice), threads over masses: the slice's
 // redshift-only factors (Ludlow+16 parameters, rho_c, Delta_vir) are formed
 // once in LDS.  a_st and dc0 are the Sheth-Tormen normalisation and the
-// bias delta_c constant, computed on the host.
+// bias delta_c constant, computed on the host.  Rows of slices past the
+// instance's n_z are not written (no kernel reads them).
 __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
                                               const Derived* __restrict__ der,
                                               const double* __restrict__ slices,
@@ -318,35 +414,40 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
   // (sk[8] = rho_c0 for the filament rows)
   const int i = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
   const Cosmo c = cosmo[b];
+  if (i >= (int)c.n_z) return;
   double* Rs = rows + ((size_t)b * n_z_max + i) * n_m * kRowStride;
   double* Fs = frows ? frows + ((size_t)b * n_z_max + i) * n_m * kFilRowStride : nullptr;
-  if (i >= (int)c.n_z) {
-    for (int q = t; q < n_m * kRowStride; q += blockDim.x) Rs[q] = 0.0;
-    if (Fs)
-      for (int q = t; q < n_m * kFilRowStride; q += blockDim.x) Fs[q] = 0.0;
-    return;
-  }
   const double* S = slices + ((size_t)b * n_z_max + i) * kSliceStride;
-  if (t == 0) {
+  if (t < 8) {
+    // Ludlow+16 with the z <= 7 clamp (sgl.py:590-606) and the Bryan-Norman
+    // Delta_vir at Omega_m(z) (subhalos.py:381-385).  The four powers a^e,
+    // the two cube roots and the two logs are one log and one exp on eight
+    // lanes: lane q takes v = f_q exp(e_q ln u_q) (q < 6) or ln u_q.
     const Derived d = der[b];
     const double z = S[kSlZ], E2 = S[kSlE2];
-    // Ludlow+16 with the z <= 7 clamp (sgl.py:590-606): redshift-only factors
     const double zc = fmin(z, 7.0), a = 1.0 + zc;
-    sk[0] = 4.135 - 0.564 * a - 0.210 * a * a + 0.0557 * a * a * a - 0.00348 * a * a * a * a;
-    sk[1] = 3.395 * pow(a, -0.215);
-    sk[2] = 0.307 * pow(a, 0.540);
-    sk[3] = 0.628 * pow(a, -0.047);
-    sk[4] = 0.317 * pow(a, -0.893);
-    sk[5] = d.rhoc0 * E2;                       // rho_c(z)
-    // Bryan-Norman Delta_vir at Omega_m(z) (subhalos.py:381-385)
+    const double poly = 4.135 - 0.564 * a - 0.210 * a * a + 0.0557 * a * a * a -
+                        0.00348 * a * a * a * a;
+    const double rhoc = d.rhoc0 * E2;
     const double dd = c.Om * (1.0 + z) * (1.0 + z) * (1.0 + z) / E2 - 1.0;
-    sk[6] = 18.0 * kPi * kPi + 82.0 * dd - 39.0 * dd * dd;
-    sk[7] = d.rhom;
-    sk[8] = d.rhoc0;
-    sk[9] = cbrt(200.0 / sk[6]);
-    sk[10] = log(sk[0]);                         // ln of the Ludlow polynomial
-    sk[11] = log(S[kSlD]);                       // ln D(z)
-    sk[12] = cbrt(3.0 / (4.0 * kPi * 200.0 * sk[5]));
+    const double Dvir = 18.0 * kPi * kPi + 82.0 * dd - 39.0 * dd * dd;
+    const double f = (t == 0) ? 3.395 : (t == 1) ? 0.307 : (t == 2) ? 0.628 : (t == 3) ? 0.317 : 1.0;
+    const double e = (t == 0) ? -0.215 : (t == 1) ? 0.540 : (t == 2) ? -0.047
+                     : (t == 3) ? -0.893 : 1.0 / 3.0;
+    const int slot = (t < 4) ? t + 1 : (t == 4) ? 9 : (t == 5) ? 12 : (t == 6) ? 10 : 11;
+    const double u = (t < 4) ? a
+                     : (t == 4) ? 200.0 / Dvir
+                     : (t == 5) ? 3.0 / (4.0 * kPi * 200.0 * rhoc)
+                     : (t == 6) ? poly : S[kSlD];
+    const double lu = log(u);
+    sk[slot] = (t < 6) ? f * exp(e * lu) : lu;
+    if (t == 0) {
+      sk[0] = poly;
+      sk[5] = rhoc;                              // rho_c(z)
+      sk[6] = Dvir;
+      sk[7] = d.rhom;
+      sk[8] = d.rhoc0;
+    }
   }
   __syncthreads();
   const double D = S[kSlD];
@@ -659,7 +760,7 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
     ck(hipStreamWaitEvent(S->s, S->e_bg, 0));
     sg = S->s;
   }
-  hipLaunchKernelGGL(k_growth, dim3((B + 63) / 64), dim3(64), 0, sg, C, B, sh.n_z_max, D, slices);
+  hipLaunchKernelGGL(k_growth, dim3(B), dim3(64), 0, sg, C, sh.n_z_max, D, slices);
   if (S) ck(hipEventRecord(S->j_growth, S->s));
   // sigma(M) needs the background only; it writes A_norm and the mass table
   hipLaunchKernelGGL(k_sigma, dim3(B), dim3(256), 0, st, C, m_nodes_h, sh.n_m, D, mass);
