@@ -499,10 +499,12 @@ __device__ inline void trunc_kg_row_f32(float x, float lx, float rx, const RowF&
 
 // Row record of a truncated NFW row (FP64 constants in, FP32 record out):
 // s2[n] = b_n M_{2n+1}(c), b_n the coefficients of (1 - sqrt(1 - y)) / y =
-// sum b_n y^n and M_m(c) = int_0^1 v^m / (v + c)^2 dv (24-point
-// Gauss-Legendre, exact to round-off for c > 0.2), so that the truncation
-// term I2 = int_0^1 v / (v + c)^2 (1 - sqrt(1 - a^2 v^2)) / (a^2 v^2) dv is
-// sum_n s2[n] a^(2n).
+// sum b_n y^n and M_m(c) = int_0^1 v^m / (v + c)^2 dv, so that the
+// truncation term I2 = int_0^1 v / (v + c)^2 (1 - sqrt(1 - a^2 v^2)) /
+// (a^2 v^2) dv is sum_n s2[n] a^(2n).  The moments take the 12-point
+// Gauss-Legendre rule (kGl12X/W mapped to [0, 1]): against 40-digit quadrature its relative error for
+// m <= 15 is below 2.3e-8 at c = 0.2, 1e-12 at c = 0.5 and 3e-15 for c >= 1,
+// under the FP32 rounding of s2 throughout.
 __device__ inline RowF make_rowf(const RowConst& c) {
   RowF f;
   f.warea = (c.valid != 0.0) ? c.warea : 0.0;
@@ -517,8 +519,9 @@ __device__ inline RowF make_rowf(const RowConst& c) {
   f.inv_s = (float)(1.0 / c.s);
   double m[kI2Terms];
   for (int n = 0; n < kI2Terms; ++n) m[n] = 0.0;
-  for (int j = 0; j < 24; ++j) {
-    const double v = kGl24X[j], g = kGl24W[j] * v / ((v + c.xt) * (v + c.xt)), v2 = v * v;
+  for (int j = 0; j < 12; ++j) {
+    const double v = 0.5 * (kGl12X[j] + 1.0), v2 = v * v;
+    const double g = 0.5 * kGl12W[j] * v / ((v + c.xt) * (v + c.xt));
     double p = g;
     for (int n = 0; n < kI2Terms; ++n) { m[n] += p; p *= v2; }
   }
