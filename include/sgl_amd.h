@@ -69,10 +69,22 @@ typedef struct sgl_shape {
 } sgl_shape;
 
 /* Per-instance input record: 16 doubles
- *   h, Om, sigma8, Ob, ns, zeq, Ok, w0, wa, z_s, n_z, 0, 0, 0, 0, 0
+ *   h, Om, sigma8, Ob, ns, zeq, Ok, w0, wa, z_s, n_z, radiation, growth, 0, 0, 0
  * (the cosmology arguments of generate_pdf_lnmu, api.py:26-31, plus the
- * source redshift and the slice count of this instance). */
+ * source redshift and the slice count of this instance).
+ *   radiation (slot SGL_REC_RADIATION): 0 = "engine", Omega_r =
+ *     Omega_m / (1 + z_eq) in E(z) and the growth ODE (the pipeline default);
+ *     1 = None, no radiation (sgl.py:93-104).
+ *   growth (slot SGL_REC_GROWTH): 0 = "ode", the growth ODE for D(z)
+ *     everywhere (the pipeline default); 1 = "legacy", LambdaCDM only: D(z)
+ *     from the Heath integral without radiation (sgl.py:270-306) in the mass
+ *     function, and the Carroll-Press-Turner growth Dg(z) in the halo and
+ *     filament bias and in beta (sgl_full.py:521-548).  The caller resolves
+ *     "auto" (legacy for LambdaCDM, ode otherwise) and keeps legacy to
+ *     w0 = -1, wa = 0. */
 #define SGL_COSMO_STRIDE 16
+#define SGL_REC_RADIATION 11
+#define SGL_REC_GROWTH 12
 
 /* Buffer sizes in bytes for a batch of B instances. */
 size_t sgl_tables_bytes(const sgl_shape* shape, int32_t B);     /* grids+derived+slices+mass+rows+cov */

@@ -10,7 +10,8 @@ The chain (one cosmology, one source redshift):
    that sgl_full.z_shells, sgl_full.py:480-518, performs on the reference grid);
 2. rows per (slice, mass): Sheth-Tormen dn/dlnM, Ludlow+16 NFW parameters,
    truncation at r200 (x_t = c, sgl_full.py:1876-1880), r_vir = eta r200
-   (subhalos.py:472-481), linear bias beta = D b_h (sgl_full.py:541-543,1871),
+   (subhalos.py:472-481), linear bias beta = Dg b_h with the bias growth Dg
+   (sgl_full.py:521-543,1861,1871),
    lens weight dchi (1+z)^2 dn (sgl_full.py:1860);
 3. cells: ``n_b`` log impact-parameter bins over [u_lo, u_hi] r_vir, each
    cell at the geometric centre of its bin (sgl_full.py:1905), weight = row
@@ -147,7 +148,7 @@ def rows(cos: Cosmology, zs: float, shape: Shape):
     M, dlnM = shape.m_nodes(cos.h)
     R = np.zeros((shape.n_z, shape.n_m, 6))
     for i, zi in enumerate(z):
-        D = cos.D(zi)
+        D = cos.Dg(zi)            # the bias growth (sgl_full.py:1861)
         wz = dchi * (1 + zi) ** 2
         for j, Mj in enumerate(M):
             C, rs, ks, fC = cos.nfw_params(Mj, zi, zs)
@@ -198,7 +199,7 @@ def dndlnM_fil(cos: Cosmology, M, z):
 def fil_bias(cos: Cosmology, M, z):
     """sgl_full.py:546-548: flat-barrier peak-background bias (q = 0.7)."""
     from .cosmo import DC0
-    qn2 = 0.7 * (DC0 / cos.D(z) / cos.sigmaM(M, 0.0)) ** 2
+    qn2 = 0.7 * (DC0 / cos.Dg(z) / cos.sigmaM(M, 0.0)) ** 2
     return 1 + (qn2 - 1) / DC0
 
 
@@ -211,7 +212,7 @@ def fil_rows(cos: Cosmology, zs: float, shape: Shape):
     M, dlnM = shape.m_nodes(cos.h)
     F = np.zeros((shape.n_z, shape.n_m, 4))
     for i, zi in enumerate(z):
-        D = cos.D(zi)
+        D = cos.Dg(zi)            # the bias growth (sgl_full.py:1861, 2001)
         wz = dchi * (1 + zi) ** 2
         scr = cos.Sigma_cr(zi, zs)
         for j, Mj in enumerate(M):

@@ -6,9 +6,10 @@ TEST INFRASTRUCTURE.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
 
 Restates ``sgl.Cosmology`` on the configuration ``pipeline.make_cosmology``
 builds (sgl_analytic/pipeline.py:36-72): window="smoothk", transfer="eh98",
-anchor="tophat", conc_model=16, growth_mode="ode", radiation="engine".  The
-same scipy routines the reference calls (quad, solve_ivp DOP853, CubicSpline)
-are used, so this is the reference algorithm, not an approximation of it.
+anchor="tophat", conc_model=16, with growth_mode "ode" (default) or "legacy"
+and radiation "engine" (default) or None.  The same scipy routines the
+reference calls (quad, solve_ivp DOP853, CubicSpline) are used, so this is the
+reference algorithm, not an approximation of it.
 
 Pins (tests/test_oracle_cosmo.py): fixture ``dark_energy.Omega_r`` and
 ``primordial_growth`` (tests/reference/pdf_zs1.json); walkthrough cell 3
@@ -39,9 +40,14 @@ class Cosmology:
     """sgl.Cosmology restricted to the make_cosmology configuration."""
 
     def __init__(self, h=0.674, Om=0.315, s8=0.811, Ob=0.0493, ns=0.965,
-                 zeq=3402.0, Ok=0.0, w0=-1.0, wa=0.0, radiation="engine"):
-        # sgl.py:68-167 (defaults, radiation closure, DE domain checks);
-        # radiation=None only for closed-form checks (the pipeline uses "engine")
+                 zeq=3402.0, Ok=0.0, w0=-1.0, wa=0.0, radiation="engine", growth_mode="ode"):
+        # sgl.py:68-167 (defaults, radiation closure, DE domain checks, growth
+        # mode; "auto" is resolved by the caller)
+        if growth_mode not in ("ode", "legacy"):
+            raise ValueError("growth_mode must be 'ode' or 'legacy'")
+        if growth_mode == "legacy" and not (w0 == -1.0 and wa == 0.0):
+            raise ValueError("legacy growth is only valid for LambdaCDM")
+        self.growth_mode = growth_mode
         self.h, self.Om, self.s8, self.Ob, self.ns = h, Om, s8, Ob, ns
         self.zeq, self.Ok, self.w0, self.wa = float(zeq), float(Ok), float(w0), float(wa)
         self.Or = self.Om / (1.0 + self.zeq) if radiation == "engine" else 0.0   # sgl.py:100
@@ -59,7 +65,10 @@ class Cosmology:
         self.rhoc0 = RHOC0_UNIT * h * h
         self.rhom = self.Om * self.rhoc0
         self._chi_cache = {}
-        self._build_growth_ode()
+        if growth_mode == "ode":
+            self._build_growth_ode()
+        else:
+            self._build_growth_legacy()
         self._build_power()
         self._build_sigma()
 
@@ -141,13 +150,51 @@ class Cosmology:
         self._growth_solution = sol
         self._D0 = float(sol.y[0, -1])
 
+    def _build_growth_legacy(self):                   # sgl.py:274-306 (Heath integral)
+        OLg = 1.0 - self.Om - self.Ok       # the integral never carries radiation
+
+        def Ea(a):
+            a = np.asarray(a, float)
+            if self.Ok == 0.0:
+                return sqrt(self.Om * a ** -3 + OLg)
+            return sqrt(self.Om * a ** -3 + self.Ok * a ** -2 + OLg)
+
+        ag = np.geomspace(1e-6, 1.0, 4096)
+        seg = np.empty(ag.size)
+        seg[0] = quad(lambda ap: 1.0 / (ap * Ea(ap)) ** 3, 0.0, ag[0])[0]
+        for i in range(1, ag.size):
+            seg[i] = quad(lambda ap: 1.0 / (ap * Ea(ap)) ** 3, ag[i - 1], ag[i])[0]
+        lnI = CubicSpline(log(ag), log(np.cumsum(seg)))
+        self._Dun = lambda a: Ea(a) * exp(lnI(log(np.asarray(a, float))))
+        self._D0 = float(self._Dun(1.0))
+
     def primordial_growth_amplitude(self):            # sgl.py:349-352
-        return self._D0
+        return self._D0 if self.growth_mode == "ode" else self._D0 * (2.5 * self.Om)
 
     def D(self, z):                                   # sgl.py:354-375
         a = 1.0 / (1 + np.asarray(z, float))
-        out = self._growth_solution.sol(log(a))[0] / self._D0
+        if self.growth_mode == "ode":
+            out = self._growth_solution.sol(log(a))[0] / self._D0
+        else:
+            out = self._Dun(a) / self._D0
         return float(out) if np.ndim(z) == 0 else out
+
+    def Dg(self, z):                                  # sgl_full.py:521-538
+        """The growth in the bias: D on the ODE growth, otherwise the
+        Carroll-Press-Turner fit with E^2 including radiation when present."""
+        if self.growth_mode == "ode":
+            return self.D(z)
+
+        def nn(zz):
+            a3 = self.Om * (1 + zz) ** 3
+            E2 = a3 + self.OL
+            if self.Ok != 0.0:
+                E2 += self.Ok * (1 + zz) ** 2
+            if self.Or != 0.0:
+                E2 += self.Or * (1 + zz) ** 4
+            Omz, OLz = a3 / E2, self.OL / E2
+            return 2.5 * Omz / (Omz ** (4 / 7) - OLz + (1 + Omz / 2) * (1 + OLz / 70)) / (1 + zz)
+        return nn(z) / nn(0.0)
 
     # ------------------------------------------------- transfer, sigma(M) --
     def T(self, k):                                   # sgl.py:391-445 (_T_eh98)
@@ -250,8 +297,8 @@ class Cosmology:
         return C, rs, ks, fC
 
     # ------------------------------------------------------- bias, virial --
-    def halo_bias(self, M, z):                        # sgl_full.py:541-543 (Dg = D in "ode")
-        qn2 = 0.8 * (DC0 / self.D(z) / self.sigmaM(M, 0.0)) ** 2
+    def halo_bias(self, M, z):                        # sgl_full.py:541-543
+        qn2 = 0.8 * (DC0 / self.Dg(z) / self.sigmaM(M, 0.0)) ** 2
         return 1 + (qn2 - 1) / DC0 + 0.6 / (DC0 * (1 + qn2 ** 0.3))
 
     def Dvir(self, z):                                # subhalos.py:381-385 (BN98 at Om(z))

@@ -15,6 +15,7 @@ SGL_OK, SGL_ERR_INVALID, SGL_ERR_HIP, SGL_ERR_UNSUPPORTED = 0, 1, 2, 3
 FLAG_CLUSTERING, FLAG_WINDOW, FLAG_FILAMENTS, FLAG_EDGE_RVIR = 1, 2, 4, 8
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 COSMO_STRIDE = 16
+REC_RADIATION, REC_GROWTH = 11, 12  # SGL_REC_*: record slots of the radiation and growth options
 
 EXPORTS = (
     "sgl_version", "sgl_tables_bytes", "sgl_hist_bytes", "sgl_filon_bytes", "sgl_phi_bytes",

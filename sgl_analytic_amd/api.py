@@ -9,9 +9,11 @@ at r200c with Sheth-Tormen abundances and Ludlow+16 concentrations on
 parameters; each cell is a jump xi = ln mu = -ln detA(kappa, gamma) of the
 1-D compound-Poisson chain of sgl.Lambda_of_k / P_of_xi (sgl.py:961-1012),
 tilted to the source plane; linear-bias clustering enters as the two-point
-term 1/2 sum_ij C_ij a_i a_j (sgl_full.py:2514-2538, 3353-3397).  The
-reference's ellipticity, filament and subhalo ingredients and its "vp"
-closure are not part of this path; asking for them raises
+term 1/2 sum_ij C_ij a_i a_j (sgl_full.py:2514-2538, 3353-3397); filaments
+join with the "fil" ingredient.  The cosmology options radiation ("engine",
+None) and growth_mode ("ode", "legacy", "auto") are computed as sgl.Cosmology
+defines them.  The reference's ellipticity and subhalo ingredients and its
+"vp" closure are not part of this path; asking for them raises
 NotImplementedError.
 
 Batched entry point (no reference counterpart): generate_pdf_lnmu_batch.
@@ -106,14 +108,18 @@ def check_cosmology(cosmo=None):
     de = OL * zp1 ** (3 * (1 + w0 + wa)) * np.exp(-3 * wa * (1 - 1 / zp1))
     if not np.all(Om * zp1 ** 3 + Or * zp1 ** 4 + Ok * zp1 ** 2 + de > 0):
         raise ValueError("E(z)^2 <= 0 somewhere in 0 <= z <= 1e6: outside the supported domain")
-    # what this path computes: the ODE growth with the engine radiation term
+    # growth: "auto" is legacy for LambdaCDM and the ODE otherwise, with the
+    # pipeline's warning when it resolves to the ODE (sgl.py:134-140,
+    # pipeline.py:166-169)
     lambda_de = w0 == -1.0 and wa == 0.0
     if cp["growth_mode"] == "legacy" and not lambda_de:
         raise ValueError("legacy growth is only valid for LambdaCDM")
-    if cp["growth_mode"] != "ode" and lambda_de:
-        raise NotImplementedError("only growth_mode='ode' (the default) is computed on this path")
-    if cp["radiation"] != "engine":
-        raise NotImplementedError("only radiation='engine' (the default) is computed on this path")
+    if cp["growth_mode"] == "auto":
+        cp["growth_mode"] = "legacy" if lambda_de else "ode"
+        if not lambda_de:
+            warnings.warn("growth_mode='auto' resolved to 'ode' for non-Lambda dark energy, "
+                          "which is not continuous with 'auto' at w = -1; compare against a "
+                          "growth_mode='ode' Lambda run.", stacklevel=3)
     return cp
 
 
@@ -171,7 +177,8 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAUL
         sv = _SOLVERS[key]
         sv_shape = sv.shape
     rec = cosmo_records(np.array([[cp["h"], cp["Om"], cp["s8"], cp["Ob"], cp["ns"], cp["zeq"],
-                                   cp["Ok"], cp["w0"], cp["wa"], float(z_s)]]), int(n_z))
+                                   cp["Ok"], cp["w0"], cp["wa"], float(z_s)]]), int(n_z),
+                        radiation=cp["radiation"], growth_mode=cp["growth_mode"])
     sv.run(rec)
     x = np.asarray(xi_out, dtype=np.float64)
     t = torch.as_tensor(x, device=sv.dev)
@@ -274,24 +281,34 @@ def clipped_moments(xi, P, clip=1.0):
                 skew=float(np.trapezoid((x - mean) ** 3 * p, x) / var ** 1.5))
 
 
-def generate_pdf_lnmu_batch(params, precision="fp32", graph=True, shape=None):
+def generate_pdf_lnmu_batch(params, precision="fp32", graph=True, shape=None, *,
+                            growth_mode="ode", radiation="engine"):
     """Batched solve (no reference counterpart): params [B, 10] in the order
     (h, Om, sigma8, Ob, ns, zeq, Ok, w0, wa, z_s); slice count per instance
-    ceil(24 (1 + ln(1 + z_s))).  Returns (lnmu grid, p [B, n_out] float32
-    numpy, diagnostics [B, 8])."""
+    ceil(24 (1 + ln(1 + z_s))); growth_mode ("ode", "legacy", "auto") and
+    radiation ("engine", None) apply to the whole batch.  Returns (lnmu grid,
+    p [B, n_out] float32 numpy, diagnostics [B, 8])."""
     from ._lib import PREC_BF16X3, PREC_FP32
     from .solver import Shape, Solver, cosmo_records, slice_count
     params = np.atleast_2d(np.asarray(params, dtype=np.float64))
+    modes = set()
     for row in params:
-        check_cosmology(dict(h=row[0], Om=row[1], s8=row[2], Ob=row[3], ns=row[4], zeq=row[5],
-                             Ok=row[6], w0=row[7], wa=row[8]))
+        cp = check_cosmology(dict(h=row[0], Om=row[1], s8=row[2], Ob=row[3], ns=row[4],
+                                  zeq=row[5], Ok=row[6], w0=row[7], wa=row[8],
+                                  growth_mode=growth_mode, radiation=radiation))
+        modes.add(cp["growth_mode"])
         if not np.isfinite(row[9]) or row[9] <= 0:
             raise ValueError("z_s must be a finite positive number")
+    if len(modes) > 1:
+        raise ValueError("growth_mode='auto' resolves differently within the batch; "
+                         "split it by dark-energy model")
+    mode = modes.pop() if modes else "ode"
     n_z = slice_count(params[:, 9])
     sh = shape or Shape(n_z_max=int(n_z.max()))
     prec = PREC_FP32 if precision == "fp32" else PREC_BF16X3
     sv = Solver(sh, params.shape[0], path="binned", precision=prec)
-    sv.set_records(cosmo_records(params, n_z))
+    sv.set_records(cosmo_records(params, n_z, radiation=None if radiation in ("none", False, None)
+                                 else radiation, growth_mode=mode))
     if graph:
         sv.capture()
     p, d = sv.run()

@@ -14,9 +14,12 @@ constexpr int kCovMaxModes = 2048;
 constexpr int kStatusCovModes = 1;      // covariance mode count clamped
 constexpr int kStatusBadShape = 2;
 
-// slice record (8 doubles)
+// slice record (8 doubles): z, chi, dchi, dchi (1+z)^2, D(z) (the growth of
+// sigma(M, z)), Sigma_cr, E^2, Dg(z) (the growth in the bias; D(z) unless the
+// legacy growth is selected)
 constexpr int kSliceStride = 8;
-constexpr int kSlZ = 0, kSlChi = 1, kSlDchi = 2, kSlWz = 3, kSlD = 4, kSlScr = 5, kSlE2 = 6;
+constexpr int kSlZ = 0, kSlChi = 1, kSlDchi = 2, kSlWz = 3, kSlD = 4, kSlScr = 5, kSlE2 = 6,
+              kSlDg = 7;
 // mass record (4 doubles): M [Msun], sigma(M, 0), dln sigma / dln M
 constexpr int kMassStride = 4;
 // row record (8 doubles)

@@ -9,6 +9,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace sgl {
@@ -24,9 +25,13 @@ constexpr double kPst = 0.3, kQst = 0.8;
 // Cosmological parameters of one instance, as laid out in HBM (16 doubles).
 struct Cosmo {
   double h, Om, s8, Ob, ns, zeq, Ok, w0, wa, zs;
-  double n_z;  // slice count of this instance (stored as double)
-  double pad[5];
+  double n_z;            // slice count of this instance (stored as double)
+  double rad_none;       // SGL_REC_RADIATION: 1 = no radiation
+  double growth_legacy;  // SGL_REC_GROWTH: 1 = Heath-integral D, CPT growth in the bias
+  double pad[3];
 };
+static_assert(offsetof(Cosmo, rad_none) == SGL_REC_RADIATION * sizeof(double), "record slot");
+static_assert(offsetof(Cosmo, growth_legacy) == SGL_REC_GROWTH * sizeof(double), "record slot");
 
 // Derived per-instance scalars (16 doubles), written by the background kernel.
 struct Derived {

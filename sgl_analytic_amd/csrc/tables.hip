@@ -2,7 +2,7 @@
 //
 // For every instance (cosmology + source redshift) the kernels below build
 //   derived scalars     Omega_r, Omega_DE, H0, rho_c0, rho_m, chi(z_s), D_A(z_s), A_norm
-//   slice table         z, chi, dchi, dchi (1+z)^2, D(z), Sigma_cr(z, z_s), E(z)
+//   slice table         z, chi, dchi, dchi (1+z)^2, D(z), Sigma_cr(z, z_s), E(z)^2, Dg(z)
 //   mass table          M, sigma(M, 0), dln sigma / dln M
 //   row table           per (slice, mass): k_s, r_s, x_t = c, r_vir, w_row, beta
 //   slice covariance    C_ij of the smoothed 1-D linear density field
@@ -17,10 +17,15 @@
 namespace sgl {
 
 // ------------------------------------------------------------ background --
-// One workgroup per instance.  chi(z) = (c/H0) int dz / E on NPANEL uniform
-// z-panels with 8-point Gauss-Legendre, fixed-order prefix sum, then Newton
-// on each slice midpoint chi_i = (i + 1/2) chi_s / n_z.
+// One workgroup of kBgThreads per instance.  chi(z) = (c/H0) int dz / E on
+// kPanels uniform z-panels with 8-point Gauss-Legendre, a fixed-order prefix
+// sum (each thread runs over its own kPanels / kBgThreads consecutive panels,
+// then a Hillis-Steele scan of the per-thread totals), then Newton on each
+// slice midpoint chi_i = (i + 1/2) chi_s / n_z.
 constexpr int kPanels = 2048;
+constexpr int kBgThreads = 256;
+constexpr int kBgPer = kPanels / kBgThreads;
+static_assert(kPanels % kBgThreads == 0, "whole panel runs per thread");
 __constant__ double kGl8X[8] = {-0.9602898564975363, -0.7966664774136267, -0.5255324099163290,
                                 -0.1834346424956498, 0.1834346424956498, 0.5255324099163290,
                                 0.7966664774136267, 0.9602898564975363};
@@ -35,26 +40,43 @@ __device__ inline double inv_e_integral(const Cosmo& c, double Or, double OL, do
   return s * r;
 }
 
-__global__ __launch_bounds__(256) void k_background(const Cosmo* __restrict__ cosmo, int n_z_max,
-                                                    Derived* __restrict__ der,
-                                                    double* __restrict__ slices) {
+__global__ __launch_bounds__(kBgThreads) void k_background(const Cosmo* __restrict__ cosmo,
+                                                           int n_z_max, Derived* __restrict__ der,
+                                                           double* __restrict__ slices) {
   __shared__ double chi_e[kPanels + 1];
+  __shared__ double run[kBgThreads];
   const int b = blockIdx.x, t = threadIdx.x;
   const Cosmo c = cosmo[b];
-  const double Or = c.Om / (1.0 + c.zeq);
+  // radiation "engine": Omega_r = Omega_m / (1 + z_eq); None: no radiation
+  // (sgl.py:93-104)
+  const double Or = (c.rad_none != 0.0) ? 0.0 : c.Om / (1.0 + c.zeq);
   const double OL = 1.0 - c.Om - c.Ok - Or;
   const double H0 = 100.0 * c.h;
   const double dh = kCkms / H0;
   const double zs = c.zs;
   const int n_z = (int)c.n_z;
   const double dz = zs / kPanels;
-  for (int p = t; p < kPanels; p += blockDim.x)
+  for (int p = t; p < kPanels; p += kBgThreads)
     chi_e[p + 1] = dh * inv_e_integral(c, Or, OL, p * dz, (p + 1) * dz);
   __syncthreads();
-  if (t == 0) {
-    chi_e[0] = 0.0;
-    for (int p = 1; p <= kPanels; ++p) chi_e[p] += chi_e[p - 1];
+  double loc[kBgPer], acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < kBgPer; ++j) {
+    acc += chi_e[t * kBgPer + j + 1];
+    loc[j] = acc;
   }
+  run[t] = acc;
+  __syncthreads();
+  for (int off = 1; off < kBgThreads; off <<= 1) {
+    const double add = (t >= off) ? run[t - off] : 0.0;
+    __syncthreads();
+    run[t] += add;
+    __syncthreads();
+  }
+  const double base = (t > 0) ? run[t - 1] : 0.0;
+#pragma unroll
+  for (int j = 0; j < kBgPer; ++j) chi_e[t * kBgPer + j + 1] = base + loc[j];
+  if (t == 0) chi_e[0] = 0.0;
   __syncthreads();
   const double chis = chi_e[kPanels];
   const double dchi = chis / n_z;
@@ -65,7 +87,7 @@ __global__ __launch_bounds__(256) void k_background(const Cosmo* __restrict__ co
     d.rhom = c.Om * d.rhoc0; d.chis = chis; d.DAs = DAs; d.dchi = dchi; d.n_z = n_z;
     der[b] = d;
   }
-  for (int i = t; i < n_z_max; i += blockDim.x) {
+  for (int i = t; i < n_z_max; i += kBgThreads) {
     double* S = slices + ((size_t)b * n_z_max + i) * kSliceStride;
     if (i >= n_z) {
       for (int q = 0; q < kSliceStride; ++q) S[q] = 0.0;
@@ -93,7 +115,7 @@ __global__ __launch_bounds__(256) void k_background(const Cosmo* __restrict__ co
     S[kSlD] = 0.0;
     S[kSlScr] = kCkms * kCkms * DAs / (4.0 * kPi * kGnewt * DAl * DAls);
     S[kSlE2] = E2;
-    S[7] = 0.0;
+    S[kSlDg] = 0.0;
   }
 }
 
@@ -155,6 +177,33 @@ __device__ inline void rk4_lin(double h, double a1, double b1, double a2, double
   y1 += h / 6.0 * (k1b + 2.0 * k2b + 2.0 * k3b + k4b);
 }
 
+// Legacy growth (growth_mode="legacy", LambdaCDM only).  D(a) is the Heath
+// integral E(a) I(a) / (E(1) I(1)), I(a) = int_0^a da' / (a' E(a'))^3, with E
+// of matter + curvature + Lambda only, Omega_L = 1 - Omega_m - Omega_k
+// (sgl.py:270-306: the integral never carries radiation).  With a' = a t^2,
+// I(a) = 2 a^(5/2) int_0^1 t^4 (Om + Ok a t^2 + OL a^3 t^6)^(-3/2) dt, whose
+// integrand is analytic: the 24-point Gauss-Legendre rule gives it to 1.3e-14
+// relative over the parameter box (CPU check against 30-digit quadrature).
+__device__ inline double heath_integral(double a, double Om, double Ok, double OLg) {
+  double s = 0.0;
+  for (int j = 0; j < 24; ++j) {
+    const double t = kGl24X[j], t2 = t * t, x = a * t2;
+    const double q = Om + Ok * x + OLg * x * x * x;
+    s += kGl24W[j] * t2 * t2 / (q * sqrt(q));
+  }
+  return 2.0 * a * a * sqrt(a) * s;
+}
+
+// Carroll-Press-Turner growth g(z) / (1 + z) with E^2 including radiation when
+// the background has it (sgl_full.py:521-538); Dg(z) = cpt(z) / cpt(0).
+__device__ inline double cpt_growth(double z, double Om, double Ok, double Or, double OL) {
+  const double zp1 = 1.0 + z;
+  const double m3 = Om * zp1 * zp1 * zp1;
+  const double E2 = m3 + OL + Ok * zp1 * zp1 + Or * zp1 * zp1 * zp1 * zp1;
+  const double Omz = m3 / E2, OLz = OL / E2;
+  return 2.5 * Omz / (pow(Omz, 4.0 / 7.0) - OLz + (1.0 + 0.5 * Omz) * (1.0 + OLz / 70.0)) / zp1;
+}
+
 __global__ __launch_bounds__(64) void k_growth(const Cosmo* __restrict__ cosmo, int n_z_max,
                                                Derived* __restrict__ der,
                                                double* __restrict__ slices) {
@@ -165,6 +214,20 @@ __global__ __launch_bounds__(64) void k_growth(const Cosmo* __restrict__ cosmo, 
   const GrowthRhs f{c.Om, c.Ok, d.Or, d.OL, c.w0, c.wa, c.w0 == -1.0 && c.wa == 0.0};
   const int n_z = (int)c.n_z;
   double* Sb = slices + (size_t)b * n_z_max * kSliceStride;
+  if (c.growth_legacy != 0.0) {
+    const double OLg = 1.0 - c.Om - c.Ok;
+    const double Eg1 = sqrt(c.Om + c.Ok + OLg);
+    const double D0 = Eg1 * heath_integral(1.0, c.Om, c.Ok, OLg);
+    const double g0 = cpt_growth(0.0, c.Om, c.Ok, d.Or, d.OL);
+    for (int i = L; i < n_z; i += 64) {
+      const double z = Sb[i * kSliceStride + kSlZ], a = 1.0 / (1.0 + z);
+      const double Eg = sqrt(c.Om / (a * a * a) + c.Ok / (a * a) + OLg);
+      Sb[i * kSliceStride + kSlD] = Eg * heath_integral(a, c.Om, c.Ok, OLg) / D0;
+      Sb[i * kSliceStride + kSlDg] = cpt_growth(z, c.Om, c.Ok, d.Or, d.OL) / g0;
+    }
+    if (L == 0) der[b].D0 = D0;
+    return;
+  }
   for (int i = L; i < n_z; i += 64) xs[i] = -log1p(Sb[i * kSliceStride + kSlZ]);
   const double x0 = log(1e-6);
   const double h = -x0 / kGrowthSteps;
@@ -225,8 +288,9 @@ __global__ __launch_bounds__(64) void k_growth(const Cosmo* __restrict__ cosmo, 
         const double s2 = s * s, s3 = s2 * s;
         const double h00 = 2 * s3 - 3 * s2 + 1, h10 = s3 - 2 * s2 + s, h01 = -2 * s3 + 3 * s2,
                      h11 = s3 - s2;
-        Sb[next * kSliceStride + kSlD] =
-            (h00 * y0 + h10 * hh * y1 + h01 * n0v + h11 * hh * n1v) / Dend;
+        const double Dz = (h00 * y0 + h10 * hh * y1 + h01 * n0v + h11 * hh * n1v) / Dend;
+        Sb[next * kSliceStride + kSlD] = Dz;
+        Sb[next * kSliceStride + kSlDg] = Dz;             // the ODE growth is the bias growth
         --next;
       }
       y0 = n0v;
@@ -410,7 +474,7 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
                                               int n_m, double dlnM, double a_st, double dc0,
                                               int edge_rvir, double* __restrict__ rows,
                                               double* __restrict__ frows) {
-  __shared__ double sk[13];
+  __shared__ double sk[14];
   // (sk[8] = rho_c0 for the filament rows)
   const int i = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
   const Cosmo c = cosmo[b];
@@ -418,10 +482,10 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
   double* Rs = rows + ((size_t)b * n_z_max + i) * n_m * kRowStride;
   double* Fs = frows ? frows + ((size_t)b * n_z_max + i) * n_m * kFilRowStride : nullptr;
   const double* S = slices + ((size_t)b * n_z_max + i) * kSliceStride;
-  if (t < 8) {
+  if (t < 9) {
     // Ludlow+16 with the z <= 7 clamp (sgl.py:590-606) and the Bryan-Norman
     // Delta_vir at Omega_m(z) (subhalos.py:381-385).  The four powers a^e,
-    // the two cube roots and the two logs are one log and one exp on eight
+    // the two cube roots and the three logs are one log and one exp on nine
     // lanes: lane q takes v = f_q exp(e_q ln u_q) (q < 6) or ln u_q.
     const Derived d = der[b];
     const double z = S[kSlZ], E2 = S[kSlE2];
@@ -434,11 +498,12 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
     const double f = (t == 0) ? 3.395 : (t == 1) ? 0.307 : (t == 2) ? 0.628 : (t == 3) ? 0.317 : 1.0;
     const double e = (t == 0) ? -0.215 : (t == 1) ? 0.540 : (t == 2) ? -0.047
                      : (t == 3) ? -0.893 : 1.0 / 3.0;
-    const int slot = (t < 4) ? t + 1 : (t == 4) ? 9 : (t == 5) ? 12 : (t == 6) ? 10 : 11;
+    const int slot = (t < 4) ? t + 1 : (t == 4) ? 9 : (t == 5) ? 12 : (t == 6) ? 10
+                     : (t == 7) ? 11 : 13;
     const double u = (t < 4) ? a
                      : (t == 4) ? 200.0 / Dvir
                      : (t == 5) ? 3.0 / (4.0 * kPi * 200.0 * rhoc)
-                     : (t == 6) ? poly : S[kSlD];
+                     : (t == 6) ? poly : (t == 7) ? S[kSlD] : S[kSlDg];
     const double lu = log(u);
     sk[slot] = (t < 6) ? f * exp(e * lu) : lu;
     if (t == 0) {
@@ -450,10 +515,12 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
     }
   }
   __syncthreads();
-  const double D = S[kSlD];
+  // D: growth of sigma(M, z) in the mass function; Dg: growth in the bias and
+  // beta (sgl_full.py:541-548, 1861), the same as D on the ODE growth
+  const double D = S[kSlD], Dg = S[kSlDg];
   const double c0 = sk[1], beta_ = sk[2], g1 = sk[3], g2 = sk[4];
   const double Dvir = sk[6], rhom = sk[7], rhoc0 = sk[8], cr = sk[9];
-  const double lpoly = sk[10], lD = sk[11], r200f = sk[12], rhoc = sk[5];
+  const double lpoly = sk[10], lD = sk[11], r200f = sk[12], rhoc = sk[5], lDg = sk[13];
   const double ldc = log(kDeltaC), lqst = log(kQst), l08dc0 = log(0.8) + 2.0 * log(dc0);
   for (int j = t; j < n_m; j += blockDim.x) {
     double* R = Rs + (size_t)j * kRowStride;
@@ -482,15 +549,16 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
     // (sgl_full.py:1874-1880)
     const double eta = eta_rvir(C, Dvir, fC, cr);
     const double rvir = eta * C * rs;
-    // linear halo bias with the growth D (sgl_full.py:541-543)
-    const double qn2 = 0.8 * (dc0 / D / sig0) * (dc0 / D / sig0);
-    const double bh = 1.0 + (qn2 - 1.0) / dc0 + 0.6 / (dc0 * (1.0 + exp(0.3 * (l08dc0 - 2.0 * lsig))));
+    // linear halo bias with the bias growth Dg (sgl_full.py:541-543)
+    const double qn2 = 0.8 * (dc0 / Dg / sig0) * (dc0 / Dg / sig0);
+    const double bh = 1.0 + (qn2 - 1.0) / dc0 +
+                      0.6 / (dc0 * (1.0 + exp(0.3 * (l08dc0 - 2.0 * (lsig0 + lDg)))));
     R[kRowKs] = ks;
     R[kRowRs] = rs;
     R[kRowXt] = edge_rvir ? eta * C : C;
     R[kRowRvir] = rvir;
     R[kRowW] = S[kSlWz] * dndlnM * dlnM;
-    R[kRowBeta] = D * bh;
+    R[kRowBeta] = Dg * bh;
     R[6] = 0.0;
     R[7] = 0.0;
     if (Fs) {
@@ -502,12 +570,12 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
       const double multf = 0.5 * 2.0 * sqrt(qf / (2.0 * kPi)) * exp(-0.5 * qf);
       const double dnf = (rhom / M) * multf * (-2.0 * dls);
       const double rsf = cbrt(M / 1e14);
-      const double qb = 0.7 * (dc0 / D / sig0) * (dc0 / D / sig0);
+      const double qb = 0.7 * (dc0 / Dg / sig0) * (dc0 / Dg / sig0);
       double* F = Fs + (size_t)j * kFilRowStride;
       F[0] = rsf * 14.4 * rhoc0 * (20.0 * rsf) / S[kSlScr];
       F[1] = rsf;
       F[2] = S[kSlWz] * dnf * dlnM;
-      F[3] = D * (1.0 + (qb - 1.0) / dc0);
+      F[3] = Dg * (1.0 + (qb - 1.0) / dc0);
     }
   }
 }
@@ -751,7 +819,7 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
   const bool cov_on = (sh.flags & SGL_FLAG_CLUSTERING) != 0;
   bool fail = false;
   auto ck = [&](hipError_t e) { fail |= (e != hipSuccess); };
-  hipLaunchKernelGGL(k_background, dim3(B), dim3(256), 0, st, C, sh.n_z_max, D, slices);
+  hipLaunchKernelGGL(k_background, dim3(B), dim3(kBgThreads), 0, st, C, sh.n_z_max, D, slices);
   // growth needs the slices and Omega_r, Omega_DE (background); it writes
   // D(z) into the slices and D0 into the derived record
   hipStream_t sg = st;

@@ -21,8 +21,7 @@ import torch
 
 from . import _lib
 from ._lib import (COSMO_STRIDE, FLAG_CLUSTERING, FLAG_EDGE_RVIR, FLAG_FILAMENTS, FLAG_WINDOW,
-                   PREC_BF16X3, PREC_FP32,
-                   PREC_FP64)
+                   PREC_BF16X3, PREC_FP32, PREC_FP64, REC_GROWTH, REC_RADIATION)
 
 PARAM_NAMES = ("h", "Om", "sigma8", "Ob", "ns", "zeq", "Ok", "w0", "wa", "z_s")
 
@@ -75,15 +74,27 @@ def slice_count(z_s) -> np.ndarray:
     return np.ceil(24.0 * (1.0 + np.log1p(np.asarray(z_s, float)))).astype(np.int64)
 
 
-def cosmo_records(params: np.ndarray, n_z) -> np.ndarray:
-    """[B, 10] parameters (PARAM_NAMES order) + slice counts -> [B, 16] records."""
+def cosmo_records(params: np.ndarray, n_z, radiation="engine", growth_mode="ode") -> np.ndarray:
+    """[B, 10] parameters (PARAM_NAMES order) + slice counts -> [B, 16] records.
+
+    radiation: "engine" (Omega_r = Omega_m / (1 + z_eq)) or None; growth_mode:
+    "ode" or "legacy" (LambdaCDM only; resolve "auto" first).  Both apply to
+    every record."""
     params = np.atleast_2d(np.asarray(params, dtype=np.float64))
     if params.shape[1] != 10:
         raise ValueError("params must have 10 columns: " + ", ".join(PARAM_NAMES))
+    if radiation not in ("engine", None):
+        raise ValueError("radiation must be 'engine' or None")
+    if growth_mode not in ("ode", "legacy"):
+        raise ValueError("growth_mode must be 'ode' or 'legacy' (resolve 'auto' first)")
     B = params.shape[0]
+    if growth_mode == "legacy" and not np.all((params[:, 7] == -1.0) & (params[:, 8] == 0.0)):
+        raise ValueError("legacy growth is only valid for LambdaCDM")
     rec = np.zeros((B, COSMO_STRIDE))
     rec[:, :10] = params
     rec[:, 10] = np.broadcast_to(np.asarray(n_z, dtype=np.float64), (B,))
+    rec[:, REC_RADIATION] = 1.0 if radiation is None else 0.0
+    rec[:, REC_GROWTH] = 1.0 if growth_mode == "legacy" else 0.0
     return rec
 
 

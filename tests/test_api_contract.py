@@ -53,8 +53,6 @@ def test_paths_not_computed_here_raise():
             api.run(1.0, "+ell", cosmo={}, xi_out=np.array([0.0, 1.0]))
         with pytest.raises(NotImplementedError):
             api.run(1.0, "+sub", cosmo={}, xi_out=np.array([0.0, 1.0]))
-        with pytest.raises(NotImplementedError):
-            api.run(1.0, "full", cosmo=dict(radiation=None), xi_out=np.array([0.0, 1.0]))
         with pytest.raises(ValueError):     # vp needs the bias ingredient, as in the reference
             api.run(1.0, "-bias", cosmo={}, xi_out=np.array([0.0, 1.0]), closure="vp")
         with pytest.raises(NotImplementedError):    # the threshold edge is not built
@@ -62,6 +60,29 @@ def test_paths_not_computed_here_raise():
         with pytest.raises(ValueError):
             api.run(1.0, "full", cosmo={}, xi_out=np.array([0.0, 1.0]), edge="r500")
         solver.assert_not_called()
+
+
+def test_growth_and_radiation_options():
+    """growth_mode "auto" resolves as sgl.Cosmology does (legacy for LambdaCDM,
+    the ODE otherwise, with the pipeline's warning), radiation "none" is None,
+    and both reach the device record slots."""
+    from sgl_analytic_amd import _lib
+    from sgl_analytic_amd.solver import cosmo_records
+    assert api.check_cosmology(dict(growth_mode="auto"))["growth_mode"] == "legacy"
+    with pytest.warns(UserWarning):
+        assert api.check_cosmology(dict(growth_mode="auto", w0=-0.9))["growth_mode"] == "ode"
+    assert api.check_cosmology(dict(radiation="none"))["radiation"] is None
+    p = np.array([[0.674, 0.315, 0.811, 0.0493, 0.965, 3402.0, 0.0, -1.0, 0.0, 1.0]])
+    r = cosmo_records(p, 8)
+    assert r[0, _lib.REC_RADIATION] == 0.0 and r[0, _lib.REC_GROWTH] == 0.0
+    r = cosmo_records(p, 8, radiation=None, growth_mode="legacy")
+    assert r[0, _lib.REC_RADIATION] == 1.0 and r[0, _lib.REC_GROWTH] == 1.0
+    q = p.copy()
+    q[0, 7] = -0.9
+    with pytest.raises(ValueError):
+        cosmo_records(q, 8, growth_mode="legacy")
+    with pytest.raises(ValueError):
+        cosmo_records(p, 8, growth_mode="auto")
 
 
 def test_invalid_redshift_and_workers_fail_before_build():

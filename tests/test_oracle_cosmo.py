@@ -106,3 +106,47 @@ def test_dndlnm_chain_rule(planck):
     f = (oc.A_ST * (1 + (oc.Q_ST * nu) ** -oc.P_ST) * np.sqrt(oc.Q_ST * nu / (2 * np.pi))
          * np.exp(-oc.Q_ST * nu / 2))
     assert abs(planck.dndlnM(M, z) / (planck.rhom / M * f * dlnnu) - 1) < 1e-6
+
+
+def test_legacy_growth_pins():
+    """growth_mode="legacy" (the Heath integral, sgl.py:270-306): for matter +
+    Lambda + curvature without radiation it is the exact growing mode, so it
+    agrees with the growth ODE run without radiation; Einstein-de Sitter gives
+    D = a; it never carries radiation (the reference's
+    test_heath_integral_excludes_radiation)."""
+    z = np.array([0.05, 0.5, 1.0, 3.0, 6.0, 10.0])
+    for Ok in (0.0, 0.05, -0.05):
+        leg = Cosmology(radiation=None, growth_mode="legacy", Ok=Ok)
+        ode = Cosmology(radiation=None, Ok=Ok)
+        np.testing.assert_allclose(leg.D(z), ode.D(z), rtol=1e-9)
+    np.testing.assert_array_equal(Cosmology(growth_mode="legacy").D(z),
+                                  Cosmology(radiation=None, growth_mode="legacy").D(z))
+    eds = Cosmology(Om=1.0, Ob=0.05, radiation=None, growth_mode="legacy")
+    np.testing.assert_allclose(eds.D(z), 1.0 / (1.0 + z), rtol=1e-13)
+    with pytest.raises(ValueError):
+        Cosmology(w0=-0.9, growth_mode="legacy")
+
+
+def test_bias_growth_pins():
+    """Dg (sgl_full.py:521-538): the ODE growth on growth_mode="ode"; the
+    Carroll-Press-Turner fit on "legacy", which is exact for Einstein-de
+    Sitter (Omega_m(z) = 1 gives g = 1, Dg = 1/(1+z)) and within 0.3 % of the
+    exact LambdaCDM growth (the fit's published accuracy is ~1 %)."""
+    z = np.array([0.05, 0.5, 1.0, 3.0, 6.0, 10.0])
+    ode = Cosmology()
+    np.testing.assert_array_equal(ode.Dg(z), ode.D(z))
+    eds = Cosmology(Om=1.0, Ob=0.05, radiation=None, growth_mode="legacy")
+    np.testing.assert_allclose(eds.Dg(z), 1.0 / (1.0 + z), rtol=1e-14)
+    for rad in ("engine", None):
+        leg = Cosmology(radiation=rad, growth_mode="legacy")
+        assert np.max(np.abs(leg.Dg(z) / leg.D(z) - 1)) < 3e-3
+        assert leg.Dg(0.0) == 1.0
+
+
+def test_no_radiation_background():
+    """radiation=None: E^2 = Om (1+z)^3 + 1 - Om for flat LambdaCDM, exactly
+    (the reference's test_library_default_has_no_radiation_and_old_expression)."""
+    z = np.array([0.1, 0.5, 1.0, 3.0, 6.0, 10.0])
+    c = Cosmology(radiation=None)
+    assert c.Or == 0.0 and c.OL == 1.0 - c.Om
+    np.testing.assert_allclose(c.E(z), np.sqrt(c.Om * (1 + z) ** 3 + 1.0 - c.Om), rtol=1e-15)
