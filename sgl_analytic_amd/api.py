@@ -21,6 +21,7 @@ Batched entry point (no reference counterpart): generate_pdf_lnmu_batch.
 from __future__ import annotations
 
 import math
+import time
 import warnings
 
 import numpy as np
@@ -179,7 +180,10 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAUL
     rec = cosmo_records(np.array([[cp["h"], cp["Om"], cp["s8"], cp["Ob"], cp["ns"], cp["zeq"],
                                    cp["Ok"], cp["w0"], cp["wa"], float(z_s)]]), int(n_z),
                         radiation=cp["radiation"], growth_mode=cp["growth_mode"])
+    t0 = time.perf_counter()
     sv.run(rec)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
     x = np.asarray(xi_out, dtype=np.float64)
     t = torch.as_tensor(x, device=sv.dev)
     p = torch.zeros(x.size, dtype=torch.float64, device=sv.dev)
@@ -187,13 +191,14 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAUL
                                        _ptr(t), int(x.size), _ptr(p), _stream()),
                "sgl_invert_fp64_at")
     torch.cuda.synchronize()
+    t2 = time.perf_counter()
     d = sv.diag[0].cpu().numpy()
-    info = dict(zs=float(z_s), cfg=config, ingredients=["halo"] + sorted(s & MODELLED),
+    info = reference_schema(cp, sv_shape, d, config, closure, edge, t_solve=t1 - t0,
+                            t_invert=t2 - t1)
+    info.update(zs=float(z_s), cfg=config, ingredients=["halo"] + sorted(s & MODELLED),
                 clustering="slice-covariance" if "bias" in s else None, closure=closure,
                 edge=edge,
-                n_slices=int(n_z), grid=dict(n_k=sv_shape.n_k, k_min=sv_shape.k_lo,
-                                             k_max=sv_shape.k_hi, n_m=sv_shape.n_m,
-                                             n_b=sv_shape.n_b),
+                n_slices=int(n_z),
                 kernel_grid=dict(xi_min=sv_shape.out_lo, xi_max=sv_shape.out_hi,
                                  n=sv_shape.n_out, mass=float(d[0]), mean=float(d[1]),
                                  var=float(d[2]), skew=float(d[3]), sigmaDL=float(d[4]),
@@ -203,6 +208,39 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAUL
                 growth_mode=cp["growth_mode"], radiation=cp["radiation"],
                 backend="MI355X HIP (libsgl_amd.so), FP64 plain path")
     return p.cpu().numpy(), info
+
+
+def reference_schema(cp, shape, diag, config, closure, edge, t_solve, t_invert):
+    """Every top-level key of the reference's run() record (pipeline.py:243-274,
+    the key set of its stored PDFs), filled with this chain's values.  Keys
+    of machinery this chain does not have (the 2-D grid sizing, the
+    importance sampler, the subhalo and positive-closure settings, the weak
+    arm) are present with None, False or 0 so that readers of the reference's
+    files find them; the chain's own grid is under "grid" and "kernel_grid"."""
+    Or = cp["Om"] / (1.0 + cp["zeq"]) if cp["radiation"] == "engine" else 0.0
+    return dict(
+        t_stages=dict(build=float(t_solve), lam=0.0, invert=float(t_invert)),
+        zint="uniform-chi midpoint", zint_n=None,
+        clustering="slice-covariance" if shape.clustering else None,
+        xi_kperp=None, edge=edge, lss_var=0.0, kap_floor=None, kbar=None,
+        h=cp["h"], Om=cp["Om"], s8=cp["s8"], Ob=cp["Ob"], ns=cp["ns"], zeq=cp["zeq"],
+        Ok=cp["Ok"], w0=cp["w0"], wa=cp["wa"],
+        dark_energy=dict(w0=cp["w0"], wa=cp["wa"], growth_w0=cp["w0"], growth_wa=cp["wa"],
+                         growth_mode=cp["growth_mode"], radiation=cp["radiation"],
+                         Omega_r=Or),
+        nord=1, grid_scale=None, split=None, nproc=1, sub_kw=None, pair_angle=None,
+        pair_scale=None, k3=None, exactS=False, s_taper=None, sigma_kappa=None, kthr=None,
+        N_expl=None, N_fil=None, V_weak=0.0,
+        mean=float(diag[1]), var=float(diag[2]), skew=float(diag[3]), sigmaDL=float(diag[4]),
+        grid=dict(n_k=shape.n_k, k_min=shape.k_lo, k_max=shape.k_hi, n_m=shape.n_m,
+                  n_b=shape.n_b, xi_min=shape.out_lo, xi_max=shape.out_hi, n_out=shape.n_out),
+        tail_box=False, fine=None, s_is=False, xi_closure=closure, window=shape.window,
+        n_xi_wide=None, pos_closure=None,
+        direct_product_backend="MI355X HIP (libsgl_amd.so)", direct_product_fallback=None,
+        s_seed=None, s_seed_effective=None, s_ess=None, is_design=None,
+        xi_clamp=False, xi_linear_m=0.0, pk_mod=None, pk_par={},
+        provenance=dict(producer="sgl_analytic_amd api.run (MI355X, 1-D ln mu jump chain)"),
+        seconds=float(t_solve + t_invert))
 
 
 def generate_pdf_lnmu(z_s, h=0.674, Om=0.315, sigma8=0.811, Ob=0.0493,

@@ -3,7 +3,8 @@
 Reads (as JSON/text, nothing executed) the stored PDFs
 tests/reference/pdf_zs*.json and the walkthrough notebook's printed outputs
 of /root/reference, and keeps the scalar values this repo's oracle is pinned
-on.  Run in the build container: python tests/golden/make_golden.py
+on, plus the top-level key set of the stored PDF records (the output schema
+the command line reproduces).  Run in the build container: python tests/golden/make_golden.py
 """
 import json
 import os
@@ -15,9 +16,11 @@ OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_scalar
 
 def main():
     fx = {}
+    keys = set()
     for name in ("pdf_zs0.5", "pdf_zs1", "pdf_zs3", "pdf_zs10"):
         with open(os.path.join(REF, "tests", "reference", name + ".json")) as f:
             d = json.load(f)
+        keys |= set(d)
         fx[name] = dict(Omega_r=d["dark_energy"]["Omega_r"],
                         primordial_growth=d["dark_energy"]["primordial_growth"],
                         sigma_kappa=d["sigma_kappa"], kbar=d["kbar"],
@@ -39,7 +42,8 @@ def main():
                                  kappa_s=float(m3.group(3)), rvir_over_rs=float(m3.group(4))))
     out = dict(provenance="sgl_analytic v0.2.0 (MIT): tests/reference/pdf_zs*.json and the "
                           "printed outputs of notebooks/sgl_analytic_walkthrough.ipynb cells 3, 7",
-               fixtures=fx, walkthrough=nbv)
+               fixtures=fx, walkthrough=nbv,
+               fixture_keys=sorted(keys))
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print("wrote", OUT)
