@@ -36,6 +36,11 @@ typedef enum sgl_status {
                                  120 u bins x 13 orientation nodes (the "fil" ingredient) */
 #define SGL_FLAG_EDGE_RVIR 8  /* truncate the halos at r_vir (x_t = r_vir / r_s) instead of
                                  r200 (x_t = c): the reference's edge="rvir" */
+#define SGL_FLAG_ELL 16       /* halo ellipticity (the "ell" ingredient): eps(M, z) of
+                                 Allgood+06 with M*(z), every halo cell split into 12
+                                 orientation nodes with the pseudo-elliptical kappa, gamma
+                                 (sgl_full.py:551-613, 1659, 2038-2064); rows carry eps in
+                                 slot 6, the tables a sigma(M) table for M* */
 
 /* Batched path: slice amplitudes and Psi_2 at every SGL_PSI2_STRIDE-th k node
  * (sub-grid node jc = k node jc * stride of the log grid, the last one up to

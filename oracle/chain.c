@@ -13,6 +13,9 @@
  *   oracle_cells      one cell per (slice, mass, impact-parameter bin):
  *                     jump xi = -ln detA (sgl.py:808-815), source-plane
  *                     Esscher tilt w e^{-xi} (sgl.py:949-951)
+ *   oracle_cells_ell  the same with halo ellipticity: n_phi orientation nodes
+ *                     per cell, pseudo-elliptical kappa, gamma
+ *                     (sgl_full.py:582-613, 1659, 2038-2064)
  *   oracle_cells_fil  one cell per (slice, mass, u bin, orientation) of the
  *                     filament population (sgl_full.py:1984-2007)
  *   oracle_lnphi      compensated compound-Poisson exponent
@@ -207,6 +210,77 @@ void oracle_cells(int64_t n_rows, const double *rows, int64_t n_b, const double 
             if (x_ > XI_MAX) { xi[o] = 0.0; ws[o] = 0.0; continue; }
             xi[o] = x_;
             ws[o] = wrow * area * detA;     /* Esscher tilt e^{-xi} = detA */
+        }
+    }
+}
+
+/*
+ * Pseudo-elliptical truncated NFW (lensing.cpp:56 kappagammaNFWeps as the
+ * reference transcribes it, sgl_full.py:582-613): the circular profile at the
+ * elliptical radius x_e of x1 = sqrt(1-eps) cos(phi) x, x2 = sqrt(1+eps)
+ * sin(phi) x, with c2 = (x1^2 - x2^2) / x_e^2,
+ *   kappa   = k0 + eps c2 g0,
+ *   gamma^2 = g0^2 + 2 eps c2 g0 k0 + eps^2 (k0^2 - (c2 g0)^2)
+ * (the engine's cos^2 form of the eps^2 term, kept by the reference), g0 = ks
+ * below x_e = 1e-4; beyond the truncation (x_e >= x_t) kappa = 0 and gamma
+ * is the monopole shear 4 ks mu(x_t) / x^2 at the circular x.
+ */
+static void kg_eps(double eps, double ks, double x, double phi, double xt, double *k, double *g) {
+    double x1 = sqrt(1.0 - eps) * cos(phi) * x, x2 = sqrt(1.0 + eps) * sin(phi) * x;
+    double xe = fmax(sqrt(x1 * x1 + x2 * x2), 1e-12);
+    double c2 = (x1 * x1 - x2 * x2) / fmax(xe * xe, 1e-300);
+    double k0, g0;
+    oracle_trunc_kg(1, &xe, ks, xt, &k0, &g0);
+    if (xe < 1e-4) g0 = ks;
+    double kk = k0 + eps * c2 * g0;
+    double g2 = g0 * g0 + 2.0 * eps * c2 * g0 * k0 + eps * eps * (k0 * k0 - (c2 * g0) * (c2 * g0));
+    if (xe >= xt) {
+        double gpm = ks * 4.0 * (log(1.0 + xt) - xt / (1.0 + xt)) / fmax(x * x, 1e-300);
+        kk = 0.0;
+        g2 = gpm * gpm;
+    }
+    *k = kk;
+    *g = sqrt(fmax(g2, 0.0));
+}
+
+void oracle_kg_eps(int64_t n, const double *x, const double *phi, double eps, double ks, double xt,
+                   double *kap, double *gam) {
+    if (!gl_ready) gl_init();
+    for (int64_t i = 0; i < n; ++i) kg_eps(eps, ks, x[i], phi[i], xt, &kap[i], &gam[i]);
+}
+
+/*
+ * Cells with halo ellipticity (the "ell" ingredient): every impact-parameter
+ * cell of a row splits into n_phi orientation nodes phi_q = (q + 1/2)
+ * (pi/2) / n_phi, each with 1/n_phi of the cell's weight and the
+ * pseudo-elliptical kappa, gamma of the row's eps (sgl_full.py:1659, 2038-2064).
+ * Output order [n_rows][n_b][n_phi]; jump rule and tilt as oracle_cells.
+ */
+void oracle_cells_ell(int64_t n_rows, const double *rows, const double *eps, int64_t n_b,
+                      const double *u_edges, int64_t n_phi, double *xi, double *ws, double *kap,
+                      double *gam) {
+    if (!gl_ready) gl_init();
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t r = 0; r < n_rows; ++r) {
+        const double *R = rows + 6 * r;
+        double ks = R[0], rs = R[1], xt = R[2], rvir = R[3], wrow = R[4];
+        for (int64_t l = 0; l < n_b; ++l) {
+            double b0 = rvir * u_edges[l], b1 = rvir * u_edges[l + 1];
+            double bc = sqrt(b0 * b1), x = bc / rs;
+            double area = M_PI * (b1 * b1 - b0 * b0) / (double)n_phi;
+            for (int64_t q = 0; q < n_phi; ++q) {
+                double phi = (q + 0.5) * (0.5 * M_PI / (double)n_phi), k, g;
+                kg_eps(eps[r], ks, x, phi, xt, &k, &g);
+                double detA = (1.0 - k) * (1.0 - k) - g * g;
+                int64_t o = (r * n_b + l) * n_phi + q;
+                kap[o] = k;
+                gam[o] = g;
+                if (!(detA > 0.0)) { xi[o] = 0.0; ws[o] = 0.0; continue; }
+                double x_ = -log(detA);
+                if (x_ > XI_MAX) { xi[o] = 0.0; ws[o] = 0.0; continue; }
+                xi[o] = x_;
+                ws[o] = wrow * area * detA;
+            }
         }
     }
 }

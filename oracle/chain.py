@@ -65,6 +65,9 @@ def lib():
         _LIB.oracle_filon_weights.argtypes = [i64, d, i64, d, d]
         _LIB.oracle_trunc_kg.argtypes = [i64, d, ctypes.c_double, ctypes.c_double, d, d]
         _LIB.oracle_nfw_Fh.argtypes = [i64, d, d, d]
+        _LIB.oracle_kg_eps.argtypes = [i64, d, d, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, d, d]
+        _LIB.oracle_cells_ell.argtypes = [i64, d, d, i64, d, i64, d, d, d, d]
     return _LIB
 
 
@@ -93,6 +96,7 @@ class Shape:
     window: bool = True
     filaments: bool = False
     edge: str = "r200"           # halo truncation: "r200" (x_t = c) or "rvir" (x_t = eta c)
+    ell: bool = False            # halo ellipticity: N_PHI orientation nodes per cell
 
     def k_nodes(self):
         """k_q: 0 followed by n_k log-spaced nodes."""
@@ -159,8 +163,10 @@ def rows(cos: Cosmology, zs: float, shape: Shape):
             # r_vir (x_t = r_vir / r_s)
             xt = C if shape.edge == "r200" else rvir / rs
             R[i, j] = (ks, rs, xt, rvir, wz * dn, beta)
+    # halo ellipticity eps(M, z) (Allgood+06 with M*(z), sgl_full.py:551-579)
+    eps = np.array([cos.epsilon(M, zi) for zi in z]) if shape.ell else np.zeros(R.shape[:2])
     return dict(rows=R, z=z, chi=chi_mid, dchi=dchi, chi_edges=chi_edges, M=M, dlnM=dlnM,
-                chis=cos.chi(zs))
+                chis=cos.chi(zs), eps=eps)
 
 
 # ----------------------------------------------------------- filaments --
@@ -237,15 +243,42 @@ def cells_fil(F: np.ndarray):
     return [o.reshape(F.shape[0], F.shape[1], n) for o in out]
 
 
-def cells(R: np.ndarray, shape: Shape):
-    """Cell arrays [n_z, n_m, n_b]: xi, ws (tilted weight, 0 = dropped), kap, gam."""
+N_PHI = 12       # orientation nodes of an elliptical halo cell (sgl_full.py:1535, 1659)
+
+
+def cells(R: np.ndarray, shape: Shape, eps: np.ndarray = None):
+    """Cell arrays [n_z, n_m, n_b] (or [n_z, n_m, n_b * N_PHI] with eps, the
+    rows' ellipticities, as for shape.ell): xi, ws (tilted weight, 0 =
+    dropped), kap, gam."""
     L = lib()
     n_rows = R.shape[0] * R.shape[1]
     rr = np.ascontiguousarray(R.reshape(n_rows, 6))
     ue = shape.u_edges()
-    out = [np.zeros(n_rows * shape.n_b) for _ in range(4)]
-    L.oracle_cells(n_rows, _p(rr), shape.n_b, _p(ue), *[_p(o) for o in out])
-    return [o.reshape(R.shape[0], R.shape[1], shape.n_b) for o in out]
+    if eps is None:
+        out = [np.zeros(n_rows * shape.n_b) for _ in range(4)]
+        L.oracle_cells(n_rows, _p(rr), shape.n_b, _p(ue), *[_p(o) for o in out])
+        return [o.reshape(R.shape[0], R.shape[1], shape.n_b) for o in out]
+    ee = np.ascontiguousarray(np.asarray(eps, dtype=np.float64).reshape(n_rows))
+    out = [np.zeros(n_rows * shape.n_b * N_PHI) for _ in range(4)]
+    L.oracle_cells_ell(n_rows, _p(rr), _p(ee), shape.n_b, _p(ue), N_PHI, *[_p(o) for o in out])
+    return [o.reshape(R.shape[0], R.shape[1], shape.n_b * N_PHI) for o in out]
+
+
+def trunc_kg(x, ks, xt):
+    """Truncated-NFW kappa, gamma at x (the C restatement, sgl_full.py:660-669)."""
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    k, g = np.zeros_like(x), np.zeros_like(x)
+    lib().oracle_trunc_kg(x.size, _p(x), float(ks), float(xt), _p(k), _p(g))
+    return k, g
+
+
+def kg_eps(x, phi, eps, ks, xt):
+    """Pseudo-elliptical truncated-NFW kappa, gamma at (x, phi) (sgl_full.py:582-613)."""
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    phi = np.ascontiguousarray(np.broadcast_to(np.asarray(phi, dtype=np.float64), x.shape))
+    k, g = np.zeros_like(x), np.zeros_like(x)
+    lib().oracle_kg_eps(x.size, _p(x), _p(phi), float(eps), float(ks), float(xt), _p(k), _p(g))
+    return k, g
 
 
 def lnphi(xi, ws, beta, k):
@@ -344,7 +377,7 @@ def pdf(zs: float, shape: Shape = None, cosmo: dict = None, keep: bool = False) 
     cos = Cosmology(**(cosmo or {}))
     T = rows(cos, zs, shape)
     R = T["rows"]
-    xi, ws, kap, gam = cells(R, shape)
+    xi, ws, kap, gam = cells(R, shape, T["eps"] if shape.ell else None)
     kq = shape.k_nodes()
     lam, amp = lnphi(xi, ws, R[:, :, 5], kq[1:])
     if shape.filaments:

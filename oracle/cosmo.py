@@ -301,6 +301,25 @@ class Cosmology:
         qn2 = 0.8 * (DC0 / self.Dg(z) / self.sigmaM(M, 0.0)) ** 2
         return 1 + (qn2 - 1) / DC0 + 0.6 / (DC0 * (1 + qn2 ** 0.3))
 
+    def Mstar(self, z):                               # sgl_full.py:551-567
+        """sigma(M*, 0) Dg(z) = delta_c0: log-interpolated root on the
+        reference's table M = logspace(0, 17, 1200) Msun of the sigma spline
+        (extrapolated below its 1e4 node as the reference does), cut to the
+        largest suffix on which sigma strictly decreases."""
+        tab = getattr(self, "_mstar_tab", None)
+        if tab is None:
+            Mg = np.logspace(0, 17, 1200)
+            s = np.array([self.sigmaM(m, 0.0) for m in Mg])
+            up = np.nonzero(np.diff(s) >= 0)[0]
+            i0 = int(up[-1]) + 1 if up.size else 0
+            tab = self._mstar_tab = (Mg[i0:], s[i0:])
+        Mg, s = tab
+        return float(np.exp(np.interp(-DC0 / self.Dg(z), -s, log(Mg))))
+
+    def epsilon(self, M, z):                          # sgl_full.py:570-579 (Allgood+06)
+        s = 0.54 * (np.asarray(M, float) / self.Mstar(z)) ** -0.05
+        return np.maximum(0.0, (1 - s) / (1 + s))
+
     def Dvir(self, z):                                # subhalos.py:381-385 (BN98 at Om(z))
         d = self.Om * (1 + z) ** 3 / self.E2(z) - 1.0
         return 18 * pi ** 2 + 82 * d - 39 * d * d

@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsgl_amd.so")
 
 SGL_OK, SGL_ERR_INVALID, SGL_ERR_HIP, SGL_ERR_UNSUPPORTED = 0, 1, 2, 3
-FLAG_CLUSTERING, FLAG_WINDOW, FLAG_FILAMENTS, FLAG_EDGE_RVIR = 1, 2, 4, 8
+FLAG_CLUSTERING, FLAG_WINDOW, FLAG_FILAMENTS, FLAG_EDGE_RVIR, FLAG_ELL = 1, 2, 4, 8, 16
+ELL_PHI, MSTAR_N = 12, 1200   # orientation nodes per elliptical cell; M* sigma table
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 COSMO_STRIDE = 16
 REC_RADIATION, REC_GROWTH = 11, 12  # SGL_REC_*: record slots of the radiation and growth options

@@ -10,11 +10,12 @@ parameters; each cell is a jump xi = ln mu = -ln detA(kappa, gamma) of the
 1-D compound-Poisson chain of sgl.Lambda_of_k / P_of_xi (sgl.py:961-1012),
 tilted to the source plane; linear-bias clustering enters as the two-point
 term 1/2 sum_ij C_ij a_i a_j (sgl_full.py:2514-2538, 3353-3397); filaments
-join with the "fil" ingredient.  The cosmology options radiation ("engine",
-None) and growth_mode ("ode", "legacy", "auto") are computed as sgl.Cosmology
-defines them.  The reference's ellipticity and subhalo ingredients and its
-"vp" closure are not part of this path; asking for them raises
-NotImplementedError.
+join with the "fil" ingredient and the pseudo-elliptical halos (12
+orientation nodes per cell, Allgood+06 ellipticity) with "ell".  The
+cosmology options radiation ("engine", None) and growth_mode ("ode",
+"legacy", "auto") are computed as sgl.Cosmology defines them.  The
+reference's subhalo ingredient and its "vp" closure are not part of this
+path; asking for them raises NotImplementedError.
 
 Batched entry point (no reference counterpart): generate_pdf_lnmu_batch.
 """
@@ -29,7 +30,7 @@ import numpy as np
 P = dict(h=0.674, Om=0.315, s8=0.811, Ob=0.0493, ns=0.965, zeq=3402.0)  # pipeline.py:20
 EXTENSIONS = dict(Ok=0.0, w0=-1.0, wa=0.0, growth_mode="ode", radiation="engine")
 INGREDIENTS = ("ell", "fil", "sub", "bias")
-MODELLED = {"bias", "fil"}     # ingredients this path computes on top of NFW halos
+MODELLED = {"bias", "fil", "ell"}   # ingredients this path computes on top of NFW halos
 DEFAULT_SLICES = 64
 
 
@@ -168,13 +169,13 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAUL
     sv = _solver(int(n_z))
     sv_shape = sv.shape
     if (("bias" in s) != sv_shape.clustering or ("fil" in s) != sv_shape.filaments
-            or edge != sv_shape.edge):
+            or edge != sv_shape.edge or ("ell" in s) != sv_shape.ell):
         from dataclasses import replace
         from .solver import Solver
-        key = ("exact", int(n_z), "bias" in s, "fil" in s, edge)
+        key = ("exact", int(n_z), "bias" in s, "fil" in s, edge, "ell" in s)
         if key not in _SOLVERS:
             _SOLVERS[key] = Solver(replace(sv_shape, clustering="bias" in s, filaments="fil" in s,
-                                           edge=edge), 1, path="exact")
+                                           edge=edge, ell="ell" in s), 1, path="exact")
         sv = _SOLVERS[key]
         sv_shape = sv.shape
     rec = cosmo_records(np.array([[cp["h"], cp["Om"], cp["s8"], cp["Ob"], cp["ns"], cp["zeq"],

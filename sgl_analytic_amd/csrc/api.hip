@@ -102,7 +102,7 @@ sgl_status sgl_tables_build(const double* cosmo, int32_t B, const sgl_shape* sha
   int rc = launch_fill_grids(sh, T + L.uedge, T + L.knodes, T + L.m_nodes, st);
   if (rc != SGL_OK) return (sgl_status)rc;
   rc = tables_build(T + L.cosmo, B, sh, T + L.m_nodes, T + L.derived, T + L.slices, T + L.mass,
-                    T + L.rows, T + L.frows, T + L.cov, status, st);
+                    T + L.rows, T + L.frows, T + L.cov, T + L.sigtab, status, st);
   return (sgl_status)rc;
 }
 

@@ -69,15 +69,20 @@ __device__ inline void stage_slice(SliceStage& S, const double* __restrict__ row
   }
 }
 
-// Cell c = (row j, column l) of the staged slice, FP64 throughout.
-__device__ __forceinline__ void gen_cell(const SliceStage& S, int n_b, int c, double& xi,
-                                         double& ws, double& bw) {
-  const int j = c / n_b, l = c - j * n_b;
+// Cell c = (row j, column l, orientation q) of the staged slice, FP64
+// throughout; n_phi = kEllPhi with halo ellipticity (each node 1/n_phi of the
+// cell's weight, the pseudo-elliptical profile), else 1.
+__device__ __forceinline__ void gen_cell(const SliceStage& S, int n_b, int n_phi, int c,
+                                         double& xi, double& ws, double& bw) {
+  const int per = n_b * n_phi;
+  const int j = c / per, r = c - j * per, l = r / n_phi, q = r - l * n_phi;
   const RowConst& rc = S.rc[j];
   if (rc.valid == 0.0) { xi = 0.0; ws = 0.0; bw = 0.0; return; }
   double kap, gam;
-  trunc_kg_row(S.ug[l] * rc.s, rc, kap, gam);
-  cell_jump(kap, gam, rc.warea * S.du2[l], xi, ws);
+  const double x = S.ug[l] * rc.s;
+  if (n_phi == 1) trunc_kg_row(x, rc, kap, gam);
+  else kg_eps_row(x, ell_cos2(q), rc, kap, gam);
+  cell_jump(kap, gam, rc.warea * S.du2[l] / n_phi, xi, ws);
   bw = rc.beta * ws;
 }
 
@@ -148,14 +153,15 @@ __global__ __launch_bounds__(256) void k_lnphi_exact(const double* __restrict__ 
     stage_slice(S, rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh);
     const bool fil = (sh.flags & SGL_FLAG_FILAMENTS) != 0;
     if (fil) stage_fil(FS, frows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kFilRowStride, sh);
-    const int n_halo = sh.n_m * sh.n_b;
+    const int n_phi = (sh.flags & SGL_FLAG_ELL) ? kEllPhi : 1;
+    const int n_halo = sh.n_m * sh.n_b * n_phi;
     const int n_cells = n_halo + (fil ? sh.n_m * kFilU * kFilPhi : 0);
     for (int c0 = 0; c0 < n_cells; c0 += kCellTile) {
       __syncthreads();
       {
         int c = c0 + t;
         double xi = 0.0, ws = 0.0, bw = 0.0;
-        if (c < n_halo) gen_cell(S, sh.n_b, c, xi, ws, bw);
+        if (c < n_halo) gen_cell(S, sh.n_b, n_phi, c, xi, ws, bw);
         else if (c < n_cells) gen_fil_cell(FS, c - n_halo, xi, ws, bw);
         sxi[t] = xi; sws[t] = ws; sbw[t] = bw;
       }
@@ -350,9 +356,10 @@ __global__ __launch_bounds__(256) void k_rowf(const double* __restrict__ rows,
 // masses at the same column, which keeps the x >= x_t and x < 1 branches
 // nearly uniform (x / x_t = u r_vir / r_t hardly depends on the mass) and
 // spreads the lanes over more histogram bins than consecutive columns do.
-template <int kBinThreads>
+template <int kBinThreads, bool kEll>
 __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restrict__ cosmo,
                                                            const double* __restrict__ uedge,
+                                                           const double* __restrict__ rows,
                                                            sgl_shape sh, double* __restrict__ hist) {
   __shared__ FineHist H;
   const int t = threadIdx.x, i = blockIdx.x, b = blockIdx.y;
@@ -375,6 +382,47 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
     cell_jump_f32log(kf, gf, rc.warea * cf.du2, xi, ws);
     if (ws != 0.0) deposit4(H, xi, ws, rc.beta * ws);
   };
+  // Halo ellipticity: orientation node q of a row is the row record with its
+  // x scaled to the elliptical radius, x_e = x f, f^2 = 1 - eps cos 2phi_q
+  // (and 1/kEllPhi of the weight); the pseudo-elliptical kappa, gamma of
+  // kg_eps_row follow from the circular profile at x_e.
+  auto cell_ell = [&](const RowF& rq, const ColF& cf, float eps, float c2, float f) {
+    const float xe = cf.ug * rq.s, rxe = cf.rug * rq.inv_s;
+    float k0, g0, kf, gf;
+    trunc_kg_row_f32(xe, cf.lug + rq.log_s, rxe, rq, k0, g0);
+    if (xe >= rq.xt) {                           // monopole shear at the circular x
+      const float rx = rxe * f;
+      kf = 0.0f;
+      gf = rq.mut4ks * rx * rx;
+    } else {
+      if (xe < 1e-4f) g0 = rq.ks;
+      const float cg = c2 * g0;
+      kf = k0 + eps * cg;
+      gf = sqrt_hw(fmaxf(g0 * g0 + 2.0f * eps * cg * k0 + eps * eps * (k0 * k0 - cg * cg), 0.0f));
+    }
+    double xi, ws;
+    cell_jump_f32log(kf, gf, rq.warea * cf.du2, xi, ws);
+    if (ws != 0.0) deposit4(H, xi, ws, rq.beta * ws);
+  };
+  // every cell of row j: cols(visit) walks this thread's columns
+  auto row_cells = [&](const RowF& rc, int j, auto&& cols) {
+    if constexpr (!kEll) {
+      cols([&](const ColF& cf) { cell(rc, cf); });
+    } else {
+      const float eps =
+          (float)rows[(((size_t)b * sh.n_z_max + i) * sh.n_m + j) * kRowStride + 6];
+      for (int q = 0; q < kEllPhi; ++q) {
+        const float cos2 = (float)ell_cos2(q), f2 = 1.0f - eps * cos2;
+        const float f = sqrt_hw(f2), c2 = (cos2 - eps) * rcp_hw(f2);
+        RowF rq = rc;
+        rq.s = rc.s * f;
+        rq.log_s = rc.log_s + 0.5f * log_hw(f2);
+        rq.inv_s = rc.inv_s * rsq_hw(f2);
+        rq.warea = rc.warea * (1.0 / kEllPhi);
+        cols([&](const ColF& cf) { cell_ell(rq, cf, eps, c2, f); });
+      }
+    }
+  };
   if (rpp % 64 == 0) {
     // a wave's lanes share the column phase: the column index is wave-uniform
     // and the column record comes through the scalar cache
@@ -382,18 +430,22 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
     for (int j = tj; j < sh.n_m; j += rpp) {
       const RowF rc = RF[j];
       if (rc.warea == 0.0) continue;
-      for (int l = wpart; l < n_b; l += parts) cell(rc, CF[l]);
+      row_cells(rc, j, [&](auto&& visit) {
+        for (int l = wpart; l < n_b; l += parts) visit(CF[l]);
+      });
     }
   } else if (part < parts) {
     for (int j = tj; j < sh.n_m; j += rpp) {
       const RowF rc = RF[j];
       if (rc.warea == 0.0) continue;
-      ColF nxt = CF[min(part, n_b - 1)];
-      for (int l = part; l < n_b; l += parts) {
-        const ColF cf = nxt;                     // the next column is fetched a cell ahead
-        nxt = CF[min(l + parts, n_b - 1)];
-        cell(rc, cf);
-      }
+      row_cells(rc, j, [&](auto&& visit) {
+        ColF nxt = CF[min(part, n_b - 1)];
+        for (int l = part; l < n_b; l += parts) {
+          const ColF cf = nxt;                   // the next column is fetched a cell ahead
+          nxt = CF[min(l + parts, n_b - 1)];
+          visit(cf);
+        }
+      });
     }
   }
   __syncthreads();
@@ -778,10 +830,20 @@ int launch_bin_pass(const double* rows, const double* frows, const double* cosmo
                      uedge, sh, B, hist);
   hipLaunchKernelGGL(k_cols, dim3((sh.n_b + 255) / 256), dim3(256), 0, st, uedge, sh, B, hist);
   const dim3 g(sh.n_z_max, B);
+  const bool ell = (sh.flags & SGL_FLAG_ELL) != 0;
   switch (g_sgl_tune[SGL_TUNE_BIN_THREADS]) {
-    case 0: hipLaunchKernelGGL(k_bin_cells<256>, g, dim3(256), 0, st, cosmo, uedge, sh, hist); break;
-    case 2: hipLaunchKernelGGL(k_bin_cells<1024>, g, dim3(1024), 0, st, cosmo, uedge, sh, hist); break;
-    default: hipLaunchKernelGGL(k_bin_cells<512>, g, dim3(512), 0, st, cosmo, uedge, sh, hist); break;
+    case 0:
+      if (ell) hipLaunchKernelGGL((k_bin_cells<256, true>), g, dim3(256), 0, st, cosmo, uedge, rows, sh, hist);
+      else hipLaunchKernelGGL((k_bin_cells<256, false>), g, dim3(256), 0, st, cosmo, uedge, rows, sh, hist);
+      break;
+    case 2:
+      if (ell) hipLaunchKernelGGL((k_bin_cells<1024, true>), g, dim3(1024), 0, st, cosmo, uedge, rows, sh, hist);
+      else hipLaunchKernelGGL((k_bin_cells<1024, false>), g, dim3(1024), 0, st, cosmo, uedge, rows, sh, hist);
+      break;
+    default:
+      if (ell) hipLaunchKernelGGL((k_bin_cells<512, true>), g, dim3(512), 0, st, cosmo, uedge, rows, sh, hist);
+      else hipLaunchKernelGGL((k_bin_cells<512, false>), g, dim3(512), 0, st, cosmo, uedge, rows, sh, hist);
+      break;
   }
   if (sh.flags & SGL_FLAG_FILAMENTS)
     hipLaunchKernelGGL(k_bin_fil, g, dim3(kFilThreads), 0, st, frows, cosmo, sh, hist);

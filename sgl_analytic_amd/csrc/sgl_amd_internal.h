@@ -22,7 +22,8 @@ constexpr int kSlZ = 0, kSlChi = 1, kSlDchi = 2, kSlWz = 3, kSlD = 4, kSlScr = 5
               kSlDg = 7;
 // mass record (4 doubles): M [Msun], sigma(M, 0), dln sigma / dln M
 constexpr int kMassStride = 4;
-// row record (8 doubles)
+// row record (8 doubles): k_s, r_s, x_t, r_vir, w_row, beta, eps (halo ellipticity,
+// 0 without SGL_FLAG_ELL), 0
 constexpr int kRowStride = 8;
 // Filament rows (k0base, r_s, w_row, beta) and the filament cell lattice:
 // kFilU u bins log-uniform from 1e-5 to 1 (first bin the disc u < 1e-5) x
@@ -34,8 +35,13 @@ constexpr int kRowKs = 0, kRowRs = 1, kRowXt = 2, kRowRvir = 3, kRowW = 4, kRowB
 
 // Offsets (in doubles) of the sub-arrays of the tables buffer.
 struct TablesLayout {
-  size_t cosmo, derived, slices, mass, rows, frows, cov, m_nodes, uedge, knodes, status, total;
+  size_t cosmo, derived, slices, mass, rows, frows, cov, sigtab, m_nodes, uedge, knodes, status,
+      total;
 };
+
+// Halo ellipticity (SGL_FLAG_ELL): the sigma(M) table of the M*(z) root,
+// M = logspace(0, 17, kMstarN) Msun (sgl_full.py:551-567).
+constexpr int kMstarN = 1200;
 
 // The shape-only grids come first, so their offsets do not depend on B: a
 // tables buffer built for any batch size serves every call that reads only
@@ -55,6 +61,7 @@ inline TablesLayout tables_layout(const sgl_shape& s, int B) {
   L.rows = take((size_t)B * s.n_z_max * s.n_m * kRowStride);
   L.frows = take((s.flags & SGL_FLAG_FILAMENTS) ? (size_t)B * s.n_z_max * s.n_m * kFilRowStride : 0);
   L.cov = take((size_t)B * s.n_z_max * s.n_z_max);
+  L.sigtab = take((s.flags & SGL_FLAG_ELL) ? (size_t)B * kMstarN : 0);
   L.total = o;
   return L;
 }
@@ -140,7 +147,7 @@ extern int g_sgl_tune[SGL_TUNE_COUNT];
 
 int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes_h,
                  double* derived, double* slices, double* mass, double* rows, double* frows,
-                 double* cov, int* status, hipStream_t st);
+                 double* cov, double* sigtab, int* status, hipStream_t st);
 int launch_fill_grids(const sgl_shape& sh, double* ue, double* kn, double* m_nodes_h,
                       hipStream_t st);
 int launch_lnphi_exact(const double* rows, const double* frows, const double* cosmo, const double* uedge,

@@ -286,7 +286,7 @@ __device__ inline void fil_ubin(int l, double& uc, double& du2) {
 
 // Per-row constants of a truncated NFW lens row (one (slice, mass) pair).
 struct RowConst {
-  double ks, s, xt, ic1, mut4ks, warea, beta, i0, jj, valid;
+  double ks, s, xt, ic1, mut4ks, warea, beta, i0, jj, valid, eps;
 };
 
 __device__ inline RowConst row_const(const double* R) {
@@ -304,8 +304,10 @@ __device__ inline RowConst row_const(const double* R) {
   c.jj = 0.5 * (u1 * u1 - xt * xt) - 3.0 * xt * (u1 - xt) + 3.0 * xt * xt * log(u1 / xt) +
          xt * xt * xt * (1.0 / u1 - 1.0 / xt);
   c.valid = (R[4] > 0.0) ? 1.0 : 0.0;
+  c.eps = R[6];
   return c;
 }
+
 
 // (kappa, gamma) of the truncated NFW row at x, the same closed forms as
 // trunc_kg with the shared subexpressions (arccosh(1/x), log x, 1/x, per-row
@@ -376,6 +378,42 @@ __device__ inline void trunc_kg_row(double x, const RowConst& rc, double& kap, d
   const double kb = 4.0 * h * rx2 - 4.0 * I2;
   kap = rc.ks * ku;
   gam = rc.ks * (kb - ku);
+}
+
+// Halo ellipticity: orientation nodes per cell (sgl_full.py:1535, 1659).
+constexpr int kEllPhi = 12;
+
+// cos(2 phi_q) of orientation node q, phi_q = (q + 1/2) (pi/2) / kEllPhi
+// (sgl_full.py:1659).
+static __constant__ double kEllCos2[kEllPhi] = {
+    0.9914448613738104, 0.9238795325112867, 0.7933533402912352, 0.6087614290087207,
+    0.38268343236508984, 0.1305261922200517, -0.1305261922200516, -0.3826834323650895,
+    -0.6087614290087207, -0.793353340291235, -0.9238795325112867, -0.9914448613738104};
+__device__ __forceinline__ double ell_cos2(int q) { return kEllCos2[q]; }
+
+// Pseudo-elliptical truncated NFW of a row (lensing.cpp:56 kappagammaNFWeps as
+// sgl_full.py:582-613 transcribes it): the circular profile at the elliptical
+// radius x_e = x sqrt(1 - eps cos 2phi), c2 = (cos 2phi - eps) / (1 - eps cos 2phi)
+// (= (x1^2 - x2^2) / x_e^2 of the reference's x1, x2),
+//   kappa = k0 + eps c2 g0,  gamma^2 = g0^2 + 2 eps c2 g0 k0 + eps^2 (k0^2 - (c2 g0)^2),
+// g0 = k_s below x_e = 1e-4; beyond the truncation kappa = 0 and gamma is the
+// monopole shear at the circular x.
+__device__ inline void kg_eps_row(double x, double cos2, const RowConst& rc, double& kap,
+                                  double& gam) {
+  const double eps = rc.eps, f2 = 1.0 - eps * cos2;
+  const double xe = fmax(x * sqrt(f2), 1e-12);
+  if (xe >= rc.xt) {
+    kap = 0.0;
+    gam = rc.mut4ks / (x * x);
+    return;
+  }
+  const double c2 = (cos2 - eps) / f2;
+  double k0, g0;
+  trunc_kg_row(xe, rc, k0, g0);
+  if (xe < 1e-4) g0 = rc.ks;
+  kap = k0 + eps * c2 * g0;
+  gam = sqrt(fmax(g0 * g0 + 2.0 * eps * c2 * g0 * k0 + eps * eps * (k0 * k0 - (c2 * g0) * (c2 * g0)),
+                  0.0));
 }
 
 // Per-row record of the batched pass (80 B, built once per row by k_rowf):

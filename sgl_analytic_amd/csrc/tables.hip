@@ -433,6 +433,86 @@ __global__ __launch_bounds__(256) void k_sigma(const Cosmo* __restrict__ cosmo,
   }
 }
 
+// sigma(M, 0) on M = logspace(0, 17, kMstarN) Msun for the M*(z) root of the
+// halo ellipticity (SGL_FLAG_ELL; sgl_full.py:551-567): the same smooth-k
+// window sums as k_sigma on the same 4000 k nodes.  (The reference tabulates
+// its 500-node sigma spline there and extrapolates it below 1e4 Msun; these
+// are the sums themselves, monotone everywhere.)  Runs after k_sigma (A_norm).
+__global__ __launch_bounds__(256) void k_mstar_table(const Cosmo* __restrict__ cosmo,
+                                                     const Derived* __restrict__ der,
+                                                     double* __restrict__ sigtab) {
+  __shared__ double g[kSigChunk];
+  __shared__ double k6[kSigChunk];
+  constexpr int kPer = (kMstarN + 255) / 256;
+  const int b = blockIdx.x, t = threadIdx.x;
+  const Cosmo c = cosmo[b];
+  const EH98Const e = eh98_const(c);
+  const double lstep = 8.0 / (kSigmaK - 1);
+  const double kLn10 = 2.302585092994046;
+  const double r = exp(kLn10 * lstep), ir = 1.0 / r;
+  const double rhom = c.Om * kRhoc0Unit * c.h * c.h;
+  const double a6 = pow(0.43, 6.0);
+  double aR[kPer], S[kPer];
+#pragma unroll
+  for (int m = 0; m < kPer; ++m) {
+    const int q = t + 256 * m;
+    const double M = exp(kLn10 * 17.0 * q / (kMstarN - 1));
+    const double R = cbrt(3.0 * M / (4.0 * kPi * rhom)), R2 = R * R;
+    aR[m] = a6 * R2 * R2 * R2;
+    S[m] = 0.0;
+  }
+  for (int i0 = 0; i0 < kSigmaK; i0 += kSigChunk) {
+    __syncthreads();
+    for (int q = t; q < kSigChunk; q += blockDim.x) {
+      const int i = i0 + q;
+      const double lk = (-5.0 + lstep * i) * kLn10;
+      const double k = exp(lk);
+      const double T = eh98_T(e, k, lk);
+      const double wl = (i == 0) ? 0.5 * k * (r - 1.0)
+                        : (i == kSigmaK - 1) ? 0.5 * k * (1.0 - ir) : 0.5 * k * (r - ir);
+      g[q] = k * k * exp(c.ns * lk) * T * T * wl / (2.0 * kPi * kPi);
+      const double k2 = k * k;
+      k6[q] = k2 * k2 * k2;
+    }
+    __syncthreads();
+    for (int q = 0; q < kSigChunk; ++q) {
+      const double gq = g[q], kq = k6[q];
+#pragma unroll
+      for (int m = 0; m < kPer; ++m) {
+        const double den = 1.0 + aR[m] * kq;
+        double Ws = __builtin_amdgcn_rcp(den);
+        Ws = Ws * (2.0 - den * Ws);
+        Ws = Ws * (2.0 - den * Ws);
+        S[m] += gq * Ws * Ws;
+      }
+    }
+  }
+  const double A = der[b].Anorm;
+#pragma unroll
+  for (int m = 0; m < kPer; ++m) {
+    const int q = t + 256 * m;
+    if (q < kMstarN) sigtab[(size_t)b * kMstarN + q] = A * sqrt(S[m]);
+  }
+}
+
+// ln M*(z) (sgl_full.py:551-567): sigma(M*, 0) Dg(z) = delta_c0 by linear
+// interpolation of ln M against sigma, clamped to the table's ends
+// (np.interp).  The reference first cuts its table to the largest strictly
+// decreasing suffix, against its spline's extrapolation; the sums here
+// decrease strictly over the whole table, so the cut keeps all of it.
+__device__ inline double ln_mstar(const double* __restrict__ tab, double target) {
+  constexpr double kLnStep = 17.0 * 2.302585092994046 / (kMstarN - 1);
+  if (target >= tab[0]) return 0.0;
+  if (target <= tab[kMstarN - 1]) return kLnStep * (kMstarN - 1);
+  int lo = 0, hi = kMstarN - 1;           // tab[lo] > target >= tab[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tab[mid] > target) lo = mid; else hi = mid;
+  }
+  const double f = (tab[lo] - target) / (tab[lo] - tab[hi]);
+  return kLnStep * (lo + f);
+}
+
 // ------------------------------------------------------------------ rows --
 // r_vir / r200 = eta solving m^3 / mu(m) = 200 c^3 / (mu(c) Delta_vir) on
 // [c, 3c] (subhalos.py:409-441): safeguarded Newton on g(m) = m^3 - T mu(m),
@@ -472,9 +552,10 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
                                               const double* __restrict__ slices,
                                               const double* __restrict__ mass, int n_z_max,
                                               int n_m, double dlnM, double a_st, double dc0,
-                                              int edge_rvir, double* __restrict__ rows,
+                                              int edge_rvir, const double* __restrict__ sigtab,
+                                              double* __restrict__ rows,
                                               double* __restrict__ frows) {
-  __shared__ double sk[14];
+  __shared__ double sk[15];
   // (sk[8] = rho_c0 for the filament rows)
   const int i = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
   const Cosmo c = cosmo[b];
@@ -513,6 +594,8 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
       sk[7] = d.rhom;
       sk[8] = d.rhoc0;
     }
+    // ln M*(z) for the halo ellipticity (sigtab is null without it)
+    if (t == 1) sk[14] = sigtab ? ln_mstar(sigtab + (size_t)b * kMstarN, dc0 / S[kSlDg]) : 0.0;
   }
   __syncthreads();
   // D: growth of sigma(M, z) in the mass function; Dg: growth in the bias and
@@ -521,6 +604,7 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
   const double c0 = sk[1], beta_ = sk[2], g1 = sk[3], g2 = sk[4];
   const double Dvir = sk[6], rhom = sk[7], rhoc0 = sk[8], cr = sk[9];
   const double lpoly = sk[10], lD = sk[11], r200f = sk[12], rhoc = sk[5], lDg = sk[13];
+  const double lmstar = sk[14];
   const double ldc = log(kDeltaC), lqst = log(kQst), l08dc0 = log(0.8) + 2.0 * log(dc0);
   for (int j = t; j < n_m; j += blockDim.x) {
     double* R = Rs + (size_t)j * kRowStride;
@@ -559,7 +643,14 @@ __global__ __launch_bounds__(256) void k_rows(const Cosmo* __restrict__ cosmo,
     R[kRowRvir] = rvir;
     R[kRowW] = S[kSlWz] * dndlnM * dlnM;
     R[kRowBeta] = Dg * bh;
-    R[6] = 0.0;
+    // halo ellipticity eps = max(0, (1 - s) / (1 + s)), s = 0.54 (M / M*)^-0.05
+    // (Allgood+06, sgl_full.py:570-579); 0 without it
+    double eps = 0.0;
+    if (sigtab) {
+      const double s = 0.54 * exp(-0.05 * (log(M) - lmstar));
+      eps = fmax(0.0, (1.0 - s) / (1.0 + s));
+    }
+    R[6] = eps;
     R[7] = 0.0;
     if (Fs) {
       // filament row (sgl_full.py:1984-2002): flat-barrier mass function
@@ -811,7 +902,7 @@ static SideStream* side_stream(hipStream_t st) {
 
 int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes_h,
                  double* derived, double* slices, double* mass, double* rows, double* frows,
-                 double* cov, int* status, hipStream_t st) {
+                 double* cov, double* sigtab, int* status, hipStream_t st) {
   const Cosmo* C = reinterpret_cast<const Cosmo*>(cosmo);
   Derived* D = reinterpret_cast<Derived*>(derived);
   const double dlnM = (log(sh.m_hi_h) - log(sh.m_lo_h)) / sh.n_m;
@@ -832,6 +923,8 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
   if (S) ck(hipEventRecord(S->j_growth, S->s));
   // sigma(M) needs the background only; it writes A_norm and the mass table
   hipLaunchKernelGGL(k_sigma, dim3(B), dim3(256), 0, st, C, m_nodes_h, sh.n_m, D, mass);
+  const bool ell = (sh.flags & SGL_FLAG_ELL) != 0;
+  if (ell) hipLaunchKernelGGL(k_mstar_table, dim3(B), dim3(256), 0, st, C, D, sigtab);
   // the covariance needs chi and A_norm (background, sigma), not the growth
   if (cov_on) {
     hipStream_t sc = st;
@@ -850,7 +943,8 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
   if (S) ck(hipStreamWaitEvent(st, S->j_growth, 0));
   const int rows_tpb = std::min(256, (sh.n_m + 63) / 64 * 64);
   hipLaunchKernelGGL(k_rows, dim3(sh.n_z_max, B), dim3(rows_tpb), 0, st, C, D, slices, mass,
-                     sh.n_z_max, sh.n_m, dlnM, a_st, dc0, (sh.flags & SGL_FLAG_EDGE_RVIR) ? 1 : 0, rows,
+                     sh.n_z_max, sh.n_m, dlnM, a_st, dc0, (sh.flags & SGL_FLAG_EDGE_RVIR) ? 1 : 0,
+                     ell ? (const double*)sigtab : (const double*)nullptr, rows,
                      (sh.flags & SGL_FLAG_FILAMENTS) ? frows : (double*)nullptr);
   if (S && cov_on) ck(hipStreamWaitEvent(st, S->j_cov, 0));
   if (fail) return SGL_ERR_HIP;

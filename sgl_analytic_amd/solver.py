@@ -20,8 +20,9 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (COSMO_STRIDE, FLAG_CLUSTERING, FLAG_EDGE_RVIR, FLAG_FILAMENTS, FLAG_WINDOW,
-                   PREC_BF16X3, PREC_FP32, PREC_FP64, REC_GROWTH, REC_RADIATION)
+from ._lib import (COSMO_STRIDE, FLAG_CLUSTERING, FLAG_EDGE_RVIR, FLAG_ELL, FLAG_FILAMENTS,
+                   FLAG_WINDOW, MSTAR_N, PREC_BF16X3, PREC_FP32, PREC_FP64, REC_GROWTH,
+                   REC_RADIATION)
 
 PARAM_NAMES = ("h", "Om", "sigma8", "Ob", "ns", "zeq", "Ok", "w0", "wa", "z_s")
 
@@ -32,7 +33,8 @@ class Shape:
     parameters, 8192 log-k nodes in [1e-2, 1e6], 2048 ln(mu) nodes on [-0.5, 3].
     ``filaments`` adds the cylinder-filament population (120 u bins x 13
     orientations per (slice, mass)).  ``edge`` is the halo truncation: "r200"
-    (x_t = c, the reference default) or "rvir" (x_t = r_vir / r_s)."""
+    (x_t = c, the reference default) or "rvir" (x_t = r_vir / r_s).  ``ell``
+    adds the halo ellipticity (12 orientation nodes per halo cell)."""
     n_z_max: int = 64
     n_m: int = 256
     n_b: int = 512
@@ -42,6 +44,7 @@ class Shape:
     window: bool = True
     filaments: bool = False
     edge: str = "r200"
+    ell: bool = False
     m_lo_h: float = 1e8
     m_hi_h: float = 1e16
     u_lo: float = 1e-4
@@ -56,7 +59,8 @@ class Shape:
             raise ValueError(f"edge must be 'r200' or 'rvir', not {self.edge!r}")
         flags = ((FLAG_CLUSTERING if self.clustering else 0) | (FLAG_WINDOW if self.window else 0)
                  | (FLAG_FILAMENTS if self.filaments else 0)
-                 | (FLAG_EDGE_RVIR if self.edge == "rvir" else 0))
+                 | (FLAG_EDGE_RVIR if self.edge == "rvir" else 0)
+                 | (FLAG_ELL if self.ell else 0))
         return _lib.CShape(self.n_z_max, self.n_m, self.n_b, self.n_k, self.n_out, flags,
                            self.m_lo_h, self.m_hi_h, self.u_lo, self.u_hi, self.k_lo, self.k_hi,
                            self.out_lo, self.out_hi)
@@ -228,7 +232,8 @@ class Solver:
                  ("cosmo", B * 16), ("derived", B * 16), ("slices", B * sh.n_z_max * 8),
                  ("mass", B * sh.n_m * 4), ("rows", B * sh.n_z_max * sh.n_m * 8),
                  ("frows", B * sh.n_z_max * sh.n_m * 4 if sh.filaments else 0),
-                 ("cov", B * sh.n_z_max * sh.n_z_max)]
+                 ("cov", B * sh.n_z_max * sh.n_z_max),
+                 ("sigtab", B * MSTAR_N if sh.ell else 0)]
         T = self.tables.view(torch.float64).cpu().numpy()
         out, o = {}, 0
         for name, n in sizes:

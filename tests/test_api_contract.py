@@ -50,8 +50,6 @@ def test_paths_not_computed_here_raise():
         with pytest.raises(NotImplementedError):
             api.run(1.0, "full", cosmo={}, xi_out=np.array([0.0, 1.0]), closure="vp")
         with pytest.raises(NotImplementedError):
-            api.run(1.0, "+ell", cosmo={}, xi_out=np.array([0.0, 1.0]))
-        with pytest.raises(NotImplementedError):
             api.run(1.0, "+sub", cosmo={}, xi_out=np.array([0.0, 1.0]))
         with pytest.raises(ValueError):     # vp needs the bias ingredient, as in the reference
             api.run(1.0, "-bias", cosmo={}, xi_out=np.array([0.0, 1.0]), closure="vp")

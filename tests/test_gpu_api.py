@@ -19,7 +19,7 @@ def test_generate_pdf_lnmu_runs_on_device_and_matches_solver():
     xo = sh.out_grid()
     x2, p2 = sa.generate_pdf_lnmu(1.0, lnmu=xo)
     from sgl_analytic_amd.api import _SOLVERS
-    sv = _SOLVERS[("exact", 64, True, True, "r200")]   # "full": clustering and filaments
+    sv = _SOLVERS[("exact", 64, True, True, "r200", True)]   # "full": clustering, filaments, ell
     np.testing.assert_allclose(p2, sv.p[0].cpu().numpy(), rtol=1e-12, atol=1e-14)
     mass = np.trapezoid(p, x)
     assert 0.99 < mass < 1.01

@@ -178,3 +178,62 @@ def test_edge_rvir_rows():
     assert np.all(eta >= 1.0 - 1e-15) and np.any(eta > 1.05)
     with pytest.raises(ValueError):
         chain.rows(cos, 2.0, chain.Shape(n_z=6, n_m=8, n_b=16, n_k=64, n_out=32, edge="r500"))
+
+
+def _psi_derivs(k0, g0, X1, X2, eps):
+    """Second derivatives of the pseudo-elliptical potential psi0(xi),
+    xi^2 = (1 - eps) X1^2 + (1 + eps) X2^2, from the circular profile at xi:
+    psi0'' = k0 - g0 and psi0'/xi = k0 + g0 (chain rule, no lensing formula)."""
+    a1, a2 = 1.0 - eps, 1.0 + eps
+    xi2 = a1 * X1 ** 2 + a2 * X2 ** 2
+    p2, p1x = k0 - g0, k0 + g0
+    u1, u2 = a1 * X1, a2 * X2
+    p11 = p2 * u1 ** 2 / xi2 + p1x * (a1 - u1 ** 2 / xi2)
+    p22 = p2 * u2 ** 2 / xi2 + p1x * (a2 - u2 ** 2 / xi2)
+    p12 = (p2 - p1x) * u1 * u2 / xi2
+    return p11, p22, p12
+
+
+def test_pseudo_elliptical_profile_pins():
+    """kg_eps (sgl_full.py:582-613): eps = 0 is the circular profile; kappa is
+    exactly half the Laplacian of the pseudo-elliptical potential; gamma^2
+    agrees with the potential's shear to first order in eps (the engine's
+    eps^2 term is its documented cos^2 form, so the difference is O(eps^2):
+    halving eps quarters it); beyond the truncation it is the monopole at x."""
+    ks, xt = 0.3, 6.0
+    x = np.geomspace(1e-3, 5.0, 41)
+    phi = np.linspace(0.03, 1.5, 41)
+    k, g = chain.kg_eps(x, phi, 0.0, ks, xt)
+    kc, gc = chain.trunc_kg(x, ks, xt)
+    np.testing.assert_allclose(k, kc, rtol=1e-14)
+    np.testing.assert_allclose(g, gc, rtol=1e-14)
+    diffs = []
+    for eps in (0.2, 0.1):
+        k, g = chain.kg_eps(x, phi, eps, ks, xt)
+        X1, X2 = x * np.cos(phi), x * np.sin(phi)
+        xe = np.sqrt((1 - eps) * X1 ** 2 + (1 + eps) * X2 ** 2)
+        k0, g0 = chain.trunc_kg(xe, ks, xt)
+        p11, p22, p12 = _psi_derivs(k0, g0, X1, X2, eps)
+        np.testing.assert_allclose(k, 0.5 * (p11 + p22), rtol=1e-12, atol=1e-15)
+        gex2 = 0.25 * (p11 - p22) ** 2 + p12 ** 2
+        diffs.append(np.max(np.abs(g ** 2 - gex2) / (k0 ** 2 + g0 ** 2)))
+    print("O(eps^2) shear differences", diffs)
+    assert diffs[0] < 0.1 and 3.0 < diffs[0] / diffs[1] < 5.0
+    xo = np.array([7.0, 9.0, 20.0])
+    k, g = chain.kg_eps(xo, 0.7, 0.25, ks, xt)
+    mut = np.log(1 + xt) - xt / (1 + xt)
+    np.testing.assert_allclose(k, 0.0)
+    np.testing.assert_allclose(g, 4 * ks * mut / xo ** 2, rtol=1e-14)
+
+
+def test_mstar_and_epsilon():
+    """M*(z) solves sigma(M*, 0) Dg(z) = delta_c0 on the reference's table, and
+    eps(M*) = (1 - 0.54) / (1 + 0.54) (Allgood+06)."""
+    from oracle.cosmo import DC0
+    c = Cosmology()
+    for z in (0.0, 0.5, 2.0):
+        m = c.Mstar(z)
+        assert abs(np.log(c.sigmaM(m, 0.0) * c.Dg(z) / DC0)) < 1e-5
+        assert abs(c.epsilon(m, z) - 0.46 / 1.54) < 1e-5
+    assert c.Mstar(0.0) > c.Mstar(2.0)
+    assert c.epsilon(1e6, 0.0) < c.epsilon(1e14, 0.0)
