@@ -371,8 +371,16 @@ class Result:
     info: dict = field(default_factory=dict)
 
 
-def pdf(zs: float, shape: Shape = None, cosmo: dict = None, keep: bool = False) -> Result:
-    """End-to-end FP64 oracle PDF on the output grid."""
+def vp_inputs(xi, ws, beta):
+    """Closure inputs of the cells [n_z, n_m, n_cells]: Y_sg(k) needs the cells,
+    wk_sg = sum w xi per (slice, mass row)."""
+    return np.sum(ws * xi, axis=2)
+
+
+def pdf(zs: float, shape: Shape = None, cosmo: dict = None, keep: bool = False,
+        closure: str = "gauss") -> Result:
+    """End-to-end FP64 oracle PDF on the output grid; closure "gauss" (Psi_2)
+    or "vp" (the positive closure of oracle/closure.py)."""
     shape = shape or Shape()
     cos = Cosmology(**(cosmo or {}))
     T = rows(cos, zs, shape)
@@ -386,7 +394,18 @@ def pdf(zs: float, shape: Shape = None, cosmo: dict = None, keep: bool = False) 
         lam_f, amp_f = lnphi(fxi, fws, F[:, :, 3], kq[1:])
         lam, amp = lam + lam_f, amp + amp_f
     C = slice_covariance(cos, T["chi_edges"], T["chis"])
-    psi2 = quadform(C, amp) if shape.clustering else np.zeros_like(lam)
+    if closure == "vp":
+        from .closure import VP
+        if shape.filaments:
+            raise ValueError("the vp closure here covers the halo population")
+        n_z, n_m = R.shape[:2]
+        vp = VP(R[:, :, 5], vp_inputs(xi, ws, R[:, :, 5]), C)
+        # Y_sg(k): every mass row as its own "slice" of the amplitude sum, beta = 1
+        _, Yr = lnphi(xi.reshape(n_z * n_m, 1, -1), ws.reshape(n_z * n_m, 1, -1),
+                      np.ones((n_z * n_m, 1)), kq[1:])
+        psi2 = vp.lnR(Yr.reshape(n_z, n_m, -1))
+    else:
+        psi2 = quadform(C, amp) if shape.clustering else np.zeros_like(lam)
     phi = np.concatenate([[1.0 + 0j], np.exp(lam + psi2)])
     if shape.window:
         phi = phi * window(kq)
