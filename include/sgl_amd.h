@@ -134,6 +134,34 @@ sgl_status sgl_lnphi_exact(const void* tables, int32_t B, const sgl_shape* shape
                            double* lam, double* amp, double* work, void* stream);
 
 /*
+ * Exact path with the positive ("vp") clustering closure in place of Psi_2:
+ *   lam[b][n] as sgl_lnphi_exact, and
+ *   lnr[b][n] = ln E exp( sum_s sum_g (lambda_sg(U_s) - 1) Y_sg(k_n) ),
+ *   Y_sg(k) = sum_{c in slice s, group g} w_c (e^{i k xi_c} - 1),
+ * interleaved (re, im) doubles; pass lnr as the psi2 argument of
+ * sgl_phi_pack.  Groups g: the halo mass rows of a slice, then (with
+ * SGL_FLAG_FILAMENTS) its filament rows.  U_s: a stationary chain on 48
+ * Gauss-Hermite nodes with correlation rho_s = C_{s,s+1}/sqrt(C_ss C_{s+1,s+1})
+ * between neighbouring slices; lambda_g: rectified rates with E lambda = 1 and
+ * Var lambda = eta f_s (beta_g^2 C_ss), f_s = (1 - rbar_s)/(1 + rbar_s); eta
+ * fitted so that the chain variance of the k^2 term equals A^T C A,
+ * A_s = sum_g beta_g sum_{c in g} w_c xi_c.  Needs SGL_FLAG_CLUSTERING (the
+ * slice covariance C of the tables).
+ * work: as sgl_lnphi_exact; vpwork: sgl_vp_bytes(shape, B) bytes, its first
+ * B doubles hold eta after the call, the next B the target A^T C A; status: OR-ed with 4 (a transition
+ * balancing that did not converge) or 8 (a rate variance beyond the 48-node
+ * cap, or no bracket for eta).
+ * Replaces: pos_closure.install / Closure (calibrate, population)
+ * (sgl_analytic/pos_closure.py:453-591, 708-733), the transition and rates
+ * (pos_closure.py:251-308), the fused group sums and the chain recursion
+ * (pos_closure_fused.py:51-140, pos_closure.py:183-240).
+ */
+size_t sgl_vp_bytes(const sgl_shape* shape, int32_t B);
+sgl_status sgl_lnphi_exact_vp(const void* tables, int32_t B, const sgl_shape* shape, double* lam,
+                              double* lnr, double* work, void* vpwork, int32_t* status,
+                              void* stream);
+
+/*
  * Convergence and shear kap[i], gam[i] of an NFW profile of amplitude ks
  * truncated at x_t, at n points x[i] (device arrays, x = R / r_s > 0):
  * SGL_PREC_FP64, the exact path's

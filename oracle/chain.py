@@ -396,14 +396,22 @@ def pdf(zs: float, shape: Shape = None, cosmo: dict = None, keep: bool = False,
     C = slice_covariance(cos, T["chi_edges"], T["chis"])
     if closure == "vp":
         from .closure import VP
+        # groups: the halo mass rows, then (with filaments) the filament rows
+        parts = [(xi, ws, R[:, :, 5])]
         if shape.filaments:
-            raise ValueError("the vp closure here covers the halo population")
-        n_z, n_m = R.shape[:2]
-        vp = VP(R[:, :, 5], vp_inputs(xi, ws, R[:, :, 5]), C)
-        # Y_sg(k): every mass row as its own "slice" of the amplitude sum, beta = 1
-        _, Yr = lnphi(xi.reshape(n_z * n_m, 1, -1), ws.reshape(n_z * n_m, 1, -1),
-                      np.ones((n_z * n_m, 1)), kq[1:])
-        psi2 = vp.lnR(Yr.reshape(n_z, n_m, -1))
+            parts.append((fxi, fws, F[:, :, 3]))
+        n_z = R.shape[0]
+        beta = np.concatenate([p[2] for p in parts], axis=1)
+        wk = np.concatenate([vp_inputs(*p) for p in parts], axis=1)
+        # Y_sg(k): every row as its own "slice" of the amplitude sum, beta = 1
+        Y = []
+        for x_, w_, b_ in parts:
+            n_g = b_.shape[1]
+            _, Yr = lnphi(x_.reshape(n_z * n_g, 1, -1), w_.reshape(n_z * n_g, 1, -1),
+                          np.ones((n_z * n_g, 1)), kq[1:])
+            Y.append(Yr.reshape(n_z, n_g, -1))
+        vp = VP(beta, wk, C)
+        psi2 = vp.lnR(np.concatenate(Y, axis=1))
     else:
         psi2 = quadform(C, amp) if shape.clustering else np.zeros_like(lam)
     phi = np.concatenate([[1.0 + 0j], np.exp(lam + psi2)])
@@ -417,6 +425,8 @@ def pdf(zs: float, shape: Shape = None, cosmo: dict = None, keep: bool = False,
     if shape.filaments:
         info.update(n_cells_fil=int(np.count_nonzero(fws)), sum_w_fil=float(fws.sum()),
                     var_xi_fil=float(np.sum(fws * fxi ** 2)))
+    if closure == "vp":
+        info.update(eta=vp.eta, vp_target=vp.target)
     if keep:
         info.update(cos=cos, tables=T, xi=xi, ws=ws, kap=kap, gam=gam, lam=lam, amp=amp, C=C,
                     psi2=psi2, phi=phi, kq=kq)

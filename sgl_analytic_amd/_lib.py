@@ -17,13 +17,15 @@ ELL_PHI, MSTAR_N = 12, 1200   # orientation nodes per elliptical cell; M* sigma 
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 COSMO_STRIDE = 16
 REC_RADIATION, REC_GROWTH = 11, 12  # SGL_REC_*: record slots of the radiation and growth options
+STATUS_VP_TRANSITION = 4   # vp closure: transition balancing not converged
+STATUS_VP_RATE = 8         # vp closure: rate variance beyond the 48-node cap, or no eta bracket
 
 EXPORTS = (
     "sgl_version", "sgl_tables_bytes", "sgl_hist_bytes", "sgl_filon_bytes", "sgl_phi_bytes",
     "sgl_tables_build", "sgl_lnphi_exact", "sgl_bin_cells", "sgl_lnphi_binned",
     "sgl_quadform", "sgl_phi_pack", "sgl_filon_build", "sgl_invert_gemm", "sgl_invert_fp64",
     "sgl_diagnostics", "sgl_invert_fp64_at", "sgl_mlp_gemm", "sgl_mlp_colsum", "sgl_to_bf16",
-    "sgl_lnphi_interp", "sgl_set_tuning", "sgl_get_tuning",
+    "sgl_lnphi_interp", "sgl_set_tuning", "sgl_get_tuning", "sgl_vp_bytes", "sgl_lnphi_exact_vp",
     "sgl_bin_pass", "sgl_bin_reduce", "sgl_psi2_nodes", "sgl_psi2_interp",
     "sgl_trunc_nfw_kg",
 )
@@ -65,6 +67,8 @@ def load(path: str = LIB_PATH):
         f = getattr(L, name)
         f.restype = sz
         f.argtypes = [sp, i32] + ([i32] if name == "sgl_phi_bytes" else [])
+    L.sgl_vp_bytes.restype = sz
+    L.sgl_vp_bytes.argtypes = [sp, i32]
     L.sgl_filon_bytes.restype = sz
     L.sgl_filon_bytes.argtypes = [sp, i32]
     L.sgl_get_tuning.restype = i32
@@ -74,6 +78,7 @@ def load(path: str = LIB_PATH):
     sigs = {
         "sgl_tables_build": [vp, i32, sp, vp, vp, vp],
         "sgl_lnphi_exact": [vp, i32, sp, vp, vp, vp, vp],
+        "sgl_lnphi_exact_vp": [vp, i32, sp, vp, vp, vp, vp, vp, vp],
         "sgl_bin_cells": [vp, i32, sp, vp, vp],
         "sgl_bin_pass": [vp, i32, sp, vp, vp],
         "sgl_bin_reduce": [vp, i32, sp, vp, vp],

@@ -159,23 +159,23 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAUL
         raise ValueError(f"xi_closure must be 'gauss' or 'vp', got {closure!r}")
     s = on(config)
     cp = check_cosmology(cosmo)
-    if closure != "gauss":
-        if "bias" not in s:
-            raise ValueError(f"xi_closure={closure!r} needs xilin clustering and the bias ingredient")
-        raise NotImplementedError("the 'vp' positive closure is not part of this path")
+    if closure != "gauss" and "bias" not in s:
+        raise ValueError(f"xi_closure={closure!r} needs xilin clustering and the bias ingredient")
     explicit = s - MODELLED
     if explicit and config != "full":
         raise NotImplementedError(f"ingredient(s) {sorted(explicit)} are not part of this path")
     sv = _solver(int(n_z))
     sv_shape = sv.shape
     if (("bias" in s) != sv_shape.clustering or ("fil" in s) != sv_shape.filaments
-            or edge != sv_shape.edge or ("ell" in s) != sv_shape.ell):
+            or edge != sv_shape.edge or ("ell" in s) != sv_shape.ell or closure != "gauss"):
         from dataclasses import replace
         from .solver import Solver
         key = ("exact", int(n_z), "bias" in s, "fil" in s, edge, "ell" in s)
+        key += (closure,) if closure != "gauss" else ()
         if key not in _SOLVERS:
             _SOLVERS[key] = Solver(replace(sv_shape, clustering="bias" in s, filaments="fil" in s,
-                                           edge=edge, ell="ell" in s), 1, path="exact")
+                                           edge=edge, ell="ell" in s), 1, path="exact",
+                                   closure=closure)
         sv = _SOLVERS[key]
         sv_shape = sv.shape
     rec = cosmo_records(np.array([[cp["h"], cp["Om"], cp["s8"], cp["Ob"], cp["ns"], cp["zeq"],
@@ -194,8 +194,18 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAUL
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     d = sv.diag[0].cpu().numpy()
+    st = int(sv.status.cpu()[0])
+    if st & (_lib.STATUS_VP_TRANSITION | _lib.STATUS_VP_RATE):
+        raise _lib.SglError(f"vp closure set-up failed (status {st:#x}): the transition balancing "
+                            "did not converge or the rate variance left the 48-node range")
     info = reference_schema(cp, sv_shape, d, config, closure, edge, t_solve=t1 - t0,
                             t_invert=t2 - t1)
+    if closure == "vp":
+        eta, target = sv.vp_calibration()
+        from .solver import vp_groups
+        info["pos_closure"] = dict(kind="vp", eta=float(eta[0]), nstate=48, shells=int(n_z),
+                                   eval_kernel="hip_chain", groups=int(n_z) * vp_groups(sv_shape),
+                                   target_kappa=float(target[0]), target_gamma=None)
     info.update(zs=float(z_s), cfg=config, ingredients=["halo"] + sorted(s & MODELLED),
                 clustering="slice-covariance" if "bias" in s else None, closure=closure,
                 edge=edge,

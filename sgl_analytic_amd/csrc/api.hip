@@ -117,6 +117,35 @@ sgl_status sgl_lnphi_exact(const void* tables, int32_t B, const sgl_shape* shape
                                         as_stream(stream));
 }
 
+size_t sgl_vp_bytes(const sgl_shape* shape, int32_t B) {
+  if (!shape_ok(shape) || !batch_ok(B)) return 0;
+  return vp_layout(*shape, B).total * sizeof(double);
+}
+
+sgl_status sgl_lnphi_exact_vp(const void* tables, int32_t B, const sgl_shape* shape, double* lam,
+                              double* lnr, double* work, void* vpwork, int32_t* status,
+                              void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !tables || !lam || !lnr || !work || !vpwork || !status)
+    return SGL_ERR_INVALID;
+  if (!(shape->flags & SGL_FLAG_CLUSTERING)) return SGL_ERR_INVALID;
+  const sgl_shape& sh = *shape;
+  hipStream_t st = as_stream(stream);
+  TablesLayout L = tables_layout(sh, B);
+  VpLayout V = vp_layout(sh, B);
+  const double* T = static_cast<const double*>(tables);
+  double* W = static_cast<double*>(vpwork);
+  int rc = launch_vp_cells(T + L.rows, T + L.frows, T + L.cosmo, T + L.uedge, T + L.knodes, sh, B,
+                           V, W, work, 0, st);
+  if (rc == SGL_OK)
+    rc = launch_vp_setup(T + L.rows, T + L.frows, T + L.cov, T + L.cosmo, sh, B, V, W, status, st);
+  if (rc == SGL_OK)
+    rc = launch_vp_cells(T + L.rows, T + L.frows, T + L.cosmo, T + L.uedge, T + L.knodes, sh, B,
+                         V, W, work, 1, st);
+  if (rc == SGL_OK) rc = launch_sum_slices(work, sh, B, lam, st);
+  if (rc == SGL_OK) rc = launch_vp_chain(T + L.cosmo, sh, B, V, W, lnr, st);
+  return (sgl_status)rc;
+}
+
 sgl_status sgl_bin_cells(const void* tables, int32_t B, const sgl_shape* shape, void* hist,
                          void* stream) {
   if (!shape_ok(shape) || !batch_ok(B) || !tables || !hist) return SGL_ERR_INVALID;

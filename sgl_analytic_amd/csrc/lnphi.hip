@@ -206,6 +206,203 @@ __global__ __launch_bounds__(256) void k_lnphi_exact(const double* __restrict__ 
   }
 }
 
+// ---------------------------------------------- positive closure cells --
+// Cells of group g of the staged slice: halo mass row g (n_b x n_phi cells)
+// for g < n_m, else filament row g - n_m (kFilU x kFilPhi cells).
+struct GroupCells {
+  int first, count;
+  bool fil;
+};
+__device__ __forceinline__ GroupCells group_cells(const sgl_shape& sh, int n_phi, int g) {
+  if (g < sh.n_m) return {g * sh.n_b * n_phi, sh.n_b * n_phi, false};
+  return {(g - sh.n_m) * kFilU * kFilPhi, kFilU * kFilPhi, true};
+}
+
+// wk[b][s][g] = sum over the cells of group g of w xi (the k^2 coefficient
+// of the group's jump sum): one wave per group, lanes over the cells.
+__global__ __launch_bounds__(256) void k_vp_wk(const double* __restrict__ rows,
+                                               const double* __restrict__ frows,
+                                               const double* __restrict__ cosmo,
+                                               const double* __restrict__ uedge, sgl_shape sh,
+                                               double* __restrict__ wk) {
+  __shared__ SliceStage S;
+  __shared__ FilStage FS;
+  const int i = blockIdx.x, b = blockIdx.y;
+  const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
+  if (i >= n_z) return;
+  const bool fil = (sh.flags & SGL_FLAG_FILAMENTS) != 0;
+  stage_slice(S, rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh);
+  if (fil) stage_fil(FS, frows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kFilRowStride, sh);
+  __syncthreads();
+  const int n_phi = (sh.flags & SGL_FLAG_ELL) ? kEllPhi : 1;
+  const int G = vp_groups(sh);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int g = wave; g < G; g += 4) {
+    const GroupCells gc = group_cells(sh, n_phi, g);
+    double acc = 0.0;
+    for (int l = lane; l < gc.count; l += 64) {
+      double xi, ws, bw;
+      if (gc.fil) gen_fil_cell(FS, gc.first + l, xi, ws, bw);
+      else gen_cell(S, sh.n_b, n_phi, gc.first + l, xi, ws, bw);
+      acc = fma(ws, xi, acc);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) wk[((size_t)b * sh.n_z_max + i) * G + g] = acc;
+  }
+}
+
+// Exact exponent with the positive closure's group sums.  As k_lnphi_exact
+// for lam (per-slice partial sums into part), but the cells run group by
+// group in the order of the groups' rate buckets, so that after the groups
+// with bucket <= j the running sums
+//   P1 = sum minv_g Y_g,  P0 = sum tm_g Y_g,  Y_g(k) = sum_{c in g} w_c (e^{i k xi_c} - 1)
+// give the rate-weighted sum at latent node j as u_j P1 - P0 (rates
+// lambda_g(u_j) = u_j minv_g - tm_g for j >= bucket_g, 0 below); written as
+// zpart[j], with ytot = sum_g Y_g, so that sum_g (lambda_g(u_j) - 1) Y_g =
+// zpart[j] - ytot.
+__global__ __launch_bounds__(256) void k_lnphi_exact_vp(const double* __restrict__ rows,
+                                                        const double* __restrict__ frows,
+                                                        const double* __restrict__ cosmo,
+                                                        const double* __restrict__ uedge,
+                                                        const double* __restrict__ knodes,
+                                                        sgl_shape sh, VpLayout V,
+                                                        double* __restrict__ vp,
+                                                        double* __restrict__ part) {
+  __shared__ SliceStage S;
+  __shared__ FilStage FS;
+  __shared__ double sxi[kCellTile], sws[kCellTile];
+  __shared__ double gmv[2 * kMaxMasses], gtm[2 * kMaxMasses];
+  __shared__ int gbk[2 * kMaxMasses], order[2 * kMaxMasses];
+  const int t = threadIdx.x;
+  const int i = blockIdx.y, b = blockIdx.z;
+  const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
+  const int n_k = sh.n_k;
+  const int G = vp_groups(sh);
+  const int kbase = blockIdx.x * (256 * kKpt);
+  double kv[kKpt], lr[kKpt], li[kKpt];
+#pragma unroll
+  for (int q = 0; q < kKpt; ++q) {
+    int n = kbase + t + 256 * q;
+    kv[q] = (n < n_k) ? knodes[n] : 0.0;
+    lr[q] = li[q] = 0.0;
+  }
+  const size_t si = (size_t)b * sh.n_z_max + i;
+  if (i < n_z) {
+    const bool fil = (sh.flags & SGL_FLAG_FILAMENTS) != 0;
+    stage_slice(S, rows + si * sh.n_m * kRowStride, uedge, sh);
+    if (fil) stage_fil(FS, frows + si * sh.n_m * kFilRowStride, sh);
+    const double* tab = vp + V.tab + si * G * 3;
+    for (int g = t; g < G; g += 256) {
+      gmv[g] = tab[3 * g];
+      gtm[g] = tab[3 * g + 1];
+      gbk[g] = (int)tab[3 * g + 2];
+    }
+    __syncthreads();
+    // stable order by bucket: rank of g among (bucket, g)
+    for (int g = t; g < G; g += 256) {
+      const int bg = gbk[g];
+      int r = 0;
+      for (int h = 0; h < G; ++h) r += (gbk[h] < bg) || (gbk[h] == bg && h < g);
+      order[r] = g;
+    }
+    const int n_phi = (sh.flags & SGL_FLAG_ELL) ? kEllPhi : 1;
+    double p1r[kKpt], p1i[kKpt], p0r[kKpt], p0i[kKpt], ytr[kKpt], yti[kKpt];
+#pragma unroll
+    for (int q = 0; q < kKpt; ++q) p1r[q] = p1i[q] = p0r[q] = p0i[q] = ytr[q] = yti[q] = 0.0;
+    double* zp = vp + V.zpart + si * kVpN * (size_t)n_k * 2;
+    auto flush = [&](int j) {
+#pragma unroll
+      for (int q = 0; q < kKpt; ++q) {
+        const int n = kbase + t + 256 * q;
+        if (n < n_k) {
+          const double u = kGhU[j];
+          zp[2 * ((size_t)j * n_k + n)] = u * p1r[q] - p0r[q];
+          zp[2 * ((size_t)j * n_k + n) + 1] = u * p1i[q] - p0i[q];
+        }
+      }
+    };
+    int jw = 0;
+    __syncthreads();
+    for (int gi = 0; gi < G; ++gi) {
+      const int g = order[gi];
+      for (; jw < gbk[g] && jw < kVpN; ++jw) flush(jw);
+      const GroupCells gc = group_cells(sh, n_phi, g);
+      double yr[kKpt], yi[kKpt];
+#pragma unroll
+      for (int q = 0; q < kKpt; ++q) yr[q] = yi[q] = 0.0;
+      for (int c0 = 0; c0 < gc.count; c0 += kCellTile) {
+        __syncthreads();
+        {
+          const int c = c0 + t;
+          double xi = 0.0, ws = 0.0, bw;
+          if (c < gc.count) {
+            if (gc.fil) gen_fil_cell(FS, gc.first + c, xi, ws, bw);
+            else gen_cell(S, sh.n_b, n_phi, gc.first + c, xi, ws, bw);
+          }
+          sxi[t] = xi;
+          sws[t] = ws;
+        }
+        __syncthreads();
+        const int nc = min(kCellTile, gc.count - c0);
+        for (int c = 0; c < nc; ++c) {
+          const double w = sws[c];
+          if (w == 0.0) continue;
+          const double xi = sxi[c];
+#pragma unroll
+          for (int q = 0; q < kKpt; ++q) {
+            double ph = kv[q] * xi;
+            double hs, hc;
+            sincos(0.5 * ph, &hs, &hc);
+            double sn = 2.0 * hs * hc;
+            double cm1 = -2.0 * hs * hs;
+            double smp;
+            if (fabs(ph) < 1e-2) {
+              double p2 = ph * ph;
+              smp = -ph * p2 / 6.0 * (1.0 - p2 / 20.0 * (1.0 - p2 / 42.0));
+            } else {
+              smp = sn - ph;
+            }
+            lr[q] = fma(w, cm1, lr[q]);
+            li[q] = fma(w, smp, li[q]);
+            yr[q] = fma(w, cm1, yr[q]);
+            yi[q] = fma(w, sn, yi[q]);
+          }
+        }
+      }
+      const double mv = gmv[g], tm = gtm[g];
+#pragma unroll
+      for (int q = 0; q < kKpt; ++q) {
+        p1r[q] = fma(mv, yr[q], p1r[q]);
+        p1i[q] = fma(mv, yi[q], p1i[q]);
+        p0r[q] = fma(tm, yr[q], p0r[q]);
+        p0i[q] = fma(tm, yi[q], p0i[q]);
+        ytr[q] += yr[q];
+        yti[q] += yi[q];
+      }
+    }
+    for (; jw < kVpN; ++jw) flush(jw);
+    double* yt = vp + V.ytot + si * (size_t)n_k * 2;
+#pragma unroll
+    for (int q = 0; q < kKpt; ++q) {
+      const int n = kbase + t + 256 * q;
+      if (n < n_k) {
+        yt[2 * n] = ytr[q];
+        yt[2 * n + 1] = yti[q];
+      }
+    }
+  }
+  const size_t base = si * n_k;
+#pragma unroll
+  for (int q = 0; q < kKpt; ++q) {
+    int n = kbase + t + 256 * q;
+    if (n < n_k) {
+      part[2 * (base + n)] = lr[q];
+      part[2 * (base + n) + 1] = li[q];
+    }
+  }
+}
+
 // lam[b][n] = sum over slices of part[b][i][n], in slice order.
 __global__ void k_sum_slices(const double* __restrict__ part, int B, int n_z_max, int n_k,
                              double* __restrict__ lam) {
@@ -819,6 +1016,29 @@ int launch_lnphi_exact(const double* rows, const double* frows, const double* co
   size_t n = (size_t)B * sh.n_k;
   hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, work, B,
                      sh.n_z_max, sh.n_k, lam);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+int launch_sum_slices(const double* work, const sgl_shape& sh, int B, double* lam, hipStream_t st) {
+  size_t n = (size_t)B * sh.n_k;
+  hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, work, B,
+                     sh.n_z_max, sh.n_k, lam);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+// pass 0: group sums wk; pass 1: exponent partials (work) and the closure's
+// zpart / ytot (needs the rate table of sgl_vp setup).
+int launch_vp_cells(const double* rows, const double* frows, const double* cosmo,
+                    const double* uedge, const double* knodes, const sgl_shape& sh, int B,
+                    const VpLayout& V, double* vp, double* work, int pass, hipStream_t st) {
+  if (pass == 0) {
+    hipLaunchKernelGGL(k_vp_wk, dim3(sh.n_z_max, B), dim3(256), 0, st, rows, frows, cosmo, uedge,
+                       sh, vp + V.wk);
+  } else {
+    dim3 grid((sh.n_k + 256 * kKpt - 1) / (256 * kKpt), sh.n_z_max, B);
+    hipLaunchKernelGGL(k_lnphi_exact_vp, grid, dim3(256), 0, st, rows, frows, cosmo, uedge,
+                       knodes, sh, V, vp, work);
+  }
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 

@@ -13,6 +13,8 @@ constexpr int kMaxSlices = 96;
 constexpr int kCovMaxModes = 2048;
 constexpr int kStatusCovModes = 1;      // covariance mode count clamped
 constexpr int kStatusBadShape = 2;
+constexpr int kStatusVpTransition = 4;  // positive closure: transition balancing not converged
+constexpr int kStatusVpRate = 8;        // positive closure: a rate variance beyond the node cap
 
 // slice record (8 doubles): z, chi, dchi, dchi (1+z)^2, D(z) (the growth of
 // sigma(M, z)), Sigma_cr, E^2, Dg(z) (the growth in the bias; D(z) unless the
@@ -62,6 +64,39 @@ inline TablesLayout tables_layout(const sgl_shape& s, int B) {
   L.frows = take((s.flags & SGL_FLAG_FILAMENTS) ? (size_t)B * s.n_z_max * s.n_m * kFilRowStride : 0);
   L.cov = take((size_t)B * s.n_z_max * s.n_z_max);
   L.sigtab = take((s.flags & SGL_FLAG_ELL) ? (size_t)B * kMstarN : 0);
+  L.total = o;
+  return L;
+}
+
+// Positive ("vp") closure workspace (doubles), per instance b and slice s:
+//   eta[2B]                      calibrated variance scale eta[b], then the
+//                                calibration target A^T C A at eta[B + b]
+//   wk[B][NZ][G]                 sum over a group's cells of w xi
+//   K[B][NZ][kVpN][kVpN]         transition slice s -> s + 1 (row = from-state)
+//   tab[B][NZ][G][3]             rate of group g at the latent nodes u_j >= node
+//                                bucket: lambda(u) = u minv - tm, (minv, tm, bucket)
+//   ytot[B][NZ][n_k][2]          sum over the slice's groups of Y_g(k)
+//   zpart[B][NZ][kVpN][n_k][2]   u_j P1_j - P0_j (the rate-weighted group sums)
+// G = n_m groups (the halo mass rows), 2 n_m with filaments (then the
+// filament rows).
+constexpr int kVpN = 48;
+__host__ __device__ inline int vp_groups(const sgl_shape& s) {
+  return s.n_m * ((s.flags & SGL_FLAG_FILAMENTS) ? 2 : 1);
+}
+struct VpLayout {
+  size_t eta, wk, K, tab, ytot, zpart, total;
+};
+inline VpLayout vp_layout(const sgl_shape& s, int B) {
+  VpLayout L;
+  size_t o = 0;
+  auto take = [&](size_t n) { size_t r = o; o += (n + 31) & ~size_t(31); return r; };
+  const size_t nz = (size_t)B * s.n_z_max, G = (size_t)vp_groups(s);
+  L.eta = take(2 * (size_t)B);
+  L.wk = take(nz * G);
+  L.K = take(nz * kVpN * kVpN);
+  L.tab = take(nz * G * 3);
+  L.ytot = take(nz * s.n_k * 2);
+  L.zpart = take(nz * kVpN * s.n_k * 2);
   L.total = o;
   return L;
 }
@@ -163,6 +198,15 @@ int launch_lnphi_binned(const double* hist, const double* cosmo, const double* k
                         const sgl_shape& sh, int B, double* lam, float* amp, hipStream_t st);
 int launch_quadform(const double* cov, const double* cosmo, const void* amp, int B,
                     const sgl_shape& sh, int prec, double* psi2, hipStream_t st);
+int launch_vp_cells(const double* rows, const double* frows, const double* cosmo,
+                    const double* uedge, const double* knodes, const sgl_shape& sh, int B,
+                    const VpLayout& V, double* vp, double* work, int pass, hipStream_t st);
+int launch_vp_setup(const double* rows, const double* frows, const double* cov,
+                    const double* cosmo, const sgl_shape& sh, int B, const VpLayout& V, double* vp,
+                    int* status, hipStream_t st);
+int launch_vp_chain(const double* cosmo, const sgl_shape& sh, int B, const VpLayout& V,
+                    const double* vp, double* lnr, hipStream_t st);
+int launch_sum_slices(const double* work, const sgl_shape& sh, int B, double* lam, hipStream_t st);
 int launch_trunc_nfw(const double* x, int n, double ks, double xt, int prec, double* kap,
                      double* gam, hipStream_t st);
 int launch_psi2_interp(const double* psi2c, const sgl_shape& sh, int B, double* psi2,

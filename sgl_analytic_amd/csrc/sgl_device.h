@@ -285,6 +285,36 @@ __device__ inline void fil_ubin(int l, double& uc, double& du2) {
 }
 
 // Per-row constants of a truncated NFW lens row (one (slice, mass) pair).
+// Probabilists' Gauss-Hermite rule with 48 nodes (numpy.polynomial.hermite_e.
+// hermegauss(48)), weights normalised to sum 1: the latent states of the
+// positive clustering closure.
+static __constant__ double kGhU[48] = {
+-12.693012315439578, -11.753178461616617, -10.973300958513551, -10.275741581733518,
+    -9.63088568694906, -9.023480280417651, -8.444371322523448, -7.887517316500365,
+    -7.348660565908913, -6.824651320745979, -6.313069914905499, -5.811999987702531,
+    -5.319884620419086, -4.8354308858789565, -4.357544108790291, -3.8852811172483914,
+    -3.4178160486847764, -2.9544146888801195, -2.494414743040023, -2.037210303259033,
+    -1.582239319674964, -1.1289732315096481, -0.6769081422063474, -0.22555707298016336,
+    0.22555707298016336, 0.6769081422063474, 1.1289732315096481, 1.582239319674964,
+    2.037210303259033, 2.494414743040023, 2.9544146888801195, 3.4178160486847764,
+    3.8852811172483914, 4.357544108790291, 4.8354308858789565, 5.319884620419086,
+    5.811999987702531, 6.313069914905499, 6.824651320745979, 7.348660565908913,
+    7.887517316500365, 8.444371322523448, 9.023480280417651, 9.63088568694906,
+    10.275741581733518, 10.973300958513551, 11.753178461616617, 12.693012315439578};
+static __constant__ double kGhW[48] = {
+    4.477155473875909e-36, 3.3764561431353977e-31, 2.0790589714186624e-27, 3.139476644799188e-24,
+    1.7988549162372895e-21, 4.925465109698984e-19, 7.419993598064828e-17, 6.7566772746571655e-15,
+    3.975805958471501e-13, 1.5883609812463425e-11, 4.474287153436362e-10, 9.154055746313818e-09,
+    1.3927916895599565e-07, 1.6063936955426115e-06, 1.4266092324246835e-05, 9.881804917611342e-05,
+    0.0005395865846271887, 0.0023430821117604002, 0.008149696855351924, 0.022838212100091373,
+    0.05180518354956028, 0.09546340056142932, 0.14328245699317615, 0.17546354181186605,
+    0.17546354181186605, 0.14328245699317615, 0.09546340056142932, 0.05180518354956028,
+    0.022838212100091373, 0.008149696855351924, 0.0023430821117604002, 0.0005395865846271887,
+    9.881804917611342e-05, 1.4266092324246835e-05, 1.6063936955426115e-06, 1.3927916895599565e-07,
+    9.154055746313818e-09, 4.474287153436362e-10, 1.5883609812463425e-11, 3.975805958471501e-13,
+    6.7566772746571655e-15, 7.419993598064828e-17, 4.925465109698984e-19, 1.7988549162372895e-21,
+    3.139476644799188e-24, 2.0790589714186624e-27, 3.3764561431353977e-31, 4.477155473875909e-36};
+
 struct RowConst {
   double ks, s, xt, ic1, mut4ks, warea, beta, i0, jj, valid, eps;
 };

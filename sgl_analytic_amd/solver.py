@@ -102,6 +102,11 @@ def cosmo_records(params: np.ndarray, n_z, radiation="engine", growth_mode="ode"
     return rec
 
 
+def vp_groups(shape: Shape) -> int:
+    """Closure groups per slice: the halo mass rows, then the filament rows."""
+    return shape.n_m * (2 if shape.filaments else 1)
+
+
 def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
     return ctypes.c_void_p(t.data_ptr())
 
@@ -118,9 +123,13 @@ class Solver:
     """
 
     def __init__(self, shape: Shape, B: int, path: str = "binned", precision: int = PREC_FP32,
-                 device: str = "cuda:0"):
+                 device: str = "cuda:0", closure: str = "gauss"):
         if path not in ("exact", "binned"):
             raise ValueError("path must be 'exact' or 'binned'")
+        if closure not in ("gauss", "vp"):
+            raise ValueError("closure must be 'gauss' or 'vp'")
+        if closure == "vp" and (path != "exact" or not shape.clustering):
+            raise ValueError("the vp closure runs on the exact path with clustering")
         if path == "exact":
             precision = PREC_FP64
         elif precision not in (PREC_FP32, PREC_BF16X3):
@@ -129,6 +138,7 @@ class Solver:
             raise _lib.SglError("no HIP device visible: the solver has no CPU path")
         self.L = _lib.load()
         self.shape, self.B, self.path, self.precision = shape, int(B), path, precision
+        self.closure = closure
         self.dev = torch.device(device)
         self.cs = shape.c()
         cs = ctypes.byref(self.cs)
@@ -157,6 +167,8 @@ class Solver:
         self.diag = torch.zeros((B, 8), dtype=torch.float64, device=self.dev)
         if path == "exact":
             self.work = torch.zeros((B, nz, nk, 2), dtype=torch.float64, device=self.dev)
+            if closure == "vp":
+                self.vpw = buf(L.sgl_vp_bytes(cs, B))
             self.phi = buf(L.sgl_phi_bytes(cs, B, PREC_FP64))
             self.p = torch.zeros((B, shape.n_out), dtype=torch.float64, device=self.dev)
         else:
@@ -174,14 +186,19 @@ class Solver:
         ck(L.sgl_tables_build(_ptr(self.records), B, cs, _ptr(self.tables), _ptr(self.status), st),
            "sgl_tables_build")
         psi = None
-        if self.path == "exact":
+        if self.closure == "vp":
+            ck(L.sgl_lnphi_exact_vp(_ptr(self.tables), B, cs, _ptr(self.lam), _ptr(self.psi2),
+                                    _ptr(self.work), _ptr(self.vpw), _ptr(self.status), st),
+               "sgl_lnphi_exact_vp")
+            psi = _ptr(self.psi2)
+        elif self.path == "exact":
             ck(L.sgl_lnphi_exact(_ptr(self.tables), B, cs, _ptr(self.lam), _ptr(self.amp),
                                  _ptr(self.work), st), "sgl_lnphi_exact")
         else:
             ck(L.sgl_bin_cells(_ptr(self.tables), B, cs, _ptr(self.hist), st), "sgl_bin_cells")
             ck(L.sgl_lnphi_binned(_ptr(self.tables), _ptr(self.hist), B, cs, _ptr(self.lam),
                                   _ptr(self.amp), st), "sgl_lnphi_binned")
-        if self.shape.clustering:
+        if self.shape.clustering and self.closure == "gauss":
             if self.path == "exact":
                 ck(L.sgl_quadform(_ptr(self.tables), _ptr(self.amp), B, cs, PREC_FP64,
                                   _ptr(self.psi2), st), "sgl_quadform")
@@ -205,6 +222,11 @@ class Solver:
             ck(L.sgl_invert_gemm(_ptr(self.E), _ptr(self.phi), B, cs, self.precision,
                                  _ptr(self.p), st), "sgl_invert_gemm")
             ck(L.sgl_diagnostics(_ptr(self.p), 0, B, cs, _ptr(self.diag), st), "sgl_diagnostics")
+
+    def vp_calibration(self):
+        """(eta [B], target [B]) of the last vp solve (host copies)."""
+        e = self.vpw[:16 * self.B].view(torch.float64).cpu().numpy()
+        return e[:self.B].copy(), e[self.B:].copy()
 
     def set_records(self, records):
         r = torch.as_tensor(np.asarray(records, dtype=np.float64))

@@ -47,8 +47,8 @@ def test_cosmology_validation():
 
 def test_paths_not_computed_here_raise():
     with patch("sgl_analytic_amd.api._solver") as solver:
-        with pytest.raises(NotImplementedError):
-            api.run(1.0, "full", cosmo={}, xi_out=np.array([0.0, 1.0]), closure="vp")
+        with pytest.raises(ValueError):
+            api.run(1.0, "full", cosmo={}, xi_out=np.array([0.0, 1.0]), closure="lognormal")
         with pytest.raises(NotImplementedError):
             api.run(1.0, "+sub", cosmo={}, xi_out=np.array([0.0, 1.0]))
         with pytest.raises(ValueError):     # vp needs the bias ingredient, as in the reference
@@ -58,6 +58,18 @@ def test_paths_not_computed_here_raise():
         with pytest.raises(ValueError):
             api.run(1.0, "full", cosmo={}, xi_out=np.array([0.0, 1.0]), edge="r500")
         solver.assert_not_called()
+
+
+def test_vp_closure_is_an_exact_path_option():
+    """The positive closure runs on the plain (exact) path and needs the slice
+    covariance; the solver rejects it elsewhere before touching a device."""
+    from sgl_analytic_amd.solver import Shape, Solver
+    with pytest.raises(ValueError):
+        Solver(Shape(), 1, path="binned", closure="vp")
+    with pytest.raises(ValueError):
+        Solver(Shape(clustering=False), 1, path="exact", closure="vp")
+    with pytest.raises(ValueError):
+        Solver(Shape(), 1, path="exact", closure="lognormal")
 
 
 def test_growth_and_radiation_options():
