@@ -60,7 +60,7 @@ def main(argv=None):
     ap.add_argument("--radiation", choices=("engine", "none"), default=EXTENSIONS["radiation"],
                     help='"engine" (default): Omega_r = Om/(1+zeq); "none": no radiation')
     ap.add_argument("--closure", choices=("gauss", "vp"), default="gauss",
-                    help='clustering closure ("vp" is not computed on this path)')
+                    help="clustering closure: gauss (two-point term) or vp (positive closure)")
     ap.add_argument("--config", default="full",
                     help='lens model: "full" (default), "halo", "-bias", ... '
                          '(pass minus arms as --config=-bias)')
