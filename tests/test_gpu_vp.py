@@ -89,3 +89,45 @@ def test_vp_through_the_api():
     assert np.isfinite(pc["eta"]) and pc["eta"] > 0 and pc["target_kappa"] > 0
     assert abs(np.trapezoid(pv, x) - 1) < 1e-3
     assert 1e-5 < float(np.trapezoid(np.abs(pv - pg), x)) < 0.1
+
+
+def test_vp_full_shape_properties():
+    """The plain-path workload with the vp closure (z_s = 0.3: 64 slices x 256
+    masses x 512 impact parameters, 8192 k nodes, 2048 ln mu nodes) checked
+    through properties that hold at any size: |phi(k)| <= 1 (given the latent
+    states the rates are non-negative, so phi is a mixture of compound-Poisson
+    characteristic functions), ln R -> -1/2 k^2 target at the smallest k (the
+    calibration's defining limit), and the grid mass and <mu> of the Gaussian
+    closure on the same grids (the grid mass misses the no-jump point mass,
+    which phi keeps as k -> infinity: e^{-sum w} times the closure's factor,
+    E e^{-sum (lambda - 1) w} for vp and e^{Psi_2} for gauss, equal to second
+    order in the clustering), and closeness to the Gaussian closure's PDF."""
+    from sgl_analytic_amd.solver import Shape, Solver
+    print("vp full shape zs=0.3", flush=True)
+    sh = Shape()
+    rec = _records([0.3], [64])
+    sv = Solver(sh, 1, path="exact", closure="vp")
+    p, d = sv.run(rec)
+    torch.cuda.synchronize()
+    assert int(sv.status.cpu()[0]) == 0
+    eta, target = sv.vp_calibration()
+    lnr = sv.psi2[0].cpu().numpy() @ np.array([1.0, 1j])
+    lam = sv.lam[0].cpu().numpy() @ np.array([1.0, 1j])
+    k = sh.k_nodes()[1:]
+    phi_abs = np.abs(np.exp(lam + lnr))
+    small = -0.5 * k[0] ** 2 * target[0]
+    pv, dv = p[0].cpu().numpy().copy(), d[0].cpu().numpy().copy()
+    sg = Solver(sh, 1, path="exact")
+    pg, dg = sg.run(rec)
+    torch.cuda.synchronize()
+    x = sh.out_grid()
+    l1 = float(np.trapezoid(np.abs(pv - pg[0].cpu().numpy()), x))
+    print(f"eta {eta[0]:.6g} target {target[0]:.4g} max|phi| {phi_abs.max():.15g} "
+          f"lnR(k0) {lnr[0]:.6g} vs {small:.6g}; mass {dv[0]:.9f} <mu> {dv[5]:.9f} "
+          f"(gauss {dg[0, 0].item():.9f}, {dg[0, 5].item():.9f}); L1 vs gauss {l1:.3g}",
+          flush=True)
+    assert phi_abs.max() <= 1.0 + 1e-12
+    assert abs(lnr[0].real / small - 1) < 1e-3 and abs(lnr[0].imag) < 1e-3 * abs(small)
+    assert abs(dv[0] - dg[0, 0].item()) < 1e-3
+    assert abs(dv[5] - dg[0, 5].item()) < 1e-4
+    assert 1e-6 < l1 < 0.05
