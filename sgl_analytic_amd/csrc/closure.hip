@@ -89,14 +89,18 @@ __global__ __launch_bounds__(64) void k_vp_transition(const double* __restrict__
 // tm = t/m; linear: minv = sqrt(v), tm = -1).
 //
 // Chain variance (pos_closure.py:389-403) of sum_s m_s(U_s),
-// m_s(u_j) = sum_g lambda_sg(u_j) wk_sg; eta from bisection in ln eta on
-// variance(eta) = target inside the reference's bracket (start [1/1.5, 1.5],
-// widened by 1.5 per side, pos_closure.py:575-591), to 4e-14 in ln eta.
+// m_s(u_j) = sum_g lambda_sg(u_j) wk_sg; eta from false position (Illinois)
+// in ln eta on variance(eta) = target inside the reference's bracket (start
+// [1/1.5, 1.5], widened by 1.5 per side, pos_closure.py:575-591), to 4e-14
+// in ln eta.
+constexpr int kVpGroupsPerLane = 2 * kMaxMasses / 64;
+
 struct VpStage {
   double u[kVpN], w[kVpN], A[kVpN], Bs[kVpN], Cs[kVpN], rv[kVpN - 1];
   double f[kMaxSlices], d[kMaxSlices];
   double ms[kMaxSlices][kVpN];
   double h[2][kVpN];
+  double hb[4][2][kVpN];                 // per wave: bucket sums H1, H0 of one slice
   double red[256];
   int cap;
 };
@@ -223,28 +227,70 @@ __global__ __launch_bounds__(256) void k_vp_setup(const double* __restrict__ row
   auto eval = [&](double le) -> double {
     const double eta = exp(le);
     bool bad = false;
-    for (int q = t; q < n_z * G; q += 256) {
-      const int s = q / G, g = q - s * G;
-      const double beta = group_beta(rows, frows, sh, (size_t)b * NZ + s, g);
-      double minv, tm;
-      int bk;
-      bad |= !vp_rate(S, eta * S.f[s] * beta * beta * S.d[s] * S.d[s], minv, tm, bk);
-      tab[3 * (size_t)q] = minv;
-      tab[3 * (size_t)q + 1] = tm;
-      tab[3 * (size_t)q + 2] = (double)bk;
+    // One wave per slice: lane l owns the groups g = l + 64 q; the rates'
+    // bucket sums H1_j = sum_{bucket g = j} wk_g / m_g, H0_j = sum t_g wk_g / m_g
+    // by wave reductions over the buckets present, then
+    // m_s(u_i) = u_i sum_{j<=i} H1_j - sum_{j<=i} H0_j.
+    const int lane = t & 63, wave = t >> 6;
+    for (int s = wave; s < n_z; s += 4) {
+      double a1[kVpGroupsPerLane], a0[kVpGroupsPerLane];
+      int bq[kVpGroupsPerLane];
+      int bmin = kVpN, bmax = -1;
+      const double vs = eta * S.f[s] * S.d[s] * S.d[s];
+#pragma unroll
+      for (int q = 0; q < kVpGroupsPerLane; ++q) {
+        const int g = lane + 64 * q;
+        a1[q] = a0[q] = 0.0;
+        bq[q] = kVpN;
+        if (g < G) {
+          const double beta = group_beta(rows, frows, sh, (size_t)b * NZ + s, g);
+          double minv, tm;
+          int bk;
+          bad |= !vp_rate(S, vs * beta * beta, minv, tm, bk);
+          const size_t o = 3 * ((size_t)s * G + g);
+          tab[o] = minv;
+          tab[o + 1] = tm;
+          tab[o + 2] = (double)bk;
+          const double wg = wk[(size_t)s * G + g];
+          a1[q] = minv * wg;
+          a0[q] = tm * wg;
+          bq[q] = bk;
+          bmin = min(bmin, bk);
+          bmax = max(bmax, bk);
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        bmin = min(bmin, __shfl_xor(bmin, o));
+        bmax = max(bmax, __shfl_xor(bmax, o));
+      }
+      for (int j = lane; j < kVpN; j += 64) {
+        S.hb[wave][0][j] = 0.0;
+        S.hb[wave][1][j] = 0.0;
+      }
+      for (int j = bmin; j <= bmax; ++j) {
+        double h1 = 0.0, h0 = 0.0;
+#pragma unroll
+        for (int q = 0; q < kVpGroupsPerLane; ++q)
+          if (bq[q] == j) { h1 += a1[q]; h0 += a0[q]; }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          h1 += __shfl_xor(h1, o);
+          h0 += __shfl_xor(h0, o);
+        }
+        if (lane == 0) { S.hb[wave][0][j] = h1; S.hb[wave][1][j] = h0; }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (lane < kVpN) {
+        double p1 = 0.0, p0 = 0.0;
+        for (int j = 0; j <= lane; ++j) { p1 += S.hb[wave][0][j]; p0 += S.hb[wave][1][j]; }
+        S.ms[s][lane] = fma(S.u[lane], p1, -p0);
+      }
+      __builtin_amdgcn_wave_barrier();
     }
     if (bad) S.cap = 1;
-    __syncthreads();
-    for (int q = t; q < n_z * kVpN; q += 256) {
-      const int s = q / kVpN, j = q - s * kVpN;
-      const double u = S.u[j];
-      double m = 0.0;
-      for (int g = 0; g < G; ++g) {
-        const size_t o = 3 * ((size_t)s * G + g);
-        if ((int)tab[o + 2] <= j) m = fma(u * tab[o] - tab[o + 1], wk[(size_t)s * G + g], m);
-      }
-      S.ms[s][j] = m;
-    }
     __syncthreads();
     // chain: var = sum_s w.mu_s^2 + 2 sum_{s>=1} mu_s . (K_{s-1}^T h_{s-1}),
     // h_s = K_{s-1}^T h_{s-1} + w mu_s
@@ -287,13 +333,20 @@ __global__ __launch_bounds__(256) void k_vp_setup(const double* __restrict__ row
     }
   }
   int flag = (fa * fb > 0.0) ? kStatusVpRate : 0;
-  for (int it = 0; it < 100 && bb - a > 4e-14; ++it) {
-    const double m = 0.5 * (a + bb);
-    const double fm = eval(m);
-    if (fm == 0.0) { a = bb = m; break; }
-    if ((fm > 0.0) == (fa > 0.0)) { a = m; fa = fm; } else { bb = m; fb = fm; }
+  // Illinois false position inside the bracket (superlinear for this smooth,
+  // monotone variance; a capped value forces a bisection step), to 4e-14 in
+  // ln eta
+  double x0 = a, f0 = fa, x1 = bb, f1 = fb;
+  for (int it = 0; it < 100 && fabs(x1 - x0) > 4e-14; ++it) {
+    double c = (f0 >= 1e299 || f1 >= 1e299) ? 0.5 * (x0 + x1) : x1 - f1 * (x1 - x0) / (f1 - f0);
+    if (!(c > fmin(x0, x1) && c < fmax(x0, x1))) c = 0.5 * (x0 + x1);
+    const double fc = eval(c);
+    if (fc == 0.0) { x0 = x1 = c; break; }
+    if ((fc > 0.0) != (f1 > 0.0)) { x0 = x1; f0 = f1; } else { f0 *= 0.5; }
+    x1 = c;
+    f1 = fc;
   }
-  const double le = 0.5 * (a + bb);
+  const double le = x1;
   if (eval(le) >= 1e299) flag |= kStatusVpRate;
   if (t == 0) {
     vp[V.eta + b] = exp(le);
