@@ -127,7 +127,7 @@ __device__ __forceinline__ void gen_fil_cell(const FilStage& F, int c, double& x
 constexpr int kCellTile = 256;
 constexpr int kKpt = 2;  // k nodes per thread
 
-__global__ __launch_bounds__(256) void k_lnphi_exact(const double* __restrict__ rows,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_lnphi_exact(const double* __restrict__ rows,
                                                      const double* __restrict__ frows,
                                                      const double* __restrict__ cosmo,
                                                      const double* __restrict__ uedge,
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void k_vp_wk(const double* __restrict__ rows,
 // lambda_g(u_j) = u_j minv_g - tm_g for j >= bucket_g, 0 below); written as
 // zpart[j], with ytot = sum_g Y_g, so that sum_g (lambda_g(u_j) - 1) Y_g =
 // zpart[j] - ytot.
-__global__ __launch_bounds__(256) void k_lnphi_exact_vp(const double* __restrict__ rows,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_lnphi_exact_vp(const double* __restrict__ rows,
                                                         const double* __restrict__ frows,
                                                         const double* __restrict__ cosmo,
                                                         const double* __restrict__ uedge,
