@@ -127,16 +127,19 @@ __device__ __forceinline__ void gen_fil_cell(const FilStage& F, int c, double& x
 constexpr int kCellTile = 256;
 constexpr int kKpt = 2;  // k nodes per thread
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_lnphi_exact(const double* __restrict__ rows,
-                                                     const double* __restrict__ frows,
-                                                     const double* __restrict__ cosmo,
-                                                     const double* __restrict__ uedge,
-                                                     const double* __restrict__ knodes,
-                                                     sgl_shape sh, double* __restrict__ part,
-                                                     double* __restrict__ amp) {
-  __shared__ SliceStage S;
-  __shared__ FilStage FS;
-  __shared__ double sxi[kCellTile], sws[kCellTile], sbw[kCellTile];
+// Body of the exact exponent kernel; the filament variant stages the
+// filament rows (FilStage) as well, the halo-only variant leaves that LDS out
+// so that three blocks fit per CU.
+template <bool kFil>
+__device__ __forceinline__ void lnphi_exact_body(const double* __restrict__ rows,
+                                                 const double* __restrict__ frows,
+                                                 const double* __restrict__ cosmo,
+                                                 const double* __restrict__ uedge,
+                                                 const double* __restrict__ knodes,
+                                                 const sgl_shape& sh, double* __restrict__ part,
+                                                 double* __restrict__ amp, SliceStage& S,
+                                                 FilStage* FS, double* sxi, double* sws,
+                                                 double* sbw) {
   const int t = threadIdx.x;
   const int i = blockIdx.y, b = blockIdx.z;
   const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
@@ -151,8 +154,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   }
   if (i < n_z) {
     stage_slice(S, rows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kRowStride, uedge, sh);
-    const bool fil = (sh.flags & SGL_FLAG_FILAMENTS) != 0;
-    if (fil) stage_fil(FS, frows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kFilRowStride, sh);
+    constexpr bool fil = kFil;
+    if constexpr (kFil) stage_fil(*FS, frows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kFilRowStride, sh);
     const int n_phi = (sh.flags & SGL_FLAG_ELL) ? kEllPhi : 1;
     const int n_halo = sh.n_m * sh.n_b * n_phi;
     const int n_cells = n_halo + (fil ? sh.n_m * kFilU * kFilPhi : 0);
@@ -162,7 +165,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         int c = c0 + t;
         double xi = 0.0, ws = 0.0, bw = 0.0;
         if (c < n_halo) gen_cell(S, sh.n_b, n_phi, c, xi, ws, bw);
-        else if (c < n_cells) gen_fil_cell(FS, c - n_halo, xi, ws, bw);
+        else if constexpr (kFil) {
+          if (c < n_cells) gen_fil_cell(*FS, c - n_halo, xi, ws, bw);
+        }
         sxi[t] = xi; sws[t] = ws; sbw[t] = bw;
       }
       __syncthreads();
@@ -401,6 +406,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       part[2 * (base + n) + 1] = li[q];
     }
   }
+}
+
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_lnphi_exact(
+    const double* __restrict__ rows, const double* __restrict__ frows,
+    const double* __restrict__ cosmo, const double* __restrict__ uedge,
+    const double* __restrict__ knodes, sgl_shape sh, double* __restrict__ part,
+    double* __restrict__ amp) {
+  __shared__ SliceStage S;
+  __shared__ double sxi[kCellTile], sws[kCellTile], sbw[kCellTile];
+  lnphi_exact_body<false>(rows, frows, cosmo, uedge, knodes, sh, part, amp, S, nullptr, sxi, sws,
+                          sbw);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_lnphi_exact_fil(
+    const double* __restrict__ rows, const double* __restrict__ frows,
+    const double* __restrict__ cosmo, const double* __restrict__ uedge,
+    const double* __restrict__ knodes, sgl_shape sh, double* __restrict__ part,
+    double* __restrict__ amp) {
+  __shared__ SliceStage S;
+  __shared__ FilStage FS;
+  __shared__ double sxi[kCellTile], sws[kCellTile], sbw[kCellTile];
+  lnphi_exact_body<true>(rows, frows, cosmo, uedge, knodes, sh, part, amp, S, &FS, sxi, sws, sbw);
 }
 
 // lam[b][n] = sum over slices of part[b][i][n], in slice order.
@@ -1011,8 +1039,12 @@ int launch_lnphi_exact(const double* rows, const double* frows, const double* co
                        const double* uedge, const double* knodes, const sgl_shape& sh, int B,
                        double* lam, double* amp, double* work, hipStream_t st) {
   dim3 grid((sh.n_k + 256 * kKpt - 1) / (256 * kKpt), sh.n_z_max, B);
-  hipLaunchKernelGGL(k_lnphi_exact, grid, dim3(256), 0, st, rows, frows, cosmo, uedge, knodes, sh,
-                     work, amp);
+  if (sh.flags & SGL_FLAG_FILAMENTS)
+    hipLaunchKernelGGL(k_lnphi_exact_fil, grid, dim3(256), 0, st, rows, frows, cosmo, uedge,
+                       knodes, sh, work, amp);
+  else
+    hipLaunchKernelGGL(k_lnphi_exact, grid, dim3(256), 0, st, rows, frows, cosmo, uedge, knodes,
+                       sh, work, amp);
   size_t n = (size_t)B * sh.n_k;
   hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, work, B,
                      sh.n_z_max, sh.n_k, lam);
