@@ -331,12 +331,18 @@ def clipped_moments(xi, P, clip=1.0):
 
 
 def generate_pdf_lnmu_batch(params, precision="fp32", graph=True, shape=None, *,
-                            growth_mode="ode", radiation="engine"):
+                            growth_mode="ode", radiation="engine", closure="gauss"):
     """Batched solve (no reference counterpart): params [B, 10] in the order
     (h, Om, sigma8, Ob, ns, zeq, Ok, w0, wa, z_s); slice count per instance
     ceil(24 (1 + ln(1 + z_s))); growth_mode ("ode", "legacy", "auto") and
     radiation ("engine", None) apply to the whole batch.  Returns (lnmu grid,
-    p [B, n_out] float32 numpy, diagnostics [B, 8])."""
+    p [B, n_out] float32 numpy, diagnostics [B, 8]).  closure="vp" (the
+    positive closure) solves the batch on the plain FP64 path instead (one
+    launch per step for all B instances; p is float64; precision and graph
+    do not apply)."""
+    if closure not in ("gauss", "vp"):
+        raise ValueError(f"closure must be 'gauss' or 'vp', got {closure!r}")
+    from . import _lib
     from ._lib import PREC_BF16X3, PREC_FP32
     from .solver import Shape, Solver, cosmo_records, slice_count
     params = np.atleast_2d(np.asarray(params, dtype=np.float64))
@@ -355,7 +361,11 @@ def generate_pdf_lnmu_batch(params, precision="fp32", graph=True, shape=None, *,
     n_z = slice_count(params[:, 9])
     sh = shape or Shape(n_z_max=int(n_z.max()))
     prec = PREC_FP32 if precision == "fp32" else PREC_BF16X3
-    sv = Solver(sh, params.shape[0], path="binned", precision=prec)
+    if closure == "vp":
+        sv = Solver(sh, params.shape[0], path="exact", closure="vp")
+        graph = False
+    else:
+        sv = Solver(sh, params.shape[0], path="binned", precision=prec)
     sv.set_records(cosmo_records(params, n_z, radiation=None if radiation in ("none", False, None)
                                  else radiation, growth_mode=mode))
     if graph:
@@ -363,4 +373,8 @@ def generate_pdf_lnmu_batch(params, precision="fp32", graph=True, shape=None, *,
     p, d = sv.run()
     import torch
     torch.cuda.synchronize()
+    if closure == "vp":
+        st = int(sv.status.cpu()[0])
+        if st & (_lib.STATUS_VP_TRANSITION | _lib.STATUS_VP_RATE):
+            raise _lib.SglError(f"vp closure set-up failed (status {st:#x})")
     return sh.out_grid(), p.cpu().numpy(), d.cpu().numpy()

@@ -131,3 +131,27 @@ def test_vp_full_shape_properties():
     assert abs(dv[0] - dg[0, 0].item()) < 1e-3
     assert abs(dv[5] - dg[0, 5].item()) < 1e-4
     assert 1e-6 < l1 < 0.05
+
+
+def test_vp_batch_api_uneven_slices():
+    """generate_pdf_lnmu_batch(closure="vp"): three cosmologies with their own
+    slice counts ceil(24 (1 + ln(1 + z_s))) in one plain-path launch, each
+    against the oracle at its own slice count."""
+    from sgl_analytic_amd import api
+    from sgl_analytic_amd.solver import Shape, slice_count
+    print("vp batch api", flush=True)
+    zs = np.array([0.3, 1.0, 2.0])
+    c = PLANCK
+    P = np.array([[c["h"], c["Om"], c["s8"], c["Ob"], c["ns"], c["zeq"], 0.0, -1.0, 0.0, z]
+                  for z in zs])
+    n_z = slice_count(zs)
+    x, p, d = api.generate_pdf_lnmu_batch(P, shape=Shape(n_z_max=int(n_z.max()), **SMALL),
+                                          closure="vp")
+    assert p.dtype == np.float64 and p.shape == (3, SMALL["n_out"])
+    for b in range(3):
+        r = chain.pdf(float(zs[b]), chain.Shape(n_z=int(n_z[b]), **SMALL), closure="vp")
+        l1 = float(np.trapezoid(np.abs(p[b] - r.p), r.x))
+        print(f"zs={zs[b]} n_z={n_z[b]} L1 {l1:.3g}", flush=True)
+        assert l1 < 1e-9
+    with pytest.raises(ValueError):
+        api.generate_pdf_lnmu_batch(P, closure="lognormal")
