@@ -67,3 +67,25 @@ def test_psi2_subgrid_node_count():
             assert n_kc == n_k
     assert L.sgl_psi2_nodes(ctypes.byref(Shape().c())) == 1025
     assert L.sgl_psi2_nodes(None) == 0
+
+
+def test_vp_workspace_size():
+    """sgl_vp_bytes: eta and target (2B), wk and the rate table per group,
+    one 48 x 48 transition per slice, ytot and the 48 rate-weighted sums per
+    (slice, k node), each sub-array padded to 32 doubles; groups are the mass
+    rows, twice that with filaments; invalid shapes give 0."""
+    import ctypes
+    from sgl_analytic_amd import _lib
+    from sgl_analytic_amd.solver import Shape
+    L = _lib.load()
+
+    def expect(sh, B):
+        pad = lambda n: (n + 31) // 32 * 32
+        nz, G = B * sh.n_z_max, sh.n_m * (2 if sh.filaments else 1)
+        return 8 * (pad(2 * B) + pad(nz * G) + pad(nz * 48 * 48) + pad(nz * G * 3)
+                    + pad(nz * sh.n_k * 2) + pad(nz * 48 * sh.n_k * 2))
+
+    for sh, B in ((Shape(), 1), (Shape(filaments=True), 3), (Shape(n_z_max=7, n_m=33, n_k=999), 5)):
+        assert L.sgl_vp_bytes(ctypes.byref(sh.c()), B) == expect(sh, B)
+    assert L.sgl_vp_bytes(ctypes.byref(Shape(n_z_max=200).c()), 1) == 0
+    assert L.sgl_vp_bytes(ctypes.byref(Shape().c()), 0) == 0
