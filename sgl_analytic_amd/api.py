@@ -338,11 +338,10 @@ def generate_pdf_lnmu_batch(params, precision="fp32", graph=True, shape=None, *,
     radiation ("engine", None) apply to the whole batch.  Returns (lnmu grid,
     p [B, n_out] float32 numpy, diagnostics [B, 8]).  closure="vp" (the
     positive closure) solves the batch on the plain FP64 path instead (one
-    launch per step for all B instances; p is float64; precision and graph
-    do not apply)."""
+    launch per step for a chunk of instances whose closure workspace fits
+    VP_BATCH_BYTES; p is float64; precision and graph do not apply)."""
     if closure not in ("gauss", "vp"):
         raise ValueError(f"closure must be 'gauss' or 'vp', got {closure!r}")
-    from . import _lib
     from ._lib import PREC_BF16X3, PREC_FP32
     from .solver import Shape, Solver, cosmo_records, slice_count
     params = np.atleast_2d(np.asarray(params, dtype=np.float64))
@@ -361,20 +360,46 @@ def generate_pdf_lnmu_batch(params, precision="fp32", graph=True, shape=None, *,
     n_z = slice_count(params[:, 9])
     sh = shape or Shape(n_z_max=int(n_z.max()))
     prec = PREC_FP32 if precision == "fp32" else PREC_BF16X3
+    rec = cosmo_records(params, n_z, radiation=None if radiation in ("none", False, None)
+                        else radiation, growth_mode=mode)
     if closure == "vp":
-        sv = Solver(sh, params.shape[0], path="exact", closure="vp")
-        graph = False
-    else:
-        sv = Solver(sh, params.shape[0], path="binned", precision=prec)
-    sv.set_records(cosmo_records(params, n_z, radiation=None if radiation in ("none", False, None)
-                                 else radiation, growth_mode=mode))
+        return sh.out_grid(), *_vp_batch(sh, rec)
+    sv = Solver(sh, params.shape[0], path="binned", precision=prec)
+    sv.set_records(rec)
     if graph:
         sv.capture()
     p, d = sv.run()
     import torch
     torch.cuda.synchronize()
-    if closure == "vp":
+    return sh.out_grid(), p.cpu().numpy(), d.cpu().numpy()
+
+
+VP_BATCH_BYTES = 32 << 30     # device workspace of one vp chunk (zpart dominates)
+
+
+def _vp_batch(sh, rec):
+    """Plain-path solves with the positive closure in chunks whose closure
+    workspace stays within VP_BATCH_BYTES; returns (p float64, diag)."""
+    import ctypes
+
+    import torch
+
+    from . import _lib
+    from .solver import Solver
+    L = _lib.load()
+    per = int(L.sgl_vp_bytes(ctypes.byref(sh.c()), 1))
+    chunk = max(1, min(rec.shape[0], VP_BATCH_BYTES // max(per, 1)))
+    ps, ds = [], []
+    sv = None
+    for i in range(0, rec.shape[0], chunk):
+        r = rec[i:i + chunk]
+        if sv is None or sv.B != r.shape[0]:
+            sv = Solver(sh, r.shape[0], path="exact", closure="vp")
+        p, d = sv.run(r)
+        torch.cuda.synchronize()
         st = int(sv.status.cpu()[0])
         if st & (_lib.STATUS_VP_TRANSITION | _lib.STATUS_VP_RATE):
             raise _lib.SglError(f"vp closure set-up failed (status {st:#x})")
-    return sh.out_grid(), p.cpu().numpy(), d.cpu().numpy()
+        ps.append(p.cpu().numpy())
+        ds.append(d.cpu().numpy())
+    return np.concatenate(ps), np.concatenate(ds)

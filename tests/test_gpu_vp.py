@@ -153,5 +153,17 @@ def test_vp_batch_api_uneven_slices():
         l1 = float(np.trapezoid(np.abs(p[b] - r.p), r.x))
         print(f"zs={zs[b]} n_z={n_z[b]} L1 {l1:.3g}", flush=True)
         assert l1 < 1e-9
+    # chunked by the workspace cap (chunks of 2 + 1): the same numbers
+    import ctypes
+    from sgl_analytic_amd import _lib
+    sh = Shape(n_z_max=int(n_z.max()), **SMALL)
+    per = _lib.load().sgl_vp_bytes(ctypes.byref(sh.c()), 1)
+    keep = api.VP_BATCH_BYTES
+    try:
+        api.VP_BATCH_BYTES = 2 * per
+        _, p2, d2 = api.generate_pdf_lnmu_batch(P, shape=sh, closure="vp")
+    finally:
+        api.VP_BATCH_BYTES = keep
+    assert np.array_equal(p2, p) and np.array_equal(d2, d)
     with pytest.raises(ValueError):
         api.generate_pdf_lnmu_batch(P, closure="lognormal")
