@@ -322,27 +322,36 @@ __global__ void k_filon_build(const double* __restrict__ knodes, sgl_shape sh, i
   }
 }
 
-// Plain path: thread per (b, j), FP64 Filon weights on the fly; output
-// points from `t` when non-null (n_t of them), else the shape's uniform grid.
+// Plain path: one workgroup per (j, b), FP64 Filon weights on the fly; the
+// 256 threads take contiguous runs of the k intervals and the partial sums
+// meet in a fixed-order tree (deterministic).  Output points from `t` when
+// non-null (n_t of them), else the shape's uniform grid.
 __global__ __launch_bounds__(256) void k_invert_fp64(const double* __restrict__ phi,
                                                      const double* __restrict__ knodes,
                                                      sgl_shape sh, int B,
                                                      const double* __restrict__ t, int n_t,
                                                      double* __restrict__ p) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
-  if (j >= n_t) return;
+  __shared__ double red[256];
+  const int j = blockIdx.x, b = blockIdx.y, tt = threadIdx.x;
   const double tj = t ? t[j] : sh.out_lo + (sh.out_hi - sh.out_lo) * j / (sh.n_out - 1);
-  const int nq = sh.n_k + 1;
-  const double* ph = phi + 2 * (size_t)b * nq;
+  const int nint = sh.n_k;                 // intervals between the n_k + 1 nodes
+  const int per = (nint + 255) / 256;
+  const int n0 = tt * per, n1 = min(nint, n0 + per);
+  const double* ph = phi + 2 * (size_t)b * (nint + 1);
   double acc = 0.0;
-  for (int n = 0; n + 1 < nq; ++n) {
+  for (int n = n0; n < n1; ++n) {
     double a = knode(knodes, n), h = knode(knodes, n + 1) - a, dr, di, ur, ui;
     filon_interval(a, h, tj, dr, di, ur, ui);
     acc += ph[2 * n] * dr - ph[2 * n + 1] * di;
     acc += ph[2 * (n + 1)] * ur - ph[2 * (n + 1) + 1] * ui;
   }
-  p[(size_t)b * n_t + j] = acc / kPi;
+  red[tt] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tt < o) red[tt] += red[tt + o];
+    __syncthreads();
+  }
+  if (tt == 0) p[(size_t)b * n_t + j] = red[0] / kPi;
 }
 
 // ----------------------------------------------------------- MFMA GEMM --
@@ -585,7 +594,8 @@ int launch_filon(const double* knodes, const sgl_shape& sh, int prec, void* out,
 int launch_invert_fp64(const double* phi, const double* knodes, const sgl_shape& sh, int B,
                        const double* t, int n_t, double* p, hipStream_t st) {
   if (t == nullptr) n_t = sh.n_out;
-  dim3 grid((n_t + 255) / 256, B);
+  if (n_t < 1) return SGL_OK;
+  dim3 grid(n_t, B);
   hipLaunchKernelGGL(k_invert_fp64, grid, dim3(256), 0, st, phi, knodes, sh, B, t, n_t, p);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
