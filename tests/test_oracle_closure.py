@@ -76,3 +76,25 @@ def test_vp_chain_limits(zs):
     np.testing.assert_allclose(lnR[:2].real / ref, 1.0, rtol=1e-3)
     np.testing.assert_allclose(psi2[:2].real / ref, 1.0, rtol=1e-3)
     assert np.all(np.abs(np.exp(lam + lnR)) <= 1 + 1e-12)
+
+
+def test_vp_filament_groups_end_to_end():
+    """With the filament rows as further groups: the calibration still meets
+    the Gaussian closure's k^2 coefficient (halo + filament A_s), phi stays a
+    characteristic function (|phi| <= 1), and the PDF keeps the grid mass and
+    mean of the Gaussian closure to the closures' higher-cumulant difference."""
+    osh = chain.Shape(n_z=6, n_m=16, n_b=32, n_k=512, n_out=256, filaments=True)
+    rv = chain.pdf(0.5, osh, closure="vp", keep=True)
+    rg = chain.pdf(0.5, osh, closure="gauss", keep=True)
+    T = rv.info["tables"]
+    F = chain.fil_rows(rv.info["cos"], 0.5, osh)
+    fxi, fws, _, _ = chain.cells_fil(F)
+    A = (np.sum(T["rows"][:, :, 5] * chain.vp_inputs(rv.info["xi"], rv.info["ws"], None), axis=1)
+         + np.sum(F[:, :, 3] * chain.vp_inputs(fxi, fws, None), axis=1))
+    assert abs(A @ rv.info["C"] @ A / rv.info["vp_target"] - 1) < 1e-12
+    assert np.all(np.abs(rv.info["phi"]) <= 1 + 1e-12)
+    dx = rv.x[1] - rv.x[0]
+    mv, mg = rv.p.sum() * dx, rg.p.sum() * dx
+    assert abs(mv - mg) < 1e-3
+    assert abs(rv.info["moments"]["mu_mean"] - rg.info["moments"]["mu_mean"]) < 1e-3
+    assert 1e-7 < np.sum(np.abs(rv.p - rg.p)) * dx < 0.05
