@@ -214,6 +214,8 @@ __device__ __forceinline__ void lnphi_exact_body(const double* __restrict__ rows
 // ---------------------------------------------- positive closure cells --
 // Cells of group g of the staged slice: halo mass row g (n_b x n_phi cells)
 // for g < n_m, else filament row g - n_m (kFilU x kFilPhi cells).
+constexpr int kVpWkSplit = 8;
+
 struct GroupCells {
   int first, count;
   bool fil;
@@ -224,7 +226,8 @@ __device__ __forceinline__ GroupCells group_cells(const sgl_shape& sh, int n_phi
 }
 
 // wk[b][s][g] = sum over the cells of group g of w xi (the k^2 coefficient
-// of the group's jump sum): one wave per group, lanes over the cells.
+// of the group's jump sum): one wave per group, lanes over the cells; the
+// groups of a slice are spread over kVpWkSplit workgroups.
 __global__ __launch_bounds__(256) void k_vp_wk(const double* __restrict__ rows,
                                                const double* __restrict__ frows,
                                                const double* __restrict__ cosmo,
@@ -242,7 +245,7 @@ __global__ __launch_bounds__(256) void k_vp_wk(const double* __restrict__ rows,
   const int n_phi = (sh.flags & SGL_FLAG_ELL) ? kEllPhi : 1;
   const int G = vp_groups(sh);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int g = wave; g < G; g += 4) {
+  for (int g = blockIdx.z * 4 + wave; g < G; g += 4 * gridDim.z) {
     const GroupCells gc = group_cells(sh, n_phi, g);
     double acc = 0.0;
     for (int l = lane; l < gc.count; l += 64) {
@@ -1064,8 +1067,8 @@ int launch_vp_cells(const double* rows, const double* frows, const double* cosmo
                     const double* uedge, const double* knodes, const sgl_shape& sh, int B,
                     const VpLayout& V, double* vp, double* work, int pass, hipStream_t st) {
   if (pass == 0) {
-    hipLaunchKernelGGL(k_vp_wk, dim3(sh.n_z_max, B), dim3(256), 0, st, rows, frows, cosmo, uedge,
-                       sh, vp + V.wk);
+    hipLaunchKernelGGL(k_vp_wk, dim3(sh.n_z_max, B, kVpWkSplit), dim3(256), 0, st, rows, frows,
+                       cosmo, uedge, sh, vp + V.wk);
   } else {
     dim3 grid((sh.n_k + 256 * kKpt - 1) / (256 * kKpt), sh.n_z_max, B);
     hipLaunchKernelGGL(k_lnphi_exact_vp, grid, dim3(256), 0, st, rows, frows, cosmo, uedge,
