@@ -41,6 +41,12 @@ typedef enum sgl_status {
                                  orientation nodes with the pseudo-elliptical kappa, gamma
                                  (sgl_full.py:551-613, 1659, 2038-2064); rows carry eps in
                                  slot 6, the tables a sigma(M) table for M* */
+#define SGL_FLAG_REF_GRID 32  /* the reference's own grids (set by sgl_ref_tables_build):
+                                 shells with the engine's 103-node log-z grid below z_s as
+                                 edges plus 0 and z_s, one midpoint node each
+                                 (sgl_full.py:480-518, zint="midpoint"), and the engine's
+                                 mass nodes 1..99 of exp(linspace(ln 1e7, ln 1e17, 100))
+                                 Msun (sgl_full.py:471-474, 1864) */
 
 /* Batched path: slice amplitudes and Psi_2 at every SGL_PSI2_STRIDE-th k node
  * (sub-grid node jc = k node jc * stride of the log grid, the last one up to

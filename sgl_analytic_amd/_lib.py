@@ -27,8 +27,13 @@ EXPORTS = (
     "sgl_diagnostics", "sgl_invert_fp64_at", "sgl_mlp_gemm", "sgl_mlp_colsum", "sgl_to_bf16",
     "sgl_lnphi_interp", "sgl_set_tuning", "sgl_get_tuning", "sgl_vp_bytes", "sgl_lnphi_exact_vp",
     "sgl_bin_pass", "sgl_bin_reduce", "sgl_psi2_nodes", "sgl_psi2_interp",
-    "sgl_trunc_nfw_kg",
+    "sgl_trunc_nfw_kg", "sgl_ref_shell_count", "sgl_ref_tasks", "sgl_ref_tables_bytes",
+    "sgl_ref_roots_bytes", "sgl_ref_tables_build", "sgl_ref_cells_count", "sgl_ref_cells_emit",
+    "sgl_ref_bin_bytes", "sgl_ref_bin", "sgl_ref_lam_axis", "sgl_ref_phi_bytes", "sgl_ref_phi",
+    "sgl_ref_p3_bytes", "sgl_ref_p3", "sgl_ref_arcs", "sgl_ref_moments", "sgl_ref_gemm_f64",
 )
+REF_MAX_SHELLS, REF_MASSES, REF_FINE_BINS, REF_ROOT_STRIDE = 104, 99, 961 * 300, 128
+REF_STATUS_ROOTS, REF_STATUS_EDGES, REF_STATUS_BAD = 16, 32, 2
 EPI_PLAIN, EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU = 0, 1, 2, 3
 TUNE_BIN_THREADS, TUNE_LNPHI_BLOCK, TUNE_GEMM_BM = 0, 1, 2
 PSI2_STRIDE = 8  # SGL_PSI2_STRIDE: batched two-point sub-grid stride
@@ -75,6 +80,21 @@ def load(path: str = LIB_PATH):
     L.sgl_get_tuning.argtypes = [i32]
     L.sgl_psi2_nodes.restype = i32
     L.sgl_psi2_nodes.argtypes = [sp]
+    f64, i64 = ctypes.c_double, ctypes.c_int64
+    L.sgl_ref_shell_count.restype = i32
+    L.sgl_ref_shell_count.argtypes = [f64]
+    L.sgl_ref_tasks.restype = i32
+    L.sgl_ref_tasks.argtypes = [i32]
+    L.sgl_ref_tables_bytes.restype = sz
+    L.sgl_ref_tables_bytes.argtypes = [i32]
+    L.sgl_ref_roots_bytes.restype = sz
+    L.sgl_ref_roots_bytes.argtypes = []
+    L.sgl_ref_bin_bytes.restype = sz
+    L.sgl_ref_bin_bytes.argtypes = [i64]
+    L.sgl_ref_phi_bytes.restype = sz
+    L.sgl_ref_phi_bytes.argtypes = [i32, i32, i32]
+    L.sgl_ref_p3_bytes.restype = sz
+    L.sgl_ref_p3_bytes.argtypes = [i32, i32, i32, i32]
     sigs = {
         "sgl_tables_build": [vp, i32, sp, vp, vp, vp],
         "sgl_lnphi_exact": [vp, i32, sp, vp, vp, vp, vp],
@@ -99,6 +119,16 @@ def load(path: str = LIB_PATH):
         "sgl_set_tuning": [i32, i32],
         "sgl_psi2_interp": [vp, i32, sp, vp, vp],
         "sgl_trunc_nfw_kg": [vp, i32, ctypes.c_double, ctypes.c_double, i32, vp, vp, vp],
+        "sgl_ref_tables_build": [vp, i32, vp, vp, vp],
+        "sgl_ref_cells_count": [vp, i32, vp, vp, vp, vp],
+        "sgl_ref_cells_emit": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp],
+        "sgl_ref_bin": [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+        "sgl_ref_lam_axis": [vp, vp, vp, i32, i32, vp, vp],
+        "sgl_ref_phi": [vp, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp],
+        "sgl_ref_p3": [vp, i32, i32, f64, f64, f64, f64, i32, f64, i32, vp, vp, vp],
+        "sgl_ref_arcs": [vp, f64, f64, i32, f64, i32, f64, vp, i32, vp, vp, vp],
+        "sgl_ref_moments": [vp, vp, i32, vp, vp],
+        "sgl_ref_gemm_f64": [i32, i32, i32, vp, i64, vp, i64, vp, i64, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -44,7 +44,12 @@ __global__ void k_fill_uedges(const sgl_shape sh, double* __restrict__ ue, doubl
     double l0 = log10(sh.k_lo), l1 = log10(sh.k_hi);
     kn[t] = pow(10.0, l0 + (l1 - l0) * t / (sh.n_k - 1));
   }
-  if (m_nodes_h != nullptr && t < sh.n_m) {
+  if (m_nodes_h != nullptr && t < sh.n_m && (sh.flags & SGL_FLAG_REF_GRID)) {
+    // engine mass node t + 1 of exp(linspace(ln 1e7, ln 1e17, 100)) Msun; the
+    // last node is exp(ln 1e17), as linspace ends exactly at its stop value
+    const double e0 = log(1e7), e1 = log(1e17);
+    m_nodes_h[t] = exp((t + 1 == 99) ? e1 : e0 + (t + 1) * ((e1 - e0) / 99.0));
+  } else if (m_nodes_h != nullptr && t < sh.n_m) {
     double e0 = log(sh.m_lo_h), e1 = log(sh.m_hi_h);
     double a = e0 + (e1 - e0) * t / sh.n_m, b = e0 + (e1 - e0) * (t + 1) / sh.n_m;
     m_nodes_h[t] = exp(0.5 * (a + b));

@@ -10,6 +10,11 @@
 namespace sgl {
 
 constexpr int kMaxSlices = 96;
+// Reference grid (SGL_FLAG_REF_GRID): up to 104 shells (the 103 engine nodes
+// below z_s = 12.34 plus the last shell) and the engine's 99 mass nodes.
+constexpr int kRefMaxShells = 104;
+constexpr int kRefMasses = 99;
+constexpr int kGrowthMaxNodes = 112;   // slice nodes the growth kernel stages in LDS
 constexpr int kCovMaxModes = 2048;
 constexpr int kStatusCovModes = 1;      // covariance mode count clamped
 constexpr int kStatusBadShape = 2;
@@ -183,6 +188,9 @@ extern int g_sgl_tune[SGL_TUNE_COUNT];
 int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes_h,
                  double* derived, double* slices, double* mass, double* rows, double* frows,
                  double* cov, double* sigtab, int* status, hipStream_t st);
+int ref_tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes,
+                     double* derived, double* slices, double* mass, double* rows, double* frows,
+                     double* sigtab, hipStream_t st);
 int launch_fill_grids(const sgl_shape& sh, double* ue, double* kn, double* m_nodes_h,
                       hipStream_t st);
 int launch_lnphi_exact(const double* rows, const double* frows, const double* cosmo, const double* uedge,

@@ -119,6 +119,92 @@ __global__ __launch_bounds__(kBgThreads) void k_background(const Cosmo* __restri
   }
 }
 
+// Reference grid (SGL_FLAG_REF_GRID): shells whose edges are the engine's
+// log-z nodes z_j = exp(ln 0.01 + j (ln 12.3412 - ln 0.01) / 102) below z_s,
+// plus 0 and z_s, one midpoint node each (sgl_full.py:471-518, zint="midpoint").
+// The caller stores the shell count (nodes below z_s + 1) in the record's n_z
+// slot.  Slice record: z = z_lo + (z_hi - z_lo)/2, chi(z), the comoving width
+// dchi = c/H(z) dz (the Limber node width of sgl_full.py:1684), dchi (1+z)^2
+// (the lens weight of sgl_full.py:1860 without its pi), Sigma_cr, E^2; D and
+// Dg are filled by the growth kernel.
+constexpr double kZminEng = 0.01, kZmaxEng = 12.341169644129371;
+constexpr int kNzEng = 103;
+__device__ inline double engine_z(int j) {
+  const double l0 = log(kZminEng), l1 = log(kZmaxEng);
+  return exp((j == kNzEng - 1) ? l1 : l0 + j * ((l1 - l0) / (kNzEng - 1)));
+}
+
+__global__ __launch_bounds__(kBgThreads) void k_ref_background(const Cosmo* __restrict__ cosmo,
+                                                               int n_z_max,
+                                                               Derived* __restrict__ der,
+                                                               double* __restrict__ slices) {
+  __shared__ double chi_e[kPanels + 1];
+  __shared__ double run[kBgThreads];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const Cosmo c = cosmo[b];
+  const double Or = (c.rad_none != 0.0) ? 0.0 : c.Om / (1.0 + c.zeq);
+  const double OL = 1.0 - c.Om - c.Ok - Or;
+  const double H0 = 100.0 * c.h;
+  const double dh = kCkms / H0;
+  const double zs = c.zs;
+  const int n_z = (int)c.n_z;
+  const double dz = zs / kPanels;
+  for (int p = t; p < kPanels; p += kBgThreads)
+    chi_e[p + 1] = dh * inv_e_integral(c, Or, OL, p * dz, (p + 1) * dz);
+  __syncthreads();
+  double loc[kBgPer], acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < kBgPer; ++j) {
+    acc += chi_e[t * kBgPer + j + 1];
+    loc[j] = acc;
+  }
+  run[t] = acc;
+  __syncthreads();
+  for (int off = 1; off < kBgThreads; off <<= 1) {
+    const double add = (t >= off) ? run[t - off] : 0.0;
+    __syncthreads();
+    run[t] += add;
+    __syncthreads();
+  }
+  const double base = (t > 0) ? run[t - 1] : 0.0;
+#pragma unroll
+  for (int j = 0; j < kBgPer; ++j) chi_e[t * kBgPer + j + 1] = base + loc[j];
+  if (t == 0) chi_e[0] = 0.0;
+  __syncthreads();
+  const double chis = chi_e[kPanels];
+  const double DAs = f_k(c.Ok, H0, chis) / (1.0 + zs);
+  if (t == 0) {
+    Derived d = {};
+    d.Or = Or; d.OL = OL; d.H0 = H0; d.rhoc0 = kRhoc0Unit * c.h * c.h;
+    d.rhom = c.Om * d.rhoc0; d.chis = chis; d.DAs = DAs; d.dchi = 0.0; d.n_z = n_z;
+    der[b] = d;
+  }
+  for (int i = t; i < n_z_max; i += kBgThreads) {
+    double* S = slices + ((size_t)b * n_z_max + i) * kSliceStride;
+    if (i >= n_z) {
+      for (int q = 0; q < kSliceStride; ++q) S[q] = 0.0;
+      continue;
+    }
+    const double zlo = (i == 0) ? 0.0 : engine_z(i - 1);
+    const double zhi = (i == n_z - 1) ? zs : engine_z(i);
+    const double z = zlo + 0.5 * (zhi - zlo) * 1.0, dzs = 0.5 * (zhi - zlo) * 2.0;
+    const int lo = min((int)(z / dz), kPanels - 1);
+    const double chi = chi_e[lo] + dh * inv_e_integral(c, Or, OL, lo * dz, z);
+    const double E2 = e2_of_z(c, Or, OL, z);
+    const double DAl = f_k(c.Ok, H0, chi) / (1.0 + z);
+    const double DAls = f_k(c.Ok, H0, chis - chi) / (1.0 + zs);
+    const double dchi = kCkms / (H0 * sqrt(E2)) * dzs;
+    S[kSlZ] = z;
+    S[kSlChi] = chi;
+    S[kSlDchi] = dchi;
+    S[kSlWz] = dchi * (1.0 + z) * (1.0 + z);
+    S[kSlD] = 0.0;
+    S[kSlScr] = kCkms * kCkms * DAs / (4.0 * kPi * kGnewt * DAl * DAls);
+    S[kSlE2] = E2;
+    S[kSlDg] = 0.0;
+  }
+}
+
 // ---------------------------------------------------------------- growth --
 // D'' + (2 + dlnH/dlna) D' - 3/2 Omega_m(a) D = 0 in x = ln a (sgl.py:310-347),
 // classical RK4 on kGrowthSteps uniform steps from a = 1e-6, cubic Hermite
@@ -207,7 +293,7 @@ __device__ inline double cpt_growth(double z, double Om, double Ok, double Or, d
 __global__ __launch_bounds__(64) void k_growth(const Cosmo* __restrict__ cosmo, int n_z_max,
                                                Derived* __restrict__ der,
                                                double* __restrict__ slices) {
-  __shared__ double xs[kMaxSlices];
+  __shared__ double xs[kGrowthMaxNodes];
   const int b = blockIdx.x, L = threadIdx.x;
   const Cosmo c = cosmo[b];
   const Derived d = der[b];
@@ -323,7 +409,7 @@ static_assert(kSigmaK % kSigChunk == 0, "whole chunks");
 // accumulators (the FP64 add latency, not the issue rate, bounds one chain).
 __global__ __launch_bounds__(256) void k_sigma(const Cosmo* __restrict__ cosmo,
                                                const double* __restrict__ m_nodes_h, int n_m,
-                                               Derived* __restrict__ der,
+                                               int m_in_msun, Derived* __restrict__ der,
                                                double* __restrict__ mass) {
   __shared__ double g[kSigChunk];
   __shared__ double k6[kSigChunk];
@@ -350,7 +436,7 @@ __global__ __launch_bounds__(256) void k_sigma(const Cosmo* __restrict__ cosmo,
   double aR = 0.0;
   int qcut = kSigmaK;   // first node with y = aR k^6 > kSigYCut
   if (j < n_m) {
-    const double M = m_nodes_h[j] / c.h;
+    const double M = m_in_msun ? m_nodes_h[j] : m_nodes_h[j] / c.h;
     const double R = cbrt(3.0 * M / (4.0 * kPi * rhom));
     const double R2 = R * R;
     aR = a6 * R2 * R2 * R2;
@@ -419,7 +505,7 @@ __global__ __launch_bounds__(256) void k_sigma(const Cosmo* __restrict__ cosmo,
   const double Anorm = c.s8 / sqrt(red[0]);
   if (t == 0) der[b].Anorm = Anorm;
   if (gq == 0 && j < n_m) {
-    const double M = m_nodes_h[j] / c.h;
+    const double M = m_in_msun ? m_nodes_h[j] : m_nodes_h[j] / c.h;
     double S = 0.0, Sd = 0.0;
     for (int q = 0; q < G; ++q) {
       S += pS[q * nm2 + j];
@@ -922,7 +1008,7 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
   hipLaunchKernelGGL(k_growth, dim3(B), dim3(64), 0, sg, C, sh.n_z_max, D, slices);
   if (S) ck(hipEventRecord(S->j_growth, S->s));
   // sigma(M) needs the background only; it writes A_norm and the mass table
-  hipLaunchKernelGGL(k_sigma, dim3(B), dim3(256), 0, st, C, m_nodes_h, sh.n_m, D, mass);
+  hipLaunchKernelGGL(k_sigma, dim3(B), dim3(256), 0, st, C, m_nodes_h, sh.n_m, 0, D, mass);
   const bool ell = (sh.flags & SGL_FLAG_ELL) != 0;
   if (ell) hipLaunchKernelGGL(k_mstar_table, dim3(B), dim3(256), 0, st, C, D, sigtab);
   // the covariance needs chi and A_norm (background, sigma), not the growth
@@ -948,6 +1034,29 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
                      (sh.flags & SGL_FLAG_FILAMENTS) ? frows : (double*)nullptr);
   if (S && cov_on) ck(hipStreamWaitEvent(st, S->j_cov, 0));
   if (fail) return SGL_ERR_HIP;
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+// Tables on the reference grid (SGL_FLAG_REF_GRID): the reference-grid
+// background, then the growth, sigma(M) (masses in Msun), M* and the rows
+// kernels of the kernel-shape tables, with dlnM the engine grid's step.
+int ref_tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes,
+                     double* derived, double* slices, double* mass, double* rows, double* frows,
+                     double* sigtab, hipStream_t st) {
+  const Cosmo* C = reinterpret_cast<const Cosmo*>(cosmo);
+  Derived* D = reinterpret_cast<Derived*>(derived);
+  const double dlnM = (log(1e17) - log(1e7)) / 99.0;
+  hipLaunchKernelGGL(k_ref_background, dim3(B), dim3(kBgThreads), 0, st, C, sh.n_z_max, D, slices);
+  hipLaunchKernelGGL(k_growth, dim3(B), dim3(64), 0, st, C, sh.n_z_max, D, slices);
+  hipLaunchKernelGGL(k_sigma, dim3(B), dim3(256), 0, st, C, m_nodes, sh.n_m, 1, D, mass);
+  const bool ell = (sh.flags & SGL_FLAG_ELL) != 0;
+  if (ell) hipLaunchKernelGGL(k_mstar_table, dim3(B), dim3(256), 0, st, C, D, sigtab);
+  const double a_st = 1.0 / (1.0 + std::pow(2.0, -kPst) * std::tgamma(0.5 - kPst) / std::sqrt(kPi));
+  const double dc0 = 0.6 * std::pow(1.5 * kPi, 2.0 / 3.0);
+  hipLaunchKernelGGL(k_rows, dim3(sh.n_z_max, B), dim3(128), 0, st, C, D, slices, mass,
+                     sh.n_z_max, sh.n_m, dlnM, a_st, dc0, (sh.flags & SGL_FLAG_EDGE_RVIR) ? 1 : 0,
+                     ell ? (const double*)sigtab : (const double*)nullptr, rows,
+                     (sh.flags & SGL_FLAG_FILAMENTS) ? frows : (double*)nullptr);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 

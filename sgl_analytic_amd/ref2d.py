@@ -1,0 +1,284 @@
+"""The reference's own 2-D (k_kappa, k_gamma) chain on the GPU.
+
+This is the chain ``pipeline.run`` executes (sgl_analytic/pipeline.py:114-274)
+for the ingredient sets built here (halos, ellipticity, filaments), with the
+reference's defaults: edge="r200", kap_floor=1e-8, zint="midpoint",
+phi_tol=1e-8, tail box, no window, n_xi_wide=500.
+
+Host side (this module): the scalar grid rules of ``invert``
+(sgl_full.py:159-270, ``sgl.vector_grid`` sgl.py:1229-1258) and the k-reach
+search (``_reach``, sgl_full.py:38-47), which read back a handful of
+device scalars.  Device side (libsgl_amd.so, include/sgl_amd.h ``sgl_ref_*``):
+the tables on the reference's shell and mass grids, the adaptive cells with
+their critical-curve refinement, the binning into the fine (kappa, gamma)
+lattice, Lambda on the reach probes and on the full tensor grid (FP64 MFMA),
+the Hankel + Fourier inversion (FP64 MFMA), the constant-mu arc integral and
+the normalisation.  There is no CPU fallback: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import FLAG_ELL, FLAG_FILAMENTS, REF_FINE_BINS, check
+
+# invert / vector_grid constants (sgl.py:1176-1186, sgl_full.py:35-37, pipeline.py:117)
+SIG_REACH, PPP_K, PPP_G, BOX_REACH = 12.0, 18.0, 96.0, 6.0
+PTS_PER_SIG_KAP, PTS_PER_SIG_GAM, XI_REACH = 72.0, 36.0, 24.0
+TAIL_KAP_MAX, TAIL_GAM_MAX, TAIL_PPP_G = 0.99, 0.8, 12.0
+PHI_TOL = 1e-8
+N_XI_WIDE = 500
+OUT_GRID_MAX = 3.4e7             # sgl_full.py:60 (geometric output axis above it)
+E_BLOCK_BYTES = 4 << 30          # Lambda operand rows per call (k_kappa row blocks)
+INGREDIENTS_2D = ("ell", "fil")  # ingredient sets this chain computes
+
+
+def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _records(cosmo: dict, zs: float, n_z: int) -> np.ndarray:
+    rec = np.zeros(_lib.COSMO_STRIDE)
+    rec[:10] = [cosmo["h"], cosmo["Om"], cosmo["s8"], cosmo["Ob"], cosmo["ns"], cosmo["zeq"],
+                cosmo.get("Ok", 0.0), cosmo.get("w0", -1.0), cosmo.get("wa", 0.0), zs]
+    rec[10] = n_z
+    rec[_lib.REC_RADIATION] = 1.0 if cosmo.get("radiation", "engine") is None else 0.0
+    rec[_lib.REC_GROWTH] = 1.0 if cosmo.get("growth_mode", "ode") == "legacy" else 0.0
+    return rec
+
+
+@dataclass
+class Population:
+    """Occupied fine bins on the device ([w][kappa centroid][gamma centroid],
+    REF_FINE_BINS each) and the builder's bookkeeping (sgl_full.py:1535-1808,
+    2742-2755)."""
+    pop: torch.Tensor
+    npop_dev: torch.Tensor
+    n_bins: int
+    sigma: float
+    kbar: float
+    n_points: int
+    N_expl: float
+    N_fil: float
+    timings: dict = field(default_factory=dict)
+
+
+def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False) -> Population:
+    """build_population(..., kap_floor=1e-8, trunc="r200", zint="midpoint")
+    + Lam's flattening (sgl_full.py:1535, 2742-2755), on the device."""
+    L = _lib.load()
+    dev = torch.device("cuda")
+    flags = (FLAG_ELL if ell else 0) | (FLAG_FILAMENTS if fil else 0)
+    n_z = int(L.sgl_ref_shell_count(float(zs)))
+    if n_z < 1 or n_z > _lib.REF_MAX_SHELLS:
+        raise ValueError("z_s outside the reference grid's shells")
+    st = _stream()
+    rec = torch.tensor(_records(cosmo, float(zs), n_z), dtype=torch.float64, device=dev)
+    T = torch.empty(L.sgl_ref_tables_bytes(flags) // 8, dtype=torch.float64, device=dev)
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    check(L.sgl_ref_tables_build(_ptr(rec), flags, _ptr(T), _ptr(status), st), "sgl_ref_tables_build")
+    roots = torch.empty(L.sgl_ref_roots_bytes() // 8, dtype=torch.float64, device=dev)
+    ntask = int(L.sgl_ref_tasks(flags))
+    counts = torch.empty(ntask + 1, dtype=torch.int64, device=dev)
+    check(L.sgl_ref_cells_count(_ptr(T), flags, _ptr(roots), _ptr(counts), _ptr(status), st),
+          "sgl_ref_cells_count")
+    n = int(counts[ntask].item())
+    if n < 1:
+        raise _lib.SglError(f"reference population is empty (status {int(status.item())})")
+    K = torch.empty(n, dtype=torch.float64, device=dev)
+    G = torch.empty_like(K)
+    W = torch.empty_like(K)
+    key = torch.empty(n, dtype=torch.int32, device=dev)
+    check(L.sgl_ref_cells_emit(_ptr(T), flags, _ptr(roots), _ptr(counts), _ptr(K), _ptr(G), _ptr(W),
+                               _ptr(key), _ptr(status), st), "sgl_ref_cells_emit")
+    ws = torch.empty(L.sgl_ref_bin_bytes(n), dtype=torch.uint8, device=dev)
+    pop = torch.empty(3 * REF_FINE_BINS, dtype=torch.float64, device=dev)
+    npop = torch.zeros(1, dtype=torch.int32, device=dev)
+    scal = torch.zeros(2, dtype=torch.float64, device=dev)
+    check(L.sgl_ref_bin(n, _ptr(key), _ptr(K), _ptr(G), _ptr(W), _ptr(ws), _ptr(pop), _ptr(npop),
+                        _ptr(scal), st), "sgl_ref_bin")
+    s = int(status.item())
+    if s:
+        raise _lib.SglError(f"reference population status {s} (16: critical-curve roots beyond the "
+                            "slots, 32: edges beyond the per-row capacity, 2: bad record)")
+    del K, G, W, key, ws
+    sc = scal.cpu().numpy()
+    # bookkeeping of the info record (sgl_full.py:2039, 1992): expected counts
+    lay_rows = _row_table(T, flags, n_z)
+    return Population(pop=pop, npop_dev=npop, n_bins=int(npop.item()), sigma=math.sqrt(sc[0]),
+                      kbar=float(sc[1]), n_points=n, N_expl=lay_rows[0], N_fil=lay_rows[1] if fil else 0.0)
+
+
+def _row_table(T: torch.Tensor, flags: int, n_z: int):
+    """N_expl = sum over rows of pi w_row (x_m r_s)^2 and N_fil = sum of
+    pi w_fil r_sF^2, read from the device rows (sgl_full.py:2039, 1992)."""
+    # tables layout: grids (m_nodes 99, uedge 2, knodes 2, status 1), cosmo,
+    # derived, slices, mass, rows, frows (each rounded to 32 doubles)
+    r32 = lambda v: (v + 31) // 32 * 32
+    S, NM = _lib.REF_MAX_SHELLS, _lib.REF_MASSES
+    o = r32(NM) + r32(2) + r32(2) + r32(1) + r32(16) + r32(16) + r32(S * 8) + r32(NM * 4)
+    rows = T[o:o + S * NM * 8].view(S, NM, 8)[:n_z].cpu().numpy()
+    ks, rs, xt, w = rows[..., 0], rows[..., 1], rows[..., 2], rows[..., 4]
+    mut = np.log(1 + xt) - xt / (1 + xt)
+    g = ks * 4 * mut / xt ** 2
+    xm = np.where(g > 1e-8, xt * np.sqrt(np.maximum(g, 0) / 1e-8), xt)
+    n_expl = float(np.sum(math.pi * w * (xm * rs) ** 2))
+    n_fil = 0.0
+    if flags & FLAG_FILAMENTS:
+        of = o + r32(S * NM * 8)
+        fr = T[of:of + S * NM * 4].view(S, NM, 4)[:n_z].cpu().numpy()
+        n_fil = float(np.sum(math.pi * fr[..., 2] * fr[..., 1] ** 2))
+    return n_expl, n_fil
+
+
+def lam_axis(pop: Population, k: np.ndarray, axis: int) -> np.ndarray:
+    """Lambda(k, 0) (axis 0) or Lambda(0, k) (axis 1) at the probe points."""
+    L = _lib.load()
+    kd = torch.tensor(np.asarray(k, float), dtype=torch.float64, device="cuda")
+    out = torch.empty(2 * kd.numel(), dtype=torch.float64, device="cuda")
+    check(L.sgl_ref_lam_axis(_ptr(pop.pop), _ptr(pop.npop_dev), _ptr(kd), kd.numel(), axis,
+                             _ptr(out), _stream()), "sgl_ref_lam_axis")
+    o = out.cpu().numpy()
+    return o[0::2] + 1j * o[1::2]
+
+
+def vector_grid(sigma: float) -> dict:
+    """sgl.vector_grid (sgl.py:1229-1258)."""
+    sig_xi = 2.0 * sigma
+    xi_max = XI_REACH * sig_xi
+    xi_min = -BOX_REACH * sig_xi
+    kap_max = 1.0 - math.exp(-0.5 * xi_max)
+    kap_min = -BOX_REACH * sigma
+    gam_max = BOX_REACH * sigma
+    k_max = SIG_REACH / sigma
+    kap_ext = max(abs(kap_min), kap_max)
+    dk_k = 2.0 * math.pi / (PPP_K * kap_ext)
+    dk_g = 2.0 * math.pi / (PPP_G * gam_max)
+    return dict(nk_k=int(math.ceil(k_max / dk_k)) + 1, nk_g=int(math.ceil(k_max / dk_g)) + 1,
+                k_k_max=k_max, k_g_max=k_max,
+                n_kap=int(math.ceil((kap_max - kap_min) / (sigma / PTS_PER_SIG_KAP))) + 1,
+                n_gam=int(math.ceil(gam_max / (sigma / PTS_PER_SIG_GAM))) + 1,
+                kap_min=kap_min, kap_max=kap_max, gam_max=gam_max, xi_min=xi_min,
+                xi_max=xi_max, sigma=sigma)
+
+
+def reach(pop: Population, k0: float, axis: int, tol: float = PHI_TOL):
+    """_reach (sgl_full.py:38-47): the first of k0 * 1.25^n, n = 0..19, with
+    |Phi| <= tol on the axis, else the smallest |Phi|."""
+    mults = 1.25 ** np.arange(0, int(np.ceil(np.log(64.0) / np.log(1.25))) + 1)
+    ks = k0 * mults
+    phi = np.abs(np.exp(lam_axis(pop, ks, axis)))
+    ok = np.nonzero(phi <= tol)[0]
+    i = int(ok[0]) if len(ok) else int(np.argmin(phi))
+    return float(ks[i]), float(phi[i])
+
+
+def grid_plan(pop: Population, tol: float = PHI_TOL) -> dict:
+    """The k and (kappa, gamma) grids of invert with the tail box
+    (sgl_full.py:190-262)."""
+    sigma = pop.sigma
+    grid = vector_grid(sigma)
+    g0 = grid["gam_max"]
+    kap_max = max(grid["kap_max"], TAIL_KAP_MAX)
+    gam_max = max(g0, TAIL_GAM_MAX)
+    dk0 = (2 * math.pi / (PPP_K * max(abs(grid["kap_min"]), grid["kap_max"])),
+           2 * math.pi / (PPP_G * grid["gam_max"]))
+    dk0 = (dk0[0], min(dk0[1], 2 * math.pi / (TAIL_PPP_G * gam_max)))
+    if kap_max != grid["kap_max"] or gam_max != grid["gam_max"]:
+        grid["kap_max"], grid["gam_max"] = float(kap_max), float(gam_max)
+        grid["n_kap"] = int(math.ceil((grid["kap_max"] - grid["kap_min"])
+                                      / (sigma / PTS_PER_SIG_KAP))) + 1
+        grid["n_gam"] = int(math.ceil(grid["gam_max"] / (sigma / PTS_PER_SIG_GAM))) + 1
+    grid["k_k_max"], phi_k = reach(pop, grid["k_k_max"], 0, tol)
+    grid["k_g_max"], phi_g = reach(pop, grid["k_g_max"], 1, tol)
+    grid["nk_k"] = int(math.ceil(grid["k_k_max"] / dk0[0])) + 1
+    grid["nk_g"] = int(math.ceil(grid["k_g_max"] / dk0[1])) + 1
+    grid.update(phi_at_k_cut=phi_k, phi_at_g_cut=phi_g, window=0.0)
+    if grid["n_kap"] * grid["n_gam"] > OUT_GRID_MAX:
+        raise NotImplementedError("the geometric output axis of the narrowest low-z_s grids "
+                                  "(n_kap * n_gam > 3.4e7, sgl_full.py:316-324) is not built")
+    return grid
+
+
+def phi_grid(pop: Population, k_k: np.ndarray, k_g: np.ndarray) -> torch.Tensor:
+    """Phi = exp(Lambda) on the tensor grid as [2 nk_k][nk_g] (rows re, im)."""
+    L = _lib.load()
+    dev = torch.device("cuda")
+    nk, ng = k_k.size, k_g.size
+    phi = torch.empty((2 * nk, ng), dtype=torch.float64, device=dev)
+    kg = torch.tensor(k_g, dtype=torch.float64, device=dev)
+    lda = (pop.n_bins + 63) // 64 * 64
+    blk = max(1, min(nk, 65535, E_BLOCK_BYTES // (16 * lda)))
+    ws = torch.empty(L.sgl_ref_phi_bytes(blk, ng, pop.n_bins) // 8, dtype=torch.float64, device=dev)
+    for lo in range(0, nk, blk):
+        hi = min(nk, lo + blk)
+        kk = torch.tensor(k_k[lo:hi], dtype=torch.float64, device=dev)
+        check(L.sgl_ref_phi(_ptr(pop.pop), _ptr(pop.npop_dev), pop.n_bins, _ptr(kk), hi - lo,
+                            _ptr(kg), ng, None, _ptr(ws), _ptr(phi[2 * lo:]), _stream()),
+              "sgl_ref_phi")
+    return phi
+
+
+@dataclass
+class Result:
+    xi: np.ndarray
+    P: np.ndarray
+    sigma: float
+    mean: float
+    var: float
+    skew: float
+    sigmaDL: float
+    norm: float
+    grid: dict
+    pop: Population = None
+
+
+def invert(pop: Population, xi_out: np.ndarray, tol: float = PHI_TOL) -> Result:
+    """sgl_full.invert (sgl_full.py:159-375), Gaussian closure, no window."""
+    L = _lib.load()
+    dev = torch.device("cuda")
+    g = grid_plan(pop, tol)
+    k_k = np.linspace(0.0, g["k_k_max"], g["nk_k"])
+    k_g = np.linspace(0.0, g["k_g_max"], g["nk_g"])
+    phi = phi_grid(pop, k_k, k_g)
+    nk, ng, nkap, ngam = g["nk_k"], g["nk_g"], g["n_kap"], g["n_gam"]
+    ws = torch.empty(L.sgl_ref_p3_bytes(nk, ng, nkap, ngam) // 8, dtype=torch.float64, device=dev)
+    p3 = torch.empty((nkap, ngam), dtype=torch.float64, device=dev)
+    st = _stream()
+    check(L.sgl_ref_p3(_ptr(phi), nk, ng, g["k_k_max"], g["k_g_max"], g["kap_min"], g["kap_max"],
+                       nkap, g["gam_max"], ngam, _ptr(ws), _ptr(p3), st), "sgl_ref_p3")
+    del phi, ws
+    g_def = min(BOX_REACH * pop.sigma, g["gam_max"])
+    xw = np.linspace(g["xi_min"], g["xi_max"], N_XI_WIDE)
+    xwd = torch.tensor(xw, dtype=torch.float64, device=dev)
+    pw = torch.empty(N_XI_WIDE, dtype=torch.float64, device=dev)
+    check(L.sgl_ref_arcs(_ptr(p3), g["kap_min"], g["kap_max"], nkap, g["gam_max"], ngam, g_def,
+                         _ptr(xwd), N_XI_WIDE, None, _ptr(pw), st), "sgl_ref_arcs")
+    mom = torch.empty(5, dtype=torch.float64, device=dev)
+    check(L.sgl_ref_moments(_ptr(xwd), _ptr(pw), N_XI_WIDE, _ptr(mom), st), "sgl_ref_moments")
+    xo = np.ascontiguousarray(xi_out, dtype=np.float64)
+    xod = torch.tensor(xo, dtype=torch.float64, device=dev)
+    P = torch.empty(xo.size, dtype=torch.float64, device=dev)
+    check(L.sgl_ref_arcs(_ptr(p3), g["kap_min"], g["kap_max"], nkap, g["gam_max"], ngam, g_def,
+                         _ptr(xod), xo.size, _ptr(mom), _ptr(P), st), "sgl_ref_arcs")
+    m = mom.cpu().numpy()
+    return Result(xo, P.cpu().numpy(), pop.sigma, float(m[1]), float(m[2]), float(m[3]),
+                  float(m[4]), float(m[0]), g, pop)
+
+
+def run(zs: float, cosmo: dict, ell: bool = False, fil: bool = False, xi_out=None) -> Result:
+    """One PDF on the reference's chain; xi_out None: the CLI body and tail
+    grids linspace(-1, 1, 1601) and linspace(1.005, 9, 1600) (pipeline.py:211-215)."""
+    pop = build(cosmo, zs, ell=ell, fil=fil)
+    if xi_out is None:
+        xi_out = np.concatenate([np.linspace(-1.0, 1.0, 1601), np.linspace(1.005, 9.0, 1600)])
+    return invert(pop, xi_out)
