@@ -282,6 +282,134 @@ sgl_status sgl_diagnostics(const void* p, int32_t p_is_double, int32_t B, const 
                            double* diag, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Reference-shape path: the reference's own 2-D (k_kappa, k_gamma) chain on one
+ * instance, with the reference's grids (shells of sgl_full.z_shells, the
+ * engine mass grid, adaptive impact-parameter cells, the 961 x 300 fine
+ * lattice, width-relative k and (kappa, gamma) grids).  The host drives it in
+ * the order of pipeline.run (sgl_analytic/pipeline.py:165-241): tables ->
+ * cells (count, then emit) -> binning -> reach probes (sgl_ref_lam_axis) ->
+ * Phi on the tensor grid -> P3 -> arc integrals and normalisation.  Grid
+ * sizes depend on device results (cell count, sigma, reach), so the host reads
+ * those scalars between calls.
+ */
+#define SGL_REF_MAX_SHELLS 104            /* 103 engine nodes below z_s + the last shell */
+#define SGL_REF_MASSES 99                 /* engine mass nodes 1..99 */
+#define SGL_REF_FINE_BINS (961 * 300)     /* fine (kappa, gamma) lattice, sgl_full.py:456-468 */
+
+/* Shell count of z_s (engine nodes below z_s, plus one); the caller stores it in
+ * the record's n_z slot.  Host function. */
+int32_t sgl_ref_shell_count(double z_s);
+/* (row, orientation) tasks of the cell kernels: 104 x 99 x (12 with SGL_FLAG_ELL, else 1). */
+int32_t sgl_ref_tasks(int32_t flags);
+size_t sgl_ref_tables_bytes(int32_t flags);    /* flags: SGL_FLAG_ELL | SGL_FLAG_FILAMENTS |
+                                                  SGL_FLAG_EDGE_RVIR (x_t = r_vir / r_s) */
+size_t sgl_ref_roots_bytes(void);
+
+/*
+ * Tables of one instance on the reference grid (cosmo: one 16-double record,
+ * device; status zeroed, then 2 if the record's z_s or shell count is invalid).
+ * Replaces: pipeline.make_cosmology (pipeline.py:36-72), z_shells
+ * (sgl_full.py:480-518, zint="midpoint") and the per-(z, M) quantities of
+ * _build_shell_into (sgl_full.py:1848-1884).
+ */
+sgl_status sgl_ref_tables_build(const double* cosmo, int32_t flags, void* tables, int32_t* status,
+                                void* stream);
+
+/*
+ * Cells, first pass: per row the critical-curve roots (roots buffer), per
+ * (row, orientation) task the cell count; counts (int64[tasks + 1]) is left
+ * holding the exclusive offsets, counts[tasks] = total cells.  status |= 16
+ * (more roots than the per-row slots) or 32 (more edges than a task holds).
+ * Replaces: _refine_edges / _refine_edges_eps_batch (sgl_full.py:2342-2372,
+ * 2279-2339) and the grid of _truncated_host (sgl_full.py:2025-2050).
+ */
+sgl_status sgl_ref_cells_count(const void* tables, int32_t flags, double* roots, int64_t* counts,
+                               int32_t* status, void* stream);
+/*
+ * Cells, second pass: kappa, gamma (clipped below at 1.0000001e-9), weight and
+ * the fine-lattice key (bin, or SGL_REF_FINE_BINS outside) of every cell, in
+ * the reference's emission order (per row: the inner cells orientation by
+ * orientation, the monopole tail, the filament cells).
+ * Replaces: _truncated_host (sgl_full.py:2025-2138), the filament block of
+ * _build_shell_into (sgl_full.py:1984-2007) and the bin search of
+ * _add_direct_kernel (sgl_full.py:971-1046).
+ */
+sgl_status sgl_ref_cells_emit(const void* tables, int32_t flags, const double* roots,
+                              const int64_t* offsets, double* kap, double* gam, double* w,
+                              int32_t* key, int32_t* status, void* stream);
+
+/*
+ * Binning: stable sort by key, per-bin sums of w, w kappa, w gamma in the cells'
+ * order, the occupied bins compacted in bin order into pop = [w][kappa
+ * centroid][gamma centroid] (SGL_REF_FINE_BINS doubles each), npop = their
+ * count, scal = [sum w kappa^2 over the bins, sum W kappa over every cell]
+ * (sigma_kappa^2 without clustering, and kbar).  ws: sgl_ref_bin_bytes(n_cells).
+ * Replaces: Cells.add (sgl_full.py:1314-1383) and the flattening of Lam.__init__
+ * (sgl_full.py:2742-2755), Lam.sigma (sgl_full.py:3437-3452) without clustering.
+ */
+size_t sgl_ref_bin_bytes(int64_t n_cells);
+sgl_status sgl_ref_bin(int64_t n_cells, const int32_t* key, const double* kap, const double* gam,
+                       const double* w, void* ws, double* pop, int32_t* npop, double* scal,
+                       void* stream);
+
+/*
+ * Lambda on an axis at n probe points k (out: [n][2]): axis 0 Lambda(k, 0),
+ * axis 1 Lambda(0, k).  Replaces: the Lam calls of _reach (sgl_full.py:38-47).
+ */
+sgl_status sgl_ref_lam_axis(const double* pop, const int32_t* npop, const double* k, int32_t n,
+                            int32_t axis, double* out, void* stream);
+
+/*
+ * Phi = exp(Lambda + psi) on the tensor grid kk[nk] x kg[ng], written as
+ * phi[2 nk][ng] (row 2n real, 2n + 1 imaginary part of k_kappa row n); psi
+ * (complex [nk][ng], may be NULL) is the two-point term.  Lambda's dense part
+ * is an FP64 MFMA contraction.  nbins: the host's copy of *npop;
+ * ws: sgl_ref_phi_bytes(nk, ng, nbins).
+ * Replaces: Lam.__call__ / _direct_call / _sep_raw (sgl_full.py:3156-3160,
+ * 3088-3131, 3040-3086), sgl_direct.direct_real (sgl_direct.py:40-51) and the
+ * exp of invert (sgl_full.py:272).
+ */
+size_t sgl_ref_phi_bytes(int32_t nk, int32_t ng, int32_t nbins);
+sgl_status sgl_ref_phi(const double* pop, const int32_t* npop, int32_t nbins, const double* kk,
+                       int32_t nk, const double* kg, int32_t ng, const double* psi, void* ws,
+                       double* phi, void* stream);
+
+/*
+ * P3(kappa, gamma) on kap = linspace(kap_min, kap_max, n_kap) x gam =
+ * linspace(0, gam_max, n_gam) from phi on k_k = linspace(0, kk_max, nk) x
+ * k_g = linspace(0, kg_max, ng): Hankel then Fourier transform as FP64 MFMA
+ * contractions, plus the Euler-Maclaurin endpoint terms of the k_gamma
+ * trapezoid.  ws: sgl_ref_p3_bytes(nk, ng, n_kap, n_gam).
+ * Replaces: sgl.P_vector_joint (sgl.py:1127-1156) and sgl_full.py:277-298.
+ */
+size_t sgl_ref_p3_bytes(int32_t nk, int32_t ng, int32_t n_kap, int32_t n_gam);
+sgl_status sgl_ref_p3(const double* phi, int32_t nk, int32_t ng, double kk_max, double kg_max,
+                      double kap_min, double kap_max, int32_t n_kap, double gam_max, int32_t n_gam,
+                      void* ws, double* p3, void* stream);
+
+/*
+ * P_S(xi) at n_xi points xi (device): pi / mu^2 times the integral of the
+ * bilinear interpolant of P3 along the constant-mu arc (2400 kappa points for
+ * gamma <= g_def, the tail-box segment by gamma at half the gamma step),
+ * divided by *norm when norm is not NULL.
+ * Replaces: P_i of invert (sgl_full.py:299-360).
+ */
+sgl_status sgl_ref_arcs(const double* p3, double kap_min, double kap_max, int32_t n_kap,
+                        double gam_max, int32_t n_gam, double g_def, const double* xi, int32_t n_xi,
+                        const double* norm, double* out, void* stream);
+
+/* mom = [norm, mean, variance, skewness, sigma_DL/D_L] of pw on xw (trapezoids).
+ * Replaces: sgl_full.py:362-371. */
+sgl_status sgl_ref_moments(const double* xw, const double* pw, int32_t n, double* mom,
+                           void* stream);
+
+/* C[M][N] = A[M][K] B[K][N], FP64 on MFMA (row-major, leading dimensions given).
+ * The contraction of sgl_ref_phi and sgl_ref_p3; replaces the float64 GEMMs of
+ * sgl_direct.direct_real (sgl_direct.py:46-50) and P_vector_joint. */
+sgl_status sgl_ref_gemm_f64(int32_t M, int32_t N, int32_t K, const double* A, int64_t lda,
+                            const double* B, int64_t ldb, double* C, int64_t ldc, void* stream);
+
+/* ------------------------------------------------------------------------
  * Surrogate MLP (no reference counterpart: the reference is a solver, not an
  * emulator, README.md:8).  A linear layer's forward and both backward
  * products are one BF16 MFMA GEMM with strided operands:

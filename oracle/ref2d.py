@@ -263,9 +263,10 @@ class Population:
         return float(sqrt(np.sum(self.w * self.k ** 2)))
 
 
-def rows(cos: Cosmology, zs: float, ell=False, fil=False):
+def rows(cos: Cosmology, zs: float, ell=False, fil=False, edge="r200"):
     """Per-(shell, mass) parameters of _build_shell_into (sgl_full.py:1848-1884,
-    1984-1991) as dicts, shell-major."""
+    1984-1991) as dicts, shell-major.  edge "r200": x_t = c; "rvir": x_t =
+    r_vir / r_s (sgl_full.py:1874-1880)."""
     _, Ml = engine_grids()
     dlnM = np.diff(log(Ml))
     out = []
@@ -277,7 +278,7 @@ def rows(cos: Cosmology, zs: float, ell=False, fil=False):
             M = Ml[jM]
             dn = cos.dndlnM(M, z) * dlnM[jM - 1]
             C, rs, ks, _ = cos.nfw_params(M, z, zs)
-            xt = C
+            xt = C if edge == "r200" else cos.eta(C, z) * C
             g_xt = ks * 4 * (log(1 + xt) - xt / (1 + xt)) / xt ** 2
             xm = xt * sqrt(g_xt / KAP_FLOOR) if g_xt > KAP_FLOOR else xt
             r = dict(shell=i, jM=jM, z=z, dz=dz, wz=wz, D=D, scr=scr, M=M, dn=dn, C=C, rs=rs,
@@ -294,7 +295,7 @@ def rows(cos: Cosmology, zs: float, ell=False, fil=False):
     return out
 
 
-def build(cos: Cosmology, zs: float, ell=False, fil=False, keep_points=False):
+def build(cos: Cosmology, zs: float, ell=False, fil=False, keep_points=False, edge="r200"):
     """build_population(..., kap_floor=1e-8, trunc="r200", zint="midpoint")
     without subhalos and clustering: every row's cells binned into the fine
     lattice with weight and centroid sums."""
@@ -326,7 +327,7 @@ def build(cos: Cosmology, zs: float, ell=False, fil=False, keep_points=False):
         if keep_points:
             pts.append((K, G, W))
 
-    for r in rows(cos, zs, ell=ell, fil=fil):
+    for r in rows(cos, zs, ell=ell, fil=fil, edge=edge):
         pop.N_expl += r["Nbar"]
         K, G, W = host_cells(r["ks"], r["xt"], r["xm"], r["Nbar"], r["eps"],
                              phis if ell else None)
@@ -580,11 +581,11 @@ XI_BODY = np.linspace(-1.0, 1.0, 1601)    # cli.py / explorer body grid
 XI_TAIL = np.linspace(1.005, 9.0, 1600)   # and tail grid
 
 
-def run(zs, cosmo: dict = None, ell=False, fil=False, xi_out=None, keep=False):
+def run(zs, cosmo: dict = None, ell=False, fil=False, xi_out=None, keep=False, edge="r200"):
     """pipeline.run for the ingredient sets without subhalos and clustering.
     xi_out None: the body and tail grids of the CLI (pipeline.py:211-215)."""
     cos = Cosmology(**(cosmo or {}))
-    pop = build(cos, zs, ell=ell, fil=fil)
+    pop = build(cos, zs, ell=ell, fil=fil, edge=edge)
     sig = pop.sigma()
     xo = np.concatenate([XI_BODY, XI_TAIL]) if xi_out is None else np.asarray(xi_out, float)
     res = invert(lambda a, b: lam(pop, a, b), sig, xo, keep=keep)

@@ -3,19 +3,24 @@ signatures, defaults, argument checks and error behaviour
 (sgl_analytic/api.py:11-113, pipeline.py:36-94, 139-164), computed by the
 MI355X kernels.
 
-Physics of this path (the kernel-shape chain): spherical NFW halos truncated
-at r200c with Sheth-Tormen abundances and Ludlow+16 concentrations on
-``n_z`` slices uniform in comoving distance x 256 masses x 512 impact
-parameters; each cell is a jump xi = ln mu = -ln detA(kappa, gamma) of the
-1-D compound-Poisson chain of sgl.Lambda_of_k / P_of_xi (sgl.py:961-1012),
-tilted to the source plane; linear-bias clustering enters as the two-point
-term 1/2 sum_ij C_ij a_i a_j (sgl_full.py:2514-2538, 3353-3397); filaments
-join with the "fil" ingredient and the pseudo-elliptical halos (12
-orientation nodes per cell, Allgood+06 ellipticity) with "ell".  The
-cosmology options radiation ("engine", None) and growth_mode ("ode",
-"legacy", "auto") are computed as sgl.Cosmology defines them.  The
-reference's subhalo ingredient and its "vp" closure are not part of this
-path; asking for them raises NotImplementedError.
+``run`` (and through it generate_pdf_lnmu / generate_pdf) computes the
+reference's own chain, pipeline.run (pipeline.py:114-274): the population
+on the reference's shells, engine mass grid and adaptive impact-parameter
+cells, binned into the 961 x 300 (kappa, gamma) lattice; the 2-D
+characteristic function ln Phi(k_kappa, k_gamma) on the width-relative grid
+with the measured k-reach; the Hankel + Fourier inversion to P3(kappa,
+gamma); the constant-mu arc integral and the normalisation
+(sgl_analytic_amd/ref2d.py over the sgl_ref_* kernels).  Ingredients built
+on this chain: spherical truncated NFW halos, "ell" (pseudo-elliptical
+halos) and "fil" (filaments); edge "r200" (default) or "rvir".  The
+ingredients "sub" (subhalos) and "bias" (the xi_lin two-point term), hence
+the default config="full", and the positive "vp" closure raise
+NotImplementedError: they are not built on this chain.
+
+``run_kernel_shape`` is the host driver of the kernel-shape chain (64
+slices uniform in comoving distance x 256 masses x 512 impact parameters,
+the 1-D ln mu jump chain of sgl.Lambda_of_k / P_of_xi, sgl.py:961-1012) that
+the batched solver and the surrogate use; it is not the reference's model.
 
 Batched entry point (no reference counterpart): generate_pdf_lnmu_batch.
 """
@@ -30,7 +35,8 @@ import numpy as np
 P = dict(h=0.674, Om=0.315, s8=0.811, Ob=0.0493, ns=0.965, zeq=3402.0)  # pipeline.py:20
 EXTENSIONS = dict(Ok=0.0, w0=-1.0, wa=0.0, growth_mode="ode", radiation="engine")
 INGREDIENTS = ("ell", "fil", "sub", "bias")
-MODELLED = {"bias", "fil", "ell"}   # ingredients this path computes on top of NFW halos
+MODELLED = {"bias", "fil", "ell"}   # ingredients of the kernel-shape chain (run_kernel_shape)
+MODELLED_2D = {"fil", "ell"}        # ingredients of the reference chain (run)
 DEFAULT_SLICES = 64
 
 
@@ -136,10 +142,85 @@ def _solver(n_z):
     return _SOLVERS[key]
 
 
-def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", n_z=DEFAULT_SLICES,
-        edge="r200"):
-    """Host driver (the role of pipeline.run, pipeline.py:114-274) for one
-    cosmology on the plain FP64 device path; returns (p on xi_out, info).
+def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200"):
+    """pipeline.run (pipeline.py:114-274) on the device: the reference's 2-D
+    chain for one cosmology; returns (P_s on xi_out, info).  info carries the
+    keys of the reference's run() record (pipeline.py:243-274)."""
+    if not np.isscalar(z_s) or not np.isfinite(z_s) or z_s <= 0:
+        raise ValueError("z_s must be a finite positive number")
+    if edge not in ("r200", "rvir"):
+        if edge == "kthr" or isinstance(edge, (int, float)):
+            raise NotImplementedError(f"edge={edge!r} is not built (r200, rvir)")
+        raise ValueError(f"edge must be 'r200' or 'rvir', got {edge!r}")
+    if closure not in ("gauss", "vp"):
+        raise ValueError(f"xi_closure must be 'gauss' or 'vp', got {closure!r}")
+    s = on(config)
+    cp = check_cosmology(cosmo)
+    if closure != "gauss" and "bias" not in s:
+        raise ValueError(f"xi_closure={closure!r} needs xilin clustering and the bias ingredient")
+    missing = s - MODELLED_2D
+    if missing:
+        raise NotImplementedError(
+            f"ingredient(s) {sorted(missing)} of config {config!r} are not built on the "
+            "reference chain; built: halos with 'ell' and 'fil' (e.g. config='halo', '+ell', "
+            "'+fil' or {'ell', 'fil'})")
+    if closure != "gauss":
+        raise NotImplementedError("the positive 'vp' closure is not built on the reference chain")
+    from . import ref2d
+    t0 = time.perf_counter()
+    pop = ref2d.build(cp, float(z_s), ell="ell" in s, fil="fil" in s, edge=edge)
+    t1 = time.perf_counter()
+    x = np.asarray(xi_out, dtype=np.float64)
+    res = ref2d.invert(pop, x)
+    t2 = time.perf_counter()
+    info = record_2d(cp, config, s, closure, edge,
+                     dict(kbar=pop.kbar, N_expl=pop.N_expl, N_fil=pop.N_fil, n_bins=pop.n_bins,
+                          n_points=pop.n_points),
+                     dict(sigma=res.sigma, mean=res.mean, var=res.var, skew=res.skew,
+                          sigmaDL=res.sigmaDL, grid=res.grid),
+                     dict(build=t1 - t0, lam=0.0, invert=t2 - t1))
+    info["zs"] = float(z_s)
+    return res.P, info
+
+
+def record_2d(cp, config, ingredients, closure, edge, pop, res, t_stages):
+    """The run() record of the reference (pipeline.py:243-274; every top-level
+    key of its stored PDFs) with the 2-D chain's values: pop = bookkeeping of
+    the population (kbar, N_expl, N_fil, n_bins, n_points), res = sigma,
+    moments and grid of the inversion, t_stages = stage times."""
+    Or = cp["Om"] / (1.0 + cp["zeq"]) if cp["radiation"] == "engine" else 0.0
+    de = dict(w0=cp["w0"], wa=cp["wa"], growth_mode=cp["growth_mode"],
+              radiation=cp["radiation"], Omega_r=Or)
+    return dict(
+        t_stages=dict(t_stages), cfg=config if isinstance(config, str) else sorted(ingredients),
+        zint="midpoint", zint_n=None, clustering=None, xi_kperp=None, edge=edge, lss_var=0.0,
+        kap_floor=1e-8, kbar=float(pop["kbar"]), h=cp["h"], Om=cp["Om"], s8=cp["s8"],
+        sigma8=cp["s8"], Ob=cp["Ob"], ns=cp["ns"], zeq=cp["zeq"], Ok=cp["Ok"], w0=cp["w0"],
+        wa=cp["wa"], growth_mode=cp["growth_mode"], radiation=cp["radiation"], Omega_r=Or,
+        dark_energy=de, nord=1, grid_scale=1.0, split=10.0, nproc=1, sub_kw=None,
+        pair_angle=None, pair_scale=None, k3=False, exactS=False, s_taper="cos2",
+        sigma_kappa=float(res["sigma"]), kthr=None, N_expl=float(pop["N_expl"]),
+        N_fil=float(pop["N_fil"]), V_weak=0.0, mean=float(res["mean"]), var=float(res["var"]),
+        skew=float(res["skew"]), sigmaDL=float(res["sigmaDL"]),
+        grid={k: v for k, v in res["grid"].items() if isinstance(v, (int, float))},
+        tail_box=True, fine=dict(k=[1e-7, 30.0, 480], g=[1e-9, 30.0, 300]), s_is=True,
+        xi_clamp=False, xi_linear_m=0.0, xi_closure=closure, window=False, n_xi_wide=500,
+        pos_closure=None, direct_product_backend="MI355X HIP FP64 MFMA (libsgl_amd.so)",
+        direct_product_fallback=None, s_seed=None, s_seed_effective=0, s_ess=None,
+        is_design=None, pk_mod=None, pk_par={},
+        provenance=dict(producer="sgl_analytic_amd api.run (MI355X, reference 2-D chain)"),
+        ingredients=["halo"] + sorted(ingredients), n_bins=int(pop["n_bins"]),
+        n_points=int(pop["n_points"]),
+        backend="MI355X HIP (libsgl_amd.so), reference 2-D chain, FP64",
+        seconds=float(sum(t_stages.values())))
+
+
+def run_kernel_shape(z_s, config="full", cosmo=None, xi_out=None, closure="gauss",
+                     n_z=DEFAULT_SLICES, edge="r200"):
+    """Host driver of the kernel-shape chain (64 slices uniform in comoving
+    distance, the 1-D ln mu jump chain) for one cosmology on the plain FP64
+    device path; returns (p on xi_out, info).  Not the reference's model: the
+    batched solver and the surrogate use this chain.
     edge: halo truncation, "r200" (the reference default) or "rvir"
     (pipeline.run's edge / build_population's trunc, sgl_full.py:1874-1880)."""
     import ctypes
@@ -263,7 +344,7 @@ def generate_pdf_lnmu(z_s, h=0.674, Om=0.315, sigma8=0.811, Ob=0.0493,
     """Return (lnmu, dP_S/dlnmu) as NumPy arrays (api.py:26-84).
 
     Same arguments and checks as the reference; the PDF comes from the
-    MI355X FP64 plain path.  The density is not renormalised to the user
+    reference's 2-D chain on the MI355X (``run``).  The density is not renormalised to the user
     grid; negative residuals stay visible with a RuntimeWarning."""
     x = _grid(lnmu, lnmu_min, lnmu_max, n_lnmu, "lnmu")
     if isinstance(nproc, bool) or not isinstance(nproc, (int, np.integer)) or nproc < 1:
@@ -278,7 +359,9 @@ def generate_pdf_lnmu(z_s, h=0.674, Om=0.315, sigma8=0.811, Ob=0.0493,
     negative_mass = float(np.trapezoid(np.maximum(-p, 0), x))
     out["diagnostics"] = dict(grid_mass=float(np.trapezoid(p, x)), negative_mass=negative_mass,
                               minimum_density=float(p.min()))
-    out["moment_interval"] = [out["kernel_grid"]["xi_min"], out["kernel_grid"]["xi_max"]]
+    g = out.get("grid", {})
+    kg = out.get("kernel_grid", {})
+    out["moment_interval"] = [g.get("xi_min", kg.get("xi_min")), g.get("xi_max", kg.get("xi_max"))]
     if negative_mass > 0:
         warnings.warn(f"PDF inversion has negative residuals (integrated mass "
                       f"{negative_mass:.3g}); inspect diagnostics before using tail probabilities.",

@@ -1,15 +1,14 @@
-"""Command line: the magnification PDF of one source redshift and cosmology,
-written as the reference's JSON record (sgl_analytic/cli.py:24-78).
+"""Command line: one magnification PDF written as the reference's JSON record.
 
-  python -m sgl_analytic_amd --zs 1
-  python -m sgl_analytic_amd --zs 3 --s8 0.75 --Om 0.30 --out output/pdf_zs3_s8lo.json
-  python -m sgl_analytic_amd --zs 1 --growth-mode auto --radiation none
+  python -m sgl_analytic_amd --zs 1 --config +fil
+  python -m sgl_analytic_amd --zs 3 --s8 0.75 --Om 0.30 --config halo --out pdf_zs3.json
 
-The record holds xi = ln mu on linspace(-1, 1, 1601) with P_s = dP/dln mu
-there, xi_tail / P_s_tail on linspace(1.005, 9, 1600) (same normalisation),
-the moments, diagnostics and clipped moments, and the rest of the
-reference's key set (api.reference_schema).  Computed on the MI355X FP64
-plain path.
+The options, their aliases and defaults are the reference's command line
+(sgl_analytic/cli.py:24-55); the record is api.run's (the keys of the
+reference's run() record) on the body grid linspace(-1, 1, 1601) and the tail
+grid linspace(1.005, 9, 1600), inverted together, plus the diagnostics, the
+clipped moments and the moment interval (cli.py:57-72).  --edge is added
+(pipeline.run's edge).
 """
 from __future__ import annotations
 
@@ -21,73 +20,73 @@ import numpy as np
 
 from .api import EXTENSIONS, P, clipped_moments, run
 
-XI_BODY = np.linspace(-1.0, 1.0, 1601)
-XI_TAIL = np.linspace(1.005, 9.0, 1600)
+BODY = (-1.0, 1.0, 1601)
+TAIL = (1.005, 9.0, 1600)
+COSMO_KEYS = ("h", "Om", "s8", "Ob", "ns", "zeq", "Ok", "w0", "wa", "growth_mode", "radiation")
+
+# (flags, destination, type, default, choices); default None with type float = required
+OPTIONS = (
+    (("--z-s", "--zs"), "zs", float, None, None),
+    (("--h",), "h", float, P["h"], None),
+    (("--Om",), "Om", float, P["Om"], None),
+    (("--sigma8", "--s8"), "s8", float, P["s8"], None),
+    (("--Ob",), "Ob", float, P["Ob"], None),
+    (("--ns",), "ns", float, P["ns"], None),
+    (("--zeq",), "zeq", float, P["zeq"], None),
+    (("--Ok", "--Omega-K", "--Omega_K"), "Ok", float, EXTENSIONS["Ok"], None),
+    (("--w0",), "w0", float, EXTENSIONS["w0"], None),
+    (("--wa",), "wa", float, EXTENSIONS["wa"], None),
+    (("--growth-mode", "--growth_mode"), "growth_mode", str, EXTENSIONS["growth_mode"],
+     ("auto", "ode", "legacy")),
+    (("--radiation",), "radiation", str, EXTENSIONS["radiation"], ("engine", "none")),
+    (("--closure",), "closure", str, "gauss", ("gauss", "vp")),
+    (("--config",), "config", str, "full", None),
+    (("--edge",), "edge", str, "r200", ("r200", "rvir")),
+    (("--nproc",), "nproc", int, 1, None),
+    (("--out",), "out", str, "", None),
+)
 
 
-def record(zs, config="full", cosmo=None, closure="gauss", edge="r200"):
-    """The reference's output record for one PDF (cli.py:57-72)."""
-    xall = np.concatenate([XI_BODY, XI_TAIL])
-    p, out = run(zs, config, cosmo=cosmo, xi_out=xall, closure=closure, edge=edge)
-    nb = XI_BODY.size
-    out.update(xi=XI_BODY.tolist(), P_s=p[:nb].tolist(), xi_tail=XI_TAIL.tolist(),
-               P_s_tail=p[nb:].tolist())
-    out["moment_interval"] = [out["grid"]["xi_min"], out["grid"]["xi_max"]]
-    out["diagnostics"] = dict(grid_mass=float(np.trapezoid(p, xall)),
-                              negative_mass=float(np.trapezoid(np.maximum(-p, 0), xall)),
+def parser() -> argparse.ArgumentParser:
+    ap = argparse.ArgumentParser(prog="python -m sgl_analytic_amd", description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    for flags, dest, typ, default, choices in OPTIONS:
+        ap.add_argument(*flags, dest=dest, type=typ, choices=choices,
+                        required=default is None, default=default)
+    return ap
+
+
+def record(zs, config="full", cosmo=None, closure="gauss", edge="r200") -> dict:
+    """One solve on the body and tail grids, split into the record's arrays."""
+    body, tail = np.linspace(*BODY), np.linspace(*TAIL)
+    p, rec = run(zs, config, cosmo=cosmo, xi_out=np.concatenate([body, tail]), closure=closure,
+                 edge=edge)
+    p = np.asarray(p, dtype=float)
+    rec.update(xi=body.tolist(), P_s=p[:body.size].tolist(), xi_tail=tail.tolist(),
+               P_s_tail=p[body.size:].tolist())
+    x = np.concatenate([body, tail])
+    rec["diagnostics"] = dict(grid_mass=float(np.trapezoid(p, x)),
+                              negative_mass=float(np.trapezoid(np.clip(-p, 0, None), x)),
                               minimum_density=float(p.min()))
-    out["clipped"] = clipped_moments(XI_BODY, p[:nb])
-    return out
+    rec["clipped"] = clipped_moments(body, p[:body.size])
+    rec["moment_interval"] = [rec["grid"]["xi_min"], rec["grid"]["xi_max"]]
+    return rec
 
 
 def main(argv=None):
-    ap = argparse.ArgumentParser(description=__doc__,
-                                 formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--z-s", "--zs", dest="zs", type=float, required=True, help="source redshift")
-    ap.add_argument("--h", type=float, default=P["h"])
-    ap.add_argument("--Om", type=float, default=P["Om"])
-    ap.add_argument("--sigma8", "--s8", dest="s8", type=float, default=P["s8"])
-    for name in ("Ob", "ns", "zeq"):
-        ap.add_argument(f"--{name}", type=float, default=P[name])
-    ap.add_argument("--Ok", "--Omega-K", "--Omega_K", dest="Ok", type=float,
-                    default=EXTENSIONS["Ok"], help="curvature: > 0 open, < 0 closed (default flat)")
-    ap.add_argument("--w0", type=float, default=EXTENSIONS["w0"],
-                    help="dark energy w(a) = w0 + wa (1 - a) (default -1)")
-    ap.add_argument("--wa", type=float, default=EXTENSIONS["wa"], help="CPL slope (default 0)")
-    ap.add_argument("--growth-mode", "--growth_mode", dest="growth_mode",
-                    choices=("auto", "ode", "legacy"), default=EXTENSIONS["growth_mode"],
-                    help='growth: "ode" (default), "legacy" (LambdaCDM) or "auto"')
-    ap.add_argument("--radiation", choices=("engine", "none"), default=EXTENSIONS["radiation"],
-                    help='"engine" (default): Omega_r = Om/(1+zeq); "none": no radiation')
-    ap.add_argument("--closure", choices=("gauss", "vp"), default="gauss",
-                    help="clustering closure: gauss (two-point term) or vp (positive closure)")
-    ap.add_argument("--config", default="full",
-                    help='lens model: "full" (default), "halo", "-bias", ... '
-                         '(pass minus arms as --config=-bias)')
-    ap.add_argument("--edge", choices=("r200", "rvir"), default="r200",
-                    help="halo truncation radius (default r200)")
-    ap.add_argument("--nproc", type=int, default=1,
-                    help="accepted for the reference's interface; the solve runs on one GPU")
-    ap.add_argument("--out", default=None, help="output JSON (default output/pdf_zs<zs>.json)")
+    ap = parser()
     a = ap.parse_args(argv)
     if a.nproc < 1:
         ap.error("nproc must be a positive integer")
+    cosmo = {k: getattr(a, k) for k in COSMO_KEYS}
     try:
-        out = record(a.zs, a.config,
-                     cosmo=dict(h=a.h, Om=a.Om, s8=a.s8, Ob=a.Ob, ns=a.ns, zeq=a.zeq, Ok=a.Ok,
-                                w0=a.w0, wa=a.wa, growth_mode=a.growth_mode,
-                                radiation=a.radiation),
-                     closure=a.closure, edge=a.edge)
-    except ValueError as exc:
+        rec = record(a.zs, a.config, cosmo=cosmo, closure=a.closure, edge=a.edge)
+    except (ValueError, NotImplementedError) as exc:
         ap.error(str(exc))
-    dest = Path(a.out) if a.out else Path("output") / f"pdf_zs{a.zs:g}.json"
+    dest = Path(a.out or f"output/pdf_zs{a.zs:g}.json")
     dest.parent.mkdir(parents=True, exist_ok=True)
-    with dest.open("w") as stream:
-        json.dump(out, stream, allow_nan=False)
-    c = out["clipped"]
-    print(f"zs={a.zs:g} clipped var={c['var']:.5e} skew={c['skew']:.3f}  "
-          f"{out['seconds']:.2f} s -> {dest}")
-
-
-if __name__ == "__main__":
-    main()
+    dest.write_text(json.dumps(rec, allow_nan=False))
+    c = rec["clipped"]
+    print(f"zs={a.zs:g} sigma_kappa={rec['sigma_kappa']:.5f} clipped var={c['var']:.5e} "
+          f"skew={c['skew']:.3f} {rec['seconds']:.2f} s -> {dest}")
+    return rec

@@ -1181,7 +1181,7 @@ sgl_shape ref_shape(int flags) {
   s.n_b = 1;
   s.n_k = 2;
   s.n_out = 2;
-  s.flags = (flags & (SGL_FLAG_ELL | SGL_FLAG_FILAMENTS)) | SGL_FLAG_REF_GRID;
+  s.flags = (flags & (SGL_FLAG_ELL | SGL_FLAG_FILAMENTS | SGL_FLAG_EDGE_RVIR)) | SGL_FLAG_REF_GRID;
   s.m_lo_h = 1.0; s.m_hi_h = 2.0; s.u_lo = 1.0; s.u_hi = 2.0;
   s.k_lo = 1.0; s.k_hi = 2.0; s.out_lo = 0.0; s.out_hi = 1.0;
   return s;
@@ -1249,7 +1249,9 @@ using namespace sgl;
 
 namespace {
 hipStream_t ref_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
-bool ref_flags_ok(int32_t f) { return (f & ~(SGL_FLAG_ELL | SGL_FLAG_FILAMENTS)) == 0; }
+bool ref_flags_ok(int32_t f) {
+  return (f & ~(SGL_FLAG_ELL | SGL_FLAG_FILAMENTS | SGL_FLAG_EDGE_RVIR)) == 0;
+}
 sgl_status ref_rc(int rc) { return (sgl_status)rc; }
 }  // namespace
 

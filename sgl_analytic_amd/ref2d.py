@@ -72,12 +72,17 @@ class Population:
     timings: dict = field(default_factory=dict)
 
 
-def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False) -> Population:
-    """build_population(..., kap_floor=1e-8, trunc="r200", zint="midpoint")
-    + Lam's flattening (sgl_full.py:1535, 2742-2755), on the device."""
+def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
+          edge: str = "r200") -> Population:
+    """build_population(..., kap_floor=1e-8, trunc=edge, zint="midpoint")
+    + Lam's flattening (sgl_full.py:1535, 2742-2755), on the device.  edge
+    "rvir" truncates every halo at its virial radius (sgl_full.py:1876-1880)."""
     L = _lib.load()
     dev = torch.device("cuda")
-    flags = (FLAG_ELL if ell else 0) | (FLAG_FILAMENTS if fil else 0)
+    if edge not in ("r200", "rvir"):
+        raise ValueError("edge must be 'r200' or 'rvir'")
+    flags = ((FLAG_ELL if ell else 0) | (FLAG_FILAMENTS if fil else 0)
+             | (_lib.FLAG_EDGE_RVIR if edge == "rvir" else 0))
     n_z = int(L.sgl_ref_shell_count(float(zs)))
     if n_z < 1 or n_z > _lib.REF_MAX_SHELLS:
         raise ValueError("z_s outside the reference grid's shells")
@@ -275,10 +280,11 @@ def invert(pop: Population, xi_out: np.ndarray, tol: float = PHI_TOL) -> Result:
                   float(m[4]), float(m[0]), g, pop)
 
 
-def run(zs: float, cosmo: dict, ell: bool = False, fil: bool = False, xi_out=None) -> Result:
+def run(zs: float, cosmo: dict, ell: bool = False, fil: bool = False, xi_out=None,
+        edge: str = "r200") -> Result:
     """One PDF on the reference's chain; xi_out None: the CLI body and tail
     grids linspace(-1, 1, 1601) and linspace(1.005, 9, 1600) (pipeline.py:211-215)."""
-    pop = build(cosmo, zs, ell=ell, fil=fil)
+    pop = build(cosmo, zs, ell=ell, fil=fil, edge=edge)
     if xi_out is None:
         xi_out = np.concatenate([np.linspace(-1.0, 1.0, 1601), np.linspace(1.005, 9.0, 1600)])
     return invert(pop, xi_out)

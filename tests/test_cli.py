@@ -14,11 +14,14 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_scalars.js
 
 
 def fake_run(zs, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200"):
-    from sgl_analytic_amd.solver import Shape
     cp = api.check_cosmology(cosmo)
-    diag = np.array([1.0, 0.01, 0.02, 1.5, 0.07, 1.0, 0.01, 0.0])
-    info = api.reference_schema(cp, Shape(), diag, config, closure, edge, 0.25, 0.05)
-    info.update(zs=float(zs), cfg=config)
+    grid = dict(nk_k=10, nk_g=12, k_k_max=1.0, k_g_max=2.0, n_kap=5, n_gam=6, kap_min=-0.1,
+                kap_max=0.99, gam_max=0.8, xi_min=-0.2, xi_max=1.3, sigma=0.03)
+    info = api.record_2d(cp, config, api.on(config), closure, edge,
+                         dict(kbar=0.05, N_expl=1e6, N_fil=10.0, n_bins=100, n_points=1000),
+                         dict(sigma=0.03, mean=0.0, var=0.01, skew=1.5, sigmaDL=0.03, grid=grid),
+                         dict(build=0.2, lam=0.0, invert=0.05))
+    info.update(zs=float(zs))
     x = np.asarray(xi_out)
     return np.exp(-0.5 * (x / 0.1) ** 2) / (0.1 * np.sqrt(2 * np.pi)), info
 
@@ -27,7 +30,7 @@ def test_record_has_the_reference_key_set(tmp_path):
     keys = set(json.load(open(GOLDEN))["fixture_keys"])
     out_file = tmp_path / "pdf.json"
     with patch("sgl_analytic_amd.cli.run", side_effect=fake_run):
-        cli.main(["--zs", "1", "--growth-mode", "auto", "--radiation", "none",
+        cli.main(["--zs", "1", "--growth-mode", "auto", "--radiation", "none", "--config", "+fil",
                   "--out", str(out_file)])
     rec = json.load(open(out_file))
     assert keys <= set(rec), sorted(keys - set(rec))
@@ -46,3 +49,6 @@ def test_cli_rejects_bad_input(tmp_path):
             cli.main(["--zs", "1", "--Om", "-1", "--out", str(tmp_path / "x.json")])
         with pytest.raises(SystemExit):
             cli.main(["--zs", "1", "--nproc", "0", "--out", str(tmp_path / "x.json")])
+    # ingredients not built on the reference chain: a usage error, not a silent other model
+    with pytest.raises(SystemExit):
+        cli.main(["--zs", "1", "--config", "+sub", "--out", str(tmp_path / "x.json")])

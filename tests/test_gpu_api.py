@@ -6,30 +6,33 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-def test_generate_pdf_lnmu_runs_on_device_and_matches_solver():
+def test_generate_pdf_lnmu_runs_the_reference_chain():
     import sgl_analytic_amd as sa
-    from sgl_analytic_amd.solver import Shape
-    print("api generate_pdf_lnmu z_s=1 default grid", flush=True)
-    x, p, info = sa.generate_pdf_lnmu(1.0, return_info=True)
+    from sgl_analytic_amd import ref2d
+    print("api generate_pdf_lnmu z_s=1 default grid, ell + fil", flush=True)
+    x, p, info = sa.generate_pdf_lnmu(1.0, config={"ell", "fil"}, return_info=True)
     assert x.shape == (3201,) and p.shape == (3201,)
     assert np.all(np.isfinite(p))
-    assert info["backend"].startswith("MI355X")
-    # the user-grid evaluation equals the kernel-grid evaluation at shared nodes
-    sh = Shape()
-    xo = sh.out_grid()
-    x2, p2 = sa.generate_pdf_lnmu(1.0, lnmu=xo)
-    from sgl_analytic_amd.api import _SOLVERS
-    sv = _SOLVERS[("exact", 64, True, True, "r200", True)]   # "full": clustering, filaments, ell
-    np.testing.assert_allclose(p2, sv.p[0].cpu().numpy(), rtol=1e-12, atol=1e-14)
+    assert info["backend"].startswith("MI355X") and info["N_fil"] > 0
+    # the user grid shares its nodes with the body / tail grids of the explorer
+    # run: the same arc integrals, so the same values
+    r = ref2d.run(1.0, sa.api.check_cosmology({}), ell=True, fil=True)
+    on_body = np.isin(np.round(r.xi, 10), np.round(x, 10))
+    xb = r.xi[on_body]
+    np.testing.assert_allclose(np.interp(xb, x, p), r.P[on_body], rtol=1e-12, atol=1e-14)
+    assert on_body.sum() > 321
     mass = np.trapezoid(p, x)
     assert 0.99 < mass < 1.01
+    # the default config "full" needs subhalos and the xi_lin term: not built here
+    with pytest.raises(NotImplementedError):
+        sa.generate_pdf_lnmu(1.0, lnmu=x[:10])
 
 
 def test_generate_pdf_jacobian_on_device():
     import sgl_analytic_amd as sa
     print("api generate_pdf", flush=True)
-    mu, pm = sa.generate_pdf(1.0, mu=np.geomspace(0.8, 3.0, 64))
-    x, pl = sa.generate_pdf_lnmu(1.0, lnmu=np.log(mu))
+    mu, pm = sa.generate_pdf(1.0, mu=np.geomspace(0.8, 3.0, 64), config="+ell")
+    x, pl = sa.generate_pdf_lnmu(1.0, lnmu=np.log(mu), config="+ell")
     np.testing.assert_allclose(pm, pl / mu, rtol=1e-14)
 
 

@@ -76,10 +76,10 @@ def test_api_growth_and_radiation():
     from sgl_analytic_amd import api
     print("api growth/radiation options", flush=True)
     x = np.linspace(-0.2, 0.6, 801)
-    p0, i0 = api.run(1.0, "halo", cosmo={}, xi_out=x, n_z=16)
-    pa, ia = api.run(1.0, "halo", cosmo=dict(growth_mode="auto"), xi_out=x, n_z=16)
-    pl, il = api.run(1.0, "halo", cosmo=dict(growth_mode="legacy"), xi_out=x, n_z=16)
-    pn, i_n = api.run(1.0, "halo", cosmo=dict(radiation=None), xi_out=x, n_z=16)
+    p0, i0 = api.run_kernel_shape(1.0, "halo", cosmo={}, xi_out=x, n_z=16)
+    pa, ia = api.run_kernel_shape(1.0, "halo", cosmo=dict(growth_mode="auto"), xi_out=x, n_z=16)
+    pl, il = api.run_kernel_shape(1.0, "halo", cosmo=dict(growth_mode="legacy"), xi_out=x, n_z=16)
+    pn, i_n = api.run_kernel_shape(1.0, "halo", cosmo=dict(radiation=None), xi_out=x, n_z=16)
     assert ia["growth_mode"] == il["growth_mode"] == "legacy" and i_n["radiation"] is None
     np.testing.assert_array_equal(pa, pl)
     d_leg = float(np.max(np.abs(pl - p0))) / float(np.max(p0))

@@ -72,8 +72,8 @@ def test_api_edge_rvir_differs_from_r200():
     print("api edge rvir", flush=True)
     from sgl_analytic_amd import api
     x = np.linspace(-0.2, 0.6, 801)
-    p200, _ = api.run(1.0, "halo", cosmo={}, xi_out=x, n_z=16)
-    pv, info = api.run(1.0, "halo", cosmo={}, xi_out=x, n_z=16, edge="rvir")
+    p200, _ = api.run_kernel_shape(1.0, "halo", cosmo={}, xi_out=x, n_z=16)
+    pv, info = api.run_kernel_shape(1.0, "halo", cosmo={}, xi_out=x, n_z=16, edge="rvir")
     assert info["edge"] == "rvir"
     # more projected mass per halo (r_vir > r200 at z ~ 0.5): a wider PDF
     v = lambda p: np.trapezoid(x * x * p, x) - np.trapezoid(x * p, x) ** 2

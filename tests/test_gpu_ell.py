@@ -71,8 +71,8 @@ def test_api_ell_differs_from_halo():
     from sgl_analytic_amd import api
     print("api +ell", flush=True)
     x = np.linspace(-0.2, 0.6, 801)
-    ph, _ = api.run(1.0, "halo", cosmo={}, xi_out=x, n_z=16)
-    pe, info = api.run(1.0, "+ell", cosmo={}, xi_out=x, n_z=16)
+    ph, _ = api.run_kernel_shape(1.0, "halo", cosmo={}, xi_out=x, n_z=16)
+    pe, info = api.run_kernel_shape(1.0, "+ell", cosmo={}, xi_out=x, n_z=16)
     assert "ell" in info["ingredients"]
     d = float(np.max(np.abs(pe - ph))) / float(np.max(ph))
     print("relative change with ellipticity", d)

@@ -179,3 +179,19 @@ def test_halo_pdf_matches_oracle(halo_pops):
     l1 = np.trapezoid(np.abs(d.P - o.P), xo)
     print(f"halo z_s=1 device vs oracle: L1 {l1:.3e}, max {np.max(np.abs(d.P - o.P)):.3e}", flush=True)
     assert l1 < 1e-6
+
+
+@pytest.mark.parametrize("zs,edge", [(0.5, "r200"), (1.0, "rvir"), (3.0, "r200"), (10.0, "r200")])
+def test_halo_pdf_other_shapes(zs, edge):
+    """Device vs oracle on the reference grids at other source redshifts
+    (their own shell counts and k grids) and with the r_vir truncation."""
+    from sgl_analytic_amd import ref2d
+    xo = np.concatenate([oref.XI_BODY, oref.XI_TAIL])
+    d = ref2d.run(zs, PLANCK, xi_out=xo, edge=edge)
+    o = oref.run(zs, xi_out=xo, edge=edge)
+    l1 = np.trapezoid(np.abs(d.P - o.P), xo)
+    print(f"halo z_s={zs} edge={edge}: bins {d.pop.n_bins} vs {o.pop.w.size}, grid "
+          f"{d.grid['nk_k']}x{d.grid['nk_g']} vs {o.grid['nk_k']}x{o.grid['nk_g']}, "
+          f"L1 {l1:.3e}", flush=True)
+    assert d.pop.n_bins == o.pop.w.size
+    assert l1 < 1e-6

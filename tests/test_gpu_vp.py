@@ -80,8 +80,8 @@ def test_vp_through_the_api():
     from sgl_analytic_amd import api
     print("vp api zs=0.3", flush=True)
     x = np.linspace(-0.5, 2.0, 801)
-    pv, info = api.run(0.3, "full", cosmo={}, xi_out=x, closure="vp", n_z=16)
-    pg, _ = api.run(0.3, "full", cosmo={}, xi_out=x, closure="gauss", n_z=16)
+    pv, info = api.run_kernel_shape(0.3, "full", cosmo={}, xi_out=x, closure="vp", n_z=16)
+    pg, _ = api.run_kernel_shape(0.3, "full", cosmo={}, xi_out=x, closure="gauss", n_z=16)
     pc = info["pos_closure"]
     print("api vp", pc, float(np.trapezoid(pv, x)), float(np.trapezoid(np.abs(pv - pg), x)),
           flush=True)
