@@ -140,6 +140,18 @@ sgl_status sgl_lnphi_exact(const void* tables, int32_t B, const sgl_shape* shape
                            double* lam, double* amp, double* work, void* stream);
 
 /*
+ * sgl_lnphi_exact without the caller's work buffer: the per-slice partial sums
+ * live in the library's own device array of at most 4,194,304 doubles (32 MiB,
+ * one per GPU, held until the process ends), so B * n_z_max * n_k * 2 must not
+ * exceed that (SGL_ERR_INVALID otherwise).  No allocation of any kind happens
+ * in the call.  Calls that overlap on one GPU share the array and race.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ */
+sgl_status sgl_lnphi_exact_noalloc(const void* tables, int32_t B, const sgl_shape* shape,
+                                   double* lam, double* amp, void* stream);
+
+/*
  * Exact path with the positive ("vp") clustering closure in place of Psi_2:
  *   lam[b][n] as sgl_lnphi_exact, and
  *   lnr[b][n] = ln E exp( sum_s sum_g (lambda_sg(U_s) - 1) Y_sg(k_n) ),

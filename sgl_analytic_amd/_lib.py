@@ -31,7 +31,9 @@ EXPORTS = (
     "sgl_ref_roots_bytes", "sgl_ref_tables_build", "sgl_ref_cells_count", "sgl_ref_cells_emit",
     "sgl_ref_bin_bytes", "sgl_ref_bin", "sgl_ref_lam_axis", "sgl_ref_phi_bytes", "sgl_ref_phi",
     "sgl_ref_p3_bytes", "sgl_ref_p3", "sgl_ref_arcs", "sgl_ref_moments", "sgl_ref_gemm_f64",
+    "sgl_lnphi_exact_noalloc",
 )
+NOALLOC_SCRATCH_DOUBLES = 1 << 22   # capacity of sgl_lnphi_exact_noalloc's own array
 REF_MAX_SHELLS, REF_MASSES, REF_FINE_BINS, REF_ROOT_STRIDE = 104, 99, 961 * 300, 128
 REF_STATUS_ROOTS, REF_STATUS_EDGES, REF_STATUS_BAD = 16, 32, 2
 EPI_PLAIN, EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU = 0, 1, 2, 3
@@ -98,6 +100,7 @@ def load(path: str = LIB_PATH):
     sigs = {
         "sgl_tables_build": [vp, i32, sp, vp, vp, vp],
         "sgl_lnphi_exact": [vp, i32, sp, vp, vp, vp, vp],
+        "sgl_lnphi_exact_noalloc": [vp, i32, sp, vp, vp, vp],
         "sgl_lnphi_exact_vp": [vp, i32, sp, vp, vp, vp, vp, vp, vp],
         "sgl_bin_cells": [vp, i32, sp, vp, vp],
         "sgl_bin_pass": [vp, i32, sp, vp, vp],

@@ -43,6 +43,28 @@ __global__ void k_zero_int(int* p) { *p = 0; }
 
 }  // namespace
 
+// Scratch of sgl_lnphi_exact_noalloc: the per-slice partial sums of the
+// exponent (B * n_z_max * n_k complex), at most kScratchDoubles doubles
+// (32 MiB).  One copy per GPU, held until the process ends; its device address
+// is looked up once per device (hipGetSymbolAddress) and only that address is
+// passed to kernels.  The kernels overwrite every slot they later read, so no
+// clearing is needed between calls.  Calls that overlap on one GPU race on it.
+constexpr size_t kScratchDoubles = (size_t)1 << 22;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kScratchDevices = 64;
+static double* g_scratch_ptr[kScratchDevices];
+
+static double* scratch_ptr() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchDevices) return nullptr;
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return nullptr;
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
 extern "C" {
 
 const char* sgl_version(void) { return "sgl_amd 0.1 (gfx950)"; }
@@ -110,6 +132,19 @@ sgl_status sgl_lnphi_exact(const void* tables, int32_t B, const sgl_shape* shape
                            double* amp, double* work, void* stream) {
   if (!shape_ok(shape) || !batch_ok(B) || !tables || !lam || !amp || !work)
     return SGL_ERR_INVALID;
+  TablesLayout L = tables_layout(*shape, B);
+  const double* T = static_cast<const double*>(tables);
+  return (sgl_status)launch_lnphi_exact(T + L.rows, T + L.frows, T + L.cosmo, T + L.uedge,
+                                        T + L.knodes, *shape, B, lam, amp, work,
+                                        as_stream(stream));
+}
+
+sgl_status sgl_lnphi_exact_noalloc(const void* tables, int32_t B, const sgl_shape* shape,
+                                   double* lam, double* amp, void* stream) {
+  if (!shape_ok(shape) || !batch_ok(B) || !tables || !lam || !amp) return SGL_ERR_INVALID;
+  if ((size_t)B * shape->n_z_max * shape->n_k * 2 > kScratchDoubles) return SGL_ERR_INVALID;
+  double* work = scratch_ptr();
+  if (work == nullptr) return SGL_ERR_HIP;
   TablesLayout L = tables_layout(*shape, B);
   const double* T = static_cast<const double*>(tables);
   return (sgl_status)launch_lnphi_exact(T + L.rows, T + L.frows, T + L.cosmo, T + L.uedge,
