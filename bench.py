@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-sur-verify", action="store_true",
                     help="skip the solver check of the surrogate batch (for kernel profiles)")
+    ap.add_argument("--ref-only", action="store_true",
+                    help="only the reference-chain leg (generate_pdf_lnmu's 2-D chain at z_s = 1)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a 1 GiB device-to-device copy: the reference kernel for scaling "
                          "the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -105,6 +107,10 @@ def main():
         return
     from sgl_analytic_amd import solver as S
     from sgl_analytic_amd._lib import PREC_BF16X3, PREC_FP32
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    if args.ref_only:
+        print(json.dumps({"ref_chain": ref_chain_leg(log)}), flush=True)
+        return
 
     prec = PREC_FP32 if args.precision == "fp32" else PREC_BF16X3
     B = args.batch
@@ -115,7 +121,6 @@ def main():
     n_z = S.slice_count(params[:, 9])
     shape = S.Shape(n_z_max=int(n_z.max()))
     rec = S.cosmo_records(params, n_z)
-    log = lambda *a: print(*a, file=sys.stderr, flush=True)
 
     # ---------------------------------------------------------- batched --
     log(f"[bench] batched B={B} n_z_max={shape.n_z_max} precision={args.precision}")
@@ -186,6 +191,8 @@ def main():
     log(f"[bench] plain FP64 path {plain_s * 1e3:.1f} ms/PDF -> {plain_rate:.2f} PDFs/s")
     del pp
 
+    ref = ref_chain_leg(log)
+
     # ------------------------------------------------------ surrogate --
     sur = None if args.no_surrogate else surrogate_leg(args, shape, sv, value, log)
     del sv
@@ -224,10 +231,47 @@ def main():
         "stage_ms": {k: round(v, 3) for k, v in stage.items()},
         "tuning": tuning["choice"] if tuning else None,
         "surrogate": sur,
+        "ref_chain": ref,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def ref_chain_leg(log, zs=1.0, reps=3):
+    """generate_pdf_lnmu's chain (the reference's 2-D chain) at z_s = 1, Planck,
+    on the default ln mu grid linspace(-1, 9, 3201): seconds per PDF and the
+    stage split, for halos and for halos + ellipticity + filaments (the
+    ingredients built on it), best of ``reps`` after one warm-up."""
+    import torch
+    from sgl_analytic_amd import api, ref2d
+    cp = api.check_cosmology({})
+    x = np.linspace(-1.0, 9.0, 3201)
+    out = {}
+    for name, kw in (("halo", {}), ("ell+fil", dict(ell=True, fil=True))):
+        ref2d.run(zs, cp, xi_out=x, **kw)
+        torch.cuda.synchronize()
+        best, stages, res = None, None, None
+        for _ in range(reps):
+            tm = {}
+            t0 = time.perf_counter()
+            res = ref2d.run(zs, cp, xi_out=x, timings=tm, **kw)
+            dt = time.perf_counter() - t0
+            if best is None or dt < best:
+                best, stages = dt, tm
+        g = res.grid
+        out[name] = dict(seconds_per_pdf=round(best, 4), pdfs_per_s=round(1.0 / best, 3),
+                         stage_s={k: round(v, 4) for k, v in stages.items()},
+                         cells=res.pop.n_points, bins=res.pop.n_bins,
+                         k_grid=[g["nk_k"], g["nk_g"]], out_grid=[g["n_kap"], g["n_gam"]],
+                         sigma_kappa=res.sigma)
+        log(f"[bench] reference chain z_s={zs} {name}: {best * 1e3:.1f} ms/PDF", out[name])
+    out["workload"] = ("generate_pdf_lnmu(z_s=1.0), Planck, default ln mu grid (3201 nodes), "
+                       "reference grids (67 shells x 99 masses, adaptive cells, 961x300 lattice)")
+    out["reference_cpu_seconds"] = {"note": "the reference's own record of a z_s = 1 full-config "
+                                    "run: 15.41 s (build 11.52, lam 0.30, invert 3.59), nproc=4, "
+                                    "Apple arm64 (tests/reference/pdf_zs1.json t_stages)"}
+    return out
 
 
 def surrogate_leg(args, shape, sv, solver_rate, log):
@@ -317,6 +361,10 @@ def precision_study(args, B):
     import torch
     from sgl_analytic_amd import solver as S
     from sgl_analytic_amd._lib import PREC_BF16X3, PREC_FP32
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    if args.ref_only:
+        print(json.dumps({"ref_chain": ref_chain_leg(log)}), flush=True)
+        return
     params = lhs_params(B)
     n_z = S.slice_count(params[:, 9])
     shape = S.Shape(n_z_max=int(n_z.max()))

@@ -72,8 +72,29 @@ class Population:
     timings: dict = field(default_factory=dict)
 
 
+class _Stages:
+    """Wall time of each stage (synchronising the device) when a dict is given."""
+
+    def __init__(self, out):
+        self.out = out
+        self.t = None
+        if out is not None:
+            import time
+            self._clock = time.perf_counter
+            torch.cuda.synchronize()
+            self.t = self._clock()
+
+    def mark(self, name):
+        if self.out is None:
+            return
+        torch.cuda.synchronize()
+        t = self._clock()
+        self.out[name] = self.out.get(name, 0.0) + (t - self.t)
+        self.t = t
+
+
 def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
-          edge: str = "r200") -> Population:
+          edge: str = "r200", timings: dict = None) -> Population:
     """build_population(..., kap_floor=1e-8, trunc=edge, zint="midpoint")
     + Lam's flattening (sgl_full.py:1535, 2742-2755), on the device.  edge
     "rvir" truncates every halo at its virial radius (sgl_full.py:1876-1880)."""
@@ -87,16 +108,19 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
     if n_z < 1 or n_z > _lib.REF_MAX_SHELLS:
         raise ValueError("z_s outside the reference grid's shells")
     st = _stream()
+    clock = _Stages(timings)
     rec = torch.tensor(_records(cosmo, float(zs), n_z), dtype=torch.float64, device=dev)
     T = torch.empty(L.sgl_ref_tables_bytes(flags) // 8, dtype=torch.float64, device=dev)
     status = torch.zeros(1, dtype=torch.int32, device=dev)
     check(L.sgl_ref_tables_build(_ptr(rec), flags, _ptr(T), _ptr(status), st), "sgl_ref_tables_build")
+    clock.mark("tables")
     roots = torch.empty(L.sgl_ref_roots_bytes() // 8, dtype=torch.float64, device=dev)
     ntask = int(L.sgl_ref_tasks(flags))
     counts = torch.empty(ntask + 1, dtype=torch.int64, device=dev)
     check(L.sgl_ref_cells_count(_ptr(T), flags, _ptr(roots), _ptr(counts), _ptr(status), st),
           "sgl_ref_cells_count")
     n = int(counts[ntask].item())
+    clock.mark("cells_count")
     if n < 1:
         raise _lib.SglError(f"reference population is empty (status {int(status.item())})")
     K = torch.empty(n, dtype=torch.float64, device=dev)
@@ -105,12 +129,14 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
     key = torch.empty(n, dtype=torch.int32, device=dev)
     check(L.sgl_ref_cells_emit(_ptr(T), flags, _ptr(roots), _ptr(counts), _ptr(K), _ptr(G), _ptr(W),
                                _ptr(key), _ptr(status), st), "sgl_ref_cells_emit")
+    clock.mark("cells_emit")
     ws = torch.empty(L.sgl_ref_bin_bytes(n), dtype=torch.uint8, device=dev)
     pop = torch.empty(3 * REF_FINE_BINS, dtype=torch.float64, device=dev)
     npop = torch.zeros(1, dtype=torch.int32, device=dev)
     scal = torch.zeros(2, dtype=torch.float64, device=dev)
     check(L.sgl_ref_bin(n, _ptr(key), _ptr(K), _ptr(G), _ptr(W), _ptr(ws), _ptr(pop), _ptr(npop),
                         _ptr(scal), st), "sgl_ref_bin")
+    clock.mark("bin")
     s = int(status.item())
     if s:
         raise _lib.SglError(f"reference population status {s} (16: critical-curve roots beyond the "
@@ -247,20 +273,25 @@ class Result:
     pop: Population = None
 
 
-def invert(pop: Population, xi_out: np.ndarray, tol: float = PHI_TOL) -> Result:
+def invert(pop: Population, xi_out: np.ndarray, tol: float = PHI_TOL,
+           timings: dict = None) -> Result:
     """sgl_full.invert (sgl_full.py:159-375), Gaussian closure, no window."""
     L = _lib.load()
     dev = torch.device("cuda")
+    clock = _Stages(timings)
     g = grid_plan(pop, tol)
     k_k = np.linspace(0.0, g["k_k_max"], g["nk_k"])
     k_g = np.linspace(0.0, g["k_g_max"], g["nk_g"])
+    clock.mark("reach")
     phi = phi_grid(pop, k_k, k_g)
+    clock.mark("phi")
     nk, ng, nkap, ngam = g["nk_k"], g["nk_g"], g["n_kap"], g["n_gam"]
     ws = torch.empty(L.sgl_ref_p3_bytes(nk, ng, nkap, ngam) // 8, dtype=torch.float64, device=dev)
     p3 = torch.empty((nkap, ngam), dtype=torch.float64, device=dev)
     st = _stream()
     check(L.sgl_ref_p3(_ptr(phi), nk, ng, g["k_k_max"], g["k_g_max"], g["kap_min"], g["kap_max"],
                        nkap, g["gam_max"], ngam, _ptr(ws), _ptr(p3), st), "sgl_ref_p3")
+    clock.mark("p3")
     del phi, ws
     g_def = min(BOX_REACH * pop.sigma, g["gam_max"])
     xw = np.linspace(g["xi_min"], g["xi_max"], N_XI_WIDE)
@@ -276,15 +307,16 @@ def invert(pop: Population, xi_out: np.ndarray, tol: float = PHI_TOL) -> Result:
     check(L.sgl_ref_arcs(_ptr(p3), g["kap_min"], g["kap_max"], nkap, g["gam_max"], ngam, g_def,
                          _ptr(xod), xo.size, _ptr(mom), _ptr(P), st), "sgl_ref_arcs")
     m = mom.cpu().numpy()
+    clock.mark("arcs")
     return Result(xo, P.cpu().numpy(), pop.sigma, float(m[1]), float(m[2]), float(m[3]),
                   float(m[4]), float(m[0]), g, pop)
 
 
 def run(zs: float, cosmo: dict, ell: bool = False, fil: bool = False, xi_out=None,
-        edge: str = "r200") -> Result:
+        edge: str = "r200", timings: dict = None) -> Result:
     """One PDF on the reference's chain; xi_out None: the CLI body and tail
     grids linspace(-1, 1, 1601) and linspace(1.005, 9, 1600) (pipeline.py:211-215)."""
-    pop = build(cosmo, zs, ell=ell, fil=fil, edge=edge)
+    pop = build(cosmo, zs, ell=ell, fil=fil, edge=edge, timings=timings)
     if xi_out is None:
         xi_out = np.concatenate([np.linspace(-1.0, 1.0, 1601), np.linspace(1.005, 9.0, 1600)])
-    return invert(pop, xi_out)
+    return invert(pop, xi_out, timings=timings)
