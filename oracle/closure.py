@@ -131,19 +131,23 @@ class VP:
               for s in range(self.wk.shape[0])]
         return chain_variance(ms, self.w, self.K)
 
-    def calibrate(self):
+    def calibrate(self, lo=1e-3, hi=1e4):
         """eta with model variance = target (pos_closure.py:575-591: brentq in ln eta,
-        the bracket widened by 1.5 per side from [1/1.5, 1.5])."""
+        the bracket widened by 1.5 per side from [1/1.5, 1.5] while it stays
+        inside [lo, hi]; a root outside raises ValueError)."""
         fun = lambda le: self.variance(np.exp(le)) - self.target
         a, b = -np.log(1.5), np.log(1.5)
         fa, fb = fun(a), fun(b)
         while fa * fb > 0:
-            if fa > 0:
+            if fa > 0 and a > np.log(lo):
                 b, fb, a = a, fa, a - np.log(1.5)
                 fa = fun(a)
-            else:
+            elif fb < 0 and b < np.log(hi):
                 a, fa, b = b, fb, b + np.log(1.5)
                 fb = fun(b)
+            else:
+                raise ValueError(f"positive closure: clustering target {self.target:.3e} "
+                                 f"outside eta in [{lo}, {hi}]")
         return float(np.exp(brentq(fun, a, b, xtol=2e-12)))
 
     def lnR(self, Y):

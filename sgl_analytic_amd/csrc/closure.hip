@@ -320,19 +320,25 @@ __global__ __launch_bounds__(256) void k_vp_setup(const double* __restrict__ row
     return capped ? 1e300 : var - target;
   };
 
-  const double step = log(1.5);
+  // bracket from [1/1.5, 1.5], widened by 1.5 per side while it stays inside
+  // eta in [1e-3, 1e4]; a root outside that range is an error, as in
+  // pos_closure.Closure.calibrate (pos_closure.py:575-591)
+  const double step = log(1.5), llo = log(1e-3), lhi = log(1e4);
   double a = -step, bb = step;
   double fa = eval(a), fb = eval(bb);
-  for (int it = 0; it < 200 && fa * fb > 0.0; ++it) {
-    if (fa > 0.0) {
+  int flag = 0;
+  while (fa * fb > 0.0) {
+    if (fa > 0.0 && a > llo) {
       bb = a; fb = fa;
       a -= step; fa = eval(a);
-    } else {
+    } else if (fb < 0.0 && bb < lhi) {
       a = bb; fa = fb;
       bb += step; fb = eval(bb);
+    } else {
+      flag = kStatusVpRate;
+      break;
     }
   }
-  int flag = (fa * fb > 0.0) ? kStatusVpRate : 0;
   // Illinois false position inside the bracket (superlinear for this smooth,
   // monotone variance; a capped value forces a bisection step), to 4e-14 in
   // ln eta

@@ -98,3 +98,24 @@ def test_vp_filament_groups_end_to_end():
     assert abs(mv - mg) < 1e-3
     assert abs(rv.info["moments"]["mu_mean"] - rg.info["moments"]["mu_mean"]) < 1e-3
     assert 1e-7 < np.sum(np.abs(rv.p - rg.p)) * dx < 0.05
+
+
+def test_vp_calibration_domain():
+    """The calibration keeps eta inside [1e-3, 1e4] (pos_closure.py:575-591): a
+    target that needs eta far below the range raises, one inside calibrates to
+    the model variance."""
+    osh = chain.Shape(n_z=8, n_m=32, n_b=64, n_k=1024, n_out=256)
+    cos = Cosmology()
+    T = chain.rows(cos, 1.0, osh)
+    R = T["rows"]
+    xi, ws, _, _ = chain.cells(R, osh)
+    C = chain.slice_covariance(cos, T["chi_edges"], T["chis"])
+    vp = closure.VP(R[:, :, 5], chain.vp_inputs(xi, ws, R[:, :, 5]), C)
+    eta0 = vp.eta
+    vp.target *= 1e-6
+    with pytest.raises(ValueError):
+        vp.calibrate()
+    vp.target *= 1e6 * 3.0
+    eta = vp.calibrate()
+    assert 1e-3 <= eta <= 1e4 and eta > eta0
+    assert abs(vp.variance(eta) / vp.target - 1) < 1e-9
