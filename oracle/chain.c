@@ -28,6 +28,12 @@
  *   oracle_invert     P(xi) = (1/pi) Re int_0^inf e^{-ik xi} phi(k) dk
  *                     (sgl.py:994-1012) with phi linear between k nodes
  *                     (Filon weights), phi(0) = 1
+ *   oracle_sub_ring   ring average of a host's projected clump density and
+ *                     its mean clump-ray cosine (subhalos.py:287-330)
+ *   oracle_sub_emit   the points of decorated host cells on the rebuilt host
+ *                     (subhalos_exact.py:802-925)
+ *   oracle_bin_fine   Cells.add's fine (kappa, gamma) lattice sums
+ *                     (sgl_full.py:1314-1383, 971-1046)
  *
  * Build: oracle/build.py (gcc -O2 -fopenmp -shared).
  */
@@ -443,4 +449,150 @@ void oracle_invert(int64_t n_t, const double *t, int64_t nq, const double *kq, c
         }
         p[j] = acc / M_PI;
     }
+}
+
+/* ------------------------------------------------------------------------
+ * Subhalo sector (exact decorated host, moments method).
+ *
+ * oracle_sub_ring: the ring average of a host's projected clump density and
+ * the mean cosine of the clump-ray angle it weights
+ * (subhalos.py:287-330, _ring_and_pair_angle_mean_cos):
+ *   ring[i][k] = (1/nt) sum_t n(R_t),  ecd[i][k] = sum_t n(R_t) cth_t / sum_t n(R_t),
+ *   R_t = sqrt(r_i^2 + d_k^2 - 2 r_i d_k cth_t), n the linear interpolant of
+ *   (xp, fp) on the uniform grid xp (fp[0] below, 0 above, np.interp's
+ *   fused slope (x - xp_j) + fp_j).
+ */
+void oracle_sub_ring(int64_t nr, const double *r, int64_t nd, const double *d, int64_t nt,
+                     const double *cth, int64_t n, const double *xp, const double *fp,
+                     double *ring, double *ecd) {
+    double x0 = xp[0], xl = xp[n - 1], inv_dx = (n - 1) / (xl - x0);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nr; ++i) {
+        for (int64_t k = 0; k < nd; ++k) {
+            double c = r[i] * r[i] + d[k] * d[k], trd = 2.0 * r[i] * d[k];
+            double s = 0.0, sc = 0.0;
+            for (int64_t t = 0; t < nt; ++t) {
+                double arg = c - trd * cth[t];
+                double x = arg > 0.0 ? sqrt(arg) : 0.0, v;
+                if (x <= x0) {
+                    v = fp[0];
+                } else if (x >= xl) {
+                    v = (x == xl) ? fp[n - 1] : 0.0;
+                } else {
+                    int64_t j = (int64_t)((x - x0) * inv_dx);
+                    if (j > n - 2) j = n - 2;
+                    if (j < 0) j = 0;
+                    double sl = (fp[j + 1] - fp[j]) / (xp[j + 1] - xp[j]);
+                    v = fma(sl, x - xp[j], fp[j]);
+                }
+                s += v;
+                sc += v * cth[t];
+            }
+            ring[i * nd + k] = s / (double)nt;
+            ecd[i * nd + k] = s > 0.0 ? sc / s : 0.0;
+        }
+    }
+}
+
+/*
+ * oracle_sub_emit: the points of decorated host cells (subhalos_exact.py:
+ * 802-925, _moments_fused_general).  For every cell (x, phi, W) and node q:
+ * the coarse-grid tables [n_g][nq] (K offset, removed mass S, weight, shear
+ * cross term A1, shear square A2) interpolated linearly in ln x (clamped),
+ * the host rebuilt at max(M - S, Mmin) with (ln r_s, ln k_s, eps) linear in
+ * ln M on the table (clamped), kept at the host's physical truncation
+ * radius, its pseudo-elliptical truncated kappa_h, gamma_h at the cell
+ * (kg_eps), and
+ *   K = kappa_h + K_q,  G = sqrt(max(gamma_h^2 + 2 gamma_h A1 + A2, 0)),  W_q = W w_q.
+ * phq NULL: circular (phi = 0, eps = 0).  Output [n_cell][nq].
+ */
+void oracle_sub_emit(int64_t n_cell, const double *xq, const double *phq, const double *W,
+                     double rs, double xt, int64_t nq, int64_t n_g, double xg0, const double *Kp,
+                     const double *Sp, const double *wp, const double *A1, const double *A2,
+                     const double *lxg, int64_t m_t, const double *lMt, const double *log_rs,
+                     const double *log_ks, const double *ept, double M, double Mmin, double *Ko,
+                     double *Go, double *Wo) {
+    if (!gl_ready) gl_init();
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n_cell; ++i) {
+        double x = xq[i], phi = phq ? phq[i] : 0.0;
+        double lx = x > xg0 ? log(x) : lxg[0];
+        if (lx > lxg[n_g - 1]) lx = lxg[n_g - 1];
+        int64_t lo = 0, hi = n_g - 1;
+        while (lo < hi - 1) {
+            int64_t mid = (lo + hi) >> 1;
+            if (lxg[mid] <= lx) lo = mid; else hi = mid;
+        }
+        double tw = (lx - lxg[lo]) / (lxg[lo + 1] - lxg[lo]), om = 1.0 - tw;
+        for (int64_t q = 0; q < nq; ++q) {
+            int64_t a = lo * nq + q, b = (lo + 1) * nq + q;
+            double kp = om * Kp[a] + tw * Kp[b], sp = om * Sp[a] + tw * Sp[b];
+            double w = om * wp[a] + tw * wp[b], a1 = om * A1[a] + tw * A1[b];
+            double a2 = om * A2[a] + tw * A2[b];
+            double Me = M - sp;
+            if (Me < Mmin) Me = Mmin;
+            double lMe = log(Me), rse, kse, epe;
+            if (lMe <= lMt[0]) {
+                rse = exp(log_rs[0]); kse = exp(log_ks[0]); epe = ept[0];
+            } else if (lMe >= lMt[m_t - 1]) {
+                rse = exp(log_rs[m_t - 1]); kse = exp(log_ks[m_t - 1]); epe = ept[m_t - 1];
+            } else {
+                int64_t l = 0, h = m_t - 1;
+                while (l < h - 1) {
+                    int64_t mid = (l + h) >> 1;
+                    if (lMt[mid] <= lMe) l = mid; else h = mid;
+                }
+                double f = (lMe - lMt[l]) / (lMt[l + 1] - lMt[l]);
+                rse = exp(log_rs[l] + f * (log_rs[l + 1] - log_rs[l]));
+                kse = exp(log_ks[l] + f * (log_ks[l + 1] - log_ks[l]));
+                epe = ept[l] + f * (ept[l + 1] - ept[l]);
+            }
+            double kh, gh;
+            kg_eps(phq ? epe : 0.0, kse, x * rs / rse, phi, xt * rs / rse, &kh, &gh);
+            double g2 = gh * gh + 2.0 * gh * a1 + a2;
+            Ko[i * nq + q] = kh + kp;
+            Go[i * nq + q] = g2 > 0.0 ? sqrt(g2) : 0.0;
+            Wo[i * nq + q] = W[i] * w;
+        }
+    }
+}
+
+/*
+ * oracle_bin_fine: Cells.add's binning into the fine (kappa, gamma) lattice
+ * (sgl_full.py:1314-1383, 971-1046): gamma clipped below at 1.0000001 g_lo,
+ * bins by searchsorted(edges, v, "right") with the last edge inside the last
+ * bin, points outside the lattice dropped (their weight summed into *dropped);
+ * weight, weight kappa and weight gamma accumulated in the points' order.
+ */
+static int64_t bin_right(const double *e, int64_t n, double v) {
+    int64_t lo = 0, hi = n;               /* first index with e[i] > v */
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (e[mid] > v) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+void oracle_bin_fine(int64_t n, const double *K, const double *G, const double *W,
+                     const double *ek, int64_t nek, const double *eg, int64_t neg,
+                     double *H0, double *H0k, double *H0g, double *dropped) {
+    int64_t nkf = nek - 1, ngf = neg - 1;
+    double drop = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        double k = K[i], g = G[i], w = W[i];
+        double gl = 1.0000001 * eg[0];
+        if (g < gl) g = gl;
+        int64_t a = bin_right(ek, nek, k), b = bin_right(eg, neg, g);
+        if (k == ek[nek - 1]) a -= 1;
+        if (g == eg[neg - 1]) b -= 1;
+        if (a >= 1 && a <= nkf && b >= 1 && b <= ngf) {
+            int64_t f = (a - 1) * ngf + (b - 1);
+            H0[f] += w;
+            H0k[f] += w * k;
+            H0g[f] += w * g;
+        } else {
+            drop += w;
+        }
+    }
+    *dropped += drop;
 }

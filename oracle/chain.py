@@ -68,6 +68,11 @@ def lib():
         _LIB.oracle_kg_eps.argtypes = [i64, d, d, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, d, d]
         _LIB.oracle_cells_ell.argtypes = [i64, d, d, i64, d, i64, d, d, d, d]
+        _LIB.oracle_sub_ring.argtypes = [i64, d, i64, d, i64, d, i64, d, d, d, d]
+        f64 = ctypes.c_double
+        _LIB.oracle_bin_fine.argtypes = [i64, d, d, d, d, i64, d, i64, d, d, d, d]
+        _LIB.oracle_sub_emit.argtypes = [i64, d, d, d, f64, f64, i64, i64, f64, d, d, d, d, d, d,
+                                         i64, d, d, d, d, f64, f64, d, d, d]
     return _LIB
 
 

@@ -48,6 +48,8 @@ class Cosmology:
         if growth_mode == "legacy" and not (w0 == -1.0 and wa == 0.0):
             raise ValueError("legacy growth is only valid for LambdaCDM")
         self.growth_mode = growth_mode
+        self.kw = dict(h=h, Om=Om, s8=s8, Ob=Ob, ns=ns, zeq=zeq, Ok=Ok, w0=w0, wa=wa,
+                       radiation=radiation, growth_mode=growth_mode)
         self.h, self.Om, self.s8, self.Ob, self.ns = h, Om, s8, Ob, ns
         self.zeq, self.Ok, self.w0, self.wa = float(zeq), float(Ok), float(w0), float(wa)
         self.Or = self.Om / (1.0 + self.zeq) if radiation == "engine" else 0.0   # sgl.py:100

@@ -47,6 +47,7 @@ closed-form density, and brute-force Lambda on a tiny table.
 from __future__ import annotations
 
 import math
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -200,11 +201,13 @@ def _centres(e):
     return np.where(e[:-1] > 0, sqrt(e[:-1] * np.maximum(e[1:], 0.0)), 0.5 * e[1])
 
 
-def host_cells(ks, xt, xm, Nbar, eps=0.0, phis=None):
-    """(K, G, W) of one (z, M) row of truncated halos (_truncated_host,
-    sgl_full.py:2025-2138, without subhalos): inner grid with the
-    truncation edges and critical-curve refinement, (pseudo-elliptical)
-    truncated NFW at the geometric centres, circular monopole tail."""
+def host_grid(ks, xt, xm, Nbar, eps=0.0, phis=None):
+    """The cell grid of one (z, M) row of truncated halos (_truncated_host,
+    sgl_full.py:2025-2087): the circular inner grid with the truncation edges
+    and critical-curve refinement (centres xc), the evaluated cells (xq,
+    orientation phq or None, weight W: per orientation on the
+    ``_refine_edges_eps_batch`` grids for elliptical halos) and the monopole
+    tail out to x_m at 12 points per decade (xo, Wo)."""
     x_ell = min(xt / sqrt(1 - eps), xm) if eps > 0 else min(xt, xm)
     x_in = 1e-6 * xt
     nx_eff = NX if ks > 0.05 else max(80, NX // 3)
@@ -213,26 +216,39 @@ def host_cells(ks, xt, xm, Nbar, eps=0.0, phis=None):
     xe = np.unique(np.concatenate([xe, [xt * (1 - 1e-9), xt, xt * (1 + 1e-9)]]))
     xe = xe[xe <= x_ell]
     xe = refine_edges(xe, ks, x_ell)
+    xc = _centres(xe)
     if phis is not None:
         eps_edges = refine_edges_eps(xe, eps, ks, x_ell, phis, xt)
         xq = np.concatenate([_centres(e) for e in eps_edges])
         phq = np.concatenate([np.full(e.size - 1, p) for e, p in zip(eps_edges, phis)])
         W = np.concatenate([Nbar * (e[1:] ** 2 - e[:-1] ** 2) / xm ** 2 / len(phis)
                             for e in eps_edges])
-        K, G = kg_eps(xq, phq, eps, ks, xt)
     else:
-        xc = _centres(xe)
+        xq, phq = xc, None
         W = Nbar * (xe[1:] ** 2 - xe[:-1] ** 2) / xm ** 2
-        K, G = trunc_kg(xc, ks, xt)
     if xm > x_ell * (1 + 1e-12):
         n_out = max(8, int(np.ceil(12 * np.log10(xm / x_ell))))
         eo = np.geomspace(x_ell, xm, n_out + 1)
         xo = sqrt(eo[:-1] * eo[1:])
-        Ko = np.zeros_like(xo)
-        Go = ks * 4 * (log(1 + xt) - xt / (1 + xt)) / xo ** 2
         Wo = Nbar * (eo[1:] ** 2 - eo[:-1] ** 2) / xm ** 2
-        K, G, W = np.concatenate([K, Ko]), np.concatenate([G, Go]), np.concatenate([W, Wo])
-    return K, G, W
+    else:
+        xo = Wo = np.zeros(0)
+    return dict(xc=xc, xq=xq, phq=phq, W=W, xo=xo, Wo=Wo)
+
+
+def host_cells(ks, xt, xm, Nbar, eps=0.0, phis=None, grid=None):
+    """(K, G, W) of one (z, M) row of truncated halos without subhalos
+    (_truncated_host, sgl_full.py:2025-2138): the (pseudo-elliptical)
+    truncated NFW at the cell centres, circular monopole shear on the tail."""
+    g = grid if grid is not None else host_grid(ks, xt, xm, Nbar, eps, phis)
+    if g["phq"] is not None:
+        K, G = kg_eps(g["xq"], g["phq"], eps, ks, xt)
+    else:
+        K, G = trunc_kg(g["xq"], ks, xt)
+    xo = g["xo"]
+    Go = ks * 4 * (log(1 + xt) - xt / (1 + xt)) / xo ** 2
+    return (np.concatenate([K, np.zeros_like(xo)]), np.concatenate([G, Go]),
+            np.concatenate([g["W"], g["Wo"]]))
 
 
 def kappa_gamma_cyl(u, kappa0):
@@ -255,6 +271,7 @@ class Population:
     N_fil: float = 0.0
     kbar: float = 0.0
     n_points: int = 0
+    n_decorated: int = 0
     dropped: float = 0.0
     rows: list = field(default_factory=list)
 
@@ -263,14 +280,16 @@ class Population:
         return float(sqrt(np.sum(self.w * self.k ** 2)))
 
 
-def rows(cos: Cosmology, zs: float, ell=False, fil=False, edge="r200"):
+def rows(cos: Cosmology, zs: float, ell=False, fil=False, edge="r200", shells=None):
     """Per-(shell, mass) parameters of _build_shell_into (sgl_full.py:1848-1884,
-    1984-1991) as dicts, shell-major.  edge "r200": x_t = c; "rvir": x_t =
-    r_vir / r_s (sgl_full.py:1874-1880)."""
+    1984-1991) as dicts, shell-major (only the listed shells when given).
+    edge "r200": x_t = c; "rvir": x_t = r_vir / r_s (sgl_full.py:1874-1880)."""
     _, Ml = engine_grids()
     dlnM = np.diff(log(Ml))
     out = []
     for i, (_, _, z, dz) in enumerate(z_shells(zs)):
+        if shells is not None and i not in shells:
+            continue
         wz = CKMS / cos.Hz(z) * dz * pi * (1 + z) ** 2
         D = cos.Dg(z)
         scr = cos.Sigma_cr(z, zs)
@@ -295,58 +314,106 @@ def rows(cos: Cosmology, zs: float, ell=False, fil=False, edge="r200"):
     return out
 
 
-def build(cos: Cosmology, zs: float, ell=False, fil=False, keep_points=False, edge="r200"):
-    """build_population(..., kap_floor=1e-8, trunc="r200", zint="midpoint")
-    without subhalos and clustering: every row's cells binned into the fine
-    lattice with weight and centroid sums."""
+def _accumulate(cos, zs, ell, fil, edge, sub, shells=None, keep_points=False):
+    """The rows of the given shells binned into the fine lattice: the sums
+    (w, w kappa, w gamma) per bin and the builder's bookkeeping."""
     ek, eg = fine_edges()
     nkf, ngf = ek.size - 1, eg.size - 1
-    H0, H0k, H0g = (np.zeros(nkf * ngf) for _ in range(3))
+    acc = dict(H0=np.zeros(nkf * ngf), H0k=np.zeros(nkf * ngf), H0g=np.zeros(nkf * ngf),
+               N_expl=0.0, N_fil=0.0, kbar=0.0, n_points=0, n_decorated=0, drop=np.zeros(1),
+               pts=[], model=None)
     phis = (np.arange(NPHI) + 0.5) * (0.5 * pi / NPHI)
     fph, fwt = chain.fil_phi_rule()
     ue = chain.fil_u_edges()
     uc = np.where(ue[:-1] > 0, sqrt(ue[:-1] * ue[1:]), 0.5 * ue[1])
     ufa = (ue[1:] ** 2 - ue[:-1] ** 2)
-    pop = Population(np.zeros(0), np.zeros(0), np.zeros(0))
-    pts = []
 
     def add(K, G, W):
-        K, G, W = K.ravel(), G.ravel(), W.ravel()
-        G = np.clip(G, 1.0000001 * FINE_G[0], None)
-        a = np.searchsorted(ek, K, side="right")
-        b = np.searchsorted(eg, G, side="right")
-        a = np.where(K == ek[-1], a - 1, a)
-        b = np.where(G == eg[-1], b - 1, b)
-        ok = (a >= 1) & (a <= nkf) & (b >= 1) & (b <= ngf)
-        f = (a[ok] - 1) * ngf + (b[ok] - 1)
-        H0[:] += np.bincount(f, W[ok], minlength=H0.size)
-        H0k[:] += np.bincount(f, W[ok] * K[ok], minlength=H0.size)
-        H0g[:] += np.bincount(f, W[ok] * G[ok], minlength=H0.size)
-        pop.dropped += float(W.sum() - W[ok].sum())
-        pop.n_points += K.size
+        K, G, W = (np.ascontiguousarray(a, dtype=float).ravel() for a in (K, G, W))
+        chain.lib().oracle_bin_fine(K.size, chain._p(K), chain._p(G), chain._p(W), chain._p(ek),
+                                    ek.size, chain._p(eg), eg.size, chain._p(acc["H0"]),
+                                    chain._p(acc["H0k"]), chain._p(acc["H0g"]),
+                                    chain._p(acc["drop"]))
+        acc["n_points"] += K.size
+        acc["kbar"] += float(np.sum(W * K))
         if keep_points:
-            pts.append((K, G, W))
+            acc["pts"].append((K, np.clip(G, 1.0000001 * FINE_G[0], None), W))
 
-    for r in rows(cos, zs, ell=ell, fil=fil, edge=edge):
-        pop.N_expl += r["Nbar"]
-        K, G, W = host_cells(r["ks"], r["xt"], r["xm"], r["Nbar"], r["eps"],
-                             phis if ell else None)
+    model = None
+    if sub:
+        from . import sub as _sub
+        zl, Ml = np.array([sh[2] for sh in z_shells(zs)]), engine_grids()[1]
+        model = acc["model"] = _sub.SubhaloModel(cos, zs, zl, Ml)
+    for r in rows(cos, zs, ell=ell, fil=fil, edge=edge, shells=shells):
+        acc["N_expl"] += r["Nbar"]
+        grid = host_grid(r["ks"], r["xt"], r["xm"], r["Nbar"], r["eps"], phis if ell else None)
+        h = model.host(r["shell"], r["jM"]) if model is not None else None
+        if h is not None:
+            K, G, W = _sub.decorate(model, h, grid, r["ks"], r["xt"], r["eps"])
+            acc["n_decorated"] += 1
+        else:
+            K, G, W = host_cells(r["ks"], r["xt"], r["xm"], r["Nbar"], r["eps"], grid=grid)
         add(K, G, W)
-        pop.kbar += float(np.sum(W * K))
         if fil:
-            pop.N_fil += r["NF"]
+            acc["N_fil"] += r["NF"]
             k0 = r["k0base"] / np.maximum(2 * r["rsF"], np.abs(np.cos(fph)) * r["LF"])
             K, G = kappa_gamma_cyl(uc[:, None], k0[None, :])
             W = r["NF"] * ufa[:, None] * fwt[None, :] * np.ones_like(K)
             add(K, G, W)
-            pop.kbar += float(np.sum(W * K))
+    return acc
+
+
+def _worker(args):
+    cos_kw, zs, ell, fil, edge, sub, shells = args
+    acc = _accumulate(Cosmology(**cos_kw), zs, ell, fil, edge, sub, shells)
+    acc.pop("model")
+    acc.pop("pts")
+    return acc
+
+
+def build(cos: Cosmology, zs: float, ell=False, fil=False, keep_points=False, edge="r200",
+          sub=False, workers=None):
+    """build_population(..., kap_floor=1e-8, trunc="r200", zint="midpoint")
+    without clustering: every row's cells (decorated by subhalos on hosts
+    that carry them when ``sub``, oracle/sub.py) binned into the fine lattice
+    with weight and centroid sums.  With subhalos the shells are shared out
+    over ``workers`` processes (default: up to 8), each binning its shells,
+    and the partial sums are added in worker order."""
+    n_sh = len(z_shells(zs))
+    if workers is None:
+        workers = min(8, os.cpu_count() or 1) if sub else 1
+    workers = max(1, min(int(workers), n_sh))
+    if workers == 1 or keep_points:
+        accs = [_accumulate(cos, zs, ell, fil, edge, sub, None, keep_points)]
+    else:
+        import multiprocessing as mp
+        jobs = [(cos.kw, zs, ell, fil, edge, sub, set(range(w, n_sh, workers)))
+                for w in range(workers)]
+        # fresh interpreters (no inherited OpenMP or device state), one
+        # OpenMP thread each
+        old = os.environ.get("OMP_NUM_THREADS")
+        os.environ["OMP_NUM_THREADS"] = "1"
+        try:
+            with mp.get_context("spawn").Pool(workers) as pool:
+                accs = pool.map(_worker, jobs)
+        finally:
+            if old is None:
+                os.environ.pop("OMP_NUM_THREADS", None)
+            else:
+                os.environ["OMP_NUM_THREADS"] = old
+    H0, H0k, H0g = (sum(a[k] for a in accs) for k in ("H0", "H0k", "H0g"))
+    pop = Population(np.zeros(0), np.zeros(0), np.zeros(0))
+    for k in ("N_expl", "N_fil", "kbar", "n_points", "n_decorated"):
+        setattr(pop, k, sum(a[k] for a in accs))
+    pop.dropped = float(sum(a["drop"][0] for a in accs))
+    pop.sub_model = accs[0].get("model")
     nz = np.nonzero(H0)[0]
     pop.w = H0[nz]
     pop.k = H0k[nz] / pop.w
     pop.g = H0g[nz] / pop.w
     pop.bins = nz
     if keep_points:
-        pop.points = pts
+        pop.points = accs[0]["pts"]
     return pop
 
 
@@ -581,11 +648,12 @@ XI_BODY = np.linspace(-1.0, 1.0, 1601)    # cli.py / explorer body grid
 XI_TAIL = np.linspace(1.005, 9.0, 1600)   # and tail grid
 
 
-def run(zs, cosmo: dict = None, ell=False, fil=False, xi_out=None, keep=False, edge="r200"):
-    """pipeline.run for the ingredient sets without subhalos and clustering.
+def run(zs, cosmo: dict = None, ell=False, fil=False, xi_out=None, keep=False, edge="r200",
+        sub=False):
+    """pipeline.run for the ingredient sets without clustering.
     xi_out None: the body and tail grids of the CLI (pipeline.py:211-215)."""
     cos = Cosmology(**(cosmo or {}))
-    pop = build(cos, zs, ell=ell, fil=fil, edge=edge)
+    pop = build(cos, zs, ell=ell, fil=fil, edge=edge, sub=sub)
     sig = pop.sigma()
     xo = np.concatenate([XI_BODY, XI_TAIL]) if xi_out is None else np.asarray(xi_out, float)
     res = invert(lambda a, b: lam(pop, a, b), sig, xo, keep=keep)

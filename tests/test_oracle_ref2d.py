@@ -1,7 +1,7 @@
 """Pins of the 2-D (k_kappa, k_gamma) oracle (oracle/ref2d.py) on the CPU.
 
 * The explorer notebook's ingredient ladder at z_s = 1 (Planck): halos,
-  + ellipticity, + filaments; the printed 5 significant figures of
+  + ellipticity, + filaments, + subhalos; the printed 5 significant figures of
   sigma_kappa, the clipped variance and skewness, sigma_DL/D_L and
   P(ln mu > 1) (tests/golden/reference_scalars.json, extracted from the
   notebook's output by tests/golden/make_golden.py).
@@ -44,7 +44,8 @@ def cos():
 
 
 @pytest.mark.parametrize("model,kw", [("halos", {}), ("+ ellipticity", dict(ell=True)),
-                                      ("+ filaments", dict(ell=True, fil=True))])
+                                      ("+ filaments", dict(ell=True, fil=True)),
+                                      ("+ subhalos", dict(ell=True, fil=True, sub=True))])
 def test_ladder_row(model, kw):
     r = ref2d.run(1.0, **kw)
     row = ref2d.ladder_row(r)
