@@ -241,14 +241,16 @@ def main():
 def ref_chain_leg(log, zs=1.0, reps=3):
     """generate_pdf_lnmu's chain (the reference's 2-D chain) at z_s = 1, Planck,
     on the default ln mu grid linspace(-1, 9, 3201): seconds per PDF and the
-    stage split, for halos and for halos + ellipticity + filaments (the
-    ingredients built on it), best of ``reps`` after one warm-up."""
+    stage split, for halos, halos + ellipticity + filaments, and with
+    subhalos on top (the ingredients built on it), best of ``reps`` after one
+    warm-up."""
     import torch
     from sgl_analytic_amd import api, ref2d
     cp = api.check_cosmology({})
     x = np.linspace(-1.0, 9.0, 3201)
     out = {}
-    for name, kw in (("halo", {}), ("ell+fil", dict(ell=True, fil=True))):
+    for name, kw in (("halo", {}), ("ell+fil", dict(ell=True, fil=True)),
+                     ("ell+fil+sub", dict(ell=True, fil=True, sub=True))):
         ref2d.run(zs, cp, xi_out=x, **kw)
         torch.cuda.synchronize()
         best, stages, res = None, None, None
@@ -362,9 +364,6 @@ def precision_study(args, B):
     from sgl_analytic_amd import solver as S
     from sgl_analytic_amd._lib import PREC_BF16X3, PREC_FP32
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
-    if args.ref_only:
-        print(json.dumps({"ref_chain": ref_chain_leg(log)}), flush=True)
-        return
     params = lhs_params(B)
     n_z = S.slice_count(params[:, 9])
     shape = S.Shape(n_z_max=int(n_z.max()))

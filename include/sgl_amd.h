@@ -47,6 +47,11 @@ typedef enum sgl_status {
                                  (sgl_full.py:480-518, zint="midpoint"), and the engine's
                                  mass nodes 1..99 of exp(linspace(ln 1e7, ln 1e17, 100))
                                  Msun (sgl_full.py:471-474, 1864) */
+#define SGL_FLAG_SUBHALOS 64  /* reference-shape path only: subhalos on their hosts (the
+                                 "sub" ingredient, exact decorated-host sector, moments
+                                 method, clumps truncated at their r200; subhalos.py:
+                                 540-772, subhalos_exact.py:1531-1931); the tables carry
+                                 the reference's sigma(M) spline and a dense growth table */
 
 /* Batched path: slice amplitudes and Psi_2 at every SGL_PSI2_STRIDE-th k node
  * (sub-grid node jc = k node jc * stride of the log grid, the last one up to
@@ -299,7 +304,8 @@ sgl_status sgl_diagnostics(const void* p, int32_t p_is_double, int32_t B, const 
  * engine mass grid, adaptive impact-parameter cells, the 961 x 300 fine
  * lattice, width-relative k and (kappa, gamma) grids).  The host drives it in
  * the order of pipeline.run (sgl_analytic/pipeline.py:165-241): tables ->
- * cells (count, then emit) -> binning -> reach probes (sgl_ref_lam_axis) ->
+ * critical curves -> subhalo tables (with SGL_FLAG_SUBHALOS) -> cells (count,
+ * then emit) -> binning -> reach probes (sgl_ref_lam_axis) ->
  * Phi on the tensor grid -> P3 -> arc integrals and normalisation.  Grid
  * sizes depend on device results (cell count, sigma, reach), so the host reads
  * those scalars between calls.
@@ -314,7 +320,8 @@ int32_t sgl_ref_shell_count(double z_s);
 /* (row, orientation) tasks of the cell kernels: 104 x 99 x (12 with SGL_FLAG_ELL, else 1). */
 int32_t sgl_ref_tasks(int32_t flags);
 size_t sgl_ref_tables_bytes(int32_t flags);    /* flags: SGL_FLAG_ELL | SGL_FLAG_FILAMENTS |
-                                                  SGL_FLAG_EDGE_RVIR (x_t = r_vir / r_s) */
+                                                  SGL_FLAG_EDGE_RVIR (x_t = r_vir / r_s) |
+                                                  SGL_FLAG_SUBHALOS */
 size_t sgl_ref_roots_bytes(void);
 
 /*
@@ -328,27 +335,56 @@ sgl_status sgl_ref_tables_build(const double* cosmo, int32_t flags, void* tables
                                 void* stream);
 
 /*
- * Cells, first pass: per row the critical-curve roots (roots buffer), per
- * (row, orientation) task the cell count; counts (int64[tasks + 1]) is left
- * holding the exclusive offsets, counts[tasks] = total cells.  status |= 16
- * (more roots than the per-row slots) or 32 (more edges than a task holds).
+ * Critical curves: per (shell, mass) row the radii where detA of the
+ * untruncated profile changes sign (45 bisections on sgl._XG) and, with
+ * SGL_FLAG_ELL, the roots of every orientation (roots: sgl_ref_roots_bytes).
+ * status |= 16 when a row has more roots than its slots.
  * Replaces: _refine_edges / _refine_edges_eps_batch (sgl_full.py:2342-2372,
- * 2279-2339) and the grid of _truncated_host (sgl_full.py:2025-2050).
+ * 2279-2339).
  */
-sgl_status sgl_ref_cells_count(const void* tables, int32_t flags, double* roots, int64_t* counts,
-                               int32_t* status, void* stream);
+sgl_status sgl_ref_roots(const void* tables, int32_t flags, double* roots, int32_t* status,
+                         void* stream);
+
+/*
+ * Subhalo tables (flags must hold SGL_FLAG_SUBHALOS, and not SGL_FLAG_EDGE_RVIR;
+ * sub: sgl_ref_sub_bytes(), after sgl_ref_roots): per host row the clump law
+ * (f_s, the 28 clump mass bins, their NFW parameters and reach, the projected
+ * clump density, the clump cells' kappa and gamma), the coarse host radii up
+ * to the largest host-grid radius within reach of a clump, and on them the
+ * decorated-host point tables (K offset, removed mass, weight, shear terms;
+ * 9 + 2 x 3 points) and the rebuilt host's mass table.
+ * Replaces: SubhaloModel.fs / host (subhalos.py:672-772), ExactSector.
+ * _clump_cells and host_cells_moments up to the emission (subhalos_exact.py:
+ * 1553-1622, 1647-1811) and _ring_and_pair_angle_mean_cos (subhalos.py:287-330).
+ */
+size_t sgl_ref_sub_bytes(void);
+sgl_status sgl_ref_sub_build(const void* tables, int32_t flags, const double* roots, void* sub,
+                             int32_t* status, void* stream);
+
+/*
+ * Cells, first pass: per (row, orientation) task the cell count; counts
+ * (int64[tasks + 1]) is left holding the exclusive offsets, counts[tasks] =
+ * total cells.  status |= 32 (more edges than a task holds).  sub: the
+ * subhalo tables when flags hold SGL_FLAG_SUBHALOS, else NULL (a host cell
+ * within reach of a clump counts 9 + 2 x 3 points, or 9).
+ * Replaces: the grid of _truncated_host (sgl_full.py:2025-2050, 2088-2127).
+ */
+sgl_status sgl_ref_cells_count(const void* tables, int32_t flags, const double* roots,
+                               const void* sub, int64_t* counts, int32_t* status, void* stream);
 /*
  * Cells, second pass: kappa, gamma (clipped below at 1.0000001e-9), weight and
- * the fine-lattice key (bin, or SGL_REF_FINE_BINS outside) of every cell, in
- * the reference's emission order (per row: the inner cells orientation by
- * orientation, the monopole tail, the filament cells).
- * Replaces: _truncated_host (sgl_full.py:2025-2138), the filament block of
+ * the fine-lattice key (bin, or SGL_REF_FINE_BINS outside) of every cell:
+ * per row the inner cells orientation by orientation (on a host with
+ * subhalos the points of the decorated cells first, then the cells beyond
+ * reach), the monopole tail (likewise), the filament cells.
+ * Replaces: _truncated_host (sgl_full.py:2025-2138), the emission of the
+ * decorated host (subhalos_exact.py:802-925), the filament block of
  * _build_shell_into (sgl_full.py:1984-2007) and the bin search of
  * _add_direct_kernel (sgl_full.py:971-1046).
  */
 sgl_status sgl_ref_cells_emit(const void* tables, int32_t flags, const double* roots,
-                              const int64_t* offsets, double* kap, double* gam, double* w,
-                              int32_t* key, int32_t* status, void* stream);
+                              const void* sub, const int64_t* offsets, double* kap, double* gam,
+                              double* w, int32_t* key, int32_t* status, void* stream);
 
 /*
  * Binning: stable sort by key, per-bin sums of w, w kappa, w gamma in the cells'

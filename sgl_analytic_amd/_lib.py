@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libsgl_amd.so")
 
 SGL_OK, SGL_ERR_INVALID, SGL_ERR_HIP, SGL_ERR_UNSUPPORTED = 0, 1, 2, 3
 FLAG_CLUSTERING, FLAG_WINDOW, FLAG_FILAMENTS, FLAG_EDGE_RVIR, FLAG_ELL = 1, 2, 4, 8, 16
+FLAG_SUBHALOS = 64
 ELL_PHI, MSTAR_N = 12, 1200   # orientation nodes per elliptical cell; M* sigma table
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 COSMO_STRIDE = 16
@@ -31,7 +32,7 @@ EXPORTS = (
     "sgl_ref_roots_bytes", "sgl_ref_tables_build", "sgl_ref_cells_count", "sgl_ref_cells_emit",
     "sgl_ref_bin_bytes", "sgl_ref_bin", "sgl_ref_lam_axis", "sgl_ref_phi_bytes", "sgl_ref_phi",
     "sgl_ref_p3_bytes", "sgl_ref_p3", "sgl_ref_arcs", "sgl_ref_moments", "sgl_ref_gemm_f64",
-    "sgl_lnphi_exact_noalloc",
+    "sgl_lnphi_exact_noalloc", "sgl_ref_roots", "sgl_ref_sub_bytes", "sgl_ref_sub_build",
 )
 NOALLOC_SCRATCH_DOUBLES = 1 << 22   # capacity of sgl_lnphi_exact_noalloc's own array
 REF_MAX_SHELLS, REF_MASSES, REF_FINE_BINS, REF_ROOT_STRIDE = 104, 99, 961 * 300, 128
@@ -91,6 +92,8 @@ def load(path: str = LIB_PATH):
     L.sgl_ref_tables_bytes.argtypes = [i32]
     L.sgl_ref_roots_bytes.restype = sz
     L.sgl_ref_roots_bytes.argtypes = []
+    L.sgl_ref_sub_bytes.restype = sz
+    L.sgl_ref_sub_bytes.argtypes = []
     L.sgl_ref_bin_bytes.restype = sz
     L.sgl_ref_bin_bytes.argtypes = [i64]
     L.sgl_ref_phi_bytes.restype = sz
@@ -123,8 +126,10 @@ def load(path: str = LIB_PATH):
         "sgl_psi2_interp": [vp, i32, sp, vp, vp],
         "sgl_trunc_nfw_kg": [vp, i32, ctypes.c_double, ctypes.c_double, i32, vp, vp, vp],
         "sgl_ref_tables_build": [vp, i32, vp, vp, vp],
-        "sgl_ref_cells_count": [vp, i32, vp, vp, vp, vp],
-        "sgl_ref_cells_emit": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp],
+        "sgl_ref_roots": [vp, i32, vp, vp, vp],
+        "sgl_ref_sub_build": [vp, i32, vp, vp, vp, vp],
+        "sgl_ref_cells_count": [vp, i32, vp, vp, vp, vp, vp],
+        "sgl_ref_cells_emit": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp],
         "sgl_ref_bin": [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp],
         "sgl_ref_lam_axis": [vp, vp, vp, i32, i32, vp, vp],
         "sgl_ref_phi": [vp, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp],

@@ -12,9 +12,10 @@ with the measured k-reach; the Hankel + Fourier inversion to P3(kappa,
 gamma); the constant-mu arc integral and the normalisation
 (sgl_analytic_amd/ref2d.py over the sgl_ref_* kernels).  Ingredients built
 on this chain: spherical truncated NFW halos, "ell" (pseudo-elliptical
-halos) and "fil" (filaments); edge "r200" (default) or "rvir".  The
-ingredients "sub" (subhalos) and "bias" (the xi_lin two-point term), hence
-the default config="full", and the positive "vp" closure raise
+halos), "fil" (filaments) and "sub" (subhalos on their hosts, the exact
+decorated-host sector; with edge "r200" only); edge "r200" (default) or
+"rvir".  The ingredient "bias" (the xi_lin two-point term), hence the
+default config="full", and the positive "vp" closure raise
 NotImplementedError: they are not built on this chain.
 
 ``run_kernel_shape`` is the host driver of the kernel-shape chain (64
@@ -36,7 +37,12 @@ P = dict(h=0.674, Om=0.315, s8=0.811, Ob=0.0493, ns=0.965, zeq=3402.0)  # pipeli
 EXTENSIONS = dict(Ok=0.0, w0=-1.0, wa=0.0, growth_mode="ode", radiation="engine")
 INGREDIENTS = ("ell", "fil", "sub", "bias")
 MODELLED = {"bias", "fil", "ell"}   # ingredients of the kernel-shape chain (run_kernel_shape)
-MODELLED_2D = {"fil", "ell"}        # ingredients of the reference chain (run)
+MODELLED_2D = {"fil", "ell", "sub"}  # ingredients of the reference chain (run)
+# the subhalo model the chain builds: the reference's defaults for edge "r200"
+# (pipeline.py:171-175, subhalos.py:541-545), as its run record states them
+SUB_KW = dict(Mhost_min=1e9, npsi=28, nd=90, nth=48, split=10.0, ntheta=4, nr=40, prune=0.0,
+              pair_angle="uniform", sector="exact", exact_kw=dict(method="moments"),
+              clump_edge="r200", clump_mass="vir", clump_shape="circular", nphi_c=12)
 DEFAULT_SLICES = 64
 
 
@@ -166,9 +172,12 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200
             "'+fil' or {'ell', 'fil'})")
     if closure != "gauss":
         raise NotImplementedError("the positive 'vp' closure is not built on the reference chain")
+    if "sub" in s and edge != "r200":
+        raise NotImplementedError("subhalos are built with edge='r200' only (clumps truncated at "
+                                  "their r200, the reference's pairing for that edge)")
     from . import ref2d
     t0 = time.perf_counter()
-    pop = ref2d.build(cp, float(z_s), ell="ell" in s, fil="fil" in s, edge=edge)
+    pop = ref2d.build(cp, float(z_s), ell="ell" in s, fil="fil" in s, edge=edge, sub="sub" in s)
     t1 = time.perf_counter()
     x = np.asarray(xi_out, dtype=np.float64)
     res = ref2d.invert(pop, x)
@@ -197,8 +206,10 @@ def record_2d(cp, config, ingredients, closure, edge, pop, res, t_stages):
         kap_floor=1e-8, kbar=float(pop["kbar"]), h=cp["h"], Om=cp["Om"], s8=cp["s8"],
         sigma8=cp["s8"], Ob=cp["Ob"], ns=cp["ns"], zeq=cp["zeq"], Ok=cp["Ok"], w0=cp["w0"],
         wa=cp["wa"], growth_mode=cp["growth_mode"], radiation=cp["radiation"], Omega_r=Or,
-        dark_energy=de, nord=1, grid_scale=1.0, split=10.0, nproc=1, sub_kw=None,
-        pair_angle=None, pair_scale=None, k3=False, exactS=False, s_taper="cos2",
+        dark_energy=de, nord=1, grid_scale=1.0, split=10.0, nproc=1,
+        sub_kw=dict(SUB_KW) if "sub" in ingredients else None,
+        pair_angle="uniform" if "sub" in ingredients else None, pair_scale=None, k3=False,
+        exactS=False, s_taper="cos2",
         sigma_kappa=float(res["sigma"]), kthr=None, N_expl=float(pop["N_expl"]),
         N_fil=float(pop["N_fil"]), V_weak=0.0, mean=float(res["mean"]), var=float(res["var"]),
         skew=float(res["skew"]), sigmaDL=float(res["sigmaDL"]),

@@ -28,7 +28,7 @@
 
 #include <rocprim/rocprim.hpp>
 
-#include "sgl_amd_internal.h"
+#include "ref_sub.h"
 
 namespace sgl {
 
@@ -281,7 +281,12 @@ __device__ inline int bin_right(const double* __restrict__ e, int n, double v) {
 // cell count (the last orientation also carries the monopole tail and the
 // filament cells, the reference's emission order); emit mode writes every
 // cell's (kappa, gamma, weight) and fine-lattice key at the task's offset.
-template <bool EMIT>
+// Modes of the cell kernel: count the cells of every (row, orientation) task,
+// emit them, or (subhalos) find per host row the largest radius of its
+// circular grid and monopole tail within reach of a clump.
+constexpr int kModeCount = 0, kModeEmit = 1, kModeXref = 2;
+
+template <int MODE>
 __global__ __launch_bounds__(256) void k_ref_cells(const double* __restrict__ cosmo,
                                                    const double* __restrict__ rows,
                                                    const double* __restrict__ frows,
@@ -292,30 +297,36 @@ __global__ __launch_bounds__(256) void k_ref_cells(const double* __restrict__ co
                                                    const long long* __restrict__ offsets,
                                                    double* __restrict__ K, double* __restrict__ G,
                                                    double* __restrict__ W, int* __restrict__ key,
-                                                   int* __restrict__ status) {
+                                                   int* __restrict__ status,
+                                                   double* __restrict__ subh,
+                                                   const double* __restrict__ subc) {
   __shared__ double v[kMaxEdges];
   __shared__ double e[kMaxEdges];
   __shared__ int tmp[256];
-  const int nph = ell ? kEllPhi : 1;
+  __shared__ int ndec_s;
+  const int nph = (ell && MODE != kModeXref) ? kEllPhi : 1;
   const int task = blockIdx.x, t = threadIdx.x;
   const int row = task / nph, q = task - row * nph;
   const int i = row / kRefMasses;
   const Cosmo c = reinterpret_cast<const Cosmo*>(cosmo)[0];
   if (i >= (int)c.n_z) {
-    if (!EMIT && t == 0) counts[task] = 0;
+    if (MODE == kModeCount && t == 0) counts[task] = 0;
     return;
   }
+  double* Hs = subh ? subh + (size_t)row * kHostStride : nullptr;
+  const bool dec = Hs && Hs[kHrValid] != 0.0 && (MODE == kModeXref || Hs[kHrNq] > 0.0);
+  if (MODE == kModeXref && !dec) return;
   const RefRow p = ref_row(rows + (size_t)row * kRowStride, ell);
   const double* rt = roots + (size_t)row * kRootStride;
   const int nc = (int)rt[0];
-  const int ne = ell ? (int)rt[8 + 9 * q] : 0;
+  const int ne = (ell && MODE != kModeXref) ? (int)rt[8 + 9 * q] : 0;
   // candidate slots: base n_in + 1, truncation 3, offsets 44 per root
   const int nb = p.n_in + 1, n_off = 2 * kEdgeOff;
   const int ncand = nb + 3 + n_off * (nc + ne);
   if (ncand > kMaxEdges) {
     if (t == 0) {
       atomicOr(status, kStatusRefEdges);
-      if (!EMIT) counts[task] = 0;
+      if (MODE == kModeCount) counts[task] = 0;
     }
     return;
   }
@@ -373,45 +384,111 @@ __global__ __launch_bounds__(256) void k_ref_cells(const double* __restrict__ co
   const bool tail = last && (p.xm > p.x_ell * (1.0 + 1e-12));
   const int n_out = tail ? max(8, (int)ceil(12.0 * log10(p.xm / p.x_ell))) : 0;
   const int n_fil = (last && fil) ? kRefFil : 0;
-  if (!EMIT) {
-    if (t == 0) counts[task] = (long long)n_in_cells + n_out + n_fil;
+  auto centre = [&](int ci) {
+    const double xlo = e[ci], xhi = e[ci + 1];
+    return (xlo > 0.0) ? sqrt(xlo * xhi) : 0.5 * xhi;
+  };
+  auto tail_centre = [&](int k) {
+    return sqrt(geom(p.x_ell, p.xm, n_out + 1, k) * geom(p.x_ell, p.xm, n_out + 1, k + 1));
+  };
+  if (MODE == kModeXref) {
+    // xref of host_cells: the circular centres and the tail centres within
+    // x_sub; the coarse grid of the host's moment tables spans up to the
+    // largest of them (subhalos_exact.py:1658, sgl_full.py:2096-2098)
+    if (t == 0) {
+      const double xs = Hs[kHrXsub];
+      double xr = 0.0;
+      for (int k = n_out - 1; k >= 0 && xr == 0.0; --k)
+        if (tail_centre(k) <= xs) xr = tail_centre(k);
+      for (int ci = n_in_cells - 1; ci >= 0 && xr == 0.0; --ci)
+        if (centre(ci) <= xs) xr = centre(ci);
+      Hs[kHrXref] = xr;
+    }
+    return;
+  }
+  // decorated cells: the inner cells and tail cells with centre <= x_sub
+  // become nq points each (their centres increase, so they lead each part)
+  int nq = 1, ndi = 0, ndo = 0;
+  if (dec) {
+    nq = (int)Hs[kHrNq];
+    const double xs = Hs[kHrXsub];
+    if (t == 0) ndec_s = 0;
+    __syncthreads();
+    int cnt = 0;
+    for (int ci = t; ci < n_in_cells; ci += 256) cnt += (centre(ci) <= xs) ? 1 : 0;
+    if (cnt) atomicAdd(&ndec_s, cnt);
+    __syncthreads();
+    ndi = ndec_s;
+    for (int k = 0; k < n_out; ++k) ndo += (tail_centre(k) <= xs) ? 1 : 0;
+  }
+  const long long n_seg0 = (long long)ndi * nq;                 // inner decorated
+  const long long n_seg1 = n_in_cells - ndi;                   // inner plain
+  const long long n_seg2 = (long long)ndo * nq;                // tail decorated
+  const long long n_seg3 = n_out - ndo;                        // tail plain
+  const long long n_all = n_seg0 + n_seg1 + n_seg2 + n_seg3 + n_fil;
+  if (MODE == kModeCount) {
+    if (t == 0) counts[task] = n_all;
     return;
   }
   const long long base = offsets[task];
   const double iw = 1.0 / (p.xm * p.xm);
   double cph = 1.0, sph = 0.0;
   if (ell) ell_phi(q, cph, sph);
-  const int n_all = n_in_cells + n_out + n_fil;
-  for (int cidx = t; cidx < n_all; cidx += 256) {
+  const double* Cm = dec ? subc + (size_t)row * kCoarseStride : nullptr;
+  for (long long cidx = t; cidx < n_all; cidx += 256) {
     double kap, gam, w;
-    if (cidx < n_in_cells) {
-      const double xlo = e[cidx], xhi = e[cidx + 1];
+    long long r = cidx;
+    if (r < n_seg0 + n_seg1) {
+      int ci, qq = -1;
+      if (r < n_seg0) {
+        ci = (int)(r / nq);
+        qq = (int)(r - (long long)ci * nq);
+      } else {
+        ci = ndi + (int)(r - n_seg0);
+      }
+      const double xlo = e[ci], xhi = e[ci + 1];
       const double xc = (xlo > 0.0) ? sqrt(xlo * xhi) : 0.5 * xhi;
       w = p.Nbar * (xhi * xhi - xlo * xlo) * iw;
-      if (ell) {
-        w = w / kEllPhi;
+      if (ell) w = w / kEllPhi;
+      if (qq >= 0) {
+        double wq;
+        sub_point(Hs, Cm, xc, cph, sph, !ell, qq, kap, gam, wq);
+        w *= wq;
+      } else if (ell) {
         kg_eps_ref(xc, cph, sph, p.eps, p.ks, p.xt, p.mut4, kap, gam);
       } else {
         trunc_kg(xc, p.ks, p.xt, kap, gam);
       }
-    } else if (cidx < n_in_cells + n_out) {
-      const int k = cidx - n_in_cells;
+    } else if ((r -= n_seg0 + n_seg1) < n_seg2 + n_seg3) {
+      int k, qq = -1;
+      if (r < n_seg2) {
+        k = (int)(r / nq);
+        qq = (int)(r - (long long)k * nq);
+      } else {
+        k = ndo + (int)(r - n_seg2);
+      }
       const double e0 = geom(p.x_ell, p.xm, n_out + 1, k), e1 = geom(p.x_ell, p.xm, n_out + 1, k + 1);
       const double xo = sqrt(e0 * e1);
-      kap = 0.0;
-      gam = p.mut4 / (xo * xo);
       w = p.Nbar * (e1 * e1 - e0 * e0) * iw;
+      if (qq >= 0) {
+        double wq;
+        sub_point(Hs, Cm, xo, 1.0, 0.0, true, qq, kap, gam, wq);
+        w *= wq;
+      } else {
+        kap = 0.0;
+        gam = p.mut4 / (xo * xo);
+      }
     } else {
       // filament cell (u bin l, orientation n), C order of the (120, 13) arrays
-      const int f = cidx - n_in_cells - n_out, l = f / kFilPhi, n = f - l * kFilPhi;
+      const int f = (int)(r - n_seg2 - n_seg3), l = f / kFilPhi, n = f - l * kFilPhi;
       const double* FR = frows + (size_t)row * kFilRowStride;
       double uc, du2, kf, wphi;
       fil_ubin(l, uc, du2);
       fil_orientation(n, kf, wphi);
       const double k0 = FR[0] * kf / FR[1];
-      const double s = sqrt(1.0 + uc * uc);
-      kap = (uc <= 1.0) ? k0 * 2.0 * kPi / s : 0.0;
-      gam = k0 * 4.0 * kPi / (s + 1.0) - kap;
+      const double sq = sqrt(1.0 + uc * uc);
+      kap = (uc <= 1.0) ? k0 * 2.0 * kPi / sq : 0.0;
+      gam = k0 * 4.0 * kPi / (sq + 1.0) - kap;
       w = kPi * FR[2] * FR[1] * FR[1] * du2 * wphi;
     }
     const double gc = fmax(gam, 1.0000001 * 1e-9);   // np.clip(G, 1.0000001 * 1e-9, None)
@@ -1060,27 +1137,42 @@ struct RefTablesView {
   const double *cosmo, *rows, *frows, *ek, *eg;
 };
 
-int ref_cells(const RefTablesView& T, int n_z_max, int ell, int fil, double* roots, long long* counts,
-              int* status, hipStream_t st) {
+int ref_roots(const RefTablesView& T, int n_z_max, int ell, double* roots, int* status,
+              hipStream_t st) {
   hipLaunchKernelGGL(k_ref_roots, dim3(kRefMasses, n_z_max), dim3(256), 0, st, T.cosmo, T.rows,
                      n_z_max, ell, roots, status);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+int ref_xref(const RefTablesView& T, int n_z_max, int ell, const double* roots, double* subh,
+             int* status, hipStream_t st) {
+  hipLaunchKernelGGL(k_ref_cells<kModeXref>, dim3(n_z_max * kRefMasses), dim3(256), 0, st, T.cosmo,
+                     T.rows, T.frows, roots, T.ek, T.eg, ell, 0, (long long*)nullptr,
+                     (const long long*)nullptr, (double*)nullptr, (double*)nullptr,
+                     (double*)nullptr, (int*)nullptr, status, subh, (const double*)nullptr);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+int ref_cells(const RefTablesView& T, int n_z_max, int ell, int fil, const double* roots,
+              long long* counts, int* status, double* subh, const double* subc, hipStream_t st) {
   const int nph = ell ? kEllPhi : 1;
   const int ntask = n_z_max * kRefMasses * nph;
-  hipLaunchKernelGGL(k_ref_cells<false>, dim3(ntask), dim3(256), 0, st, T.cosmo, T.rows, T.frows,
-                     roots, T.ek, T.eg, ell, fil, counts, (const long long*)nullptr,
-                     (double*)nullptr, (double*)nullptr, (double*)nullptr, (int*)nullptr, status);
+  hipLaunchKernelGGL(k_ref_cells<kModeCount>, dim3(ntask), dim3(256), 0, st, T.cosmo, T.rows,
+                     T.frows, roots, T.ek, T.eg, ell, fil, counts, (const long long*)nullptr,
+                     (double*)nullptr, (double*)nullptr, (double*)nullptr, (int*)nullptr, status,
+                     subh, subc);
   hipLaunchKernelGGL(k_excl_scan_i64, dim3(1), dim3(256), 0, st, counts, ntask);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
 int ref_emit(const RefTablesView& T, int n_z_max, int ell, int fil, const double* roots,
              const long long* offsets, double* K, double* G, double* W, int* key, int* status,
-             hipStream_t st) {
+             double* subh, const double* subc, hipStream_t st) {
   const int nph = ell ? kEllPhi : 1;
   const int ntask = n_z_max * kRefMasses * nph;
-  hipLaunchKernelGGL(k_ref_cells<true>, dim3(ntask), dim3(256), 0, st, T.cosmo, T.rows, T.frows,
-                     roots, T.ek, T.eg, ell, fil, (long long*)nullptr, offsets, K, G, W, key,
-                     status);
+  hipLaunchKernelGGL(k_ref_cells<kModeEmit>, dim3(ntask), dim3(256), 0, st, T.cosmo, T.rows,
+                     T.frows, roots, T.ek, T.eg, ell, fil, (long long*)nullptr, offsets, K, G, W,
+                     key, status, subh, subc);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
@@ -1181,7 +1273,8 @@ sgl_shape ref_shape(int flags) {
   s.n_b = 1;
   s.n_k = 2;
   s.n_out = 2;
-  s.flags = (flags & (SGL_FLAG_ELL | SGL_FLAG_FILAMENTS | SGL_FLAG_EDGE_RVIR)) | SGL_FLAG_REF_GRID;
+  s.flags = (flags & (SGL_FLAG_ELL | SGL_FLAG_FILAMENTS | SGL_FLAG_EDGE_RVIR | SGL_FLAG_SUBHALOS)) |
+            SGL_FLAG_REF_GRID;
   s.m_lo_h = 1.0; s.m_hi_h = 2.0; s.u_lo = 1.0; s.u_hi = 2.0;
   s.k_lo = 1.0; s.k_hi = 2.0; s.out_lo = 0.0; s.out_hi = 1.0;
   return s;
@@ -1249,8 +1342,11 @@ using namespace sgl;
 
 namespace {
 hipStream_t ref_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+// subhalos only with the r200 edge (the reference pairs clump_edge with the
+// halo edge; its r_vir clumps are not built here)
 bool ref_flags_ok(int32_t f) {
-  return (f & ~(SGL_FLAG_ELL | SGL_FLAG_FILAMENTS | SGL_FLAG_EDGE_RVIR)) == 0;
+  if (f & ~(SGL_FLAG_ELL | SGL_FLAG_FILAMENTS | SGL_FLAG_EDGE_RVIR | SGL_FLAG_SUBHALOS)) return false;
+  return !((f & SGL_FLAG_SUBHALOS) && (f & SGL_FLAG_EDGE_RVIR));
 }
 sgl_status ref_rc(int rc) { return (sgl_status)rc; }
 }  // namespace
@@ -1293,7 +1389,8 @@ sgl_status sgl_ref_tables_build(const double* cosmo, int32_t flags, void* tables
   int rc = launch_fill_grids(sh, T + L.T.uedge, T + L.T.knodes, T + L.T.m_nodes, st);
   if (rc != SGL_OK) return ref_rc(rc);
   rc = ref_tables_build(T + L.T.cosmo, 1, sh, T + L.T.m_nodes, T + L.T.derived, T + L.T.slices,
-                        T + L.T.mass, T + L.T.rows, T + L.T.frows, T + L.T.sigtab, st);
+                        T + L.T.mass, T + L.T.rows, T + L.T.frows, T + L.T.sigtab,
+                        T + L.T.sigspl, T + L.T.gdense, st);
   if (rc != SGL_OK) return ref_rc(rc);
   hipLaunchKernelGGL(k_ref_fine_edges, dim3(4), dim3(256), 0, st, T + L.ek, T + L.eg);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
@@ -1311,23 +1408,63 @@ static RefTablesView ref_view(const void* tables, int flags) {
   return v;
 }
 
-sgl_status sgl_ref_cells_count(const void* tables, int32_t flags, double* roots, int64_t* counts,
-                               int32_t* status, void* stream) {
-  if (!ref_flags_ok(flags) || !tables || !roots || !counts || !status) return SGL_ERR_INVALID;
+size_t sgl_ref_sub_bytes(void) { return sub_layout().total * sizeof(double); }
+
+sgl_status sgl_ref_sub_build(const void* tables, int32_t flags, const double* roots, void* sub,
+                             int32_t* status, void* stream) {
+  if (!ref_flags_ok(flags) || !(flags & SGL_FLAG_SUBHALOS) || !tables || !roots || !sub || !status)
+    return SGL_ERR_INVALID;
+  const RefLayout L = ref_layout(flags);
+  const double* T = static_cast<const double*>(tables);
+  const SubLayout S = sub_layout();
+  double* D = static_cast<double*>(sub);
+  hipStream_t st = ref_stream(stream);
+  int rc = sub_hosts_launch(T + L.T.cosmo, T + L.T.derived, T + L.T.slices, T + L.T.mass,
+                            T + L.T.rows, T + L.T.sigspl, T + L.T.gdense, D + S.host, st);
+  if (rc != SGL_OK) return ref_rc(rc);
+  rc = ref_xref(ref_view(tables, flags), kRefMaxShells, (flags & SGL_FLAG_ELL) ? 1 : 0, roots,
+                D + S.host, status, st);
+  if (rc != SGL_OK) return ref_rc(rc);
+  return ref_rc(sub_coarse_launch(T + L.T.cosmo, T + L.T.derived, T + L.T.slices, T + L.T.sigspl,
+                                  (flags & SGL_FLAG_ELL) ? T + L.T.sigtab : nullptr, D + S.host,
+                                  D + S.coarse, D + S.ring, st));
+}
+
+sgl_status sgl_ref_roots(const void* tables, int32_t flags, double* roots, int32_t* status,
+                         void* stream) {
+  if (!ref_flags_ok(flags) || !tables || !roots || !status) return SGL_ERR_INVALID;
+  return ref_rc(ref_roots(ref_view(tables, flags), kRefMaxShells, (flags & SGL_FLAG_ELL) ? 1 : 0,
+                          roots, status, ref_stream(stream)));
+}
+
+static bool sub_ok(int32_t flags, const void* sub) {
+  return ((flags & SGL_FLAG_SUBHALOS) != 0) == (sub != nullptr);
+}
+
+sgl_status sgl_ref_cells_count(const void* tables, int32_t flags, const double* roots,
+                               const void* sub, int64_t* counts, int32_t* status, void* stream) {
+  if (!ref_flags_ok(flags) || !tables || !roots || !counts || !status || !sub_ok(flags, sub))
+    return SGL_ERR_INVALID;
+  const SubLayout S = sub_layout();
+  double* D = const_cast<double*>(static_cast<const double*>(sub));
   return ref_rc(ref_cells(ref_view(tables, flags), kRefMaxShells, (flags & SGL_FLAG_ELL) ? 1 : 0,
                           (flags & SGL_FLAG_FILAMENTS) ? 1 : 0, roots,
-                          reinterpret_cast<long long*>(counts), status, ref_stream(stream)));
+                          reinterpret_cast<long long*>(counts), status, D ? D + S.host : nullptr,
+                          D ? D + S.coarse : nullptr, ref_stream(stream)));
 }
 
 sgl_status sgl_ref_cells_emit(const void* tables, int32_t flags, const double* roots,
-                              const int64_t* offsets, double* kap, double* gam, double* w,
-                              int32_t* key, int32_t* status, void* stream) {
+                              const void* sub, const int64_t* offsets, double* kap, double* gam,
+                              double* w, int32_t* key, int32_t* status, void* stream) {
   if (!ref_flags_ok(flags) || !tables || !roots || !offsets || !kap || !gam || !w || !key ||
-      !status)
+      !status || !sub_ok(flags, sub))
     return SGL_ERR_INVALID;
+  const SubLayout S = sub_layout();
+  double* D = const_cast<double*>(static_cast<const double*>(sub));
   return ref_rc(ref_emit(ref_view(tables, flags), kRefMaxShells, (flags & SGL_FLAG_ELL) ? 1 : 0,
                          (flags & SGL_FLAG_FILAMENTS) ? 1 : 0, roots,
                          reinterpret_cast<const long long*>(offsets), kap, gam, w, key, status,
+                         D ? D + S.host : nullptr, D ? D + S.coarse : nullptr,
                          ref_stream(stream)));
 }
 

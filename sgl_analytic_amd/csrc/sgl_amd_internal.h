@@ -42,13 +42,41 @@ constexpr int kRowKs = 0, kRowRs = 1, kRowXt = 2, kRowRvir = 3, kRowW = 4, kRowB
 
 // Offsets (in doubles) of the sub-arrays of the tables buffer.
 struct TablesLayout {
-  size_t cosmo, derived, slices, mass, rows, frows, cov, sigtab, m_nodes, uedge, knodes, status,
-      total;
+  size_t cosmo, derived, slices, mass, rows, frows, cov, sigtab, sigspl, gdense, m_nodes, uedge,
+      knodes, status, total;
 };
 
 // Halo ellipticity (SGL_FLAG_ELL): the sigma(M) table of the M*(z) root,
 // M = logspace(0, 17, kMstarN) Msun (sgl_full.py:551-567).
 constexpr int kMstarN = 1200;
+
+// Subhalos (SGL_FLAG_SUBHALOS): the reference's sigma(M, 0) spline, ln sigma on
+// M = logspace(4, 17, kSigSplN) Msun with scipy's not-a-knot CubicSpline in
+// ln M (sgl.py:536-538): [kSigSplN][kMassStride] window-sum records, then
+// the knots x = ln M, values y = ln sigma and slopes s (kSigSplN each); and
+// the growth ODE's state on every RK4 step node x_n = ln a of the growth
+// kernel, (D, dD/dx) / D(a = 1) [kGrowthSteps + 1][2], for D at any z.
+// ln M*(z) (sgl_full.py:551-567): sigma(M*, 0) Dg(z) = delta_c0 by linear
+// interpolation of ln M against sigma, clamped to the table's ends
+// (np.interp).  The reference first cuts its table to the largest strictly
+// decreasing suffix, against its spline's extrapolation; the sums here
+// decrease strictly over the whole table, so the cut keeps all of it.
+__device__ inline double ln_mstar(const double* __restrict__ tab, double target) {
+  constexpr double kLnStep = 17.0 * 2.302585092994046 / (kMstarN - 1);
+  if (target >= tab[0]) return 0.0;
+  if (target <= tab[kMstarN - 1]) return kLnStep * (kMstarN - 1);
+  int lo = 0, hi = kMstarN - 1;           // tab[lo] > target >= tab[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tab[mid] > target) lo = mid; else hi = mid;
+  }
+  const double f = (tab[lo] - target) / (tab[lo] - tab[hi]);
+  return kLnStep * (lo + f);
+}
+
+constexpr int kSigSplN = 500;
+constexpr int kSigSplDoubles = kSigSplN * 4 + 3 * kSigSplN;
+constexpr int kGrowthSteps = 4096;
 
 // The shape-only grids come first, so their offsets do not depend on B: a
 // tables buffer built for any batch size serves every call that reads only
@@ -69,6 +97,9 @@ inline TablesLayout tables_layout(const sgl_shape& s, int B) {
   L.frows = take((s.flags & SGL_FLAG_FILAMENTS) ? (size_t)B * s.n_z_max * s.n_m * kFilRowStride : 0);
   L.cov = take((size_t)B * s.n_z_max * s.n_z_max);
   L.sigtab = take((s.flags & SGL_FLAG_ELL) ? (size_t)B * kMstarN : 0);
+  const bool sub = (s.flags & SGL_FLAG_SUBHALOS) != 0;
+  L.sigspl = take(sub ? (size_t)B * kSigSplDoubles : 0);
+  L.gdense = take(sub ? (size_t)B * (kGrowthSteps + 1) * 2 : 0);
   L.total = o;
   return L;
 }
@@ -190,7 +221,7 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
                  double* cov, double* sigtab, int* status, hipStream_t st);
 int ref_tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes,
                      double* derived, double* slices, double* mass, double* rows, double* frows,
-                     double* sigtab, hipStream_t st);
+                     double* sigtab, double* sigspl, double* gdense, hipStream_t st);
 int launch_fill_grids(const sgl_shape& sh, double* ue, double* kn, double* m_nodes_h,
                       hipStream_t st);
 int launch_lnphi_exact(const double* rows, const double* frows, const double* cosmo, const double* uedge,

@@ -61,6 +61,44 @@ __device__ __forceinline__ double f_k(double Ok, double H0, double chi) {
   return chi * ((Ok > 0.0) ? sinh(x) / x : sin(x) / x);
 }
 
+// r_vir / r200 = eta solving m^3 / mu(m) = 200 c^3 / (mu(c) Delta_vir) on
+// [c, 3c] (subhalos.py:409-441): safeguarded Newton on g(m) = m^3 - T mu(m),
+// T = 200 c^3 / (mu(c) Delta_vir) (one log1p and two divisions a step).
+// mu_c = mu(c) and cr = cbrt(200 / Delta_vir) come from the caller; Newton
+// starts at m = c cr (the root if mu(m) = mu(c), a few per cent off).
+__device__ inline double eta_rvir(double c, double Dvir, double mu_c, double cr) {
+  const double T = 200.0 * c * c * c / (mu_c * Dvir);
+  double lo = c, hi = 3.0 * c, m = fmin(fmax(c * cr, lo), hi);
+  for (int it = 0; it < 80; ++it) {
+    const double iop = 1.0 / (1.0 + m);
+    const double mu = log1p(m) - m * iop;
+    const double g = m * m * m - T * mu;
+    const double gp = 3.0 * m * m - T * m * iop * iop;
+    const double step = g / gp;
+    // Newton is quadratic here: a step below 1e-13 m leaves an error far below
+    // 1 ulp.  The test comes before the bracket update, so that a point where
+    // g rounds to 0 (the step lands on the new bracket end) ends the solve
+    // instead of falling back to bisection for the last bits.
+    if (fabs(step) <= 1e-13 * m) { m -= step; break; }
+    if (g < 0.0) lo = m; else hi = m;
+    double mn = m - step;
+    if (!(lo < mn && mn < hi)) mn = 0.5 * (lo + hi);
+    if (hi - lo <= 4e-16 * m) { m = mn; break; }
+    m = mn;
+  }
+  return m / c;
+}
+
+// Carroll-Press-Turner growth g(z) / (1 + z) with E^2 including radiation when
+// the background has it (sgl_full.py:521-538); Dg(z) = cpt(z) / cpt(0).
+__device__ inline double cpt_growth(double z, double Om, double Ok, double Or, double OL) {
+  const double zp1 = 1.0 + z;
+  const double m3 = Om * zp1 * zp1 * zp1;
+  const double E2 = m3 + OL + Ok * zp1 * zp1 + Or * zp1 * zp1 * zp1 * zp1;
+  const double Omz = m3 / E2, OLz = OL / E2;
+  return 2.5 * Omz / (pow(Omz, 4.0 / 7.0) - OLz + (1.0 + 0.5 * Omz) * (1.0 + OLz / 70.0)) / zp1;
+}
+
 // Eisenstein & Hu 1998 CDM+baryon transfer function T(k), k in 1/Mpc
 // (the formula set of sgl.py:391-445).  The k-independent constants are
 // computed once per instance into EH98Const.

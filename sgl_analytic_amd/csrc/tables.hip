@@ -218,7 +218,6 @@ __global__ __launch_bounds__(kBgThreads) void k_ref_background(const Cosmo* __re
 // the steps before its own, and each lane then steps its own chunk once more
 // from its starting state, writing the slice nodes that fall in it.  The
 // serial depth is 2 x 64 steps and 6 scan levels instead of 4096 steps.
-constexpr int kGrowthSteps = 4096;
 constexpr int kGrowthLane = kGrowthSteps / 64;
 static_assert(kGrowthSteps % 64 == 0, "whole chunks per lane");
 
@@ -280,19 +279,10 @@ __device__ inline double heath_integral(double a, double Om, double Ok, double O
   return 2.0 * a * a * sqrt(a) * s;
 }
 
-// Carroll-Press-Turner growth g(z) / (1 + z) with E^2 including radiation when
-// the background has it (sgl_full.py:521-538); Dg(z) = cpt(z) / cpt(0).
-__device__ inline double cpt_growth(double z, double Om, double Ok, double Or, double OL) {
-  const double zp1 = 1.0 + z;
-  const double m3 = Om * zp1 * zp1 * zp1;
-  const double E2 = m3 + OL + Ok * zp1 * zp1 + Or * zp1 * zp1 * zp1 * zp1;
-  const double Omz = m3 / E2, OLz = OL / E2;
-  return 2.5 * Omz / (pow(Omz, 4.0 / 7.0) - OLz + (1.0 + 0.5 * Omz) * (1.0 + OLz / 70.0)) / zp1;
-}
-
 __global__ __launch_bounds__(64) void k_growth(const Cosmo* __restrict__ cosmo, int n_z_max,
                                                Derived* __restrict__ der,
-                                               double* __restrict__ slices) {
+                                               double* __restrict__ slices,
+                                               double* __restrict__ gdense) {
   __shared__ double xs[kGrowthMaxNodes];
   const int b = blockIdx.x, L = threadIdx.x;
   const Cosmo c = cosmo[b];
@@ -386,6 +376,30 @@ __global__ __launch_bounds__(64) void k_growth(const Cosmo* __restrict__ cosmo, 
     }
   }
   if (L == 0) der[b].D0 = Dend;
+  if (gdense) {
+    // the state on every step node of this lane's chunk (and the end node),
+    // normalised by D(a = 1): the dense table of D(z) for any z
+    double* Gd = gdense + (size_t)b * (kGrowthSteps + 1) * 2;
+    double g0 = e00 * yi0 + e01 * yi1, g1 = e10 * yi0 + e11 * yi1;
+    double c1, d1;
+    f.coef(xat(n0), c1, d1);
+    const double iD = 1.0 / Dend;
+    for (int n = n0; n < n0 + kGrowthLane; ++n) {
+      Gd[2 * n] = g0 * iD;
+      Gd[2 * n + 1] = g1 * iD;
+      const double x = xat(n), xe = xat(n + 1), hh = xe - x;
+      double a2, b2, a4, b4;
+      f.coef(x + 0.5 * hh, a2, b2);
+      f.coef(xe, a4, b4);
+      rk4_lin(hh, c1, d1, a2, b2, a4, b4, g0, g1);
+      c1 = a4;
+      d1 = b4;
+    }
+    if (L == 63) {
+      Gd[2 * kGrowthSteps] = g0 * iD;
+      Gd[2 * kGrowthSteps + 1] = g1 * iD;
+    }
+  }
 }
 
 // ----------------------------------------------------------------- sigma --
@@ -581,53 +595,64 @@ __global__ __launch_bounds__(256) void k_mstar_table(const Cosmo* __restrict__ c
   }
 }
 
-// ln M*(z) (sgl_full.py:551-567): sigma(M*, 0) Dg(z) = delta_c0 by linear
-// interpolation of ln M against sigma, clamped to the table's ends
-// (np.interp).  The reference first cuts its table to the largest strictly
-// decreasing suffix, against its spline's extrapolation; the sums here
-// decrease strictly over the whole table, so the cut keeps all of it.
-__device__ inline double ln_mstar(const double* __restrict__ tab, double target) {
-  constexpr double kLnStep = 17.0 * 2.302585092994046 / (kMstarN - 1);
-  if (target >= tab[0]) return 0.0;
-  if (target <= tab[kMstarN - 1]) return kLnStep * (kMstarN - 1);
-  int lo = 0, hi = kMstarN - 1;           // tab[lo] > target >= tab[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (tab[mid] > target) lo = mid; else hi = mid;
+// The reference's sigma(M) spline (sgl.py:536-538, SGL_FLAG_SUBHALOS): knots
+// M = logspace(4, 17, 500) Msun (10^(4 + i 13/499), ends exact), filled here,
+// then their window sums by k_sigma (two launches of 250 masses), then
+// scipy's not-a-knot CubicSpline slopes in x = ln M (scipy/interpolate/
+// _cubic.py: the tridiagonal slope system with the not-a-knot end rows),
+// solved by one thread per instance with the Thomas algorithm.
+__global__ void k_sigspl_nodes(double* __restrict__ m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kSigSplN)
+    m[i] = (i == 0) ? 1e4 : (i == kSigSplN - 1) ? 1e17
+                                                : pow(10.0, i * (13.0 / (kSigSplN - 1)) + 4.0);
+}
+
+__global__ void k_sigspl_solve(double* __restrict__ spl, int B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  double* S = spl + (size_t)b * kSigSplDoubles;
+  double* x = S + kSigSplN * kMassStride;
+  double* y = x + kSigSplN;
+  double* s = y + kSigSplN;
+  constexpr int n = kSigSplN;
+  for (int i = 0; i < n; ++i) {
+    x[i] = log(S[i * kMassStride]);
+    y[i] = S[i * kMassStride + 3];
   }
-  const double f = (tab[lo] - target) / (tab[lo] - tab[hi]);
-  return kLnStep * (lo + f);
+  // rows: lower l[i] s[i-1] + diag d[i] s[i] + upper u[i] s[i+1] = r[i]; the
+  // forward sweep keeps the modified upper diagonal in s[] and the right side
+  // in the caller-visible slope array's place until the back substitution
+  auto h = [&](int i) { return x[i + 1] - x[i]; };
+  auto dl = [&](int i) { return (y[i + 1] - y[i]) / h(i); };
+  double cp[n];
+  double rp;
+  {
+    const double h0 = h(0), h1 = h(1), d = x[2] - x[0];
+    const double diag = h1, up = d;
+    const double r = ((h0 + 2.0 * d) * h1 * dl(0) + h0 * h0 * dl(1)) / d;
+    cp[0] = up / diag;
+    s[0] = r / diag;
+  }
+  for (int i = 1; i < n - 1; ++i) {
+    const double lo = h(i), di = 2.0 * (h(i - 1) + h(i)), up = h(i - 1);
+    const double r = 3.0 * (h(i) * dl(i - 1) + h(i - 1) * dl(i));
+    const double den = di - lo * cp[i - 1];
+    cp[i] = up / den;
+    s[i] = (r - lo * s[i - 1]) / den;
+  }
+  {
+    const double hm = h(n - 2), hm1 = h(n - 3), d = x[n - 1] - x[n - 3];
+    const double lo = d, di = hm1;
+    const double r = (hm * hm * dl(n - 3) + (2.0 * d + hm) * hm1 * dl(n - 2)) / d;
+    const double den = di - lo * cp[n - 2];
+    rp = (r - lo * s[n - 2]) / den;
+  }
+  s[n - 1] = rp;
+  for (int i = n - 2; i >= 0; --i) s[i] -= cp[i] * s[i + 1];
 }
 
 // ------------------------------------------------------------------ rows --
-// r_vir / r200 = eta solving m^3 / mu(m) = 200 c^3 / (mu(c) Delta_vir) on
-// [c, 3c] (subhalos.py:409-441): safeguarded Newton on g(m) = m^3 - T mu(m),
-// T = 200 c^3 / (mu(c) Delta_vir) (one log1p and two divisions a step).
-// mu_c = mu(c) and cr = cbrt(200 / Delta_vir) come from the caller; Newton
-// starts at m = c cr (the root if mu(m) = mu(c), a few per cent off).
-__device__ inline double eta_rvir(double c, double Dvir, double mu_c, double cr) {
-  const double T = 200.0 * c * c * c / (mu_c * Dvir);
-  double lo = c, hi = 3.0 * c, m = fmin(fmax(c * cr, lo), hi);
-  for (int it = 0; it < 80; ++it) {
-    const double iop = 1.0 / (1.0 + m);
-    const double mu = log1p(m) - m * iop;
-    const double g = m * m * m - T * mu;
-    const double gp = 3.0 * m * m - T * m * iop * iop;
-    const double step = g / gp;
-    // Newton is quadratic here: a step below 1e-13 m leaves an error far below
-    // 1 ulp.  The test comes before the bracket update, so that a point where
-    // g rounds to 0 (the step lands on the new bracket end) ends the solve
-    // instead of falling back to bisection for the last bits.
-    if (fabs(step) <= 1e-13 * m) { m -= step; break; }
-    if (g < 0.0) lo = m; else hi = m;
-    double mn = m - step;
-    if (!(lo < mn && mn < hi)) mn = 0.5 * (lo + hi);
-    if (hi - lo <= 4e-16 * m) { m = mn; break; }
-    m = mn;
-  }
-  return m / c;
-}
-
 // One workgroup per (instance, slice), threads over masses: the slice's
 // redshift-only factors (Ludlow+16 parameters, rho_c, Delta_vir) are formed
 // once in LDS.  a_st and dc0 are the Sheth-Tormen normalisation and the
@@ -1005,7 +1030,8 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
     ck(hipStreamWaitEvent(S->s, S->e_bg, 0));
     sg = S->s;
   }
-  hipLaunchKernelGGL(k_growth, dim3(B), dim3(64), 0, sg, C, sh.n_z_max, D, slices);
+  hipLaunchKernelGGL(k_growth, dim3(B), dim3(64), 0, sg, C, sh.n_z_max, D, slices,
+                     (double*)nullptr);
   if (S) ck(hipEventRecord(S->j_growth, S->s));
   // sigma(M) needs the background only; it writes A_norm and the mass table
   hipLaunchKernelGGL(k_sigma, dim3(B), dim3(256), 0, st, C, m_nodes_h, sh.n_m, 0, D, mass);
@@ -1042,13 +1068,27 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
 // kernels of the kernel-shape tables, with dlnM the engine grid's step.
 int ref_tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes,
                      double* derived, double* slices, double* mass, double* rows, double* frows,
-                     double* sigtab, hipStream_t st) {
+                     double* sigtab, double* sigspl, double* gdense, hipStream_t st) {
   const Cosmo* C = reinterpret_cast<const Cosmo*>(cosmo);
   Derived* D = reinterpret_cast<Derived*>(derived);
   const double dlnM = (log(1e17) - log(1e7)) / 99.0;
+  const bool sub = (sh.flags & SGL_FLAG_SUBHALOS) != 0;
   hipLaunchKernelGGL(k_ref_background, dim3(B), dim3(kBgThreads), 0, st, C, sh.n_z_max, D, slices);
-  hipLaunchKernelGGL(k_growth, dim3(B), dim3(64), 0, st, C, sh.n_z_max, D, slices);
+  hipLaunchKernelGGL(k_growth, dim3(B), dim3(64), 0, st, C, sh.n_z_max, D, slices,
+                     sub ? gdense : (double*)nullptr);
   hipLaunchKernelGGL(k_sigma, dim3(B), dim3(256), 0, st, C, m_nodes, sh.n_m, 1, D, mass);
+  if (sub) {
+    // spline knots into the x[] slot of instance 0 (overwritten by the solve)
+    double* knots = sigspl + kSigSplN * kMassStride;
+    hipLaunchKernelGGL(k_sigspl_nodes, dim3((kSigSplN + 255) / 256), dim3(256), 0, st, knots);
+    for (int b = 0; b < B; ++b) {
+      double* Sb = sigspl + (size_t)b * kSigSplDoubles;
+      for (int half = 0; half < 2; ++half)
+        hipLaunchKernelGGL(k_sigma, dim3(1), dim3(256), 0, st, C + b, knots + half * (kSigSplN / 2),
+                           kSigSplN / 2, 1, D + b, Sb + half * (kSigSplN / 2) * kMassStride);
+    }
+    hipLaunchKernelGGL(k_sigspl_solve, dim3((B + 63) / 64), dim3(64), 0, st, sigspl, B);
+  }
   const bool ell = (sh.flags & SGL_FLAG_ELL) != 0;
   if (ell) hipLaunchKernelGGL(k_mstar_table, dim3(B), dim3(256), 0, st, C, D, sigtab);
   const double a_st = 1.0 / (1.0 + std::pow(2.0, -kPst) * std::tgamma(0.5 - kPst) / std::sqrt(kPi));

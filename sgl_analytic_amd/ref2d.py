@@ -35,7 +35,7 @@ PHI_TOL = 1e-8
 N_XI_WIDE = 500
 OUT_GRID_MAX = 3.4e7             # sgl_full.py:60 (geometric output axis above it)
 E_BLOCK_BYTES = 4 << 30          # Lambda operand rows per call (k_kappa row blocks)
-INGREDIENTS_2D = ("ell", "fil")  # ingredient sets this chain computes
+INGREDIENTS_2D = ("ell", "fil", "sub")  # ingredient sets this chain computes
 
 
 def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
@@ -94,16 +94,23 @@ class _Stages:
 
 
 def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
-          edge: str = "r200", timings: dict = None) -> Population:
+          edge: str = "r200", timings: dict = None, sub: bool = False,
+          keep_sub: bool = False) -> Population:
     """build_population(..., kap_floor=1e-8, trunc=edge, zint="midpoint")
     + Lam's flattening (sgl_full.py:1535, 2742-2755), on the device.  edge
-    "rvir" truncates every halo at its virial radius (sgl_full.py:1876-1880)."""
+    "rvir" truncates every halo at its virial radius (sgl_full.py:1876-1880).
+    sub: subhalos on their hosts (the exact decorated-host sector with
+    clumps truncated at r200, the pipeline's default for edge "r200";
+    pipeline.py:171-183); keep_sub keeps the subhalo tables on the result."""
     L = _lib.load()
     dev = torch.device("cuda")
     if edge not in ("r200", "rvir"):
         raise ValueError("edge must be 'r200' or 'rvir'")
+    if sub and edge != "r200":
+        raise NotImplementedError("subhalos are built with the r200 edge only (clump_edge r200)")
     flags = ((FLAG_ELL if ell else 0) | (FLAG_FILAMENTS if fil else 0)
-             | (_lib.FLAG_EDGE_RVIR if edge == "rvir" else 0))
+             | (_lib.FLAG_EDGE_RVIR if edge == "rvir" else 0)
+             | (_lib.FLAG_SUBHALOS if sub else 0))
     n_z = int(L.sgl_ref_shell_count(float(zs)))
     if n_z < 1 or n_z > _lib.REF_MAX_SHELLS:
         raise ValueError("z_s outside the reference grid's shells")
@@ -115,9 +122,17 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
     check(L.sgl_ref_tables_build(_ptr(rec), flags, _ptr(T), _ptr(status), st), "sgl_ref_tables_build")
     clock.mark("tables")
     roots = torch.empty(L.sgl_ref_roots_bytes() // 8, dtype=torch.float64, device=dev)
+    check(L.sgl_ref_roots(_ptr(T), flags, _ptr(roots), _ptr(status), st), "sgl_ref_roots")
+    subw = None
+    if sub:
+        subw = torch.empty(L.sgl_ref_sub_bytes() // 8, dtype=torch.float64, device=dev)
+        check(L.sgl_ref_sub_build(_ptr(T), flags, _ptr(roots), _ptr(subw), _ptr(status), st),
+              "sgl_ref_sub_build")
+        clock.mark("subhalos")
+    sp = _ptr(subw) if subw is not None else None
     ntask = int(L.sgl_ref_tasks(flags))
     counts = torch.empty(ntask + 1, dtype=torch.int64, device=dev)
-    check(L.sgl_ref_cells_count(_ptr(T), flags, _ptr(roots), _ptr(counts), _ptr(status), st),
+    check(L.sgl_ref_cells_count(_ptr(T), flags, _ptr(roots), sp, _ptr(counts), _ptr(status), st),
           "sgl_ref_cells_count")
     n = int(counts[ntask].item())
     clock.mark("cells_count")
@@ -127,8 +142,8 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
     G = torch.empty_like(K)
     W = torch.empty_like(K)
     key = torch.empty(n, dtype=torch.int32, device=dev)
-    check(L.sgl_ref_cells_emit(_ptr(T), flags, _ptr(roots), _ptr(counts), _ptr(K), _ptr(G), _ptr(W),
-                               _ptr(key), _ptr(status), st), "sgl_ref_cells_emit")
+    check(L.sgl_ref_cells_emit(_ptr(T), flags, _ptr(roots), sp, _ptr(counts), _ptr(K), _ptr(G),
+                               _ptr(W), _ptr(key), _ptr(status), st), "sgl_ref_cells_emit")
     clock.mark("cells_emit")
     ws = torch.empty(L.sgl_ref_bin_bytes(n), dtype=torch.uint8, device=dev)
     pop = torch.empty(3 * REF_FINE_BINS, dtype=torch.float64, device=dev)
@@ -145,8 +160,12 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
     sc = scal.cpu().numpy()
     # bookkeeping of the info record (sgl_full.py:2039, 1992): expected counts
     lay_rows = _row_table(T, flags, n_z)
-    return Population(pop=pop, npop_dev=npop, n_bins=int(npop.item()), sigma=math.sqrt(sc[0]),
-                      kbar=float(sc[1]), n_points=n, N_expl=lay_rows[0], N_fil=lay_rows[1] if fil else 0.0)
+    out = Population(pop=pop, npop_dev=npop, n_bins=int(npop.item()), sigma=math.sqrt(sc[0]),
+                     kbar=float(sc[1]), n_points=n, N_expl=lay_rows[0],
+                     N_fil=lay_rows[1] if fil else 0.0)
+    if keep_sub:
+        out.sub_tables = subw
+    return out
 
 
 def _row_table(T: torch.Tensor, flags: int, n_z: int):
@@ -313,10 +332,10 @@ def invert(pop: Population, xi_out: np.ndarray, tol: float = PHI_TOL,
 
 
 def run(zs: float, cosmo: dict, ell: bool = False, fil: bool = False, xi_out=None,
-        edge: str = "r200", timings: dict = None) -> Result:
+        edge: str = "r200", timings: dict = None, sub: bool = False) -> Result:
     """One PDF on the reference's chain; xi_out None: the CLI body and tail
     grids linspace(-1, 1, 1601) and linspace(1.005, 9, 1600) (pipeline.py:211-215)."""
-    pop = build(cosmo, zs, ell=ell, fil=fil, edge=edge, timings=timings)
+    pop = build(cosmo, zs, ell=ell, fil=fil, edge=edge, timings=timings, sub=sub)
     if xi_out is None:
         xi_out = np.concatenate([np.linspace(-1.0, 1.0, 1601), np.linspace(1.005, 9.0, 1600)])
     return invert(pop, xi_out, timings=timings)

@@ -51,4 +51,4 @@ def test_cli_rejects_bad_input(tmp_path):
             cli.main(["--zs", "1", "--nproc", "0", "--out", str(tmp_path / "x.json")])
     # ingredients not built on the reference chain: a usage error, not a silent other model
     with pytest.raises(SystemExit):
-        cli.main(["--zs", "1", "--config", "+sub", "--out", str(tmp_path / "x.json")])
+        cli.main(["--zs", "1", "--config", "+bias", "--out", str(tmp_path / "x.json")])
