@@ -258,18 +258,35 @@ __device__ inline int block_excl_scan(int v, int* tmp, int& total) {
   return ex;
 }
 
-// searchsorted(e, v, side="right") over n sorted edges, with a value on the
-// last edge moved into the last bin (np.histogramdd, sgl_full.py:888-934):
-// bin index 1..n-1 inside, 0 or n outside (NaN: outside)
-__device__ inline int bin_right(const double* __restrict__ e, int n, double v) {
+// searchsorted(e, v, side="right") over the n sorted edges of the fine
+// lattice, with a value on the last edge moved into the last bin
+// (np.histogramdd, sgl_full.py:888-934): bin index 1..n-1 inside, 0 or n
+// outside (NaN: outside).  The index is guessed from the edges' logarithmic
+// spacing and then walked to the first edge above v: one or two loads
+// instead of a ten-step bisection, the same result.
+__device__ inline int walk_right(const double* __restrict__ e, int n, double v, int g) {
+  g = min(max(g, 0), n);
+  while (g > 0 && e[g - 1] > v) --g;
+  while (g < n && e[g] <= v) ++g;
+  if (g == n && v == e[n - 1]) --g;
+  return g;
+}
+__device__ inline int bin_right_k(const double* __restrict__ ek, double v) {
+  constexpr int n = kFineKRef + 1, npos = n / 2;        // 481 edges on each side
   if (!(v == v)) return n;
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (e[mid] <= v) lo = mid + 1; else hi = mid;
+  const double av = fabs(v);
+  int g = npos;
+  if (av >= 1e-7) {
+    const int m = (int)(log(av * 1e7) * ((npos - 1) / log(30.0 * 1e7)));
+    g = (v > 0.0) ? npos + m + 1 : npos - 1 - m;
   }
-  if (v == e[n - 1]) --lo;
-  return lo;
+  return walk_right(ek, n, v, g);
+}
+__device__ inline int bin_right_g(const double* __restrict__ eg, double v) {
+  constexpr int n = kFineGRef + 1;
+  if (!(v == v)) return n;
+  const int g = (v >= 1e-9) ? (int)(log(v * 1e9) * ((n - 1) / log(30.0 * 1e9))) + 1 : 0;
+  return walk_right(eg, n, v, g);
 }
 
 // ------------------------------------------------ cells (count or emit) --
@@ -287,7 +304,7 @@ __device__ inline int bin_right(const double* __restrict__ e, int n, double v) {
 constexpr int kModeCount = 0, kModeEmit = 1, kModeXref = 2;
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_ref_cells(const double* __restrict__ cosmo,
+__global__ __launch_bounds__(256, 2) void k_ref_cells(const double* __restrict__ cosmo,
                                                    const double* __restrict__ rows,
                                                    const double* __restrict__ frows,
                                                    const double* __restrict__ roots,
@@ -427,60 +444,67 @@ __global__ __launch_bounds__(256) void k_ref_cells(const double* __restrict__ co
   const long long n_seg3 = n_out - ndo;                        // tail plain
   const long long n_all = n_seg0 + n_seg1 + n_seg2 + n_seg3 + n_fil;
   if (MODE == kModeCount) {
-    if (t == 0) counts[task] = n_all;
+    if (t == 0) {
+      counts[task] = n_all;
+      if (dec) {
+        // the decorated cells of this task, for the point expansion
+        int* ts = reinterpret_cast<int*>(subh + (size_t)kSubRows * kHostStride) + (size_t)task * 4;
+        ts[0] = ndi;
+        ts[1] = ndo;
+        ts[2] = n_in_cells;
+        ts[3] = nq;
+      }
+    }
     return;
   }
   const long long base = offsets[task];
   const double iw = 1.0 / (p.xm * p.xm);
   double cph = 1.0, sph = 0.0;
   if (ell) ell_phi(q, cph, sph);
-  const double* Cm = dec ? subc + (size_t)row * kCoarseStride : nullptr;
-  for (long long cidx = t; cidx < n_all; cidx += 256) {
+  // work items: decorated inner cells, plain inner cells, decorated tail
+  // cells, plain tail cells, filament cells.  A decorated cell leaves its
+  // centre and weight in the first of its nq output slots; k_sub_emit
+  // expands it into the points (decorated cells lead their parts, so the
+  // output position of item r follows from the segment sizes).
+  const long long m_all = ndi + n_seg1 + ndo + n_seg3 + n_fil;
+  for (long long it = t; it < m_all; it += 256) {
     double kap, gam, w;
-    long long r = cidx;
-    if (r < n_seg0 + n_seg1) {
-      int ci, qq = -1;
-      if (r < n_seg0) {
-        ci = (int)(r / nq);
-        qq = (int)(r - (long long)ci * nq);
-      } else {
-        ci = ndi + (int)(r - n_seg0);
-      }
+    long long r = it, o;
+    bool stash = false;
+    if (r < ndi + n_seg1) {
+      const int ci = (int)r;
       const double xlo = e[ci], xhi = e[ci + 1];
       const double xc = (xlo > 0.0) ? sqrt(xlo * xhi) : 0.5 * xhi;
       w = p.Nbar * (xhi * xhi - xlo * xlo) * iw;
       if (ell) w = w / kEllPhi;
-      if (qq >= 0) {
-        double wq;
-        sub_point(Hs, Cm, xc, cph, sph, !ell, qq, kap, gam, wq);
-        w *= wq;
-      } else if (ell) {
-        kg_eps_ref(xc, cph, sph, p.eps, p.ks, p.xt, p.mut4, kap, gam);
+      if (r < ndi) {
+        o = base + r * nq;
+        kap = xc;
+        stash = true;
       } else {
-        trunc_kg(xc, p.ks, p.xt, kap, gam);
+        o = base + n_seg0 + (r - ndi);
+        if (ell) kg_eps_ref(xc, cph, sph, p.eps, p.ks, p.xt, p.mut4, kap, gam);
+        else trunc_kg(xc, p.ks, p.xt, kap, gam);
       }
-    } else if ((r -= n_seg0 + n_seg1) < n_seg2 + n_seg3) {
-      int k, qq = -1;
-      if (r < n_seg2) {
-        k = (int)(r / nq);
-        qq = (int)(r - (long long)k * nq);
-      } else {
-        k = ndo + (int)(r - n_seg2);
-      }
+    } else if ((r -= ndi + n_seg1) < ndo + n_seg3) {
+      const int k = (int)r;
       const double e0 = geom(p.x_ell, p.xm, n_out + 1, k), e1 = geom(p.x_ell, p.xm, n_out + 1, k + 1);
       const double xo = sqrt(e0 * e1);
       w = p.Nbar * (e1 * e1 - e0 * e0) * iw;
-      if (qq >= 0) {
-        double wq;
-        sub_point(Hs, Cm, xo, 1.0, 0.0, true, qq, kap, gam, wq);
-        w *= wq;
+      const long long b2 = base + n_seg0 + n_seg1;
+      if (r < ndo) {
+        o = b2 + r * nq;
+        kap = xo;
+        stash = true;
       } else {
+        o = b2 + n_seg2 + (r - ndo);
         kap = 0.0;
         gam = p.mut4 / (xo * xo);
       }
     } else {
       // filament cell (u bin l, orientation n), C order of the (120, 13) arrays
-      const int f = (int)(r - n_seg2 - n_seg3), l = f / kFilPhi, n = f - l * kFilPhi;
+      const int f = (int)(r - ndo - n_seg3), l = f / kFilPhi, n = f - l * kFilPhi;
+      o = base + n_seg0 + n_seg1 + n_seg2 + n_seg3 + f;
       const double* FR = frows + (size_t)row * kFilRowStride;
       double uc, du2, kf, wphi;
       fil_ubin(l, uc, du2);
@@ -491,15 +515,76 @@ __global__ __launch_bounds__(256) void k_ref_cells(const double* __restrict__ co
       gam = k0 * 4.0 * kPi / (sq + 1.0) - kap;
       w = kPi * FR[2] * FR[1] * FR[1] * du2 * wphi;
     }
+    if (stash) {
+      K[o] = kap;
+      W[o] = w;
+      continue;
+    }
     const double gc = fmax(gam, 1.0000001 * 1e-9);   // np.clip(G, 1.0000001 * 1e-9, None)
-    const int a = bin_right(ek, kFineKRef + 1, kap);
-    const int b = bin_right(eg, kFineGRef + 1, (gam == gam) ? gc : gam);
+    const int a = bin_right_k(ek, kap);
+    const int b = bin_right_g(eg, (gam == gam) ? gc : gam);
     const bool in = (a >= 1 && a <= kFineKRef && b >= 1 && b <= kFineGRef);
-    const long long o = base + cidx;
     K[o] = kap;
     G[o] = (gam == gam) ? gc : gam;
     W[o] = w;
     key[o] = in ? (a - 1) * kFineGRef + (b - 1) : kKeyInvalid;
+  }
+}
+
+// The points of the decorated cells (after k_ref_cells' emit pass): one
+// workgroup per task; the stashed (x, w) of up to 256 cells go to LDS, then
+// the cells' nq points each are written over their slots.
+__global__ __launch_bounds__(256, 2) void k_sub_emit(const double* __restrict__ cosmo,
+                                                  const double* __restrict__ ek,
+                                                  const double* __restrict__ eg, int ell,
+                                                  const long long* __restrict__ offsets,
+                                                  const double* __restrict__ subh,
+                                                  const double* __restrict__ subc,
+                                                  double* __restrict__ K, double* __restrict__ G,
+                                                  double* __restrict__ W, int* __restrict__ key) {
+  __shared__ double sx[256], sw[256];
+  const int nph = ell ? kEllPhi : 1;
+  const int task = blockIdx.x, t = threadIdx.x;
+  const int row = task / nph, q = task - row * nph;
+  const int i = row / kRefMasses;
+  const Cosmo c = reinterpret_cast<const Cosmo*>(cosmo)[0];
+  if (i >= (int)c.n_z) return;
+  const double* Hs = subh + (size_t)row * kHostStride;
+  if (Hs[kHrValid] == 0.0 || !(Hs[kHrNq] > 0.0)) return;
+  const int* ts = reinterpret_cast<const int*>(subh + (size_t)kSubRows * kHostStride) + (size_t)task * 4;
+  const int ndi = ts[0], ndo = ts[1], n_in = ts[2], nq = ts[3];
+  const double* Cm = subc + (size_t)row * kCoarseStride;
+  double cph = 1.0, sph = 0.0;
+  if (ell) ell_phi(q, cph, sph);
+  const long long base = offsets[task];
+  for (int seg = 0; seg < 2; ++seg) {
+    const int ncell = seg ? ndo : ndi;
+    const long long b0 = seg ? base + (long long)ndi * nq + (n_in - ndi) : base;
+    const bool circ = seg ? true : !ell;
+    for (int c0 = 0; c0 < ncell; c0 += 256) {
+      const int nc = min(256, ncell - c0);
+      __syncthreads();
+      if (t < nc) {
+        const long long o = b0 + (long long)(c0 + t) * nq;
+        sx[t] = K[o];
+        sw[t] = W[o];
+      }
+      __syncthreads();
+      for (int pq = t; pq < nc * nq; pq += 256) {
+        const int cl = pq / nq, qq = pq - cl * nq;
+        double kap, gam, wq;
+        sub_point(Hs, Cm, sx[cl], circ ? 1.0 : cph, circ ? 0.0 : sph, circ, qq, kap, gam, wq);
+        const double gc = fmax(gam, 1.0000001 * 1e-9);
+        const int a = bin_right_k(ek, kap);
+        const int b = bin_right_g(eg, (gam == gam) ? gc : gam);
+        const bool in = (a >= 1 && a <= kFineKRef && b >= 1 && b <= kFineGRef);
+        const long long o = b0 + (long long)(c0 + cl) * nq + qq;
+        K[o] = kap;
+        G[o] = (gam == gam) ? gc : gam;
+        W[o] = sw[cl] * wq;
+        key[o] = in ? (a - 1) * kFineGRef + (b - 1) : kKeyInvalid;
+      }
+    }
   }
 }
 
@@ -536,9 +621,14 @@ __global__ void k_zero_tail_runs(int* __restrict__ rlen, const int* __restrict__
   if (i <= kRefBins && i >= *nruns) rlen[i] = 0;
 }
 
-__global__ void k_iota(int* __restrict__ a, long long n) {
+// a cell's (kappa, gamma, w), the payload the key sort carries
+struct KGW {
+  double k, g, w;
+};
+__global__ void k_pack_kgw(long long n, const double* __restrict__ K, const double* __restrict__ G,
+                           const double* __restrict__ W, KGW* __restrict__ out) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = (int)i;
+  if (i < n) out[i] = KGW{K[i], G[i], W[i]};
 }
 
 // Block sums of x[i] * y[i] (y may be null) over contiguous chunks, fixed order.
@@ -574,29 +664,47 @@ __global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ pa
   if (t == 0) *out = red[0];
 }
 
-// One wave per run of equal keys (the sorted cells of one fine bin): the
-// sums of w, w kappa and w gamma over the run in the cells' order (lane-strided
-// partial sums, then a fixed shuffle tree).
-__global__ __launch_bounds__(256) void k_run_sums(const int* __restrict__ ukeys,
-                                                  const int* __restrict__ rlen,
-                                                  const int* __restrict__ nruns,
-                                                  const long long* __restrict__ rstart,
-                                                  const int* __restrict__ sidx,
-                                                  const double* __restrict__ K,
-                                                  const double* __restrict__ G,
-                                                  const double* __restrict__ W,
-                                                  double* __restrict__ rsum) {
-  const int wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
-  if (wv >= *nruns) return;
-  const long long s0 = rstart[wv];
-  const int n = rlen[wv];
+// Runs of equal keys (the sorted cells of one fine bin, in emission order)
+// are summed in parts of kRunPart cells, one wave per part (lane-strided
+// partial sums, then a fixed shuffle tree), and the parts of a run are added
+// in order: a fixed summation order, whatever the run lengths.
+constexpr int kRunPart = 4096;
+
+__global__ void k_run_nparts(const int* __restrict__ rlen, const int* __restrict__ nruns,
+                             int* __restrict__ nparts) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > kRefBins) return;
+  nparts[r] = (r < *nruns) ? (rlen[r] + kRunPart - 1) / kRunPart : 0;
+}
+
+__global__ __launch_bounds__(256) void k_run_parts(const int* __restrict__ rlen,
+                                                   const int* __restrict__ nruns,
+                                                   const long long* __restrict__ rstart,
+                                                   const int* __restrict__ nparts,
+                                                   const long long* __restrict__ poff,
+                                                   const KGW* __restrict__ v,
+                                                   double* __restrict__ psum) {
+  const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int nr = *nruns;
+  if (nr < 1) return;
+  const long long total = poff[nr - 1] + nparts[nr - 1];
+  if (wv >= total) return;
+  int lo = 0, hi = nr - 1;                  // last run with poff <= wv
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (poff[mid] <= wv) lo = mid; else hi = mid - 1;
+  }
+  const int r = lo;
+  const long long p = wv - poff[r];
+  const long long a = rstart[r] + p * kRunPart;
+  const long long b = min(rstart[r] + (long long)rlen[r], a + kRunPart);
   double sw = 0.0, sk = 0.0, sgm = 0.0;
-  for (int q = lane; q < n; q += 64) {
-    const int c = sidx[s0 + q];
-    const double w = W[c];
-    sw += w;
-    sk += w * K[c];
-    sgm += w * G[c];
+  for (long long q = a + lane; q < b; q += 64) {
+    const KGW c = v[q];
+    sw += c.w;
+    sk += c.w * c.k;
+    sgm += c.w * c.g;
   }
   for (int off = 32; off > 0; off >>= 1) {
     sw += __shfl_down(sw, off);
@@ -604,12 +712,29 @@ __global__ __launch_bounds__(256) void k_run_sums(const int* __restrict__ ukeys,
     sgm += __shfl_down(sgm, off);
   }
   if (lane == 0) {
-    double* o = rsum + (size_t)wv * 4;
+    double* o = psum + (size_t)wv * 3;
     o[0] = sw;
     o[1] = sk;
     o[2] = sgm;
-    o[3] = (ukeys[wv] != kKeyInvalid && sw != 0.0) ? 1.0 : 0.0;
   }
+}
+
+__global__ void k_run_combine(const int* __restrict__ ukeys, const int* __restrict__ nruns,
+                              const int* __restrict__ nparts, const long long* __restrict__ poff,
+                              const double* __restrict__ psum, double* __restrict__ rsum) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= *nruns) return;
+  double sw = 0.0, sk = 0.0, sgm = 0.0;
+  for (long long q = poff[r]; q < poff[r] + nparts[r]; ++q) {
+    sw += psum[q * 3];
+    sk += psum[q * 3 + 1];
+    sgm += psum[q * 3 + 2];
+  }
+  double* o = rsum + (size_t)r * 4;
+  o[0] = sw;
+  o[1] = sk;
+  o[2] = sgm;
+  o[3] = (ukeys[r] != kKeyInvalid && sw != 0.0) ? 1.0 : 0.0;
 }
 
 // Exclusive scan of the runs' keep flags (one workgroup) and the compacted
@@ -1173,35 +1298,45 @@ int ref_emit(const RefTablesView& T, int n_z_max, int ell, int fil, const double
   hipLaunchKernelGGL(k_ref_cells<kModeEmit>, dim3(ntask), dim3(256), 0, st, T.cosmo, T.rows,
                      T.frows, roots, T.ek, T.eg, ell, fil, (long long*)nullptr, offsets, K, G, W,
                      key, status, subh, subc);
+  if (subh)
+    hipLaunchKernelGGL(k_sub_emit, dim3(ntask), dim3(256), 0, st, T.cosmo, T.ek, T.eg, ell, offsets,
+                       subh, subc, K, G, W, key);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
 // Workspace of the binning (bytes): sorted keys and indices, cell indices,
 // run-length outputs, run starts and sums, rocPRIM temporaries, partials.
 struct BinWs {
-  size_t skey, sidx, idx, ukey, rlen, nruns, rstart, rsum, part, tmp, total, tmp_bytes;
+  size_t skey, vin, vout, ukey, rlen, nruns, rstart, nparts, poff, psum, rsum, part, tmp, total,
+      tmp_bytes;
+  long long max_parts;
 };
 inline size_t al256(size_t n) { return (n + 255) & ~size_t(255); }
 BinWs bin_ws(long long n) {
   BinWs L;
   size_t o = 0;
   auto take = [&](size_t b) { size_t r = o; o += al256(b); return r; };
-  size_t sort_b = 0, rle_b = 0, scan_b = 0;
-  (void)rocprim::radix_sort_pairs((void*)nullptr, sort_b, (int*)nullptr, (int*)nullptr, (int*)nullptr,
-                            (int*)nullptr, (size_t)n, 0, 20);
+  size_t sort_b = 0, rle_b = 0, scan_b = 0, scan2_b = 0;
+  (void)rocprim::radix_sort_pairs((void*)nullptr, sort_b, (int*)nullptr, (int*)nullptr,
+                                  (KGW*)nullptr, (KGW*)nullptr, (size_t)n, 0, 20);
   (void)rocprim::run_length_encode((void*)nullptr, rle_b, (int*)nullptr, (size_t)n, (int*)nullptr,
-                             (int*)nullptr, (int*)nullptr);
+                                   (int*)nullptr, (int*)nullptr);
   (void)rocprim::exclusive_scan((void*)nullptr, scan_b, (int*)nullptr, (long long*)nullptr, 0LL,
-                          (size_t)kRefBins + 1, rocprim::plus<long long>());
-  L.tmp_bytes = std::max(sort_b, std::max(rle_b, scan_b));
-  L.skey = take(n * sizeof(int));
-  L.sidx = take(n * sizeof(int));
-  L.idx = take(n * sizeof(int));
+                                (size_t)kRefBins + 1, rocprim::plus<long long>());
+  scan2_b = scan_b;
+  L.tmp_bytes = std::max(std::max(sort_b, rle_b), std::max(scan_b, scan2_b));
   const size_t nr = (size_t)kRefBins + 1;   // runs: every bin and the outside key
+  L.max_parts = (long long)nr + n / kRunPart + 1;
+  L.skey = take(n * sizeof(int));
+  L.vin = take(n * sizeof(KGW));
+  L.vout = take(n * sizeof(KGW));
   L.ukey = take(nr * sizeof(int));
   L.rlen = take(nr * sizeof(int));
   L.nruns = take(sizeof(int));
   L.rstart = take(nr * sizeof(long long));
+  L.nparts = take(nr * sizeof(int));
+  L.poff = take(nr * sizeof(long long));
+  L.psum = take((size_t)L.max_parts * 3 * sizeof(double));
   L.rsum = take((size_t)kRefBins * 4 * sizeof(double) + 4 * sizeof(double));
   L.part = take(4096 * sizeof(double));
   L.tmp = take(L.tmp_bytes);
@@ -1218,18 +1353,23 @@ int ref_bin(long long n, const int* key, const double* K, const double* G, const
   const BinWs L = bin_ws(n);
   char* base = static_cast<char*>(ws);
   int* skey = (int*)(base + L.skey);
-  int* sidx = (int*)(base + L.sidx);
-  int* idx = (int*)(base + L.idx);
+  KGW* vin = (KGW*)(base + L.vin);
+  KGW* vout = (KGW*)(base + L.vout);
   int* ukey = (int*)(base + L.ukey);
   int* rlen = (int*)(base + L.rlen);
   int* nruns = (int*)(base + L.nruns);
   long long* rstart = (long long*)(base + L.rstart);
+  int* nparts = (int*)(base + L.nparts);
+  long long* poff = (long long*)(base + L.poff);
+  double* psum = (double*)(base + L.psum);
   double* rsum = (double*)(base + L.rsum);
   double* part = (double*)(base + L.part);
   void* tmp = base + L.tmp;
   size_t tb = L.tmp_bytes;
-  hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, idx, n);
-  if (rocprim::radix_sort_pairs(tmp, tb, key, skey, idx, sidx, (size_t)n, 0, 20, st) != hipSuccess)
+  hipLaunchKernelGGL(k_pack_kgw, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, K, G, W,
+                     vin);
+  // stable: equal keys keep the cells' emission order
+  if (rocprim::radix_sort_pairs(tmp, tb, key, skey, vin, vout, (size_t)n, 0, 20, st) != hipSuccess)
     return SGL_ERR_HIP;
   tb = L.tmp_bytes;
   if (rocprim::run_length_encode(tmp, tb, skey, (size_t)n, ukey, rlen, nruns, st) != hipSuccess)
@@ -1241,9 +1381,16 @@ int ref_bin(long long n, const int* key, const double* K, const double* G, const
   if (rocprim::exclusive_scan(tmp, tb, rlen, rstart, 0LL, (size_t)kRefBins + 1,
                               rocprim::plus<long long>(), st) != hipSuccess)
     return SGL_ERR_HIP;
-  // at most kRefBins + 1 runs (the invalid key last)
-  hipLaunchKernelGGL(k_run_sums, dim3((kRefBins + 1 + 3) / 4), dim3(256), 0, st, ukey, rlen, nruns,
-                     rstart, sidx, K, G, W, rsum);
+  tb = L.tmp_bytes;
+  hipLaunchKernelGGL(k_run_nparts, dim3((kRefBins + 1 + 255) / 256), dim3(256), 0, st, rlen, nruns,
+                     nparts);
+  if (rocprim::exclusive_scan(tmp, tb, nparts, poff, 0LL, (size_t)kRefBins + 1,
+                              rocprim::plus<long long>(), st) != hipSuccess)
+    return SGL_ERR_HIP;
+  hipLaunchKernelGGL(k_run_parts, dim3((unsigned)((L.max_parts + 3) / 4)), dim3(256), 0, st, rlen,
+                     nruns, rstart, nparts, poff, vout, psum);
+  hipLaunchKernelGGL(k_run_combine, dim3((kRefBins + 1 + 255) / 256), dim3(256), 0, st, ukey,
+                     nruns, nparts, poff, psum, rsum);
   hipLaunchKernelGGL(k_compact_bins, dim3(1), dim3(256), 0, st, rsum, nruns, pw, pk, pg, npop);
   // sum w kappa^2 = sum over bins of (w kappa) kappa; sum W K over every cell
   const long long chunk_b = 256;

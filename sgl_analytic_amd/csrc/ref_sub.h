@@ -62,13 +62,19 @@ constexpr int kCoarseStride = (kCmEpt + kSubNmt + 31) / 32 * 32;
 constexpr int kRingStride = 2 * kSubNc * kSubNd;
 
 constexpr int kSubRows = kRefMaxShells * kRefMasses;
+// after the host records: per (row, orientation) task the decorated inner and
+// tail cell counts, the inner cell count and nq (4 ints, written by the count
+// pass of the cell kernel, read by the point expansion)
+constexpr int kSubTaskInts = 4;
+constexpr size_t kSubTaskDoubles = ((size_t)kSubRows * kEllPhi * kSubTaskInts / 2 + 31) / 32 * 32;
 struct SubLayout {
-  size_t host, coarse, ring, total;
+  size_t host, tasks, coarse, ring, total;
 };
 inline SubLayout sub_layout() {
   SubLayout L;
   L.host = 0;
-  L.coarse = L.host + (size_t)kSubRows * kHostStride;
+  L.tasks = L.host + (size_t)kSubRows * kHostStride;
+  L.coarse = L.tasks + kSubTaskDoubles;
   L.ring = L.coarse + (size_t)kSubRows * kCoarseStride;
   L.total = L.ring + (size_t)kSubRows * kRingStride;
   return L;

@@ -12,7 +12,7 @@
 //                  ray-distance grid and every clump cell's kappa, gamma
 //                  (ExactSector._clump_cells, subhalos_exact.py:1234-1304);
 //   (ref2d.hip)    the largest host-grid radius within reach of a clump;
-//   k_sub_ring     one workgroup per (row, coarse radius): the ring average
+//   k_sub_ring     one workgroup per row: the ring average
 //                  of the projected clump density and its mean clump-ray
 //                  cosine (subhalos.py:287-330);
 //   k_sub_coarse   one workgroup per row: the rare-mark cut, the bulk
@@ -294,45 +294,57 @@ __global__ __launch_bounds__(256) void k_sub_hosts(const Cosmo* __restrict__ cos
 }
 
 // ------------------------------------------------------------------- ring --
-// grid (kSubNc, rows), 128 threads: thread dd < 90 takes one distance cell
-__global__ __launch_bounds__(128) void k_sub_ring(const double* __restrict__ host,
+// one workgroup per row, threads over the (coarse radius, distance cell)
+// pairs; the 48 angle cosines and the density table in LDS
+__global__ __launch_bounds__(256) void k_sub_ring(const double* __restrict__ host,
                                                   double* __restrict__ coarse,
                                                   double* __restrict__ ring) {
-  const int r = blockIdx.x, row = blockIdx.y, t = threadIdx.x;
+  __shared__ double cth[kSubNth];
+  __shared__ double fp[kSubNR];
+  __shared__ double dcs[kSubNd];
+  __shared__ double rr[kSubNc];
+  const int row = blockIdx.x, t = threadIdx.x;
   const double* H = host + (size_t)row * kHostStride;
   if (H[kHrValid] == 0.0 || !(H[kHrXref] > 0.0)) return;
-  const double xmin = 0.5 * H[kHrXin], xmax = H[kHrXref];
-  const double xg = geom_np(xmin, xmax, kSubNc, r);
-  if (t == 0) coarse[(size_t)row * kCoarseStride + kCmLx + r] = log(xg);
-  if (t >= kSubNd) return;
-  const double rr = xg * H[kHrRs], dc = H[kHrDc + t], r200 = H[kHrR200];
-  const double* fp = H + kHrN2;
+  const double xmin = 0.5 * H[kHrXin], xmax = H[kHrXref], rs = H[kHrRs], r200 = H[kHrR200];
+  if (t < kSubNth) cth[t] = cos((t + 0.5) * kPi / kSubNth);
+  fp[t] = H[kHrN2 + t];
+  if (t < kSubNd) dcs[t] = H[kHrDc + t];
+  if (t < kSubNc) {
+    const double xg = geom_np(xmin, xmax, kSubNc, t);
+    rr[t] = xg * rs;
+    coarse[(size_t)row * kCoarseStride + kCmLx + t] = log(xg);
+  }
+  __syncthreads();
   const double xp0 = (0.5 / kSubNR) * r200, xpl = ((kSubNR - 0.5) / kSubNR) * r200;
   const double inv_dx = 1.0 / ((xpl - xp0) / (kSubNR - 1));
-  const double cc = rr * rr + dc * dc, trd = 2.0 * rr * dc;
-  double s = 0.0, sc = 0.0;
-  for (int k = 0; k < kSubNth; ++k) {
-    const double cth = cos((k + 0.5) * kPi / kSubNth);
-    const double arg = cc - trd * cth;
-    const double x = (arg > 0.0) ? sqrt(arg) : 0.0;
-    double v;
-    if (x <= xp0) {
-      v = fp[0];
-    } else if (x >= xpl) {
-      v = (x == xpl) ? fp[kSubNR - 1] : 0.0;
-    } else {
-      int jj = (int)((x - xp0) * inv_dx);
-      jj = min(max(jj, 0), kSubNR - 2);
-      const double xj = ((jj + 0.5) / kSubNR) * r200, xj1 = ((jj + 1.5) / kSubNR) * r200;
-      const double sl = (fp[jj + 1] - fp[jj]) / (xj1 - xj);
-      v = fma(sl, x - xj, fp[jj]);
-    }
-    s += v;
-    sc += v * cth;
-  }
   double* RG = ring + (size_t)row * kRingStride;
-  RG[r * kSubNd + t] = s * (1.0 / kSubNth);
-  RG[kSubNc * kSubNd + r * kSubNd + t] = (s > 0.0) ? sc / s : 0.0;
+  for (int e = t; e < kSubNc * kSubNd; e += 256) {
+    const int r = e / kSubNd, dd = e - r * kSubNd;
+    const double rv = rr[r], dc = dcs[dd];
+    const double cc = rv * rv + dc * dc, trd = 2.0 * rv * dc;
+    double s = 0.0, sc = 0.0;
+    for (int k = 0; k < kSubNth; ++k) {
+      const double arg = cc - trd * cth[k];
+      const double x = (arg > 0.0) ? sqrt(arg) : 0.0;
+      double v;
+      if (x <= xp0) {
+        v = fp[0];
+      } else if (x >= xpl) {
+        v = (x == xpl) ? fp[kSubNR - 1] : 0.0;
+      } else {
+        int jj = (int)((x - xp0) * inv_dx);
+        jj = min(max(jj, 0), kSubNR - 2);
+        const double xj = ((jj + 0.5) / kSubNR) * r200, xj1 = ((jj + 1.5) / kSubNR) * r200;
+        const double sl = (fp[jj + 1] - fp[jj]) / (xj1 - xj);
+        v = fma(sl, x - xj, fp[jj]);
+      }
+      s += v;
+      sc += v * cth[k];
+    }
+    RG[e] = s * (1.0 / kSubNth);
+    RG[kSubNc * kSubNd + e] = (s > 0.0) ? sc / s : 0.0;
+  }
 }
 
 // ----------------------------------------------------------------- coarse --
@@ -416,6 +428,8 @@ __device__ inline void gauss3(double t3, double t4, double t5, double lo, double
 
 constexpr int kSortN = 4096;
 static_assert(kSubCells <= kSortN, "the clump cells fit the sort");
+static_assert(kSubNd * 16 + kSubNc * 16 + 6 * kSubNd * kSubNbr <= kSortN + kSortN / 2,
+              "LDS buffer");
 
 __global__ __launch_bounds__(256) void k_sub_coarse(const Cosmo* __restrict__ cosmo,
                                                     const Derived* __restrict__ der,
@@ -432,6 +446,9 @@ __global__ __launch_bounds__(256) void k_sub_coarse(const Cosmo* __restrict__ co
   __shared__ double red[256];
   __shared__ double scal[8];
   __shared__ double sWk[kSubNc], sCross[kSubNc];
+  __shared__ double sdc[kSubNd], sDb[kSubNpsi];
+  __shared__ unsigned char cls[kSubCells];    // 0 no cell, 1 bulk, 2 rare
+  __shared__ int cnt[256];
   const int row = blockIdx.x, t = threadIdx.x;
   double* H = host + (size_t)row * kHostStride;
   if (H[kHrValid] == 0.0 || !(H[kHrXref] > 0.0)) return;
@@ -450,16 +467,18 @@ __global__ __launch_bounds__(256) void k_sub_coarse(const Cosmo* __restrict__ co
     colmax[t] = mx;
     colmean[t] = sm / kSubNc;
     sdA[t] = H[kHrDA + t];
+    sdc[t] = H[kHrDc + t];
   }
   if (t < kSubNpsi) {
     sdN[t] = H[kHrDN + t];
     smc[t] = H[kHrMc + t];
+    sDb[t] = H[kHrD + t];
   }
   __syncthreads();
   // a cell (b, dd) exists where dc < D_b; rate nu = dN_b dA_dd max_r ring
   auto valid = [&](int cidx) {
     const int b = cidx / kSubNd, dd = cidx - b * kSubNd;
-    return H[kHrDc + dd] < H[kHrD + b];
+    return sdc[dd] < sDb[b];
   };
   double* skey = big;
   int* sidx = reinterpret_cast<int*>(big + kSortN);
@@ -489,30 +508,64 @@ __global__ __launch_bounds__(256) void k_sub_coarse(const Cosmo* __restrict__ co
       __syncthreads();
     }
   // rare-mark cut (_cut_for): the first mark (from the top) whose cumulative
-  // rate exceeds eps_rare sets kappa*; none: kappa* = 0
-  if (t == 0) {
-    double cr = 0.0, kstar = 0.0;
-    for (int s = 0; s < kSubCells; ++s) {
-      if (!(skey[s] > -__builtin_huge_val())) break;
-      const int cidx = sidx[s], b = cidx / kSubNd, dd = cidx - b * kSubNd;
-      cr += sdN[b] * sdA[dd] * colmax[dd];
-      if (cr > kSubEpsRare) { kstar = skey[s]; break; }
+  // rate exceeds eps_rare sets kappa*; none: kappa* = 0.  The cumulative sum
+  // in sorted order as a block scan over runs of 16 marks per thread.
+  {
+    constexpr int kPerT = kSortN / 256;
+    double loc[kPerT];
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < kPerT; ++u) {
+      const int sp = t * kPerT + u;
+      double nu = 0.0;
+      if (skey[sp] > -__builtin_huge_val()) {
+        const int cidx = sidx[sp], b = cidx / kSubNd, dd = cidx - b * kSubNd;
+        nu = sdN[b] * sdA[dd] * colmax[dd];
+      }
+      loc[u] = nu;
+      acc += nu;
     }
-    scal[0] = kstar;
+    red[t] = acc;
+    __syncthreads();
+    if (t == 0) {
+      double run = 0.0;
+      for (int q = 0; q < 256; ++q) { const double v = red[q]; red[q] = run; run += v; }
+    }
+    __syncthreads();
+    double cr = red[t];
+    int first = kSortN;
+#pragma unroll
+    for (int u = 0; u < kPerT; ++u) {
+      cr += loc[u];
+      if (first == kSortN && loc[u] > 0.0 && cr > kSubEpsRare) first = t * kPerT + u;
+    }
+    __syncthreads();
+    cnt[t] = first;
+    __syncthreads();
+    if (t == 0) {
+      int f = kSortN;
+      for (int q = 0; q < 256; ++q) f = min(f, cnt[q]);
+      scal[0] = (f < kSortN) ? skey[f] : 0.0;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   const double kstar = scal[0];
-  // mass scale, rare count and range
+  // classify the cells; mass scale, rare count and range
   double pms = 0.0, nr = 0.0, kmn = __builtin_huge_val(), kmx = 0.0;
   for (int cidx = t; cidx < kSubCells; cidx += 256) {
-    if (!valid(cidx)) continue;
-    const int b = cidx / kSubNd, dd = cidx - b * kSubNd;
-    pms += sdN[b] * sdA[dd] * colmean[dd] * smc[b];
-    if (ck[cidx] > kstar) {
-      nr += 1.0;
-      kmn = fmin(kmn, ck[cidx]);
-      kmx = fmax(kmx, ck[cidx]);
+    unsigned char c = 0;
+    if (valid(cidx)) {
+      const int b = cidx / kSubNd, dd = cidx - b * kSubNd;
+      pms += sdN[b] * sdA[dd] * colmean[dd] * smc[b];
+      const double k = ck[cidx];
+      c = (k <= kstar) ? 1 : 2;
+      if (c == 2) {
+        nr += 1.0;
+        kmn = fmin(kmn, k);
+        kmx = fmax(kmx, k);
+      }
     }
+    cls[cidx] = c;
   }
   const double ms = fmax(block_sum(pms, red), 1e-300);
   const double nrare = block_sum(nr, red);
@@ -527,9 +580,8 @@ __global__ __launch_bounds__(256) void k_sub_coarse(const Cosmo* __restrict__ co
     double acc = 0.0;
     for (int b = 0; b < kSubNpsi; ++b) {
       const int cidx = b * kSubNd + dd;
-      if (!valid(cidx)) continue;
+      if (cls[cidx] != 1) continue;
       const double k = ck[cidx];
-      if (!(k <= kstar)) continue;
       const double p = sdN[b] * sdA[dd], g = cg[cidx], mh = smc[b] / ms;
       double v;
       switch (col) {
@@ -578,7 +630,7 @@ __global__ __launch_bounds__(256) void k_sub_coarse(const Cosmo* __restrict__ co
       if (lane < kSubNpsi) {
         for (int dd = 0; dd < kSubNd; ++dd) {
           const int cidx = lane * kSubNd + dd;
-          if (!valid(cidx) || !(ck[cidx] <= kstar)) continue;
+          if (cls[cidx] != 1) continue;
           const double v = RG[r * kSubNd + dd] * sdN[lane] * sdA[dd];
           n_ += v;
           a_ += v * ck[cidx];
@@ -589,7 +641,7 @@ __global__ __launch_bounds__(256) void k_sub_coarse(const Cosmo* __restrict__ co
       if (lane < kSubNpsi) {
         for (int dd = 0; dd < kSubNd; ++dd) {
           const int cidx = lane * kSubNd + dd;
-          if (!valid(cidx) || !(ck[cidx] <= kstar)) continue;
+          if (cls[cidx] != 1) continue;
           const double dk = ck[cidx] - ab;
           w += RG[r * kSubNd + dd] * sdN[lane] * sdA[dd] * dk * dk;
         }
@@ -619,6 +671,54 @@ __global__ __launch_bounds__(256) void k_sub_coarse(const Cosmo* __restrict__ co
     for (int q = 0; q <= kSubNbr; ++q)
       be[q] = geom_np(kr_min * (1 - 1e-12), kr_max * (1 + 1e-12), kSubNbr + 1, q);
   }
+  auto rare_bin = [&](double k) {
+    int bi = 0;
+    while (bi < kSubNbr && be[bi + 1] <= k) ++bi;   // searchsorted(be, k, "right") - 1
+    return min(max(bi, 0), kSubNbr - 1);
+  };
+  // per (distance cell, rare bin): rate prefactor sum W, mean kappa mu, the
+  // two-pass spread V = sum p (k - mu)^2, and the sums of p m, p g^2, p g.
+  // Every rate of a distance cell shares its ring factor, so the per-radius
+  // sums are ring-weighted sums of these, and the within-bin kappa variance
+  // is sum_d ring (V_d + W_d (mu_d - kbar)^2): both terms >= 0, exact for a
+  // one-cell bin (the two-pass form of _rare_bins_var)
+  double* ra = big + kSubNd * 16 + kSubNc * 16;       // [kSubNd][kSubNbr][6]
+  if (nbr) {
+    for (int e = t; e < kSubNd * kSubNbr; e += 256) {
+      const int dd = e / kSubNbr, bi = e - dd * kSubNbr;
+      double W0 = 0.0, W1 = 0.0, M1 = 0.0, G2 = 0.0, G1 = 0.0;
+      for (int b = 0; b < kSubNpsi; ++b) {
+        const int cidx = b * kSubNd + dd;
+        if (cls[cidx] != 2) continue;
+        const double k = ck[cidx];
+        if (rare_bin(k) != bi) continue;
+        const double p = sdN[b] * sdA[dd], g = cg[cidx];
+        W0 += p;
+        W1 += p * k;
+        M1 += p * smc[b];
+        G2 += p * g * g;
+        G1 += p * g;
+      }
+      const double mu = (W0 > 0.0) ? W1 / W0 : 0.0;
+      double V = 0.0;
+      for (int b = 0; b < kSubNpsi; ++b) {
+        const int cidx = b * kSubNd + dd;
+        if (cls[cidx] != 2) continue;
+        const double k = ck[cidx];
+        if (rare_bin(k) != bi) continue;
+        const double dk = k - mu;
+        V += sdN[b] * sdA[dd] * dk * dk;
+      }
+      double* o = ra + (size_t)e * 6;
+      o[0] = W0;
+      o[1] = mu;
+      o[2] = V;
+      o[3] = M1;
+      o[4] = G2;
+      o[5] = G1;
+    }
+  }
+  __syncthreads();
   double pmax = 0.0;
   if (t < kSubNc) {
     const int r = t;
@@ -629,32 +729,29 @@ __global__ __launch_bounds__(256) void k_sub_coarse(const Cosmo* __restrict__ co
     double nub[kSubNbr] = {}, k1b[kSubNbr] = {}, m1b[kSubNbr] = {}, g2b[kSubNbr] = {},
            e1b[kSubNbr] = {}, v2b[kSubNbr] = {};
     if (nbr) {
-      for (int cidx = 0; cidx < kSubCells; ++cidx) {
-        if (!valid(cidx) || !(ck[cidx] > kstar)) continue;
-        const int b = cidx / kSubNd, dd = cidx - b * kSubNd;
-        const double k = ck[cidx];
-        int bi = 0;
-        while (bi < kSubNbr && be[bi + 1] <= k) ++bi;   // searchsorted(be, k, "right") - 1
-        bi = min(max(bi, 0), kSubNbr - 1);
-        const double p = sdN[b] * sdA[dd], rv = RG[r * kSubNd + dd], g = cg[cidx];
-        nub[bi] += rv * p;
-        k1b[bi] += rv * (p * k);
-        m1b[bi] += rv * (p * smc[b]);
-        g2b[bi] += rv * (p * g * g);
-        e1b[bi] += rv * EC[r * kSubNd + dd] * (p * g);
+      for (int dd = 0; dd < kSubNd; ++dd) {
+        const double rv = RG[r * kSubNd + dd], re = rv * EC[r * kSubNd + dd];
+#pragma unroll
+        for (int bi = 0; bi < kSubNbr; ++bi) {
+          const double* o = ra + (dd * kSubNbr + bi) * 6;
+          nub[bi] += rv * o[0];
+          k1b[bi] += rv * (o[0] * o[1]);
+          m1b[bi] += rv * o[3];
+          g2b[bi] += rv * o[4];
+          e1b[bi] += re * o[5];
+        }
       }
       double kbg[kSubNbr];
 #pragma unroll
       for (int q = 0; q < kSubNbr; ++q) kbg[q] = k1b[q] / ((nub[q] > 0.0) ? nub[q] : 1.0);
-      for (int cidx = 0; cidx < kSubCells; ++cidx) {
-        if (!valid(cidx) || !(ck[cidx] > kstar)) continue;
-        const int b = cidx / kSubNd, dd = cidx - b * kSubNd;
-        const double k = ck[cidx];
-        int bi = 0;
-        while (bi < kSubNbr && be[bi + 1] <= k) ++bi;
-        bi = min(max(bi, 0), kSubNbr - 1);
-        const double dk = k - kbg[bi];
-        v2b[bi] += RG[r * kSubNd + dd] * (sdN[b] * sdA[dd]) * dk * dk;
+      for (int dd = 0; dd < kSubNd; ++dd) {
+        const double rv = RG[r * kSubNd + dd];
+#pragma unroll
+        for (int bi = 0; bi < kSubNbr; ++bi) {
+          const double* o = ra + (dd * kSubNbr + bi) * 6;
+          const double dm = o[1] - kbg[bi];
+          v2b[bi] += rv * (o[2] + o[0] * dm * dm);
+        }
       }
     }
     // K rule, S | K rule (regression on K plus the residual with the marginal
@@ -797,7 +894,7 @@ int sub_hosts_launch(const double* cosmo, const double* derived, const double* s
 int sub_coarse_launch(const double* cosmo, const double* derived, const double* slices,
                       const double* spl, const double* sigtab, double* host, double* coarse,
                       double* ring, hipStream_t st) {
-  hipLaunchKernelGGL(k_sub_ring, dim3(kSubNc, kSubRows), dim3(128), 0, st, host, coarse, ring);
+  hipLaunchKernelGGL(k_sub_ring, dim3(kSubRows), dim3(256), 0, st, host, coarse, ring);
   hipLaunchKernelGGL(k_sub_coarse, dim3(kSubRows), dim3(256), 0, st,
                      reinterpret_cast<const Cosmo*>(cosmo), reinterpret_cast<const Derived*>(derived),
                      slices, spl, sigtab, host, coarse, ring);

@@ -43,6 +43,7 @@ CM_LMT = CM_A2 + NC * NQ
 CM_LRS, CM_LKS, CM_EPT = CM_LMT + NMT, CM_LMT + 2 * NMT, CM_LMT + 3 * NMT
 COARSE_STRIDE = (CM_EPT + NMT + 31) // 32 * 32
 ROWS = 104 * 99
+TASK_DOUBLES = (ROWS * 12 * 4 // 2 + 31) // 32 * 32   # per-task counts after the host records
 
 
 @pytest.fixture(scope="module")
@@ -51,7 +52,8 @@ def dev_sub():
     pop = ref2d.build(PLANCK, 1.0, sub=True, keep_sub=True)
     w = pop.sub_tables.cpu().numpy()
     host = w[:ROWS * HOST_STRIDE].reshape(ROWS, HOST_STRIDE)
-    coarse = w[ROWS * HOST_STRIDE:ROWS * (HOST_STRIDE + COARSE_STRIDE)].reshape(ROWS, COARSE_STRIDE)
+    c0 = ROWS * HOST_STRIDE + TASK_DOUBLES
+    coarse = w[c0:c0 + ROWS * COARSE_STRIDE].reshape(ROWS, COARSE_STRIDE)
     return pop, host, coarse
 
 
