@@ -34,6 +34,8 @@
  *                     (subhalos_exact.py:802-925)
  *   oracle_bin_fine   Cells.add's fine (kappa, gamma) lattice sums
  *                     (sgl_full.py:1314-1383, 971-1046)
+ *   oracle_xi_deposit the xi_lin deposit of one row's points on the coarse
+ *                     lattice (sgl_full.py:1081-1183, 1397-1441)
  *
  * Build: oracle/build.py (gcc -O2 -fopenmp -shared).
  */
@@ -595,4 +597,111 @@ void oracle_bin_fine(int64_t n, const double *K, const double *G, const double *
         }
     }
     *dropped += drop;
+}
+
+/* ------------------------------------------------------------------------
+ * xi_lin deposit (sgl_full.py:1081-1130 _xi_add_grouped, 1142-1183
+ * _bessel_pack_nb, 1397-1441 Cells._xi_deposit) for one row's points.
+ * Points inside the coarse (kappa, gamma) box are grouped by their lens
+ * distance R (stable order); per distinct R the table J_{2m}(k_j R), m = 0..2
+ * (m >= 1 only for tangential shear), by the power series below x = 2 and by
+ * J0, J1 and the upward recurrence above; then
+ *   Hx[bin][m][j] += (sum of beta w of the group's points in the bin) J_{2m}(k_j R),
+ *   lx[0][j] += (sum beta w kappa) J0(k_j R),  lx[1][j] += (sum beta w gamma) J2(k_j R).
+ * Hx is [nkc * ngc][3][nkp], lx [2][nkp].
+ */
+static const double *xi_sort_R;
+static int xi_cmp(const void *a, const void *b) {
+    int64_t i = *(const int64_t *)a, j = *(const int64_t *)b;
+    double ri = xi_sort_R[i], rj = xi_sort_R[j];
+    if (ri < rj) return -1;
+    if (ri > rj) return 1;
+    return (i < j) ? -1 : (i > j);
+}
+
+static void xi_bessel(double x, int tang, double *J0, double *J2, double *J4) {
+    if (x < 2.0) {
+        double h = 0.5 * x, h2 = h * h;
+        double t0 = 1.0, s0 = 1.0, t2 = 0.5, s2 = 0.5, t4 = 1.0 / 24.0, s4 = 1.0 / 24.0;
+        int nt = (x < 0.02) ? 3 : ((x < 0.3) ? 6 : 14);
+        for (int k = 1; k < nt; ++k) {
+            double f = (-h2) / k;
+            t0 *= f / k;
+            s0 += t0;
+            t2 *= f / (k + 2);
+            s2 += t2;
+            t4 *= f / (k + 4);
+            s4 += t4;
+        }
+        *J0 = s0;
+        *J2 = tang ? s2 * (h * h) : 0.0;
+        *J4 = tang ? s4 * (h * h * h * h) : 0.0;
+        return;
+    }
+    double j0v = j0(x);
+    *J0 = j0v;
+    if (!tang) { *J2 = 0.0; *J4 = 0.0; return; }
+    double inv = 1.0 / x, j1v = j1(x);
+    double j2v = 2.0 * inv * j1v - j0v;
+    double j3v = 4.0 * inv * j2v - j1v;
+    *J2 = j2v;
+    *J4 = 6.0 * inv * j3v - j2v;
+}
+
+void oracle_xi_deposit(int64_t n, const double *K, const double *G, const double *W,
+                       const double *R, double beta, int64_t tang, int64_t nkp, const double *kp,
+                       const double *ekc, int64_t nek, const double *egc, int64_t neg,
+                       double *Hx, double *lx) {
+    int64_t nkc = nek - 1, ngc = neg - 1;
+    int64_t *ord = malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    int64_t *bins = malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    double *slot = calloc((size_t)(nkc * ngc), sizeof(double));
+    int64_t *mark = malloc((size_t)(nkc * ngc) * sizeof(int64_t));
+    int64_t *touched = malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    double *Jt = malloc((size_t)(3 * nkp) * sizeof(double));
+    for (int64_t f = 0; f < nkc * ngc; ++f) mark[f] = -1;
+    for (int64_t i = 0; i < n; ++i) {
+        ord[i] = i;
+        double k = K[i], g = G[i];
+        int64_t a = bin_right(ekc, nek, k), b = bin_right(egc, neg, g);
+        if (k == ekc[nek - 1]) a -= 1;
+        if (g == egc[neg - 1]) b -= 1;
+        bins[i] = (a >= 1 && a <= nkc && b >= 1 && b <= ngc) ? (a - 1) * ngc + (b - 1) : -1;
+    }
+    xi_sort_R = R;
+    qsort(ord, (size_t)n, sizeof(int64_t), xi_cmp);
+    int64_t t = 0, u = 0;
+    while (t < n) {
+        int64_t t1 = t;
+        double r = R[ord[t]];
+        while (t1 < n && R[ord[t1]] == r) ++t1;
+        int64_t nt = 0;
+        double sk = 0.0, sg = 0.0;
+        for (int64_t q = t; q < t1; ++q) {
+            int64_t i = ord[q], f = bins[i];
+            if (f < 0) continue;
+            double bw = beta * W[i];
+            if (mark[f] != u) { mark[f] = u; touched[nt++] = f; }
+            slot[f] += bw;
+            sk += bw * K[i];
+            sg += bw * G[i];
+        }
+        if (nt > 0) {
+            for (int64_t j = 0; j < nkp; ++j)
+                xi_bessel(kp[j] * r, (int)tang, &Jt[j], &Jt[nkp + j], &Jt[2 * nkp + j]);
+            for (int64_t q = 0; q < nt; ++q) {
+                int64_t f = touched[q];
+                double v = slot[f];
+                slot[f] = 0.0;
+                double *H = Hx + f * 3 * nkp;
+                for (int m = 0; m < 3; ++m)
+                    for (int64_t j = 0; j < nkp; ++j) H[m * nkp + j] += v * Jt[m * nkp + j];
+            }
+            for (int64_t j = 0; j < nkp; ++j) lx[j] += sk * Jt[j];
+            for (int64_t j = 0; j < nkp; ++j) lx[nkp + j] += sg * Jt[nkp + j];
+        }
+        ++u;
+        t = t1;
+    }
+    free(ord); free(bins); free(slot); free(mark); free(touched); free(Jt);
 }

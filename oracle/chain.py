@@ -71,6 +71,8 @@ def lib():
         _LIB.oracle_sub_ring.argtypes = [i64, d, i64, d, i64, d, i64, d, d, d, d]
         f64 = ctypes.c_double
         _LIB.oracle_bin_fine.argtypes = [i64, d, d, d, d, i64, d, i64, d, d, d, d]
+        _LIB.oracle_xi_deposit.argtypes = [i64, d, d, d, d, ctypes.c_double, i64, i64, d, d, i64,
+                                           d, i64, d, d]
         _LIB.oracle_sub_emit.argtypes = [i64, d, d, d, f64, f64, i64, i64, f64, d, d, d, d, d, d,
                                          i64, d, d, d, d, f64, f64, d, d, d]
     return _LIB

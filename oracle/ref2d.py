@@ -314,14 +314,21 @@ def rows(cos: Cosmology, zs: float, ell=False, fil=False, edge="r200", shells=No
     return out
 
 
-def _accumulate(cos, zs, ell, fil, edge, sub, shells=None, keep_points=False):
+def _accumulate(cos, zs, ell, fil, edge, sub, shells=None, keep_points=False, xi=False):
     """The rows of the given shells binned into the fine lattice: the sums
-    (w, w kappa, w gamma) per bin and the builder's bookkeeping."""
+    (w, w kappa, w gamma) per bin and the builder's bookkeeping.  With ``xi``
+    also the xi_lin deposit of every row (oracle/xilin.py), one Limber node of
+    width dchi = c dz / H(z) per shell (sgl_full.py:1673-1683), closed into
+    compressed row groups at the shell's end."""
     ek, eg = fine_edges()
     nkf, ngf = ek.size - 1, eg.size - 1
     acc = dict(H0=np.zeros(nkf * ngf), H0k=np.zeros(nkf * ngf), H0g=np.zeros(nkf * ngf),
                N_expl=0.0, N_fil=0.0, kbar=0.0, n_points=0, n_decorated=0, drop=np.zeros(1),
-               pts=[], model=None)
+               pts=[], model=None, xi_groups=[])
+    if xi:
+        from . import xilin
+        kp, cj = xilin.kperp_grid(cos)
+    shell_xi = None
     phis = (np.arange(NPHI) + 0.5) * (0.5 * pi / NPHI)
     fph, fwt = chain.fil_phi_rule()
     ue = chain.fil_u_edges()
@@ -345,34 +352,54 @@ def _accumulate(cos, zs, ell, fil, edge, sub, shells=None, keep_points=False):
         zl, Ml = np.array([sh[2] for sh in z_shells(zs)]), engine_grids()[1]
         model = acc["model"] = _sub.SubhaloModel(cos, zs, zl, Ml)
     for r in rows(cos, zs, ell=ell, fil=fil, edge=edge, shells=shells):
+        if xi and (shell_xi is None or shell_xi[0] != r["shell"]):
+            if shell_xi is not None:
+                acc["xi_groups"] += shell_xi[1].end(shell_xi[0])
+            shell_xi = (r["shell"], xilin.Shell(kp, cj, CKMS / cos.Hz(r["z"]) * r["dz"]))
         acc["N_expl"] += r["Nbar"]
         grid = host_grid(r["ks"], r["xt"], r["xm"], r["Nbar"], r["eps"], phis if ell else None)
         h = model.host(r["shell"], r["jM"]) if model is not None else None
         if h is not None:
-            K, G, W = _sub.decorate(model, h, grid, r["ks"], r["xt"], r["eps"])
+            K, G, W, X = _sub.decorate(model, h, grid, r["ks"], r["xt"], r["eps"], with_x=True)
             acc["n_decorated"] += 1
         else:
             K, G, W = host_cells(r["ks"], r["xt"], r["xm"], r["Nbar"], r["eps"], grid=grid)
+            X = np.concatenate([grid["xq"], grid["xo"]])
         add(K, G, W)
+        if xi:
+            # halos: lens distance R = x r_s (1 + z), tangential shear, and the
+            # analytic shear tail beyond x_m (sgl_full.py:2128-2136)
+            a = r["rs"] * (1 + r["z"])
+            shell_xi[1].deposit(K, G, W, X * a, r["beta"], True)
+            Rm = r["xm"] * a
+            shell_xi[1].shear_tail(r["beta"], r["ks"] * 4 * (log(1 + r["xt"]) - r["xt"] / (1 + r["xt"]))
+                                   * a ** 2, Rm, r["Nbar"] / (pi * Rm ** 2))
         if fil:
             acc["N_fil"] += r["NF"]
             k0 = r["k0base"] / np.maximum(2 * r["rsF"], np.abs(np.cos(fph)) * r["LF"])
             K, G = kappa_gamma_cyl(uc[:, None], k0[None, :])
             W = r["NF"] * ufa[:, None] * fwt[None, :] * np.ones_like(K)
             add(K, G, W)
+            if xi:
+                # filaments: R = u r_sF (1 + z), axis not tangential (m = 0),
+                # sgl_full.py:2002-2004
+                Rf = uc[:, None] * (r["rsF"] * (1 + r["z"])) * np.ones_like(K)
+                shell_xi[1].deposit(K, G, W, Rf, r["beta_fil"], False)
+    if shell_xi is not None:
+        acc["xi_groups"] += shell_xi[1].end(shell_xi[0])
     return acc
 
 
 def _worker(args):
-    cos_kw, zs, ell, fil, edge, sub, shells = args
-    acc = _accumulate(Cosmology(**cos_kw), zs, ell, fil, edge, sub, shells)
+    cos_kw, zs, ell, fil, edge, sub, shells, xi = args
+    acc = _accumulate(Cosmology(**cos_kw), zs, ell, fil, edge, sub, shells, xi=xi)
     acc.pop("model")
     acc.pop("pts")
     return acc
 
 
 def build(cos: Cosmology, zs: float, ell=False, fil=False, keep_points=False, edge="r200",
-          sub=False, workers=None):
+          sub=False, workers=None, xi=False):
     """build_population(..., kap_floor=1e-8, trunc="r200", zint="midpoint")
     without clustering: every row's cells (decorated by subhalos on hosts
     that carry them when ``sub``, oracle/sub.py) binned into the fine lattice
@@ -381,13 +408,13 @@ def build(cos: Cosmology, zs: float, ell=False, fil=False, keep_points=False, ed
     and the partial sums are added in worker order."""
     n_sh = len(z_shells(zs))
     if workers is None:
-        workers = min(8, os.cpu_count() or 1) if sub else 1
+        workers = min(8, os.cpu_count() or 1) if (sub or xi) else 1
     workers = max(1, min(int(workers), n_sh))
     if workers == 1 or keep_points:
-        accs = [_accumulate(cos, zs, ell, fil, edge, sub, None, keep_points)]
+        accs = [_accumulate(cos, zs, ell, fil, edge, sub, None, keep_points, xi=xi)]
     else:
         import multiprocessing as mp
-        jobs = [(cos.kw, zs, ell, fil, edge, sub, set(range(w, n_sh, workers)))
+        jobs = [(cos.kw, zs, ell, fil, edge, sub, set(range(w, n_sh, workers)), xi)
                 for w in range(workers)]
         # fresh interpreters (no inherited OpenMP or device state), one
         # OpenMP thread each
@@ -412,6 +439,9 @@ def build(cos: Cosmology, zs: float, ell=False, fil=False, keep_points=False, ed
     pop.k = H0k[nz] / pop.w
     pop.g = H0g[nz] / pop.w
     pop.bins = nz
+    # xi_lin row groups in shell order, m ascending within a shell
+    pop.xi_groups = sorted((g for a in accs for g in a["xi_groups"]),
+                           key=lambda g: (g["shell"], g["m"]))
     if keep_points:
         pop.points = accs[0]["pts"]
     return pop
@@ -649,14 +679,22 @@ XI_TAIL = np.linspace(1.005, 9.0, 1600)   # and tail grid
 
 
 def run(zs, cosmo: dict = None, ell=False, fil=False, xi_out=None, keep=False, edge="r200",
-        sub=False):
-    """pipeline.run for the ingredient sets without clustering.
+        sub=False, bias=False):
+    """pipeline.run (pipeline.py:114-275) with the Gaussian pair closure.
+    ``bias``: the xi_lin clustering term (oracle/xilin.py) in Lambda and in
+    sigma_kappa (sgl_full.py:3094-3128, 3437-3446).
     xi_out None: the body and tail grids of the CLI (pipeline.py:211-215)."""
     cos = Cosmology(**(cosmo or {}))
-    pop = build(cos, zs, ell=ell, fil=fil, edge=edge, sub=sub)
-    sig = pop.sigma()
+    pop = build(cos, zs, ell=ell, fil=fil, edge=edge, sub=sub, xi=bias)
     xo = np.concatenate([XI_BODY, XI_TAIL]) if xi_out is None else np.asarray(xi_out, float)
-    res = invert(lambda a, b: lam(pop, a, b), sig, xo, keep=keep)
+    if bias:
+        from . import xilin
+        psi = pop.psi2 = xilin.Psi2(pop.xi_groups)
+        sig = float(sqrt(pop.sigma() ** 2 + psi.var()))
+        res = invert(lambda a, b: lam(pop, a, b) + psi(a, b), sig, xo, keep=keep)
+    else:
+        sig = pop.sigma()
+        res = invert(lambda a, b: lam(pop, a, b), sig, xo, keep=keep)
     res.pop = pop
     return res
 

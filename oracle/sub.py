@@ -406,11 +406,13 @@ class SubhaloModel:
         return out
 
 
-def decorate(model, h, grid, ks, xt, eps):
+def decorate(model, h, grid, ks, xt, eps, with_x=False):
     """The subhalo branch of _truncated_host (sgl_full.py:2088-2127): host cells
     within reach of a clump, x <= x_sub = (r_t + max D) / r_s (1 + 1e-3), are
     replaced by the decorated points; returns (K, G, W) in the reference's
-    order (plain inner cells, decorated inner cells, plain tail, decorated tail)."""
+    order (plain inner cells, decorated inner cells, plain tail, decorated tail),
+    and with ``with_x`` also each point's host-cell radius x (a decorated cell's
+    x repeated for each of its points, sgl_full.py:2079-2085)."""
     from .ref2d import kg_eps, trunc_kg
     rs = h["rs"]
     x_sub = (xt * rs + float(np.max(h["D"]))) / rs * 1.001
@@ -419,21 +421,28 @@ def decorate(model, h, grid, ks, xt, eps):
     xref = xref[xref <= x_sub]
     cm = model.coarse(h, xref)
     parts = []
+
+    def emitted(x, Wx, ph):
+        K, G, Wd = model.emit(h, cm, xt, x, Wx, ph)
+        nq, rem = divmod(K.size, max(x.size, 1))
+        if rem or nq == 0:
+            raise RuntimeError("decorated host: not nq points per cell")
+        return K, G, Wd, np.repeat(x, nq)
+
     mi = xq <= x_sub
     if (~mi).any():
         if phq is not None:
             Kn, Gn = kg_eps(xq[~mi], phq[~mi], eps, ks, xt)
         else:
             Kn, Gn = trunc_kg(xq[~mi], ks, xt)
-        parts.append((Kn, Gn, W[~mi]))
+        parts.append((Kn, Gn, W[~mi], xq[~mi]))
     if mi.any():
-        parts.append(tuple(model.emit(h, cm, xt, xq[mi], W[mi],
-                                      None if phq is None else phq[mi])))
+        parts.append(emitted(xq[mi], W[mi], None if phq is None else phq[mi]))
     if xo.size:
         mo = xo <= x_sub
         if (~mo).any():
             xn = xo[~mo]
-            parts.append((np.zeros(xn.size), ks * 4 * _mu(xt) / xn ** 2, Wo[~mo]))
+            parts.append((np.zeros(xn.size), ks * 4 * _mu(xt) / xn ** 2, Wo[~mo], xn))
         if mo.any():
-            parts.append(tuple(model.emit(h, cm, xt, xo[mo], Wo[mo], None)))
-    return tuple(np.concatenate([p[j] for p in parts]) for j in range(3))
+            parts.append(emitted(xo[mo], Wo[mo], None))
+    return tuple(np.concatenate([p[j] for p in parts]) for j in range(4 if with_x else 3))
