@@ -354,7 +354,7 @@ def _accumulate(cos, zs, ell, fil, edge, sub, shells=None, keep_points=False, xi
     for r in rows(cos, zs, ell=ell, fil=fil, edge=edge, shells=shells):
         if xi and (shell_xi is None or shell_xi[0] != r["shell"]):
             if shell_xi is not None:
-                acc["xi_groups"] += shell_xi[1].end(shell_xi[0])
+                acc["xi_groups"] += shell_xi[1].end(shell_xi[0], compress=xi != "dense")
             shell_xi = (r["shell"], xilin.Shell(kp, cj, CKMS / cos.Hz(r["z"]) * r["dz"]))
         acc["N_expl"] += r["Nbar"]
         grid = host_grid(r["ks"], r["xt"], r["xm"], r["Nbar"], r["eps"], phis if ell else None)
@@ -386,7 +386,7 @@ def _accumulate(cos, zs, ell, fil, edge, sub, shells=None, keep_points=False, xi
                 Rf = uc[:, None] * (r["rsF"] * (1 + r["z"])) * np.ones_like(K)
                 shell_xi[1].deposit(K, G, W, Rf, r["beta_fil"], False)
     if shell_xi is not None:
-        acc["xi_groups"] += shell_xi[1].end(shell_xi[0])
+        acc["xi_groups"] += shell_xi[1].end(shell_xi[0], compress=xi != "dense")
     return acc
 
 
@@ -682,14 +682,15 @@ def run(zs, cosmo: dict = None, ell=False, fil=False, xi_out=None, keep=False, e
         sub=False, bias=False):
     """pipeline.run (pipeline.py:114-275) with the Gaussian pair closure.
     ``bias``: the xi_lin clustering term (oracle/xilin.py) in Lambda and in
-    sigma_kappa (sgl_full.py:3094-3128, 3437-3446).
+    sigma_kappa (sgl_full.py:3094-3128, 3437-3446); bias="dense" keeps every
+    shell's rows uncompressed and unpruned (one per k_perp node).
     xi_out None: the body and tail grids of the CLI (pipeline.py:211-215)."""
     cos = Cosmology(**(cosmo or {}))
     pop = build(cos, zs, ell=ell, fil=fil, edge=edge, sub=sub, xi=bias)
     xo = np.concatenate([XI_BODY, XI_TAIL]) if xi_out is None else np.asarray(xi_out, float)
     if bias:
         from . import xilin
-        psi = pop.psi2 = xilin.Psi2(pop.xi_groups)
+        psi = pop.psi2 = xilin.Psi2(pop.xi_groups, prune=0 if bias == "dense" else xilin.XI_PRUNE)
         sig = float(sqrt(pop.sigma() ** 2 + psi.var()))
         res = invert(lambda a, b: lam(pop, a, b) + psi(a, b), sig, xo, keep=keep)
     else:

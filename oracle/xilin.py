@@ -104,8 +104,10 @@ class Shell:
         a = self.kp * Rm
         self.lx[1] += beta * n_area * 2 * pi * Gam * sp.j1(a) / a
 
-    def end(self, shell):
-        """xi_end (sgl_full.py:1478-1523): the compressed rows per m."""
+    def end(self, shell, compress=True):
+        """xi_end (sgl_full.py:1478-1523): the compressed rows per m.
+        compress=False keeps A^T itself (one row per k_perp node): the same
+        quadratic form without the dropped modes below 1e-6 of the largest."""
         kmc, gmc = coarse_centres()
         ngc, nkp = self.ngc, self.kp.size
         sw = sqrt(self.c / self.dchi)
@@ -125,6 +127,9 @@ class Shell:
             else:
                 lres = np.zeros(nkp)
             A = np.vstack([Hm, lres[None, :]]) * sw[None, :]
+            if not compress:
+                out.append(dict(shell=shell, m=m, idx=idx, rows=np.ascontiguousarray(A.T)))
+                continue
             ev, V = np.linalg.eigh(A.T @ A)
             S = sqrt(np.maximum(ev[::-1], 0.0))
             Vd = V[:, ::-1]

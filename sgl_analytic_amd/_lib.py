@@ -33,10 +33,15 @@ EXPORTS = (
     "sgl_ref_bin_bytes", "sgl_ref_bin", "sgl_ref_lam_axis", "sgl_ref_phi_bytes", "sgl_ref_phi",
     "sgl_ref_p3_bytes", "sgl_ref_p3", "sgl_ref_arcs", "sgl_ref_moments", "sgl_ref_gemm_f64",
     "sgl_lnphi_exact_noalloc", "sgl_ref_roots", "sgl_ref_sub_bytes", "sgl_ref_sub_build",
+    "sgl_ref_cells_emit_xi", "sgl_ref_xi_state_bytes", "sgl_ref_xi_init", "sgl_ref_xi_sort_bytes",
+    "sgl_ref_xi_sort", "sgl_ref_xi_rows_bytes", "sgl_ref_xi_rows", "sgl_ref_xi_axis",
+    "sgl_ref_xi_grid_bytes", "sgl_ref_xi_grid", "sgl_ref_xi_info",
 )
 NOALLOC_SCRATCH_DOUBLES = 1 << 22   # capacity of sgl_lnphi_exact_noalloc's own array
 REF_MAX_SHELLS, REF_MASSES, REF_FINE_BINS, REF_ROOT_STRIDE = 104, 99, 961 * 300, 128
 REF_STATUS_ROOTS, REF_STATUS_EDGES, REF_STATUS_BAD = 16, 32, 2
+# sgl_ref_xi_info record: [n_runs, n_parts, n_z, nP, nQ, variance, largest s_0]
+XI_INFO_RUNS, XI_INFO_PARTS, XI_INFO_NZ, XI_INFO_NP, XI_INFO_NQ, XI_INFO_VAR = 0, 1, 2, 3, 4, 5
 EPI_PLAIN, EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU = 0, 1, 2, 3
 TUNE_BIN_THREADS, TUNE_LNPHI_BLOCK, TUNE_GEMM_BM = 0, 1, 2
 PSI2_STRIDE = 8  # SGL_PSI2_STRIDE: batched two-point sub-grid stride
@@ -98,6 +103,14 @@ def load(path: str = LIB_PATH):
     L.sgl_ref_bin_bytes.argtypes = [i64]
     L.sgl_ref_phi_bytes.restype = sz
     L.sgl_ref_phi_bytes.argtypes = [i32, i32, i32]
+    L.sgl_ref_xi_state_bytes.restype = sz
+    L.sgl_ref_xi_state_bytes.argtypes = []
+    L.sgl_ref_xi_sort_bytes.restype = sz
+    L.sgl_ref_xi_sort_bytes.argtypes = [i64, i64]
+    L.sgl_ref_xi_rows_bytes.restype = sz
+    L.sgl_ref_xi_rows_bytes.argtypes = [i64, i64]
+    L.sgl_ref_xi_grid_bytes.restype = sz
+    L.sgl_ref_xi_grid_bytes.argtypes = [i32, i32, i32]
     L.sgl_ref_p3_bytes.restype = sz
     L.sgl_ref_p3_bytes.argtypes = [i32, i32, i32, i32]
     sigs = {
@@ -130,6 +143,13 @@ def load(path: str = LIB_PATH):
         "sgl_ref_sub_build": [vp, i32, vp, vp, vp, vp],
         "sgl_ref_cells_count": [vp, i32, vp, vp, vp, vp, vp],
         "sgl_ref_cells_emit": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+        "sgl_ref_cells_emit_xi": [vp, i32, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+        "sgl_ref_xi_init": [vp, i32, vp, vp],
+        "sgl_ref_xi_info": [vp, vp, vp],
+        "sgl_ref_xi_sort": [vp, i32, i64, vp, i64, vp, vp, vp, vp],
+        "sgl_ref_xi_rows": [vp, i32, i64, vp, vp, vp, vp, i64, vp, vp, i64, i64, vp, vp, vp],
+        "sgl_ref_xi_axis": [vp, i32, vp, vp, i32, i32, vp, vp],
+        "sgl_ref_xi_grid": [vp, i32, vp, vp, i64, i32, i32, i32, vp, f64, f64, i32, i32, vp, vp],
         "sgl_ref_bin": [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp],
         "sgl_ref_lam_axis": [vp, vp, vp, i32, i32, vp, vp],
         "sgl_ref_phi": [vp, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp],

@@ -29,6 +29,7 @@
 #include <rocprim/rocprim.hpp>
 
 #include "ref_sub.h"
+#include "ref_xi.h"
 
 namespace sgl {
 
@@ -289,6 +290,34 @@ __device__ inline int bin_right_g(const double* __restrict__ eg, double v) {
   return walk_right(eg, n, v, g);
 }
 
+// The same search on the coarse lattice of the xi_lin term: 342 signed-log
+// kappa edges, 111 log gamma edges (sgl_full.py:457, 1081-1130), and the key
+// shell * kXiBins + bin of a point inside it.
+__device__ inline int coarse_bin_k(const double* __restrict__ ekc, double v) {
+  constexpr int n = kXiKc + 1, npos = n / 2;
+  if (!(v == v)) return n;
+  const double av = fabs(v);
+  int g = npos;
+  if (av >= 1e-7) {
+    const int m = (int)(log(av * 1e7) * ((npos - 1) / log(30.0 * 1e7)));
+    g = (v > 0.0) ? npos + m + 1 : npos - 1 - m;
+  }
+  return walk_right(ekc, n, v, g);
+}
+__device__ inline int coarse_bin_g(const double* __restrict__ egc, double v) {
+  constexpr int n = kXiGc + 1;
+  if (!(v == v)) return n;
+  const int g = (v >= 1e-9) ? (int)(log(v * 1e9) * ((n - 1) / log(30.0 * 1e9))) + 1 : 0;
+  return walk_right(egc, n, v, g);
+}
+__device__ inline int xi_ckey(const double* __restrict__ ekc, const double* __restrict__ egc,
+                              int shell, double kap, double g) {
+  const int a = coarse_bin_k(ekc, kap), b = coarse_bin_g(egc, g);
+  return (a >= 1 && a <= kXiKc && b >= 1 && b <= kXiGc)
+             ? shell * kXiBins + (a - 1) * kXiGc + (b - 1)
+             : kXiKeyInvalid;
+}
+
 // ------------------------------------------------ cells (count or emit) --
 // One workgroup per (row, orientation) task.  The row's edge list is merged
 // in LDS: [0, geomspace(x_in, x_ell, n_in)], the three truncation edges kept
@@ -301,6 +330,11 @@ __device__ inline int bin_right_g(const double* __restrict__ eg, double v) {
 // Modes of the cell kernel: count the cells of every (row, orientation) task,
 // emit them, or (subhalos) find per host row the largest radius of its
 // circular grid and monopole tail within reach of a clump.
+// With the xi_lin term (xe.ccounts / xe.pcid set) the count pass also counts
+// the task's lens radii and the emit pass writes each point's radius id and
+// coarse key: an orientation without roots of its own has the circular edge
+// list, so its inner cells share the radii of the first such orientation
+// (the reference groups a row's points by equal R, sgl_full.py:1397-1441).
 constexpr int kModeCount = 0, kModeEmit = 1, kModeXref = 2;
 
 template <int MODE>
@@ -316,7 +350,7 @@ __global__ __launch_bounds__(256, 2) void k_ref_cells(const double* __restrict__
                                                    double* __restrict__ W, int* __restrict__ key,
                                                    int* __restrict__ status,
                                                    double* __restrict__ subh,
-                                                   const double* __restrict__ subc) {
+                                                   const double* __restrict__ subc, XiEmit xe) {
   __shared__ double v[kMaxEdges];
   __shared__ double e[kMaxEdges];
   __shared__ int tmp[256];
@@ -327,7 +361,10 @@ __global__ __launch_bounds__(256, 2) void k_ref_cells(const double* __restrict__
   const int i = row / kRefMasses;
   const Cosmo c = reinterpret_cast<const Cosmo*>(cosmo)[0];
   if (i >= (int)c.n_z) {
-    if (MODE == kModeCount && t == 0) counts[task] = 0;
+    if (MODE == kModeCount && t == 0) {
+      counts[task] = 0;
+      if (xe.ccounts) xe.ccounts[task] = 0;
+    }
     return;
   }
   double* Hs = subh ? subh + (size_t)row * kHostStride : nullptr;
@@ -343,7 +380,10 @@ __global__ __launch_bounds__(256, 2) void k_ref_cells(const double* __restrict__
   if (ncand > kMaxEdges) {
     if (t == 0) {
       atomicOr(status, kStatusRefEdges);
-      if (MODE == kModeCount) counts[task] = 0;
+      if (MODE == kModeCount) {
+        counts[task] = 0;
+        if (xe.ccounts) xe.ccounts[task] = 0;
+      }
     }
     return;
   }
@@ -443,9 +483,20 @@ __global__ __launch_bounds__(256, 2) void k_ref_cells(const double* __restrict__
   const long long n_seg2 = (long long)ndo * nq;                // tail decorated
   const long long n_seg3 = n_out - ndo;                        // tail plain
   const long long n_all = n_seg0 + n_seg1 + n_seg2 + n_seg3 + n_fil;
+  // xi_lin radii: the first orientation without roots of its own carries the
+  // circular inner cells' radii, an orientation with roots its own
+  int q_can = 0;
+  if (ell) {
+    q_can = -1;
+    for (int qq = 0; qq < nph; ++qq)
+      if ((int)rt[8 + 9 * qq] == 0) { q_can = qq; break; }
+  }
+  const bool own_inner = (ne > 0) || (q == q_can);
+  const long long n_own = own_inner ? n_in_cells : 0;
   if (MODE == kModeCount) {
     if (t == 0) {
       counts[task] = n_all;
+      if (xe.ccounts) xe.ccounts[task] = n_own + n_out + (n_fil ? kFilU : 0);
       if (dec) {
         // the decorated cells of this task, for the point expansion
         int* ts = reinterpret_cast<int*>(subh + (size_t)kSubRows * kHostStride) + (size_t)task * 4;
@@ -467,16 +518,35 @@ __global__ __launch_bounds__(256, 2) void k_ref_cells(const double* __restrict__
   // expands it into the points (decorated cells lead their parts, so the
   // output position of item r follows from the segment sizes).
   const long long m_all = ndi + n_seg1 + ndo + n_seg3 + n_fil;
+  const bool xi = xe.pcid != nullptr;
+  long long cinner = 0, ctail = 0;
+  double a_r = 0.0, zp1 = 1.0, beta_h = 0.0;
+  if (xi) {
+    const long long cbase = xe.coffsets[task];
+    cinner = own_inner ? cbase : xe.coffsets[(long long)row * nph + q_can];
+    ctail = cbase + n_own;
+    zp1 = 1.0 + xe.slices[(size_t)i * kSliceStride + kSlZ];
+    a_r = p.rs * zp1;
+    beta_h = rows[(size_t)row * kRowStride + kRowBeta];
+  }
   for (long long it = t; it < m_all; it += 256) {
     double kap, gam, w;
     long long r = it, o;
     bool stash = false;
+    int cid = 0;
     if (r < ndi + n_seg1) {
       const int ci = (int)r;
       const double xlo = e[ci], xhi = e[ci + 1];
       const double xc = (xlo > 0.0) ? sqrt(xlo * xhi) : 0.5 * xhi;
       w = p.Nbar * (xhi * xhi - xlo * xlo) * iw;
       if (ell) w = w / kEllPhi;
+      if (xi) {
+        cid = (int)(cinner + ci);
+        if (own_inner) {
+          xe.cells[2 * (size_t)cid] = xc * a_r;
+          xe.cells[2 * (size_t)cid + 1] = beta_h;
+        }
+      }
       if (r < ndi) {
         o = base + r * nq;
         kap = xc;
@@ -491,6 +561,11 @@ __global__ __launch_bounds__(256, 2) void k_ref_cells(const double* __restrict__
       const double e0 = geom(p.x_ell, p.xm, n_out + 1, k), e1 = geom(p.x_ell, p.xm, n_out + 1, k + 1);
       const double xo = sqrt(e0 * e1);
       w = p.Nbar * (e1 * e1 - e0 * e0) * iw;
+      if (xi) {
+        cid = (int)(ctail + k);
+        xe.cells[2 * (size_t)cid] = xo * a_r;
+        xe.cells[2 * (size_t)cid + 1] = beta_h;
+      }
       const long long b2 = base + n_seg0 + n_seg1;
       if (r < ndo) {
         o = b2 + r * nq;
@@ -514,10 +589,19 @@ __global__ __launch_bounds__(256, 2) void k_ref_cells(const double* __restrict__
       kap = (uc <= 1.0) ? k0 * 2.0 * kPi / sq : 0.0;
       gam = k0 * 4.0 * kPi / (sq + 1.0) - kap;
       w = kPi * FR[2] * FR[1] * FR[1] * du2 * wphi;
+      if (xi) {
+        // the filament axis is not tangential: R < 0 marks m = 0 only
+        cid = (int)(ctail + n_out + l);
+        if (n == 0) {
+          xe.cells[2 * (size_t)cid] = -(uc * (FR[1] * zp1));
+          xe.cells[2 * (size_t)cid + 1] = FR[3];
+        }
+      }
     }
     if (stash) {
       K[o] = kap;
       W[o] = w;
+      if (xi) xe.pcid[o] = cid;
       continue;
     }
     const double gc = fmax(gam, 1.0000001 * 1e-9);   // np.clip(G, 1.0000001 * 1e-9, None)
@@ -528,6 +612,10 @@ __global__ __launch_bounds__(256, 2) void k_ref_cells(const double* __restrict__
     G[o] = (gam == gam) ? gc : gam;
     W[o] = w;
     key[o] = in ? (a - 1) * kFineGRef + (b - 1) : kKeyInvalid;
+    if (xi) {
+      xe.pcid[o] = cid;
+      xe.pck[o] = xi_ckey(xe.ekc, xe.egc, i, kap, (gam == gam) ? gc : gam);
+    }
   }
 }
 
@@ -541,8 +629,10 @@ __global__ __launch_bounds__(256, 2) void k_sub_emit(const double* __restrict__ 
                                                   const double* __restrict__ subh,
                                                   const double* __restrict__ subc,
                                                   double* __restrict__ K, double* __restrict__ G,
-                                                  double* __restrict__ W, int* __restrict__ key) {
+                                                  double* __restrict__ W, int* __restrict__ key,
+                                                  XiEmit xe) {
   __shared__ double sx[256], sw[256];
+  __shared__ int scid[256];
   const int nph = ell ? kEllPhi : 1;
   const int task = blockIdx.x, t = threadIdx.x;
   const int row = task / nph, q = task - row * nph;
@@ -568,6 +658,7 @@ __global__ __launch_bounds__(256, 2) void k_sub_emit(const double* __restrict__ 
         const long long o = b0 + (long long)(c0 + t) * nq;
         sx[t] = K[o];
         sw[t] = W[o];
+        if (xe.pcid) scid[t] = xe.pcid[o];
       }
       __syncthreads();
       for (int pq = t; pq < nc * nq; pq += 256) {
@@ -583,6 +674,10 @@ __global__ __launch_bounds__(256, 2) void k_sub_emit(const double* __restrict__ 
         G[o] = (gam == gam) ? gc : gam;
         W[o] = sw[cl] * wq;
         key[o] = in ? (a - 1) * kFineGRef + (b - 1) : kKeyInvalid;
+        if (xe.pcid) {
+          xe.pcid[o] = scid[cl];
+          xe.pck[o] = xi_ckey(xe.ekc, xe.egc, i, kap, (gam == gam) ? gc : gam);
+        }
       }
     }
   }
@@ -899,15 +994,20 @@ struct EpiArgs {
   const double* vb;    // kEpiP3: b[i]
   double cfac, gam_max;
   int n_gam;
+  long sA, sB, sC;     // batch strides (blockIdx.z), zero for a single product
 };
 
-template <int EPI>
+// TA: A is given transposed (stored K x M, leading dimension lda)
+template <int EPI, bool TA = false>
 __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const double* __restrict__ A,
                                                   long lda, const double* __restrict__ B, long ldb,
                                                   double* __restrict__ C, long ldc, EpiArgs ep) {
   constexpr int BM = 64, BN = 64, BK = 16, PAD = 2;
   __shared__ double As[BK][BM + PAD];
   __shared__ double Bs[BK][BN + PAD];
+  A += (size_t)blockIdx.z * ep.sA;
+  B += (size_t)blockIdx.z * ep.sB;
+  C += (size_t)blockIdx.z * ep.sC;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int wm = wv >> 1, wn = wv & 1;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
@@ -920,7 +1020,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const dou
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int gk = k0 + ak + u;
-      ra[u] = (gm < M && gk < K) ? A[(size_t)gm * lda + gk] : 0.0;
+      ra[u] = (gm < M && gk < K) ? (TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk])
+                                 : 0.0;
     }
     const int gk = k0 + bk;
 #pragma unroll
@@ -1274,33 +1375,38 @@ int ref_xref(const RefTablesView& T, int n_z_max, int ell, const double* roots, 
   hipLaunchKernelGGL(k_ref_cells<kModeXref>, dim3(n_z_max * kRefMasses), dim3(256), 0, st, T.cosmo,
                      T.rows, T.frows, roots, T.ek, T.eg, ell, 0, (long long*)nullptr,
                      (const long long*)nullptr, (double*)nullptr, (double*)nullptr,
-                     (double*)nullptr, (int*)nullptr, status, subh, (const double*)nullptr);
+                     (double*)nullptr, (int*)nullptr, status, subh, (const double*)nullptr,
+                     XiEmit{});
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
+// counts: [ntask + 1] points, with xi (xe.ccounts = counts + ntask + 1) also
+// [ntask + 1] radii; both scanned in place, totals last
 int ref_cells(const RefTablesView& T, int n_z_max, int ell, int fil, const double* roots,
-              long long* counts, int* status, double* subh, const double* subc, hipStream_t st) {
+              long long* counts, int* status, double* subh, const double* subc, const XiEmit& xe,
+              hipStream_t st) {
   const int nph = ell ? kEllPhi : 1;
   const int ntask = n_z_max * kRefMasses * nph;
   hipLaunchKernelGGL(k_ref_cells<kModeCount>, dim3(ntask), dim3(256), 0, st, T.cosmo, T.rows,
                      T.frows, roots, T.ek, T.eg, ell, fil, counts, (const long long*)nullptr,
                      (double*)nullptr, (double*)nullptr, (double*)nullptr, (int*)nullptr, status,
-                     subh, subc);
+                     subh, subc, xe);
   hipLaunchKernelGGL(k_excl_scan_i64, dim3(1), dim3(256), 0, st, counts, ntask);
+  if (xe.ccounts) hipLaunchKernelGGL(k_excl_scan_i64, dim3(1), dim3(256), 0, st, xe.ccounts, ntask);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
 int ref_emit(const RefTablesView& T, int n_z_max, int ell, int fil, const double* roots,
              const long long* offsets, double* K, double* G, double* W, int* key, int* status,
-             double* subh, const double* subc, hipStream_t st) {
+             double* subh, const double* subc, const XiEmit& xe, hipStream_t st) {
   const int nph = ell ? kEllPhi : 1;
   const int ntask = n_z_max * kRefMasses * nph;
   hipLaunchKernelGGL(k_ref_cells<kModeEmit>, dim3(ntask), dim3(256), 0, st, T.cosmo, T.rows,
                      T.frows, roots, T.ek, T.eg, ell, fil, (long long*)nullptr, offsets, K, G, W,
-                     key, status, subh, subc);
+                     key, status, subh, subc, xe);
   if (subh)
     hipLaunchKernelGGL(k_sub_emit, dim3(ntask), dim3(256), 0, st, T.cosmo, T.ek, T.eg, ell, offsets,
-                       subh, subc, K, G, W, key);
+                       subh, subc, K, G, W, key, xe);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
@@ -1404,6 +1510,25 @@ int ref_bin(long long n, const int* key, const double* K, const double* G, const
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
+int xi_gemm_batched(int M, int N, int K, const double* A, long lda, long sA, bool transA,
+                    const double* B, long ldb, long sB, double* C, long ldc, long sC, int batch,
+                    hipStream_t st) {
+  if (M < 1 || N < 1 || K < 1 || batch < 1 || (M + 63) / 64 > 65535 || batch > 65535)
+    return SGL_ERR_INVALID;
+  EpiArgs ep = {};
+  ep.sA = sA;
+  ep.sB = sB;
+  ep.sC = sC;
+  const dim3 grid((N + 63) / 64, (M + 63) / 64, batch);
+  if (transA)
+    hipLaunchKernelGGL((k_gemm_f64<kEpiStore, true>), grid, dim3(256), 0, st, M, N, K, A, lda, B,
+                       ldb, C, ldc, ep);
+  else
+    hipLaunchKernelGGL((k_gemm_f64<kEpiStore, false>), grid, dim3(256), 0, st, M, N, K, A, lda, B,
+                       ldb, C, ldc, ep);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
 template <int EPI>
 static void gemm_f64(int M, int N, int K, const double* A, long lda, const double* B, long ldb,
                      double* C, long ldc, const EpiArgs& ep, hipStream_t st) {
@@ -1437,6 +1562,10 @@ RefLayout ref_layout(int flags) {
   L.eg = L.ek + round32(kFineKRef + 1);
   L.total = L.eg + round32(kFineGRef + 1);
   return L;
+}
+RefTablesOffsets ref_tables_offsets(int flags) {
+  const RefLayout L = ref_layout(flags);
+  return RefTablesOffsets{L.T.cosmo, L.T.derived, L.T.slices, L.T.rows};
 }
 struct PhiWs {
   size_t E, Bm, a1p, a1, total;
@@ -1492,7 +1621,9 @@ hipStream_t ref_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 // subhalos only with the r200 edge (the reference pairs clump_edge with the
 // halo edge; its r_vir clumps are not built here)
 bool ref_flags_ok(int32_t f) {
-  if (f & ~(SGL_FLAG_ELL | SGL_FLAG_FILAMENTS | SGL_FLAG_EDGE_RVIR | SGL_FLAG_SUBHALOS)) return false;
+  if (f & ~(SGL_FLAG_ELL | SGL_FLAG_FILAMENTS | SGL_FLAG_EDGE_RVIR | SGL_FLAG_SUBHALOS |
+            SGL_FLAG_CLUSTERING))
+    return false;
   return !((f & SGL_FLAG_SUBHALOS) && (f & SGL_FLAG_EDGE_RVIR));
 }
 sgl_status ref_rc(int rc) { return (sgl_status)rc; }
@@ -1594,10 +1725,13 @@ sgl_status sgl_ref_cells_count(const void* tables, int32_t flags, const double* 
     return SGL_ERR_INVALID;
   const SubLayout S = sub_layout();
   double* D = const_cast<double*>(static_cast<const double*>(sub));
+  XiEmit xe{};
+  if (flags & SGL_FLAG_CLUSTERING)
+    xe.ccounts = reinterpret_cast<long long*>(counts) + sgl_ref_tasks(flags) + 1;
   return ref_rc(ref_cells(ref_view(tables, flags), kRefMaxShells, (flags & SGL_FLAG_ELL) ? 1 : 0,
                           (flags & SGL_FLAG_FILAMENTS) ? 1 : 0, roots,
                           reinterpret_cast<long long*>(counts), status, D ? D + S.host : nullptr,
-                          D ? D + S.coarse : nullptr, ref_stream(stream)));
+                          D ? D + S.coarse : nullptr, xe, ref_stream(stream)));
 }
 
 sgl_status sgl_ref_cells_emit(const void* tables, int32_t flags, const double* roots,
@@ -1611,7 +1745,37 @@ sgl_status sgl_ref_cells_emit(const void* tables, int32_t flags, const double* r
   return ref_rc(ref_emit(ref_view(tables, flags), kRefMaxShells, (flags & SGL_FLAG_ELL) ? 1 : 0,
                          (flags & SGL_FLAG_FILAMENTS) ? 1 : 0, roots,
                          reinterpret_cast<const long long*>(offsets), kap, gam, w, key, status,
-                         D ? D + S.host : nullptr, D ? D + S.coarse : nullptr,
+                         D ? D + S.host : nullptr, D ? D + S.coarse : nullptr, XiEmit{},
+                         ref_stream(stream)));
+}
+
+sgl_status sgl_ref_cells_emit_xi(const void* tables, int32_t flags, const double* roots,
+                                 const void* sub, const int64_t* offsets, int64_t n_points,
+                                 double* kap, double* gam, double* w, int32_t* key,
+                                 const void* xi_state, int32_t* xi_pts, double* xi_cells,
+                                 int32_t* status, void* stream) {
+  if (!ref_flags_ok(flags) || !(flags & SGL_FLAG_CLUSTERING) || !tables || !roots || !offsets ||
+      n_points < 1 || n_points > ((int64_t)1 << 31) - 1 || !kap || !gam || !w || !key ||
+      !xi_state || !xi_pts || !xi_cells || !status || !sub_ok(flags, sub))
+    return SGL_ERR_INVALID;
+  const SubLayout S = sub_layout();
+  double* D = const_cast<double*>(static_cast<const double*>(sub));
+  const int ntask = sgl_ref_tasks(flags);
+  const long long* off = reinterpret_cast<const long long*>(offsets);
+  const RefLayout RL = ref_layout(flags);
+  const XiState X = xi_state_layout();
+  const double* xs = static_cast<const double*>(xi_state);
+  XiEmit xe{};
+  xe.coffsets = off + ntask + 1;
+  xe.ekc = xs + X.ekc;
+  xe.egc = xs + X.egc;
+  xe.slices = static_cast<const double*>(tables) + RL.T.slices;
+  xe.pcid = xi_pts;
+  xe.pck = xi_pts + n_points;
+  xe.cells = xi_cells;
+  return ref_rc(ref_emit(ref_view(tables, flags), kRefMaxShells, (flags & SGL_FLAG_ELL) ? 1 : 0,
+                         (flags & SGL_FLAG_FILAMENTS) ? 1 : 0, roots, off, kap, gam, w, key,
+                         status, D ? D + S.host : nullptr, D ? D + S.coarse : nullptr, xe,
                          ref_stream(stream)));
 }
 

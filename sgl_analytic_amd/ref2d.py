@@ -35,7 +35,7 @@ PHI_TOL = 1e-8
 N_XI_WIDE = 500
 OUT_GRID_MAX = 3.4e7             # sgl_full.py:60 (geometric output axis above it)
 E_BLOCK_BYTES = 4 << 30          # Lambda operand rows per call (k_kappa row blocks)
-INGREDIENTS_2D = ("ell", "fil", "sub")  # ingredient sets this chain computes
+INGREDIENTS_2D = ("ell", "fil", "sub", "bias")  # ingredient sets this chain computes
 
 
 def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
@@ -57,10 +57,27 @@ def _records(cosmo: dict, zs: float, n_z: int) -> np.ndarray:
 
 
 @dataclass
+class XiTerm:
+    """The xi_lin two-point term on the device (sgl_ref_xi_*): the state (k_perp
+    rule, first moments, rotated per-shell tables, spline), the per-run rows
+    that the coarse grid reads, and their sizes."""
+    tables: torch.Tensor
+    flags: int
+    state: torch.Tensor
+    rows: torch.Tensor
+    n_runs: int
+    n_parts: int
+    n_z: int
+    nP: int
+    nQ: int
+    var: float
+
+
+@dataclass
 class Population:
     """Occupied fine bins on the device ([w][kappa centroid][gamma centroid],
     REF_FINE_BINS each) and the builder's bookkeeping (sgl_full.py:1535-1808,
-    2742-2755)."""
+    2742-2755); xi: the clustering term when built with bias."""
     pop: torch.Tensor
     npop_dev: torch.Tensor
     n_bins: int
@@ -70,6 +87,7 @@ class Population:
     N_expl: float
     N_fil: float
     timings: dict = field(default_factory=dict)
+    xi: XiTerm = None
 
 
 class _Stages:
@@ -93,15 +111,25 @@ class _Stages:
         self.t = t
 
 
+def _xi_info(L, state: torch.Tensor) -> np.ndarray:
+    out = torch.empty(8, dtype=torch.float64, device=state.device)
+    check(L.sgl_ref_xi_info(_ptr(state), _ptr(out), _stream()), "sgl_ref_xi_info")
+    return out.cpu().numpy()
+
+
 def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
           edge: str = "r200", timings: dict = None, sub: bool = False,
-          keep_sub: bool = False) -> Population:
+          keep_sub: bool = False, bias: bool = False) -> Population:
     """build_population(..., kap_floor=1e-8, trunc=edge, zint="midpoint")
     + Lam's flattening (sgl_full.py:1535, 2742-2755), on the device.  edge
     "rvir" truncates every halo at its virial radius (sgl_full.py:1876-1880).
     sub: subhalos on their hosts (the exact decorated-host sector with
     clumps truncated at r200, the pipeline's default for edge "r200";
-    pipeline.py:171-183); keep_sub keeps the subhalo tables on the result."""
+    pipeline.py:171-183); keep_sub keeps the subhalo tables on the result.
+    bias: the xi_lin clustering term (clustering="xilin", the pipeline's
+    default for the "bias" ingredient, pipeline.py:135-207): every point's
+    lens distance deposited per shell and coarse bin, the rows compressed,
+    and sigma_kappa with the clustering variance (Lam.sigma)."""
     L = _lib.load()
     dev = torch.device("cuda")
     if edge not in ("r200", "rvir"):
@@ -110,7 +138,8 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
         raise NotImplementedError("subhalos are built with the r200 edge only (clump_edge r200)")
     flags = ((FLAG_ELL if ell else 0) | (FLAG_FILAMENTS if fil else 0)
              | (_lib.FLAG_EDGE_RVIR if edge == "rvir" else 0)
-             | (_lib.FLAG_SUBHALOS if sub else 0))
+             | (_lib.FLAG_SUBHALOS if sub else 0)
+             | (_lib.FLAG_CLUSTERING if bias else 0))
     n_z = int(L.sgl_ref_shell_count(float(zs)))
     if n_z < 1 or n_z > _lib.REF_MAX_SHELLS:
         raise ValueError("z_s outside the reference grid's shells")
@@ -131,10 +160,11 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
         clock.mark("subhalos")
     sp = _ptr(subw) if subw is not None else None
     ntask = int(L.sgl_ref_tasks(flags))
-    counts = torch.empty(ntask + 1, dtype=torch.int64, device=dev)
+    counts = torch.empty((2 if bias else 1) * (ntask + 1), dtype=torch.int64, device=dev)
     check(L.sgl_ref_cells_count(_ptr(T), flags, _ptr(roots), sp, _ptr(counts), _ptr(status), st),
           "sgl_ref_cells_count")
-    n = int(counts[ntask].item())
+    tot = counts[[ntask, 2 * ntask + 1] if bias else [ntask]].cpu().numpy()
+    n = int(tot[0])
     clock.mark("cells_count")
     if n < 1:
         raise _lib.SglError(f"reference population is empty (status {int(status.item())})")
@@ -142,8 +172,19 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
     G = torch.empty_like(K)
     W = torch.empty_like(K)
     key = torch.empty(n, dtype=torch.int32, device=dev)
-    check(L.sgl_ref_cells_emit(_ptr(T), flags, _ptr(roots), sp, _ptr(counts), _ptr(K), _ptr(G),
-                               _ptr(W), _ptr(key), _ptr(status), st), "sgl_ref_cells_emit")
+    if bias:
+        n_cells = int(tot[1])
+        xs = torch.empty(L.sgl_ref_xi_state_bytes() // 8, dtype=torch.float64, device=dev)
+        check(L.sgl_ref_xi_init(_ptr(T), flags, _ptr(xs), st), "sgl_ref_xi_init")
+        pts = torch.empty(2 * n, dtype=torch.int32, device=dev)
+        cells = torch.empty(2 * n_cells, dtype=torch.float64, device=dev)
+        check(L.sgl_ref_cells_emit_xi(_ptr(T), flags, _ptr(roots), sp, _ptr(counts), n, _ptr(K),
+                                      _ptr(G), _ptr(W), _ptr(key), _ptr(xs), _ptr(pts),
+                                      _ptr(cells), _ptr(status), st), "sgl_ref_cells_emit_xi")
+    else:
+        check(L.sgl_ref_cells_emit(_ptr(T), flags, _ptr(roots), sp, _ptr(counts), _ptr(K),
+                                   _ptr(G), _ptr(W), _ptr(key), _ptr(status), st),
+              "sgl_ref_cells_emit")
     clock.mark("cells_emit")
     ws = torch.empty(L.sgl_ref_bin_bytes(n), dtype=torch.uint8, device=dev)
     pop = torch.empty(3 * REF_FINE_BINS, dtype=torch.float64, device=dev)
@@ -156,13 +197,32 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
     if s:
         raise _lib.SglError(f"reference population status {s} (16: critical-curve roots beyond the "
                             "slots, 32: edges beyond the per-row capacity, 2: bad record)")
-    del K, G, W, key, ws
+    del ws
+    xi = None
+    if bias:
+        sws = torch.empty(L.sgl_ref_xi_sort_bytes(n, n_cells), dtype=torch.uint8, device=dev)
+        check(L.sgl_ref_xi_sort(_ptr(T), flags, n, _ptr(pts), n_cells, _ptr(cells), _ptr(sws),
+                                _ptr(xs), st), "sgl_ref_xi_sort")
+        info = _xi_info(L, xs)
+        n_runs, n_parts = int(info[_lib.XI_INFO_RUNS]), int(info[_lib.XI_INFO_PARTS])
+        rws = torch.empty(L.sgl_ref_xi_rows_bytes(n_runs, n_parts), dtype=torch.uint8, device=dev)
+        check(L.sgl_ref_xi_rows(_ptr(T), flags, n, _ptr(K), _ptr(G), _ptr(W), _ptr(pts), n_cells,
+                                _ptr(cells), _ptr(sws), n_runs, n_parts, _ptr(rws), _ptr(xs), st),
+              "sgl_ref_xi_rows")
+        info = _xi_info(L, xs)
+        del sws, pts, cells
+        xi = XiTerm(tables=T, flags=flags, state=xs, rows=rws, n_runs=n_runs, n_parts=n_parts,
+                    n_z=int(info[_lib.XI_INFO_NZ]), nP=int(info[_lib.XI_INFO_NP]),
+                    nQ=int(info[_lib.XI_INFO_NQ]), var=float(info[_lib.XI_INFO_VAR]))
+        clock.mark("xi_rows")
+    del K, G, W, key
     sc = scal.cpu().numpy()
     # bookkeeping of the info record (sgl_full.py:2039, 1992): expected counts
     lay_rows = _row_table(T, flags, n_z)
-    out = Population(pop=pop, npop_dev=npop, n_bins=int(npop.item()), sigma=math.sqrt(sc[0]),
+    var = sc[0] + (xi.var if xi is not None else 0.0)
+    out = Population(pop=pop, npop_dev=npop, n_bins=int(npop.item()), sigma=math.sqrt(var),
                      kbar=float(sc[1]), n_points=n, N_expl=lay_rows[0],
-                     N_fil=lay_rows[1] if fil else 0.0)
+                     N_fil=lay_rows[1] if fil else 0.0, xi=xi)
     if keep_sub:
         out.sub_tables = subw
     return out
@@ -197,6 +257,10 @@ def lam_axis(pop: Population, k: np.ndarray, axis: int) -> np.ndarray:
     out = torch.empty(2 * kd.numel(), dtype=torch.float64, device="cuda")
     check(L.sgl_ref_lam_axis(_ptr(pop.pop), _ptr(pop.npop_dev), _ptr(kd), kd.numel(), axis,
                              _ptr(out), _stream()), "sgl_ref_lam_axis")
+    if pop.xi is not None:
+        x = pop.xi
+        check(L.sgl_ref_xi_axis(_ptr(x.tables), x.flags, _ptr(x.state), _ptr(kd), kd.numel(), axis,
+                                _ptr(out), _stream()), "sgl_ref_xi_axis")
     o = out.cpu().numpy()
     return o[0::2] + 1j * o[1::2]
 
@@ -268,14 +332,33 @@ def phi_grid(pop: Population, k_k: np.ndarray, k_g: np.ndarray) -> torch.Tensor:
     kg = torch.tensor(k_g, dtype=torch.float64, device=dev)
     lda = (pop.n_bins + 63) // 64 * 64
     blk = max(1, min(nk, 65535, E_BLOCK_BYTES // (16 * lda)))
+    psi = xi_psi(pop, k_k, k_g) if pop.xi is not None else None
     ws = torch.empty(L.sgl_ref_phi_bytes(blk, ng, pop.n_bins) // 8, dtype=torch.float64, device=dev)
     for lo in range(0, nk, blk):
         hi = min(nk, lo + blk)
         kk = torch.tensor(k_k[lo:hi], dtype=torch.float64, device=dev)
         check(L.sgl_ref_phi(_ptr(pop.pop), _ptr(pop.npop_dev), pop.n_bins, _ptr(kk), hi - lo,
-                            _ptr(kg), ng, None, _ptr(ws), _ptr(phi[2 * lo:]), _stream()),
+                            _ptr(kg), ng, _ptr(psi[lo]) if psi is not None else None, _ptr(ws),
+                            _ptr(phi[2 * lo:]), _stream()),
               "sgl_ref_phi")
     return phi
+
+
+def xi_psi(pop: Population, k_k: np.ndarray, k_g: np.ndarray) -> torch.Tensor:
+    """The xi_lin term Psi_2 on the tensor grid k_k x k_g (linspaces from 0),
+    [nk][ng][2]: the 101 x 101 coarse grid and its bicubic spline
+    (Lam._Q_interpolated with fn=_Qxi, sgl_full.py:2953-3022, 3114-3128)."""
+    L = _lib.load()
+    x = pop.xi
+    nk, ng = k_k.size, k_g.size
+    gws = torch.empty(L.sgl_ref_xi_grid_bytes(x.n_z, x.nP, x.nQ), dtype=torch.uint8,
+                      device=x.state.device)
+    psi = torch.empty((nk, ng, 2), dtype=torch.float64, device=x.state.device)
+    check(L.sgl_ref_xi_grid(_ptr(x.tables), x.flags, _ptr(x.state), _ptr(x.rows), x.n_runs, x.n_z,
+                            x.nP, x.nQ, _ptr(gws), float(k_k[-1]), float(k_g[-1]), nk, ng,
+                            _ptr(psi), _stream()), "sgl_ref_xi_grid")
+    del gws
+    return psi
 
 
 @dataclass
@@ -332,10 +415,11 @@ def invert(pop: Population, xi_out: np.ndarray, tol: float = PHI_TOL,
 
 
 def run(zs: float, cosmo: dict, ell: bool = False, fil: bool = False, xi_out=None,
-        edge: str = "r200", timings: dict = None, sub: bool = False) -> Result:
+        edge: str = "r200", timings: dict = None, sub: bool = False,
+        bias: bool = False) -> Result:
     """One PDF on the reference's chain; xi_out None: the CLI body and tail
     grids linspace(-1, 1, 1601) and linspace(1.005, 9, 1600) (pipeline.py:211-215)."""
-    pop = build(cosmo, zs, ell=ell, fil=fil, edge=edge, timings=timings, sub=sub)
+    pop = build(cosmo, zs, ell=ell, fil=fil, edge=edge, timings=timings, sub=sub, bias=bias)
     if xi_out is None:
         xi_out = np.concatenate([np.linspace(-1.0, 1.0, 1601), np.linspace(1.005, 9.0, 1600)])
     return invert(pop, xi_out, timings=timings)
