@@ -387,6 +387,93 @@ sgl_status sgl_ref_cells_emit(const void* tables, int32_t flags, const double* r
                               double* w, int32_t* key, int32_t* status, void* stream);
 
 /*
+ * The xi_lin (Limber) two-point term, the "bias" ingredient with
+ * clustering="xilin" and the Gaussian closure (pipeline.py:135-207).  flags
+ * must hold SGL_FLAG_CLUSTERING on every call below and on sgl_ref_cells_count,
+ * whose counts then has 2 (tasks + 1) slots: the second half holds the
+ * exclusive offsets of the lens cells (distinct lens radii per task) and
+ * counts[2 tasks + 1] their total.  Order: sgl_ref_xi_init -> count ->
+ * sgl_ref_cells_emit_xi -> sgl_ref_bin -> sgl_ref_xi_sort -> sgl_ref_xi_rows ->
+ * sgl_ref_xi_axis (reach probes) / sgl_ref_xi_grid (psi of sgl_ref_phi).
+ * xi_state (sgl_ref_xi_state_bytes) is the per-PDF state: the k_perp rule, the
+ * coarse lattice, the first moments, the compressed rows of every (shell, m)
+ * group and the coarse-grid Psi_2 with its spline coefficients.
+ */
+size_t sgl_ref_xi_state_bytes(void);
+/* k_perp rule (42 Gauss-Legendre nodes in ln k, weights k^2 P0(k) w / 2 pi) and
+ * the coarse (kappa, gamma) edges.  Replaces: xi_kperp_grid
+ * (sgl_full.py:2483-2511) and the coarse edges of Cells (sgl_full.py:457-468). */
+sgl_status sgl_ref_xi_init(const void* tables, int32_t flags, void* xi_state, void* stream);
+/*
+ * sgl_ref_cells_emit plus, per point, its lens-cell id and coarse key (shell *
+ * 341 * 110 + coarse bin, or past every key outside the lattice) in xi_pts
+ * (int32[2 n_points]: ids, then keys) and per lens cell (R [Mpc], beta) in
+ * xi_cells (double[2 n_cells]; R < 0 for points without tangential shear).
+ * offsets: the counts of sgl_ref_cells_count (both halves).
+ * Replaces: the point-to-radius bookkeeping of Cells._xi_deposit
+ * (sgl_full.py:1397-1441) done while the cells are emitted.
+ */
+sgl_status sgl_ref_cells_emit_xi(const void* tables, int32_t flags, const double* roots,
+                                 const void* sub, const int64_t* offsets, int64_t n_points,
+                                 double* kap, double* gam, double* w, int32_t* key,
+                                 const void* xi_state, int32_t* xi_pts, double* xi_cells,
+                                 int32_t* status, void* stream);
+/* out (device, 8 doubles) = [runs, parts, shells, kappa bins nP, gamma bins nQ,
+ * clustering variance, largest singular value, 0]: the sizes the host needs
+ * between sgl_ref_xi_sort, sgl_ref_xi_rows and sgl_ref_xi_grid. */
+sgl_status sgl_ref_xi_info(const void* xi_state, double* out, void* stream);
+/*
+ * Stable sort of the points by coarse key (emission order kept inside a key),
+ * runs of one key, their parts of at most 2048 points, and the J_{2m}(k_j R)
+ * table of every lens cell.  ws: sgl_ref_xi_sort_bytes(n_points, n_cells),
+ * read again by sgl_ref_xi_rows.
+ * Replaces: the grouping of _xi_add_grouped and _bessel_pack_nb
+ * (sgl_full.py:1080-1184).
+ */
+size_t sgl_ref_xi_sort_bytes(int64_t n_points, int64_t n_cells);
+sgl_status sgl_ref_xi_sort(const void* tables, int32_t flags, int64_t n_points,
+                           const int32_t* xi_pts, int64_t n_cells, const double* xi_cells,
+                           void* ws, void* xi_state, void* stream);
+/*
+ * Deposit and compression: H = sum beta w J_{2m}(k_j R) per (shell, coarse
+ * bin, m, j) summed run by run, the exact first moments and the halo rows'
+ * analytic shear tail, then per (shell, m) group the eigenvectors of A^T A
+ * (cyclic Jacobi in LDS) with the reference's singular-value cut and global
+ * prune, the rotated axis tables and the clustering variance.
+ * rows_ws: sgl_ref_xi_rows_bytes(n_runs, n_parts) (from sgl_ref_xi_info),
+ * kept for sgl_ref_xi_grid.
+ * Replaces: Cells._xi_deposit, xi_add_shear_tail and xi_end
+ * (sgl_full.py:1397-1523), the prune of Lam (sgl_full.py:2801-2812) and the
+ * clustering part of Lam.sigma (sgl_full.py:3437-3446).
+ */
+size_t sgl_ref_xi_rows_bytes(int64_t n_runs, int64_t n_parts);
+sgl_status sgl_ref_xi_rows(const void* tables, int32_t flags, int64_t n_points, const double* kap,
+                           const double* gam, const double* w, const int32_t* xi_pts,
+                           int64_t n_cells, const double* xi_cells, const void* sort_ws,
+                           int64_t n_runs, int64_t n_parts, void* rows_ws, void* xi_state,
+                           void* stream);
+/* Psi_2 on an axis at n probe points k (out: [n][2], complex): axis 0
+ * Psi_2(k, 0), axis 1 Psi_2(0, k).  Replaces: _Qxi_gauss on the reach probes
+ * (sgl_full.py:2869-2951, 38-47). */
+sgl_status sgl_ref_xi_axis(const void* tables, int32_t flags, const void* xi_state,
+                           const double* k, int32_t n, int32_t axis, double* out, void* stream);
+/*
+ * Psi_2 on the tensor grid linspace(0, kk_max, nk) x linspace(0, kg_max, ng)
+ * (psi: complex [nk][ng], the psi of sgl_ref_phi): on the 101 x 101 coarse
+ * grid k = k_max s^2 as batched FP64 MFMA contractions (gamma bins x J_m,
+ * kappa bins x [cos; sin]), then the not-a-knot bicubic spline in
+ * s = sqrt(k / k_max).  n_z, nP, nQ: from sgl_ref_xi_info;
+ * grid_ws: sgl_ref_xi_grid_bytes(n_z, nP, nQ).
+ * Replaces: _Qxi_gauss and Lam._Q_interpolated (sgl_full.py:2869-3022,
+ * 3114-3128).
+ */
+size_t sgl_ref_xi_grid_bytes(int32_t n_z, int32_t nP, int32_t nQ);
+sgl_status sgl_ref_xi_grid(const void* tables, int32_t flags, const void* xi_state,
+                           const void* rows_ws, int64_t n_runs, int32_t n_z, int32_t nP,
+                           int32_t nQ, void* grid_ws, double kk_max, double kg_max, int32_t nk,
+                           int32_t ng, double* psi, void* stream);
+
+/*
  * Binning: stable sort by key, per-bin sums of w, w kappa, w gamma in the cells'
  * order, the occupied bins compacted in bin order into pop = [w][kappa
  * centroid][gamma centroid] (SGL_REF_FINE_BINS doubles each), npop = their

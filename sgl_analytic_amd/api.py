@@ -13,10 +13,11 @@ gamma); the constant-mu arc integral and the normalisation
 (sgl_analytic_amd/ref2d.py over the sgl_ref_* kernels).  Ingredients built
 on this chain: spherical truncated NFW halos, "ell" (pseudo-elliptical
 halos), "fil" (filaments) and "sub" (subhalos on their hosts, the exact
-decorated-host sector; with edge "r200" only); edge "r200" (default) or
-"rvir".  The ingredient "bias" (the xi_lin two-point term), hence the
-default config="full", and the positive "vp" closure raise
-NotImplementedError: they are not built on this chain.
+decorated-host sector; with edge "r200" only) and "bias" (the xi_lin
+Limber two-point term, clustering="xilin", the pipeline's default); edge
+"r200" (default) or "rvir".  The default config="full" therefore runs every
+ingredient, as the reference does.  The positive "vp" closure raises
+NotImplementedError: it is not built on this chain.
 
 ``run_kernel_shape`` is the host driver of the kernel-shape chain (64
 slices uniform in comoving distance x 256 masses x 512 impact parameters,
@@ -37,7 +38,7 @@ P = dict(h=0.674, Om=0.315, s8=0.811, Ob=0.0493, ns=0.965, zeq=3402.0)  # pipeli
 EXTENSIONS = dict(Ok=0.0, w0=-1.0, wa=0.0, growth_mode="ode", radiation="engine")
 INGREDIENTS = ("ell", "fil", "sub", "bias")
 MODELLED = {"bias", "fil", "ell"}   # ingredients of the kernel-shape chain (run_kernel_shape)
-MODELLED_2D = {"fil", "ell", "sub"}  # ingredients of the reference chain (run)
+MODELLED_2D = {"fil", "ell", "sub", "bias"}  # ingredients of the reference chain (run)
 # the subhalo model the chain builds: the reference's defaults for edge "r200"
 # (pipeline.py:171-175, subhalos.py:541-545), as its run record states them
 SUB_KW = dict(Mhost_min=1e9, npsi=28, nd=90, nth=48, split=10.0, ntheta=4, nr=40, prune=0.0,
@@ -164,12 +165,6 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200
     cp = check_cosmology(cosmo)
     if closure != "gauss" and "bias" not in s:
         raise ValueError(f"xi_closure={closure!r} needs xilin clustering and the bias ingredient")
-    missing = s - MODELLED_2D
-    if missing:
-        raise NotImplementedError(
-            f"ingredient(s) {sorted(missing)} of config {config!r} are not built on the "
-            "reference chain; built: halos with 'ell' and 'fil' (e.g. config='halo', '+ell', "
-            "'+fil' or {'ell', 'fil'})")
     if closure != "gauss":
         raise NotImplementedError("the positive 'vp' closure is not built on the reference chain")
     if "sub" in s and edge != "r200":
@@ -177,7 +172,8 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200
                                   "their r200, the reference's pairing for that edge)")
     from . import ref2d
     t0 = time.perf_counter()
-    pop = ref2d.build(cp, float(z_s), ell="ell" in s, fil="fil" in s, edge=edge, sub="sub" in s)
+    pop = ref2d.build(cp, float(z_s), ell="ell" in s, fil="fil" in s, edge=edge, sub="sub" in s,
+                      bias="bias" in s)
     t1 = time.perf_counter()
     x = np.asarray(xi_out, dtype=np.float64)
     res = ref2d.invert(pop, x)

@@ -59,11 +59,18 @@ constexpr int kMstarN = 1200;
 // ln M*(z) (sgl_full.py:551-567): sigma(M*, 0) Dg(z) = delta_c0 by linear
 // interpolation of ln M against sigma, clamped to the table's ends
 // (np.interp).  The reference first cuts its table to the largest strictly
-// decreasing suffix, against its spline's extrapolation; the sums here
-// decrease strictly over the whole table, so the cut keeps all of it.
+// decreasing suffix, against its spline's extrapolation below 1e4 Msun: the
+// reference-grid table (k_ref_mstar_table) holds that spline with the entries
+// before the suffix flat at its first value; the kernel-shape table holds the
+// window sums themselves, strictly decreasing over the whole table.
 __device__ inline double ln_mstar(const double* __restrict__ tab, double target) {
   constexpr double kLnStep = 17.0 * 2.302585092994046 / (kMstarN - 1);
-  if (target >= tab[0]) return 0.0;
+  if (target >= tab[0]) {
+    // a table whose head is flat (k_ref_mstar_table) clamps to the end of that plateau
+    int j = 0;
+    while (j < kMstarN - 1 && tab[j + 1] == tab[0]) ++j;
+    return kLnStep * j;
+  }
   if (target <= tab[kMstarN - 1]) return kLnStep * (kMstarN - 1);
   int lo = 0, hi = kMstarN - 1;           // tab[lo] > target >= tab[hi]
   while (hi - lo > 1) {
@@ -98,7 +105,9 @@ inline TablesLayout tables_layout(const sgl_shape& s, int B) {
   L.cov = take((size_t)B * s.n_z_max * s.n_z_max);
   L.sigtab = take((s.flags & SGL_FLAG_ELL) ? (size_t)B * kMstarN : 0);
   const bool sub = (s.flags & SGL_FLAG_SUBHALOS) != 0;
-  L.sigspl = take(sub ? (size_t)B * kSigSplDoubles : 0);
+  // the reference's sigma spline: subhalos, and M* of the reference grid's ellipticity
+  const bool spl = sub || ((s.flags & SGL_FLAG_REF_GRID) && (s.flags & SGL_FLAG_ELL));
+  L.sigspl = take(spl ? (size_t)B * kSigSplDoubles : 0);
   L.gdense = take(sub ? (size_t)B * (kGrowthSteps + 1) * 2 : 0);
   L.total = o;
   return L;

@@ -13,6 +13,7 @@
 #include <cmath>
 
 #include "sgl_amd_internal.h"
+#include "ref_sub.h"
 
 namespace sgl {
 
@@ -535,9 +536,9 @@ __global__ __launch_bounds__(256) void k_sigma(const Cosmo* __restrict__ cosmo,
 
 // sigma(M, 0) on M = logspace(0, 17, kMstarN) Msun for the M*(z) root of the
 // halo ellipticity (SGL_FLAG_ELL; sgl_full.py:551-567): the same smooth-k
-// window sums as k_sigma on the same 4000 k nodes.  (The reference tabulates
-// its 500-node sigma spline there and extrapolates it below 1e4 Msun; these
-// are the sums themselves, monotone everywhere.)  Runs after k_sigma (A_norm).
+// window sums as k_sigma on the same 4000 k nodes (the kernel-shape tables;
+// the reference grid uses the reference's own spline, k_ref_mstar_table).
+// Runs after k_sigma (A_norm).
 __global__ __launch_bounds__(256) void k_mstar_table(const Cosmo* __restrict__ cosmo,
                                                      const Derived* __restrict__ der,
                                                      double* __restrict__ sigtab) {
@@ -650,6 +651,36 @@ __global__ void k_sigspl_solve(double* __restrict__ spl, int B) {
   }
   s[n - 1] = rp;
   for (int i = n - 2; i >= 0; --i) s[i] -= cp[i] * s[i + 1];
+}
+
+// sigma(M, 0) on M = logspace(0, 17, kMstarN) Msun from the reference's
+// spline (the reference-grid tables): Cosmology.Mstar tabulates sigmaM, the
+// not-a-knot spline on 1e4..1e17 Msun extrapolated below 1e4 by its first
+// piece, and keeps the largest suffix on which it strictly decreases
+// (sgl_full.py:551-567).  The entries before that suffix are set to its first
+// value, so ln_mstar's plateau rule returns the suffix's first mass for any
+// target at or above it, as np.interp clamps to its first node.
+__global__ __launch_bounds__(256) void k_ref_mstar_table(const double* __restrict__ sigspl,
+                                                         double* __restrict__ sigtab) {
+  __shared__ int last_up;
+  const int t = threadIdx.x;
+  const double* spl = sigspl + (size_t)blockIdx.x * kSigSplDoubles;
+  double* tab = sigtab + (size_t)blockIdx.x * kMstarN;
+  const double kLn10 = 2.302585092994046;
+  if (t == 0) last_up = -1;
+  for (int q = t; q < kMstarN; q += blockDim.x) {
+    const double lnM = (q == 0) ? 0.0 : (q == kMstarN - 1) ? 17.0 * kLn10
+                                                         : kLn10 * (17.0 * q / (kMstarN - 1));
+    tab[q] = sigma_spl(spl, lnM);
+  }
+  __syncthreads();
+  for (int q = t; q < kMstarN - 1; q += blockDim.x)
+    if (tab[q + 1] >= tab[q]) atomicMax(&last_up, q);
+  __syncthreads();
+  const int i0 = last_up + 1;
+  const double top = tab[i0];
+  __syncthreads();
+  for (int q = t; q < i0; q += blockDim.x) tab[q] = top;
 }
 
 // ------------------------------------------------------------------ rows --
@@ -1073,11 +1104,12 @@ int ref_tables_build(const double* cosmo, int B, const sgl_shape& sh, const doub
   Derived* D = reinterpret_cast<Derived*>(derived);
   const double dlnM = (log(1e17) - log(1e7)) / 99.0;
   const bool sub = (sh.flags & SGL_FLAG_SUBHALOS) != 0;
+  const bool ell = (sh.flags & SGL_FLAG_ELL) != 0;
   hipLaunchKernelGGL(k_ref_background, dim3(B), dim3(kBgThreads), 0, st, C, sh.n_z_max, D, slices);
   hipLaunchKernelGGL(k_growth, dim3(B), dim3(64), 0, st, C, sh.n_z_max, D, slices,
                      sub ? gdense : (double*)nullptr);
   hipLaunchKernelGGL(k_sigma, dim3(B), dim3(256), 0, st, C, m_nodes, sh.n_m, 1, D, mass);
-  if (sub) {
+  if (sub || ell) {
     // spline knots into the x[] slot of instance 0 (overwritten by the solve)
     double* knots = sigspl + kSigSplN * kMassStride;
     hipLaunchKernelGGL(k_sigspl_nodes, dim3((kSigSplN + 255) / 256), dim3(256), 0, st, knots);
@@ -1089,8 +1121,7 @@ int ref_tables_build(const double* cosmo, int B, const sgl_shape& sh, const doub
     }
     hipLaunchKernelGGL(k_sigspl_solve, dim3((B + 63) / 64), dim3(64), 0, st, sigspl, B);
   }
-  const bool ell = (sh.flags & SGL_FLAG_ELL) != 0;
-  if (ell) hipLaunchKernelGGL(k_mstar_table, dim3(B), dim3(256), 0, st, C, D, sigtab);
+  if (ell) hipLaunchKernelGGL(k_ref_mstar_table, dim3(B), dim3(256), 0, st, sigspl, sigtab);
   const double a_st = 1.0 / (1.0 + std::pow(2.0, -kPst) * std::tgamma(0.5 - kPst) / std::sqrt(kPi));
   const double dc0 = 0.6 * std::pow(1.5 * kPi, 2.0 / 3.0);
   hipLaunchKernelGGL(k_rows, dim3(sh.n_z_max, B), dim3(128), 0, st, C, D, slices, mass,

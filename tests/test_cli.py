@@ -49,6 +49,6 @@ def test_cli_rejects_bad_input(tmp_path):
             cli.main(["--zs", "1", "--Om", "-1", "--out", str(tmp_path / "x.json")])
         with pytest.raises(SystemExit):
             cli.main(["--zs", "1", "--nproc", "0", "--out", str(tmp_path / "x.json")])
-    # ingredients not built on the reference chain: a usage error, not a silent other model
+    # options not built on the reference chain: a usage error, not a silent other model
     with pytest.raises(SystemExit):
-        cli.main(["--zs", "1", "--config", "+bias", "--out", str(tmp_path / "x.json")])
+        cli.main(["--zs", "1", "--closure", "vp", "--out", str(tmp_path / "x.json")])
