@@ -36,6 +36,12 @@
  *                     (sgl_full.py:1314-1383, 971-1046)
  *   oracle_xi_deposit the xi_lin deposit of one row's points on the coarse
  *                     lattice (sgl_full.py:1081-1183, 1397-1441)
+ *   oracle_xi_rate    the positive closure's lens rate per coarse bin
+ *                     (sgl_full.py:1270-1280)
+ *   oracle_vp_lnr     ln R of the positive closure on a tensor grid: group
+ *                     sums through the rates' interval form and the chain
+ *                     recursion (pos_closure_fused.py:52-140,
+ *                     pos_closure.py:183-240)
  *
  * Build: oracle/build.py (gcc -O2 -fopenmp -shared).
  */
@@ -704,4 +710,182 @@ void oracle_xi_deposit(int64_t n, const double *K, const double *G, const double
         t = t1;
     }
     free(ord); free(bins); free(slot); free(mark); free(touched); free(Jt);
+}
+
+/* ------------------------------------------------------------------------
+ * oracle_xi_rate: the positive closure's lens rate per coarse (kappa, gamma)
+ * bin of one row's xi deposit points (_xi_rate_deposit, sgl_full.py:1270-1280):
+ * the xi kernels' binning (searchsorted right, the last edge inside the last
+ * bin), points outside the box dropped, Wb[bin] += w in the points' order.
+ */
+void oracle_xi_rate(int64_t n, const double *K, const double *G, const double *W,
+                    const double *ekc, int64_t nek, const double *egc, int64_t neg, double *Wb) {
+    int64_t nkc = nek - 1, ngc = neg - 1;
+    for (int64_t i = 0; i < n; ++i) {
+        double k = K[i], g = G[i];
+        int64_t a = bin_right(ekc, nek, k), b = bin_right(egc, neg, g);
+        if (k == ekc[nek - 1]) a -= 1;
+        if (g == egc[neg - 1]) b -= 1;
+        if (a >= 1 && a <= nkc && b >= 1 && b <= ngc) Wb[(a - 1) * ngc + (b - 1)] += W[i];
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * oracle_vp_lnr: ln R of the positive closure on the tensor grid kv x qv
+ * (FusedVP.lnR_blocked / _fused_groups, pos_closure_fused.py:52-140, and
+ * _chain_recursion, pos_closure.py:183-240).
+ *
+ * Cells of a shell are grouped (gstart over cells, sh_g over groups); cell c
+ * has the unique (kappa, gamma) index cu[c] and weight cw[c].  Per point p =
+ * (k_a, q_b) and group g
+ *   Y_g = sum_{c in g} w_c [(e^{i k kappa_u} - 1) J0(q gamma_u) + (J0 - 1)],
+ * e - 1 as numpy's complex expm1 ((-2 sin^2(x/2), sin x)), J0 - 1 by plain
+ * subtraction, as the reference forms them; the rates enter through their
+ * interval form (rate_intervals: D_j = c + a (u_j - anchor) + b on the active
+ * states lo..hi, c elsewhere): linear groups (c = 0) by the two sums
+ * u_j sum a Y + sum b Y, clipped groups (c = -1) anchored at their first
+ * (hi = ns - 1) or last active state, summed by prefix over the states.
+ * Then per point the log-scaled forward recursion over the shells with the
+ * transitions K[s] ([ns][ns], row i = from-state): alpha_0 = w e^{z_0},
+ * alpha_s = (K[s-1]^T alpha_{s-1}) e^{z_s}, rescaled by max Re z and the
+ * largest |alpha| each shell; ln R = scale + ln |sum alpha| + i arg(sum alpha).
+ */
+void oracle_vp_lnr(int64_t nsh, const int64_t *sh_g, const int64_t *gstart, const int64_t *cu,
+                   const double *cw, const double *aa, const double *bb, const double *cc,
+                   const int64_t *lo, const int64_t *hi, int64_t ns, const double *u,
+                   const double *K, const double *w, int64_t nU, const double *ku,
+                   const double *gu, int64_t nk, const double *kv, int64_t nq, const double *qv,
+                   double *out_re, double *out_im) {
+    const int64_t nkb = 4;
+    double *J = malloc((size_t)(nU * nq) * sizeof(double));
+    double *Jm1 = malloc((size_t)(nU * nq) * sizeof(double));
+    double *er = malloc((size_t)(nU * nkb) * sizeof(double));
+    double *ei = malloc((size_t)(nU * nkb) * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int64_t x = 0; x < nU; ++x)
+        for (int64_t b = 0; b < nq; ++b) {
+            double v = j0(gu[x] * qv[b]);
+            J[x * nq + b] = v;
+            Jm1[x * nq + b] = v - 1.0;
+        }
+    const int64_t Pmax = nkb * nq;
+    double *zr = malloc((size_t)(nsh * ns * Pmax) * sizeof(double));
+    double *zi = malloc((size_t)(nsh * ns * Pmax) * sizeof(double));
+    for (int64_t a0 = 0; a0 < nk; a0 += nkb) {
+        const int64_t na = (nk - a0 < nkb) ? nk - a0 : nkb, P = na * nq;
+#pragma omp parallel for schedule(static)
+        for (int64_t x = 0; x < nU; ++x)
+            for (int64_t a = 0; a < na; ++a) {
+                double y = kv[a0 + a] * ku[x], h = sin(0.5 * y);
+                er[x * nkb + a] = -2.0 * h * h;
+                ei[x * nkb + a] = sin(y);
+            }
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int64_t s = 0; s < nsh; ++s) {
+            double *buf = calloc((size_t)(8 * ns * P + 6 * P + 2 * P + nq), sizeof(double));
+            double *ar = buf, *ai = ar + ns * P, *br = ai + ns * P, *bi = br + ns * P;
+            double *nar = bi + ns * P, *nai = nar + ns * P, *nbr = nai + ns * P, *nbi = nbr + ns * P;
+            double *la = nbi + ns * P, *lai = la + P, *lb = lai + P, *lbi = lb + P;
+            double *cr = lbi + P, *ci = cr + P, *accr = ci + P, *acci = accr + P, *S = acci + P;
+            for (int64_t g = sh_g[s]; g < sh_g[s + 1]; ++g) {
+                if (aa[g] == 0.0 && bb[g] == 0.0 && cc[g] == 0.0) continue;
+                for (int64_t p = 0; p < P; ++p) { accr[p] = 0.0; acci[p] = 0.0; }
+                for (int64_t b = 0; b < nq; ++b) S[b] = 0.0;
+                for (int64_t c = gstart[g]; c < gstart[g + 1]; ++c) {
+                    const int64_t x = cu[c];
+                    const double wc = cw[c];
+                    const double *Jx = J + x * nq, *Jm = Jm1 + x * nq;
+                    for (int64_t b = 0; b < nq; ++b) S[b] += wc * Jm[b];
+                    for (int64_t a = 0; a < na; ++a) {
+                        const double xr = wc * er[x * nkb + a], xi = wc * ei[x * nkb + a];
+                        double *Ar = accr + a * nq, *Ai = acci + a * nq;
+                        for (int64_t b = 0; b < nq; ++b) {
+                            Ar[b] += xr * Jx[b];
+                            Ai[b] += xi * Jx[b];
+                        }
+                    }
+                }
+                for (int64_t a = 0; a < na; ++a)
+                    for (int64_t b = 0; b < nq; ++b) {
+                        const int64_t p = a * nq + b;
+                        const double X = accr[p] + S[b], Yv = acci[p];
+                        const double ax = aa[g] * X, ay = aa[g] * Yv, bx = bb[g] * X, by = bb[g] * Yv;
+                        if (cc[g] == 0.0) {
+                            la[p] += ax; lai[p] += ay; lb[p] += bx; lbi[p] += by;
+                        } else if (hi[g] == ns - 1) {
+                            const int64_t l = lo[g];
+                            ar[l * P + p] += ax; ai[l * P + p] += ay;
+                            br[l * P + p] += bx; bi[l * P + p] += by;
+                            cr[p] -= X; ci[p] -= Yv;
+                        } else {
+                            const int64_t r = hi[g];
+                            nar[r * P + p] += ax; nai[r * P + p] += ay;
+                            nbr[r * P + p] += bx; nbi[r * P + p] += by;
+                            cr[p] -= X; ci[p] -= Yv;
+                        }
+                    }
+            }
+            double *Zr = zr + s * ns * P, *Zi = zi + s * ns * P;
+            for (int64_t p = 0; p < P; ++p) {
+                double sx = 0.0, sy = 0.0, vx = 0.0, vy = 0.0;
+                for (int64_t j = 0; j < ns; ++j) {
+                    if (j > 0) { const double du = u[j] - u[j - 1]; vx += du * sx; vy += du * sy; }
+                    vx += br[j * P + p]; vy += bi[j * P + p];
+                    sx += ar[j * P + p]; sy += ai[j * P + p];
+                    Zr[j * P + p] = u[j] * la[p] + lb[p] + vx + cr[p];
+                    Zi[j * P + p] = u[j] * lai[p] + lbi[p] + vy + ci[p];
+                }
+                sx = sy = vx = vy = 0.0;
+                for (int64_t j = ns - 1; j >= 0; --j) {
+                    if (j < ns - 1) { const double du = u[j] - u[j + 1]; vx += du * sx; vy += du * sy; }
+                    vx += nbr[j * P + p]; vy += nbi[j * P + p];
+                    sx += nar[j * P + p]; sy += nai[j * P + p];
+                    Zr[j * P + p] += vx;
+                    Zi[j * P + p] += vy;
+                }
+            }
+            free(buf);
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t p = 0; p < P; ++p) {
+            double arr[64], aii[64], brr[64], bii[64];
+            double scale = 0.0;
+            for (int64_t s = 0; s < nsh; ++s) {
+                const double *Zr = zr + s * ns * P, *Zi = zi + s * ns * P;
+                if (s == 0) {
+                    for (int64_t j = 0; j < ns; ++j) { brr[j] = w[j]; bii[j] = 0.0; }
+                } else {
+                    const double *Ks = K + (s - 1) * ns * ns;
+                    for (int64_t j = 0; j < ns; ++j) {
+                        double tr = 0.0, ti = 0.0;
+                        for (int64_t i = 0; i < ns; ++i) {
+                            tr += Ks[i * ns + j] * arr[i];
+                            ti += Ks[i * ns + j] * aii[i];
+                        }
+                        brr[j] = tr; bii[j] = ti;
+                    }
+                }
+                double mx = Zr[p];
+                for (int64_t j = 1; j < ns; ++j) if (Zr[j * P + p] > mx) mx = Zr[j * P + p];
+                double nrm = 0.0;
+                for (int64_t j = 0; j < ns; ++j) {
+                    const double m = exp(Zr[j * P + p] - mx);
+                    const double e_r = m * cos(Zi[j * P + p]), e_i = m * sin(Zi[j * P + p]);
+                    arr[j] = brr[j] * e_r - bii[j] * e_i;
+                    aii[j] = brr[j] * e_i + bii[j] * e_r;
+                    const double a2 = arr[j] * arr[j] + aii[j] * aii[j];
+                    if (a2 > nrm) nrm = a2;
+                }
+                nrm = sqrt(nrm);
+                for (int64_t j = 0; j < ns; ++j) { arr[j] /= nrm; aii[j] /= nrm; }
+                scale += mx + log(nrm);
+            }
+            double sr = 0.0, si = 0.0;
+            for (int64_t j = 0; j < ns; ++j) { sr += arr[j]; si += aii[j]; }
+            const int64_t a = p / nq, b = p - a * nq;
+            out_re[(a0 + a) * nq + b] = scale + 0.5 * log(sr * sr + si * si);
+            out_im[(a0 + a) * nq + b] = atan2(si, sr);
+        }
+    }
+    free(J); free(Jm1); free(er); free(ei); free(zr); free(zi);
 }

@@ -75,6 +75,9 @@ def lib():
                                            d, i64, d, d]
         _LIB.oracle_sub_emit.argtypes = [i64, d, d, d, f64, f64, i64, i64, f64, d, d, d, d, d, d,
                                          i64, d, d, d, d, f64, f64, d, d, d]
+        _LIB.oracle_xi_rate.argtypes = [i64, d, d, d, d, i64, d, i64, d]
+        _LIB.oracle_vp_lnr.argtypes = [i64, p64, p64, p64, d, d, d, d, p64, p64, i64, d, d, d,
+                                       i64, d, d, i64, d, i64, d, d, d]
     return _LIB
 
 

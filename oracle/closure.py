@@ -84,7 +84,8 @@ def vp_rates(v, u, w):
         r2 = np.where(q2 != 0, (-q1 + disc) / (2 * q2), -q0 / q1)
     t = np.where((r1 >= lo) & (r1 < hi), r1, r2)
     x = np.maximum(u[None, :] - t[:, None], 0.0)
-    lam = x / (x @ w)[:, None]
+    with np.errstate(divide="ignore", invalid="ignore"):   # the linear rows are set below
+        lam = x / (x @ w)[:, None]
     lin = j == 0
     lam[lin] = 1.0 + np.sqrt(v[lin])[:, None] * u[None, :]
     return lam

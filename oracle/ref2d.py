@@ -314,17 +314,32 @@ def rows(cos: Cosmology, zs: float, ell=False, fil=False, edge="r200", shells=No
     return out
 
 
-def _accumulate(cos, zs, ell, fil, edge, sub, shells=None, keep_points=False, xi=False):
+def _accumulate(cos, zs, ell, fil, edge, sub, shells=None, keep_points=False, xi=False,
+                pos=False):
     """The rows of the given shells binned into the fine lattice: the sums
     (w, w kappa, w gamma) per bin and the builder's bookkeeping.  With ``xi``
     also the xi_lin deposit of every row (oracle/xilin.py), one Limber node of
     width dchi = c dz / H(z) per shell (sgl_full.py:1673-1683), closed into
-    compressed row groups at the shell's end."""
+    compressed row groups at the shell's end.  With ``pos`` (the positive
+    closure, xi['pos'] = True) also every shell's own fine-lattice sums and
+    its node geometry (z, chi, dchi) (the pos_pieces of build_population,
+    sgl_full.py:1693-1699, 1800-1802) and the lens rate per coarse bin."""
     ek, eg = fine_edges()
     nkf, ngf = ek.size - 1, eg.size - 1
     acc = dict(H0=np.zeros(nkf * ngf), H0k=np.zeros(nkf * ngf), H0g=np.zeros(nkf * ngf),
                N_expl=0.0, N_fil=0.0, kbar=0.0, n_points=0, n_decorated=0, drop=np.zeros(1),
-               pts=[], model=None, xi_groups=[])
+               pts=[], model=None, xi_groups=[], pieces={})
+    cur = dict(shell=None)
+
+    def close_piece():
+        if not pos or cur["shell"] is None:
+            return
+        h0, h0k, h0g = cur["H0"], cur["H0k"], cur["H0g"]
+        idx = np.flatnonzero((h0 != 0) | (h0k != 0) | (h0g != 0))
+        r = cur["row"]
+        acc["pieces"][cur["shell"]] = dict(
+            idx=idx, h0=h0[idx], h0k=h0k[idx], h0g=h0g[idx],
+            node=(float(r["z"]), float(cos.chi(r["z"])), float(CKMS / cos.Hz(r["z"]) * r["dz"])))
     if xi:
         from . import xilin
         kp, cj = xilin.kperp_grid(cos)
@@ -341,6 +356,11 @@ def _accumulate(cos, zs, ell, fil, edge, sub, shells=None, keep_points=False, xi
                                     ek.size, chain._p(eg), eg.size, chain._p(acc["H0"]),
                                     chain._p(acc["H0k"]), chain._p(acc["H0g"]),
                                     chain._p(acc["drop"]))
+        if pos:
+            chain.lib().oracle_bin_fine(K.size, chain._p(K), chain._p(G), chain._p(W),
+                                        chain._p(ek), ek.size, chain._p(eg), eg.size,
+                                        chain._p(cur["H0"]), chain._p(cur["H0k"]),
+                                        chain._p(cur["H0g"]), chain._p(cur["drop"]))
         acc["n_points"] += K.size
         acc["kbar"] += float(np.sum(W * K))
         if keep_points:
@@ -352,10 +372,14 @@ def _accumulate(cos, zs, ell, fil, edge, sub, shells=None, keep_points=False, xi
         zl, Ml = np.array([sh[2] for sh in z_shells(zs)]), engine_grids()[1]
         model = acc["model"] = _sub.SubhaloModel(cos, zs, zl, Ml)
     for r in rows(cos, zs, ell=ell, fil=fil, edge=edge, shells=shells):
+        if pos and cur["shell"] != r["shell"]:
+            close_piece()
+            cur.update(shell=r["shell"], row=r, H0=np.zeros(nkf * ngf), H0k=np.zeros(nkf * ngf),
+                       H0g=np.zeros(nkf * ngf), drop=np.zeros(1))
         if xi and (shell_xi is None or shell_xi[0] != r["shell"]):
             if shell_xi is not None:
                 acc["xi_groups"] += shell_xi[1].end(shell_xi[0], compress=xi != "dense")
-            shell_xi = (r["shell"], xilin.Shell(kp, cj, CKMS / cos.Hz(r["z"]) * r["dz"]))
+            shell_xi = (r["shell"], xilin.Shell(kp, cj, CKMS / cos.Hz(r["z"]) * r["dz"], pos=pos))
         acc["N_expl"] += r["Nbar"]
         grid = host_grid(r["ks"], r["xt"], r["xm"], r["Nbar"], r["eps"], phis if ell else None)
         h = model.host(r["shell"], r["jM"]) if model is not None else None
@@ -387,19 +411,20 @@ def _accumulate(cos, zs, ell, fil, edge, sub, shells=None, keep_points=False, xi
                 shell_xi[1].deposit(K, G, W, Rf, r["beta_fil"], False)
     if shell_xi is not None:
         acc["xi_groups"] += shell_xi[1].end(shell_xi[0], compress=xi != "dense")
+    close_piece()
     return acc
 
 
 def _worker(args):
-    cos_kw, zs, ell, fil, edge, sub, shells, xi = args
-    acc = _accumulate(Cosmology(**cos_kw), zs, ell, fil, edge, sub, shells, xi=xi)
+    cos_kw, zs, ell, fil, edge, sub, shells, xi, pos = args
+    acc = _accumulate(Cosmology(**cos_kw), zs, ell, fil, edge, sub, shells, xi=xi, pos=pos)
     acc.pop("model")
     acc.pop("pts")
     return acc
 
 
 def build(cos: Cosmology, zs: float, ell=False, fil=False, keep_points=False, edge="r200",
-          sub=False, workers=None, xi=False):
+          sub=False, workers=None, xi=False, pos=False):
     """build_population(..., kap_floor=1e-8, trunc="r200", zint="midpoint")
     without clustering: every row's cells (decorated by subhalos on hosts
     that carry them when ``sub``, oracle/sub.py) binned into the fine lattice
@@ -411,10 +436,10 @@ def build(cos: Cosmology, zs: float, ell=False, fil=False, keep_points=False, ed
         workers = min(8, os.cpu_count() or 1) if (sub or xi) else 1
     workers = max(1, min(int(workers), n_sh))
     if workers == 1 or keep_points:
-        accs = [_accumulate(cos, zs, ell, fil, edge, sub, None, keep_points, xi=xi)]
+        accs = [_accumulate(cos, zs, ell, fil, edge, sub, None, keep_points, xi=xi, pos=pos)]
     else:
         import multiprocessing as mp
-        jobs = [(cos.kw, zs, ell, fil, edge, sub, set(range(w, n_sh, workers)), xi)
+        jobs = [(cos.kw, zs, ell, fil, edge, sub, set(range(w, n_sh, workers)), xi, pos)
                 for w in range(workers)]
         # fresh interpreters (no inherited OpenMP or device state), one
         # OpenMP thread each
@@ -442,6 +467,9 @@ def build(cos: Cosmology, zs: float, ell=False, fil=False, keep_points=False, ed
     # xi_lin row groups in shell order, m ascending within a shell
     pop.xi_groups = sorted((g for a in accs for g in a["xi_groups"]),
                            key=lambda g: (g["shell"], g["m"]))
+    if pos:
+        pop.pieces = {k: v for a in accs for k, v in a["pieces"].items()}
+        pop.H0, pop.H0k, pop.H0g = H0, H0k, H0g
     if keep_points:
         pop.points = accs[0]["pts"]
     return pop

@@ -83,16 +83,22 @@ def coarse_centres():
 class Shell:
     """The deposit of one lens-redshift shell (xi_begin .. xi_end)."""
 
-    def __init__(self, kp, c, dchi):
+    def __init__(self, kp, c, dchi, pos=False):
         self.kp, self.c, self.dchi = kp, c, float(dchi)
         self.ekc, self.egc = coarse_edges()
         self.ngc = self.egc.size - 1
         self.Hx = np.zeros(((self.ekc.size - 1) * self.ngc, 3, kp.size))
         self.lx = np.zeros((2, kp.size))
+        # positive closure: lens rate per coarse bin (Cells.xi_begin, sgl_full.py:1394-1395)
+        self.Wb = np.zeros((self.ekc.size - 1) * self.ngc) if pos else None
 
     def deposit(self, K, G, W, R, beta, tang):
         K, G, W, R = (np.ascontiguousarray(a, dtype=float).ravel() for a in (K, G, W, R))
         G = np.clip(G, 1.0000001e-9, None)
+        if self.Wb is not None:
+            chain.lib().oracle_xi_rate(K.size, chain._p(K), chain._p(G), chain._p(W),
+                                       chain._p(self.ekc), self.ekc.size, chain._p(self.egc),
+                                       self.egc.size, chain._p(self.Wb))
         chain.lib().oracle_xi_deposit(K.size, chain._p(K), chain._p(G), chain._p(W), chain._p(R),
                                       float(beta), int(bool(tang)), self.kp.size,
                                       chain._p(self.kp), chain._p(self.ekc), self.ekc.size,
@@ -137,8 +143,34 @@ class Shell:
             if not keep.any():
                 continue
             rows = (A @ Vd[:, keep]).T
-            out.append(dict(shell=shell, m=m, idx=idx, rows=rows))
+            grp = dict(shell=shell, m=m, idx=idx, rows=rows)
+            if m == 0 and self.Wb is not None:
+                # positive closure: bin rates and the leading k_perp profile
+                # (sgl_full.py:1520-1521)
+                grp.update(W=self.Wb[idx].copy(), v0=Vd[:, 0].copy())
+            out.append(grp)
         return out
+
+
+def spline_on_coarse(fn, kk, kg, n, nk):
+    """Lam._Q_interpolated (sgl_full.py:2953-3022) of fn: fn on the coarse grid
+    k_kappa = kk_max s_k^2 (nk nodes) x k_gamma = kg_max s^2 (n nodes), s
+    uniform on [0, 1], then scipy's not-a-knot cubic spline in s_k along
+    k_kappa and in s along k_gamma, evaluated on the full grid."""
+    from scipy.interpolate import BSpline, make_interp_spline
+    kk, kg = np.asarray(kk, float), np.asarray(kg, float)
+    s = np.linspace(0.0, 1.0, n)
+    s_k = s if nk == n else np.linspace(0.0, 1.0, nk)
+    Qc = fn(float(kk[-1]) * s_k ** 2, float(kg[-1]) * s ** 2)
+    sk = sqrt(np.clip(kk / kk[-1], 0.0, 1.0))
+    sg = sqrt(np.clip(kg / kg[-1], 0.0, 1.0))
+    out = np.empty((kk.size, kg.size), complex)
+    for part, setter in ((Qc.real, "real"), (Qc.imag, "imag")):
+        sp0 = make_interp_spline(s_k, part, k=3, axis=0)
+        sp1 = make_interp_spline(s, sp0.c, k=3, axis=1)
+        val = BSpline.design_matrix(sk, sp0.t, 3) @ (BSpline.design_matrix(sg, sp1.t, 3) @ sp1.c).T
+        setattr(out, setter, val)
+    return out
 
 
 class Psi2:
@@ -235,18 +267,7 @@ class Psi2:
     def interpolated(self, kk, kg, n=XI_COARSE):
         """_Q_interpolated (sgl_full.py:2953-3022): the coarse grid and scipy's
         not-a-knot bicubic spline in s = sqrt(k / k_max)."""
-        from scipy.interpolate import BSpline, make_interp_spline
-        kk, kg = np.asarray(kk, float), np.asarray(kg, float)
-        s, Qc = self.coarse(float(kk[-1]), float(kg[-1]), n)
-        sk = sqrt(np.clip(kk / kk[-1], 0.0, 1.0))
-        sg = sqrt(np.clip(kg / kg[-1], 0.0, 1.0))
-        out = np.empty((kk.size, kg.size), complex)
-        for part, setter in ((Qc.real, "real"), (Qc.imag, "imag")):
-            sp0 = make_interp_spline(s, part, k=3, axis=0)
-            sp1 = make_interp_spline(s, sp0.c, k=3, axis=1)
-            val = BSpline.design_matrix(sk, sp0.t, 3) @ (BSpline.design_matrix(sg, sp1.t, 3) @ sp1.c).T
-            setattr(out, setter, val)
-        return out
+        return spline_on_coarse(self.direct, kk, kg, n, n)
 
     def __call__(self, kk, kg):
         """Lam._direct_call's xi term (sgl_full.py:3114-3128): interpolated on a

@@ -21,7 +21,7 @@ OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_scalar
 
 
 def main():
-    fx = {}
+    fx, fxv = {}, {}
     keys = set()
     pdfs = {}
     for name in ("pdf_zs0.5", "pdf_zs1", "pdf_zs3", "pdf_zs10", "pdf_zs0.5_vp", "pdf_zs1_vp"):
@@ -31,6 +31,12 @@ def main():
         pdfs[tag + "_P_s"] = np.asarray(d["P_s"], float)
         pdfs[tag + "_P_s_tail"] = np.asarray(d["P_s_tail"], float)
         if name.endswith("_vp"):
+            pc = d["pos_closure"]
+            fxv[name] = dict(eta=pc["eta"], cells=pc["cells"], groups=pc["groups"],
+                             shells=pc["shells"], target_kappa=pc["target_kappa"],
+                             target_gamma=pc["target_gamma"], sigma_kappa=d["sigma_kappa"],
+                             mean=d["mean"], var=d["var"], skew=d["skew"], sigmaDL=d["sigmaDL"],
+                             grid={k: v for k, v in d["grid"].items() if isinstance(v, (int, float))})
             continue
         keys |= set(d)
         fx[name] = dict(Omega_r=d["dark_energy"]["Omega_r"],
@@ -78,7 +84,7 @@ def main():
     out = dict(provenance="sgl_analytic v0.2.0 (MIT): tests/reference/pdf_zs*.json and the "
                           "printed outputs of notebooks/sgl_analytic_walkthrough.ipynb cells 3, 7 and "
                           "notebooks/sgl_pdf_explorer.ipynb cell 14 (ingredient ladder)",
-               fixtures=fx, walkthrough=nbv, explorer_ladder_zs1=ladder,
+               fixtures=fx, fixtures_vp=fxv, walkthrough=nbv, explorer_ladder_zs1=ladder,
                fixture_keys=sorted(keys))
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
