@@ -474,6 +474,88 @@ sgl_status sgl_ref_xi_grid(const void* tables, int32_t flags, const void* xi_sta
                            int32_t ng, double* psi, void* stream);
 
 /*
+ * The positive ("vp") clustering closure on the xi_lin population
+ * (xi_closure="vp", pipeline.py:154-207; pos_closure.install,
+ * pos_closure.py:708-733): ln R = ln E_U exp(sum_s sum_c W_sc (lam_sc(U_s) - 1)
+ * F_c) - shear_var k_gamma^2 / 4 replaces Psi_2.  Order, after
+ * sgl_ref_cells_emit_xi: sgl_ref_vp_runs (before the cells are freed) ->
+ * sgl_ref_bin -> the xi sort and rows -> sgl_ref_vp_setup -> sgl_ref_vp_variance
+ * (as often as the eta root search needs) -> sgl_ref_vp_rates ->
+ * sgl_ref_vp_lnr + sgl_ref_vp_qxi (reach probes, coarse grid) ->
+ * sgl_ref_vp_spline (psi of sgl_ref_phi) -> sgl_ref_window when the reach did
+ * not converge.
+ */
+/* The emitted cells keyed by (shell, fine bin), stably sorted and summed per
+ * run (w, w kappa, w gamma) in fixed parts: the per-shell fine-lattice pieces.
+ * offsets: the point offsets of sgl_ref_cells_count; *n_cells (device int64)
+ * receives the number of (shell, bin) cells.  ws: sgl_ref_vp_runs_bytes,
+ * read by sgl_ref_vp_setup.
+ * Replaces: the per-shell Cells of build_population and their pos_pieces
+ * (sgl_full.py:1693-1699, 1790-1802). */
+size_t sgl_ref_vp_runs_bytes(int64_t n_points);
+sgl_status sgl_ref_vp_runs(const void* tables, int32_t flags, int64_t n_points,
+                           const int64_t* offsets, const int32_t* key, const double* kap,
+                           const double* gam, const double* w, void* ws, int64_t* n_cells,
+                           void* stream);
+/*
+ * Closure set-up: each cell's pooled bin and centroid (pop, npop of
+ * sgl_ref_bin), its own centroid's coarse parent and that parent's loading
+ * (from the xi rows: the m = 0 group's kappa mode over the lens rate), groups
+ * of one shell and parent, the line-of-sight correlation of consecutive
+ * shells on 30001 k_par nodes, f_s and the stationary transitions.  state:
+ * sgl_ref_vp_state_bytes(n_cells); meta_out (device int64[16]): [0] cells
+ * (bins with nonzero sums), [1] groups, [2] shells, [5] status bits (1 bins
+ * disagree with pop, 2 a shell without a kappa mode, 8 a transition not
+ * balanced).
+ * Replaces: pos_closure.population and Closure.__init__ up to the
+ * calibration (pos_closure.py:388-479), los_rho and transition
+ * (pos_closure.py:251-267, 372-385).
+ */
+size_t sgl_ref_vp_state_bytes(int64_t n_cells);
+sgl_status sgl_ref_vp_setup(const void* tables, int32_t flags, int64_t n_points,
+                            const void* runs_ws, int64_t n_cells, const double* pop,
+                            const int32_t* npop, const void* xi_state, const void* xi_rows_ws,
+                            int64_t n_xi_runs, int64_t n_xi_parts, void* state, int64_t* meta_out,
+                            void* stream);
+/* *out (device) = the chain's kappa clustering variance at eta (status bit 4
+ * in the state's record when a rate variance exceeds the 48-state cap).
+ * Replaces: Closure.variance (pos_closure.py:560-573) with chain_variance
+ * (pos_closure.py:352-369). */
+sgl_status sgl_ref_vp_variance(void* state, int64_t n_cells, int32_t n_z, double eta, double* out,
+                               void* stream);
+/* The rates at the calibrated eta in interval form and the visiting order of
+ * the groups (meta_out as sgl_ref_vp_setup's, [3] visited groups, [4] visited
+ * cells).  Replaces: vp_rates / group_rates and rate_intervals
+ * (pos_closure.py:273-318, pos_closure_fused.py:18-49). */
+sgl_status sgl_ref_vp_rates(void* state, int64_t n_cells, int32_t n_z, double eta,
+                            int64_t* meta_out, void* stream);
+/* ln R on the tensor grid kv[nk] x qv[nq] (out: complex [nk][nq]; pop as in
+ * sgl_ref_phi, n_bins its occupied count); ws: sgl_ref_vp_lnr_bytes.
+ * Replaces: FusedVP.lnR_blocked / _fused_groups (pos_closure_fused.py:52-140)
+ * and _chain_recursion (pos_closure.py:183-240). */
+size_t sgl_ref_vp_lnr_bytes(int32_t n_z, int32_t n_bins, int32_t nk, int32_t nq);
+sgl_status sgl_ref_vp_lnr(const void* state, int64_t n_cells, int32_t n_z, const double* pop,
+                          int32_t n_bins, const double* kv, int32_t nk, const double* qv,
+                          int32_t nq, void* ws, double* out, void* stream);
+/* In place on ln R [nk][nq]: Im unwrapped along k_kappa at the first k_gamma,
+ * then along k_gamma (numpy.unwrap), shear_var qv^2 / 4 subtracted from Re.
+ * Replaces: Closure.qxi (pos_closure.py:690-705). */
+sgl_status sgl_ref_vp_qxi(double* lnr, int32_t nk, int32_t nq, const double* qv,
+                          double shear_var, void* stream);
+/* psi [nk][ng] (complex) on k_kappa = linspace(0, K, nk) x k_gamma =
+ * linspace(0, Q, ng) from coarse [nkc][nc] on k = K s_k^2 x Q s^2 (s uniform
+ * on [0, 1]): the not-a-knot bicubic interpolant in (s_k, s).
+ * ws: sgl_ref_vp_spline_bytes.  Replaces: Lam._Q_interpolated with a
+ * separate k_kappa node count (sgl_full.py:2953-3022, 226-238). */
+size_t sgl_ref_vp_spline_bytes(int32_t nkc, int32_t nc, int32_t nk);
+sgl_status sgl_ref_vp_spline(const double* coarse, int32_t nkc, int32_t nc, int32_t nk,
+                             int32_t ng, void* ws, double* psi, void* stream);
+/* phi [2 nk][ng] (as sgl_ref_phi writes it) times the positive-definite window
+ * (cubic B-spline in k_kappa x Wendland phi_{3,1} in k_gamma on the box).
+ * Replaces: _window_pd and its use in invert (sgl_full.py:83-94, 328-333). */
+sgl_status sgl_ref_window(double* phi, int32_t nk, int32_t ng, void* stream);
+
+/*
  * Binning: stable sort by key, per-bin sums of w, w kappa, w gamma in the cells'
  * order, the occupied bins compacted in bin order into pop = [w][kappa
  * centroid][gamma centroid] (SGL_REF_FINE_BINS doubles each), npop = their

@@ -36,12 +36,22 @@ EXPORTS = (
     "sgl_ref_cells_emit_xi", "sgl_ref_xi_state_bytes", "sgl_ref_xi_init", "sgl_ref_xi_sort_bytes",
     "sgl_ref_xi_sort", "sgl_ref_xi_rows_bytes", "sgl_ref_xi_rows", "sgl_ref_xi_axis",
     "sgl_ref_xi_grid_bytes", "sgl_ref_xi_grid", "sgl_ref_xi_info",
+    "sgl_ref_vp_runs_bytes", "sgl_ref_vp_runs", "sgl_ref_vp_state_bytes", "sgl_ref_vp_setup",
+    "sgl_ref_vp_variance", "sgl_ref_vp_rates", "sgl_ref_vp_lnr_bytes", "sgl_ref_vp_lnr",
+    "sgl_ref_vp_qxi", "sgl_ref_vp_spline_bytes", "sgl_ref_vp_spline", "sgl_ref_window",
 )
 NOALLOC_SCRATCH_DOUBLES = 1 << 22   # capacity of sgl_lnphi_exact_noalloc's own array
 REF_MAX_SHELLS, REF_MASSES, REF_FINE_BINS, REF_ROOT_STRIDE = 104, 99, 961 * 300, 128
 REF_STATUS_ROOTS, REF_STATUS_EDGES, REF_STATUS_BAD = 16, 32, 2
-# sgl_ref_xi_info record: [n_runs, n_parts, n_z, nP, nQ, variance, largest s_0]
+# sgl_ref_xi_info record: [n_runs, n_parts, n_z, nP, nQ, variance, largest s_0,
+# shear variance of the m = 1 groups]
 XI_INFO_RUNS, XI_INFO_PARTS, XI_INFO_NZ, XI_INFO_NP, XI_INFO_NQ, XI_INFO_VAR = 0, 1, 2, 3, 4, 5
+XI_INFO_SHEAR = 7
+# sgl_ref_vp_setup / sgl_ref_vp_rates record (int64): cells, groups, shells,
+# visited groups, visited cells, status bits
+VP_META_CELLS, VP_META_GROUPS, VP_META_SHELLS, VP_META_VGROUPS, VP_META_VCELLS = 0, 1, 2, 3, 4
+VP_META_STATUS = 5
+VP_STATUS_BINS, VP_STATUS_MODE, VP_STATUS_RATE, VP_STATUS_TRANSITION = 1, 2, 4, 8
 EPI_PLAIN, EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU = 0, 1, 2, 3
 TUNE_BIN_THREADS, TUNE_LNPHI_BLOCK, TUNE_GEMM_BM = 0, 1, 2
 PSI2_STRIDE = 8  # SGL_PSI2_STRIDE: batched two-point sub-grid stride
@@ -113,6 +123,14 @@ def load(path: str = LIB_PATH):
     L.sgl_ref_xi_grid_bytes.argtypes = [i32, i32, i32]
     L.sgl_ref_p3_bytes.restype = sz
     L.sgl_ref_p3_bytes.argtypes = [i32, i32, i32, i32]
+    L.sgl_ref_vp_runs_bytes.restype = sz
+    L.sgl_ref_vp_runs_bytes.argtypes = [i64]
+    L.sgl_ref_vp_state_bytes.restype = sz
+    L.sgl_ref_vp_state_bytes.argtypes = [i64]
+    L.sgl_ref_vp_lnr_bytes.restype = sz
+    L.sgl_ref_vp_lnr_bytes.argtypes = [i32, i32, i32, i32]
+    L.sgl_ref_vp_spline_bytes.restype = sz
+    L.sgl_ref_vp_spline_bytes.argtypes = [i32, i32, i32]
     sigs = {
         "sgl_tables_build": [vp, i32, sp, vp, vp, vp],
         "sgl_lnphi_exact": [vp, i32, sp, vp, vp, vp, vp],
@@ -157,6 +175,14 @@ def load(path: str = LIB_PATH):
         "sgl_ref_arcs": [vp, f64, f64, i32, f64, i32, f64, vp, i32, vp, vp, vp],
         "sgl_ref_moments": [vp, vp, i32, vp, vp],
         "sgl_ref_gemm_f64": [i32, i32, i32, vp, i64, vp, i64, vp, i64, vp],
+        "sgl_ref_vp_runs": [vp, i32, i64, vp, vp, vp, vp, vp, vp, vp, vp],
+        "sgl_ref_vp_setup": [vp, i32, i64, vp, i64, vp, vp, vp, vp, i64, i64, vp, vp, vp],
+        "sgl_ref_vp_variance": [vp, i64, i32, f64, vp, vp],
+        "sgl_ref_vp_rates": [vp, i64, i32, f64, vp, vp],
+        "sgl_ref_vp_lnr": [vp, i64, i32, vp, i32, vp, i32, vp, i32, vp, vp, vp],
+        "sgl_ref_vp_qxi": [vp, i32, i32, vp, f64, vp],
+        "sgl_ref_vp_spline": [vp, i32, i32, i32, i32, vp, vp, vp],
+        "sgl_ref_window": [vp, i32, i32, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

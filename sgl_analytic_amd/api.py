@@ -15,9 +15,9 @@ on this chain: spherical truncated NFW halos, "ell" (pseudo-elliptical
 halos), "fil" (filaments) and "sub" (subhalos on their hosts, the exact
 decorated-host sector; with edge "r200" only) and "bias" (the xi_lin
 Limber two-point term, clustering="xilin", the pipeline's default); edge
-"r200" (default) or "rvir".  The default config="full" therefore runs every
-ingredient, as the reference does.  The positive "vp" closure raises
-NotImplementedError: it is not built on this chain.
+"r200" (default) or "rvir"; closure "gauss" (the Gaussian pair term) or
+"vp" (the positive variance-preserving closure, pos_closure.install).  The
+default config="full" therefore runs every ingredient, as the reference does.
 
 ``run_kernel_shape`` is the host driver of the kernel-shape chain (64
 slices uniform in comoving distance x 256 masses x 512 impact parameters,
@@ -165,15 +165,13 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200
     cp = check_cosmology(cosmo)
     if closure != "gauss" and "bias" not in s:
         raise ValueError(f"xi_closure={closure!r} needs xilin clustering and the bias ingredient")
-    if closure != "gauss":
-        raise NotImplementedError("the positive 'vp' closure is not built on the reference chain")
     if "sub" in s and edge != "r200":
         raise NotImplementedError("subhalos are built with edge='r200' only (clumps truncated at "
                                   "their r200, the reference's pairing for that edge)")
     from . import ref2d
     t0 = time.perf_counter()
     pop = ref2d.build(cp, float(z_s), ell="ell" in s, fil="fil" in s, edge=edge, sub="sub" in s,
-                      bias="bias" in s)
+                      bias="bias" in s, vp=closure == "vp")
     t1 = time.perf_counter()
     x = np.asarray(xi_out, dtype=np.float64)
     res = ref2d.invert(pop, x)
@@ -185,6 +183,12 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200
                           sigmaDL=res.sigmaDL, grid=res.grid),
                      dict(build=t1 - t0, lam=0.0, invert=t2 - t1))
     info["zs"] = float(z_s)
+    if pop.vp is not None:
+        v = pop.vp
+        info["pos_closure"] = dict(kind="vp", eta=v.eta, nstate=48, shells=v.n_z,
+                                   cells=v.cells, groups=v.n_groups,
+                                   target_kappa=v.target_kappa, target_gamma=v.target_gamma,
+                                   eval_kernel="MI355X HIP (sgl_ref_vp_lnr)")
     return res.P, info
 
 
@@ -198,7 +202,9 @@ def record_2d(cp, config, ingredients, closure, edge, pop, res, t_stages):
               radiation=cp["radiation"], Omega_r=Or)
     return dict(
         t_stages=dict(t_stages), cfg=config if isinstance(config, str) else sorted(ingredients),
-        zint="midpoint", zint_n=None, clustering=None, xi_kperp=None, edge=edge, lss_var=0.0,
+        zint="midpoint", zint_n=None, clustering="xilin",
+        xi_kperp=(dict(kmin=1e-4, kmax=1e4, per_decade=[2, 2, 12, 8, 6, 4, 4, 4], mmax=2)
+                  if "bias" in ingredients else None), edge=edge, lss_var=0.0,
         kap_floor=1e-8, kbar=float(pop["kbar"]), h=cp["h"], Om=cp["Om"], s8=cp["s8"],
         sigma8=cp["s8"], Ob=cp["Ob"], ns=cp["ns"], zeq=cp["zeq"], Ok=cp["Ok"], w0=cp["w0"],
         wa=cp["wa"], growth_mode=cp["growth_mode"], radiation=cp["radiation"], Omega_r=Or,
@@ -211,7 +217,8 @@ def record_2d(cp, config, ingredients, closure, edge, pop, res, t_stages):
         skew=float(res["skew"]), sigmaDL=float(res["sigmaDL"]),
         grid={k: v for k, v in res["grid"].items() if isinstance(v, (int, float))},
         tail_box=True, fine=dict(k=[1e-7, 30.0, 480], g=[1e-9, 30.0, 300]), s_is=True,
-        xi_clamp=False, xi_linear_m=0.0, xi_closure=closure, window=False, n_xi_wide=500,
+        xi_clamp=False, xi_linear_m=0.0, xi_closure=closure,
+        window="auto" if closure == "vp" else False, n_xi_wide=4000 if closure == "vp" else 500,
         pos_closure=None, direct_product_backend="MI355X HIP FP64 MFMA (libsgl_amd.so)",
         direct_product_fallback=None, s_seed=None, s_seed_effective=0, s_ess=None,
         is_design=None, pk_mod=None, pk_par={},

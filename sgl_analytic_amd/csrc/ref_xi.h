@@ -37,7 +37,8 @@ struct XiState {
       total;
 };
 // scal: [0] clustering variance, [1] largest singular value over groups,
-// [2] k_kappa max and [3] k_gamma max of the coarse grid
+// [2] k_kappa max and [3] k_gamma max of the coarse grid, [4] the shear
+// variance of the m = 1 groups (the positive closure's shear target)
 __host__ __device__ inline XiState xi_state_layout() {
   XiState L;
   size_t o = 0;
@@ -91,5 +92,14 @@ struct XiEmit {
   int* pck;
   double* cells;
 };
+
+// What the positive closure reads of the xi rows workspace (sgl_ref_xi_rows):
+// the run keys, each run's loading phic and lens rate, the shells' leading
+// k_perp profiles |v0|.
+struct XiRowsView {
+  const int* ukey;
+  const double *phic, *wrun, *v0;
+};
+XiRowsView xi_rows_view(const void* rows_ws, long long n_runs, long long n_parts);
 
 }  // namespace sgl

@@ -353,7 +353,8 @@ __global__ __launch_bounds__(256) void k_xi_walk(
     const long long* __restrict__ hoff, int total_parts, const int* __restrict__ sval,
     const double* __restrict__ K, const double* __restrict__ G, const double* __restrict__ W,
     const int* __restrict__ pcid, const double* __restrict__ cells, const double* __restrict__ JT,
-    double* __restrict__ Hx, double* __restrict__ Hp, double* __restrict__ Lp) {
+    double* __restrict__ Hx, double* __restrict__ Hp, double* __restrict__ Lp,
+    double* __restrict__ Wp) {
   const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (wv >= total_parts) return;
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(256) void k_xi_walk(
   const long long p = wv - poff[r];
   const long long a = rstart[r] + p * kXiPart;
   const long long b = min(rstart[r] + (long long)rlen[r], a + kXiPart);
-  double h0 = 0.0, h1 = 0.0, lk = 0.0, lg = 0.0;
+  double h0 = 0.0, h1 = 0.0, lk = 0.0, lg = 0.0, wsum = 0.0;
   for (long long base = a; base < b; base += 64) {
     const long long q = base + lane;
     const bool ok = q < b;
@@ -375,6 +376,7 @@ __global__ __launch_bounds__(256) void k_xi_walk(
     if (ok) {
       const int idx = sval[q];
       cid = pcid[idx];
+      wsum += W[idx];
       const double bw = W[idx] * cells[2 * (size_t)cid + 1];
       vs = bw;
       vk = bw * K[idx];
@@ -413,6 +415,9 @@ __global__ __launch_bounds__(256) void k_xi_walk(
       lg += Sg * ((lane >= kXiNk) ? v0 : v1);
     }
   }
+  // the part's lens rate (the points' weights, lane-strided then a fixed tree)
+  for (int off = 32; off > 0; off >>= 1) wsum += __shfl_xor(wsum, off);
+  if (lane == 0) Wp[wv] = wsum;
   double* H = (nparts[r] > 1) ? Hp + (size_t)(hoff[r] + p) * kXiVals : Hx + (size_t)r * kXiVals;
   H[lane] = h0;
   if (lane < kXiVals - 64) H[64 + lane] = h1;
@@ -434,6 +439,18 @@ __global__ void k_xi_combine(int n_runs, const int* __restrict__ nparts,
   double s = 0.0;
   for (int p = 0; p < np; ++p) s += Hp[(size_t)(hoff[r] + p) * kXiVals + v];
   Hx[(size_t)r * kXiVals + v] = s;
+}
+
+// lens rate per run (the positive closure's W_b of a (shell, coarse bin)):
+// its parts in order
+__global__ void k_xi_wrun(int n_runs, const int* __restrict__ nparts,
+                          const long long* __restrict__ poff, const double* __restrict__ Wp,
+                          double* __restrict__ wrun) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_runs) return;
+  double s = 0.0;
+  for (long long p = poff[r]; p < poff[r] + nparts[r]; ++p) s += Wp[p];
+  wrun[r] = s;
 }
 
 // occupied coarse kappa and gamma bins over every shell
@@ -704,6 +721,78 @@ __global__ __launch_bounds__(256) void k_xi_group(const double* __restrict__ cos
   }
 }
 
+// Loadings of the positive closure (pos_closure.population,
+// pos_closure.py:405-421), one workgroup per shell, on the m = 0 group before
+// the global prune: the kept rows (s_r > 1e-6 s_0) are (A V)_r, so the
+// 'kappa' mode m_r = rows_r . kappa_c + res_r is V^T (sum_b A_b kappa_b + A_res)
+// and r0_b = (m / |m|) . rows_b = A_b . (V m / |m|); phic_b = sgn r0_b / W_b
+// (0 where W_b <= 0), sgn the sign of r0 . W; v0 = |V[:, 0]|, the leading
+// k_perp profile (sgl_full.py:1520-1521).
+__global__ __launch_bounds__(256) void k_xi_loading(const double* __restrict__ cosmo,
+                                                    const double* __restrict__ slices,
+                                                    const int* __restrict__ ukey,
+                                                    const double* __restrict__ Hx,
+                                                    const double* __restrict__ wrun,
+                                                    const double* __restrict__ xs,
+                                                    double* __restrict__ phic,
+                                                    double* __restrict__ v0) {
+  __shared__ double sw[kXiNk], vec[kXiNk], mm[kXiNk], y[kXiNk], red[256];
+  const XiState L = xi_state_layout();
+  const int* meta = reinterpret_cast<const int*>(xs + L.meta);
+  const int s = blockIdx.x, t = threadIdx.x, g = 3 * s;
+  if (s >= xi_nz(cosmo)) return;
+  const int r0 = meta[kXmRfirst + s], r1 = meta[kXmRfirst + s + 1];
+  const double* lam = xs + L.lam + (size_t)g * kXiNk;
+  const double* V = xs + L.V + (size_t)g * kXiNk * kXiNk;
+  const double s0 = sqrt(fmax(lam[0], 0.0));
+  if (t < kXiNk) {
+    sw[t] = sqrt(xs[L.c + t] / slices[(size_t)s * kSliceStride + kSlDchi]);
+    double a = 0.0;
+    for (int r = r0; r < r1; ++r) {
+      const int b = ukey[r] - s * kXiBins;
+      a += Hx[(size_t)r * kXiVals + t] * sw[t] * xs[L.kmc + b / kXiGc];
+    }
+    vec[t] = a + xs[L.res + (size_t)g * kXiNk + t];
+    v0[(size_t)s * kXiNk + t] = fabs(V[t * kXiNk]);
+  }
+  __syncthreads();
+  if (t < kXiNk) {
+    const double sr = sqrt(fmax(lam[t], 0.0));
+    double a = 0.0;
+    if (s0 > 0.0 && sr > kXiSvdTol * s0)
+      for (int j = 0; j < kXiNk; ++j) a += vec[j] * V[j * kXiNk + t];
+    mm[t] = a;
+  }
+  __syncthreads();
+  if (t < kXiNk) {
+    double n2 = 0.0;
+    for (int r = 0; r < kXiNk; ++r) n2 += mm[r] * mm[r];
+    const double inv = 1.0 / sqrt(n2);
+    double a = 0.0;
+    for (int r = 0; r < kXiNk; ++r) a += V[t * kXiNk + r] * (mm[r] * inv);
+    y[t] = a;
+  }
+  __syncthreads();
+  double part = 0.0;
+  for (int r = r0 + t; r < r1; r += 256) {
+    double a = 0.0;
+    for (int j = 0; j < kXiNk; ++j) a += Hx[(size_t)r * kXiVals + j] * sw[j] * y[j];
+    phic[r] = a;
+    part += a * wrun[r];
+  }
+  red[t] = part;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  const double sgn = (red[0] >= 0.0) ? 1.0 : -1.0;
+  for (int r = r0 + t; r < r1; r += 256) {
+    const double wb = wrun[r];
+    phic[r] = (wb > 0.0) ? sgn * phic[r] / wb : 0.0;
+  }
+}
+
 // keep: s_r > 1e-6 s_0 of the group and s_r > 1e-7 of the largest s_0 of any
 // group; the dropped columns of V are zeroed (one workgroup)
 __global__ __launch_bounds__(256) void k_xi_keep(double* __restrict__ xs) {
@@ -760,7 +849,7 @@ __global__ __launch_bounds__(256) void k_xi_tables(const double* __restrict__ co
   double* W0 = xs + L.W0 + (size_t)s * kXiKc * kXiNk;   // [p'][r], m = 0 only
   if (t < kXiNk) {
     const int j = t;
-    double srow = 0.0, kmom = 0.0, wp = 0.0;
+    double srow = 0.0, kmom = 0.0, gmom = 0.0, wp = 0.0;
     int pcur = -1;
     for (int r = r0; r < r1; ++r) {
       const int b = ukey[r] - s * kXiBins;
@@ -768,6 +857,7 @@ __global__ __launch_bounds__(256) void k_xi_tables(const double* __restrict__ co
       const double A = Hx[(size_t)r * kXiVals + m * kXiNk + j] * sw[j];
       srow += A;
       kmom += A * xs[L.kmc + p];
+      gmom += A * xs[L.gmc + q];
       Uq[meta[kXmQmap + q]][j] += A;
       if (m == 0) {
         if (p != pcur) {
@@ -781,7 +871,9 @@ __global__ __launch_bounds__(256) void k_xi_tables(const double* __restrict__ co
     if (m == 0 && pcur >= 0) W0[(size_t)meta[kXmPmap + pcur] * kXiNk + j] = wp;
     vec[0][j] = srow;
     vec[1][j] = xs[L.res + (size_t)g * kXiNk + j];
-    vec[2][j] = kmom + vec[1][j];
+    // m = 0: the kappa moment (the variance); m = 1: the gamma moment (the
+    // positive closure's shear target, pos_closure.py:443-446)
+    vec[2][j] = ((m == 1) ? gmom : kmom) + vec[1][j];
   }
   __syncthreads();
   // rotate: out_r = sum_j x_j V[j][r]
@@ -799,7 +891,7 @@ __global__ __launch_bounds__(256) void k_xi_tables(const double* __restrict__ co
   }
   if (t == 0) {
     double v = 0.0;
-    if (m == 0)
+    if (m < 2)
       for (int r = 0; r < kXiNk; ++r) v += rot[2][r] * rot[2][r];
     varpart[g] = v;
   }
@@ -832,9 +924,11 @@ __global__ __launch_bounds__(256) void k_xi_var(const double* __restrict__ varpa
                                                 double* __restrict__ xs) {
   const XiState L = xi_state_layout();
   if (threadIdx.x != 0) return;
-  double v = 0.0;
-  for (int g = 0; g < kXiGroups; ++g) v += varpart[g];
+  double v = 0.0, vg = 0.0;
+  for (int g = 0; g < kXiGroups; g += 3) v += varpart[g];
+  for (int g = 1; g < kXiGroups; g += 3) vg += varpart[g];
   xs[L.scal] = v;
+  xs[L.scal + 4] = vg;
 }
 
 // zero W0 for shells without rows of their own kappa bins (set once per PDF)
@@ -1064,7 +1158,7 @@ __global__ void k_xi_info(const double* __restrict__ xs, double* __restrict__ ou
   if (t < 5) out[t] = (double)meta[t];
   if (t == 5) out[5] = xs[L.scal];
   if (t == 6) out[6] = xs[L.scal + 1];
-  if (t == 7) out[7] = 0.0;
+  if (t == 7) out[7] = xs[L.scal + 4];
 }
 
 inline size_t al256(size_t n) { return (n + 255) & ~size_t(255); }
@@ -1109,7 +1203,7 @@ XiSortWs xi_sort_ws(long long n, long long n_cells) {
 // rows workspace: the run keys and Hx per run (read again by the grid stage),
 // the multi-part partials, first-moment parts
 struct XiRowsWs {
-  size_t ukey, hx, hp, lp, lc, occ, varpart, total;
+  size_t ukey, hx, hp, lp, lc, occ, varpart, wp, wrun, phic, v0, total;
 };
 XiRowsWs xi_rows_ws(long long n_runs, long long n_parts) {
   XiRowsWs W;
@@ -1122,6 +1216,10 @@ XiRowsWs xi_rows_ws(long long n_runs, long long n_parts) {
   W.lc = take((size_t)(kRefMaxShells + 1) * kXiLxChunks * 2 * kXiNk * sizeof(double));
   W.occ = take((kXiKc + kXiGc) * sizeof(int));
   W.varpart = take(kXiGroups * sizeof(double));
+  W.wp = take((size_t)std::max<long long>(n_parts, 1) * sizeof(double));
+  W.wrun = take((size_t)std::max<long long>(n_runs, 1) * sizeof(double));
+  W.phic = take((size_t)std::max<long long>(n_runs, 1) * sizeof(double));
+  W.v0 = take((size_t)kRefMaxShells * kXiNk * sizeof(double));
   W.total = o;
   return W;
 }
@@ -1149,6 +1247,13 @@ XiGridWs xi_grid_ws(int n_z, int nP, int nQ) {
 }
 
 }  // namespace
+
+XiRowsView xi_rows_view(const void* rows_ws, long long n_runs, long long n_parts) {
+  const XiRowsWs W = xi_rows_ws(n_runs, n_parts);
+  const char* b = static_cast<const char*>(rows_ws);
+  return XiRowsView{(const int*)(b + W.ukey), (const double*)(b + W.phic),
+                    (const double*)(b + W.wrun), (const double*)(b + W.v0)};
+}
 }  // namespace sgl
 
 using namespace sgl;
@@ -1289,6 +1394,10 @@ sgl_status sgl_ref_xi_rows(const void* tables, int32_t flags, int64_t n_points, 
   double* Lc = (double*)(rb + W.lc);
   int* occ = (int*)(rb + W.occ);
   double* varpart = (double*)(rb + W.varpart);
+  double* Wp = (double*)(rb + W.wp);
+  double* wrun = (double*)(rb + W.wrun);
+  double* phic = (double*)(rb + W.phic);
+  double* v0 = (double*)(rb + W.v0);
   int* ukey_keep = (int*)(rb + W.ukey);
   const int nr = (int)n_runs;
   if (nr > 0)
@@ -1296,10 +1405,12 @@ sgl_status sgl_ref_xi_rows(const void* tables, int32_t flags, int64_t n_points, 
   if (n_parts > 0)
     hipLaunchKernelGGL(k_xi_walk, dim3(nblk(n_parts * 64)), dim3(256), 0, st, rlen, nr, rstart,
                        nparts, poff, hoff, (int)n_parts, vout, kap, gam, w, xi_pts, xi_cells, JT,
-                       Hx, Hp, Lp);
-  if (nr > 0)
+                       Hx, Hp, Lp, Wp);
+  if (nr > 0) {
     hipLaunchKernelGGL(k_xi_combine, dim3(nblk((long long)nr * kXiVals)), dim3(256), 0, st, nr,
                        nparts, hoff, Hp, Hx);
+    hipLaunchKernelGGL(k_xi_wrun, dim3(nblk(nr)), dim3(256), 0, st, nr, nparts, poff, Wp, wrun);
+  }
   hipLaunchKernelGGL(k_xi_zero, dim3(1), dim3(256), 0, st, (double*)occ,
                      (long long)((kXiKc + kXiGc) / 2 + 1));
   if (nr > 0) hipLaunchKernelGGL(k_xi_occ, dim3(nblk(nr)), dim3(256), 0, st, nr, ukey, occ);
@@ -1310,6 +1421,8 @@ sgl_status sgl_ref_xi_rows(const void* tables, int32_t flags, int64_t n_points, 
                      xs);
   hipLaunchKernelGGL(k_xi_group, dim3(kXiGroups), dim3(256), 0, st, T.cosmo, T.slices, ukey, Hx,
                      xs);
+  hipLaunchKernelGGL(k_xi_loading, dim3(kRefMaxShells), dim3(256), 0, st, T.cosmo, T.slices, ukey,
+                     Hx, wrun, xs, phic, v0);
   hipLaunchKernelGGL(k_xi_keep, dim3(1), dim3(256), 0, st, xs);
   hipLaunchKernelGGL(k_xi_zero, dim3(nblk((long long)kRefMaxShells * kXiKc * kXiNk)), dim3(256), 0,
                      st, xs + L.W0, (long long)kRefMaxShells * kXiKc * kXiNk);

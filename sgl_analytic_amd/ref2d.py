@@ -33,6 +33,8 @@ PTS_PER_SIG_KAP, PTS_PER_SIG_GAM, XI_REACH = 72.0, 36.0, 24.0
 TAIL_KAP_MAX, TAIL_GAM_MAX, TAIL_PPP_G = 0.99, 0.8, 12.0
 PHI_TOL = 1e-8
 N_XI_WIDE = 500
+N_XI_WIDE_VP = 4000              # pipeline.py:236 (positive closure)
+XI_COARSE = 101                  # Lam xi_coarse
 OUT_GRID_MAX = 3.4e7             # sgl_full.py:60 (geometric output axis above it)
 E_BLOCK_BYTES = 4 << 30          # Lambda operand rows per call (k_kappa row blocks)
 INGREDIENTS_2D = ("ell", "fil", "sub", "bias")  # ingredient sets this chain computes
@@ -74,6 +76,22 @@ class XiTerm:
 
 
 @dataclass
+class VpTerm:
+    """The positive closure on the device (sgl_ref_vp_*): the per-PDF state
+    (cells, groups, rates, transitions), its sizes, the calibrated eta and the
+    two clustering targets."""
+    state: torch.Tensor
+    n_cells: int            # (shell, bin) runs: the state's allocation
+    cells: int              # the shells' cells (bins with nonzero sums)
+    n_groups: int
+    n_z: int
+    eta: float
+    target_kappa: float
+    target_gamma: float
+    n_bins: int
+
+
+@dataclass
 class Population:
     """Occupied fine bins on the device ([w][kappa centroid][gamma centroid],
     REF_FINE_BINS each) and the builder's bookkeeping (sgl_full.py:1535-1808,
@@ -88,6 +106,7 @@ class Population:
     N_fil: float
     timings: dict = field(default_factory=dict)
     xi: XiTerm = None
+    vp: VpTerm = None
 
 
 class _Stages:
@@ -119,7 +138,7 @@ def _xi_info(L, state: torch.Tensor) -> np.ndarray:
 
 def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
           edge: str = "r200", timings: dict = None, sub: bool = False,
-          keep_sub: bool = False, bias: bool = False) -> Population:
+          keep_sub: bool = False, bias: bool = False, vp: bool = False) -> Population:
     """build_population(..., kap_floor=1e-8, trunc=edge, zint="midpoint")
     + Lam's flattening (sgl_full.py:1535, 2742-2755), on the device.  edge
     "rvir" truncates every halo at its virial radius (sgl_full.py:1876-1880).
@@ -129,13 +148,20 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
     bias: the xi_lin clustering term (clustering="xilin", the pipeline's
     default for the "bias" ingredient, pipeline.py:135-207): every point's
     lens distance deposited per shell and coarse bin, the rows compressed,
-    and sigma_kappa with the clustering variance (Lam.sigma)."""
+    and sigma_kappa with the clustering variance (Lam.sigma).
+    vp: the positive closure on that population (xi_closure="vp",
+    pos_closure.install, pos_closure.py:708-733): the per-shell cells, their
+    loadings and groups, the line-of-sight chain, eta calibrated on the
+    clustering variance (brentq in ln eta, as pos_closure.py:575-591) and the
+    rates in interval form; needs bias."""
     L = _lib.load()
     dev = torch.device("cuda")
     if edge not in ("r200", "rvir"):
         raise ValueError("edge must be 'r200' or 'rvir'")
     if sub and edge != "r200":
         raise NotImplementedError("subhalos are built with the r200 edge only (clump_edge r200)")
+    if vp and not bias:
+        raise ValueError("the positive closure needs the xi_lin term (bias)")
     flags = ((FLAG_ELL if ell else 0) | (FLAG_FILAMENTS if fil else 0)
              | (_lib.FLAG_EDGE_RVIR if edge == "rvir" else 0)
              | (_lib.FLAG_SUBHALOS if sub else 0)
@@ -186,6 +212,14 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
                                    _ptr(G), _ptr(W), _ptr(key), _ptr(status), st),
               "sgl_ref_cells_emit")
     clock.mark("cells_emit")
+    vruns = None
+    if vp:
+        # the cells keyed by (shell, fine bin): the per-shell pieces
+        vruns = torch.empty(L.sgl_ref_vp_runs_bytes(n), dtype=torch.uint8, device=dev)
+        ncell_d = torch.zeros(1, dtype=torch.int64, device=dev)
+        check(L.sgl_ref_vp_runs(_ptr(T), flags, n, _ptr(counts), _ptr(key), _ptr(K), _ptr(G),
+                                _ptr(W), _ptr(vruns), _ptr(ncell_d), st), "sgl_ref_vp_runs")
+        clock.mark("vp_runs")
     ws = torch.empty(L.sgl_ref_bin_bytes(n), dtype=torch.uint8, device=dev)
     pop = torch.empty(3 * REF_FINE_BINS, dtype=torch.float64, device=dev)
     npop = torch.zeros(1, dtype=torch.int32, device=dev)
@@ -215,6 +249,12 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
                     n_z=int(info[_lib.XI_INFO_NZ]), nP=int(info[_lib.XI_INFO_NP]),
                     nQ=int(info[_lib.XI_INFO_NQ]), var=float(info[_lib.XI_INFO_VAR]))
         clock.mark("xi_rows")
+    vpt = None
+    if vp:
+        vpt = _vp_setup(L, T, flags, n, vruns, int(ncell_d.item()), pop, npop, xi, n_z,
+                        float(info[_lib.XI_INFO_SHEAR]))
+        del vruns
+        clock.mark("vp_setup")
     del K, G, W, key
     sc = scal.cpu().numpy()
     # bookkeeping of the info record (sgl_full.py:2039, 1992): expected counts
@@ -222,10 +262,106 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
     var = sc[0] + (xi.var if xi is not None else 0.0)
     out = Population(pop=pop, npop_dev=npop, n_bins=int(npop.item()), sigma=math.sqrt(var),
                      kbar=float(sc[1]), n_points=n, N_expl=lay_rows[0],
-                     N_fil=lay_rows[1] if fil else 0.0, xi=xi)
+                     N_fil=lay_rows[1] if fil else 0.0, xi=xi, vp=vpt)
+    if vpt is not None:
+        vpt.n_bins = out.n_bins
     if keep_sub:
         out.sub_tables = subw
     return out
+
+
+def _vp_setup(L, T, flags, n_points, vruns, n_cells, pop, npop, xi, n_z, shear_var) -> VpTerm:
+    """The positive closure's set-up on the device, then eta: the root of
+    variance(eta) = the clustering variance in ln eta, bracketed from
+    [1/1.5, 1.5] widened by 1.5 per side inside [1e-3, 1e4] and refined by
+    brentq to 2e-12 (pos_closure.Closure.calibrate, pos_closure.py:575-591);
+    each variance is one device evaluation (rates, per-shell sums, chain)."""
+    from scipy.optimize import brentq
+    dev = T.device
+    st = _stream()
+    state = torch.empty(L.sgl_ref_vp_state_bytes(n_cells), dtype=torch.uint8, device=dev)
+    meta = torch.zeros(16, dtype=torch.int64, device=dev)
+    check(L.sgl_ref_vp_setup(_ptr(T), flags, n_points, _ptr(vruns), n_cells, _ptr(pop),
+                             _ptr(npop), _ptr(xi.state), _ptr(xi.rows), xi.n_runs, xi.n_parts,
+                             _ptr(state), _ptr(meta), st), "sgl_ref_vp_setup")
+    m = meta.cpu().numpy()
+    bad = int(m[_lib.VP_META_STATUS])
+    if bad & _lib.VP_STATUS_BINS:
+        raise _lib.SglError("positive closure: the per-shell bins disagree with the pooled lattice")
+    if bad & _lib.VP_STATUS_MODE:
+        raise NotImplementedError("positive closure: a shell without a kappa clustering mode")
+    if bad & _lib.VP_STATUS_TRANSITION:
+        raise _lib.SglError("positive closure: transition balancing did not converge")
+    target = xi.var
+    vd = torch.empty(1, dtype=torch.float64, device=dev)
+
+    def variance(eta):
+        check(L.sgl_ref_vp_variance(_ptr(state), n_cells, n_z, float(eta), _ptr(vd), st),
+              "sgl_ref_vp_variance")
+        return float(vd.item())
+
+    def fun(le):
+        return variance(math.exp(le)) - target
+
+    lo, hi = 1e-3, 1e4
+    a, b = -math.log(1.5), math.log(1.5)
+    fa, fb = fun(a), fun(b)
+    while fa * fb > 0:
+        if fa > 0 and a > math.log(lo):
+            b, fb = a, fa
+            a -= math.log(1.5)
+            fa = fun(a)
+        elif fb < 0 and b < math.log(hi):
+            a, fa = b, fb
+            b += math.log(1.5)
+            fb = fun(b)
+        else:
+            raise ValueError(f"positive closure: kappa clustering target {target:.3e} outside "
+                             f"[{lo}, {hi}]")
+    eta = float(math.exp(brentq(fun, a, b, xtol=2e-12)))
+    check(L.sgl_ref_vp_rates(_ptr(state), n_cells, n_z, eta, _ptr(meta), st), "sgl_ref_vp_rates")
+    m = meta.cpu().numpy()
+    if int(m[_lib.VP_META_STATUS]) & _lib.VP_STATUS_RATE:
+        raise ValueError("vp closure: a relative rate variance beyond the 48-state cap")
+    return VpTerm(state=state, n_cells=n_cells, cells=int(m[_lib.VP_META_CELLS]),
+                  n_groups=int(m[_lib.VP_META_GROUPS]), n_z=n_z,
+                  eta=eta, target_kappa=target, target_gamma=shear_var, n_bins=0)
+
+
+def vp_qxi(pop: Population, kv: np.ndarray, qv: np.ndarray) -> torch.Tensor:
+    """The positive closure's ln R - shear_var k_gamma^2 / 4 on the tensor grid
+    kv x qv, Im ln R unwrapped (pos_closure.py:690-705): [nk][nq][2]."""
+    L = _lib.load()
+    v = pop.vp
+    dev = pop.pop.device
+    kd = torch.tensor(np.asarray(kv, float), dtype=torch.float64, device=dev)
+    qd = torch.tensor(np.asarray(qv, float), dtype=torch.float64, device=dev)
+    nk, nq = kd.numel(), qd.numel()
+    ws = torch.empty(L.sgl_ref_vp_lnr_bytes(v.n_z, v.n_bins, nk, nq), dtype=torch.uint8,
+                     device=dev)
+    out = torch.empty((nk, nq, 2), dtype=torch.float64, device=dev)
+    st = _stream()
+    check(L.sgl_ref_vp_lnr(_ptr(v.state), v.n_cells, v.n_z, _ptr(pop.pop), v.n_bins, _ptr(kd), nk,
+                           _ptr(qd), nq, _ptr(ws), _ptr(out), st), "sgl_ref_vp_lnr")
+    del ws
+    check(L.sgl_ref_vp_qxi(_ptr(out), nk, nq, _ptr(qd), v.target_gamma, st), "sgl_ref_vp_qxi")
+    return out
+
+
+def vp_psi(pop: Population, k_k: np.ndarray, k_g: np.ndarray, n: int, nk: int) -> torch.Tensor:
+    """qxi on the closure's coarse grid (k_kappa = k_k_max s_k^2 on nk nodes,
+    k_gamma = k_g_max s^2 on n nodes) and its not-a-knot bicubic interpolant on
+    the full grid (Lam._Q_interpolated, sgl_full.py:2953-3022): [nk_k][nk_g][2]."""
+    L = _lib.load()
+    dev = pop.pop.device
+    s = np.linspace(0.0, 1.0, n)
+    s_k = s if nk == n else np.linspace(0.0, 1.0, nk)
+    coarse = vp_qxi(pop, float(k_k[-1]) * s_k ** 2, float(k_g[-1]) * s ** 2)
+    ws = torch.empty(L.sgl_ref_vp_spline_bytes(nk, n, k_k.size), dtype=torch.uint8, device=dev)
+    psi = torch.empty((k_k.size, k_g.size, 2), dtype=torch.float64, device=dev)
+    check(L.sgl_ref_vp_spline(_ptr(coarse), nk, n, k_k.size, k_g.size, _ptr(ws), _ptr(psi),
+                              _stream()), "sgl_ref_vp_spline")
+    return psi
 
 
 def _row_table(T: torch.Tensor, flags: int, n_z: int):
@@ -257,6 +393,12 @@ def lam_axis(pop: Population, k: np.ndarray, axis: int) -> np.ndarray:
     out = torch.empty(2 * kd.numel(), dtype=torch.float64, device="cuda")
     check(L.sgl_ref_lam_axis(_ptr(pop.pop), _ptr(pop.npop_dev), _ptr(kd), kd.numel(), axis,
                              _ptr(out), _stream()), "sgl_ref_lam_axis")
+    if pop.vp is not None:
+        zero = np.zeros(1)
+        q = vp_qxi(pop, k, zero) if axis == 0 else vp_qxi(pop, zero, k)
+        o = out.cpu().numpy()
+        qq = q.reshape(-1, 2).cpu().numpy()
+        return (o[0::2] + qq[:, 0]) + 1j * (o[1::2] + qq[:, 1])
     if pop.xi is not None:
         x = pop.xi
         check(L.sgl_ref_xi_axis(_ptr(x.tables), x.flags, _ptr(x.state), _ptr(kd), kd.numel(), axis,
@@ -317,14 +459,24 @@ def grid_plan(pop: Population, tol: float = PHI_TOL) -> dict:
     grid["nk_k"] = int(math.ceil(grid["k_k_max"] / dk0[0])) + 1
     grid["nk_g"] = int(math.ceil(grid["k_g_max"] / dk0[1])) + 1
     grid.update(phi_at_k_cut=phi_k, phi_at_g_cut=phi_g, window=0.0)
+    if pop.vp is not None:
+        # the closure's coarse grid follows the box, per axis (sgl_full.py:226-238)
+        n = int(min(401, max(XI_COARSE, 1 + math.ceil(12.65 * math.sqrt(grid["k_g_max"] * sigma)))))
+        nk = int(min(4001, max(n, 1 + math.ceil(66.7 * math.sqrt(grid["k_k_max"] * sigma)))))
+        grid.update(xi_coarse=float(n), xi_coarse_k=float(nk))
+        if max(phi_k, phi_g) > tol:
+            grid["window"] = 1.0
     if grid["n_kap"] * grid["n_gam"] > OUT_GRID_MAX:
         raise NotImplementedError("the geometric output axis of the narrowest low-z_s grids "
                                   "(n_kap * n_gam > 3.4e7, sgl_full.py:316-324) is not built")
     return grid
 
 
-def phi_grid(pop: Population, k_k: np.ndarray, k_g: np.ndarray) -> torch.Tensor:
-    """Phi = exp(Lambda) on the tensor grid as [2 nk_k][nk_g] (rows re, im)."""
+def phi_grid(pop: Population, k_k: np.ndarray, k_g: np.ndarray,
+             grid: dict = None) -> torch.Tensor:
+    """Phi = exp(Lambda + clustering term) on the tensor grid as [2 nk_k][nk_g]
+    (rows re, im); with the positive closure the grid plan gives its coarse
+    grid and whether the positive-definite window applies."""
     L = _lib.load()
     dev = torch.device("cuda")
     nk, ng = k_k.size, k_g.size
@@ -332,7 +484,10 @@ def phi_grid(pop: Population, k_k: np.ndarray, k_g: np.ndarray) -> torch.Tensor:
     kg = torch.tensor(k_g, dtype=torch.float64, device=dev)
     lda = (pop.n_bins + 63) // 64 * 64
     blk = max(1, min(nk, 65535, E_BLOCK_BYTES // (16 * lda)))
-    psi = xi_psi(pop, k_k, k_g) if pop.xi is not None else None
+    if grid is not None and pop.vp is not None:
+        psi = vp_psi(pop, k_k, k_g, int(grid["xi_coarse"]), int(grid["xi_coarse_k"]))
+    else:
+        psi = xi_psi(pop, k_k, k_g) if pop.xi is not None else None
     ws = torch.empty(L.sgl_ref_phi_bytes(blk, ng, pop.n_bins) // 8, dtype=torch.float64, device=dev)
     for lo in range(0, nk, blk):
         hi = min(nk, lo + blk)
@@ -341,6 +496,8 @@ def phi_grid(pop: Population, k_k: np.ndarray, k_g: np.ndarray) -> torch.Tensor:
                             _ptr(kg), ng, _ptr(psi[lo]) if psi is not None else None, _ptr(ws),
                             _ptr(phi[2 * lo:]), _stream()),
               "sgl_ref_phi")
+    if grid is not None and grid.get("window", 0.0):
+        check(L.sgl_ref_window(_ptr(phi), nk, ng, _stream()), "sgl_ref_window")
     return phi
 
 
@@ -385,7 +542,7 @@ def invert(pop: Population, xi_out: np.ndarray, tol: float = PHI_TOL,
     k_k = np.linspace(0.0, g["k_k_max"], g["nk_k"])
     k_g = np.linspace(0.0, g["k_g_max"], g["nk_g"])
     clock.mark("reach")
-    phi = phi_grid(pop, k_k, k_g)
+    phi = phi_grid(pop, k_k, k_g, g)
     clock.mark("phi")
     nk, ng, nkap, ngam = g["nk_k"], g["nk_g"], g["n_kap"], g["n_gam"]
     ws = torch.empty(L.sgl_ref_p3_bytes(nk, ng, nkap, ngam) // 8, dtype=torch.float64, device=dev)
@@ -396,13 +553,14 @@ def invert(pop: Population, xi_out: np.ndarray, tol: float = PHI_TOL,
     clock.mark("p3")
     del phi, ws
     g_def = min(BOX_REACH * pop.sigma, g["gam_max"])
-    xw = np.linspace(g["xi_min"], g["xi_max"], N_XI_WIDE)
+    n_wide = N_XI_WIDE_VP if pop.vp is not None else N_XI_WIDE
+    xw = np.linspace(g["xi_min"], g["xi_max"], n_wide)
     xwd = torch.tensor(xw, dtype=torch.float64, device=dev)
-    pw = torch.empty(N_XI_WIDE, dtype=torch.float64, device=dev)
+    pw = torch.empty(n_wide, dtype=torch.float64, device=dev)
     check(L.sgl_ref_arcs(_ptr(p3), g["kap_min"], g["kap_max"], nkap, g["gam_max"], ngam, g_def,
-                         _ptr(xwd), N_XI_WIDE, None, _ptr(pw), st), "sgl_ref_arcs")
+                         _ptr(xwd), n_wide, None, _ptr(pw), st), "sgl_ref_arcs")
     mom = torch.empty(5, dtype=torch.float64, device=dev)
-    check(L.sgl_ref_moments(_ptr(xwd), _ptr(pw), N_XI_WIDE, _ptr(mom), st), "sgl_ref_moments")
+    check(L.sgl_ref_moments(_ptr(xwd), _ptr(pw), n_wide, _ptr(mom), st), "sgl_ref_moments")
     xo = np.ascontiguousarray(xi_out, dtype=np.float64)
     xod = torch.tensor(xo, dtype=torch.float64, device=dev)
     P = torch.empty(xo.size, dtype=torch.float64, device=dev)
@@ -416,10 +574,12 @@ def invert(pop: Population, xi_out: np.ndarray, tol: float = PHI_TOL,
 
 def run(zs: float, cosmo: dict, ell: bool = False, fil: bool = False, xi_out=None,
         edge: str = "r200", timings: dict = None, sub: bool = False,
-        bias: bool = False) -> Result:
+        bias: bool = False, vp: bool = False) -> Result:
     """One PDF on the reference's chain; xi_out None: the CLI body and tail
-    grids linspace(-1, 1, 1601) and linspace(1.005, 9, 1600) (pipeline.py:211-215)."""
-    pop = build(cosmo, zs, ell=ell, fil=fil, edge=edge, timings=timings, sub=sub, bias=bias)
+    grids linspace(-1, 1, 1601) and linspace(1.005, 9, 1600) (pipeline.py:211-215);
+    vp: the positive clustering closure (needs bias)."""
+    pop = build(cosmo, zs, ell=ell, fil=fil, edge=edge, timings=timings, sub=sub, bias=bias,
+                vp=vp)
     if xi_out is None:
         xi_out = np.concatenate([np.linspace(-1.0, 1.0, 1601), np.linspace(1.005, 9.0, 1600)])
     return invert(pop, xi_out, timings=timings)

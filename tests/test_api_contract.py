@@ -49,8 +49,6 @@ def test_paths_not_computed_here_raise():
     with patch("sgl_analytic_amd.api._solver") as solver:
         with pytest.raises(ValueError):
             api.run(1.0, "full", cosmo={}, xi_out=np.array([0.0, 1.0]), closure="lognormal")
-        with pytest.raises(NotImplementedError):    # the positive closure is not built here
-            api.run(1.0, "full", cosmo={}, xi_out=np.array([0.0, 1.0]), closure="vp")
         with pytest.raises(NotImplementedError):    # subhalos pair with the r200 edge
             api.run(1.0, "+sub", cosmo={}, xi_out=np.array([0.0, 1.0]), edge="rvir")
         with pytest.raises(ValueError):     # vp needs the bias ingredient, as in the reference

@@ -49,6 +49,6 @@ def test_cli_rejects_bad_input(tmp_path):
             cli.main(["--zs", "1", "--Om", "-1", "--out", str(tmp_path / "x.json")])
         with pytest.raises(SystemExit):
             cli.main(["--zs", "1", "--nproc", "0", "--out", str(tmp_path / "x.json")])
-    # options not built on the reference chain: a usage error, not a silent other model
+    # subhalos pair with the r200 edge only: a usage error, not a silent other model
     with pytest.raises(SystemExit):
-        cli.main(["--zs", "1", "--closure", "vp", "--out", str(tmp_path / "x.json")])
+        cli.main(["--zs", "1", "--edge", "rvir", "--out", str(tmp_path / "x.json")])

@@ -23,9 +23,9 @@ def test_generate_pdf_lnmu_runs_the_reference_chain():
     assert on_body.sum() > 321
     mass = np.trapezoid(p, x)
     assert 0.99 < mass < 1.01
-    # the positive closure is not built on this chain
-    with pytest.raises(NotImplementedError):
-        sa.generate_pdf_lnmu(1.0, lnmu=x[:10], closure="vp")
+    # the positive closure needs the bias ingredient, as in the reference
+    with pytest.raises(ValueError):
+        sa.generate_pdf_lnmu(1.0, lnmu=x[:10], closure="vp", config="-bias")
 
 
 def test_generate_pdf_jacobian_on_device():
