@@ -241,16 +241,21 @@ def main():
 def ref_chain_leg(log, zs=1.0, reps=3):
     """generate_pdf_lnmu's chain (the reference's 2-D chain) at z_s = 1, Planck,
     on the default ln mu grid linspace(-1, 9, 3201): seconds per PDF and the
-    stage split, for halos, halos + ellipticity + filaments, and with
-    subhalos on top (the ingredients built on it), best of ``reps`` after one
-    warm-up."""
+    stage split for the ingredient ladder up to the default config="full"
+    (halos + ellipticity + filaments + subhalos + the xi_lin term) and for
+    "full" with the positive closure (closure="vp"), best of ``reps`` after
+    one warm-up; then "full" and "full vp" once more on the reference's own
+    body and tail grids against its stored pdf_zs1.json / pdf_zs1_vp.json
+    (max |dP| over the peak, the reference's gate being 1e-9)."""
     import torch
     from sgl_analytic_amd import api, ref2d
     cp = api.check_cosmology({})
     x = np.linspace(-1.0, 9.0, 3201)
     out = {}
+    full = dict(ell=True, fil=True, sub=True, bias=True)
     for name, kw in (("halo", {}), ("ell+fil", dict(ell=True, fil=True)),
-                     ("ell+fil+sub", dict(ell=True, fil=True, sub=True))):
+                     ("ell+fil+sub", dict(ell=True, fil=True, sub=True)),
+                     ("full", full), ("full vp", dict(full, vp=True))):
         ref2d.run(zs, cp, xi_out=x, **kw)
         torch.cuda.synchronize()
         best, stages, res = None, None, None
@@ -268,11 +273,22 @@ def ref_chain_leg(log, zs=1.0, reps=3):
                          k_grid=[g["nk_k"], g["nk_g"]], out_grid=[g["n_kap"], g["n_gam"]],
                          sigma_kappa=res.sigma)
         log(f"[bench] reference chain z_s={zs} {name}: {best * 1e3:.1f} ms/PDF", out[name])
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "reference_pdfs.npz"))
+    xb = np.concatenate([np.linspace(-1.0, 1.0, 1601), np.linspace(1.005, 9.0, 1600)])
+    for name, kw, key in (("full", full, "zs1"), ("full vp", dict(full, vp=True), "zs1_vp")):
+        res = ref2d.run(zs, cp, xi_out=xb, **kw)
+        P0 = np.concatenate([gold[key + "_P_s"], gold[key + "_P_s_tail"]])
+        out[name]["vs_stored_reference"] = dict(
+            fixture=f"tests/reference/pdf_{key}.json",
+            max_abs_dP_over_peak=float(np.max(np.abs(res.P - P0)) / gold[key + "_P_s"].max()))
+        log(f"[bench] {name} vs the stored reference", out[name]["vs_stored_reference"])
     out["workload"] = ("generate_pdf_lnmu(z_s=1.0), Planck, default ln mu grid (3201 nodes), "
                        "reference grids (67 shells x 99 masses, adaptive cells, 961x300 lattice)")
-    out["reference_cpu_seconds"] = {"note": "the reference's own record of a z_s = 1 full-config "
-                                    "run: 15.41 s (build 11.52, lam 0.30, invert 3.59), nproc=4, "
-                                    "Apple arm64 (tests/reference/pdf_zs1.json t_stages)"}
+    out["reference_cpu_seconds"] = {"note": "the reference's own records of its z_s = 1 "
+                                    "full-config runs, nproc=4, Apple arm64: gauss 15.41 s "
+                                    "(build 11.52, lam 0.30, invert 3.59; tests/reference/"
+                                    "pdf_zs1.json t_stages), vp 34.91 s (build 11.46, lam 2.86, "
+                                    "invert 20.59; pdf_zs1_vp.json)"}
     return out
 
 
