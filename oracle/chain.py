@@ -134,6 +134,19 @@ def slice_nodes(cos: Cosmology, zs: float, n_z: int):
     return z, mid, chis / n_z, edges
 
 
+def p1d(cos: Cosmology, kq: np.ndarray, Rs: float = 20.0):
+    """The 1-D power of the top-hat-smoothed linear field along the line of
+    sight, (1/2 pi) int_{k}^{inf} K P0(K) W^2(K Rs) dK, as BiasField1D forms it
+    (sgl_full.py:2514-2530): a running trapezoid on 20000 geometric nodes from
+    k_1 / 2 to 200 / Rs, interpolated at kq."""
+    Kg = np.geomspace(kq[0] * 0.5, 200.0 / Rs, 20000)
+    x = Kg * Rs
+    W = np.where(x < 1e-2, 1 - x ** 2 / 10 * (1 - x ** 2 / 28), 3 * (np.sin(x) - x * np.cos(x)) / x ** 3)
+    f = Kg * cos.P0(Kg) * W ** 2
+    cum = np.concatenate([[0.0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(Kg))])
+    return (cum[-1] - np.interp(kq, Kg, cum)) / (2 * np.pi)
+
+
 def slice_covariance(cos: Cosmology, chi_edges: np.ndarray, chis: float, Rs: float = 20.0):
     """sgl_full.py:2514-2538 (BiasField1D, bias_window=1) on the given
     comoving segments: exact mode sum over k_q = 2 pi q / L, L = 1.05 chi_s."""
@@ -141,12 +154,7 @@ def slice_covariance(cos: Cosmology, chi_edges: np.ndarray, chis: float, Rs: flo
     L = 1.05 * chis
     Nmax = max(4, int(np.floor(L / Rs)))
     kq = 2 * np.pi * np.arange(1, Nmax + 1) / L
-    Kg = np.geomspace(kq[0] * 0.5, 200.0 / Rs, 20000)
-    x = Kg * Rs
-    W = np.where(x < 1e-2, 1 - x ** 2 / 10 * (1 - x ** 2 / 28), 3 * (np.sin(x) - x * np.cos(x)) / x ** 3)
-    f = Kg * cos.P0(Kg) * W ** 2
-    cum = np.concatenate([[0.0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(Kg))])
-    P1D = (cum[-1] - np.interp(kq, Kg, cum)) / (2 * np.pi)
+    P1D = p1d(cos, kq, Rs)
     Lh = b - a
     rw = np.sqrt(2 * P1D / L)
     cq = rw[:, None] * (np.sin(kq[:, None] * b) - np.sin(kq[:, None] * a)) / (kq[:, None] * Lh)

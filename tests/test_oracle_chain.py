@@ -107,16 +107,13 @@ def test_slice_covariance_against_numerical_projection():
     C = chain.slice_covariance(cos, edges, chis)
     assert np.allclose(C, C.T)
     assert np.all(np.linalg.eigvalsh(C) > -1e-12 * np.max(C))
-    # independent: rebuild P1D at the modes and project numerically
+    # the slice projection: P1D (pinned on its own against a perpendicular
+    # quadrature, tests/test_oracle_pins.py) at the modes, the slice averages
+    # by Gauss-Legendre instead of the closed form
     Rs, L = 20.0, 1.05 * chis
     nq = max(4, int(np.floor(L / Rs)))
     kq = 2 * np.pi * np.arange(1, nq + 1) / L
-    Kg = np.geomspace(kq[0] * 0.5, 200.0 / Rs, 20000)
-    x = Kg * Rs
-    W = np.where(x < 1e-2, 1 - x ** 2 / 10 * (1 - x ** 2 / 28), 3 * (np.sin(x) - x * np.cos(x)) / x ** 3)
-    f = Kg * cos.P0(Kg) * W ** 2
-    cum = np.concatenate([[0.0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(Kg))])
-    P1D = (cum[-1] - np.interp(kq, Kg, cum)) / (2 * np.pi)
+    P1D = chain.p1d(cos, kq, Rs)
     g, gw = np.polynomial.legendre.leggauss(64)
     S = np.zeros((nz, nq), complex)
     for i in range(nz):
