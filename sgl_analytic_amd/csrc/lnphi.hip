@@ -15,7 +15,9 @@
 //   k_rowf / k_cols   per-row and per-column records of the slice lattice;
 //   k_bin_cells       every cell with the FP32 lens profile (a form without
 //                     cancellation at x = 1) into a per-slice LDS histogram of fine
-//                     signed-log jump bins (w, w xi, w xi^2, beta w), then
+//                     signed-log jump bins (w and its centred first and second
+//                     xi moments, beta w; 64-bit fixed point, so the sums do
+//                     not depend on the order of the atomics), then
 //                     the slice's fine partials and its coarse two-point
 //                     moments (beta w, beta w xi, beta w xi^2 per kCoarse
 //                     fine bins);
@@ -457,58 +459,193 @@ __global__ void k_sum_slices(const double* __restrict__ part, int B, int n_z_max
 }
 
 // ----------------------------------------------------------- binned path --
-// Bin index from xi with the FP32 hardware log: which of two neighbouring
-// bins an edge case lands in does not matter, every bin keeps its exact
-// centroid and variance.
-__device__ __forceinline__ int fine_bin(double xi) {
-  constexpr float kPosScale = (float)(kFinePos / 31.032218497590705);   // ln(30 / 1e-12)
-  constexpr float kNegScale = (float)(kFineNeg / 21.82187812561452);    // ln(30 / 1e-8)
-  constexpr float kLnLo = -27.631021115928547f;                          // ln 1e-12
-  constexpr float kLnNegLo = -18.420680743952367f;                       // ln 1e-8
-  if (xi > 0.0) {
-    float lx = log_hw(fmaxf((float)xi, 1e-37f));
-    int k = (int)((lx - kLnLo) * kPosScale);
-    return min(max(k, 0), kFinePos - 1);
-  }
-  float lx = log_hw(fmaxf((float)(-xi), 1e-37f));
-  int k = (int)((lx - kLnNegLo) * kNegScale);
-  return kFinePos + min(max(k, 0), kFineNeg - 1);
-}
-
-
 // Fine histogram of a slice in LDS, moment-major (a bin's moment at an 8-byte
-// stride, so the lanes of a wave spread over all banks), four FP64 moments:
-// w, w xi, w xi^2 (centroid and in-bin variance of the exponent's records)
-// and beta w (the two-point weight).  The slice's coarse two-point moments
-// are formed from them at the end, kCoarse fine bins per coarse bin, each fine
-// bin's beta w placed at the bin's w centroid m_f with its w variance v_f:
-// sum beta w, sum beta w m_f, sum beta w (m_f^2 + v_f).  Within one fine bin
-// (3.1 % relative width) beta and xi are taken as uncorrelated, which moves
-// a coarse record's centroid by at most 1.6 % of xi.  A cell costs four LDS
-// atomics on fine bins, which neighbouring lanes rarely share.  (FP32 LDS
-// atomic adds measured four times slower here than FP64 ones.)
+// stride, so the lanes of a wave spread over all banks), four moments per
+// fine bin f, each a 64-bit fixed-point sum:
+//   A = sum ws g_f,  Bm = sum ws g_f d,  C = sum ws g_f d^2,  E = sum bw g_f,
+// ws the tilted weight w e^{-xi}, bw = beta ws the two-point weight, d the
+// cell's place in the bin (|d| <= 0.99, below), g_f a tilt bound: 1 on the
+// positive side, where ws <= w, and a power of two <= e^{-|xi|_max} on the
+// negative side, where ws = w e^{|xi|}; so ws g_f <= w.
+//
+// Why fixed point: an FP64 sum of atomics depends on the order in which the
+// lanes and waves arrive, which changes from run to run.  Here every deposit
+// is rounded once, to an integer on a per-slice power-of-two scale, and the
+// integer LDS adds are associative: the histogram, and with it every PDF of
+// the batched path, is the same on every run and in every graph replay.  The
+// scales are S_w = 2^(49 - ilogb(W)) and S_b = 2^(49 - ilogb(D)), W a bound
+// on the slice's sum of w (u_max^2 sum_j warea_j over its mass rows) and D the
+// same with |beta_j| warea_j, so every deposit and every sum stays below
+// 2^51 in magnitude: one FP64 fma with 1.5 * 2^52 rounds a deposit to the
+// integer in its low mantissa bits.  A deposit is resolved to 2^-50 W
+// (~1e-15 of the slice's weight; the strong-lensing bins of a slice, xi >
+// 0.1, hold ~1e-6 of it).  The flush turns the sums back into the FP64
+// moments the rest of the path reads: w_f = A / (S_w g_f), the centroid from
+// Bm / A and the in-bin variance from C / A - (Bm / A)^2 -- in centred form,
+// free of the cancellation of (sum w xi^2) / w - m^2 -- and
+// beta w = E / (S_b g_f).  A cell on the negative side beyond the last bin
+// (xi < -30, detA > e^30) is dropped, as the positive side drops xi > 30.
+//
+// The slice's coarse two-point moments are formed at the flush, kCoarse fine
+// bins per coarse bin, each fine bin's beta w placed at the bin's w centroid
+// m_f with its w variance v_f: sum beta w, sum beta w m_f,
+// sum beta w (m_f^2 + v_f).  Within one fine bin (3.1 % relative width) beta
+// and xi are taken as uncorrelated, which moves a coarse record's centroid by
+// at most 1.6 % of xi.  A cell costs four 64-bit integer LDS atomics on fine
+// bins, which neighbouring lanes rarely share.
 constexpr int kFineMom = 4;
 struct FineHist {
-  double d[kFineMom * kFineBins];
+  unsigned long long d[kFineMom * kFineBins];
 };
 
+// Place of a cell in its bin: d = |xi| / base_f * C2 - C1 C2, with base_f = 2^e
+// a power of two taken from the bin index alone -- 2^floor(log2 lo_f) for bin
+// q >= 1 of a side, whose |xi| / base_f then lies in [1, 2R) (R = hi / lo,
+// the bin ratio, at most kNegR), and 2^ceil(log2 hi_0) for bin 0, which
+// reaches down to 0 and lies in (0, 1] -- and C1 = kNegR, C2 = 1 / (1.01
+// kNegR) the centre and reciprocal half-width of [0, 2 kNegR) widened by 1 %,
+// so |d| <= 0.99 on every bin with room for the FP32 log's choice at an
+// edge.  Scaling by a power of two is exact and needs no transcendental; one
+// range for all bins keeps per-bin constants out of the cell loop (the
+// in-bin spread of d is then narrower than [-1, 1], by ~40x on the positive
+// side, which the 2^-50 resolution absorbs).  The tilt bound of negative bin
+// q is g = 2^-k with k = max(1, 2^(e + 2)) >= |xi|_max log2 e, as
+// |xi|_max < 2 kNegR 2^e < 2^(e + 2) / log2 e.
+constexpr float kPosScale = (float)(kFinePos / 31.032218497590705);   // ln(30 / 1e-12)
+constexpr float kNegScale = (float)(kFineNeg / 21.82187812561452);    // ln(30 / 1e-8)
+constexpr float kLnLo = -27.631021115928547f;                          // ln 1e-12
+constexpr float kLnNegLo = -18.420680743952367f;                       // ln 1e-8
+// log2 of the lowest edge (+1e-4, so floor() of an edge that rounds a hair
+// low in FP32 still gives its own exponent) and the log2 step of each side
+constexpr float kPosLo2 = (float)(-27.631021115928547 * 1.4426950408889634 + 1e-4);
+constexpr float kPosStep2 = (float)(31.032218497590705 / kFinePos * 1.4426950408889634);
+constexpr float kNegLo2 = (float)(-18.420680743952367 * 1.4426950408889634 + 1e-4);
+constexpr float kNegStep2 = (float)(21.82187812561452 / kFineNeg * 1.4426950408889634);
+constexpr double kNegR = 1.185877993301057;    // e^(21.82187812561452 / 128)
+constexpr int kPosE0 = -39;                    // ceil(log2(1e-12 e^(31.03 / 1024)))
+constexpr int kNegE0 = -26;                    // ceil(log2(1e-8 kNegR))
+constexpr double kPlaceC2 = 1.0 / (1.01 * kNegR);
+constexpr double kPlaceC12 = kNegR * kPlaceC2;
+
+// A cell's fine bin f, its base exponent e and tilt shift k (g = 2^-k).
+struct FineSlot {
+  int f, e, k;
+};
+__device__ __forceinline__ int pos_bin_exp(int q) {
+  return (q == 0) ? kPosE0 : (int)floorf(fmaf((float)q, kPosStep2, kPosLo2));
+}
+__device__ __forceinline__ int neg_bin_exp(int q) {
+  return (q == 0) ? kNegE0 : (int)floorf(fmaf((float)q, kNegStep2, kNegLo2));
+}
+__device__ __forceinline__ int neg_tilt(int e) { return (e >= -2) ? (1 << min(e + 2, 30)) : 1; }
+// Bin from xi with the FP32 hardware log: which of two neighbouring bins an
+// edge case lands in does not matter, every bin keeps its exact centroid and
+// variance.
+__device__ __forceinline__ FineSlot fine_slot(double xi) {
+  FineSlot s;
+  if (xi > 0.0) {
+    const float lx = log_hw(fmaxf((float)xi, 1e-37f));
+    const int q = min(max((int)((lx - kLnLo) * kPosScale), 0), kFinePos - 1);
+    s.f = q;
+    s.e = pos_bin_exp(q);
+    s.k = 0;
+  } else {
+    const float lx = log_hw(fmaxf((float)(-xi), 1e-37f));
+    const int q = min(max((int)((lx - kLnNegLo) * kNegScale), 0), kFineNeg - 1);
+    s.f = kFinePos + q;
+    s.e = neg_bin_exp(q);
+    s.k = neg_tilt(s.e);
+  }
+  return s;
+}
+// The same constants from the bin index (the flush).
+__device__ __forceinline__ FineSlot bin_slot(int f) {
+  FineSlot s;
+  s.f = f;
+  if (f < kFinePos) {
+    s.e = pos_bin_exp(f);
+    s.k = 0;
+  } else {
+    s.e = neg_bin_exp(f - kFinePos);
+    s.k = neg_tilt(s.e);
+  }
+  return s;
+}
+
 __device__ __forceinline__ void fine_hist_clear(FineHist& H, int t, int nt) {
-  for (int q = t; q < kFineMom * kFineBins; q += nt) H.d[q] = 0.0;
+  for (int q = t; q < kFineMom * kFineBins; q += nt) H.d[q] = 0ull;
 }
 
-__device__ __forceinline__ void deposit4(FineHist& H, double xi, double ws, double bw) {
-  double* q = H.d + fine_bin(xi);
-  atomicAdd(q, ws);
-  atomicAdd(q + kFineBins, ws * xi);
-  atomicAdd(q + 2 * kFineBins, ws * xi * xi);
-  atomicAdd(q + 3 * kFineBins, bw);
+// x (|x| < 2^51) rounded to the nearest integer, as the two's-complement bits
+// the LDS adds accumulate: 1.5 * 2^52 + x holds the integer in its low
+// mantissa bits.
+constexpr double kMagic = 6755399441055744.0;            // 1.5 * 2^52
+__device__ __forceinline__ unsigned long long magic_bits(double t) {
+  return (unsigned long long)__double_as_longlong(t) -
+         (unsigned long long)__double_as_longlong(kMagic);
 }
 
-// Add (or store) the slice's fine partials and coarse moments from H.
-__device__ inline void fine_hist_flush(const FineHist& H, double* __restrict__ P,
-                                       double* __restrict__ Q, bool add, int t, int nt) {
+// One cell into H: sw, sb the log2 of the slice's scales S_w, S_b.
+__device__ __forceinline__ void deposit4(FineHist& H, double xi, double ws, double bw, int sw,
+                                         int sb) {
+  if (xi < -kXiNegHi) return;
+  const FineSlot s = fine_slot(xi);
+  const double d = fma(fabs(xi), ldexp(kPlaceC2, -s.e), -kPlaceC12);
+  const double a = ldexp(ws, sw - s.k);          // ws g S_w
+  const double ad = a * d;
+  unsigned long long* q = H.d + s.f;
+  atomicAdd(q, magic_bits(a + kMagic));
+  atomicAdd(q + kFineBins, magic_bits(ad + kMagic));
+  atomicAdd(q + 2 * kFineBins, magic_bits(fma(ad, d, kMagic)));
+  atomicAdd(q + 3 * kFineBins, magic_bits(ldexp(bw, sb - s.k) + kMagic));
+}
+
+// log2 of the power-of-two scale 2^(49 - ilogb(bound)) of a slice's
+// fixed-point sums (0 for an empty slice).
+__device__ __forceinline__ int fixed_scale(double bound) {
+  return (bound > 0.0) ? 49 - ilogb(bound) : 0;
+}
+
+// Sum of v over the workgroup, the same value in every thread and on every
+// run (butterfly within each wave, then the waves in order).  red holds one
+// double per wave.
+__device__ __forceinline__ double block_sum(double v, double* red, int t, int nt) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  __syncthreads();
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < nt / 64; ++w) s += red[w];
+  return s;
+}
+
+// Bin f's FP64 moments from the fixed-point sums: w, centroid m, in-bin
+// variance v, two-point weight bw (w = 0: empty bin).
+__device__ __forceinline__ void fine_moments(const FineHist& H, int f, int sw, int sb, double& w,
+                                             double& m, double& v, double& bw) {
+  const long long A = (long long)H.d[f];
+  if (A == 0) { w = m = v = bw = 0.0; return; }
+  const FineSlot s = bin_slot(f);
+  const double a = (double)A;
+  const double e1 = (double)(long long)H.d[kFineBins + f] / a;
+  const double e2 = (double)(long long)H.d[2 * kFineBins + f] / a;
+  const double h = ldexp(1.0 / kPlaceC2, s.e);  // |xi| per unit of d
+  const double mx = (kPlaceC12 + e1) * h;
+  w = ldexp(a, s.k - sw);
+  m = (f >= kFinePos) ? -mx : mx;
+  v = fmax(e2 - e1 * e1, 0.0) * h * h;
+  bw = ldexp((double)(long long)H.d[3 * kFineBins + f], s.k - sb);
+}
+
+// Add (or store) the slice's fine partials (w, w m, w (m^2 + v)) and coarse
+// moments from H.
+__device__ inline void fine_hist_flush(const FineHist& H, int sw, int sb,
+                                       double* __restrict__ P, double* __restrict__ Q, bool add,
+                                       int t, int nt) {
   for (int f = t; f < kFineBins; f += nt) {
-    const double w = H.d[f], wx = H.d[kFineBins + f], wxx = H.d[2 * kFineBins + f];
+    double w, m, v, bw;
+    fine_moments(H, f, sw, sb, w, m, v, bw);
+    const double wx = w * m, wxx = w * (m * m + v);
     if (add) {
       if (w == 0.0) continue;
       P[f * kMoments] += w;
@@ -523,11 +660,9 @@ __device__ inline void fine_hist_flush(const FineHist& H, double* __restrict__ P
   for (int cb = t; cb < kSliceBins; cb += nt) {
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;
     for (int f = cb * kCoarse; f < (cb + 1) * kCoarse; ++f) {
-      const double w = H.d[f];
+      double w, m, v, bw;
+      fine_moments(H, f, sw, sb, w, m, v, bw);
       if (w == 0.0) continue;
-      const double bw = H.d[3 * kFineBins + f];
-      const double m = H.d[kFineBins + f] / w;
-      const double v = fmax(H.d[2 * kFineBins + f] / w - m * m, 0.0);
       m0 += bw;
       m1 += bw * m;
       m2 += bw * (m * m + v);
@@ -590,7 +725,7 @@ __global__ __launch_bounds__(256) void k_rowf(const double* __restrict__ rows,
 // nearly uniform (x / x_t = u r_vir / r_t hardly depends on the mass) and
 // spreads the lanes over more histogram bins than consecutive columns do.
 template <int kBinThreads, bool kEll>
-__global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restrict__ cosmo,
+__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(kBinThreads == 256 ? 4 : (kEll ? 7 : 8)))) void k_bin_cells(const double* __restrict__ cosmo,
                                                            const double* __restrict__ uedge,
                                                            const double* __restrict__ rows,
                                                            sgl_shape sh, double* __restrict__ hist) {
@@ -605,6 +740,19 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
                    ((size_t)b * sh.n_z_max + i) * sh.n_m;
   const ColF* CF = reinterpret_cast<const ColF*>(hist + HL.cols);
   const int n_b = sh.n_b;
+  // fixed-point scales of the slice: sum_l du2_l <= u_max^2 bounds a row's
+  // weight by warea u_max^2 (with ellipticity, its kEllPhi orientations
+  // share warea)
+  __shared__ double red[kBinThreads / 64];
+  double sw = 0.0, sb = 0.0;
+  for (int j = t; j < sh.n_m; j += kBinThreads) {
+    const double wa = RF[j].warea;
+    sw += wa;
+    sb += fabs(RF[j].beta) * wa;
+  }
+  const double umax2 = uedge[n_b] * uedge[n_b];
+  const int lsw = fixed_scale(umax2 * block_sum(sw, red, t, kBinThreads));
+  const int lsb = fixed_scale(umax2 * block_sum(sb, red, t, kBinThreads));
   const int rpp = min(sh.n_m, kBinThreads);     // rows per pass
   const int parts = kBinThreads / rpp;          // column phases
   const int tj = t % rpp, part = t / rpp;
@@ -613,7 +761,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
     trunc_kg_row_f32(cf.ug * rc.s, cf.lug + rc.log_s, cf.rug * rc.inv_s, rc, kf, gf);
     double xi, ws;
     cell_jump_f32log(kf, gf, rc.warea * cf.du2, xi, ws);
-    if (ws != 0.0) deposit4(H, xi, ws, rc.beta * ws);
+    if (ws != 0.0) deposit4(H, xi, ws, rc.beta * ws, lsw, lsb);
   };
   // Halo ellipticity: orientation node q of a row is the row record with its
   // x scaled to the elliptical radius, x_e = x f, f^2 = 1 - eps cos 2phi_q
@@ -635,7 +783,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
     }
     double xi, ws;
     cell_jump_f32log(kf, gf, rq.warea * cf.du2, xi, ws);
-    if (ws != 0.0) deposit4(H, xi, ws, rq.beta * ws);
+    if (ws != 0.0) deposit4(H, xi, ws, rq.beta * ws, lsw, lsb);
   };
   // every cell of row j: cols(visit) walks this thread's columns
   auto row_cells = [&](const RowF& rc, int j, auto&& cols) {
@@ -682,14 +830,15 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_cells(const double* __restr
     }
   }
   __syncthreads();
-  fine_hist_flush(H, hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments,
+  fine_hist_flush(H, lsw, lsb,
+                  hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments,
                   hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3, false, t,
                   kBinThreads);
 }
 
 // Filament pass (SGL_FLAG_FILAMENTS): every filament cell of the slice --
 // kappa = k0 fk(u), gamma = k0 fg(u) in FP32 (no cancellation anywhere on the
-// cylinder profile), FP64 weights, detA and accumulation -- into a
+// cylinder profile), FP64 weights and detA, fixed-point accumulation -- into a
 // moment-major LDS histogram, then added into the slice's fine partials and
 // coarse moments (the main pass has written them; this workgroup is the only
 // one touching the slice now).  Lanes run over orientation nodes and u bins
@@ -724,6 +873,19 @@ __global__ __launch_bounds__(kFilThreads) void k_bin_fil(const double* __restric
   const double* FR = frows + ((size_t)b * sh.n_z_max + i) * sh.n_m * kFilRowStride;
   const int per = kFilU * kFilPhi;
   const int n_cells = sh.n_m * per;
+  // fixed-point scales: a row's cells weigh wrow pi r^2 sum_l du2_l sum_n wp_n
+  __shared__ double red[kFilThreads / 64];
+  double sw = 0.0, sb = 0.0;
+  for (int j = t; j < sh.n_m; j += kFilThreads) {
+    const double* R = FR + (size_t)j * kFilRowStride;
+    const double wr = R[2] * kPi * R[1] * R[1];
+    sw += wr;
+    sb += fabs(R[3]) * wr;
+  }
+  const double su = block_sum(t < kFilU ? sdu2[t] : 0.0, red, t, kFilThreads);
+  const double sp = block_sum(t < kFilPhi ? swp[t] : 0.0, red, t, kFilThreads);
+  const int lsw = fixed_scale(su * sp * block_sum(sw, red, t, kFilThreads));
+  const int lsb = fixed_scale(su * sp * block_sum(sb, red, t, kFilThreads));
   for (int c = t; c < n_cells; c += kFilThreads) {
     const int j = c / per, r = c - j * per, l = r / kFilPhi, n = r - l * kFilPhi;
     const double* R = FR + (size_t)j * kFilRowStride;
@@ -733,10 +895,11 @@ __global__ __launch_bounds__(kFilThreads) void k_bin_fil(const double* __restric
     double xi, ws;
     cell_jump_f32log(k0 * sfk[l], k0 * sfg[l], wrow * kPi * R[1] * R[1] * sdu2[l] * swp[n], xi, ws);
     if (ws == 0.0) continue;
-    deposit4(H, xi, ws, R[3] * ws);
+    deposit4(H, xi, ws, R[3] * ws, lsw, lsb);
   }
   __syncthreads();
-  fine_hist_flush(H, hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments,
+  fine_hist_flush(H, lsw, lsb,
+                  hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments,
                   hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3, true, t,
                   kFilThreads);
 }

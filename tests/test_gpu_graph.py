@@ -2,8 +2,10 @@
 
 The captured solve (tables on two streams, binning, exponent, two-point
 term, inversion) is replayed several times with eager solves of other
-records in between; every replay must agree with an eager solve of the same
-records (to the round-off of the FP64 histogram atomics)."""
+records in between; every replay must equal an eager solve of the same
+records bit for bit: the histogram's sums are fixed-point integer atomics,
+whose result does not depend on the order in which they arrive, and every
+other reduction of the batched path runs in a fixed order."""
 import numpy as np
 import pytest
 
@@ -44,4 +46,29 @@ def test_graph_replays_match_eager(clustering):
             p, _ = g.run(r)
             torch.cuda.synchronize()
             d = float((p - want[q]).abs().max()) / float(want[q].abs().max())
-            assert d < 1e-5, (rep, q, d)
+            assert torch.equal(p, want[q]), (rep, q, d)
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_batched_solve_is_reproducible(prec):
+    """Two solvers, the same records, two runs each: the same bits, at the
+    default 256-mass x 512-column lattice (the histogram's atomics collide
+    most there), with and without ellipticity and filaments."""
+    from sgl_analytic_amd.solver import Shape, Solver
+    zs = (0.4, 1.1, 2.5, 6.0)
+    for kw in (dict(), dict(ell=True, filaments=True)):
+        sh = Shape(n_z_max=16, n_k=1024, n_out=256, **kw)
+        print(f"reproducibility prec={prec} {kw}", flush=True)
+        r = _records(zs, 7)
+        r[:, 10] = 16
+        out = []
+        for _ in range(2):
+            s = Solver(sh, 4, path="binned", precision=prec)
+            for _ in range(2):
+                p, d = s.run(r)
+                torch.cuda.synchronize()
+                out.append((p.clone(), d.clone(), s.hist.clone()))
+            del s
+        for p, d, h in out[1:]:
+            assert torch.equal(p, out[0][0]) and torch.equal(d, out[0][1])
+            assert torch.equal(h, out[0][2])
