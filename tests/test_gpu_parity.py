@@ -55,7 +55,9 @@ def test_tables_match_oracle(plain, zs):
     rel = np.abs(Rd - R) / np.abs(R)
     worst = rel.reshape(-1, 6).max(axis=0)
     print("max rel err per column", worst)
-    assert np.all(worst < 2e-6), worst
+    # measured: 3e-10 (k_s, r_s, x_t), 1e-11 (r_vir), 1e-7 (the Poisson weight,
+    # from the sigma(M) and mass-function quadratures), 2e-9 (beta)
+    assert np.all(worst < np.array([1e-9, 1e-9, 1e-9, 1e-10, 3e-7, 1e-8])), worst
     z_dev = tv["slices"][0, :, 0]
     assert np.max(np.abs(z_dev - T["z"])) < 1e-10
     if osh.clustering:
