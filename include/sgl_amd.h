@@ -233,7 +233,9 @@ sgl_status sgl_lnphi_binned(const void* tables, const void* hist, int32_t B,
  * written by sgl_lnphi_exact), FP64 sums, psi2 [B][n_k][2] on every node;
  * SGL_PREC_FP32: amp is FP32 on the sub-grid (as written by
  * sgl_lnphi_binned), U = A C on FP32 MFMA with FP32 accumulation, psi2
- * [B][n_kc][2] on the sub-grid (expand it with sgl_psi2_interp).
+ * [B][n_kc][2] on the sub-grid (expand it with sgl_psi2_interp);
+ * SGL_PREC_BF16X3: the same with A and C split into BF16 head + tail and
+ * U = Ahi Chi + Ahi Clo + Alo Chi on BF16 MFMA, FP32 accumulation.
  * Replaces: Lam.Q (sgl_full.py:3353-3397), nord = 1.
  */
 sgl_status sgl_quadform(const void* tables, const void* amp, int32_t B, const sgl_shape* shape,

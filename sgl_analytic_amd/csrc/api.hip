@@ -221,7 +221,8 @@ sgl_status sgl_lnphi_binned(const void* tables, const void* hist, int32_t B,
 sgl_status sgl_quadform(const void* tables, const void* amp, int32_t B, const sgl_shape* shape,
                         int32_t precision, double* psi2, void* stream) {
   if (!shape_ok(shape) || !batch_ok(B) || !tables || !amp || !psi2) return SGL_ERR_INVALID;
-  if (precision != SGL_PREC_FP64 && precision != SGL_PREC_FP32) return SGL_ERR_UNSUPPORTED;
+  if (precision != SGL_PREC_FP64 && precision != SGL_PREC_FP32 && precision != SGL_PREC_BF16X3)
+    return SGL_ERR_UNSUPPORTED;
   TablesLayout L = tables_layout(*shape, B);
   const double* T = static_cast<const double*>(tables);
   return (sgl_status)launch_quadform(T + L.cov, T + L.cosmo, amp, B, *shape, precision, psi2,

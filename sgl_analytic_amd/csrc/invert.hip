@@ -144,6 +144,111 @@ __global__ __launch_bounds__(256) void k_quadform_mfma(const double* __restrict_
   }
 }
 
+// Split-BF16x3 variant of the same product on v_mfma_f32_16x16x32_bf16: A and
+// C are each split into a BF16 head and a BF16 tail (x = hi + lo, lo the
+// rounded remainder), U = Ahi Chi + Ahi Clo + Alo Chi with FP32 accumulation
+// (the dropped Alo Clo term is 2^-16 of the product).  C is symmetric, so the
+// B operand C[k][j] is read as row j of C, 8 consecutive k per lane in one
+// 16-byte LDS load.  The epilogue (U against the FP32 amplitudes) is the FP32
+// kernel's.
+constexpr int kQbK = 32;                    // K per MFMA step
+constexpr int kQbSteps = kQmPad / kQbK;     // 3 steps cover 96 slices
+constexpr int kQbLd = kQmPad + 8;           // bf16 row stride of the C tiles (shorts)
+typedef short bf16x8q __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ unsigned short bf16_hi(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_val(unsigned short h) {
+  return __uint_as_float(((uint32_t)h) << 16);
+}
+
+__global__ __launch_bounds__(256) void k_quadform_bf16x3(const double* __restrict__ cov,
+                                                         const double* __restrict__ cosmo,
+                                                         const float* __restrict__ amp,
+                                                         int n_z_max, int n_k,
+                                                         double* __restrict__ psi2) {
+  __shared__ __align__(16) unsigned short sCh[kQmPad][kQbLd];
+  __shared__ __align__(16) unsigned short sCl[kQmPad][kQbLd];
+  const int b = blockIdx.y, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
+  const double* C = cov + (size_t)b * n_z_max * n_z_max;
+  for (int q = t; q < kQmPad * kQmPad; q += 256) {
+    const int i = q / kQmPad, j = q % kQmPad;
+    const float c = (i < n_z && j < n_z) ? (float)C[(size_t)i * n_z_max + j] : 0.0f;
+    const unsigned short h = bf16_hi(c);
+    sCh[i][j] = h;
+    sCl[i][j] = bf16_hi(c - bf16_val(h));
+  }
+  __syncthreads();
+  const int ns = (n_z + kQbK - 1) / kQbK, nt = (n_z + 15) / 16;
+  const int col = lane & 15, grp = lane >> 4;
+  const float2* A2 = (const float2*)amp + (size_t)b * n_z_max * n_k;
+  for (int g = 0; g < kQmTiles; ++g) {
+    const int n0 = (blockIdx.x * 4 * kQmTiles + wave * kQmTiles + g) * 16;
+    if (n0 >= n_k) break;
+    // A fragments: lane supplies A[n0 + col][32 s + 8 grp .. + 7], split
+    bf16x8q arh[kQbSteps], arl[kQbSteps], aih[kQbSteps], ail[kQbSteps];
+#pragma unroll
+    for (int s = 0; s < kQbSteps; ++s) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int i = kQbK * s + 8 * grp + e, n = n0 + col;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (s < ns && i < n_z && n < n_k) v = A2[(size_t)i * n_k + n];
+        const unsigned short hr = bf16_hi(v.x), hi = bf16_hi(v.y);
+        arh[s][e] = (short)hr;
+        arl[s][e] = (short)bf16_hi(v.x - bf16_val(hr));
+        aih[s][e] = (short)hi;
+        ail[s][e] = (short)bf16_hi(v.y - bf16_val(hi));
+      }
+    }
+    float sr[4] = {0.f, 0.f, 0.f, 0.f}, si[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int jt = 0; jt < nt; ++jt) {
+      f32x4q ur = {0.f, 0.f, 0.f, 0.f}, ui = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < kQbSteps; ++s) {
+        if (s < ns) {
+          // B operand C[32 s + 8 grp .. + 7][16 jt + col] = row (16 jt + col) of C
+          const bf16x8q ch = *(const bf16x8q*)&sCh[16 * jt + col][kQbK * s + 8 * grp];
+          const bf16x8q cl = *(const bf16x8q*)&sCl[16 * jt + col][kQbK * s + 8 * grp];
+          ur = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl[s], ch, ur, 0, 0, 0);
+          ur = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh[s], cl, ur, 0, 0, 0);
+          ur = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh[s], ch, ur, 0, 0, 0);
+          ui = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail[s], ch, ui, 0, 0, 0);
+          ui = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih[s], cl, ui, 0, 0, 0);
+          ui = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih[s], ch, ui, 0, 0, 0);
+        }
+      }
+      // lane holds U[n0 + 4 grp + r][16 jt + col]
+      const int j = 16 * jt + col;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + 4 * grp + r;
+        if (j < n_z && n < n_k) {
+          const float2 v = A2[(size_t)j * n_k + n];
+          sr[r] += ur[r] * v.x - ui[r] * v.y;
+          si[r] += ur[r] * v.y + ui[r] * v.x;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double dr = sr[r], di = si[r];
+      for (int m = 8; m > 0; m >>= 1) {
+        dr += __shfl_xor(dr, m, 64);
+        di += __shfl_xor(di, m, 64);
+      }
+      const int n = n0 + 4 * grp + r;
+      if (col == 0 && n < n_k) {
+        psi2[2 * ((size_t)b * n_k + n)] = 0.5 * dr;
+        psi2[2 * ((size_t)b * n_k + n) + 1] = 0.5 * di;
+      }
+    }
+  }
+}
+
 // Psi_2 from the sub-grid (node jc at k node jc S) onto every k node: cubic
 // Lagrange in the node index (ln k is uniform in it) on the four sub-grid
 // nodes around n, one-sided in the first and last interval.  At n = jc S the
@@ -561,8 +666,12 @@ int launch_quadform(const double* cov, const double* cosmo, const void* amp, int
     // batched path: amplitudes and Psi_2 on the sub-grid
     const int per_wg = 4 * kQmTiles * 16, n_kc = psi2_nodes(sh);
     dim3 grid((n_kc + per_wg - 1) / per_wg, B);
-    hipLaunchKernelGGL(k_quadform_mfma, grid, dim3(256), 0, st, cov, cosmo, (const float*)amp,
-                       sh.n_z_max, n_kc, psi2);
+    if (prec == SGL_PREC_BF16X3)
+      hipLaunchKernelGGL(k_quadform_bf16x3, grid, dim3(256), 0, st, cov, cosmo,
+                         (const float*)amp, sh.n_z_max, n_kc, psi2);
+    else
+      hipLaunchKernelGGL(k_quadform_mfma, grid, dim3(256), 0, st, cov, cosmo,
+                         (const float*)amp, sh.n_z_max, n_kc, psi2);
   }
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }

@@ -203,7 +203,7 @@ class Solver:
                 ck(L.sgl_quadform(_ptr(self.tables), _ptr(self.amp), B, cs, PREC_FP64,
                                   _ptr(self.psi2), st), "sgl_quadform")
             else:
-                ck(L.sgl_quadform(_ptr(self.tables), _ptr(self.amp), B, cs, PREC_FP32,
+                ck(L.sgl_quadform(_ptr(self.tables), _ptr(self.amp), B, cs, self.precision,
                                   _ptr(self.psi2c), st), "sgl_quadform")
                 ck(L.sgl_psi2_interp(_ptr(self.psi2c), B, cs, _ptr(self.psi2), st),
                    "sgl_psi2_interp")

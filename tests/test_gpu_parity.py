@@ -125,3 +125,41 @@ def test_batched_pdf_vs_oracle(zs, prec):
           m["p_mu_gt_2"])
     assert l1 < 1e-3
     assert abs(d[5] - m["mu_mean"]) < 1e-3
+
+
+@pytest.mark.parametrize("n_z", [8, 40, 82])
+def test_quadform_fp32_and_bf16x3_vs_fp64(n_z):
+    """The batched path's two-point quadratic form Psi_2 = 1/2 a^T C a on MFMA,
+    FP32 and split-BF16x3, against FP64 sums of the same FP32 amplitudes and
+    covariance, slice counts that leave 1, 3 and 6 partial 16-slice tiles:
+    FP32 to 2e-6 of max |Psi_2|; BF16x3 to 4e-5 (a BF16 head + BF16 tail
+    carries 16 of FP32's 24 mantissa bits, 2^-16 ~ 1.5e-5 per product)."""
+    import ctypes
+    from sgl_analytic_amd import _lib
+    from sgl_analytic_amd.solver import Shape, Solver, _ptr, _stream
+    dsh = Shape(n_z_max=n_z, n_m=32, n_b=64, n_k=1024, n_out=256)
+    print(f"quadform n_z={n_z}", flush=True)
+    s = Solver(dsh, 2, path="binned")
+    rec = np.concatenate([_record(0.7, n_z), _record(4.0, n_z - 3)])
+    s.run(rec)
+    torch.cuda.synchronize()
+    A = s.amp.cpu().numpy().astype(np.float64)          # [B][nz][n_kc][2]
+    a = A[..., 0] + 1j * A[..., 1]
+    C = s.table_views()["cov"].astype(np.float32).astype(np.float64)
+    L = _lib.load()
+    out = {}
+    for prec in (_lib.PREC_FP32, _lib.PREC_BF16X3):
+        ps = torch.zeros_like(s.psi2c)
+        _lib.check(L.sgl_quadform(_ptr(s.tables), _ptr(s.amp), 2, ctypes.byref(s.cs), prec,
+                                  _ptr(ps), _stream()), "sgl_quadform")
+        torch.cuda.synchronize()
+        q = ps.cpu().numpy()
+        out[prec] = q[..., 0] + 1j * q[..., 1]
+    for b, nz in enumerate((n_z, n_z - 3)):
+        ab = a[b, :nz]
+        ref = 0.5 * np.einsum("in,ij,jn->n", ab, C[b, :nz, :nz], ab)
+        scale = np.abs(ref).max()
+        for prec, q in out.items():
+            err = np.abs(q[b] - ref).max() / scale
+            print(f"  b={b} n_z={nz} prec={prec}: max |dPsi_2| / max |Psi_2| = {err:.2e}")
+            assert err < (2e-6 if prec == _lib.PREC_FP32 else 4e-5), (b, prec, err)
