@@ -31,6 +31,9 @@
 //                     and accumulation in FP64;
 //   k_amp_binned      the slice amplitudes from the coarse records, the same
 //                     way, at every kPsi2Stride-th k node.
+#include <algorithm>
+#include <mutex>
+
 #include "sgl_amd_internal.h"
 
 namespace sgl {
@@ -724,16 +727,15 @@ __global__ __launch_bounds__(256) void k_rowf(const double* __restrict__ rows,
 // masses at the same column, which keeps the x >= x_t and x < 1 branches
 // nearly uniform (x / x_t = u r_vir / r_t hardly depends on the mass) and
 // spreads the lanes over more histogram bins than consecutive columns do.
+// bin_slice takes one slice (instance b, slice i) into the workgroup's LDS
+// histogram, then into its fine partials and coarse moments.
 template <int kBinThreads, bool kEll>
-__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(kBinThreads == 256 ? 4 : (kEll ? 7 : 8)))) void k_bin_cells(const double* __restrict__ cosmo,
-                                                           const double* __restrict__ uedge,
-                                                           const double* __restrict__ rows,
-                                                           sgl_shape sh, double* __restrict__ hist) {
-  __shared__ FineHist H;
-  const int t = threadIdx.x, i = blockIdx.x, b = blockIdx.y;
-  const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
-  if (i >= n_z) return;                         // slices past n_z are never read
-  const HistLayout HL = hist_layout(sh, gridDim.y);
+__device__ __forceinline__ void bin_slice(const double* __restrict__ cosmo,
+                                          const double* __restrict__ uedge,
+                                          const double* __restrict__ rows, const sgl_shape& sh,
+                                          const HistLayout& HL, int b, int i, FineHist& H,
+                                          double* __restrict__ hist) {
+  const int t = threadIdx.x;
   fine_hist_clear(H, t, kBinThreads);
   __syncthreads();
   const RowF* RF = reinterpret_cast<const RowF*>(hist + HL.rowf) +
@@ -834,6 +836,69 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(kBi
                   hist + HL.partial + ((size_t)b * sh.n_z_max + i) * kFineBins * kMoments,
                   hist + HL.cpart + ((size_t)b * sh.n_z_max + i) * kSliceBins * 3, false, t,
                   kBinThreads);
+}
+
+// Persistent launch: as many workgroups as the chip holds at once, each taking
+// (instance, slice) items from the work queue that k_bin_items wrote (one
+// item per slice below the instance's own n_z, so uneven slice counts leave
+// no idle workgroups) until it is empty.  Each item is binned whole by one
+// workgroup, so which workgroup takes it does not change any sum.
+template <int kBinThreads, bool kEll>
+__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(kBinThreads == 256 ? 4 : (kEll ? 7 : 8)))) void k_bin_cells(const double* __restrict__ cosmo,
+                                                           const double* __restrict__ uedge,
+                                                           const double* __restrict__ rows,
+                                                           sgl_shape sh, int B,
+                                                           double* __restrict__ hist) {
+  __shared__ FineHist H;
+  __shared__ int s_item;
+  const int t = threadIdx.x;
+  const HistLayout HL = hist_layout(sh, B);
+  int* queue = reinterpret_cast<int*>(hist + HL.items);
+  for (;;) {
+    if (t == 0) s_item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int item = s_item;
+    if (item >= queue[1]) break;                // uniform: every thread read the same item
+    const int packed = queue[2 + item];
+    bin_slice<kBinThreads, kEll>(cosmo, uedge, rows, sh, HL, packed & 0xFFFF, packed >> 16, H,
+                                 hist);
+    __syncthreads();                            // the flush is done before the next clear
+  }
+}
+
+// Work items of the binning pass: item (b, i) for every slice i < n_z[b], in
+// instance order (exclusive scan of the slice counts in one workgroup), and
+// the queue counter reset -- by a kernel, so a captured graph replays it.
+constexpr int kItemThreads = 1024;
+__global__ __launch_bounds__(kItemThreads) void k_bin_items(const double* __restrict__ cosmo,
+                                                            sgl_shape sh, int B,
+                                                            double* __restrict__ hist) {
+  __shared__ int scan[kItemThreads];
+  const int t = threadIdx.x;
+  const HistLayout HL = hist_layout(sh, B);
+  int* queue = reinterpret_cast<int*>(hist + HL.items);
+  const int per = (B + kItemThreads - 1) / kItemThreads;
+  const int b0 = min(B, t * per), b1 = min(B, b0 + per);
+  int cnt = 0;
+  for (int b = b0; b < b1; ++b)
+    cnt += min(max((int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10], 0), sh.n_z_max);
+  scan[t] = cnt;
+  __syncthreads();
+  for (int o = 1; o < kItemThreads; o <<= 1) {  // inclusive Hillis-Steele scan
+    const int v = (t >= o) ? scan[t - o] : 0;
+    __syncthreads();
+    scan[t] += v;
+    __syncthreads();
+  }
+  int pos = scan[t] - cnt;
+  for (int b = b0; b < b1; ++b) {
+    const int nz = min(max((int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10], 0), sh.n_z_max);
+    for (int i = 0; i < nz; ++i) queue[2 + pos++] = (i << 16) | b;
+  }
+  if (t == kItemThreads - 1) {
+    queue[0] = 0;
+    queue[1] = scan[t];
+  }
 }
 
 // Filament pass (SGL_FLAG_FILAMENTS): every filament cell of the slice --
@@ -1245,6 +1310,40 @@ int launch_vp_cells(const double* rows, const double* frows, const double* cosmo
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
+// Launch a persistent kernel with as many workgroups of `threads` as the
+// device holds at once (occupancy x compute units, looked up once per kernel
+// and device), at most max_wg.
+static int resident_workgroups(const void* kernel, int threads) {
+  struct Entry { const void* k; int dev, threads, n; };
+  static Entry cache[32];
+  static int used = 0;
+  static std::mutex mu;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mu);
+  for (int e = 0; e < used; ++e)
+    if (cache[e].k == kernel && cache[e].dev == dev && cache[e].threads == threads)
+      return cache[e].n;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = 1;
+  const int n = per_cu * cus;
+  if (used < 32) cache[used++] = Entry{kernel, dev, threads, n};
+  return n;
+}
+
+template <typename... P, typename... Args>
+static void launch_persistent(void (*kernel)(P...), int threads, int max_wg, hipStream_t st,
+                              Args... args) {
+  const int n = resident_workgroups(reinterpret_cast<const void*>(kernel), threads);
+  hipLaunchKernelGGL(kernel, dim3((unsigned)std::max(1, std::min(n, max_wg))), dim3(threads), 0,
+                     st, args...);
+}
+
 int launch_bin_pass(const double* rows, const double* frows, const double* cosmo,
                     const double* uedge, const sgl_shape& sh, int B, double* hist,
                     hipStream_t st) {
@@ -1252,24 +1351,26 @@ int launch_bin_pass(const double* rows, const double* frows, const double* cosmo
   hipLaunchKernelGGL(k_rowf, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, st, rows, cosmo,
                      uedge, sh, B, hist);
   hipLaunchKernelGGL(k_cols, dim3((sh.n_b + 255) / 256), dim3(256), 0, st, uedge, sh, B, hist);
-  const dim3 g(sh.n_z_max, B);
+  hipLaunchKernelGGL(k_bin_items, dim3(1), dim3(kItemThreads), 0, st, cosmo, sh, B, hist);
   const bool ell = (sh.flags & SGL_FLAG_ELL) != 0;
+  const int max_items = B * sh.n_z_max;
   switch (g_sgl_tune[SGL_TUNE_BIN_THREADS]) {
     case 0:
-      if (ell) hipLaunchKernelGGL((k_bin_cells<256, true>), g, dim3(256), 0, st, cosmo, uedge, rows, sh, hist);
-      else hipLaunchKernelGGL((k_bin_cells<256, false>), g, dim3(256), 0, st, cosmo, uedge, rows, sh, hist);
+      if (ell) launch_persistent(k_bin_cells<256, true>, 256, max_items, st, cosmo, uedge, rows, sh, B, hist);
+      else launch_persistent(k_bin_cells<256, false>, 256, max_items, st, cosmo, uedge, rows, sh, B, hist);
       break;
     case 2:
-      if (ell) hipLaunchKernelGGL((k_bin_cells<1024, true>), g, dim3(1024), 0, st, cosmo, uedge, rows, sh, hist);
-      else hipLaunchKernelGGL((k_bin_cells<1024, false>), g, dim3(1024), 0, st, cosmo, uedge, rows, sh, hist);
+      if (ell) launch_persistent(k_bin_cells<1024, true>, 1024, max_items, st, cosmo, uedge, rows, sh, B, hist);
+      else launch_persistent(k_bin_cells<1024, false>, 1024, max_items, st, cosmo, uedge, rows, sh, B, hist);
       break;
     default:
-      if (ell) hipLaunchKernelGGL((k_bin_cells<512, true>), g, dim3(512), 0, st, cosmo, uedge, rows, sh, hist);
-      else hipLaunchKernelGGL((k_bin_cells<512, false>), g, dim3(512), 0, st, cosmo, uedge, rows, sh, hist);
+      if (ell) launch_persistent(k_bin_cells<512, true>, 512, max_items, st, cosmo, uedge, rows, sh, B, hist);
+      else launch_persistent(k_bin_cells<512, false>, 512, max_items, st, cosmo, uedge, rows, sh, B, hist);
       break;
   }
   if (sh.flags & SGL_FLAG_FILAMENTS)
-    hipLaunchKernelGGL(k_bin_fil, g, dim3(kFilThreads), 0, st, frows, cosmo, sh, hist);
+    hipLaunchKernelGGL(k_bin_fil, dim3(sh.n_z_max, B), dim3(kFilThreads), 0, st, frows, cosmo, sh,
+                       hist);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
