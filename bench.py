@@ -39,17 +39,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 SEED = 20261203
-PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "hbm": 8000.0}
-# Work of the dominant kernel, k_bin_cells, per generated cell: the FP32 and
-# FP64 VALU operations it issues (FMA = 2, transcendental = 1, 64 lanes per
-# wave instruction), i.e. SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_{F32,F64} summed
-# over one dispatch of the B = 4096 workload (27.55e9 cells) divided by its
-# cell count (profiles/batched_B4096_counters.txt).
-CELL_FLOPS = 85.2
+# MI355X dense peaks; FP32 vector without packed-FP32 instructions (this build
+# disables them), which is the FP32 rate the binning kernel can reach.
+PEAK = {"fp64": 78.6, "fp32": 78.65, "bf16": 2500.0, "hbm": 8000.0}
+# Algorithmic work of the dominant kernel, k_bin_cells, per generated cell: the
+# arithmetic of its formula on the common path, counted at source level (FMA =
+# 2, transcendental = 1): column position and log (3), 1/x^2 (1), the FP32
+# truncated-NFW profile -- P, Q, R, z with three square roots (14), G(z) (4),
+# kappa (7), the mean convergence with its log (17), kappa and gamma (3) --
+# the FP64 jump -ln detA and tilted weight (11), the bin from the FP32 log and
+# its base exponent (7), the fixed-point place and the four rounded moments
+# (13): 80.  The compiled kernel issues 126.0 VALU lane instructions per cell
+# (profiles/batched_B4096_counters.txt: moves, compares, selects, conversions
+# and both sides of divergent profile branches on top of these), at a
+# saturated VALU issue rate.
+CELL_FLOPS = 80
 # HBM bytes of one k_bin_cells dispatch of the B = 4096 workload: FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 KiB, the factors being what those counters read for a 1 GiB
 # device copy (bench.py --copy-calibration) divided by 1 GiB.
-BIN_TRAFFIC_B4096 = 2.10816e6 * 1024 * 2.0 + 6.03000e6 * 1024 * 1.0
+BIN_TRAFFIC_B4096 = 2.10804e6 * 1024 * 2.0 + 6.03000e6 * 1024 * 1.0
 
 
 def lhs_params(B, seed=SEED):
@@ -85,6 +93,11 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-sur-verify", action="store_true",
                     help="skip the solver check of the surrogate batch (for kernel profiles)")
+    ap.add_argument("--sur-fp64-frac", type=float, default=0.01,
+                    help="accepted surrogate PDFs with the largest network disagreement checked "
+                         "against the plain FP64 path (fraction of the accepted)")
+    ap.add_argument("--sur-fp64-random", type=int, default=64,
+                    help="further accepted surrogate PDFs, drawn at random, checked vs FP64")
     ap.add_argument("--ref-only", action="store_true",
                     help="only the reference-chain leg (generate_pdf_lnmu's 2-D chain at z_s = 1)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -194,7 +207,7 @@ def main():
     ref = ref_chain_leg(log)
 
     # ------------------------------------------------------ surrogate --
-    sur = None if args.no_surrogate else surrogate_leg(args, shape, sv, value, log)
+    sur = None if args.no_surrogate else surrogate_leg(args, shape, sv, value, log, plain_rate)
     del sv
 
     # ------------------------------------------------------- roofline --
@@ -219,7 +232,10 @@ def main():
         "data": "synthetic (seeded Latin-hypercube cosmologies; no external data)",
         "config": {"workload": f"batched solve, B={B} LHS cosmologies, z_s in [0.3,10], "
                                f"ceil(24(1+ln(1+z_s))) slices x 256 masses x 512 b, 8192 k, "
-                               f"2048 ln mu nodes", "batch": B, "seed": SEED,
+                               f"2048 ln mu nodes; ingredients: truncated NFW halos (r200 "
+                               f"edge) + the slice-covariance two-point term (Gaussian "
+                               f"closure), no ellipticity, filaments or subhalos",
+                   "ingredients": ["halos", "two-point (gauss)"], "batch": B, "seed": SEED,
                    "precision": args.precision, "graph": not args.no_graph},
         "plain_gpu_pdfs_per_s": round(plain_rate, 3),
         "batched_over_plain": round(value / plain_rate, 1),
@@ -292,7 +308,7 @@ def ref_chain_leg(log, zs=1.0, reps=3):
     return out
 
 
-def surrogate_leg(args, shape, sv, solver_rate, log):
+def surrogate_leg(args, shape, sv, solver_rate, log, plain_rate=None):
     """Surrogate path on an inference batch of fresh LHS cosmologies: timed
     throughput (cheap solve + both networks + interpolation + MFMA inversion +
     diagnostics + gates), then outside the timing every instance is solved by
@@ -357,13 +373,17 @@ def surrogate_leg(args, shape, sv, solver_rate, log):
     log(f"[bench] surrogate verified: {int(ok.sum())}/{n_acc} accepted within 1e-3 (max "
         f"{L1[acc].max() if n_acc else float('nan'):.3g}); L1 median with nets "
         f"{np.median(L1):.3g}, without {np.median(L1c):.3g}")
+    fp64 = surrogate_fp64_check(args, shape, rec, p_sur, acc, dis.cpu().numpy(), log)
     return {
         **base,
         "blended_pdfs_per_s_with_solver_fallback": round(blended, 1),
+        "blended_pdfs_per_s_with_plain_fp64_fallback":
+            round(Bs / (t_sur + (Bs - n_acc) / plain_rate), 2) if plain_rate else None,
         "verified": {"against": "batched solver, every instance (itself <= 1e-3 L1 vs FP64)",
                      "accepted": n_acc, "accepted_within_1e3": int(ok.sum()),
                      "accepted_L1_max": float(L1[acc].max()) if n_acc else None,
                      "accepted_L1_median": float(np.median(L1[acc])) if n_acc else None},
+        "verified_fp64": fp64,
         "network_gain": {"L1_median_with": float(np.median(L1)),
                          "L1_median_without": float(np.median(L1c)),
                          "L1_p90_with": float(np.quantile(L1, 0.9)),
@@ -371,6 +391,47 @@ def surrogate_leg(args, shape, sv, solver_rate, log):
                          "within_1e3_with": float((L1 <= 1e-3).mean()),
                          "within_1e3_without": float((L1c <= 1e-3).mean())},
     }
+
+
+def surrogate_fp64_check(args, shape, rec, p_sur, acc, dis, log, chunk=64):
+    """Accepted surrogate PDFs against the plain FP64 path: the worst
+    --sur-fp64-frac of them by the networks' disagreement, plus
+    --sur-fp64-random drawn from the rest (seeded); L1 of each."""
+    import torch
+    from sgl_analytic_amd import solver as S
+    ia = np.flatnonzero(acc)
+    if ia.size == 0 or args.sur_fp64_frac <= 0:
+        return None
+    n_worst = int(np.ceil(args.sur_fp64_frac * ia.size))
+    order = ia[np.argsort(-dis[ia], kind="stable")]
+    rng = np.random.default_rng(SEED + 3)
+    rand = rng.choice(order[n_worst:], size=min(args.sur_fp64_random, order.size - n_worst),
+                      replace=False)
+    pick = np.concatenate([order[:n_worst], rand])
+    xo = shape.out_grid()
+    pl = S.Solver(shape, chunk, path="exact")
+    l1 = np.zeros(pick.size)
+    t0 = time.perf_counter()
+    for a in range(0, pick.size, chunk):
+        ids = pick[a:a + chunk]
+        r = rec[ids]
+        if ids.size < chunk:
+            r = np.concatenate([r, np.repeat(r[-1:], chunk - ids.size, 0)])
+        pr, _ = pl.run(r)
+        torch.cuda.synchronize()
+        l1[a:a + ids.size] = np.trapezoid(np.abs(p_sur[ids] - pr[:ids.size].cpu().numpy()), xo,
+                                          axis=1)
+        log(f"[bench] surrogate FP64 check {a + ids.size}/{pick.size}")
+    del pl
+    torch.cuda.empty_cache()
+    out = {"against": "plain FP64 GPU path", "worst_by_disagreement": n_worst,
+           "worst_fraction": args.sur_fp64_frac, "random": int(rand.size),
+           "L1_max": float(l1.max()), "L1_max_worst": float(l1[:n_worst].max()),
+           "L1_max_random": float(l1[n_worst:].max()) if rand.size else None,
+           "L1_median": float(np.median(l1)), "within_1e3": int((l1 <= 1e-3).sum()),
+           "seconds": round(time.perf_counter() - t0, 1)}
+    log("[bench] surrogate FP64 check", out)
+    return out
 
 
 def precision_study(args, B):
@@ -466,10 +527,11 @@ def bin_pass_ms(sv, S, reps=3):
 
 
 def roofline(t_bin_ms, shape, B, n_z):
-    """Roofline of the dominant kernel, k_bin_cells: VALU-bound (per cell ~10
-    hardware transcendentals and ~150 other FP32/FP64 operations, no reuse to
-    stage through LDS beyond the per-slice row constants), priced against the
-    FP32 vector peak; HBM traffic from the committed FETCH/WRITE counters."""
+    """Roofline of the dominant kernel, k_bin_cells: VALU-bound (its VALU issue
+    is saturated; its four LDS atomics per cell alone would take about half its
+    time; it moves ~15 GB per dispatch, 2 % of HBM bandwidth), priced as its
+    algorithmic operations per cell against the non-packed FP32 vector peak;
+    HBM traffic from the committed FETCH/WRITE counters."""
     cells = float(np.sum(n_z)) * shape.n_m * shape.n_b
     flops = CELL_FLOPS * cells
     ach = flops / (t_bin_ms * 1e-3) / 1e12
@@ -478,7 +540,8 @@ def roofline(t_bin_ms, shape, B, n_z):
             "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": round(ach / PEAK["fp32"], 4),
             "traffic": BIN_TRAFFIC_B4096 if default else None,
             "ms": round(t_bin_ms, 3),
-            "algorithmic": f"{CELL_FLOPS} VALU flop per cell x {cells:.4e} cells",
+            "algorithmic": f"{CELL_FLOPS} FP ops per cell (source count of the formula) x "
+                           f"{cells:.4e} cells",
             "hbm_GBps": round(BIN_TRAFFIC_B4096 / (t_bin_ms * 1e-3) / 1e9, 1) if default else None}
 
 
