@@ -204,7 +204,9 @@ sgl_status sgl_trunc_nfw_kg(const double* x, int32_t n, double ks, double xt, in
  * moments.  sgl_bin_cells = sgl_bin_pass (every cell, FP32 lens profile in
  * a form without cancellation at x = 1, FP32 jump log) followed by
  * sgl_bin_reduce (the slice-order reduction); the two halves are exported
- * so that each can be timed on its own.
+ * so that each can be timed on its own.  The histogram sums are 64-bit
+ * fixed point (a per-slice power-of-two scale), so the result is the same
+ * bits on every call for the same inputs.
  * Replaces: Cells.add / _add_direct_kernel (sgl_full.py:1314-1383, 971-1046).
  */
 sgl_status sgl_bin_cells(const void* tables, int32_t B, const sgl_shape* shape, void* hist,
