@@ -331,6 +331,8 @@ def surrogate_leg(args, shape, sv, solver_rate, log, plain_rate=None):
     x = torch.as_tensor(SU.features(params), dtype=torch.float32, device="cuda")
     inf = SU.Inference(nets, shape, Bs, n_z_max=shape.n_z_max)
     inf.set_records(rec)
+    if not args.no_tune:
+        log("[bench] surrogate cheap-solve launch shapes", inf.tune())
     inf.run(x)
     torch.cuda.synchronize()
     reps = 3

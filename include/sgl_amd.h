@@ -71,7 +71,9 @@ typedef enum sgl_status {
 #define SGL_TUNE_BIN_THREADS 0   /* binning block: 0 = 256, 1 = 512, 2 = 1024 threads */
 #define SGL_TUNE_LNPHI_BLOCK 1   /* binned exponent block: 0 = 128, 1 = 256 threads */
 #define SGL_TUNE_GEMM_BM 2       /* inversion GEMM tile rows: 0 = 64, 1 = 128 */
-#define SGL_TUNE_COUNT 3
+#define SGL_TUNE_BIN_SCHEDULE 3  /* binning schedule: 0 = one workgroup per (instance, slice),
+                                    1 = persistent workgroups over a work queue of slices */
+#define SGL_TUNE_COUNT 4
 
 /* Grids of one solve (host struct, passed by pointer). */
 typedef struct sgl_shape {

@@ -53,7 +53,7 @@ VP_META_CELLS, VP_META_GROUPS, VP_META_SHELLS, VP_META_VGROUPS, VP_META_VCELLS =
 VP_META_STATUS = 5
 VP_STATUS_BINS, VP_STATUS_MODE, VP_STATUS_RATE, VP_STATUS_TRANSITION = 1, 2, 4, 8
 EPI_PLAIN, EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU = 0, 1, 2, 3
-TUNE_BIN_THREADS, TUNE_LNPHI_BLOCK, TUNE_GEMM_BM = 0, 1, 2
+TUNE_BIN_THREADS, TUNE_LNPHI_BLOCK, TUNE_GEMM_BM, TUNE_BIN_SCHEDULE = 0, 1, 2, 3
 PSI2_STRIDE = 8  # SGL_PSI2_STRIDE: batched two-point sub-grid stride
 
 

@@ -1,6 +1,6 @@
 """Launch-shape autotuner for the batched solver.
 
-Three knobs of the library choose launch shapes (``sgl_set_tuning``):
+Four knobs of the library choose launch shapes (``sgl_set_tuning``):
 
 - ``bin_threads``: threads per workgroup of the binning kernel (one
   workgroup per (slice, instance); more threads give more cells in flight
@@ -8,14 +8,18 @@ Three knobs of the library choose launch shapes (``sgl_set_tuning``):
 - ``lnphi_block``: k nodes per workgroup of the binned exponent kernel
   (the block also stages that many bin records through LDS per pass);
 - ``gemm_bm``: rows of the Filon matrix per inversion GEMM tile (64 or 128;
-  the larger tile halves the re-reads of phi at the price of occupancy).
+  the larger tile halves the re-reads of phi at the price of occupancy);
+- ``bin_schedule``: 0 = one binning workgroup per (instance, slice), the
+  hardware dispatcher refilling the CUs; 1 = persistent workgroups (as many
+  as the device holds) taking (instance, slice) items from a work queue.
 
-Each knob drives a different kernel, so they are tuned one at a time: every
+The knobs are tuned one at a time, in this order (the schedule after the
+binning block size it runs with): every
 candidate runs its stage on the solver's own buffers and shape (records
 already set), timed with HIP events on the current stream, best of ``reps``
-after one warm-up.  Every candidate yields the same histogram and spectrum up
-to the order of floating-point adds, so tuning never changes results beyond
-round-off.  Results are cached per (device name, shape, B, precision) in
+after one warm-up.  Every candidate yields the same histogram and spectrum
+bit for bit (the histogram sums are fixed point; the other kernels keep
+their reduction order), so tuning never changes results.  Results are cached per (device name, shape, B, precision) in
 memory; ``apply`` sets the knobs for a later solve (a captured graph keeps the
 launch shapes it was captured with, so tune before ``Solver.capture``).
 """
@@ -33,6 +37,7 @@ KNOBS = {
     "bin_threads": (_lib.TUNE_BIN_THREADS, (256, 512, 1024)),
     "lnphi_block": (_lib.TUNE_LNPHI_BLOCK, (128, 256)),
     "gemm_bm": (_lib.TUNE_GEMM_BM, (64, 128)),
+    "bin_schedule": (_lib.TUNE_BIN_SCHEDULE, (0, 1)),
 }
 
 _CACHE: dict = {}
@@ -58,7 +63,7 @@ def _key(s: Solver):
 def _stage(s: Solver, name: str):
     """The launch that a knob affects, on the solver's buffers."""
     L, B, cs, st = s.L, s.B, ctypes.byref(s.cs), _stream()
-    if name == "bin_threads":
+    if name in ("bin_threads", "bin_schedule"):
         return lambda: _lib.check(L.sgl_bin_cells(_ptr(s.tables), B, cs, _ptr(s.hist), st),
                                   "sgl_bin_cells")
     if name == "lnphi_block":

@@ -11,7 +11,7 @@
 using namespace sgl;
 
 namespace sgl {
-int g_sgl_tune[SGL_TUNE_COUNT] = {2, 0, 0};
+int g_sgl_tune[SGL_TUNE_COUNT] = {2, 0, 0, 0};
 }
 
 namespace {
@@ -70,7 +70,7 @@ extern "C" {
 const char* sgl_version(void) { return "sgl_amd 0.1 (gfx950)"; }
 
 sgl_status sgl_set_tuning(int32_t knob, int32_t value) {
-  static const int kMax[SGL_TUNE_COUNT] = {2, 1, 1};
+  static const int kMax[SGL_TUNE_COUNT] = {2, 1, 1, 1};
   if (knob < 0 || knob >= SGL_TUNE_COUNT || value < 0 || value > kMax[knob]) return SGL_ERR_INVALID;
   g_sgl_tune[knob] = value;
   return SGL_OK;

@@ -31,6 +31,9 @@
 //                     and accumulation in FP64;
 //   k_amp_binned      the slice amplitudes from the coarse records, the same
 //                     way, at every kPsi2Stride-th k node.
+#include <algorithm>
+#include <mutex>
+
 #include "sgl_amd_internal.h"
 
 namespace sgl {
@@ -850,13 +853,20 @@ __device__ __forceinline__ void bin_slice(const double* __restrict__ cosmo,
                   kBinThreads);
 }
 
-// One workgroup per (instance, slice); workgroups of slices past the
-// instance's own n_z return at once.  (A persistent variant -- as many
-// workgroups as the device holds, each taking (instance, slice) items from a
-// global work queue -- measured 74.1 ms against 73.7 ms here at the B = 4096
-// workload and 14.6 ms against 8.5 ms on the surrogate's cheap lattice, whose
-// slices are 16 cells per thread: the per-item queue fetch cost more than the
-// hardware dispatcher's refill of finished workgroups.)
+// Two schedules of the binning pass, chosen by the SGL_TUNE_BIN_SCHEDULE knob
+// (the autotuner times both; each slice is binned whole by one workgroup
+// either way, so both give the same bits):
+// - k_bin_cells: one workgroup per (instance, slice); workgroups of slices
+//   past the instance's own n_z return at once;
+// - k_bin_cells_queue: persistent -- as many workgroups as the device holds
+//   at once, each taking (instance, slice) items from the work queue that
+//   k_bin_items wrote (one item per slice below the instance's n_z) until it
+//   is empty.
+// On MI355X the two tie at B = 4096 (73.7 and 74.1 ms; the autotuner's timings
+// of sgl_bin_cells differ by 0.1 %), and the first wins on the surrogate's
+// cheap lattice of 16 cells per thread (8.5 against 14.6 ms): there the
+// per-item queue fetch and its barrier cost more than the hardware
+// dispatcher's refill of finished workgroups.
 template <int kBinThreads, bool kEll>
 __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(kBinThreads == 256 ? 4 : (kEll ? 7 : 8)))) void k_bin_cells(const double* __restrict__ cosmo,
                                                            const double* __restrict__ uedge,
@@ -867,6 +877,63 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(kBi
   const int n_z = (int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10];
   if (i >= n_z) return;                         // slices past n_z are never read
   bin_slice<kBinThreads, kEll>(cosmo, uedge, rows, sh, hist_layout(sh, gridDim.y), b, i, H, hist);
+}
+
+template <int kBinThreads, bool kEll>
+__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(kBinThreads == 256 ? 4 : (kEll ? 7 : 8)))) void k_bin_cells_queue(
+    const double* __restrict__ cosmo, const double* __restrict__ uedge,
+    const double* __restrict__ rows, sgl_shape sh, int B, double* __restrict__ hist) {
+  __shared__ FineHist H;
+  __shared__ int s_item;
+  const int t = threadIdx.x;
+  const HistLayout HL = hist_layout(sh, B);
+  int* queue = reinterpret_cast<int*>(hist + HL.items);
+  const int n_items = queue[1];                 // written by k_bin_items before this kernel
+  for (;;) {
+    if (t == 0) s_item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int item = s_item;
+    if (item >= n_items) break;                 // uniform: every thread read the same item
+    const int packed = queue[2 + item];
+    bin_slice<kBinThreads, kEll>(cosmo, uedge, rows, sh, HL, packed >> 7, packed & 127, H, hist);
+    __syncthreads();                            // the flush is done before the next item
+  }
+}
+
+// Work items of the queue schedule: item b * 128 + i for every slice
+// i < n_z[b] (n_z <= 96), in instance order (exclusive scan of the slice
+// counts in one workgroup), and the queue counter reset -- by a kernel, so a
+// captured graph replays it.
+constexpr int kItemThreads = 1024;
+__global__ __launch_bounds__(kItemThreads) void k_bin_items(const double* __restrict__ cosmo,
+                                                            sgl_shape sh, int B,
+                                                            double* __restrict__ hist) {
+  __shared__ int scan[kItemThreads];
+  const int t = threadIdx.x;
+  const HistLayout HL = hist_layout(sh, B);
+  int* queue = reinterpret_cast<int*>(hist + HL.items);
+  const int per = (B + kItemThreads - 1) / kItemThreads;
+  const int b0 = min(B, t * per), b1 = min(B, b0 + per);
+  int cnt = 0;
+  for (int b = b0; b < b1; ++b)
+    cnt += min(max((int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10], 0), sh.n_z_max);
+  scan[t] = cnt;
+  __syncthreads();
+  for (int o = 1; o < kItemThreads; o <<= 1) {  // inclusive Hillis-Steele scan
+    const int v = (t >= o) ? scan[t - o] : 0;
+    __syncthreads();
+    scan[t] += v;
+    __syncthreads();
+  }
+  int pos = scan[t] - cnt;
+  for (int b = b0; b < b1; ++b) {
+    const int nz = min(max((int)cosmo[(size_t)b * SGL_COSMO_STRIDE + 10], 0), sh.n_z_max);
+    for (int i = 0; i < nz; ++i) queue[2 + pos++] = b * 128 + i;
+  }
+  if (t == kItemThreads - 1) {
+    queue[0] = 0;
+    queue[1] = scan[t];
+  }
 }
 
 // Filament pass (SGL_FLAG_FILAMENTS): every filament cell of the slice --
@@ -1278,6 +1345,48 @@ int launch_vp_cells(const double* rows, const double* frows, const double* cosmo
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
+// Workgroups of `threads` the device holds at once for `kernel` (occupancy x
+// compute units), looked up once per kernel, size and device.
+static int resident_workgroups(const void* kernel, int threads) {
+  struct Entry { const void* k; int dev, threads, n; };
+  static Entry cache[32];
+  static int used = 0;
+  static std::mutex mu;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mu);
+  for (int e = 0; e < used; ++e)
+    if (cache[e].k == kernel && cache[e].dev == dev && cache[e].threads == threads)
+      return cache[e].n;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = 1;
+  const int n = per_cu * cus;
+  if (used < 32) cache[used++] = Entry{kernel, dev, threads, n};
+  return n;
+}
+
+// The binning kernel of one launch shape, on either schedule.
+template <int kThreads, bool kEll>
+static void launch_bin_cells(const double* cosmo, const double* uedge, const double* rows,
+                             const sgl_shape& sh, int B, double* hist, hipStream_t st) {
+  if (g_sgl_tune[SGL_TUNE_BIN_SCHEDULE] == 1) {
+    hipLaunchKernelGGL(k_bin_items, dim3(1), dim3(kItemThreads), 0, st, cosmo, sh, B, hist);
+    const int n = resident_workgroups(reinterpret_cast<const void*>(&k_bin_cells_queue<kThreads, kEll>),
+                                      kThreads);
+    hipLaunchKernelGGL((k_bin_cells_queue<kThreads, kEll>),
+                       dim3((unsigned)std::max(1, std::min(n, B * sh.n_z_max))), dim3(kThreads), 0,
+                       st, cosmo, uedge, rows, sh, B, hist);
+  } else {
+    hipLaunchKernelGGL((k_bin_cells<kThreads, kEll>), dim3(sh.n_z_max, B), dim3(kThreads), 0, st,
+                       cosmo, uedge, rows, sh, hist);
+  }
+}
+
 int launch_bin_pass(const double* rows, const double* frows, const double* cosmo,
                     const double* uedge, const sgl_shape& sh, int B, double* hist,
                     hipStream_t st) {
@@ -1289,16 +1398,16 @@ int launch_bin_pass(const double* rows, const double* frows, const double* cosmo
   const bool ell = (sh.flags & SGL_FLAG_ELL) != 0;
   switch (g_sgl_tune[SGL_TUNE_BIN_THREADS]) {
     case 0:
-      if (ell) hipLaunchKernelGGL((k_bin_cells<256, true>), g, dim3(256), 0, st, cosmo, uedge, rows, sh, hist);
-      else hipLaunchKernelGGL((k_bin_cells<256, false>), g, dim3(256), 0, st, cosmo, uedge, rows, sh, hist);
+      if (ell) launch_bin_cells<256, true>(cosmo, uedge, rows, sh, B, hist, st);
+      else launch_bin_cells<256, false>(cosmo, uedge, rows, sh, B, hist, st);
       break;
     case 2:
-      if (ell) hipLaunchKernelGGL((k_bin_cells<1024, true>), g, dim3(1024), 0, st, cosmo, uedge, rows, sh, hist);
-      else hipLaunchKernelGGL((k_bin_cells<1024, false>), g, dim3(1024), 0, st, cosmo, uedge, rows, sh, hist);
+      if (ell) launch_bin_cells<1024, true>(cosmo, uedge, rows, sh, B, hist, st);
+      else launch_bin_cells<1024, false>(cosmo, uedge, rows, sh, B, hist, st);
       break;
     default:
-      if (ell) hipLaunchKernelGGL((k_bin_cells<512, true>), g, dim3(512), 0, st, cosmo, uedge, rows, sh, hist);
-      else hipLaunchKernelGGL((k_bin_cells<512, false>), g, dim3(512), 0, st, cosmo, uedge, rows, sh, hist);
+      if (ell) launch_bin_cells<512, true>(cosmo, uedge, rows, sh, B, hist, st);
+      else launch_bin_cells<512, false>(cosmo, uedge, rows, sh, B, hist, st);
       break;
   }
   if (sh.flags & SGL_FLAG_FILAMENTS)

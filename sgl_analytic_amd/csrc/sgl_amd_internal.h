@@ -200,7 +200,7 @@ __device__ inline double knode_at(const sgl_shape& s, int n) {
 // two per slice for the positive coarse records, the 80-byte RowF record of
 // every (slice, mass) row, and the 32-byte record of every column.
 struct HistLayout {
-  size_t partial, cpart, frec, fcnt, srec, scnt, fpos, fpfx, cpos, cpfx, rowf, cols, total;
+  size_t partial, cpart, frec, fcnt, srec, scnt, fpos, fpfx, cpos, cpfx, rowf, cols, items, total;
 };
 __host__ __device__ inline size_t round32(size_t n) { return (n + 31) & ~size_t(31); }
 __host__ __device__ inline HistLayout hist_layout(const sgl_shape& s, int B) {
@@ -217,7 +217,10 @@ __host__ __device__ inline HistLayout hist_layout(const sgl_shape& s, int B) {
   L.cpfx = L.cpos + round32((size_t)B * s.n_z_max * (kCoarsePos + 1));
   L.rowf = L.cpfx + round32((size_t)B * s.n_z_max * (kCoarsePos + 1) * kTayC);
   L.cols = L.rowf + round32((size_t)B * s.n_z_max * s.n_m * (sizeof(RowF) / sizeof(double)));
-  L.total = L.cols + round32((size_t)s.n_b * 4);
+  // work queue of the persistent binning schedule (ints): [0] next item,
+  // [1] item count, [2 ..] one (instance, slice) item per slice
+  L.items = L.cols + round32((size_t)s.n_b * 4);
+  L.total = L.items + round32(((size_t)B * s.n_z_max + 2 + 1) / 2);
   return L;
 }
 

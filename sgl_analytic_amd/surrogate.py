@@ -355,6 +355,7 @@ class Inference:
         L = self.L
         self.hsh = cheap_shape(replace(self.shape, n_z_max=self.n_z_max))
         self.cheap = Solver(self.hsh, min(self.B, self.chunk), path="binned", precision=PREC_FP32)
+        self.tuning = None          # launch knobs of the cheap solves (tune())
         # inversion side: psi2 is inside the exponent; one slice, only the k
         # grid of the tables is read (its offset does not depend on B)
         self.sh = replace(self.shape, clustering=False, n_z_max=1)
@@ -385,7 +386,34 @@ class Inference:
         r[:, 10] = cheap_slices(r[:, 10])
         self.records = torch.as_tensor(r, device=torch.device("cuda:0"))
 
+    def tune(self):
+        """Autotune the cheap solver's launch knobs on the first chunk of the
+        current records (its slices are far smaller than the full solver's,
+        so the full solver's choice need not suit it); run() applies the
+        choice around its cheap solves and restores the knobs after."""
+        from . import autotune
+        c = self.cheap.B
+        rec = self.records[:c]
+        if rec.shape[0] < c:
+            rec = torch.cat([rec, rec[-1:].expand(c - rec.shape[0], -1)])
+        self.cheap.records.copy_(rec)
+        before = autotune.current()
+        self.tuning = autotune.tune(self.cheap)["choice"]
+        autotune.apply(before)
+        return self.tuning
+
     def cheap_solve(self):
+        from . import autotune
+        before = autotune.current() if self.tuning else None
+        if self.tuning:
+            autotune.apply(self.tuning)
+        try:
+            self._cheap_chunks()
+        finally:
+            if before:
+                autotune.apply(before)
+
+    def _cheap_chunks(self):
         c = self.cheap.B
         for a in range(0, self.B, c):
             b = min(self.B, a + c)
