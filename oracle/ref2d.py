@@ -428,12 +428,12 @@ def build(cos: Cosmology, zs: float, ell=False, fil=False, keep_points=False, ed
     """build_population(..., kap_floor=1e-8, trunc="r200", zint="midpoint")
     without clustering: every row's cells (decorated by subhalos on hosts
     that carry them when ``sub``, oracle/sub.py) binned into the fine lattice
-    with weight and centroid sums.  With subhalos the shells are shared out
-    over ``workers`` processes (default: up to 8), each binning its shells,
-    and the partial sums are added in worker order."""
+    with weight and centroid sums.  With any ingredient beyond plain halos the
+    shells are shared out over ``workers`` processes (default: up to 8), each
+    binning its shells, and the partial sums are added in worker order."""
     n_sh = len(z_shells(zs))
     if workers is None:
-        workers = min(8, os.cpu_count() or 1) if (sub or xi) else 1
+        workers = min(8, os.cpu_count() or 1) if (sub or xi or ell or fil) else 1
     workers = max(1, min(int(workers), n_sh))
     if workers == 1 or keep_points:
         accs = [_accumulate(cos, zs, ell, fil, edge, sub, None, keep_points, xi=xi, pos=pos)]
@@ -491,6 +491,7 @@ def lam(pop: Population, kk, kg, kblock=256, gblock=512):
     sum_b w_b (e^{ix} - 1 - ix) exact per bin (compensated cos - 1 and the
     sin - x series below |x| < 1e-2, sgl_full.py:2598-2611) plus the dense
     product sum_b w_b e^{i kk k_b} (J0 - 1) (sgl_direct.py:40-51)."""
+    from concurrent.futures import ThreadPoolExecutor
     kk = np.atleast_1d(np.asarray(kk, float))
     kg = np.atleast_1d(np.asarray(kg, float))
     nk = kk.size
@@ -498,7 +499,8 @@ def lam(pop: Population, kk, kg, kblock=256, gblock=512):
     w, k0, g0 = pop.w, pop.k, pop.g
     packed = np.empty((2 * nk, w.size))
     a1 = np.empty(nk, complex)
-    for lo in range(0, nk, kblock):
+
+    def kblock_rows(lo):
         hi = min(lo + kblock, nk)
         ph = np.outer(kk[lo:hi], k0)
         hs = np.sin(0.5 * ph)
@@ -510,10 +512,20 @@ def lam(pop: Population, kk, kg, kblock=256, gblock=512):
         a1[lo:hi] = (cm1 @ w) + 1j * (smp @ w)
         packed[lo:hi] = (cm1 + 1.0) * w
         packed[nk + lo:nk + hi] = sn * w
-    for glo in range(0, kg.size, gblock):
+
+    def gblock_cols(glo):
         ghi = min(glo + gblock, kg.size)
         v = packed @ j0m1(np.outer(g0, kg[glo:ghi]))
         out[:, glo:ghi] = v[:nk] + 1j * v[nk:] + a1[:, None]
+
+    # blocks are independent (each writes its own rows / columns), so the
+    # result does not depend on how they are spread over threads (numpy's
+    # elementwise kernels release the GIL); one BLAS thread per block thread
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1, user_api="blas"), \
+            ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        list(ex.map(kblock_rows, range(0, nk, kblock)))
+        list(ex.map(gblock_cols, range(0, kg.size, gblock)))
     return out
 
 
