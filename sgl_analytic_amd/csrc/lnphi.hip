@@ -649,18 +649,19 @@ __device__ __forceinline__ void fine_moments(const FineHist& H, int f, int sw, i
 
 // Add (or store) the slice's fine partials (w, w m, w (m^2 + v)) and coarse
 // moments from H.  Each bin's FP64 moments are formed once and left in H in
-// place of its sums (as doubles: w, m, v, beta w), which the coarse pass then
+// place of its sums (the bits of w, m, v, beta w), which the coarse pass then
 // reads.
 __device__ inline void fine_hist_flush(FineHist& H, int sw, int sb, double* __restrict__ P,
                                        double* __restrict__ Q, bool add, int t, int nt) {
-  double* D = reinterpret_cast<double*>(H.d);
+  auto put = [&](int q, double x) { H.d[q] = (unsigned long long)__double_as_longlong(x); };
+  auto get = [&](int q) { return __longlong_as_double((long long)H.d[q]); };
   for (int f = t; f < kFineBins; f += nt) {
     double w, m, v, bw;
     fine_moments(H, f, sw, sb, w, m, v, bw);
-    D[f] = w;
-    D[kFineBins + f] = m;
-    D[2 * kFineBins + f] = v;
-    D[3 * kFineBins + f] = bw;
+    put(f, w);
+    put(kFineBins + f, m);
+    put(2 * kFineBins + f, v);
+    put(3 * kFineBins + f, bw);
     const double wx = w * m, wxx = w * (m * m + v);
     if (add) {
       if (w == 0.0) continue;
@@ -677,8 +678,8 @@ __device__ inline void fine_hist_flush(FineHist& H, int sw, int sb, double* __re
   for (int cb = t; cb < kSliceBins; cb += nt) {
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;
     for (int f = cb * kCoarse; f < (cb + 1) * kCoarse; ++f) {
-      if (D[f] == 0.0) continue;
-      const double m = D[kFineBins + f], v = D[2 * kFineBins + f], bw = D[3 * kFineBins + f];
+      if (get(f) == 0.0) continue;
+      const double m = get(kFineBins + f), v = get(2 * kFineBins + f), bw = get(3 * kFineBins + f);
       m0 += bw;
       m1 += bw * m;
       m2 += bw * (m * m + v);
@@ -761,7 +762,7 @@ __device__ __forceinline__ void bin_slice(const double* __restrict__ cosmo,
   // each holds the same bound without a barrier.
   double sw = 0.0, sb = 0.0;
   for (int j = t & 63; j < sh.n_m; j += 64) {
-    const double wa = RF[j].warea;
+    const double wa = fabs(RF[j].warea);
     sw += wa;
     sb += fabs(RF[j].beta) * wa;
   }
@@ -978,7 +979,7 @@ __global__ __launch_bounds__(kFilThreads) void k_bin_fil(const double* __restric
   double sw = 0.0, sb = 0.0;
   for (int j = t; j < sh.n_m; j += kFilThreads) {
     const double* R = FR + (size_t)j * kFilRowStride;
-    const double wr = R[2] * kPi * R[1] * R[1];
+    const double wr = fabs(R[2]) * kPi * R[1] * R[1];
     sw += wr;
     sb += fabs(R[3]) * wr;
   }
