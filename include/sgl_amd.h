@@ -576,6 +576,28 @@ sgl_status sgl_ref_bin(int64_t n_cells, const int32_t* key, const double* kap, c
                        void* stream);
 
 /*
+ * The Gaussian large-scale-structure term of clustering="limber" with the
+ * "bias" ingredient (pipeline.py:126-134, 195-202): the population is built
+ * without SGL_FLAG_CLUSTERING, and ln Phi gains -var/2 (k_kappa^2 +
+ * k_gamma^2 / 2), sigma_kappa^2 gains var.
+ * sgl_ref_lss_var: var of the instance in tables (after sgl_ref_tables_build
+ * with the same flags) into *var (device, one double); ws:
+ * sgl_ref_lss_bytes() of device memory.  Replaces: limber_lss_var
+ * (sgl_full.py:2541-2572).
+ * sgl_ref_lss_psi: the term on the tensor grid k_k x k_g into psi [nk][ng][2]
+ * (the psi input of sgl_ref_phi).  sgl_ref_lss_axis: the term added to out
+ * [n][2] of sgl_ref_lam_axis (axis 0: k_kappa = k, axis 1: k_gamma = k).
+ * Replaces: the lss_var line of Lam.__call__ (sgl_full.py:3104-3105) on the
+ * grid and on the reach probes (sgl_full.py:38-47).
+ */
+size_t sgl_ref_lss_bytes(void);
+sgl_status sgl_ref_lss_var(const void* tables, int32_t flags, void* ws, double* var, void* stream);
+sgl_status sgl_ref_lss_psi(double var, const double* k_k, int32_t nk, const double* k_g,
+                           int32_t ng, double* psi, void* stream);
+sgl_status sgl_ref_lss_axis(double var, const double* k, int32_t n, int32_t axis, double* out,
+                            void* stream);
+
+/*
  * Lambda on an axis at n probe points k (out: [n][2]): axis 0 Lambda(k, 0),
  * axis 1 Lambda(0, k).  Replaces: the Lam calls of _reach (sgl_full.py:38-47).
  */

@@ -718,17 +718,63 @@ XI_BODY = np.linspace(-1.0, 1.0, 1601)    # cli.py / explorer body grid
 XI_TAIL = np.linspace(1.005, 9.0, 1600)   # and tail grid
 
 
+def limber_lss_var(cos: Cosmology, zs, lmin=1.0, lmax=1e7, nchi=600):
+    """limber_lss_var (sgl_full.py:2541-2572): the Limber variance of the
+    Gaussian large-scale-structure convergence of clustering="limber",
+      v = int dchi W^2 Dg^2 int_{lmin/f_K}^{lmax/f_K} dk k P0(k) / (2 pi),
+      W = 3/2 Om H0^2 (1 + z) f_K(chi) f_K(chi_s - chi) / f_K(chi_s),
+    on the reference's nodes: chi(z) on 400 uniform z and z(chi) by linear
+    interpolation, 600 uniform chi in [1e-4, 1 - 1e-4] chi_s, the k integral
+    a cumulative trapezoid on 40000 geometric k in [1e-7, 1e5] read by
+    linear interpolation, the chi integral a trapezoid."""
+    chis = cos.chi(zs)
+    zt = np.linspace(0.0, zs, 400)
+    ct = np.array([cos.chi(z) for z in zt])
+    chi = np.linspace(chis * 1e-4, chis * (1 - 1e-4), nchi)
+    z = np.interp(chi, ct, zt)
+    H0 = 100.0 * cos.h / CKMS
+    if cos.Ok == 0.0:
+        transverse = chi
+        W = 1.5 * cos.Om * H0 ** 2 * (1 + z) * chi * (chis - chi) / chis
+    else:
+        transverse = cos.f_K(chi)
+        W = 1.5 * cos.Om * H0 ** 2 * (1 + z) * transverse * cos.f_K(chis - chi) / cos.f_K(chis)
+    D = np.array([cos.Dg(zz) for zz in z])
+    Kg = np.geomspace(1e-7, 1e5, 40000)
+    f = Kg * cos.P0(Kg) / (2 * pi)
+    cum = np.concatenate([[0.0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(Kg))])
+    I = np.interp(lmax / transverse, Kg, cum) - np.interp(lmin / transverse, Kg, cum)
+    return float(np.trapezoid(W ** 2 * D ** 2 * I, chi))
+
+
+def lss_term(v, kk, kg):
+    """The limber term of ln Phi, -v/2 (k_kappa^2 + k_gamma^2 / 2)
+    (sgl_full.py:3104-3105)."""
+    kk = np.atleast_1d(np.asarray(kk, float))
+    kg = np.atleast_1d(np.asarray(kg, float))
+    return -0.5 * v * (kk[:, None] ** 2 + 0.5 * kg[None, :] ** 2)
+
+
 def run(zs, cosmo: dict = None, ell=False, fil=False, xi_out=None, keep=False, edge="r200",
-        sub=False, bias=False):
+        sub=False, bias=False, lss=False):
     """pipeline.run (pipeline.py:114-275) with the Gaussian pair closure.
     ``bias``: the xi_lin clustering term (oracle/xilin.py) in Lambda and in
     sigma_kappa (sgl_full.py:3094-3128, 3437-3446); bias="dense" keeps every
     shell's rows uncompressed and unpruned (one per k_perp node).
+    ``lss``: clustering="limber" with the bias ingredient instead
+    (pipeline.py:195-202): no xi_lin deposit, the Gaussian term of
+    limber_lss_var in Lambda and in sigma_kappa (sgl_full.py:3104-3105, 3439).
     xi_out None: the body and tail grids of the CLI (pipeline.py:211-215)."""
+    if lss and bias:
+        raise ValueError("the limber term replaces the xi_lin term")
     cos = Cosmology(**(cosmo or {}))
     pop = build(cos, zs, ell=ell, fil=fil, edge=edge, sub=sub, xi=bias)
     xo = np.concatenate([XI_BODY, XI_TAIL]) if xi_out is None else np.asarray(xi_out, float)
-    if bias:
+    if lss:
+        v = pop.lss_var = limber_lss_var(cos, zs)
+        sig = float(sqrt(float(np.sum(pop.w * pop.k ** 2)) + v))
+        res = invert(lambda a, b: lam(pop, a, b) + lss_term(v, a, b), sig, xo, keep=keep)
+    elif bias:
         from . import xilin
         psi = pop.psi2 = xilin.Psi2(pop.xi_groups, prune=0 if bias == "dense" else xilin.XI_PRUNE)
         sig = float(sqrt(pop.sigma() ** 2 + psi.var()))

@@ -39,6 +39,7 @@ EXPORTS = (
     "sgl_ref_vp_runs_bytes", "sgl_ref_vp_runs", "sgl_ref_vp_state_bytes", "sgl_ref_vp_setup",
     "sgl_ref_vp_variance", "sgl_ref_vp_rates", "sgl_ref_vp_lnr_bytes", "sgl_ref_vp_lnr",
     "sgl_ref_vp_qxi", "sgl_ref_vp_spline_bytes", "sgl_ref_vp_spline", "sgl_ref_window",
+    "sgl_ref_lss_bytes", "sgl_ref_lss_var", "sgl_ref_lss_psi", "sgl_ref_lss_axis",
 )
 NOALLOC_SCRATCH_DOUBLES = 1 << 22   # capacity of sgl_lnphi_exact_noalloc's own array
 REF_MAX_SHELLS, REF_MASSES, REF_FINE_BINS, REF_ROOT_STRIDE = 104, 99, 961 * 300, 128
@@ -131,6 +132,8 @@ def load(path: str = LIB_PATH):
     L.sgl_ref_vp_lnr_bytes.argtypes = [i32, i32, i32, i32]
     L.sgl_ref_vp_spline_bytes.restype = sz
     L.sgl_ref_vp_spline_bytes.argtypes = [i32, i32, i32]
+    L.sgl_ref_lss_bytes.restype = sz
+    L.sgl_ref_lss_bytes.argtypes = []
     sigs = {
         "sgl_tables_build": [vp, i32, sp, vp, vp, vp],
         "sgl_lnphi_exact": [vp, i32, sp, vp, vp, vp, vp],
@@ -183,6 +186,9 @@ def load(path: str = LIB_PATH):
         "sgl_ref_vp_qxi": [vp, i32, i32, vp, f64, vp],
         "sgl_ref_vp_spline": [vp, i32, i32, i32, i32, vp, vp, vp],
         "sgl_ref_window": [vp, i32, i32, vp],
+        "sgl_ref_lss_var": [vp, i32, vp, vp, vp],
+        "sgl_ref_lss_psi": [f64, vp, i32, vp, i32, vp, vp],
+        "sgl_ref_lss_axis": [f64, vp, i32, i32, vp, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -149,31 +149,67 @@ def _solver(n_z):
     return _SOLVERS[key]
 
 
-def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200"):
+ZINT_MODES = ("engine", "midpoint", "gauss")
+CLUSTERING_MODES = ("xilin", "linear", "limber", "closure")
+
+
+def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200",
+        clustering="xilin", zint="midpoint", zint_n=2, nproc=1):
     """pipeline.run (pipeline.py:114-274) on the device: the reference's 2-D
     chain for one cosmology; returns (P_s on xi_out, info).  info carries the
-    keys of the reference's run() record (pipeline.py:243-274)."""
+    keys of the reference's run() record (pipeline.py:243-274).
+    The arguments are checked as pipeline.run checks them (pipeline.py:139-164;
+    the same ValueErrors).  clustering: "xilin" (the default: the xi_lin pair
+    term) or "limber" (the Gaussian large-scale-structure term of
+    limber_lss_var, pipeline.py:195-202); without the "bias" ingredient every
+    clustering mode but "closure" gives the same chain (no two-point term).
+    Not built, NotImplementedError: "linear" with "bias" (the 20 Mpc
+    slice-covariance Cox term), "closure", edge "kthr", zint "engine" /
+    "gauss", subhalos with edge "rvir".  nproc: the reference's host worker
+    count, checked and recorded (the device needs no host pool)."""
+    if clustering not in CLUSTERING_MODES:
+        raise ValueError(f"clustering must be 'xilin', 'linear', 'limber' or 'closure', "
+                         f"got {clustering!r}")
     if not np.isscalar(z_s) or not np.isfinite(z_s) or z_s <= 0:
         raise ValueError("z_s must be a finite positive number")
-    if edge not in ("r200", "rvir"):
-        if edge == "kthr" or isinstance(edge, (int, float)):
-            raise NotImplementedError(f"edge={edge!r} is not built (r200, rvir)")
-        raise ValueError(f"edge must be 'r200' or 'rvir', got {edge!r}")
+    if isinstance(nproc, bool) or not isinstance(nproc, (int, np.integer)) or nproc < 1:
+        raise ValueError("nproc must be a positive integer")
+    if edge not in ("r200", "rvir", "kthr"):
+        raise ValueError("edge must be 'r200', 'rvir' or 'kthr'")
     if closure not in ("gauss", "vp"):
         raise ValueError(f"xi_closure must be 'gauss' or 'vp', got {closure!r}")
     s = on(config)
-    cp = check_cosmology(cosmo)
-    if closure != "gauss" and "bias" not in s:
+    use_xi = clustering == "xilin" and "bias" in s
+    if use_xi and edge == "kthr":
+        raise ValueError("clustering='xilin' needs edge='r200' or 'rvir'")
+    if closure != "gauss" and not use_xi:
         raise ValueError(f"xi_closure={closure!r} needs xilin clustering and the bias ingredient")
+    if zint not in ZINT_MODES:
+        raise ValueError(f"zint must be one of {ZINT_MODES}, got {zint!r}")
+    if zint == "gauss" and int(zint_n) < 1:
+        raise ValueError("zint_n must be >= 1")
+    cp = check_cosmology(cosmo)
+    if edge == "kthr":
+        raise NotImplementedError("edge='kthr' (the engine's Gaussian faint-lens split) is not built")
+    if clustering == "closure":
+        raise NotImplementedError("clustering='closure' (the lognormal Hermite closure) is not built")
+    if clustering == "linear" and "bias" in s:
+        raise NotImplementedError("clustering='linear' with the bias ingredient (the slice-"
+                                  "covariance Cox term) is not built; use 'xilin' or 'limber'")
+    if zint != "midpoint":
+        raise NotImplementedError(f"zint={zint!r} is not built (midpoint)")
     if "sub" in s and edge != "r200":
         raise NotImplementedError("subhalos are built with edge='r200' only (clumps truncated at "
                                   "their r200, the reference's pairing for that edge)")
+    lss = clustering == "limber" and "bias" in s
     from . import ref2d
     t0 = time.perf_counter()
     pop = ref2d.build(cp, float(z_s), ell="ell" in s, fil="fil" in s, edge=edge, sub="sub" in s,
-                      bias="bias" in s, vp=closure == "vp")
+                      bias=use_xi, vp=closure == "vp", lss=lss)
     t1 = time.perf_counter()
-    x = np.asarray(xi_out, dtype=np.float64)
+    # xi_out None: the CLI's body and tail grids (pipeline.py:211-215)
+    x = (np.concatenate([np.linspace(-1.0, 1.0, 1601), np.linspace(1.005, 9.0, 1600)])
+         if xi_out is None else np.asarray(xi_out, dtype=np.float64))
     res = ref2d.invert(pop, x)
     t2 = time.perf_counter()
     info = record_2d(cp, config, s, closure, edge,
@@ -181,7 +217,8 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200
                           n_points=pop.n_points),
                      dict(sigma=res.sigma, mean=res.mean, var=res.var, skew=res.skew,
                           sigmaDL=res.sigmaDL, grid=res.grid),
-                     dict(build=t1 - t0, lam=0.0, invert=t2 - t1))
+                     dict(build=t1 - t0, lam=0.0, invert=t2 - t1), clustering=clustering,
+                     lss_var=pop.lss_var, nproc=int(nproc))
     info["zs"] = float(z_s)
     if pop.vp is not None:
         v = pop.vp
@@ -192,7 +229,8 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200
     return res.P, info
 
 
-def record_2d(cp, config, ingredients, closure, edge, pop, res, t_stages):
+def record_2d(cp, config, ingredients, closure, edge, pop, res, t_stages, clustering="xilin",
+              lss_var=0.0, nproc=1):
     """The run() record of the reference (pipeline.py:243-274; every top-level
     key of its stored PDFs) with the 2-D chain's values: pop = bookkeeping of
     the population (kbar, N_expl, N_fil, n_bins, n_points), res = sigma,
@@ -202,13 +240,14 @@ def record_2d(cp, config, ingredients, closure, edge, pop, res, t_stages):
               radiation=cp["radiation"], Omega_r=Or)
     return dict(
         t_stages=dict(t_stages), cfg=config if isinstance(config, str) else sorted(ingredients),
-        zint="midpoint", zint_n=None, clustering="xilin",
+        zint="midpoint", zint_n=None, clustering=clustering,
         xi_kperp=(dict(kmin=1e-4, kmax=1e4, per_decade=[2, 2, 12, 8, 6, 4, 4, 4], mmax=2)
-                  if "bias" in ingredients else None), edge=edge, lss_var=0.0,
+                  if "bias" in ingredients and clustering == "xilin" else None), edge=edge,
+        lss_var=float(lss_var),
         kap_floor=1e-8, kbar=float(pop["kbar"]), h=cp["h"], Om=cp["Om"], s8=cp["s8"],
         sigma8=cp["s8"], Ob=cp["Ob"], ns=cp["ns"], zeq=cp["zeq"], Ok=cp["Ok"], w0=cp["w0"],
         wa=cp["wa"], growth_mode=cp["growth_mode"], radiation=cp["radiation"], Omega_r=Or,
-        dark_energy=de, nord=1, grid_scale=1.0, split=10.0, nproc=1,
+        dark_energy=de, nord=1, grid_scale=1.0, split=10.0, nproc=nproc,
         sub_kw=dict(SUB_KW) if "sub" in ingredients else None,
         pair_angle="uniform" if "sub" in ingredients else None, pair_scale=None, k3=False,
         exactS=False, s_taper="cos2",

@@ -1612,6 +1612,31 @@ __global__ void k_ref_init(const double* __restrict__ src, double* __restrict__ 
   }
 }
 
+// The Gaussian large-scale-structure term of clustering="limber":
+// -var/2 (k_kappa^2 + k_gamma^2 / 2) added to ln Phi (sgl_full.py:3104-3105;
+// the shear carries the convergence's variance split over two components).
+// On the tensor grid into psi [nk][ng][2] (the clustering-term input of
+// sgl_ref_phi), or added to the axis probes out [n][2].
+__global__ __launch_bounds__(256) void k_ref_lss_psi(double var, const double* __restrict__ k_k,
+                                                     int nk, const double* __restrict__ k_g,
+                                                     int ng, double* __restrict__ psi) {
+  const long long n = (long long)nk * ng;
+  const double hv = 0.5 * var;
+  for (long long q = blockIdx.x * 256ll + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
+    const double a = k_k[q / ng], b = k_g[q % ng];
+    psi[2 * q] = -(hv * (a * a + 0.5 * (b * b)));
+    psi[2 * q + 1] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ref_lss_axis(double var, const double* __restrict__ k,
+                                                      int n, int axis, double* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double hv = 0.5 * var, kk = k[i];
+  out[2 * i] -= (axis == 0) ? hv * (kk * kk + 0.0) : hv * (0.0 + 0.5 * (kk * kk));
+}
+
 }  // namespace sgl
 
 using namespace sgl;
@@ -1897,6 +1922,36 @@ sgl_status sgl_ref_moments(const double* xw, const double* pw, int32_t n, double
                            void* stream) {
   if (!xw || !pw || !mom || n < 2) return SGL_ERR_INVALID;
   hipLaunchKernelGGL(k_ref_moments, dim3(1), dim3(256), 0, ref_stream(stream), xw, pw, n, mom);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+size_t sgl_ref_lss_bytes(void) { return ref_lss_doubles() * sizeof(double); }
+
+sgl_status sgl_ref_lss_var(const void* tables, int32_t flags, void* ws, double* var, void* stream) {
+  if (!ref_flags_ok(flags) || !tables || !ws || !var) return SGL_ERR_INVALID;
+  const RefLayout L = ref_layout(flags);
+  const double* T = static_cast<const double*>(tables);
+  return ref_rc(launch_ref_lss(T + L.T.cosmo, T + L.T.derived, T + L.T.gdense,
+                               static_cast<double*>(ws), var, ref_stream(stream)));
+}
+
+sgl_status sgl_ref_lss_psi(double var, const double* k_k, int32_t nk, const double* k_g,
+                           int32_t ng, double* psi, void* stream) {
+  if (!k_k || !k_g || !psi || nk < 1 || ng < 1 || !(var >= 0.0) || !std::isfinite(var))
+    return SGL_ERR_INVALID;
+  const long long n = (long long)nk * ng;
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_ref_lss_psi, dim3(blocks), dim3(256), 0, ref_stream(stream), var, k_k, nk,
+                     k_g, ng, psi);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
+sgl_status sgl_ref_lss_axis(double var, const double* k, int32_t n, int32_t axis, double* out,
+                            void* stream) {
+  if (!k || !out || n < 1 || (axis != 0 && axis != 1) || !(var >= 0.0) || !std::isfinite(var))
+    return SGL_ERR_INVALID;
+  hipLaunchKernelGGL(k_ref_lss_axis, dim3((n + 255) / 256), dim3(256), 0, ref_stream(stream), var,
+                     k, n, axis, out);
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 

@@ -108,7 +108,9 @@ inline TablesLayout tables_layout(const sgl_shape& s, int B) {
   // the reference's sigma spline: subhalos, and M* of the reference grid's ellipticity
   const bool spl = sub || ((s.flags & SGL_FLAG_REF_GRID) && (s.flags & SGL_FLAG_ELL));
   L.sigspl = take(spl ? (size_t)B * kSigSplDoubles : 0);
-  L.gdense = take(sub ? (size_t)B * (kGrowthSteps + 1) * 2 : 0);
+  // the dense growth table: subhalos, and every reference-grid build (the
+  // Limber term's D(z))
+  L.gdense = take((sub || (s.flags & SGL_FLAG_REF_GRID)) ? (size_t)B * (kGrowthSteps + 1) * 2 : 0);
   L.total = o;
   return L;
 }
@@ -234,6 +236,9 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
 int ref_tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* m_nodes,
                      double* derived, double* slices, double* mass, double* rows, double* frows,
                      double* sigtab, double* sigspl, double* gdense, hipStream_t st);
+size_t ref_lss_doubles();
+int launch_ref_lss(const double* cosmo, const double* derived, const double* gdense, double* ws,
+                   double* var, hipStream_t st);
 int launch_fill_grids(const sgl_shape& sh, double* ue, double* kn, double* m_nodes_h,
                       hipStream_t st);
 int launch_lnphi_exact(const double* rows, const double* frows, const double* cosmo, const double* uedge,

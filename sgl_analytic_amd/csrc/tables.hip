@@ -41,22 +41,12 @@ __device__ inline double inv_e_integral(const Cosmo& c, double Or, double OL, do
   return s * r;
 }
 
-__global__ __launch_bounds__(kBgThreads) void k_background(const Cosmo* __restrict__ cosmo,
-                                                           int n_z_max, Derived* __restrict__ der,
-                                                           double* __restrict__ slices) {
-  __shared__ double chi_e[kPanels + 1];
-  __shared__ double run[kBgThreads];
-  const int b = blockIdx.x, t = threadIdx.x;
-  const Cosmo c = cosmo[b];
-  // radiation "engine": Omega_r = Omega_m / (1 + z_eq); None: no radiation
-  // (sgl.py:93-104)
-  const double Or = (c.rad_none != 0.0) ? 0.0 : c.Om / (1.0 + c.zeq);
-  const double OL = 1.0 - c.Om - c.Ok - Or;
-  const double H0 = 100.0 * c.h;
-  const double dh = kCkms / H0;
-  const double zs = c.zs;
-  const int n_z = (int)c.n_z;
-  const double dz = zs / kPanels;
+// chi at the kPanels + 1 panel edges j dz (j = 0..kPanels) into chi_e, by
+// one workgroup of kBgThreads (run: kBgThreads doubles of LDS scratch).  Ends
+// with every edge written but without a barrier after the last writes.
+__device__ inline void chi_panels(const Cosmo& c, double Or, double OL, double dh, double dz,
+                                  double* chi_e, double* run) {
+  const int t = threadIdx.x;
   for (int p = t; p < kPanels; p += kBgThreads)
     chi_e[p + 1] = dh * inv_e_integral(c, Or, OL, p * dz, (p + 1) * dz);
   __syncthreads();
@@ -78,6 +68,25 @@ __global__ __launch_bounds__(kBgThreads) void k_background(const Cosmo* __restri
 #pragma unroll
   for (int j = 0; j < kBgPer; ++j) chi_e[t * kBgPer + j + 1] = base + loc[j];
   if (t == 0) chi_e[0] = 0.0;
+}
+
+__global__ __launch_bounds__(kBgThreads) void k_background(const Cosmo* __restrict__ cosmo,
+                                                           int n_z_max, Derived* __restrict__ der,
+                                                           double* __restrict__ slices) {
+  __shared__ double chi_e[kPanels + 1];
+  __shared__ double run[kBgThreads];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const Cosmo c = cosmo[b];
+  // radiation "engine": Omega_r = Omega_m / (1 + z_eq); None: no radiation
+  // (sgl.py:93-104)
+  const double Or = (c.rad_none != 0.0) ? 0.0 : c.Om / (1.0 + c.zeq);
+  const double OL = 1.0 - c.Om - c.Ok - Or;
+  const double H0 = 100.0 * c.h;
+  const double dh = kCkms / H0;
+  const double zs = c.zs;
+  const int n_z = (int)c.n_z;
+  const double dz = zs / kPanels;
+  chi_panels(c, Or, OL, dh, dz, chi_e, run);
   __syncthreads();
   const double chis = chi_e[kPanels];
   const double dchi = chis / n_z;
@@ -150,27 +159,7 @@ __global__ __launch_bounds__(kBgThreads) void k_ref_background(const Cosmo* __re
   const double zs = c.zs;
   const int n_z = (int)c.n_z;
   const double dz = zs / kPanels;
-  for (int p = t; p < kPanels; p += kBgThreads)
-    chi_e[p + 1] = dh * inv_e_integral(c, Or, OL, p * dz, (p + 1) * dz);
-  __syncthreads();
-  double loc[kBgPer], acc = 0.0;
-#pragma unroll
-  for (int j = 0; j < kBgPer; ++j) {
-    acc += chi_e[t * kBgPer + j + 1];
-    loc[j] = acc;
-  }
-  run[t] = acc;
-  __syncthreads();
-  for (int off = 1; off < kBgThreads; off <<= 1) {
-    const double add = (t >= off) ? run[t - off] : 0.0;
-    __syncthreads();
-    run[t] += add;
-    __syncthreads();
-  }
-  const double base = (t > 0) ? run[t - 1] : 0.0;
-#pragma unroll
-  for (int j = 0; j < kBgPer; ++j) chi_e[t * kBgPer + j + 1] = base + loc[j];
-  if (t == 0) chi_e[0] = 0.0;
+  chi_panels(c, Or, OL, dh, dz, chi_e, run);
   __syncthreads();
   const double chis = chi_e[kPanels];
   const double DAs = f_k(c.Ok, H0, chis) / (1.0 + zs);
@@ -1094,6 +1083,138 @@ int tables_build(const double* cosmo, int B, const sgl_shape& sh, const double* 
   return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
 }
 
+// ------------------------------------------------------------ limber lss --
+// limber_lss_var (sgl_full.py:2541-2572): the Limber variance of a Gaussian
+// large-scale-structure convergence with the chain's own linear P0,
+//   v = int dchi W^2 Dg^2 int_{lmin/f_K(chi)}^{lmax/f_K(chi)} dk k P0(k) / (2 pi),
+//   W = 3/2 Omega_m H0^2 (1 + z) f_K(chi) f_K(chi_s - chi) / f_K(chi_s),
+// lmin = 1, lmax = 1e7, on the reference's own nodes: z(chi) linear in the
+// table of chi on 400 uniform z in [0, z_s]; 600 uniform chi in
+// [1e-4, 1 - 1e-4] chi_s; the k integral a cumulative trapezoid on 40000
+// geometric k in [1e-7, 1e5], read by linear interpolation clamped at both
+// ends; the chi integral a trapezoid.  Dg is the growth of the bias field:
+// Carroll-Press-Turner normalised at z = 0 under the legacy growth, the ODE
+// growth otherwise (sgl_full.py:521-538).  One workgroup of kBgThreads for
+// instance 0 of the tables; kn and cum (kLssK doubles each, device memory)
+// take the k nodes and the running integral; the variance goes to *var.
+constexpr int kLssZ = 400, kLssChi = 600, kLssK = 40000;
+constexpr int kLssPer = (kLssK + kBgThreads - 1) / kBgThreads;
+
+// linear interpolation between the bracketing nodes of the increasing xp,
+// clamped to fp[0] / fp[n - 1] outside [xp[0], xp[n - 1]] (np.interp)
+__device__ inline double lin_interp(double x, const double* xp, const double* fp, int n) {
+  if (!(x > xp[0])) return fp[0];
+  if (x >= xp[n - 1]) return fp[n - 1];
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    const int m = (lo + hi) >> 1;
+    if (xp[m] <= x) lo = m; else hi = m;
+  }
+  return (fp[hi] - fp[lo]) / (xp[hi] - xp[lo]) * (x - xp[lo]) + fp[lo];
+}
+
+__global__ __launch_bounds__(kBgThreads) void k_ref_lss(const Cosmo* __restrict__ cosmo,
+                                                        const Derived* __restrict__ der,
+                                                        const double* __restrict__ gd,
+                                                        double* __restrict__ kn,
+                                                        double* __restrict__ cum,
+                                                        double* __restrict__ var) {
+  __shared__ double chi_e[kPanels + 1];
+  __shared__ double run[kBgThreads];
+  __shared__ double zt[kLssZ], ct[kLssZ], xc[kLssChi], yc[kLssChi];
+  const int t = threadIdx.x;
+  const Cosmo c = cosmo[0];
+  const Derived d = der[0];
+  const double dh = kCkms / d.H0;
+  const double zs = c.zs;
+  const double dz = zs / kPanels;
+  chi_panels(c, d.Or, d.OL, dh, dz, chi_e, run);
+  // the k table: nodes 10^(-7 + 12 i / 39999) with exact end points, and
+  // this thread's run of trapezoid intervals [i, i + 1], i in [n0, n1)
+  const EH98Const e = eh98_const(c);
+  const double l0 = log10(1e-7), l1 = log10(1e5);
+  const double lstep = (l1 - l0) / (kLssK - 1);
+  auto knode = [&](int i) {
+    return (i == 0) ? 1e-7 : (i == kLssK - 1) ? 1e5 : pow(10.0, i * lstep + l0);
+  };
+  auto fk = [&](double K) {
+    const double lK = log(K);
+    const double T = eh98_T(e, K, lK);
+    return K * (d.Anorm * d.Anorm * exp(c.ns * lK) * T * T) / (2.0 * kPi);
+  };
+  const int n0 = t * kLssPer, n1 = min(kLssK - 1, n0 + kLssPer);
+  double acc = 0.0;
+  if (n0 < kLssK) {
+    double kp = knode(n0), fp = fk(kp);
+    kn[n0] = kp;
+    for (int i = n0; i < n1; ++i) {
+      const double kq = knode(i + 1), fq = fk(kq);
+      acc += 0.5 * (fq + fp) * (kq - kp);
+      cum[i + 1] = acc;
+      if (i + 1 < n0 + kLssPer) kn[i + 1] = kq;
+      kp = kq;
+      fp = fq;
+    }
+  }
+  __syncthreads();                               // chi_e complete, run[] free
+  run[t] = acc;
+  __syncthreads();
+  if (t == 0) {
+    double r = 0.0;
+    for (int q = 0; q < kBgThreads; ++q) {
+      const double v = run[q];
+      run[q] = r;
+      r += v;
+    }
+    cum[0] = 0.0;
+  }
+  __syncthreads();
+  for (int i = n0; i < n1; ++i) cum[i + 1] += run[t];
+  // chi on the z table
+  for (int i = t; i < kLssZ; i += kBgThreads) {
+    const double z = (i == kLssZ - 1) ? zs : i * (zs / (kLssZ - 1));
+    const int lo = min((int)(z / dz), kPanels - 1);
+    zt[i] = z;
+    ct[i] = chi_e[lo] + dh * inv_e_integral(c, d.Or, d.OL, lo * dz, z);
+  }
+  __syncthreads();                               // z table, k table complete
+  const double chis = d.chis;
+  const double H0 = d.H0 / kCkms, H02 = H0 * H0;
+  const double a0 = chis * 1e-4, a1 = chis * (1.0 - 1e-4);
+  const double cstep = (a1 - a0) / (kLssChi - 1);
+  const bool flat = c.Ok == 0.0;
+  const double fks = f_k(c.Ok, d.H0, chis);
+  const bool legacy = c.growth_legacy != 0.0;
+  const double g0 = cpt_growth(0.0, c.Om, c.Ok, d.Or, d.OL);
+  for (int j = t; j < kLssChi; j += kBgThreads) {
+    const double chi = (j == kLssChi - 1) ? a1 : j * cstep + a0;
+    const double z = lin_interp(chi, ct, zt, kLssZ);
+    const double tr = f_k(c.Ok, d.H0, chi);
+    const double W = flat ? 1.5 * c.Om * H02 * (1.0 + z) * chi * (chis - chi) / chis
+                          : 1.5 * c.Om * H02 * (1.0 + z) * tr * f_k(c.Ok, d.H0, chis - chi) / fks;
+    const double D = legacy ? cpt_growth(z, c.Om, c.Ok, d.Or, d.OL) / g0 : growth_dense_at(gd, z);
+    const double I = lin_interp(1e7 / tr, kn, cum, kLssK) - lin_interp(1.0 / tr, kn, cum, kLssK);
+    xc[j] = chi;
+    yc[j] = W * W * (D * D) * I;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double sum = 0.0;
+    for (int j = 0; j + 1 < kLssChi; ++j) sum += (xc[j + 1] - xc[j]) * (yc[j + 1] + yc[j]) / 2.0;
+    *var = sum;
+  }
+}
+
+size_t ref_lss_doubles() { return 2 * (size_t)kLssK; }
+
+int launch_ref_lss(const double* cosmo, const double* derived, const double* gdense, double* ws,
+                   double* var, hipStream_t st) {
+  hipLaunchKernelGGL(k_ref_lss, dim3(1), dim3(kBgThreads), 0, st,
+                     reinterpret_cast<const Cosmo*>(cosmo),
+                     reinterpret_cast<const Derived*>(derived), gdense, ws, ws + kLssK, var);
+  return hipGetLastError() == hipSuccess ? SGL_OK : SGL_ERR_HIP;
+}
+
 // Tables on the reference grid (SGL_FLAG_REF_GRID): the reference-grid
 // background, then the growth, sigma(M) (masses in Msun), M* and the rows
 // kernels of the kernel-shape tables, with dlnM the engine grid's step.
@@ -1106,8 +1227,8 @@ int ref_tables_build(const double* cosmo, int B, const sgl_shape& sh, const doub
   const bool sub = (sh.flags & SGL_FLAG_SUBHALOS) != 0;
   const bool ell = (sh.flags & SGL_FLAG_ELL) != 0;
   hipLaunchKernelGGL(k_ref_background, dim3(B), dim3(kBgThreads), 0, st, C, sh.n_z_max, D, slices);
-  hipLaunchKernelGGL(k_growth, dim3(B), dim3(64), 0, st, C, sh.n_z_max, D, slices,
-                     sub ? gdense : (double*)nullptr);
+  // the dense growth table: the subhalo model's and the Limber term's D(z)
+  hipLaunchKernelGGL(k_growth, dim3(B), dim3(64), 0, st, C, sh.n_z_max, D, slices, gdense);
   hipLaunchKernelGGL(k_sigma, dim3(B), dim3(256), 0, st, C, m_nodes, sh.n_m, 1, D, mass);
   if (sub || ell) {
     // spline knots into the x[] slot of instance 0 (overwritten by the solve)

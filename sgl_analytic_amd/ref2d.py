@@ -95,7 +95,8 @@ class VpTerm:
 class Population:
     """Occupied fine bins on the device ([w][kappa centroid][gamma centroid],
     REF_FINE_BINS each) and the builder's bookkeeping (sgl_full.py:1535-1808,
-    2742-2755); xi: the clustering term when built with bias."""
+    2742-2755); xi: the clustering term when built with bias; lss_var: the
+    variance of the Gaussian term of clustering="limber" (0: none)."""
     pop: torch.Tensor
     npop_dev: torch.Tensor
     n_bins: int
@@ -107,6 +108,7 @@ class Population:
     timings: dict = field(default_factory=dict)
     xi: XiTerm = None
     vp: VpTerm = None
+    lss_var: float = 0.0
 
 
 class _Stages:
@@ -138,7 +140,8 @@ def _xi_info(L, state: torch.Tensor) -> np.ndarray:
 
 def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
           edge: str = "r200", timings: dict = None, sub: bool = False,
-          keep_sub: bool = False, bias: bool = False, vp: bool = False) -> Population:
+          keep_sub: bool = False, bias: bool = False, vp: bool = False,
+          lss: bool = False) -> Population:
     """build_population(..., kap_floor=1e-8, trunc=edge, zint="midpoint")
     + Lam's flattening (sgl_full.py:1535, 2742-2755), on the device.  edge
     "rvir" truncates every halo at its virial radius (sgl_full.py:1876-1880).
@@ -153,7 +156,10 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
     pos_closure.install, pos_closure.py:708-733): the per-shell cells, their
     loadings and groups, the line-of-sight chain, eta calibrated on the
     clustering variance (brentq in ln eta, as pos_closure.py:575-591) and the
-    rates in interval form; needs bias."""
+    rates in interval form; needs bias.
+    lss: clustering="limber" with the bias ingredient (pipeline.py:195-202):
+    no xi_lin deposit; the Gaussian term of limber_lss_var
+    (sgl_full.py:2541-2572) in Lambda and in sigma_kappa; excludes bias."""
     L = _lib.load()
     dev = torch.device("cuda")
     if edge not in ("r200", "rvir"):
@@ -162,6 +168,8 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
         raise NotImplementedError("subhalos are built with the r200 edge only (clump_edge r200)")
     if vp and not bias:
         raise ValueError("the positive closure needs the xi_lin term (bias)")
+    if lss and bias:
+        raise ValueError("the limber term replaces the xi_lin term (bias)")
     flags = ((FLAG_ELL if ell else 0) | (FLAG_FILAMENTS if fil else 0)
              | (_lib.FLAG_EDGE_RVIR if edge == "rvir" else 0)
              | (_lib.FLAG_SUBHALOS if sub else 0)
@@ -175,6 +183,13 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
     T = torch.empty(L.sgl_ref_tables_bytes(flags) // 8, dtype=torch.float64, device=dev)
     status = torch.zeros(1, dtype=torch.int32, device=dev)
     check(L.sgl_ref_tables_build(_ptr(rec), flags, _ptr(T), _ptr(status), st), "sgl_ref_tables_build")
+    lss_var = 0.0
+    if lss:
+        lws = torch.empty(L.sgl_ref_lss_bytes() // 8, dtype=torch.float64, device=dev)
+        lv = torch.empty(1, dtype=torch.float64, device=dev)
+        check(L.sgl_ref_lss_var(_ptr(T), flags, _ptr(lws), _ptr(lv), st), "sgl_ref_lss_var")
+        lss_var = float(lv.item())
+        del lws
     clock.mark("tables")
     roots = torch.empty(L.sgl_ref_roots_bytes() // 8, dtype=torch.float64, device=dev)
     check(L.sgl_ref_roots(_ptr(T), flags, _ptr(roots), _ptr(status), st), "sgl_ref_roots")
@@ -259,10 +274,10 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
     sc = scal.cpu().numpy()
     # bookkeeping of the info record (sgl_full.py:2039, 1992): expected counts
     lay_rows = _row_table(T, flags, n_z)
-    var = sc[0] + (xi.var if xi is not None else 0.0)
+    var = sc[0] + (xi.var if xi is not None else 0.0) + lss_var
     out = Population(pop=pop, npop_dev=npop, n_bins=int(npop.item()), sigma=math.sqrt(var),
                      kbar=float(sc[1]), n_points=n, N_expl=lay_rows[0],
-                     N_fil=lay_rows[1] if fil else 0.0, xi=xi, vp=vpt)
+                     N_fil=lay_rows[1] if fil else 0.0, xi=xi, vp=vpt, lss_var=lss_var)
     if vpt is not None:
         vpt.n_bins = out.n_bins
     if keep_sub:
@@ -403,6 +418,9 @@ def lam_axis(pop: Population, k: np.ndarray, axis: int) -> np.ndarray:
         x = pop.xi
         check(L.sgl_ref_xi_axis(_ptr(x.tables), x.flags, _ptr(x.state), _ptr(kd), kd.numel(), axis,
                                 _ptr(out), _stream()), "sgl_ref_xi_axis")
+    if pop.lss_var:
+        check(L.sgl_ref_lss_axis(pop.lss_var, _ptr(kd), kd.numel(), axis, _ptr(out), _stream()),
+              "sgl_ref_lss_axis")
     o = out.cpu().numpy()
     return o[0::2] + 1j * o[1::2]
 
@@ -486,8 +504,15 @@ def phi_grid(pop: Population, k_k: np.ndarray, k_g: np.ndarray,
     blk = max(1, min(nk, 65535, E_BLOCK_BYTES // (16 * lda)))
     if grid is not None and pop.vp is not None:
         psi = vp_psi(pop, k_k, k_g, int(grid["xi_coarse"]), int(grid["xi_coarse_k"]))
+    elif pop.xi is not None:
+        psi = xi_psi(pop, k_k, k_g)
+    elif pop.lss_var:
+        psi = torch.empty((nk, ng, 2), dtype=torch.float64, device=dev)
+        kkd = torch.tensor(k_k, dtype=torch.float64, device=dev)
+        check(L.sgl_ref_lss_psi(pop.lss_var, _ptr(kkd), nk, _ptr(kg), ng, _ptr(psi), _stream()),
+              "sgl_ref_lss_psi")
     else:
-        psi = xi_psi(pop, k_k, k_g) if pop.xi is not None else None
+        psi = None
     ws = torch.empty(L.sgl_ref_phi_bytes(blk, ng, pop.n_bins) // 8, dtype=torch.float64, device=dev)
     for lo in range(0, nk, blk):
         hi = min(nk, lo + blk)
@@ -574,12 +599,13 @@ def invert(pop: Population, xi_out: np.ndarray, tol: float = PHI_TOL,
 
 def run(zs: float, cosmo: dict, ell: bool = False, fil: bool = False, xi_out=None,
         edge: str = "r200", timings: dict = None, sub: bool = False,
-        bias: bool = False, vp: bool = False) -> Result:
+        bias: bool = False, vp: bool = False, lss: bool = False) -> Result:
     """One PDF on the reference's chain; xi_out None: the CLI body and tail
     grids linspace(-1, 1, 1601) and linspace(1.005, 9, 1600) (pipeline.py:211-215);
-    vp: the positive clustering closure (needs bias)."""
+    vp: the positive clustering closure (needs bias); lss: the limber term
+    in place of bias."""
     pop = build(cosmo, zs, ell=ell, fil=fil, edge=edge, timings=timings, sub=sub, bias=bias,
-                vp=vp)
+                vp=vp, lss=lss)
     if xi_out is None:
         xi_out = np.concatenate([np.linspace(-1.0, 1.0, 1601), np.linspace(1.005, 9.0, 1600)])
     return invert(pop, xi_out, timings=timings)

@@ -54,10 +54,39 @@ def test_paths_not_computed_here_raise():
         with pytest.raises(ValueError):     # vp needs the bias ingredient, as in the reference
             api.run(1.0, "-bias", cosmo={}, xi_out=np.array([0.0, 1.0]), closure="vp")
         with pytest.raises(NotImplementedError):    # the threshold edge is not built
-            api.run(1.0, "full", cosmo={}, xi_out=np.array([0.0, 1.0]), edge="kthr")
+            api.run(1.0, "-bias", cosmo={}, xi_out=np.array([0.0, 1.0]), edge="kthr")
         with pytest.raises(ValueError):
             api.run(1.0, "full", cosmo={}, xi_out=np.array([0.0, 1.0]), edge="r500")
         solver.assert_not_called()
+
+
+def test_run_checks_arguments_as_the_reference_does():
+    """pipeline.run's argument errors (pipeline.py:139-164, its own
+    test_invalid_redshift_and_workers_fail_before_build) raise the same
+    ValueError before any device work; the modes that are not built raise
+    NotImplementedError, also before any device work."""
+    from sgl_analytic_amd import ref2d
+    with patch.object(ref2d, "build") as build:
+        for zs in (0, -1, np.nan, np.inf):
+            with pytest.raises(ValueError):
+                api.run(zs, "full")
+        for nproc in (0, -1, 1.5, True):
+            with pytest.raises(ValueError):
+                api.run(1, "full", nproc=nproc)
+        for kw in (dict(edge="r500"), dict(closure="lognormal"),
+                   dict(closure="vp", clustering="linear"), dict(edge="kthr"),
+                   dict(clustering="halofit"), dict(zint="trapezoid"),
+                   dict(zint="gauss", zint_n=0), dict(closure="vp", clustering="limber")):
+            with pytest.raises(ValueError):
+                api.run(1, "full", **kw)
+        with pytest.raises(ValueError):     # the vp closure needs the bias ingredient
+            api.run(1, "-bias", closure="vp")
+        for cfg, kw in (("full", dict(clustering="linear")), ("full", dict(clustering="closure")),
+                        ("-bias", dict(clustering="closure")), ("-bias", dict(edge="kthr")),
+                        ("full", dict(zint="engine")), ("full", dict(zint="gauss", zint_n=3))):
+            with pytest.raises(NotImplementedError):
+                api.run(1, cfg, **kw)
+        build.assert_not_called()
 
 
 def test_vp_closure_is_an_exact_path_option():

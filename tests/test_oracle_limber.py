@@ -1,0 +1,76 @@
+"""The oracle's clustering="limber" term pinned without its own formulas.
+
+* limber_lss_var (sgl_full.py:2541-2572) against the same variance in the
+  order the reference's docstring derives it, int dl l / (2 pi) C_l with
+  C_l = int dchi W^2 Dg^2 / f_K^2 P0(l / f_K): the l integral outside, a
+  Gauss-Legendre chi quadrature inside, chi(z) from a fine cumulative
+  integral of c / H, and the lensing efficiency from the critical surface
+  density, W = rho_m0 (1 + z)^2 / Sigma_cr(z, z_s), instead of the
+  3/2 Omega_m H0^2 form.  A dropped growth or (1 + z) factor, a wrong power
+  of H0 or 2 pi, or swapped l limits fails it by far more than the 3e-5
+  tolerance (the reference's own node counts put its discretisation near
+  1e-5).
+* lss_term (sgl_full.py:3104-3105) against the characteristic function of a
+  Gaussian convergence of variance v with an isotropic Gaussian shear of the
+  same total variance: E[e^{i k_k kappa}] E[J0(k_g |gamma|)] with |gamma|
+  Rayleigh distributed, by quadrature.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+from scipy.integrate import quad
+from scipy.interpolate import CubicSpline
+from scipy.special import j0
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+from oracle import ref2d as oref  # noqa: E402
+from oracle.cosmo import CKMS, Cosmology  # noqa: E402
+
+
+def ell_space_variance(cos, zs, n_chi=256, n_ell=4000):
+    zf = np.linspace(0.0, zs, 40001)
+    g = CKMS / (cos.H0 * cos.E(zf))
+    chif = np.concatenate([[0.0], np.cumsum(0.5 * (g[1:] + g[:-1]) * np.diff(zf))])
+    chis = chif[-1]
+    z_of_chi = CubicSpline(chif, zf)
+    x, w = np.polynomial.legendre.leggauss(n_chi)
+    a, b = chis * 1e-4, chis * (1 - 1e-4)
+    chi = a + 0.5 * (b - a) * (x + 1)
+    wc = 0.5 * (b - a) * w
+    z = z_of_chi(chi)
+    W = np.array([cos.rhom * (1 + zz) ** 2 / cos.Sigma_cr(zz, zs) for zz in z])
+    D = np.array([cos.Dg(zz) for zz in z])
+    fk = np.asarray(cos.f_K(chi), float)
+    ell = np.geomspace(1.0, 1e7, n_ell)
+    k = ell[:, None] / fk[None, :]
+    inside = (k >= 1e-7) & (k <= 1e5)
+    pk = np.where(inside, cos.P0(np.clip(k, 1e-7, 1e5)), 0.0)
+    cl = pk @ (wc * W ** 2 * D ** 2 / fk ** 2)
+    return float(np.trapezoid(ell ** 2 * cl / (2 * np.pi), np.log(ell)))
+
+
+@pytest.mark.parametrize("zs,kw", [(1.0, {}), (0.5, dict(growth_mode="legacy")),
+                                   (2.0, dict(Ok=0.05, w0=-0.9))])
+def test_limber_variance_matches_ell_space_integral(zs, kw):
+    cos = Cosmology(**kw)
+    v = oref.limber_lss_var(cos, zs)
+    ref = ell_space_variance(cos, zs)
+    print(f"z_s={zs} {kw}: limber_lss_var {v:.8e}, l-space {ref:.8e}, rel {v / ref - 1:.2e}")
+    assert abs(v / ref - 1) < 3e-5
+
+
+def test_lss_term_is_the_gaussian_characteristic_function():
+    v = 3.7e-4
+    kk = np.array([0.0, 12.0, 55.0])
+    kg = np.array([0.0, 9.0, 40.0, 90.0])
+    got = oref.lss_term(v, kk, kg)
+    s2 = 0.5 * v                      # variance of each shear component
+    for j, g in enumerate(kg):
+        rayleigh = quad(lambda r: j0(g * r) * r / s2 * np.exp(-r * r / (2 * s2)), 0.0,
+                        40.0 * np.sqrt(s2), limit=400, epsabs=1e-14)[0]
+        for i, k in enumerate(kk):
+            want = -0.5 * v * k * k + np.log(rayleigh)
+            assert abs(got[i, j] - want) < 1e-9 * max(1.0, abs(want)), (k, g)
