@@ -87,7 +87,7 @@ typedef struct sgl_shape {
 } sgl_shape;
 
 /* Per-instance input record: 16 doubles
- *   h, Om, sigma8, Ob, ns, zeq, Ok, w0, wa, z_s, n_z, radiation, growth, 0, 0, 0
+ *   h, Om, sigma8, Ob, ns, zeq, Ok, w0, wa, z_s, n_z, radiation, growth, zint, 0, 0
  * (the cosmology arguments of generate_pdf_lnmu, api.py:26-31, plus the
  * source redshift and the slice count of this instance).
  *   radiation (slot SGL_REC_RADIATION): 0 = "engine", Omega_r =
@@ -99,10 +99,18 @@ typedef struct sgl_shape {
  *     function, and the Carroll-Press-Turner growth Dg(z) in the halo and
  *     filament bias and in beta (sgl_full.py:521-548).  The caller resolves
  *     "auto" (legacy for LambdaCDM, ode otherwise) and keeps legacy to
- *     w0 = -1, wa = 0. */
+ *     w0 = -1, wa = 0.
+ *   zint (slot SGL_REC_ZINT, read by sgl_ref_tables_build only): 0 =
+ *     "midpoint", the engine nodes below z_s as shell edges plus 0 and z_s,
+ *     one node at each shell's middle, n_z = sgl_ref_shell_count(z_s) (the
+ *     pipeline default); 1 = "engine", the shells [z_{j-1}, z_j] of the
+ *     engine grid with z_j < z_s, j >= 1, one node at the upper edge with
+ *     width z_j - z_{j-1}, n_z = sgl_ref_shell_count(z_s) - 2
+ *     (sgl_full.py:480-518, 1858-1859).  The kernel-shape path ignores it. */
 #define SGL_COSMO_STRIDE 16
 #define SGL_REC_RADIATION 11
 #define SGL_REC_GROWTH 12
+#define SGL_REC_ZINT 13
 
 /* Buffer sizes in bytes for a batch of B instances. */
 size_t sgl_tables_bytes(const sgl_shape* shape, int32_t B);     /* grids+derived+slices+mass+rows+cov */

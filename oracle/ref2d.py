@@ -85,10 +85,19 @@ def engine_grids():
     return z, M
 
 
-def z_shells(zs: float):
-    """Midpoint shells (sgl_full.py:480-518, zint="midpoint"): list of
-    (z_lo, z_hi, z_node, dz)."""
+def z_shells(zs: float, zint: str = "midpoint"):
+    """The lens shells of z_shells (sgl_full.py:480-518) as a list of
+    (z_lo, z_hi, z_node, dz).  "midpoint": the engine nodes below z_s as
+    edges plus 0 and z_s, one node at each shell's middle; "engine": the
+    shells [z_{j-1}, z_j] for j >= 1 with z_j < z_s, one node at the upper
+    edge with the backward difference as its width (the builder's rule for
+    dz=None, sgl_full.py:1858-1859)."""
     zl, _ = engine_grids()
+    if zint == "engine":
+        return [(float(zl[j - 1]), float(zl[j]), float(zl[j]), float(zl[j] - zl[j - 1]))
+                for j in range(1, zl.size) if zl[j] < zs]
+    if zint != "midpoint":
+        raise ValueError(f"zint {zint!r} is not restated here")
     edges = np.concatenate([[0.0], zl[zl < zs], [float(zs)]])
     out = []
     for a, b in zip(edges[:-1], edges[1:]):
@@ -280,14 +289,15 @@ class Population:
         return float(sqrt(np.sum(self.w * self.k ** 2)))
 
 
-def rows(cos: Cosmology, zs: float, ell=False, fil=False, edge="r200", shells=None):
+def rows(cos: Cosmology, zs: float, ell=False, fil=False, edge="r200", shells=None,
+         zint="midpoint"):
     """Per-(shell, mass) parameters of _build_shell_into (sgl_full.py:1848-1884,
     1984-1991) as dicts, shell-major (only the listed shells when given).
     edge "r200": x_t = c; "rvir": x_t = r_vir / r_s (sgl_full.py:1874-1880)."""
     _, Ml = engine_grids()
     dlnM = np.diff(log(Ml))
     out = []
-    for i, (_, _, z, dz) in enumerate(z_shells(zs)):
+    for i, (_, _, z, dz) in enumerate(z_shells(zs, zint)):
         if shells is not None and i not in shells:
             continue
         wz = CKMS / cos.Hz(z) * dz * pi * (1 + z) ** 2
@@ -315,7 +325,7 @@ def rows(cos: Cosmology, zs: float, ell=False, fil=False, edge="r200", shells=No
 
 
 def _accumulate(cos, zs, ell, fil, edge, sub, shells=None, keep_points=False, xi=False,
-                pos=False):
+                pos=False, zint="midpoint"):
     """The rows of the given shells binned into the fine lattice: the sums
     (w, w kappa, w gamma) per bin and the builder's bookkeeping.  With ``xi``
     also the xi_lin deposit of every row (oracle/xilin.py), one Limber node of
@@ -368,10 +378,12 @@ def _accumulate(cos, zs, ell, fil, edge, sub, shells=None, keep_points=False, xi
 
     model = None
     if sub:
+        if zint != "midpoint":
+            raise ValueError("subhalos are restated on the midpoint shells only")
         from . import sub as _sub
         zl, Ml = np.array([sh[2] for sh in z_shells(zs)]), engine_grids()[1]
         model = acc["model"] = _sub.SubhaloModel(cos, zs, zl, Ml)
-    for r in rows(cos, zs, ell=ell, fil=fil, edge=edge, shells=shells):
+    for r in rows(cos, zs, ell=ell, fil=fil, edge=edge, shells=shells, zint=zint):
         if pos and cur["shell"] != r["shell"]:
             close_piece()
             cur.update(shell=r["shell"], row=r, H0=np.zeros(nkf * ngf), H0k=np.zeros(nkf * ngf),
@@ -416,30 +428,33 @@ def _accumulate(cos, zs, ell, fil, edge, sub, shells=None, keep_points=False, xi
 
 
 def _worker(args):
-    cos_kw, zs, ell, fil, edge, sub, shells, xi, pos = args
-    acc = _accumulate(Cosmology(**cos_kw), zs, ell, fil, edge, sub, shells, xi=xi, pos=pos)
+    cos_kw, zs, ell, fil, edge, sub, shells, xi, pos, zint = args
+    acc = _accumulate(Cosmology(**cos_kw), zs, ell, fil, edge, sub, shells, xi=xi, pos=pos,
+                      zint=zint)
     acc.pop("model")
     acc.pop("pts")
     return acc
 
 
 def build(cos: Cosmology, zs: float, ell=False, fil=False, keep_points=False, edge="r200",
-          sub=False, workers=None, xi=False, pos=False):
-    """build_population(..., kap_floor=1e-8, trunc="r200", zint="midpoint")
+          sub=False, workers=None, xi=False, pos=False, zint="midpoint"):
+    """build_population(..., kap_floor=1e-8, trunc="r200", zint="midpoint"
+    or "engine")
     without clustering: every row's cells (decorated by subhalos on hosts
     that carry them when ``sub``, oracle/sub.py) binned into the fine lattice
     with weight and centroid sums.  With any ingredient beyond plain halos the
     shells are shared out over ``workers`` processes (default: up to 8), each
     binning its shells, and the partial sums are added in worker order."""
-    n_sh = len(z_shells(zs))
+    n_sh = len(z_shells(zs, zint))
     if workers is None:
         workers = min(8, os.cpu_count() or 1) if (sub or xi or ell or fil) else 1
     workers = max(1, min(int(workers), n_sh))
     if workers == 1 or keep_points:
-        accs = [_accumulate(cos, zs, ell, fil, edge, sub, None, keep_points, xi=xi, pos=pos)]
+        accs = [_accumulate(cos, zs, ell, fil, edge, sub, None, keep_points, xi=xi, pos=pos,
+                            zint=zint)]
     else:
         import multiprocessing as mp
-        jobs = [(cos.kw, zs, ell, fil, edge, sub, set(range(w, n_sh, workers)), xi, pos)
+        jobs = [(cos.kw, zs, ell, fil, edge, sub, set(range(w, n_sh, workers)), xi, pos, zint)
                 for w in range(workers)]
         # fresh interpreters (no inherited OpenMP or device state), one
         # OpenMP thread each
@@ -756,7 +771,7 @@ def lss_term(v, kk, kg):
 
 
 def run(zs, cosmo: dict = None, ell=False, fil=False, xi_out=None, keep=False, edge="r200",
-        sub=False, bias=False, lss=False):
+        sub=False, bias=False, lss=False, zint="midpoint"):
     """pipeline.run (pipeline.py:114-275) with the Gaussian pair closure.
     ``bias``: the xi_lin clustering term (oracle/xilin.py) in Lambda and in
     sigma_kappa (sgl_full.py:3094-3128, 3437-3446); bias="dense" keeps every
@@ -764,11 +779,12 @@ def run(zs, cosmo: dict = None, ell=False, fil=False, xi_out=None, keep=False, e
     ``lss``: clustering="limber" with the bias ingredient instead
     (pipeline.py:195-202): no xi_lin deposit, the Gaussian term of
     limber_lss_var in Lambda and in sigma_kappa (sgl_full.py:3104-3105, 3439).
+    ``zint``: the lens shells, "midpoint" (the default) or "engine".
     xi_out None: the body and tail grids of the CLI (pipeline.py:211-215)."""
     if lss and bias:
         raise ValueError("the limber term replaces the xi_lin term")
     cos = Cosmology(**(cosmo or {}))
-    pop = build(cos, zs, ell=ell, fil=fil, edge=edge, sub=sub, xi=bias)
+    pop = build(cos, zs, ell=ell, fil=fil, edge=edge, sub=sub, xi=bias, zint=zint)
     xo = np.concatenate([XI_BODY, XI_TAIL]) if xi_out is None else np.asarray(xi_out, float)
     if lss:
         v = pop.lss_var = limber_lss_var(cos, zs)

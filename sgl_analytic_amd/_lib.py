@@ -17,7 +17,7 @@ FLAG_SUBHALOS = 64
 ELL_PHI, MSTAR_N = 12, 1200   # orientation nodes per elliptical cell; M* sigma table
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 COSMO_STRIDE = 16
-REC_RADIATION, REC_GROWTH = 11, 12  # SGL_REC_*: record slots of the radiation and growth options
+REC_RADIATION, REC_GROWTH, REC_ZINT = 11, 12, 13  # SGL_REC_*: record slots of the options
 STATUS_VP_TRANSITION = 4   # vp closure: transition balancing not converged
 STATUS_VP_RATE = 8         # vp closure: rate variance beyond the 48-node cap, or no eta bracket
 

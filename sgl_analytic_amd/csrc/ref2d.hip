@@ -1609,6 +1609,8 @@ __global__ void k_ref_init(const double* __restrict__ src, double* __restrict__ 
     *status = 0;
     const int nz = (int)src[10];
     if (!(src[9] > 0.0) || nz < 1 || nz > kRefMaxShells) *status = kStatusBadShape;
+    // engine shells: at most the 102 intervals of the 103-node engine grid
+    if (src[SGL_REC_ZINT] != 0.0 && nz > 102) *status = kStatusBadShape;
   }
 }
 

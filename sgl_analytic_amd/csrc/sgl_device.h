@@ -28,10 +28,12 @@ struct Cosmo {
   double n_z;            // slice count of this instance (stored as double)
   double rad_none;       // SGL_REC_RADIATION: 1 = no radiation
   double growth_legacy;  // SGL_REC_GROWTH: 1 = Heath-integral D, CPT growth in the bias
-  double pad[3];
+  double zint_engine;    // SGL_REC_ZINT: 1 = the engine's shells (reference grid only)
+  double pad[2];
 };
 static_assert(offsetof(Cosmo, rad_none) == SGL_REC_RADIATION * sizeof(double), "record slot");
 static_assert(offsetof(Cosmo, growth_legacy) == SGL_REC_GROWTH * sizeof(double), "record slot");
+static_assert(offsetof(Cosmo, zint_engine) == SGL_REC_ZINT * sizeof(double), "record slot");
 
 // Derived per-instance scalars (16 doubles), written by the background kernel.
 struct Derived {

@@ -175,9 +175,13 @@ __global__ __launch_bounds__(kBgThreads) void k_ref_background(const Cosmo* __re
       for (int q = 0; q < kSliceStride; ++q) S[q] = 0.0;
       continue;
     }
-    const double zlo = (i == 0) ? 0.0 : engine_z(i - 1);
-    const double zhi = (i == n_z - 1) ? zs : engine_z(i);
-    const double z = zlo + 0.5 * (zhi - zlo) * 1.0, dzs = 0.5 * (zhi - zlo) * 2.0;
+    // midpoint shells: edges 0, the engine nodes below z_s, z_s; engine
+    // shells: [z_i, z_{i+1}] of the engine grid, node at the upper edge
+    const bool engine = c.zint_engine != 0.0;
+    const double zlo = engine ? engine_z(i) : (i == 0) ? 0.0 : engine_z(i - 1);
+    const double zhi = engine ? engine_z(i + 1) : (i == n_z - 1) ? zs : engine_z(i);
+    const double z = engine ? zhi : zlo + 0.5 * (zhi - zlo) * 1.0;
+    const double dzs = engine ? zhi - zlo : 0.5 * (zhi - zlo) * 2.0;
     const int lo = min((int)(z / dz), kPanels - 1);
     const double chi = chi_e[lo] + dh * inv_e_integral(c, Or, OL, lo * dz, z);
     const double E2 = e2_of_z(c, Or, OL, z);

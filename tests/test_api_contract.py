@@ -83,7 +83,9 @@ def test_run_checks_arguments_as_the_reference_does():
             api.run(1, "-bias", closure="vp")
         for cfg, kw in (("full", dict(clustering="linear")), ("full", dict(clustering="closure")),
                         ("-bias", dict(clustering="closure")), ("-bias", dict(edge="kthr")),
-                        ("full", dict(zint="engine")), ("full", dict(zint="gauss", zint_n=3))):
+                        ("full", dict(zint="engine")), ("+sub", dict(zint="engine")),
+                        ("+bias", dict(zint="engine", closure="vp")),
+                        ("full", dict(zint="gauss", zint_n=3))):
             with pytest.raises(NotImplementedError):
                 api.run(1, cfg, **kw)
         build.assert_not_called()

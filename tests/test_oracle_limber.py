@@ -74,3 +74,27 @@ def test_lss_term_is_the_gaussian_characteristic_function():
         for i, k in enumerate(kk):
             want = -0.5 * v * k * k + np.log(rayleigh)
             assert abs(got[i, j] - want) < 1e-9 * max(1.0, abs(want)), (k, g)
+
+
+def test_engine_shells():
+    """z_shells(zint="engine") (sgl_full.py:480-518): the engine nodes z_j
+    below z_s with j >= 1 as the nodes, the backward differences as widths
+    (so they telescope to z_last - z_0), two shells fewer than the midpoint
+    rule, and the lensing weight sum dchi (1 + z)^2 / Sigma_cr a few per
+    cent below the midpoint rule's, as the reference states ("~3.4% low in
+    lensing weight": it drops [0, z_0] and [z_last, z_s])."""
+    zl, _ = oref.engine_grids()
+    cos = Cosmology()
+    for zs in (0.5, 1.0, 3.0):
+        eng, mid = oref.z_shells(zs, "engine"), oref.z_shells(zs)
+        below = zl[zl < zs]
+        assert [s[2] for s in eng] == list(below[1:])
+        assert abs(sum(s[3] for s in eng) - (below[-1] - below[0])) < 1e-14
+        assert len(eng) == len(mid) - 2
+
+        def weight(sh):
+            return sum(CKMS / cos.Hz(z) * dz * (1 + z) ** 2 / cos.Sigma_cr(z, zs)
+                       for _, _, z, dz in sh)
+        ratio = weight(eng) / weight(mid)
+        print(f"z_s={zs}: engine / midpoint lensing weight {ratio:.4f}")
+        assert 0.95 < ratio < 0.98
