@@ -87,7 +87,7 @@ typedef struct sgl_shape {
 } sgl_shape;
 
 /* Per-instance input record: 16 doubles
- *   h, Om, sigma8, Ob, ns, zeq, Ok, w0, wa, z_s, n_z, radiation, growth, zint, 0, 0
+ *   h, Om, sigma8, Ob, ns, zeq, Ok, w0, wa, z_s, n_z, radiation, growth, zint, zint_n, 0
  * (the cosmology arguments of generate_pdf_lnmu, api.py:26-31, plus the
  * source redshift and the slice count of this instance).
  *   radiation (slot SGL_REC_RADIATION): 0 = "engine", Omega_r =
@@ -106,11 +106,15 @@ typedef struct sgl_shape {
  *     pipeline default); 1 = "engine", the shells [z_{j-1}, z_j] of the
  *     engine grid with z_j < z_s, j >= 1, one node at the upper edge with
  *     width z_j - z_{j-1}, n_z = sgl_ref_shell_count(z_s) - 2
- *     (sgl_full.py:480-518, 1858-1859).  The kernel-shape path ignores it. */
+ *     (sgl_full.py:480-518, 1858-1859); 2 = "gauss", the midpoint shells at
+ *     zint_n (slot SGL_REC_ZINT_N) Gauss-Legendre nodes each, shell-major,
+ *     n_z = zint_n sgl_ref_shell_count(z_s) <= SGL_REF_MAX_SHELLS.  The
+ *     kernel-shape path ignores both. */
 #define SGL_COSMO_STRIDE 16
 #define SGL_REC_RADIATION 11
 #define SGL_REC_GROWTH 12
 #define SGL_REC_ZINT 13
+#define SGL_REC_ZINT_N 14
 
 /* Buffer sizes in bytes for a batch of B instances. */
 size_t sgl_tables_bytes(const sgl_shape* shape, int32_t B);     /* grids+derived+slices+mass+rows+cov */
@@ -324,7 +328,8 @@ sgl_status sgl_diagnostics(const void* p, int32_t p_is_double, int32_t B, const 
  * sizes depend on device results (cell count, sigma, reach), so the host reads
  * those scalars between calls.
  */
-#define SGL_REF_MAX_SHELLS 104            /* 103 engine nodes below z_s + the last shell */
+#define SGL_REF_MAX_SHELLS 208            /* lens nodes: (103 engine nodes below z_s + the last
+                                             shell) x 2 Gauss-Legendre nodes */
 #define SGL_REF_MASSES 99                 /* engine mass nodes 1..99 */
 #define SGL_REF_FINE_BINS (961 * 300)     /* fine (kappa, gamma) lattice, sgl_full.py:456-468 */
 

@@ -96,6 +96,13 @@ def z_shells(zs: float, zint: str = "midpoint"):
     if zint == "engine":
         return [(float(zl[j - 1]), float(zl[j]), float(zl[j]), float(zl[j] - zl[j - 1]))
                 for j in range(1, zl.size) if zl[j] < zs]
+    if zint.startswith("gauss"):
+        # "gaussN": the midpoint shells at N Gauss-Legendre nodes each, one
+        # entry per node (z_shells' tasks, sgl_full.py:503-518)
+        t, w = np.polynomial.legendre.leggauss(int(zint[5:]))
+        edges = np.concatenate([[0.0], zl[zl < zs], [float(zs)]])
+        return [(float(a), float(b), float(a + 0.5 * (b - a) * (tk + 1.0)), float(0.5 * (b - a) * wk))
+                for a, b in zip(edges[:-1], edges[1:]) if b > a for tk, wk in zip(t, w)]
     if zint != "midpoint":
         raise ValueError(f"zint {zint!r} is not restated here")
     edges = np.concatenate([[0.0], zl[zl < zs], [float(zs)]])

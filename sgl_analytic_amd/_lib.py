@@ -17,7 +17,7 @@ FLAG_SUBHALOS = 64
 ELL_PHI, MSTAR_N = 12, 1200   # orientation nodes per elliptical cell; M* sigma table
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 COSMO_STRIDE = 16
-REC_RADIATION, REC_GROWTH, REC_ZINT = 11, 12, 13  # SGL_REC_*: record slots of the options
+REC_RADIATION, REC_GROWTH, REC_ZINT, REC_ZINT_N = 11, 12, 13, 14  # SGL_REC_*: option slots
 STATUS_VP_TRANSITION = 4   # vp closure: transition balancing not converged
 STATUS_VP_RATE = 8         # vp closure: rate variance beyond the 48-node cap, or no eta bracket
 
@@ -42,7 +42,7 @@ EXPORTS = (
     "sgl_ref_lss_bytes", "sgl_ref_lss_var", "sgl_ref_lss_psi", "sgl_ref_lss_axis",
 )
 NOALLOC_SCRATCH_DOUBLES = 1 << 22   # capacity of sgl_lnphi_exact_noalloc's own array
-REF_MAX_SHELLS, REF_MASSES, REF_FINE_BINS, REF_ROOT_STRIDE = 104, 99, 961 * 300, 128
+REF_MAX_SHELLS, REF_MASSES, REF_FINE_BINS, REF_ROOT_STRIDE = 208, 99, 961 * 300, 128
 REF_STATUS_ROOTS, REF_STATUS_EDGES, REF_STATUS_BAD = 16, 32, 2
 # sgl_ref_xi_info record: [n_runs, n_parts, n_z, nP, nQ, variance, largest s_0,
 # shear variance of the m = 1 groups]

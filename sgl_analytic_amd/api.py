@@ -163,11 +163,13 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200
     term) or "limber" (the Gaussian large-scale-structure term of
     limber_lss_var, pipeline.py:195-202); without the "bias" ingredient every
     clustering mode but "closure" gives the same chain (no two-point term).
-    zint: "midpoint" (the default) or "engine" (the engine grid's shells
-    below z_s, one node at the upper edge; without subhalos and the vp
-    closure).  Not built, NotImplementedError: "linear" with "bias" (the
-    20 Mpc slice-covariance Cox term), "closure", edge "kthr", zint "gauss",
-    zint "engine" with subhalos or the vp closure, subhalos with edge "rvir".
+    zint: "midpoint" (the default), "engine" (the engine grid's shells below
+    z_s, one node at the upper edge) or "gauss" (the midpoint shells at
+    zint_n Gauss-Legendre nodes each, up to 208 nodes in all); engine and
+    gauss without subhalos and the vp closure.  Not built,
+    NotImplementedError: "linear" with "bias" (the 20 Mpc slice-covariance
+    Cox term), "closure", edge "kthr", zint other than midpoint with
+    subhalos or the vp closure, subhalos with edge "rvir".
     nproc: the reference's host worker count, checked and recorded (the
     device needs no host pool)."""
     if clustering not in CLUSTERING_MODES:
@@ -199,10 +201,8 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200
     if clustering == "linear" and "bias" in s:
         raise NotImplementedError("clustering='linear' with the bias ingredient (the slice-"
                                   "covariance Cox term) is not built; use 'xilin' or 'limber'")
-    if zint == "gauss":
-        raise NotImplementedError("zint='gauss' is not built (midpoint, engine)")
-    if zint == "engine" and ("sub" in s or closure == "vp"):
-        raise NotImplementedError("zint='engine' is built without subhalos and the vp closure")
+    if zint != "midpoint" and ("sub" in s or closure == "vp"):
+        raise NotImplementedError(f"zint={zint!r} is built without subhalos and the vp closure")
     if "sub" in s and edge != "r200":
         raise NotImplementedError("subhalos are built with edge='r200' only (clumps truncated at "
                                   "their r200, the reference's pairing for that edge)")
@@ -210,7 +210,7 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200
     from . import ref2d
     t0 = time.perf_counter()
     pop = ref2d.build(cp, float(z_s), ell="ell" in s, fil="fil" in s, edge=edge, sub="sub" in s,
-                      bias=use_xi, vp=closure == "vp", lss=lss, zint=zint)
+                      bias=use_xi, vp=closure == "vp", lss=lss, zint=zint, zint_n=int(zint_n))
     t1 = time.perf_counter()
     # xi_out None: the CLI's body and tail grids (pipeline.py:211-215)
     x = (np.concatenate([np.linspace(-1.0, 1.0, 1601), np.linspace(1.005, 9.0, 1600)])
@@ -223,7 +223,8 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200
                      dict(sigma=res.sigma, mean=res.mean, var=res.var, skew=res.skew,
                           sigmaDL=res.sigmaDL, grid=res.grid),
                      dict(build=t1 - t0, lam=0.0, invert=t2 - t1), clustering=clustering,
-                     lss_var=pop.lss_var, nproc=int(nproc), zint=zint)
+                     lss_var=pop.lss_var, nproc=int(nproc), zint=zint,
+                     zint_n=int(zint_n) if zint == "gauss" else None)
     info["zs"] = float(z_s)
     if pop.vp is not None:
         v = pop.vp
@@ -235,7 +236,7 @@ def run(z_s, config="full", cosmo=None, xi_out=None, closure="gauss", edge="r200
 
 
 def record_2d(cp, config, ingredients, closure, edge, pop, res, t_stages, clustering="xilin",
-              lss_var=0.0, nproc=1, zint="midpoint"):
+              lss_var=0.0, nproc=1, zint="midpoint", zint_n=None):
     """The run() record of the reference (pipeline.py:243-274; every top-level
     key of its stored PDFs) with the 2-D chain's values: pop = bookkeeping of
     the population (kbar, N_expl, N_fil, n_bins, n_points), res = sigma,
@@ -245,7 +246,7 @@ def record_2d(cp, config, ingredients, closure, edge, pop, res, t_stages, cluste
               radiation=cp["radiation"], Omega_r=Or)
     return dict(
         t_stages=dict(t_stages), cfg=config if isinstance(config, str) else sorted(ingredients),
-        zint=zint, zint_n=None, clustering=clustering,
+        zint=zint, zint_n=zint_n, clustering=clustering,
         xi_kperp=(dict(kmin=1e-4, kmax=1e4, per_decade=[2, 2, 12, 8, 6, 4, 4, 4], mmax=2)
                   if "bias" in ingredients and clustering == "xilin" else None), edge=edge,
         lss_var=float(lss_var),

@@ -1609,8 +1609,13 @@ __global__ void k_ref_init(const double* __restrict__ src, double* __restrict__ 
     *status = 0;
     const int nz = (int)src[10];
     if (!(src[9] > 0.0) || nz < 1 || nz > kRefMaxShells) *status = kStatusBadShape;
-    // engine shells: at most the 102 intervals of the 103-node engine grid
-    if (src[SGL_REC_ZINT] != 0.0 && nz > 102) *status = kStatusBadShape;
+    // engine shells: at most the 102 intervals of the 103-node engine grid;
+    // gauss: whole shells of zint_n >= 1 nodes
+    const double zi = src[SGL_REC_ZINT], zn = src[SGL_REC_ZINT_N];
+    if (zi == 1.0 && nz > 102) *status = kStatusBadShape;
+    if (zi == 2.0 && !(zn >= 1.0 && zn <= kRefMaxShells && nz % (int)zn == 0))
+      *status = kStatusBadShape;
+    if (zi != 0.0 && zi != 1.0 && zi != 2.0) *status = kStatusBadShape;
   }
 }
 

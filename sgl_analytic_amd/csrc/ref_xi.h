@@ -15,7 +15,7 @@ constexpr int kXiVals = 3 * kXiNk;        // J0, J2, J4 (m = 0, 1, 2) per node
 constexpr int kXiKc = 341, kXiGc = 110;
 constexpr int kXiBins = kXiKc * kXiGc;
 constexpr int kXiKeyInvalid = kRefMaxShells * kXiBins;   // outside: sorts last
-constexpr int kXiKeyBits = 22;
+constexpr int kXiKeyBits = 23;
 static_assert(kXiKeyInvalid < (1 << kXiKeyBits), "coarse keys exceed the sort bits");
 constexpr int kXiGroups = 3 * kRefMaxShells;             // (shell, m)
 constexpr int kXiCoarse = 101;                           // Q's coarse grid (Lam xi_coarse)

@@ -10,11 +10,12 @@
 namespace sgl {
 
 constexpr int kMaxSlices = 96;
-// Reference grid (SGL_FLAG_REF_GRID): up to 104 shells (the 103 engine nodes
-// below z_s = 12.34 plus the last shell) and the engine's 99 mass nodes.
-constexpr int kRefMaxShells = 104;
+// Reference grid (SGL_FLAG_REF_GRID): up to 208 lens nodes ("slices"): the
+// 104 midpoint shells (the 103 engine nodes below z_s = 12.34 plus the last
+// shell) at two Gauss-Legendre nodes each, and the engine's 99 mass nodes.
+constexpr int kRefMaxShells = 208;
 constexpr int kRefMasses = 99;
-constexpr int kGrowthMaxNodes = 112;   // slice nodes the growth kernel stages in LDS
+constexpr int kGrowthMaxNodes = 208;   // slice nodes the growth kernel stages in LDS
 constexpr int kCovMaxModes = 2048;
 constexpr int kStatusCovModes = 1;      // covariance mode count clamped
 constexpr int kStatusBadShape = 2;

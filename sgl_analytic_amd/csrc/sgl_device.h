@@ -28,12 +28,14 @@ struct Cosmo {
   double n_z;            // slice count of this instance (stored as double)
   double rad_none;       // SGL_REC_RADIATION: 1 = no radiation
   double growth_legacy;  // SGL_REC_GROWTH: 1 = Heath-integral D, CPT growth in the bias
-  double zint_engine;    // SGL_REC_ZINT: 1 = the engine's shells (reference grid only)
-  double pad[2];
+  double zint;           // SGL_REC_ZINT: 0 midpoint, 1 engine, 2 gauss (reference grid only)
+  double zint_n;         // SGL_REC_ZINT_N: Gauss-Legendre nodes per shell with zint 2
+  double pad[1];
 };
 static_assert(offsetof(Cosmo, rad_none) == SGL_REC_RADIATION * sizeof(double), "record slot");
 static_assert(offsetof(Cosmo, growth_legacy) == SGL_REC_GROWTH * sizeof(double), "record slot");
-static_assert(offsetof(Cosmo, zint_engine) == SGL_REC_ZINT * sizeof(double), "record slot");
+static_assert(offsetof(Cosmo, zint) == SGL_REC_ZINT * sizeof(double), "record slot");
+static_assert(offsetof(Cosmo, zint_n) == SGL_REC_ZINT_N * sizeof(double), "record slot");
 
 // Derived per-instance scalars (16 doubles), written by the background kernel.
 struct Derived {

@@ -139,6 +139,35 @@ __global__ __launch_bounds__(kBgThreads) void k_background(const Cosmo* __restri
 // Dg are filled by the growth kernel.
 constexpr double kZminEng = 0.01, kZmaxEng = 12.341169644129371;
 constexpr int kNzEng = 103;
+// Node q (ascending) of the n-point Gauss-Legendre rule on [-1, 1] and its
+// weight: Newton on P_n from cos(pi (q + 3/4) / (n + 1/2)) to convergence,
+// w = 2 / ((1 - x^2) P_n'(x)^2) (numpy.polynomial.legendre.leggauss's rule).
+__device__ inline void gl_node(int n, int q, double& x, double& w) {
+  double z = -cos(kPi * (q + 0.75) / (n + 0.5));
+  double dp = 1.0;
+  for (int it = 0; it < 64; ++it) {
+    double p0 = 1.0, p1 = z;
+    for (int k = 2; k <= n; ++k) {
+      const double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    dp = n * (z * p1 - p0) / (z * z - 1.0);
+    const double dz = p1 / dp;
+    z -= dz;
+    if (fabs(dz) <= 1e-16 * fmax(1.0, fabs(z))) break;
+  }
+  double p0 = 1.0, p1 = z;
+  for (int k = 2; k <= n; ++k) {
+    const double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+    p0 = p1;
+    p1 = p2;
+  }
+  dp = (n == 1) ? 1.0 : n * (z * p1 - p0) / (z * z - 1.0);
+  x = z;
+  w = 2.0 / ((1.0 - z * z) * dp * dp);
+}
+
 __device__ inline double engine_z(int j) {
   const double l0 = log(kZminEng), l1 = log(kZmaxEng);
   return exp((j == kNzEng - 1) ? l1 : l0 + j * ((l1 - l0) / (kNzEng - 1)));
@@ -175,13 +204,19 @@ __global__ __launch_bounds__(kBgThreads) void k_ref_background(const Cosmo* __re
       for (int q = 0; q < kSliceStride; ++q) S[q] = 0.0;
       continue;
     }
-    // midpoint shells: edges 0, the engine nodes below z_s, z_s; engine
-    // shells: [z_i, z_{i+1}] of the engine grid, node at the upper edge
-    const bool engine = c.zint_engine != 0.0;
-    const double zlo = engine ? engine_z(i) : (i == 0) ? 0.0 : engine_z(i - 1);
-    const double zhi = engine ? engine_z(i + 1) : (i == n_z - 1) ? zs : engine_z(i);
-    const double z = engine ? zhi : zlo + 0.5 * (zhi - zlo) * 1.0;
-    const double dzs = engine ? zhi - zlo : 0.5 * (zhi - zlo) * 2.0;
+    // midpoint shells: edges 0, the engine nodes below z_s, z_s, one node
+    // in the middle; gauss: the same shells at ng Gauss-Legendre nodes each
+    // (slice i = node i % ng of shell i / ng); engine shells: [z_i, z_{i+1}]
+    // of the engine grid, node at the upper edge
+    const bool engine = c.zint == 1.0;
+    const int ng = (c.zint == 2.0 && c.zint_n >= 1.0 && c.zint_n <= kRefMaxShells) ? (int)c.zint_n : 1;
+    const int sh = i / ng, n_sh = n_z / ng;
+    double t = 0.0, w = 2.0;
+    if (ng > 1) gl_node(ng, i - sh * ng, t, w);
+    const double zlo = engine ? engine_z(i) : (sh == 0) ? 0.0 : engine_z(sh - 1);
+    const double zhi = engine ? engine_z(i + 1) : (sh == n_sh - 1) ? zs : engine_z(sh);
+    const double z = engine ? zhi : zlo + 0.5 * (zhi - zlo) * (t + 1.0);
+    const double dzs = engine ? zhi - zlo : 0.5 * (zhi - zlo) * w;
     const int lo = min((int)(z / dz), kPanels - 1);
     const double chi = chi_e[lo] + dh * inv_e_integral(c, Or, OL, lo * dz, z);
     const double E2 = e2_of_z(c, Or, OL, z);

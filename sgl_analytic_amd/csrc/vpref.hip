@@ -51,16 +51,20 @@
 namespace sgl {
 namespace {
 
+// The positive closure runs on the midpoint shells only (one Limber node
+// per shell): at most 104, the 103 engine nodes below z_s plus the last shell.
+constexpr int kVpShells = 104;
+static_assert(kVpShells <= kRefMaxShells, "vp shells within the table capacity");
 constexpr int kNs = 48;                                  // latent states
 constexpr int kFine = 961 * 300;                         // fine (kappa, gamma) bins
 constexpr int kCkeyBits = 25;
 constexpr int kCkeyInvalid = (1 << kCkeyBits) - 1;       // outside the lattice: sorts last
-static_assert(kRefMaxShells * kFine < kCkeyInvalid, "composite keys exceed the sort bits");
+static_assert(kVpShells * kFine < kCkeyInvalid, "composite keys exceed the sort bits");
 constexpr int kGroupSlots = kXiBins + 1;                 // coarse parents + the zero group
 constexpr int kGkeyBits = 22;
-static_assert(kRefMaxShells * kGroupSlots < (1 << kGkeyBits), "group keys exceed the sort bits");
+static_assert(kVpShells * kGroupSlots < (1 << kGkeyBits), "group keys exceed the sort bits");
 constexpr int kOkeyBits = 14;                            // (shell, type, anchor) visiting order
-static_assert(kRefMaxShells * 3 * kNs < (1 << kOkeyBits), "order keys exceed the sort bits");
+static_assert(kVpShells * 3 * kNs < (1 << kOkeyBits), "order keys exceed the sort bits");
 constexpr int kLosN = 30001;                             // los_rho k_par nodes on [0, 3]
 constexpr double kLosKmax = 3.0;
 constexpr int kPart = 4096;                              // cells per fixed summation part
@@ -87,7 +91,7 @@ RunWs run_ws(long long n) {
   RunWs L;
   size_t o = 0;
   auto take = [&](size_t b) { size_t r = o; o += al256(b); return r; };
-  L.cap = std::min<long long>(n, (long long)kRefMaxShells * kFine) + 1;
+  L.cap = std::min<long long>(n, (long long)kVpShells * kFine) + 1;
   L.max_parts = L.cap + n / kPart + 1;
   size_t sort_b = 0, rle_b = 0, scan_b = 0;
   (void)rocprim::radix_sort_pairs((void*)nullptr, sort_b, (int*)nullptr, (int*)nullptr,
@@ -133,7 +137,7 @@ VpState vp_state(long long n) {
   auto take = [&](size_t b) { size_t r = o; o += al256(b); return r; };
   const long long nn = std::max<long long>(n, 1);
   L.n = nn;
-  L.gcap = nn + kRefMaxShells + 1;
+  L.gcap = nn + kVpShells + 1;
   size_t sort_b = 0, sort2_b = 0, rle_b = 0, scan_b = 0;
   (void)rocprim::radix_sort_pairs((void*)nullptr, sort_b, (int*)nullptr, (int*)nullptr,
                                   (int*)nullptr, (int*)nullptr, (size_t)nn, 0, kGkeyBits);
@@ -188,13 +192,13 @@ VpState vp_state(long long n) {
   L.bmap = take((size_t)kFine * sizeof(long long));
   L.bocc = take((size_t)kFine * sizeof(int));
   L.bocc64 = take((size_t)kFine * sizeof(long long));
-  L.shg = take((size_t)(kRefMaxShells + 1) * 2 * sizeof(long long));
-  L.f = take(kRefMaxShells * sizeof(double));
-  L.rho = take(kRefMaxShells * sizeof(double));
-  L.K = take((size_t)kRefMaxShells * kNs * kNs * sizeof(double));
-  L.ms = take((size_t)kRefMaxShells * kNs * sizeof(double));
+  L.shg = take((size_t)(kVpShells + 1) * 2 * sizeof(long long));
+  L.f = take(kVpShells * sizeof(double));
+  L.rho = take(kVpShells * sizeof(double));
+  L.K = take((size_t)kVpShells * kNs * kNs * sizeof(double));
+  L.ms = take((size_t)kVpShells * kNs * sizeof(double));
   L.P0 = take((size_t)kXiNk * kLosN * sizeof(double));
-  L.los = take((size_t)kRefMaxShells * kXiNk * sizeof(double));
+  L.los = take((size_t)kVpShells * kXiNk * sizeof(double));
   L.tmp = take(L.tmp_bytes);
   L.total = o;
   return L;
@@ -363,7 +367,7 @@ __global__ void k_vp_cells(long long n, const int* __restrict__ ukey,
   // a bin whose sums are all zero is not a cell of the shell (the pieces keep
   // h0, h0k or h0g != 0): its key sorts after every shell
   const bool absent = w == 0.0 && wk == 0.0 && wg == 0.0;
-  gkey[c] = absent ? kRefMaxShells * kGroupSlots : s * kGroupSlots + slot;
+  gkey[c] = absent ? kVpShells * kGroupSlots : s * kGroupSlots + slot;
   idx[c] = (int)c;
 }
 
@@ -383,7 +387,7 @@ __global__ void k_vp_groups(const int* __restrict__ gkey, const int* __restrict_
     wk += cw[c] * ckap[c];
   }
   gshell[g] = gkey[g] / kGroupSlots;
-  gphi[g] = (gkey[g] % kGroupSlots == kXiBins || gkey[g] >= kRefMaxShells * kGroupSlots)
+  gphi[g] = (gkey[g] % kGroupSlots == kXiBins || gkey[g] >= kVpShells * kGroupSlots)
                 ? 0.0
                 : cphi[idx[a]];
   gwk[g] = wk;
@@ -474,7 +478,7 @@ __global__ __launch_bounds__(256) void k_vp_los(const double* __restrict__ slice
 __global__ void k_vp_rho(const double* __restrict__ los, const double* __restrict__ v0, int n_z,
                          double* __restrict__ rho, double* __restrict__ f,
                          long long* __restrict__ meta) {
-  __shared__ double r[kRefMaxShells];
+  __shared__ double r[kVpShells];
   const int t = threadIdx.x;
   for (int s = t; s < n_z - 1; s += blockDim.x) {
     const double* a = v0 + (size_t)s * kXiNk;
@@ -1269,7 +1273,7 @@ sgl_status sgl_ref_vp_runs(const void* tables, int32_t flags, int64_t n_points,
       return SGL_ERR_HIP;
     n_z = (int)nz;
   }
-  if (n_z < 1 || n_z > kRefMaxShells) return SGL_ERR_INVALID;
+  if (n_z < 1 || n_z > kVpShells) return SGL_ERR_INVALID;
   const int tps = sgl_ref_tasks(flags) / kRefMaxShells;
   const long long n = n_points;
   hipLaunchKernelGGL(k_vp_ckey, dim3(nblk(n)), dim3(256), 0, st, n,
@@ -1335,7 +1339,7 @@ sgl_status sgl_ref_vp_setup(const void* tables, int32_t flags, int64_t n_points,
       return SGL_ERR_HIP;
     n_z = (int)nz;
   }
-  if (n_z < 1 || n_z > kRefMaxShells) return SGL_ERR_INVALID;
+  if (n_z < 1 || n_z > kVpShells) return SGL_ERR_INVALID;
   if (hipMemsetAsync(meta, 0, 16 * sizeof(long long), st) != hipSuccess) return SGL_ERR_HIP;
   int* occ = at<int>(S, L.bocc);
   long long* bmap = at<long long>(S, L.bmap);
@@ -1455,7 +1459,7 @@ extern "C" {
 
 sgl_status sgl_ref_vp_variance(void* state, int64_t n_cells, int32_t n_z, double eta, double* out,
                                void* stream) {
-  if (!state || !out || n_cells < 1 || n_z < 1 || n_z > kRefMaxShells || !(eta > 0.0))
+  if (!state || !out || n_cells < 1 || n_z < 1 || n_z > kVpShells || !(eta > 0.0))
     return SGL_ERR_INVALID;
   const VpState L = vp_state(n_cells);
   void* S = state;
@@ -1474,7 +1478,7 @@ sgl_status sgl_ref_vp_variance(void* state, int64_t n_cells, int32_t n_z, double
 
 sgl_status sgl_ref_vp_rates(void* state, int64_t n_cells, int32_t n_z, double eta,
                             int64_t* meta_out, void* stream) {
-  if (!state || !meta_out || n_cells < 1 || n_z < 1 || n_z > kRefMaxShells || !(eta > 0.0))
+  if (!state || !meta_out || n_cells < 1 || n_z < 1 || n_z > kVpShells || !(eta > 0.0))
     return SGL_ERR_INVALID;
   const VpState L = vp_state(n_cells);
   void* S = state;
@@ -1513,7 +1517,7 @@ sgl_status sgl_ref_vp_rates(void* state, int64_t n_cells, int32_t n_z, double et
                      at<int>(S, L.idx_out), at<int>(S, L.cu), at<double>(S, L.cw),
                      at<int>(S, L.vcu), at<double>(S, L.vcw), at<int>(S, L.vlast),
                      at<int>(S, L.vgrp));
-  long long* shv = at<long long>(S, L.shg) + (kRefMaxShells + 1);
+  long long* shv = at<long long>(S, L.shg) + (kVpShells + 1);
   hipLaunchKernelGGL(k_vp_visit_shells, dim3(1), dim3(128), 0, st, gnum, ok_out, vstart, n_z, shv,
                      at<long long>(S, L.meta));
   if (hipMemcpyAsync(meta_out, at<long long>(S, L.meta), 16 * sizeof(long long),
@@ -1523,7 +1527,7 @@ sgl_status sgl_ref_vp_rates(void* state, int64_t n_cells, int32_t n_z, double et
 }
 
 size_t sgl_ref_vp_lnr_bytes(int32_t n_z, int32_t n_bins, int32_t nk, int32_t nq) {
-  if (n_z < 1 || n_z > kRefMaxShells || n_bins < 1 || nk < 1 || nq < 1) return 0;
+  if (n_z < 1 || n_z > kVpShells || n_bins < 1 || nk < 1 || nq < 1) return 0;
   return lnr_ws(n_z, n_bins, nk, nq).total;
 }
 
@@ -1531,7 +1535,7 @@ sgl_status sgl_ref_vp_lnr(const void* state, int64_t n_cells, int32_t n_z, const
                           int32_t n_bins, const double* kv, int32_t nk, const double* qv,
                           int32_t nq, void* ws, double* out, void* stream) {
   if (!state || !pop || !kv || !qv || !ws || !out || n_cells < 1 || n_z < 1 ||
-      n_z > kRefMaxShells || n_bins < 1 || nk < 1 || nq < 1)
+      n_z > kVpShells || n_bins < 1 || nk < 1 || nq < 1)
     return SGL_ERR_INVALID;
   const VpState L = vp_state(n_cells);
   const LnrWs W = lnr_ws(n_z, n_bins, nk, nq);
@@ -1545,7 +1549,7 @@ sgl_status sgl_ref_vp_lnr(const void* state, int64_t n_cells, int32_t n_z, const
   const double* pg = pop + 2 * kFine;
   hipLaunchKernelGGL(k_vp_jtab, dim3(nblk((long long)n_bins * nq)), dim3(256), 0, st, n_bins, pg,
                      qv, nq, Jt);
-  const long long* shv = at<long long>(S, L.shg) + (kRefMaxShells + 1);
+  const long long* shv = at<long long>(S, L.shg) + (kVpShells + 1);
   const int ntq = (nq + kTq - 1) / kTq;
   for (int k0 = 0; k0 < nk; k0 += W.rows) {
     const int rows = std::min(W.rows, nk - k0);

@@ -48,14 +48,16 @@ def _stream() -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def _records(cosmo: dict, zs: float, n_z: int, zint: str = "midpoint") -> np.ndarray:
+def _records(cosmo: dict, zs: float, n_z: int, zint: str = "midpoint",
+             zint_n: int = 1) -> np.ndarray:
     rec = np.zeros(_lib.COSMO_STRIDE)
     rec[:10] = [cosmo["h"], cosmo["Om"], cosmo["s8"], cosmo["Ob"], cosmo["ns"], cosmo["zeq"],
                 cosmo.get("Ok", 0.0), cosmo.get("w0", -1.0), cosmo.get("wa", 0.0), zs]
     rec[10] = n_z
     rec[_lib.REC_RADIATION] = 1.0 if cosmo.get("radiation", "engine") is None else 0.0
     rec[_lib.REC_GROWTH] = 1.0 if cosmo.get("growth_mode", "ode") == "legacy" else 0.0
-    rec[_lib.REC_ZINT] = 1.0 if zint == "engine" else 0.0
+    rec[_lib.REC_ZINT] = {"midpoint": 0.0, "engine": 1.0, "gauss": 2.0}[zint]
+    rec[_lib.REC_ZINT_N] = float(zint_n) if zint == "gauss" else 0.0
     return rec
 
 
@@ -142,7 +144,7 @@ def _xi_info(L, state: torch.Tensor) -> np.ndarray:
 def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
           edge: str = "r200", timings: dict = None, sub: bool = False,
           keep_sub: bool = False, bias: bool = False, vp: bool = False,
-          lss: bool = False, zint: str = "midpoint") -> Population:
+          lss: bool = False, zint: str = "midpoint", zint_n: int = 2) -> Population:
     """build_population(..., kap_floor=1e-8, trunc=edge, zint="midpoint")
     + Lam's flattening (sgl_full.py:1535, 2742-2755), on the device.  edge
     "rvir" truncates every halo at its virial radius (sgl_full.py:1876-1880).
@@ -162,8 +164,10 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
     no xi_lin deposit; the Gaussian term of limber_lss_var
     (sgl_full.py:2541-2572) in Lambda and in sigma_kappa; excludes bias.
     zint: the lens shells of z_shells (sgl_full.py:480-518), "midpoint" (the
-    pipeline default) or "engine" (the engine grid's intervals below z_s,
-    one node at the upper edge; not with sub or vp)."""
+    pipeline default), "engine" (the engine grid's intervals below z_s,
+    one node at the upper edge) or "gauss" (the midpoint shells at zint_n
+    Gauss-Legendre nodes each, every node its own slice); engine and gauss
+    not with sub or vp."""
     L = _lib.load()
     dev = torch.device("cuda")
     if edge not in ("r200", "rvir"):
@@ -178,16 +182,21 @@ def build(cosmo: dict, zs: float, ell: bool = False, fil: bool = False,
              | (_lib.FLAG_EDGE_RVIR if edge == "rvir" else 0)
              | (_lib.FLAG_SUBHALOS if sub else 0)
              | (_lib.FLAG_CLUSTERING if bias else 0))
-    if zint not in ("midpoint", "engine"):
-        raise ValueError("zint must be 'midpoint' or 'engine'")
-    if zint == "engine" and (sub or vp):
-        raise NotImplementedError("engine shells are built without subhalos and the vp closure")
-    n_z = int(L.sgl_ref_shell_count(float(zs))) - (2 if zint == "engine" else 0)
+    if zint not in ("midpoint", "engine", "gauss"):
+        raise ValueError("zint must be 'midpoint', 'engine' or 'gauss'")
+    if zint != "midpoint" and (sub or vp):
+        raise NotImplementedError(f"{zint} shells are built without subhalos and the vp closure")
+    n_mid = int(L.sgl_ref_shell_count(float(zs)))
+    n_z = {"midpoint": n_mid, "engine": n_mid - 2, "gauss": n_mid * int(zint_n)}[zint]
+    if zint == "gauss" and n_z > _lib.REF_MAX_SHELLS:
+        raise NotImplementedError(f"zint='gauss' with {zint_n} nodes on {n_mid} shells exceeds the "
+                                  f"{_lib.REF_MAX_SHELLS} lens nodes of the tables")
     if n_z < 1 or n_z > _lib.REF_MAX_SHELLS:
         raise ValueError("z_s outside the reference grid's shells")
     st = _stream()
     clock = _Stages(timings)
-    rec = torch.tensor(_records(cosmo, float(zs), n_z, zint), dtype=torch.float64, device=dev)
+    rec = torch.tensor(_records(cosmo, float(zs), n_z, zint, int(zint_n)), dtype=torch.float64,
+                       device=dev)
     T = torch.empty(L.sgl_ref_tables_bytes(flags) // 8, dtype=torch.float64, device=dev)
     status = torch.zeros(1, dtype=torch.int32, device=dev)
     check(L.sgl_ref_tables_build(_ptr(rec), flags, _ptr(T), _ptr(status), st), "sgl_ref_tables_build")
@@ -608,13 +617,13 @@ def invert(pop: Population, xi_out: np.ndarray, tol: float = PHI_TOL,
 def run(zs: float, cosmo: dict, ell: bool = False, fil: bool = False, xi_out=None,
         edge: str = "r200", timings: dict = None, sub: bool = False,
         bias: bool = False, vp: bool = False, lss: bool = False,
-        zint: str = "midpoint") -> Result:
+        zint: str = "midpoint", zint_n: int = 2) -> Result:
     """One PDF on the reference's chain; xi_out None: the CLI body and tail
     grids linspace(-1, 1, 1601) and linspace(1.005, 9, 1600) (pipeline.py:211-215);
     vp: the positive clustering closure (needs bias); lss: the limber term
     in place of bias."""
     pop = build(cosmo, zs, ell=ell, fil=fil, edge=edge, timings=timings, sub=sub, bias=bias,
-                vp=vp, lss=lss, zint=zint)
+                vp=vp, lss=lss, zint=zint, zint_n=zint_n)
     if xi_out is None:
         xi_out = np.concatenate([np.linspace(-1.0, 1.0, 1601), np.linspace(1.005, 9.0, 1600)])
     return invert(pop, xi_out, timings=timings)

@@ -85,7 +85,8 @@ def test_run_checks_arguments_as_the_reference_does():
                         ("-bias", dict(clustering="closure")), ("-bias", dict(edge="kthr")),
                         ("full", dict(zint="engine")), ("+sub", dict(zint="engine")),
                         ("+bias", dict(zint="engine", closure="vp")),
-                        ("full", dict(zint="gauss", zint_n=3))):
+                        ("full", dict(zint="gauss", zint_n=3)),
+                        ("+bias", dict(zint="gauss", closure="vp"))):
             with pytest.raises(NotImplementedError):
                 api.run(1, cfg, **kw)
         build.assert_not_called()

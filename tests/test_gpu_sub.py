@@ -12,6 +12,8 @@
 import json
 import os
 
+from sgl_analytic_amd import _lib  # noqa: E402
+
 import numpy as np
 import pytest
 import torch
@@ -42,7 +44,7 @@ CM_S, CM_W, CM_A1, CM_A2 = (CM_K + NC * NQ * k for k in range(1, 5))
 CM_LMT = CM_A2 + NC * NQ
 CM_LRS, CM_LKS, CM_EPT = CM_LMT + NMT, CM_LMT + 2 * NMT, CM_LMT + 3 * NMT
 COARSE_STRIDE = (CM_EPT + NMT + 31) // 32 * 32
-ROWS = 104 * 99
+ROWS = _lib.REF_MAX_SHELLS * _lib.REF_MASSES   # the table capacity, not the shell count
 TASK_DOUBLES = (ROWS * 12 * 4 // 2 + 31) // 32 * 32   # per-task counts after the host records
 
 

@@ -98,3 +98,21 @@ def test_engine_shells():
         ratio = weight(eng) / weight(mid)
         print(f"z_s={zs}: engine / midpoint lensing weight {ratio:.4f}")
         assert 0.95 < ratio < 0.98
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_gauss_nodes_integrate_polynomials_per_shell(n):
+    """z_shells(zint="gauss", zint_n=n) (sgl_full.py:503-518): the n nodes
+    of every midpoint shell integrate z^p exactly for p < 2n, and n = 1 is
+    the midpoint rule."""
+    mid = oref.z_shells(1.0)
+    g = oref.z_shells(1.0, f"gauss{n}")
+    assert len(g) == n * len(mid)
+    for s, (a, b, _, _) in enumerate(mid):
+        nodes = g[n * s:n * (s + 1)]
+        assert all(x[0] == a and x[1] == b and a < x[2] < b for x in nodes)
+        for p in range(2 * n):
+            got = sum(dz * z ** p for _, _, z, dz in nodes)
+            assert abs(got - (b ** (p + 1) - a ** (p + 1)) / (p + 1)) < 1e-14 * max(1.0, b ** (p + 1))
+    if n == 1:
+        assert g == mid
